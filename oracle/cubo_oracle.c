@@ -1,0 +1,803 @@
+/*
+ * cubo_oracle.c — test infrastructure only: the CPU restatement of CuboKit's .cpu hot path that every
+ * GPU result is checked against, and the CPU column of the benchmark. It is never linked into the
+ * product library (libcubo_hip.so) and never called on a product path.
+ *
+ * Each function names the reference file:line it restates (paths relative to the CuboKit repository).
+ * The code is written from the definitions, in C, with Double arithmetic.
+ *
+ * Pinning (which check fixes which function) is listed in tests/test_oracle_pins.py and
+ * tests/test_oracle_math.py; the functions without a reference golden vector (mask stack, bilinear shift,
+ * adjoints, DPC step) are pinned by brute force, closed forms, finite differences and dot-product tests.
+ */
+#include "cubo_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Detector masks: Sources/CuboVirtual/DetectorGeometry.swift:43-98 (py4DSTEM make_detector).      */
+/* ---------------------------------------------------------------------------------------------- */
+
+int oc_mask_point(int qy, int qx, int row, int col, float *out) {
+    if (qy <= 0 || qx <= 0) return OC_INVALID_SHAPE;
+    memset(out, 0, sizeof(float) * (size_t)qy * (size_t)qx);
+    if (row < 0 || row >= qy || col < 0 || col >= qx) return OC_POINT_OUTSIDE;   /* :48-52 */
+    out[(size_t)row * qx + col] = 1.0f;
+    return OC_OK;
+}
+
+/* d^2 = (i - cr)^2 + (j - cc)^2 in Double, row term first (DetectorGeometry.swift:88-98). */
+static void fill_pred(float *out, int qy, int qx, double cr, double cc, double lo2, double hi2, int annulus) {
+    for (int i = 0; i < qy; ++i) {
+        double di = (double)i - cr;
+        double di2 = di * di;
+        for (int j = 0; j < qx; ++j) {
+            double dj = (double)j - cc;
+            double d2 = di2 + dj * dj;
+            int sel = annulus ? (d2 > lo2 && d2 < hi2) : (d2 < hi2);
+            out[(size_t)i * qx + j] = sel ? 1.0f : 0.0f;
+        }
+    }
+}
+
+int oc_mask_circle(int qy, int qx, double cr, double cc, double r, float *out) {
+    if (qy <= 0 || qx <= 0) return OC_INVALID_SHAPE;
+    if (!isfinite(cr) || !isfinite(cc) || !isfinite(r)) return OC_NON_FINITE;     /* :54 */
+    if (r < 0) return OC_NEGATIVE_RADIUS;                                          /* :55 */
+    fill_pred(out, qy, qx, cr, cc, 0.0, r * r, 0);
+    return OC_OK;
+}
+
+int oc_mask_annulus(int qy, int qx, double cr, double cc, double inner, double outer, float *out) {
+    if (qy <= 0 || qx <= 0) return OC_INVALID_SHAPE;
+    if (!isfinite(cr) || !isfinite(cc) || !isfinite(inner) || !isfinite(outer)) return OC_NON_FINITE; /* :59 */
+    if (inner < 0) return OC_NEGATIVE_RADIUS;                                                          /* :62 */
+    if (!(outer > inner)) return OC_INNER_NOT_BELOW_OUTER;                                             /* :63 */
+    fill_pred(out, qy, qx, cr, cc, inner * inner, outer * outer, 1);
+    return OC_OK;
+}
+
+int oc_mask_rectangle(int qy, int qx, double r0, double r1, double c0, double c1, float *out) {
+    if (qy <= 0 || qx <= 0) return OC_INVALID_SHAPE;
+    double b[4] = {r0, r1, c0, c1};
+    for (int k = 0; k < 4; ++k) {
+        if (!isfinite(b[k])) return OC_NON_FINITE;                                  /* :68 */
+        b[k] = nearbyint(b[k]);                      /* round half to even in the default rounding mode */
+        if (b[k] < 0) return OC_NEGATIVE_RECT_BOUND;                                /* :70 */
+    }
+    memset(out, 0, sizeof(float) * (size_t)qy * (size_t)qx);
+    long a0 = (long)fmin(b[0], (double)qy), a1 = (long)fmin(b[1], (double)qy);      /* :72-73 clip */
+    long e0 = (long)fmin(b[2], (double)qx), e1 = (long)fmin(b[3], (double)qx);
+    if (a0 < a1 && e0 < e1)
+        for (long i = a0; i < a1; ++i)
+            for (long j = e0; j < e1; ++j) out[i * qx + j] = 1.0f;
+    return OC_OK;
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Virtual image: Sources/CuboVirtual/CPUReducers.swift:13-39 — widen to Double, dot with the Double  */
+/* mask over every pixel, per position.                                                             */
+/* ---------------------------------------------------------------------------------------------- */
+
+void oc_virtual_image(const float *cube, int64_t npos, int64_t ndet, const float *mask, double *out) {
+    for (int64_t p = 0; p < npos; ++p) {
+        const float *x = cube + p * ndet;
+        double s = 0.0;
+        for (int64_t i = 0; i < ndet; ++i) s += (double)x[i] * (double)mask[i];
+        out[p] = s;
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Mean and max: CPUReducers.swift:46-97 and CompensatedSum.swift:134-161 — per pixel TwoSum in     */
+/* scan order seeded at -0, value() keeps a non-finite sum and a -0 sum; max in Float32 seeded at   */
+/* -inf, NaN wins (vDSP_vmax, NaNMaxAndPolarThresholdTests.swift:12-26); among equal values the     */
+/* later one is kept (Swift.max(a, b) = b >= a ? b : a, MetalReducers.swift:482).                     */
+/* ---------------------------------------------------------------------------------------------- */
+
+static inline float max_step(float m, float x) {
+    if (isnan(m) || isnan(x)) return NAN;
+    return x >= m ? x : m;
+}
+
+static inline double comp_value(double sum, double corr) {
+    if (!isfinite(sum)) return sum;
+    if (sum == 0.0 && signbit(sum) && corr == 0.0) return sum;
+    return sum + corr;
+}
+
+static void mean_max_block(const float *cube, int64_t npos, int64_t ndet, int64_t lo, int64_t hi,
+                           double *sum, double *corr, float *mx) {
+    for (int64_t p = 0; p < npos; ++p) {
+        const float *x = cube + p * ndet;
+        for (int64_t i = lo; i < hi; ++i) {
+            double v = (double)x[i];
+            double a = sum[i];
+            double t = a + v;
+            double bb = t - a;
+            corr[i] += (a - (t - bb)) + (v - bb);
+            sum[i] = t;
+            mx[i] = max_step(mx[i], x[i]);
+        }
+    }
+}
+
+void oc_mean_max(const float *cube, int64_t npos, int64_t ndet, double *mean, double *max) {
+    double *sum = (double *)malloc(sizeof(double) * ndet);
+    double *corr = (double *)malloc(sizeof(double) * ndet);
+    float *mx = (float *)malloc(sizeof(float) * ndet);
+    for (int64_t i = 0; i < ndet; ++i) { sum[i] = -0.0; corr[i] = -0.0; mx[i] = -INFINITY; }
+    mean_max_block(cube, npos, ndet, 0, ndet, sum, corr, mx);
+    for (int64_t i = 0; i < ndet; ++i) {
+        mean[i] = comp_value(sum[i], corr[i]) / (double)npos;
+        max[i] = (double)mx[i];
+    }
+    free(sum); free(corr); free(mx);
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Centre of mass: Sources/CuboVirtual/CenterOfMass.swift:51-91 — w, w*i, w*j formed in Double,      */
+/* Neumaier sums in pixel order, a non-finite step carries no correction.                             */
+/* ---------------------------------------------------------------------------------------------- */
+
+typedef struct { double sum, corr; } neumaier;
+static inline void nm_add(neumaier *n, double x) {
+    double t = n->sum + x;
+    if (isfinite(t)) n->corr += fabs(n->sum) >= fabs(x) ? (n->sum - t) + x : (x - t) + n->sum;
+    n->sum = t;
+}
+static inline double nm_value(const neumaier *n) { return n->sum + n->corr; }
+
+void oc_com_sums(const float *cube, int64_t npos, int qy, int qx, const float *weights,
+                 double *s, double *mr, double *mc) {
+    int64_t ndet = (int64_t)qy * qx;
+    double *w = (double *)malloc(sizeof(double) * ndet);
+    double *wi = (double *)malloc(sizeof(double) * ndet);
+    double *wj = (double *)malloc(sizeof(double) * ndet);
+    for (int64_t k = 0; k < ndet; ++k) {
+        w[k] = weights ? (double)weights[k] : 1.0;
+        wi[k] = w[k] * (double)(k / qx);
+        wj[k] = w[k] * (double)(k % qx);
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < npos; ++p) {
+        const float *x = cube + p * ndet;
+        neumaier a = {0, 0}, b = {0, 0}, d = {0, 0};
+        for (int64_t k = 0; k < ndet; ++k) {
+            double v = (double)x[k];
+            nm_add(&a, v * w[k]);
+            nm_add(&b, v * wi[k]);
+            nm_add(&d, v * wj[k]);
+        }
+        s[p] = nm_value(&a); mr[p] = nm_value(&b); mc[p] = nm_value(&d);
+    }
+    free(w); free(wi); free(wj);
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Mask stack (no reference function: K virtual images plus the CoM weights are one contraction).    */
+/* Y[p,k] = sum_i double(x[p,i]) * W[k,i] with a Neumaier sum, so the FP64 error stays ~1e-16.        */
+/* ---------------------------------------------------------------------------------------------- */
+
+void oc_mask_stack(const float *cube, int64_t npos, int64_t ndet, const double *w, int k, double *y) {
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < npos; ++p) {
+        const float *x = cube + p * ndet;
+        for (int q = 0; q < k; ++q) {
+            neumaier a = {0, 0};
+            const double *wq = w + (int64_t)q * ndet;
+            for (int64_t i = 0; i < ndet; ++i) nm_add(&a, (double)x[i] * wq[i]);
+            y[p * k + q] = nm_value(&a);
+        }
+    }
+}
+
+/* dW[k,i] = sum_p dY[p,k] * x[p,i]: the adjoint of oc_mask_stack with respect to W. */
+void oc_mask_stack_grad_w(const float *cube, int64_t npos, int64_t ndet, const double *dy, int k, double *dw) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < ndet; ++i) {
+        for (int q = 0; q < k; ++q) {
+            neumaier a = {0, 0};
+            for (int64_t p = 0; p < npos; ++p) nm_add(&a, dy[p * k + q] * (double)cube[p * ndet + i]);
+            dw[(int64_t)q * ndet + i] = nm_value(&a);
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Bilinear centre shift: Sources/CuboVirtual/ShiftedImaging.swift:43-73 and 109-122.                */
+/* dr = origin_r - mean(origin_r), fr = floor(dr), wr = dr - fr; the mask is the blend of the four    */
+/* periodic rolls by (fr, fc), (fr+1, fc), (fr, fc+1), (fr+1, fc+1);                                  */
+/* roll: out[(i+sr)%qy, (j+sc)%qx] = a[i, j].                                                        */
+/* ---------------------------------------------------------------------------------------------- */
+
+static int wrap_shift(double v, int n) {                 /* ShiftedImaging.swift:109-113 */
+    double m = fmod(v, (double)n);
+    if (m < 0) m += (double)n;
+    return ((int)m) % n;
+}
+
+static double mean_plain(const double *a, int64_t n) {   /* the origin map mean (OriginMap.meanRow) */
+    return oc_pairwise_sum(a, n) / (double)n;
+}
+
+/* value of roll(mask, sr, sc) at (i, j) = mask[(i - sr) mod qy, (j - sc) mod qx] */
+static inline double rolled(const double *m, int qy, int qx, int sr, int sc, int i, int j) {
+    int a = i - sr; a %= qy; if (a < 0) a += qy;
+    int b = j - sc; b %= qx; if (b < 0) b += qx;
+    return m[(int64_t)a * qx + b];
+}
+
+typedef struct { int ir, ic; double wr, wc; } bil_shift;
+
+static bil_shift bil_of(double dr, double dc, int qy, int qx) {
+    bil_shift s;
+    double fr = floor(dr), fc = floor(dc);
+    s.wr = dr - fr; s.wc = dc - fc;
+    s.ir = wrap_shift(fr, qy); s.ic = wrap_shift(fc, qx);
+    return s;
+}
+
+int oc_vi_bilinear(const float *cube, int64_t npos, int qy, int qx, const double *mask,
+                   const double *origin_r, const double *origin_c, double *out) {
+    for (int64_t p = 0; p < npos; ++p)
+        if (!isfinite(origin_r[p]) || !isfinite(origin_c[p])) return OC_NON_FINITE;
+    int64_t ndet = (int64_t)qy * qx;
+    double mr = mean_plain(origin_r, npos), mc = mean_plain(origin_c, npos);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < npos; ++p) {
+        bil_shift s = bil_of(origin_r[p] - mr, origin_c[p] - mc, qy, qx);
+        double w00 = (1 - s.wr) * (1 - s.wc), w10 = s.wr * (1 - s.wc), w01 = (1 - s.wr) * s.wc, w11 = s.wr * s.wc;
+        neumaier a = {0, 0};
+        const float *x = cube + p * ndet;
+        for (int i = 0; i < qy; ++i)
+            for (int j = 0; j < qx; ++j) {
+                double m = rolled(mask, qy, qx, s.ir, s.ic, i, j) * w00 +
+                           rolled(mask, qy, qx, s.ir + 1, s.ic, i, j) * w10 +
+                           rolled(mask, qy, qx, s.ir, s.ic + 1, i, j) * w01 +
+                           rolled(mask, qy, qx, s.ir + 1, s.ic + 1, i, j) * w11;
+                nm_add(&a, (double)x[(int64_t)i * qx + j] * m);
+            }
+        out[p] = nm_value(&a);
+    }
+    return OC_OK;
+}
+
+/* dW[u] = sum_p dY[p] * sum_corners w_corner * x[p, u + shift_corner]: the adjoint with respect to the
+ * unshifted mask (x at pixel (i, j) meets mask[(i - sr) mod qy, (j - sc) mod qx]). */
+void oc_vi_bilinear_grad_w(const float *cube, int64_t npos, int qy, int qx, const double *origin_r,
+                           const double *origin_c, const double *dy, double *dw) {
+    int64_t ndet = (int64_t)qy * qx;
+    double mr = mean_plain(origin_r, npos), mc = mean_plain(origin_c, npos);
+    for (int64_t u = 0; u < ndet; ++u) dw[u] = 0.0;
+    for (int64_t p = 0; p < npos; ++p) {
+        bil_shift s = bil_of(origin_r[p] - mr, origin_c[p] - mc, qy, qx);
+        double wts[4] = {(1 - s.wr) * (1 - s.wc), s.wr * (1 - s.wc), (1 - s.wr) * s.wc, s.wr * s.wc};
+        int sr[4] = {s.ir, s.ir + 1, s.ir, s.ir + 1}, sc[4] = {s.ic, s.ic, s.ic + 1, s.ic + 1};
+        const float *x = cube + p * ndet;
+        for (int c = 0; c < 4; ++c) {
+            double g = dy[p] * wts[c];
+            for (int a = 0; a < qy; ++a)
+                for (int b = 0; b < qx; ++b) {
+                    int i = (a + sr[c]) % qy, j = (b + sc[c]) % qx;
+                    dw[(int64_t)a * qx + b] += g * (double)x[(int64_t)i * qx + j];
+                }
+        }
+    }
+}
+
+/* Gradient with respect to the per-position origins, inside the bilinear cell (d wr / d dr = 1), then
+ * through the mean: dr_p = o_p - mean(o), so dL/do_q = g_q - mean(g). */
+int oc_vi_bilinear_grad_origin(const float *cube, int64_t npos, int qy, int qx, const double *mask,
+                               const double *origin_r, const double *origin_c, const double *dy,
+                               double *d_origin_r, double *d_origin_c) {
+    int64_t ndet = (int64_t)qy * qx;
+    double mr = mean_plain(origin_r, npos), mc = mean_plain(origin_c, npos);
+    double *gr = (double *)malloc(sizeof(double) * npos), *gc = (double *)malloc(sizeof(double) * npos);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < npos; ++p) {
+        bil_shift s = bil_of(origin_r[p] - mr, origin_c[p] - mc, qy, qx);
+        const float *x = cube + p * ndet;
+        double c00 = 0, c10 = 0, c01 = 0, c11 = 0;
+        for (int i = 0; i < qy; ++i)
+            for (int j = 0; j < qx; ++j) {
+                double v = (double)x[(int64_t)i * qx + j];
+                c00 += v * rolled(mask, qy, qx, s.ir, s.ic, i, j);
+                c10 += v * rolled(mask, qy, qx, s.ir + 1, s.ic, i, j);
+                c01 += v * rolled(mask, qy, qx, s.ir, s.ic + 1, i, j);
+                c11 += v * rolled(mask, qy, qx, s.ir + 1, s.ic + 1, i, j);
+            }
+        /* Y = (1-wr)(1-wc)c00 + wr(1-wc)c10 + (1-wr)wc c01 + wr wc c11 */
+        gr[p] = dy[p] * ((1 - s.wc) * (c10 - c00) + s.wc * (c11 - c01));
+        gc[p] = dy[p] * ((1 - s.wr) * (c01 - c00) + s.wr * (c11 - c10));
+    }
+    double mgr = 0, mgc = 0;
+    for (int64_t p = 0; p < npos; ++p) { mgr += gr[p]; mgc += gc[p]; }
+    mgr /= (double)npos; mgc /= (double)npos;
+    for (int64_t p = 0; p < npos; ++p) { d_origin_r[p] = gr[p] - mgr; d_origin_c[p] = gc[p] - mgc; }
+    free(gr); free(gc);
+    return OC_OK;
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* NumPy pairwise sum: Sources/CuboCore/NumPySum.swift:23-48 (numpy pairwise_sum: < 8 in order;       */
+/* <= 128 in eight interleaved partials combined ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)) then the rest;   */
+/* larger: split at a multiple of 8 and recurse).                                                    */
+/* ---------------------------------------------------------------------------------------------- */
+
+double oc_pairwise_sum(const double *a, int64_t n) {
+    if (n < 8) {
+        double res = 0.0;
+        for (int64_t i = 0; i < n; ++i) res += a[i];
+        return res;
+    } else if (n <= 128) {
+        double r[8];
+        for (int k = 0; k < 8; ++k) r[k] = a[k];
+        int64_t i = 8;
+        for (; i < n - (n % 8); i += 8)
+            for (int k = 0; k < 8; ++k) r[k] += a[i + k];
+        double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        for (; i < n; ++i) res += a[i];
+        return res;
+    } else {
+        int64_t n2 = n / 2;
+        n2 -= n2 % 8;
+        return oc_pairwise_sum(a, n2) + oc_pairwise_sum(a + n2, n - n2);
+    }
+}
+
+static double np_mean(const double *a, int64_t n) { return n ? oc_pairwise_sum(a, n) / (double)n : NAN; }
+
+/* ---------------------------------------------------------------------------------------------- */
+/* FFT in Double with NumPy's conventions (Sources/CuboCore/DoubleFFT.swift:119-195: forward           */
+/* exp(-2 pi i jk/N) unnormalised, inverse exp(+) divided by N, rows then columns). Radix-2 for powers */
+/* of two, an exact-index direct DFT otherwise (the oracle only meets small grids).                    */
+/* ---------------------------------------------------------------------------------------------- */
+
+static void fft1d(double *re, double *im, int n, int inverse, double *tr, double *ti) {
+    if (n <= 1) return;
+    double sign = inverse ? 1.0 : -1.0;
+    if ((n & (n - 1)) == 0) {
+        for (int i = 1, j = 0; i < n; ++i) {            /* bit reversal */
+            int bit = n >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) { double t = re[i]; re[i] = re[j]; re[j] = t; t = im[i]; im[i] = im[j]; im[j] = t; }
+        }
+        for (int len = 2; len <= n; len <<= 1) {
+            int half = len >> 1;
+            for (int k = 0; k < half; ++k) {
+                double ang = sign * 2.0 * M_PI * (double)k / (double)len;
+                double wr = cos(ang), wi = sin(ang);
+                for (int s = 0; s < n; s += len) {
+                    int a = s + k, b = a + half;
+                    double xr = re[b] * wr - im[b] * wi, xi = re[b] * wi + im[b] * wr;
+                    re[b] = re[a] - xr; im[b] = im[a] - xi;
+                    re[a] += xr; im[a] += xi;
+                }
+            }
+        }
+    } else {
+        for (int k = 0; k < n; ++k) {
+            double sr = 0, si = 0;
+            for (int j = 0; j < n; ++j) {
+                long idx = ((long)j * k) % n;
+                double ang = sign * 2.0 * M_PI * (double)idx / (double)n;
+                double c = cos(ang), s = sin(ang);
+                sr += re[j] * c - im[j] * s;
+                si += re[j] * s + im[j] * c;
+            }
+            tr[k] = sr; ti[k] = si;
+        }
+        memcpy(re, tr, sizeof(double) * n);
+        memcpy(im, ti, sizeof(double) * n);
+    }
+}
+
+void oc_fft2d(double *re, double *im, int rows, int cols, int inverse) {
+    int m = rows > cols ? rows : cols;
+    double *tr = (double *)malloc(sizeof(double) * m * 4);
+    double *ti = tr + m, *cr = tr + 2 * m, *ci = tr + 3 * m;
+    for (int r = 0; r < rows; ++r) fft1d(re + (int64_t)r * cols, im + (int64_t)r * cols, cols, inverse, tr, ti);
+    for (int c = 0; c < cols; ++c) {
+        for (int r = 0; r < rows; ++r) { cr[r] = re[(int64_t)r * cols + c]; ci[r] = im[(int64_t)r * cols + c]; }
+        fft1d(cr, ci, rows, inverse, tr, ti);
+        for (int r = 0; r < rows; ++r) { re[(int64_t)r * cols + c] = cr[r]; im[(int64_t)r * cols + c] = ci[r]; }
+    }
+    if (inverse) {
+        double s = 1.0 / ((double)rows * (double)cols);
+        for (int64_t k = 0; k < (int64_t)rows * cols; ++k) { re[k] *= s; im[k] *= s; }
+    }
+    free(tr);
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Plane fit and normalisation: Sources/CuboPhase/CenterOfMassReference.swift:27-84.                 */
+/* ---------------------------------------------------------------------------------------------- */
+
+int oc_plane_fit(const double *m, int R, int C, double tol, double *out) {
+    int64_t N = (int64_t)R * C, nf = 0;
+    double rs = 0, cs = 0;
+    for (int64_t k = 0; k < N; ++k)
+        if (isfinite(m[k])) { ++nf; rs += (double)(k / C); cs += (double)(k % C); }
+    if (nf < 3) return OC_PLANE_DEGENERATE;
+    double n = (double)nf, rMean = rs / n, cMean = cs / n;
+    double suu = 0, suv = 0, svv = 0, sum = 0;
+    for (int64_t k = 0; k < N; ++k) {
+        if (!isfinite(m[k])) continue;
+        double u = (double)(k / C) - rMean, v = (double)(k % C) - cMean;
+        suu += u * u; suv += u * v; svv += v * v; sum += m[k];
+    }
+    double a = sum / n, sum_um = 0, sum_vm = 0;
+    for (int64_t k = 0; k < N; ++k) {
+        if (!isfinite(m[k])) continue;
+        double u = (double)(k / C) - rMean, v = (double)(k % C) - cMean;
+        sum_um += u * (m[k] - a); sum_vm += v * (m[k] - a);
+    }
+    double det = suu * svv - suv * suv;
+    if (!(det > tol * fmax(suu * svv, 1.0))) return OC_PLANE_DEGENERATE;
+    double b = (svv * sum_um - suv * sum_vm) / det;
+    double c = (suu * sum_vm - suv * sum_um) / det;
+    for (int64_t k = 0; k < N; ++k) out[k] = a + b * ((double)(k / C) - rMean) + c * ((double)(k % C) - cMean);
+    return OC_OK;
+}
+
+static double clean_nan(double v) {
+    if (isnan(v)) return 0.0;
+    if (v == INFINITY) return DBL_MAX;
+    if (v == -INFINITY) return -DBL_MAX;
+    return v;
+}
+
+int oc_normalize(const double *mx, const double *my, int R, int C, int reference, double sampling, double tol,
+                 double *fx, double *fy, double *nx, double *ny) {
+    int64_t N = (int64_t)R * C;
+    if (reference == 1) {
+        int st = oc_plane_fit(mx, R, C, tol, fx);
+        if (st) return st;
+        st = oc_plane_fit(my, R, C, tol, fy);
+        if (st) return st;
+    } else if (reference == 2) {
+        double a = np_mean(mx, N), b = np_mean(my, N);
+        for (int64_t k = 0; k < N; ++k) { fx[k] = a; fy[k] = b; }
+    } else {
+        for (int64_t k = 0; k < N; ++k) { fx[k] = 0; fy[k] = 0; }
+    }
+    for (int64_t k = 0; k < N; ++k) {
+        nx[k] = clean_nan(mx[k] - fx[k]) * sampling;
+        ny[k] = clean_nan(my[k] - fy[k]) * sampling;
+    }
+    return OC_OK;
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* R-Q rotation: Sources/CuboPhase/RotationSolver.swift:48-150.                                      */
+/* ---------------------------------------------------------------------------------------------- */
+
+void oc_rotate(const double *x, const double *y, int64_t n, double radians, int transpose, double *ox, double *oy) {
+    double c = cos(radians), s = sin(radians);
+    const double *a = transpose ? y : x, *b = transpose ? x : y;
+    for (int64_t k = 0; k < n; ++k) { ox[k] = c * a[k] - s * b[k]; oy[k] = s * a[k] + c * b[k]; }
+}
+
+double oc_curl(const double *x, const double *y, int R, int C) {
+    int64_t n = (int64_t)(R - 2) * (C - 2), k = 0;
+    double *v = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    for (int r = 1; r < R - 1; ++r)
+        for (int c = 1; c < C - 1; ++c)
+            v[k++] = fabs((y[(int64_t)(r + 1) * C + c] - y[(int64_t)(r - 1) * C + c]) -
+                          (x[(int64_t)r * C + c + 1] - x[(int64_t)r * C + c - 1]));
+    double m = np_mean(v, n);
+    free(v);
+    return m;
+}
+
+double oc_divergence(const double *x, const double *y, int R, int C) {
+    int64_t n = (int64_t)(R - 2) * (C - 2), k = 0;
+    double *v = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    for (int r = 1; r < R - 1; ++r)
+        for (int c = 1; c < C - 1; ++c)
+            v[k++] = fabs((x[(int64_t)(r + 1) * C + c] - x[(int64_t)(r - 1) * C + c]) +
+                          (y[(int64_t)r * C + c + 1] - y[(int64_t)r * C + c - 1]));
+    double m = np_mean(v, n);
+    free(v);
+    return m;
+}
+
+static int usable(const double *nx, const double *ny, int R, int C) {
+    if (R < 3 || C < 3) return OC_SCAN_TOO_SMALL;
+    for (int64_t k = 0; k < (int64_t)R * C; ++k)
+        if (!isfinite(nx[k]) || !isfinite(ny[k])) return OC_NON_FINITE;
+    return OC_OK;
+}
+
+int oc_rotation_curve(const double *nx, const double *ny, int R, int C, const double *angles_deg, int nangles,
+                      int transpose, int maximize, double *curve) {
+    int st = usable(nx, ny, R, C);
+    if (st) return st;
+    int64_t N = (int64_t)R * C;
+#pragma omp parallel for schedule(dynamic)
+    for (int a = 0; a < nangles; ++a) {
+        double *x = (double *)malloc(sizeof(double) * N * 2), *y = x + N;
+        oc_rotate(nx, ny, N, angles_deg[a] * (M_PI / 180.0), transpose, x, y);
+        curve[a] = maximize ? oc_divergence(x, y, R, C) : oc_curl(x, y, R, C);
+        free(x);
+    }
+    return OC_OK;
+}
+
+int oc_rotation_solve(const double *nx, const double *ny, int R, int C, const double *angles_deg, int nangles,
+                      int maximize, double *degrees, int *transposed, double *curve, double *curve_t,
+                      double *cx, double *cy) {
+    int st = oc_rotation_curve(nx, ny, R, C, angles_deg, nangles, 0, maximize, curve);
+    if (st) return st;
+    oc_rotation_curve(nx, ny, R, C, angles_deg, nangles, 1, maximize, curve_t);
+    int i = 0, j = 0;
+    for (int k = 1; k < nangles; ++k) {
+        if (maximize ? curve[k] > curve[i] : curve[k] < curve[i]) i = k;
+        if (maximize ? curve_t[k] > curve_t[j] : curve_t[k] < curve_t[j]) j = k;
+    }
+    int t = maximize ? curve_t[j] > curve[i] : curve_t[j] < curve[i];   /* the plain orientation wins ties */
+    *transposed = t;
+    *degrees = angles_deg[t ? j : i];
+    oc_rotate(nx, ny, (int64_t)R * C, *degrees * (M_PI / 180.0), t, cx, cy);
+    return OC_OK;
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Iterative DPC: Sources/CuboPhase/DPC.swift:26-102 (py4DSTEM DPC.reconstruct, Double).              */
+/* ---------------------------------------------------------------------------------------------- */
+
+static void fftfreq(int n, double d, double *out) {
+    double scale = 1.0 / ((double)n * d);
+    int positive = (n - 1) / 2 + 1;
+    for (int k = 0; k < n; ++k) out[k] = (double)(k < positive ? k : k - n) * scale;
+}
+
+typedef struct {
+    int R, C, Rp, Cp;
+    double sr, sc;
+    double *gx, *gy;      /* gx = -0.25 kx kden, the operator is i*gx */
+} dpc_ops;
+
+static void dpc_ops_init(dpc_ops *o, int R, int C, int Rp, int Cp, double sr, double sc) {
+    o->R = R; o->C = C; o->Rp = Rp; o->Cp = Cp; o->sr = sr; o->sc = sc;
+    double *kx = (double *)malloc(sizeof(double) * (Rp + Cp)), *ky = kx + Rp;
+    fftfreq(Rp, sr, kx);
+    fftfreq(Cp, sc, ky);
+    o->gx = (double *)malloc(sizeof(double) * (int64_t)Rp * Cp * 2);
+    o->gy = o->gx + (int64_t)Rp * Cp;
+    for (int r = 0; r < Rp; ++r)
+        for (int c = 0; c < Cp; ++c) {
+            double den = kx[r] * kx[r] + ky[c] * ky[c];
+            if (r == 0 && c == 0) den = INFINITY;
+            double kden = 1.0 / den;
+            o->gx[(int64_t)r * Cp + c] = -0.25 * kx[r] * kden;
+            o->gy[(int64_t)r * Cp + c] = -0.25 * ky[c] * kden;
+        }
+    free(kx);
+}
+static void dpc_ops_free(dpc_ops *o) { free(o->gx); }
+
+/* The forward differences plus the field on the scan region (DPC.swift:59-69); returns the mean square. */
+static double dpc_forward(const dpc_ops *o, const double *phi, const double *fx, const double *fy,
+                          double *dx, double *dy, double *tmp) {
+    int Rp = o->Rp, Cp = o->Cp;
+    memset(dx, 0, sizeof(double) * (int64_t)Rp * Cp);
+    memset(dy, 0, sizeof(double) * (int64_t)Rp * Cp);
+    for (int r = 0; r < o->R; ++r)
+        for (int c = 0; c < o->C; ++c) {
+            double up = phi[(int64_t)((r - 1 + Rp) % Rp) * Cp + c], down = phi[(int64_t)((r + 1) % Rp) * Cp + c];
+            double left = phi[(int64_t)r * Cp + (c - 1 + Cp) % Cp], right = phi[(int64_t)r * Cp + (c + 1) % Cp];
+            int64_t k = (int64_t)r * Cp + c;
+            dx[k] = (up - down) / (2 * o->sr) + fx[(int64_t)r * o->C + c];
+            dy[k] = (left - right) / (2 * o->sc) + fy[(int64_t)r * o->C + c];
+            tmp[(int64_t)r * o->C + c] = dx[k] * dx[k] + dy[k] * dy[k];
+        }
+    return np_mean(tmp, (int64_t)o->R * o->C);
+}
+
+/* update = Re(IFFT(FFT(dx) * (i gx) + FFT(dy) * (i gy))) (DPC.swift:71-79); dx, dy are overwritten. */
+static void dpc_update(const dpc_ops *o, double *dx, double *dy, double *xi, double *yi, double *ur, double *ui) {
+    int64_t n = (int64_t)o->Rp * o->Cp;
+    memset(xi, 0, sizeof(double) * n);
+    memset(yi, 0, sizeof(double) * n);
+    oc_fft2d(dx, xi, o->Rp, o->Cp, 0);
+    oc_fft2d(dy, yi, o->Rp, o->Cp, 0);
+    for (int64_t k = 0; k < n; ++k) {
+        ur[k] = -xi[k] * o->gx[k] + -yi[k] * o->gy[k];
+        ui[k] = dx[k] * o->gx[k] + dy[k] * o->gy[k];
+    }
+    oc_fft2d(ur, ui, o->Rp, o->Cp, 1);
+}
+
+int oc_dpc_reconstruct(const double *fx, const double *fy, int R, int C, double sr, double sc, double padding,
+                       int max_iter, double step, double stop, int backtrack, double *phase, double *errors,
+                       int *accepted, int *n_iter, double *final_step) {
+    if (!(sr > 0) || !(sc > 0) || !isfinite(sr) || !isfinite(sc)) return OC_INVALID_PARAMETER;
+    if (!(padding >= 1) || !isfinite(padding) || max_iter < 0 || !(step > 0) || !isfinite(step))
+        return OC_INVALID_PARAMETER;
+    int64_t N = (int64_t)R * C;
+    for (int64_t k = 0; k < N; ++k)
+        if (!isfinite(fx[k]) || !isfinite(fy[k])) return OC_NON_FINITE;
+    double *tmp = (double *)malloc(sizeof(double) * N);
+    for (int64_t k = 0; k < N; ++k) tmp[k] = fx[k] * fx[k] + fy[k] * fy[k];
+    double denom = np_mean(tmp, N);
+    if (!(denom > 0)) { free(tmp); return OC_ZERO_FIELD; }
+    int Rp = (int)nearbyint((double)R * padding), Cp = (int)nearbyint((double)C * padding);
+    dpc_ops o;
+    dpc_ops_init(&o, R, C, Rp, Cp, sr, sc);
+    int64_t n = (int64_t)Rp * Cp;
+    double *phi = (double *)calloc(n * 7, sizeof(double));
+    double *dx = phi + n, *dy = dx + n, *xi = dy + n, *yi = xi + n, *ur = yi + n, *ui = ur + n;
+    double error = INFINITY;
+    int ne = 0, ni = 0;
+    for (int it = 0; it < max_iter; ++it) {
+        if (step < stop) break;
+        double newError = dpc_forward(&o, phi, fx, fy, dx, dy, tmp) / denom;
+        if (newError > error && backtrack) {
+            step /= 2;
+            accepted[ni++] = 0;
+            continue;
+        }
+        error = newError;
+        dpc_update(&o, dx, dy, xi, yi, ur, ui);
+        for (int64_t k = 0; k < n; ++k) phi[k] += step * ur[k];
+        errors[ne++] = error;
+        accepted[ni++] = 1;
+    }
+    for (int r = 0; r < R; ++r)
+        for (int c = 0; c < C; ++c) phase[(int64_t)r * C + c] = phi[(int64_t)r * Cp + c];
+    *n_iter = ni;
+    *final_step = step;
+    (void)ne;
+    dpc_ops_free(&o);
+    free(phi); free(tmp);
+    return OC_OK;
+}
+
+/* One accepted DPC iteration without the backtracking decision: phi_out = phi + step * update, and the
+ * error of phi (DPC.swift:59-82). */
+int oc_dpc_step(const double *phi, const double *fx, const double *fy, int R, int C, int Rp, int Cp,
+                double sr, double sc, double step, double *phi_out, double *error_out) {
+    int64_t N = (int64_t)R * C, n = (int64_t)Rp * Cp;
+    double *tmp = (double *)malloc(sizeof(double) * N);
+    for (int64_t k = 0; k < N; ++k) tmp[k] = fx[k] * fx[k] + fy[k] * fy[k];
+    double denom = np_mean(tmp, N);
+    if (!(denom > 0)) { free(tmp); return OC_ZERO_FIELD; }
+    dpc_ops o;
+    dpc_ops_init(&o, R, C, Rp, Cp, sr, sc);
+    double *buf = (double *)calloc(n * 6, sizeof(double));
+    double *dx = buf, *dy = dx + n, *xi = dy + n, *yi = xi + n, *ur = yi + n, *ui = ur + n;
+    *error_out = dpc_forward(&o, phi, fx, fy, dx, dy, tmp) / denom;
+    dpc_update(&o, dx, dy, xi, yi, ur, ui);
+    for (int64_t k = 0; k < n; ++k) phi_out[k] = phi[k] + step * ur[k];
+    dpc_ops_free(&o);
+    free(buf); free(tmp);
+    return OC_OK;
+}
+
+/* A^T(g) for A(u) = Re(IFFT(FFT(u) * H)), H = i*gk: Re(IFFT(FFT(g) * conj(H))). out = A_x^T(g), A_y^T(g). */
+static void dpc_update_adjoint(const dpc_ops *o, const double *g, double *ax, double *ay) {
+    int64_t n = (int64_t)o->Rp * o->Cp;
+    double *re = (double *)malloc(sizeof(double) * n * 4), *im = re + n, *r2 = im + n, *i2 = r2 + n;
+    memcpy(re, g, sizeof(double) * n);
+    memset(im, 0, sizeof(double) * n);
+    oc_fft2d(re, im, o->Rp, o->Cp, 0);
+    /* (a + ib) * conj(i g) = (a + ib)(-i g) = b g - i a g */
+    for (int64_t k = 0; k < n; ++k) { r2[k] = im[k] * o->gx[k]; i2[k] = -re[k] * o->gx[k]; }
+    oc_fft2d(r2, i2, o->Rp, o->Cp, 1);
+    memcpy(ax, r2, sizeof(double) * n);
+    for (int64_t k = 0; k < n; ++k) { r2[k] = im[k] * o->gy[k]; i2[k] = -re[k] * o->gy[k]; }
+    oc_fft2d(r2, i2, o->Rp, o->Cp, 1);
+    memcpy(ay, r2, sizeof(double) * n);
+    free(re);
+}
+
+/* Vector-Jacobian product of oc_dpc_step: given dL/dphi_out and dL/derror, dL/dphi, dL/dfx, dL/dfy. */
+int oc_dpc_step_vjp(const double *phi, const double *fx, const double *fy, int R, int C, int Rp, int Cp,
+                    double sr, double sc, double step, const double *g_phi_out, double g_err,
+                    double *g_phi, double *g_fx, double *g_fy) {
+    int64_t N = (int64_t)R * C, n = (int64_t)Rp * Cp;
+    double *tmp = (double *)malloc(sizeof(double) * N);
+    for (int64_t k = 0; k < N; ++k) tmp[k] = fx[k] * fx[k] + fy[k] * fy[k];
+    double denom = np_mean(tmp, N);
+    if (!(denom > 0)) { free(tmp); return OC_ZERO_FIELD; }
+    dpc_ops o;
+    dpc_ops_init(&o, R, C, Rp, Cp, sr, sc);
+    double *buf = (double *)calloc(n * 4, sizeof(double));
+    double *dx = buf, *dy = dx + n, *gdx = dy + n, *gdy = gdx + n;
+    double msq = dpc_forward(&o, phi, fx, fy, dx, dy, tmp);
+    /* phi_out = phi + step*(A_x dx + A_y dy): d/d dx = step * A_x^T g */
+    dpc_update_adjoint(&o, g_phi_out, gdx, gdy);
+    for (int64_t k = 0; k < n; ++k) { gdx[k] *= step; gdy[k] *= step; }
+    /* error = mean_scan(dx^2 + dy^2) / denom: d/d dx = 2 dx / (N denom) on the scan region */
+    for (int r = 0; r < R; ++r)
+        for (int c = 0; c < C; ++c) {
+            int64_t k = (int64_t)r * Cp + c;
+            gdx[k] += g_err * 2.0 * dx[k] / ((double)N * denom);
+            gdy[k] += g_err * 2.0 * dy[k] / ((double)N * denom);
+        }
+    /* dx is zero off the scan region, so only the scan entries of gdx, gdy flow back */
+    for (int64_t k = 0; k < n; ++k) g_phi[k] = g_phi_out[k];
+    for (int r = 0; r < R; ++r)
+        for (int c = 0; c < C; ++c) {
+            int64_t k = (int64_t)r * Cp + c;
+            double a = gdx[k] / (2 * sr), b = gdy[k] / (2 * sc);
+            /* dx[r,c] = (phi[r-1,c] - phi[r+1,c]) / (2 sr) + fx; dy[r,c] = (phi[r,c-1] - phi[r,c+1]) / (2 sc) + fy */
+            g_phi[(int64_t)((r - 1 + Rp) % Rp) * Cp + c] += a;
+            g_phi[(int64_t)((r + 1) % Rp) * Cp + c] -= a;
+            g_phi[(int64_t)r * Cp + (c - 1 + Cp) % Cp] += b;
+            g_phi[(int64_t)r * Cp + (c + 1) % Cp] -= b;
+            g_fx[(int64_t)r * C + c] = gdx[k];
+            g_fy[(int64_t)r * C + c] = gdy[k];
+        }
+    /* the denominator depends on the field: error = msq / denom, denom = mean(fx^2 + fy^2) */
+    double dd = -g_err * msq / (denom * denom);
+    for (int64_t k = 0; k < N; ++k) {
+        g_fx[k] += dd * 2.0 * fx[k] / (double)N;
+        g_fy[k] += dd * 2.0 * fy[k] / (double)N;
+    }
+    dpc_ops_free(&o);
+    free(buf); free(tmp);
+    return OC_OK;
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* CPU baseline: the reference's .cpu algorithms over a uint16 cube, tile by tile converted to        */
+/* Float32 (MappedCube.swift:150-164, ScanTiling.automatic CubeSource.swift:96-100), VI per position  */
+/* serially (CPUReducers.swift:13-39), mean+max on 128-pixel blocks across threads (:46-97), CoM       */
+/* parallel over positions (CenterOfMass.swift:51-79).                                              */
+/* ---------------------------------------------------------------------------------------------- */
+
+void oc_cpu_first_processing_u16(const uint16_t *cube, int64_t npos, int qy, int qx, int64_t rows_per_tile,
+                                 int64_t positions_per_row, const float *bf, const float *adf, int threads,
+                                 double *vi_bf, double *vi_adf, double *mean, double *max,
+                                 double *s, double *mr, double *mc) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    int64_t ndet = (int64_t)qy * qx;
+    int64_t tile_pos = rows_per_tile * positions_per_row;
+    float *tile = (float *)malloc(sizeof(float) * tile_pos * ndet);
+    double *sum = (double *)malloc(sizeof(double) * ndet), *corr = (double *)malloc(sizeof(double) * ndet);
+    float *mx = (float *)malloc(sizeof(float) * ndet);
+    double *pat = (double *)malloc(sizeof(double) * ndet);
+    double *bfd = (double *)malloc(sizeof(double) * ndet), *adfd = (double *)malloc(sizeof(double) * ndet);
+    for (int64_t i = 0; i < ndet; ++i) {
+        sum[i] = -0.0; corr[i] = -0.0; mx[i] = -INFINITY; bfd[i] = bf[i]; adfd[i] = adf[i];
+    }
+    const int64_t block = 128, nblocks = (ndet + block - 1) / block;
+    for (int64_t p0 = 0; p0 < npos; p0 += tile_pos) {
+        int64_t np = npos - p0 < tile_pos ? npos - p0 : tile_pos;
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < np * ndet; ++k) tile[k] = (float)cube[p0 * ndet + k];
+        for (int64_t p = 0; p < np; ++p) {                 /* VI: serial over positions, as the .cpu path */
+            const float *x = tile + p * ndet;
+            for (int64_t i = 0; i < ndet; ++i) pat[i] = (double)x[i];
+            double a = 0, b = 0;
+            for (int64_t i = 0; i < ndet; ++i) a += pat[i] * bfd[i];
+            for (int64_t i = 0; i < ndet; ++i) b += pat[i] * adfd[i];
+            vi_bf[p0 + p] = a; vi_adf[p0 + p] = b;
+        }
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t b = 0; b < nblocks; ++b) {
+            int64_t lo = b * block, hi = lo + block < ndet ? lo + block : ndet;
+            mean_max_block(tile, np, ndet, lo, hi, sum, corr, mx);
+        }
+        oc_com_sums(tile, np, qy, qx, NULL, s + p0, mr + p0, mc + p0);
+    }
+    for (int64_t i = 0; i < ndet; ++i) {
+        mean[i] = comp_value(sum[i], corr[i]) / (double)npos;
+        max[i] = (double)mx[i];
+    }
+    free(tile); free(sum); free(corr); free(mx); free(pat); free(bfd); free(adfd);
+}
