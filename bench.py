@@ -1,0 +1,145 @@
+"""Benchmark of the fused first-processing pass on one MI355X.
+
+Workload: the 256x256x128x128 uint16 synthetic cube (2 GiB, seeded Poisson counts: disk r=12 at 500 counts/px
+with a +-2 px sinusoidal deflection, Gaussian rings at q=30 and 45 px of amplitude 20). One step is one fused
+pass over the HBM-resident cube yielding the BF (r=14) and ADF (20-60) virtual images, the mean and maximum
+patterns and the centre of mass, with the Double results copied to host memory.
+
+Prints ONE JSON line. `value` is cube bytes per second of whole steps; `roofline.achieved` is the cube bytes
+of one fused-kernel launch over that kernel's average duration from HIP events on the cube's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as ct
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fused VI+COM cube GB/s (% HBM peak); BF16-split vs FP32 speedup at ≤1e-6 rel err"
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per fused-kernel launch measured with rocprofv3 counters (profiles/traffic.json), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("fused_u16_int_bandwidth_u16", {}).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(cube, bf, adf, seconds: float):
+    """The oracle's restatement of CuboKit's .cpu pass, on a bounded sample of the same cube."""
+    from oracle import oracle as O
+    rows = 16
+    sample = cube.read_scan_tile(range(0, rows))
+    threads = min(16, os.cpu_count() or 1)
+    nbytes = sample.nbytes
+    passes, t0 = 0, time.perf_counter()
+    while True:
+        O.cpu_first_processing_u16(sample, 128, 128, sample.shape[1], bf, adf, threads=threads)
+        passes += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or passes >= 50:
+            break
+    return {"value": round(nbytes * passes / el / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} of 256 scan rows ({nbytes / 2**20:.0f} MiB of uint16), {passes} passes of the oracle's "
+                      f".cpu restatement (VI serial, mean+max and CoM on {threads} threads), {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload runs on one MI355X "
+              "and no multi-GPU path is built).", file=sys.stderr)
+        return 2
+
+    import cubokit_amd as C
+    from cubokit_amd import _lib as L
+    from cubokit_amd import synthetic
+
+    info = C.device_info(0)
+    log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units} HBM={info.hbm_bytes / 2**30:.0f} GiB")
+    shape, dtype, params = synthetic.bandwidth_u16()
+    cube = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
+    cube.fill_synthetic(params)
+    cube_bytes = cube.byte_count
+    log(f"cube {shape} uint16 {cube_bytes / 2**30:.2f} GiB synthesised")
+
+    s = cube.shape
+    bf = C.DetectorMask.make(C.DetectorGeometry.circle(64, 64, 14), s)
+    adf = C.DetectorMask.make(C.DetectorGeometry.annulus(64, 64, 20, 60), s)
+    masks = np.ascontiguousarray(np.stack([bf, adf]))
+    assert C.reduce_path(cube, masks, want_com=True) == 1
+    req = L.ReduceRequest(masks=masks.ctypes.data, n_masks=2, want_mean_max=1, want_com=1, com_weights=None)
+    npos, ndet = s.scan_count, s.detector_count
+    out = {"images": np.empty((2, npos)), "mean": np.empty(ndet), "max": np.empty(ndet)}
+    for k in ("com_intensity", "com_moment_row", "com_moment_column", "com_row", "com_column"):
+        out[k] = np.empty(npos)
+    res = L.ReduceResult(**{k: v.ctypes.data for k, v in out.items()})
+    lib = L.lib()
+    h = ct.c_void_p()
+    if lib.cubo_hip_reduce_begin(cube.handle, ct.byref(req), ct.byref(h)) != 0:
+        raise RuntimeError(L.last_error())
+
+    def step():
+        if lib.cubo_hip_reduce_reset(h) or lib.cubo_hip_reduce_rows(h, 0, s.scan_rows) or \
+                lib.cubo_hip_reduce_results(h, ct.byref(res)):
+            raise RuntimeError(L.last_error())
+
+    for _ in range(args.warmup):
+        step()
+    cube.kernel_time()                                  # drop the warm-up launches
+    log(f"warmup {args.warmup} done; timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()                                          # results() waits for the device: each step is complete
+    el = time.perf_counter() - t0
+    kms, launches = cube.kernel_time()
+    lib.cubo_hip_reduce_release(h)
+    ms_per_step = el / args.steps * 1e3
+    value = cube_bytes / (el / args.steps) / 1e9
+    kernel_ms = kms / max(1, launches)
+    achieved = cube_bytes / (kernel_ms / 1e3) / 1e9
+    log(f"step {ms_per_step:.3f} ms ({value:.0f} GB/s); fused kernel {kernel_ms:.3f} ms ({achieved:.0f} GB/s)")
+    assert out["max"].max() > 400 and np.all(np.isfinite(out["images"]))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(cube, bf, adf, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads")
+    traffic = traffic_from_profiles()
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u16", "data": "synthetic",
+        "config": {"workload": "bandwidth_u16: 256x256x128x128 uint16 Poisson cube (2 GiB), one fused pass: "
+                               "BF r=14 + ADF 20-60 images, mean, max, centre of mass",
+                   "cube_bytes": cube_bytes, "pct_hbm_peak_whole_step": round(100 * value / HBM_PEAK_GBS, 2),
+                   "fused_kernel_ms": round(kernel_ms, 4), "kernel_path": "exact integer (fused_u16_int)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,109 @@
+"""ctypes binding of libcubo_hip.so, the C ABI declared in include/cubo_hip.h.
+
+The library is built in-tree by cubokit_amd/build.py. Loading fails loudly when it is missing: there is no
+fallback implementation behind this package.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libcubo_hip.so")
+
+OK = 0
+ERR_NO_DEVICE = -1
+ERR_OUT_OF_MEMORY = -2
+ERR_EXCEEDS_CAP = -3
+ERR_INVALID_ARGUMENT = -4
+ERR_HIP = -5
+ERR_UNSUPPORTED = -6
+ERR_INCOMPLETE = -7
+ERR_EXCEEDS_BUFFER_LIMIT = -8
+
+U16 = 1
+F32 = 2
+
+AVAILABLE = 0
+UNAVAILABLE_NO_DEVICE = 1
+UNAVAILABLE_EXCEEDS_MEMORY_CAP = 2
+UNAVAILABLE_EXCEEDS_BUFFER_LIMIT = 3
+
+
+class DeviceInfo(ct.Structure):
+    _fields_ = [("hbm_bytes", ct.c_uint64), ("free_bytes", ct.c_uint64), ("max_alloc_bytes", ct.c_uint64),
+                ("compute_units", ct.c_int32), ("wavefront_size", ct.c_int32), ("name", ct.c_char * 64),
+                ("arch", ct.c_char * 32)]
+
+
+class ReduceRequest(ct.Structure):
+    _fields_ = [("masks", ct.c_void_p), ("n_masks", ct.c_int32), ("want_mean_max", ct.c_int32),
+                ("want_com", ct.c_int32), ("com_weights", ct.c_void_p)]
+
+
+class ReduceResult(ct.Structure):
+    _fields_ = [(n, ct.c_void_p) for n in ("images", "mean", "max", "com_intensity", "com_moment_row",
+                                           "com_moment_column", "com_row", "com_column")]
+
+
+class SynthParams(ct.Structure):
+    _fields_ = [("seed", ct.c_uint64), ("center_row", ct.c_double), ("center_column", ct.c_double),
+                ("radius", ct.c_double), ("edge", ct.c_double), ("counts", ct.c_double),
+                ("background", ct.c_double), ("n_rings", ct.c_int32), ("ring_q", ct.c_double * 4),
+                ("ring_amplitude", ct.c_double * 4), ("ring_sigma", ct.c_double * 4), ("n_waves", ct.c_int32),
+                ("wave_row_amp", ct.c_double * 8), ("wave_col_amp", ct.c_double * 8), ("wave_fr", ct.c_double * 8),
+                ("wave_fc", ct.c_double * 8), ("wave_phase", ct.c_double * 8), ("poisson", ct.c_int32)]
+
+
+# name -> (restype, argtypes); every symbol include/cubo_hip.h declares.
+SIGNATURES = {
+    "cubo_hip_device_count": (ct.c_int, [ct.POINTER(ct.c_int)]),
+    "cubo_hip_get_device_info": (ct.c_int, [ct.c_int, ct.POINTER(DeviceInfo)]),
+    "cubo_hip_last_error": (ct.c_int, [ct.c_char_p, ct.c_size_t]),
+    "cubo_hip_availability": (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int64), ct.c_int, ct.c_uint64,
+                                         ct.POINTER(ct.c_int), ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_uint64)]),
+    "cubo_hip_cube_create": (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int64), ct.c_int, ct.POINTER(ct.c_void_p)]),
+    "cubo_hip_cube_upload_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p]),
+    "cubo_hip_cube_read_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p]),
+    "cubo_hip_cube_destroy": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_cube_byte_count": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_uint64)]),
+    "cubo_hip_reduce_begin": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ct.c_void_p)]),
+    "cubo_hip_reduce_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64]),
+    "cubo_hip_reduce_end": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceResult)]),
+    "cubo_hip_reduce_cancel": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_reduce_results": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceResult)]),
+    "cubo_hip_reduce_reset": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_reduce_release": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_reduce": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ReduceResult)]),
+    "cubo_hip_reduce_path": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ct.c_int)]),
+    "cubo_hip_cube_kernel_time": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),
+    "cubo_hip_cube_fill_synthetic": (ct.c_int, [ct.c_void_p, ct.POINTER(SynthParams)]),
+}
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library. Raises LibraryMissing when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(f"{LIB_PATH} is not built; run `python cubokit_amd/build.py` "
+                                 "(the HIP path has no fallback)")
+        L = ct.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = ct.create_string_buffer(512)
+    lib().cubo_hip_last_error(buf, 512)
+    return buf.value.decode(errors="replace")

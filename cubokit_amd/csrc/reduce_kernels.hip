@@ -1,0 +1,514 @@
+// reduce_kernels.hip — the fused, cube-streaming reduction for gfx950.
+//
+// Replaces the three per-tile Metal kernels cubo_virtual_image, cubo_sum_and_max and cubo_center_of_mass
+// (Sources/CuboVirtual/MetalReducers.swift:86-176) and the .cpu loops they mirror (CPUReducers.swift:13-97,
+// CenterOfMass.swift:51-79) with one pass over the cube.
+//
+// Layout. The cube is [position][pixel], positions contiguous. A pattern's pixels are cut into slices; a
+// wave owns one slice and walks a contiguous run of positions. Inside a slice, lane l of chunk c holds
+// pixels s0 + (c*64 + l)*VEC ... + VEC-1, so every load instruction of a wave reads one contiguous span
+// (1 KiB for 16-byte loads). Because a lane keeps the same pixels for every position, the column
+// statistics (sum and maximum over positions) stay in registers for the whole run. The row statistics
+// (pattern . weight row) are per-lane partials that a wave reduces through LDS in batches: each lane
+// writes its partials, then each lane sums a row of the transposed batch — one add per value instead of
+// a shuffle tree per position.
+//
+// Column partials of the four waves of a block are combined in LDS and written once per block (one
+// partial per position group); row partials are written once per (slice, position). Two small kernels
+// then sum slices (rows) and groups (columns) in a fixed order, so every result is deterministic.
+#include "reduce_kernels.hpp"
+
+namespace cubo {
+
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t udot2(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_udot2(__builtin_bit_cast(us2, a), __builtin_bit_cast(us2, b), c, false);
+}
+
+__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(us2, a), __builtin_bit_cast(us2, b)));
+}
+
+// The maximum step of the reference: NaN wins, and among equal values the later one is kept
+// (Swift.max(a, b) = b >= a ? b : a; MetalReducers.swift:482, CPUReducers.swift:79).
+__device__ __forceinline__ float max_step(float m, float x) {
+    return (__builtin_isnan(m) || __builtin_isnan(x)) ? __builtin_nanf("") : (x >= m ? x : m);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Row reduction through LDS. ROWS rows of 64 lane values per batch; lane l sums row l % ROWS over the
+// columns of its group l / ROWS, then the groups are combined with shuffles. B = ROWS / NV positions per
+// batch.
+// ------------------------------------------------------------------------------------------------
+constexpr int kLdsStride = kWave + 1;   // one pad word per row: a row's 64 values fall in distinct banks
+
+template <typename Acc, int NV, int ROWS>
+struct RowBatch {
+    static constexpr int B = ROWS / NV;
+    static constexpr int G = kWave / ROWS;       // lane groups per row
+    static constexpr int PER = kWave / G;         // columns each lane sums
+    static_assert(B >= 1, "too many rows per position for one batch");
+    Acc *buf;                                     // this wave's ROWS x kLdsStride region
+
+    __device__ __forceinline__ void put(int b, const Acc (&acc)[NV], int lane) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) buf[(v * B + b) * kLdsStride + lane] = acc[v];
+    }
+
+    template <typename Out>
+    __device__ __forceinline__ void flush(int nb, int64_t first, Out *rowpart_slice, int lane) {
+        wave_sync();
+        const int r = lane % ROWS, g = lane / ROWS;
+        const Acc *row = buf + r * kLdsStride + g * PER;
+        Out s = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) s += (Out)row[k];
+#pragma unroll
+        for (int off = ROWS; off < kWave; off <<= 1) s += shfl_down(s, off);
+        const int v = r / B, b = r % B;
+        if (g == 0 && v < NV && b < nb) rowpart_slice[(first + b) * NV + v] = s;
+        wave_sync();
+    }
+
+    __device__ __forceinline__ static uint64_t shfl_down(uint64_t x, int off) {
+        uint32_t lo = __shfl_down((uint32_t)x, off, kWave), hi = __shfl_down((uint32_t)(x >> 32), off, kWave);
+        return ((uint64_t)hi << 32) | lo;
+    }
+    __device__ __forceinline__ static double shfl_down(double x, int off) { return __shfl_down(x, off, kWave); }
+};
+
+// ------------------------------------------------------------------------------------------------
+// Exact integer kernel: uint16 cube, NW weight rows of uint16 (packed in pairs), optional arithmetic
+// centre-of-mass rows (w = 1: S, sum i*x, sum j*x) when the detector rows are multiples of 8 pixels.
+// ------------------------------------------------------------------------------------------------
+template <int CH, int NW, bool COM1>
+__global__ __launch_bounds__(kBlock) void fused_u16_int(IntLaunch a) {
+    constexpr int NV0 = NW + (COM1 ? 3 : 0);
+    constexpr int NV = NV0 > 0 ? NV0 : 1;       // row outputs (one unused row when only columns are asked)
+    constexpr int ROWS = 32;
+    constexpr int SLICE = kWave * CH * 8;
+    constexpr int LDS_ROWS = kWavesPerBlock * ROWS * kLdsStride;
+    constexpr int LDS_COLS = (kWavesPerBlock - 1) * CH * 12 * kWave;
+    __shared__ uint32_t lds[LDS_ROWS > LDS_COLS ? LDS_ROWS : LDS_COLS];
+
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int slice = blockIdx.x % a.nslices, group = blockIdx.x / a.nslices;
+    const int s0 = slice * SLICE;
+    const int pairs = a.ndet >> 1;
+
+    int px[CH];
+    bool valid[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        px[c] = s0 + (c * kWave + lane) * 8;
+        valid[c] = px[c] < a.ndet;
+    }
+    uint32_t w[NW > 0 ? NW : 1][CH][4];
+#pragma unroll
+    for (int v = 0; v < NW; ++v)
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            uint4 t = valid[c] ? *reinterpret_cast<const uint4 *>(a.wpk + (size_t)v * pairs + (px[c] >> 1))
+                               : make_uint4(0, 0, 0, 0);
+            w[v][c][0] = t.x; w[v][c][1] = t.y; w[v][c][2] = t.z; w[v][c][3] = t.w;
+        }
+    uint32_t ic[CH], jc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        ic[c] = COM1 ? (uint32_t)(px[c] / a.qx) : 0u;
+        jc[c] = COM1 ? (uint32_t)(px[c] % a.qx) : 0u;
+    }
+
+    uint32_t csum[CH][8], cmax[CH][4];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) csum[c][e] = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) cmax[c][e] = 0;
+    }
+
+    RowBatch<uint32_t, NV, ROWS> rb{lds + wave * ROWS * kLdsStride};
+    uint64_t *rowslice = a.rowpart + (size_t)slice * a.npos * NV;
+
+    const int64_t gbeg = (int64_t)group * a.ppg;
+    const int64_t gend = min(a.npos, gbeg + a.ppg);
+    const int64_t pbeg = min(gend, gbeg + (int64_t)wave * a.ppw);
+    const int64_t pend = min(gend, pbeg + a.ppw);
+
+    auto load = [&](u32x4 (&d)[CH], int64_t p) {
+        const uint16_t *row = a.cube + p * a.ndet;
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            d[c] = valid[c] ? __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(row + px[c]))
+                            : u32x4{0u, 0u, 0u, 0u};
+    };
+
+    int b = 0;
+    int64_t first = pbeg;
+    auto process = [&](const u32x4 (&d)[CH]) {
+        uint32_t acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const uint32_t x[4] = {d[c].x, d[c].y, d[c].z, d[c].w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                csum[c][2 * k] += x[k] & 0xffffu;
+                csum[c][2 * k + 1] += x[k] >> 16;
+                cmax[c][k] = pk_max_u16(cmax[c][k], x[k]);
+#pragma unroll
+                for (int v = 0; v < NW; ++v) acc[v] = udot2(x[k], w[v][c][k], acc[v]);
+            }
+            if constexpr (COM1) {
+                uint32_t cs = udot2(x[0], 0x00010001u, 0u);
+                cs = udot2(x[1], 0x00010001u, cs);
+                cs = udot2(x[2], 0x00010001u, cs);
+                cs = udot2(x[3], 0x00010001u, cs);
+                uint32_t jj = udot2(x[0], 0x00010000u, 0u);       // pixel offsets (0, 1)
+                jj = udot2(x[1], 0x00030002u, jj);                 // (2, 3)
+                jj = udot2(x[2], 0x00050004u, jj);                 // (4, 5)
+                jj = udot2(x[3], 0x00070006u, jj);                 // (6, 7)
+                acc[NW] += cs;
+                acc[NW + 1] += ic[c] * cs;
+                acc[NW + 2] += jc[c] * cs + jj;
+            }
+        }
+        if constexpr (NV0 > 0) {
+            rb.put(b, acc, lane);
+            if (++b == rb.B) {
+                rb.flush(b, first, rowslice, lane);
+                first += b;
+                b = 0;
+            }
+        }
+    };
+
+    // Three position buffers in rotation: two positions in flight while one is reduced.
+    u32x4 A[CH], Bf[CH], Cf[CH];
+    int64_t p = pbeg;
+    if (p < pend) load(A, p);
+    if (p + 1 < pend) load(Bf, p + 1);
+    for (; p + 3 <= pend; p += 3) {
+        load(Cf, p + 2);
+        process(A);
+        if (p + 3 < pend) load(A, p + 3);
+        process(Bf);
+        if (p + 4 < pend) load(Bf, p + 4);
+        process(Cf);
+    }
+    if (p < pend) process(A);
+    if (p + 1 < pend) process(Bf);
+    if (b > 0) rb.flush(b, first, rowslice, lane);
+
+    // Combine the four waves' column partials (same pixels, consecutive position runs) and write one
+    // partial per group.
+    __syncthreads();
+    if (wave > 0) {
+        uint32_t *dst = lds + (size_t)(wave - 1) * CH * 12 * kWave;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) dst[(c * 12 + e) * kWave + lane] = csum[c][e];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) dst[(c * 12 + 8 + e) * kWave + lane] = cmax[c][e];
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+#pragma unroll
+            for (int ww = 0; ww < kWavesPerBlock - 1; ++ww) {
+                const uint32_t *src = lds + (size_t)ww * CH * 12 * kWave;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) csum[c][e] += src[(c * 12 + e) * kWave + lane];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) cmax[c][e] = pk_max_u16(cmax[c][e], src[(c * 12 + 8 + e) * kWave + lane]);
+            }
+            if (valid[c]) {
+                uint32_t *cs = a.colsum + (size_t)group * a.ndet + px[c];
+                uint32_t *cm = a.colmax + (size_t)group * a.ndet + px[c];
+                reinterpret_cast<uint4 *>(cs)[0] = make_uint4(csum[c][0], csum[c][1], csum[c][2], csum[c][3]);
+                reinterpret_cast<uint4 *>(cs)[1] = make_uint4(csum[c][4], csum[c][5], csum[c][6], csum[c][7]);
+                reinterpret_cast<uint4 *>(cm)[0] = make_uint4(cmax[c][0] & 0xffffu, cmax[c][0] >> 16,
+                                                              cmax[c][1] & 0xffffu, cmax[c][1] >> 16);
+                reinterpret_cast<uint4 *>(cm)[1] = make_uint4(cmax[c][2] & 0xffffu, cmax[c][2] >> 16,
+                                                              cmax[c][3] & 0xffffu, cmax[c][3] >> 16);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP64 kernel: any pattern length, uint16 or Float32 cube, NW weight rows in Double. One pixel per
+// lane per chunk (2- or 4-byte loads, coalesced across the wave).
+// ------------------------------------------------------------------------------------------------
+template <typename T, int NW>
+__global__ __launch_bounds__(kBlock) void fused_f64(F64Launch a) {
+    constexpr int CH = kF64Chunks;
+    constexpr int ROWS = 16;
+    constexpr int LDS_ROWS = kWavesPerBlock * ROWS * kLdsStride;        // doubles
+    constexpr int LDS_COLS = (kWavesPerBlock - 1) * CH * 3 * kWave / 2 + 1;  // doubles (sum: 2 words, max: 1)
+    __shared__ double lds[LDS_ROWS > LDS_COLS ? LDS_ROWS : LDS_COLS];
+
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int slice = blockIdx.x % a.nslices, group = blockIdx.x / a.nslices;
+    const int s0 = slice * kF64Slice;
+    const T *cube = reinterpret_cast<const T *>(a.cube);
+
+    int px[CH];
+    bool valid[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        px[c] = s0 + c * kWave + lane;
+        valid[c] = px[c] < a.ndet;
+    }
+    double csum[CH];
+    float cmax[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) { csum[c] = -0.0; cmax[c] = -__builtin_inff(); }
+
+    RowBatch<double, NW, ROWS> rb{lds + wave * ROWS * kLdsStride};
+    double *rowslice = a.rowpart + (size_t)slice * a.npos * NW;
+
+    const int64_t gbeg = (int64_t)group * a.ppg;
+    const int64_t gend = min(a.npos, gbeg + a.ppg);
+    const int64_t pbeg = min(gend, gbeg + (int64_t)wave * a.ppw);
+    const int64_t pend = min(gend, pbeg + a.ppw);
+
+    int b = 0;
+    int64_t first = pbeg;
+    for (int64_t p = pbeg; p < pend; ++p) {
+        const T *row = cube + p * a.ndet;
+        T xv[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) xv[c] = valid[c] ? __builtin_nontemporal_load(row + px[c]) : (T)0;
+        double acc[NW];
+#pragma unroll
+        for (int v = 0; v < NW; ++v) acc[v] = 0.0;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            if (!valid[c]) continue;
+            const float xf = (float)xv[c];
+            const double x = (double)xf;
+            csum[c] += x;
+            cmax[c] = max_step(cmax[c], xf);
+#pragma unroll
+            for (int v = 0; v < NW; ++v) acc[v] = fma(x, a.w[(size_t)v * a.ndet + px[c]], acc[v]);
+        }
+        rb.put(b, acc, lane);
+        if (++b == rb.B) {
+            rb.flush(b, first, rowslice, lane);
+            first += b;
+            b = 0;
+        }
+    }
+    if (b > 0) rb.flush(b, first, rowslice, lane);
+
+    __syncthreads();
+    double *dsum = lds;
+    float *dmax = reinterpret_cast<float *>(lds + (kWavesPerBlock - 1) * CH * kWave);
+    if (wave > 0) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            dsum[((wave - 1) * CH + c) * kWave + lane] = csum[c];
+            dmax[((wave - 1) * CH + c) * kWave + lane] = cmax[c];
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+#pragma unroll
+            for (int ww = 0; ww < kWavesPerBlock - 1; ++ww) {      // waves in position order
+                csum[c] += dsum[(ww * CH + c) * kWave + lane];
+                cmax[c] = max_step(cmax[c], dmax[(ww * CH + c) * kWave + lane]);
+            }
+            if (valid[c]) {
+                a.colsum[(size_t)group * a.ndet + px[c]] = csum[c];
+                a.colmax[(size_t)group * a.ndet + px[c]] = cmax[c];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Second-stage kernels.
+// ------------------------------------------------------------------------------------------------
+template <typename Acc>
+__global__ void finalize_rows(const Acc *rowpart, int nslices, int64_t npos, int nv, double *rows_out, int64_t pos0,
+                              int64_t total_pos) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npos * nv) return;
+    const int64_t p = i / nv;
+    const int v = (int)(i % nv);
+    Acc s = 0;
+    for (int sl = 0; sl < nslices; ++sl) s += rowpart[((int64_t)sl * npos + p) * nv + v];
+    rows_out[(int64_t)v * total_pos + pos0 + p] = (double)s;
+}
+
+__global__ void accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups, int ndet,
+                                    uint64_t *sum, uint32_t *max) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndet) return;
+    uint64_t s = sum[i];
+    uint32_t m = max[i];
+    for (int g = 0; g < ngroups; ++g) {
+        s += colsum[(size_t)g * ndet + i];
+        uint32_t v = colmax[(size_t)g * ndet + i];
+        m = v > m ? v : m;
+    }
+    sum[i] = s;
+    max[i] = m;
+}
+
+__global__ void accumulate_cols_f64(const double *colsum, const float *colmax, int ngroups, int ndet, double *sum,
+                                    float *max) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndet) return;
+    double s = sum[i];
+    float m = max[i];
+    for (int g = 0; g < ngroups; ++g) {
+        s += colsum[(size_t)g * ndet + i];
+        m = max_step(m, colmax[(size_t)g * ndet + i]);
+    }
+    sum[i] = s;
+    max[i] = m;
+}
+
+__global__ void finish_cols_int(const uint64_t *sum, const uint32_t *max, int ndet, double count, double *mean,
+                                double *maxd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndet) return;
+    mean[i] = (double)sum[i] / count;
+    maxd[i] = (double)max[i];
+}
+
+__global__ void finish_cols_f64(const double *sum, const float *max, int ndet, double count, double *mean,
+                                double *maxd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndet) return;
+    mean[i] = sum[i] / count;
+    maxd[i] = (double)max[i];
+}
+
+__global__ void fill_cols_init(uint64_t *isum, uint32_t *imax, double *fsum, float *fmax, int ndet) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndet) return;
+    if (isum) { isum[i] = 0; imax[i] = 0; }
+    if (fsum) { fsum[i] = -0.0; fmax[i] = -__builtin_inff(); }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Host launchers.
+// ------------------------------------------------------------------------------------------------
+int int_slice_pixels(int chunks) { return kWave * chunks * 8; }
+
+#define CUBO_INT_CASES(X) \
+    X(4, 0, true) X(4, 0, false) X(4, 1, true) X(4, 2, true) X(4, 4, true) X(2, 8, true) \
+    X(4, 1, false) X(4, 2, false) X(4, 4, false) X(2, 8, false) X(1, 16, false) \
+    X(2, 0, true) X(2, 1, true) X(2, 2, true) X(2, 4, true) \
+    X(2, 1, false) X(2, 2, false) X(2, 4, false) \
+    X(1, 0, true) X(1, 1, true) X(1, 2, true) X(1, 4, true) X(1, 8, true) \
+    X(1, 1, false) X(1, 2, false) X(1, 4, false) X(1, 8, false)
+
+bool supported_int(int nw, int chunks, bool com_ones) {
+#define X(CH, NW, C1) if (chunks == CH && nw == NW && com_ones == C1) return true;
+    CUBO_INT_CASES(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_int(const IntLaunch &a, int nw, int chunks, bool com_ones, hipStream_t s) {
+    const dim3 grid((unsigned)(a.nslices * a.ngroups)), block(kBlock);
+#define X(CH, NW, C1) \
+    if (chunks == CH && nw == NW && com_ones == C1) { \
+        hipLaunchKernelGGL((fused_u16_int<CH, NW, C1>), grid, block, 0, s, a); \
+        return hipGetLastError(); \
+    }
+    CUBO_INT_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+#define CUBO_F64_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(11) X(16)
+
+bool supported_f64(int nw) {
+#define X(NW) if (nw == NW) return true;
+    CUBO_F64_CASES(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_f64(const F64Launch &a, bool is_u16, int nw, hipStream_t s) {
+    const dim3 grid((unsigned)(a.nslices * a.ngroups)), block(kBlock);
+#define X(NW) \
+    if (nw == NW) { \
+        if (is_u16) hipLaunchKernelGGL((fused_f64<uint16_t, NW>), grid, block, 0, s, a); \
+        else hipLaunchKernelGGL((fused_f64<float, NW>), grid, block, 0, s, a); \
+        return hipGetLastError(); \
+    }
+    CUBO_F64_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+static unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t launch_finalize_rows_u64(const uint64_t *rowpart, int nslices, int64_t npos, int nv, double *rows_out,
+                                    int64_t pos0, int64_t total_pos, hipStream_t s) {
+    hipLaunchKernelGGL(finalize_rows<uint64_t>, dim3(blocks_for(npos * nv, 256)), dim3(256), 0, s, rowpart, nslices,
+                       npos, nv, rows_out, pos0, total_pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_rows_f64(const double *rowpart, int nslices, int64_t npos, int nv, double *rows_out,
+                                    int64_t pos0, int64_t total_pos, hipStream_t s) {
+    hipLaunchKernelGGL(finalize_rows<double>, dim3(blocks_for(npos * nv, 256)), dim3(256), 0, s, rowpart, nslices,
+                       npos, nv, rows_out, pos0, total_pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups, int ndet,
+                                      uint64_t *sum, uint32_t *max, hipStream_t s) {
+    hipLaunchKernelGGL(accumulate_cols_int, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, colsum, colmax, ngroups,
+                       ndet, sum, max);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate_cols_f64(const double *colsum, const float *colmax, int ngroups, int ndet,
+                                      double *sum, float *max, hipStream_t s) {
+    hipLaunchKernelGGL(accumulate_cols_f64, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, colsum, colmax, ngroups,
+                       ndet, sum, max);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish_cols_int(const uint64_t *sum, const uint32_t *max, int ndet, double count, double *mean,
+                                  double *maxd, hipStream_t s) {
+    hipLaunchKernelGGL(finish_cols_int, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, sum, max, ndet, count, mean,
+                       maxd);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish_cols_f64(const double *sum, const float *max, int ndet, double count, double *mean,
+                                  double *maxd, hipStream_t s) {
+    hipLaunchKernelGGL(finish_cols_f64, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, sum, max, ndet, count, mean,
+                       maxd);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_cols_init(uint64_t *isum, uint32_t *imax, double *fsum, float *fmax, int ndet, hipStream_t s) {
+    hipLaunchKernelGGL(fill_cols_init, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, isum, imax, fsum, fmax, ndet);
+    return hipGetLastError();
+}
+
+}  // namespace cubo

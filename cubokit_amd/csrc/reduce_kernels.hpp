@@ -1,0 +1,79 @@
+// reduce_kernels.hpp — launch interface of the fused streaming reduction kernels (reduce_kernels.hip).
+//
+// One pass over a resident cube yields, per scan position, the dot products of the pattern with a stack
+// of weight rows (virtual-detector masks and the centre-of-mass weights w, w*i, w*j), and, per detector
+// pixel, the partial sums and maxima over the positions of each position group. Two kernel families:
+//   - exact integer (uint16 cube, non-negative integral weights): per-lane uint32 partials from packed
+//     16-bit dot products, uint64 totals, so every output is the exact integer;
+//   - FP64 (uint16 or Float32 cube, any finite weights): per-lane double accumulation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cubo {
+
+// Lanes per wave and waves per block of the streaming kernels.
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = kWave * kWavesPerBlock;
+
+struct IntLaunch {
+    const uint16_t *cube;   // first position of this launch
+    int64_t npos;           // positions in this launch
+    int32_t ndet;           // pixels per pattern (multiple of 8)
+    int32_t qx;             // detector columns (multiple of 8 for the arithmetic CoM rows)
+    int32_t nslices;        // pixel slices per pattern
+    int32_t ngroups;        // position groups
+    int32_t ppg;            // positions per group
+    int32_t ppw;            // positions per wave
+    const uint32_t *wpk;    // [nw][ndet/2] weights as packed uint16 pairs
+    uint64_t *rowpart;      // [nslices][npos][nv]
+    uint32_t *colsum;       // [ngroups][ndet]
+    uint32_t *colmax;       // [ngroups][ndet]
+};
+
+struct F64Launch {
+    const void *cube;       // first position of this launch (uint16_t or float)
+    int64_t npos;
+    int32_t ndet;
+    int32_t nslices;
+    int32_t ngroups;
+    int32_t ppg;
+    int32_t ppw;
+    const double *w;        // [nw][ndet] weight rows in Double
+    double *rowpart;        // [nslices][npos][nw]
+    double *colsum;         // [ngroups][ndet]
+    float *colmax;          // [ngroups][ndet]
+};
+
+// Pixels per lane of the integer kernel (chunks of 8) and of the FP64 kernel (one pixel per chunk).
+int int_slice_pixels(int chunks);
+constexpr int kF64Chunks = 16;
+constexpr int kF64Slice = kWave * kF64Chunks;
+
+// Returns hipSuccess or the launch error. nw / com_ones select the instantiation; the caller checks
+// that (nw, chunks) is one of the compiled combinations with supported_int().
+bool supported_int(int nw, int chunks, bool com_ones);
+hipError_t launch_int(const IntLaunch &a, int nw, int chunks, bool com_ones, hipStream_t s);
+bool supported_f64(int nw);
+hipError_t launch_f64(const F64Launch &a, bool is_u16, int nw, hipStream_t s);
+
+// Row totals over slices into rows_out[v][pos0 + p] (Double), v < nv.
+hipError_t launch_finalize_rows_u64(const uint64_t *rowpart, int nslices, int64_t npos, int nv, double *rows_out,
+                                    int64_t pos0, int64_t total_pos, hipStream_t s);
+hipError_t launch_finalize_rows_f64(const double *rowpart, int nslices, int64_t npos, int nv, double *rows_out,
+                                    int64_t pos0, int64_t total_pos, hipStream_t s);
+// Column partials of one launch, in group order, into the reduction's running sums and maxima.
+hipError_t launch_accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups, int ndet,
+                                      uint64_t *sum, uint32_t *max, hipStream_t s);
+hipError_t launch_accumulate_cols_f64(const double *colsum, const float *colmax, int ngroups, int ndet,
+                                      double *sum, float *max, hipStream_t s);
+// mean = sum / count, max widened; CoM quotients.
+hipError_t launch_finish_cols_int(const uint64_t *sum, const uint32_t *max, int ndet, double count, double *mean,
+                                  double *maxd, hipStream_t s);
+hipError_t launch_finish_cols_f64(const double *sum, const float *max, int ndet, double count, double *mean,
+                                  double *maxd, hipStream_t s);
+hipError_t launch_fill_cols_init(uint64_t *isum, uint32_t *imax, double *fsum, float *fmax, int ndet, hipStream_t s);
+
+}  // namespace cubo

@@ -1,0 +1,57 @@
+"""Seeded synthetic cubes of the benchmark workloads, generated on the device (cubo_hip_cube_fill_synthetic).
+
+Workloads (scan rows x scan columns x detector rows x detector columns):
+- ``small_f32``: 20x20x512x512 Float32; disk r=30 with a 2 px sigmoid edge, 1000 counts/px, centre (256, 256)
+  deflected by (3 sin(2 pi x/20), 3 cos(2 pi y/20)); background 2 counts/px; Poisson.
+- ``bandwidth_u16``: 256x256x128x128 uint16; disk r=12 at 500 counts/px with a +-2 px sinusoidal deflection;
+  Gaussian rings at q=30 and q=45 px, amplitude 20; Poisson.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+
+def params(seed: int, center, radius: float, counts: float, edge: float = 0.0, background: float = 0.0,
+           rings=(), waves=(), poisson: bool = True) -> L.SynthParams:
+    p = L.SynthParams()
+    p.seed = seed
+    p.center_row, p.center_column = center
+    p.radius, p.edge, p.counts, p.background = radius, edge, counts, background
+    p.n_rings = len(rings)
+    for k, (q, a, s) in enumerate(rings):
+        p.ring_q[k], p.ring_amplitude[k], p.ring_sigma[k] = q, a, s
+    p.n_waves = len(waves)
+    for k, (ar, ac, fr, fc, ph) in enumerate(waves):
+        p.wave_row_amp[k], p.wave_col_amp[k], p.wave_fr[k], p.wave_fc[k], p.wave_phase[k] = ar, ac, fr, fc, ph
+    p.poisson = int(poisson)
+    return p
+
+
+def small_f32(seed: int = 20261101):
+    """The 20x20x512x512 Float32 parity workload. The deflection is 3 sin(2 pi x/20) along rows and
+    3 cos(2 pi y/20) along columns: two waves, one per scan axis (the other amplitude zero)."""
+    shape = (20, 20, 512, 512)
+    waves = [(3.0, 0.0, 1 / 20, 0.0, 0.0), (0.0, 3.0, 0.0, 1 / 20, 0.0)]
+    return shape, np.float32, params(seed, (256.0, 256.0), 30.0, 1000.0, edge=2.0, background=2.0, waves=waves)
+
+
+def bandwidth_u16(seed: int = 20261102, scan=(256, 256)):
+    """The 256x256x128x128 uint16 bandwidth workload (2 GiB)."""
+    shape = (scan[0], scan[1], 128, 128)
+    waves = [(2.0, 0.0, 1 / 64, 0.0, 0.0), (0.0, 2.0, 0.0, 1 / 64, 0.0)]
+    rings = [(30.0, 20.0, 1.5), (45.0, 20.0, 1.5)]
+    return shape, np.uint16, params(seed, (64.0, 64.0), 12.0, 500.0, edge=0.5, rings=rings, waves=waves)
+
+
+def random_waves(rng: np.random.Generator, n: int, amplitude: float, scan):
+    """n deflection waves whose summed amplitude stays within +-amplitude."""
+    out = []
+    for _ in range(n):
+        ar, ac = rng.uniform(-1, 1, 2) * amplitude / n
+        fr, fc = rng.integers(1, 4, 2) / np.array(scan, float)
+        out.append((float(ar), float(ac), float(fr), float(fc), float(rng.uniform(0, 2 * math.pi))))
+    return out

@@ -1,0 +1,600 @@
+"""Host-side mirror of CuboKit's CuboVirtual API for the `.hip` compute path.
+
+Names, argument meanings and refusals follow the Swift reference:
+- CubeShape and ScanTiling: Sources/CuboCore/CubeShape.swift:11-58, Sources/CuboCore/CubeSource.swift:85-112;
+- DetectorGeometry and DetectorMask.make: Sources/CuboVirtual/DetectorGeometry.swift:23-98;
+- ComputePath, ScanImage, DetectorImage, CuboVirtualError: Sources/CuboVirtual/Results.swift:13-97;
+- VirtualImaging.image, VirtualDiffraction.mean/max/meanAndMax: Sources/CuboVirtual/VirtualImaging.swift:20-66;
+- CenterOfMass.measure and CenterOfMassMaps: Sources/CuboVirtual/CenterOfMass.swift:14-47;
+- ResidentCube (here HIPResidentCube): Sources/CuboVirtual/ResidentCube.swift:31-128;
+- progress / isCancelled: Sources/CuboVirtual/Interruption.swift:10-25.
+
+Every reduction runs on the GPU through the C ABI (include/cubo_hip.h); this module only prepares masks,
+walks the scan tiles in order (asking `is_cancelled` before each, reporting `progress` after each) and wraps
+the Double results.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import enum
+import math
+from dataclasses import dataclass
+from typing import Callable, Optional, Sequence, Union
+
+import numpy as np
+
+from . import _lib as L
+
+# ------------------------------------------------------------------------------------------------
+# Errors
+# ------------------------------------------------------------------------------------------------
+
+
+class CuboError(Exception):
+    """CuboCore errors (Sources/CuboCore/CuboError.swift)."""
+
+
+class CuboVirtualError(Exception):
+    """CuboVirtualError (Results.swift:65-97) plus the `.hip` cases that replace `.noMetalDevice` / `.metal`."""
+
+    def __init__(self, case: str, detail: str = ""):
+        super().__init__(f"{case}: {detail}" if detail else case)
+        self.case = case
+        self.detail = detail
+
+
+class CancellationError(Exception):
+    """Thrown when `is_cancelled` answers True before a tile (Interruption.swift:15-18)."""
+
+
+class ResidentCubeError(Exception):
+    """ResidentCubeError.unavailable(reason) (ResidentCube.swift:14-21)."""
+
+    def __init__(self, reason: "UnavailableReason"):
+        super().__init__(reason.message)
+        self.reason = reason
+
+
+def _check(status: int, what: str) -> None:
+    if status == L.OK:
+        return
+    msg = L.last_error()
+    if status == L.ERR_NO_DEVICE:
+        raise CuboVirtualError("noHIPDevice", msg or what)
+    if status == L.ERR_INVALID_ARGUMENT:
+        raise CuboVirtualError("invalidInput", f"{what}: {msg}")
+    raise CuboVirtualError("hip", f"{what}: {msg} (status {status})")
+
+
+# ------------------------------------------------------------------------------------------------
+# Shape and tiling
+# ------------------------------------------------------------------------------------------------
+
+
+@dataclass(frozen=True)
+class CubeShape:
+    scan_rows: int
+    scan_columns: int
+    detector_rows: int
+    detector_columns: int
+
+    def __post_init__(self):
+        ext = self.extents
+        if any(int(e) <= 0 for e in ext) or math.prod(ext) * 8 >= 2 ** 63:
+            raise CuboError(f"invalid shape {ext}")
+
+    @staticmethod
+    def of(extents: Sequence[int]) -> "CubeShape":
+        if len(extents) != 4:
+            raise CuboError(f"invalid shape {list(extents)}")
+        return CubeShape(*[int(e) for e in extents])
+
+    @property
+    def extents(self):
+        return [self.scan_rows, self.scan_columns, self.detector_rows, self.detector_columns]
+
+    @property
+    def scan_count(self):
+        return self.scan_rows * self.scan_columns
+
+    @property
+    def detector_count(self):
+        return self.detector_rows * self.detector_columns
+
+    @property
+    def element_count(self):
+        return self.scan_count * self.detector_count
+
+    @property
+    def elements_per_scan_row(self):
+        return self.scan_columns * self.detector_count
+
+
+class ScanTiling:
+    """`.automatic` (largest row count whose Float32 tile is at most 64 MiB) or `.rows(n)`."""
+
+    automatic_tile_bytes = 64 * 1024 * 1024
+
+    def __init__(self, rows: Optional[int] = None):
+        self._rows = rows
+
+    @staticmethod
+    def rows(n: int) -> "ScanTiling":
+        return ScanTiling(n)
+
+    def rows_per_tile(self, shape: CubeShape) -> int:
+        if self._rows is None:
+            row_bytes = shape.elements_per_scan_row * 4
+            return max(1, min(shape.scan_rows, self.automatic_tile_bytes // max(1, row_bytes)))
+        if self._rows <= 0:
+            raise CuboError(f"invalid tiling {self._rows}")
+        return min(self._rows, shape.scan_rows)
+
+    def ranges(self, shape: CubeShape):
+        step = self.rows_per_tile(shape)
+        return [range(r, min(shape.scan_rows, r + step)) for r in range(0, shape.scan_rows, step)]
+
+    def __repr__(self):
+        return "ScanTiling.automatic" if self._rows is None else f"ScanTiling.rows({self._rows})"
+
+
+ScanTiling.automatic = ScanTiling()
+
+# ------------------------------------------------------------------------------------------------
+# Results
+# ------------------------------------------------------------------------------------------------
+
+
+class ComputePath(str, enum.Enum):
+    """Where a reduction ran. `.cpu` and `.metal` are CuboKit's Swift paths; this package is the `.hip` path."""
+    cpu = "cpu"
+    metal = "metal"
+    hip = "hip"
+
+
+@dataclass
+class ScanImage:
+    scan_rows: int
+    scan_columns: int
+    values: np.ndarray
+    path: ComputePath
+
+    def __post_init__(self):
+        if self.values.size != self.scan_rows * self.scan_columns:
+            raise ValueError("ScanImage: wrong number of values")
+
+    def __getitem__(self, rc):
+        return self.values[rc[0] * self.scan_columns + rc[1]]
+
+
+@dataclass
+class DetectorImage:
+    detector_rows: int
+    detector_columns: int
+    values: np.ndarray
+    path: ComputePath
+
+    def __getitem__(self, rc):
+        return self.values[rc[0] * self.detector_columns + rc[1]]
+
+
+@dataclass
+class CenterOfMassMaps:
+    row: ScanImage
+    column: ScanImage
+    intensity: ScanImage
+    moment_row: ScanImage
+    moment_column: ScanImage
+
+
+# ------------------------------------------------------------------------------------------------
+# Detector geometry (py4DSTEM make_detector, evaluated in Double)
+# ------------------------------------------------------------------------------------------------
+
+
+@dataclass(frozen=True)
+class DetectorGeometry:
+    kind: str
+    params: tuple
+
+    @staticmethod
+    def point(row: int, column: int):
+        return DetectorGeometry("point", (int(row), int(column)))
+
+    @staticmethod
+    def circle(center_row: float, center_column: float, radius: float):
+        return DetectorGeometry("circle", (float(center_row), float(center_column), float(radius)))
+
+    @staticmethod
+    def annulus(center_row: float, center_column: float, inner: float, outer: float):
+        return DetectorGeometry("annulus", (float(center_row), float(center_column), float(inner), float(outer)))
+
+    @staticmethod
+    def rectangle(row_min: float, row_max: float, column_min: float, column_max: float):
+        return DetectorGeometry("rectangle", (float(row_min), float(row_max), float(column_min), float(column_max)))
+
+    @staticmethod
+    def mask(weights):
+        return DetectorGeometry("mask", (np.asarray(weights, dtype=np.float32).reshape(-1),))
+
+
+class DetectorMask:
+    @staticmethod
+    def make(geometry: DetectorGeometry, shape: CubeShape = None, detector_rows: int = None,
+             detector_columns: int = None) -> np.ndarray:
+        """The Float32 mask of `geometry`, row-major (DetectorGeometry.swift:43-98)."""
+        qy = shape.detector_rows if shape is not None else detector_rows
+        qx = shape.detector_columns if shape is not None else detector_columns
+        if qy is None or qx is None or qy <= 0 or qx <= 0:
+            raise CuboError(f"invalid shape {[qy, qx]}")
+        m = np.zeros(qy * qx, np.float32)
+        k, p = geometry.kind, geometry.params
+        if k == "point":
+            r, c = p
+            if not (0 <= r < qy and 0 <= c < qx):
+                raise CuboVirtualError("pointOutsideDetector", f"({r}, {c})")
+            m[r * qx + c] = 1
+        elif k in ("circle", "annulus"):
+            if not all(math.isfinite(v) for v in p):
+                raise CuboVirtualError("nonFiniteGeometry")
+            cr, cc = p[0], p[1]
+            di = np.arange(qy, dtype=np.float64) - cr
+            dj = np.arange(qx, dtype=np.float64) - cc
+            d2 = (di * di)[:, None] + (dj * dj)[None, :]
+            if k == "circle":
+                if p[2] < 0:
+                    raise CuboVirtualError("negativeRadius", str(p[2]))
+                sel = d2 < p[2] * p[2]
+            else:
+                inner, outer = p[2], p[3]
+                if inner < 0:
+                    raise CuboVirtualError("negativeRadius", str(inner))
+                if not outer > inner:
+                    raise CuboVirtualError("innerNotBelowOuter", f"{inner} {outer}")
+                sel = (d2 > inner * inner) & (d2 < outer * outer)
+            m[sel.reshape(-1)] = 1
+        elif k == "rectangle":
+            if not all(math.isfinite(v) for v in p):
+                raise CuboVirtualError("nonFiniteGeometry")
+            rb = [float(np.round(v)) for v in p]          # numpy round: half to even
+            if any(v < 0 for v in rb):
+                raise CuboVirtualError("negativeRectangleBound")
+            r0, r1 = int(min(rb[0], qy)), int(min(rb[1], qy))
+            c0, c1 = int(min(rb[2], qx)), int(min(rb[3], qx))
+            if r0 < r1 and c0 < c1:
+                m.reshape(qy, qx)[r0:r1, c0:c1] = 1
+        elif k == "mask":
+            w = p[0]
+            if w.size != qy * qx:
+                raise CuboVirtualError("maskLengthMismatch", f"expected {qy * qx}, received {w.size}")
+            if not np.all(np.isfinite(w)):
+                raise CuboVirtualError("nonFiniteGeometry")
+            m = w.astype(np.float32).copy()
+        else:
+            raise ValueError(k)
+        return m
+
+
+# ------------------------------------------------------------------------------------------------
+# Devices and the resident cube
+# ------------------------------------------------------------------------------------------------
+
+
+def device_count() -> int:
+    n = ct.c_int(0)
+    _check(L.lib().cubo_hip_device_count(ct.byref(n)), "device count")
+    return n.value
+
+
+def hip_is_available() -> bool:
+    """Whether this machine can run the `.hip` path (VirtualImaging.swift:69-72's metalIsAvailable)."""
+    try:
+        return device_count() > 0
+    except (L.LibraryMissing, OSError):
+        return False
+
+
+def device_info(device: int = 0) -> L.DeviceInfo:
+    info = L.DeviceInfo()
+    _check(L.lib().cubo_hip_get_device_info(device, ct.byref(info)), "device info")
+    return info
+
+
+@dataclass(frozen=True)
+class UnavailableReason:
+    kind: str                  # "noHIPDevice" | "exceedsMemoryCap" | "exceedsBufferLimit"
+    bytes: int = 0
+    cap: int = 0
+
+    @property
+    def message(self) -> str:
+        if self.kind == "noHIPDevice":
+            return "This machine has no HIP device, so the cube cannot be held resident."
+        if self.kind == "exceedsMemoryCap":
+            return (f"The cube is {self.bytes / 1e9:.1f} GB in device memory; a resident cube may use at most "
+                    f"{self.cap / 1e9:.1f} GB of HBM.")
+        return f"The cube is {self.bytes / 1e9:.1f} GB; that much HBM is not free."
+
+
+_DTYPES = {np.dtype(np.uint16): L.U16, np.dtype(np.float32): L.F32}
+
+
+def native_dtype(dtype) -> np.dtype:
+    """uint16 stays native; every other element type is held as Float32, the reference's read type."""
+    return np.dtype(np.uint16) if np.dtype(dtype) == np.uint16 else np.dtype(np.float32)
+
+
+class HIPResidentCube:
+    """A cube held in one HBM buffer in its native element type (uint16) or Float32."""
+
+    memory_cap_fraction = 0.9
+
+    def __init__(self, loading=None, *, shape: CubeShape = None, dtype=np.uint16, tiling: ScanTiling = None,
+                 device: int = 0):
+        if loading is not None:
+            arr = np.asarray(loading)
+            if arr.ndim != 4:
+                raise CuboError(f"invalid shape {list(arr.shape)}")
+            shape = CubeShape.of(arr.shape)
+            dtype = native_dtype(arr.dtype)
+        if shape is None:
+            raise CuboError("a shape or an array to load is needed")
+        self.shape = shape
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in _DTYPES:
+            raise CuboError(f"unsupported resident dtype {self.dtype}")
+        self.device = device
+        avail = HIPResidentCube.availability(shape, self.dtype, device=device)
+        if avail is not None and avail.kind != "noHIPDevice":
+            raise ResidentCubeError(avail)
+        h = ct.c_void_p()
+        ext = (ct.c_int64 * 4)(*shape.extents)
+        _check(L.lib().cubo_hip_cube_create(device, ext, _DTYPES[self.dtype], ct.byref(h)), "allocating the cube")
+        self._h = h
+        if loading is not None:
+            for rg in (tiling or ScanTiling.automatic).ranges(shape):
+                tile = np.ascontiguousarray(arr[rg.start:rg.stop].astype(self.dtype, copy=False))
+                _check(L.lib().cubo_hip_cube_upload_rows(self._h, rg.start, len(rg),
+                                                         tile.ctypes.data_as(ct.c_void_p)), "uploading")
+
+    @staticmethod
+    def availability(shape: CubeShape, dtype=np.uint16, hbm_bytes: int = 0, device: int = 0):
+        """None when available, else the UnavailableReason (checks the cap, then the device, then the buffer)."""
+        reason, b, cap = ct.c_int(), ct.c_uint64(), ct.c_uint64()
+        ext = (ct.c_int64 * 4)(*shape.extents)
+        _check(L.lib().cubo_hip_availability(device, ext, _DTYPES[np.dtype(dtype)], hbm_bytes, ct.byref(reason),
+                                             ct.byref(b), ct.byref(cap)), "availability")
+        kind = {L.AVAILABLE: None, L.UNAVAILABLE_NO_DEVICE: "noHIPDevice",
+                L.UNAVAILABLE_EXCEEDS_MEMORY_CAP: "exceedsMemoryCap",
+                L.UNAVAILABLE_EXCEEDS_BUFFER_LIMIT: "exceedsBufferLimit"}[reason.value]
+        return None if kind is None else UnavailableReason(kind, b.value, cap.value)
+
+    @staticmethod
+    def fits(shape: CubeShape, dtype=np.uint16) -> bool:
+        return HIPResidentCube.availability(shape, dtype) is None
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise CuboVirtualError("hip", "the cube has been released")
+        return self._h
+
+    @property
+    def byte_count(self) -> int:
+        b = ct.c_uint64()
+        _check(L.lib().cubo_hip_cube_byte_count(self.handle, ct.byref(b)), "byte count")
+        return b.value
+
+    def read_scan_tile(self, rows: range) -> np.ndarray:
+        out = np.empty((len(rows), self.shape.scan_columns, self.shape.detector_rows, self.shape.detector_columns),
+                       self.dtype)
+        _check(L.lib().cubo_hip_cube_read_rows(self.handle, rows.start, len(rows), out.ctypes.data_as(ct.c_void_p)),
+               "reading rows")
+        return out
+
+    def read_pattern(self, row: int, column: int) -> np.ndarray:
+        if not (0 <= row < self.shape.scan_rows and 0 <= column < self.shape.scan_columns):
+            raise CuboError(f"position ({row}, {column}) outside the scan")
+        return self.read_scan_tile(range(row, row + 1))[0, column].copy()
+
+    def fill_synthetic(self, params: L.SynthParams) -> None:
+        _check(L.lib().cubo_hip_cube_fill_synthetic(self.handle, ct.byref(params)), "synthesising the cube")
+
+    def kernel_time(self):
+        """(total ms, launches) of the fused streaming kernel since the last query (HIP events)."""
+        t, n = ct.c_double(), ct.c_int64()
+        _check(L.lib().cubo_hip_cube_kernel_time(self.handle, ct.byref(t), ct.byref(n)), "kernel time")
+        return t.value, n.value
+
+    def release(self):
+        if getattr(self, "_h", None) is not None:
+            L.lib().cubo_hip_cube_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.release()
+
+
+Source = Union[HIPResidentCube, np.ndarray]
+
+
+def _resident(source: Source, tiling: ScanTiling):
+    """A resident cube for `source` and whether it is a temporary staging copy (the TileFeed staging case,
+    MetalReducers.swift:504-573)."""
+    if isinstance(source, HIPResidentCube):
+        return source, False
+    return HIPResidentCube(np.asarray(source), tiling=tiling), True
+
+
+# ------------------------------------------------------------------------------------------------
+# The reductions
+# ------------------------------------------------------------------------------------------------
+
+Progress = Optional[Callable[[float], None]]
+IsCancelled = Optional[Callable[[], bool]]
+
+
+def _require_hip(path: ComputePath):
+    if ComputePath(path) != ComputePath.hip:
+        raise CuboVirtualError("unsupportedPath", f"this package implements the .hip path, not .{ComputePath(path).value}")
+
+
+def _reduce(source: Source, *, masks: Optional[np.ndarray], want_cols: bool, want_com: bool,
+            com_weights: Optional[np.ndarray], tiling: ScanTiling, progress: Progress, is_cancelled: IsCancelled):
+    tiling = tiling or ScanTiling.automatic
+    if is_cancelled is not None and is_cancelled():
+        raise CancellationError()
+    cube, temporary = _resident(source, tiling)
+    try:
+        shape = cube.shape
+        n, npos = shape.detector_count, shape.scan_count
+        m = None if masks is None else np.ascontiguousarray(masks, np.float32).reshape(-1, n)
+        cw = None if com_weights is None else np.ascontiguousarray(com_weights, np.float32).reshape(n)
+        req = L.ReduceRequest(masks=None if m is None else m.ctypes.data, n_masks=0 if m is None else m.shape[0],
+                              want_mean_max=int(want_cols), want_com=int(want_com),
+                              com_weights=None if cw is None else cw.ctypes.data)
+        out = {}
+        res = L.ReduceResult()
+        if m is not None:
+            out["images"] = np.empty((m.shape[0], npos))
+            res.images = out["images"].ctypes.data
+        if want_cols:
+            out["mean"], out["max"] = np.empty(n), np.empty(n)
+            res.mean, res.max = out["mean"].ctypes.data, out["max"].ctypes.data
+        if want_com:
+            for k in ("com_intensity", "com_moment_row", "com_moment_column", "com_row", "com_column"):
+                out[k] = np.empty(npos)
+                setattr(res, k, out[k].ctypes.data)
+        h = ct.c_void_p()
+        _check(L.lib().cubo_hip_reduce_begin(cube.handle, ct.byref(req), ct.byref(h)), "starting the reduction")
+        try:
+            for rg in tiling.ranges(shape):
+                if is_cancelled is not None and is_cancelled():
+                    raise CancellationError()
+                _check(L.lib().cubo_hip_reduce_rows(h, rg.start, len(rg)), "reducing rows")
+                if progress is not None:
+                    progress(rg.stop / shape.scan_rows)
+        except BaseException:
+            L.lib().cubo_hip_reduce_cancel(h)
+            raise
+        _check(L.lib().cubo_hip_reduce_end(h, ct.byref(res)), "finishing the reduction")
+        return shape, out
+    finally:
+        if temporary:
+            cube.release()
+
+
+def reduce_path(source: HIPResidentCube, masks=None, want_com=False, com_weights=None) -> int:
+    """1 = exact integer kernel, 2 = FP64 kernel, for this request on this cube."""
+    n = source.shape.detector_count
+    m = None if masks is None else np.ascontiguousarray(masks, np.float32).reshape(-1, n)
+    cw = None if com_weights is None else np.ascontiguousarray(com_weights, np.float32)
+    req = L.ReduceRequest(masks=None if m is None else m.ctypes.data, n_masks=0 if m is None else m.shape[0],
+                          want_mean_max=1, want_com=int(want_com), com_weights=None if cw is None else cw.ctypes.data)
+    p = ct.c_int()
+    _check(L.lib().cubo_hip_reduce_path(source.handle, ct.byref(req), ct.byref(p)), "path")
+    return p.value
+
+
+def _shape_of(source: Source) -> CubeShape:
+    return source.shape if isinstance(source, HIPResidentCube) else CubeShape.of(np.asarray(source).shape)
+
+
+class VirtualImaging:
+    @staticmethod
+    def image(source: Source, geometry: DetectorGeometry, path: ComputePath = ComputePath.hip,
+              tiling: ScanTiling = None, progress: Progress = None, is_cancelled: IsCancelled = None) -> ScanImage:
+        """For every scan position, the sum over the detector of pattern x mask (VirtualImaging.swift:20-31)."""
+        _require_hip(path)
+        shape = _shape_of(source)
+        mask = DetectorMask.make(geometry, shape)
+        return VirtualImaging.images(source, mask[None], tiling=tiling, progress=progress,
+                                     is_cancelled=is_cancelled)[0]
+
+    @staticmethod
+    def images(source: Source, masks: np.ndarray, tiling: ScanTiling = None, progress: Progress = None,
+               is_cancelled: IsCancelled = None):
+        """Several masks in one pass over the cube (at most 16 weight rows per pass)."""
+        shape, out = _reduce(source, masks=masks, want_cols=False, want_com=False, com_weights=None, tiling=tiling,
+                             progress=progress, is_cancelled=is_cancelled)
+        return [ScanImage(shape.scan_rows, shape.scan_columns, v, ComputePath.hip) for v in out["images"]]
+
+
+class VirtualDiffraction:
+    @staticmethod
+    def mean(source: Source, path: ComputePath = ComputePath.hip, tiling: ScanTiling = None,
+             progress: Progress = None, is_cancelled: IsCancelled = None) -> DetectorImage:
+        return VirtualDiffraction.mean_and_max(source, path, tiling, progress, is_cancelled)[0]
+
+    @staticmethod
+    def max(source: Source, path: ComputePath = ComputePath.hip, tiling: ScanTiling = None,
+            progress: Progress = None, is_cancelled: IsCancelled = None) -> DetectorImage:
+        return VirtualDiffraction.mean_and_max(source, path, tiling, progress, is_cancelled)[1]
+
+    @staticmethod
+    def mean_and_max(source: Source, path: ComputePath = ComputePath.hip, tiling: ScanTiling = None,
+                     progress: Progress = None, is_cancelled: IsCancelled = None):
+        """Both patterns in one pass over the cube (VirtualImaging.swift:56-66)."""
+        _require_hip(path)
+        shape, out = _reduce(source, masks=None, want_cols=True, want_com=False, com_weights=None, tiling=tiling,
+                             progress=progress, is_cancelled=is_cancelled)
+        qy, qx = shape.detector_rows, shape.detector_columns
+        return (DetectorImage(qy, qx, out["mean"], ComputePath.hip), DetectorImage(qy, qx, out["max"], ComputePath.hip))
+
+    @staticmethod
+    def hip_is_available() -> bool:
+        return hip_is_available()
+
+
+class CenterOfMass:
+    @staticmethod
+    def measure(source: Source, mask: Optional[DetectorGeometry] = None, path: ComputePath = ComputePath.hip,
+                tiling: ScanTiling = None, progress: Progress = None,
+                is_cancelled: IsCancelled = None) -> CenterOfMassMaps:
+        """The centre of mass of every pattern, weighted by `mask` (CenterOfMass.swift:27-47)."""
+        _require_hip(path)
+        shape = _shape_of(source)
+        w = None if mask is None else DetectorMask.make(mask, shape)
+        shape, out = _reduce(source, masks=None, want_cols=False, want_com=True, com_weights=w, tiling=tiling,
+                             progress=progress, is_cancelled=is_cancelled)
+        return _com_maps(shape, out)
+
+
+def _com_maps(shape: CubeShape, out: dict) -> CenterOfMassMaps:
+    def img(v):
+        return ScanImage(shape.scan_rows, shape.scan_columns, v, ComputePath.hip)
+    return CenterOfMassMaps(row=img(out["com_row"]), column=img(out["com_column"]), intensity=img(out["com_intensity"]),
+                            moment_row=img(out["com_moment_row"]), moment_column=img(out["com_moment_column"]))
+
+
+@dataclass
+class FirstLook:
+    """Everything one fused pass yields: virtual images, mean and max pattern, centre of mass."""
+    images: list
+    mean: DetectorImage
+    max: DetectorImage
+    com: Optional[CenterOfMassMaps]
+
+
+def first_look(source: Source, geometries: Sequence[DetectorGeometry] = (), com: bool = True,
+               com_mask: Optional[DetectorGeometry] = None, tiling: ScanTiling = None, progress: Progress = None,
+               is_cancelled: IsCancelled = None) -> FirstLook:
+    """Virtual images of `geometries`, the mean and max pattern and the centre of mass in ONE pass."""
+    shape = _shape_of(source)
+    masks = np.stack([DetectorMask.make(g, shape) for g in geometries]) if geometries else None
+    w = None if com_mask is None else DetectorMask.make(com_mask, shape)
+    shape, out = _reduce(source, masks=masks, want_cols=True, want_com=com, com_weights=w, tiling=tiling,
+                         progress=progress, is_cancelled=is_cancelled)
+    qy, qx = shape.detector_rows, shape.detector_columns
+    images = [ScanImage(shape.scan_rows, shape.scan_columns, v, ComputePath.hip) for v in out.get("images", [])]
+    return FirstLook(images, DetectorImage(qy, qx, out["mean"], ComputePath.hip),
+                     DetectorImage(qy, qx, out["max"], ComputePath.hip), _com_maps(shape, out) if com else None)

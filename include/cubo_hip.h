@@ -1,0 +1,169 @@
+/*
+ * cubo_hip.h — C ABI of the MI355X (gfx950) backend of CuboKit's first-processing hot path.
+ *
+ * This is the boundary a third `ComputePath.hip` binds, beside `.cpu` and `.metal`
+ * (Sources/CuboVirtual/Results.swift:13-19). Plain C: opaque handles, pointers and sizes, int status
+ * codes (0 = success, negative = failure, the shape of the package's existing C targets,
+ * Sources/CStrainLAPACK/LeastSquares.c:4-14), a thread-local message for the last failure, and
+ * caller-owned host output buffers. No callbacks cross the ABI: cancellation and progress are the
+ * caller's loop over cubo_hip_reduce_rows calls, one per scan tile.
+ *
+ * Threading: every cube handle owns one HIP stream; calls on different handles may run from
+ * different threads; calls on one handle must not overlap.
+ */
+#ifndef CUBO_HIP_H
+#define CUBO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The Swift side maps them to CuboVirtualError.noHIPDevice / .hip(String) and
+ * ResidentCubeError.unavailable (Sources/CuboVirtual/Results.swift:65-75,
+ * Sources/CuboVirtual/ResidentCube.swift:14-21). A refusal is never a fallback. */
+enum {
+    CUBO_HIP_OK = 0,
+    CUBO_HIP_ERR_NO_DEVICE = -1,
+    CUBO_HIP_ERR_OUT_OF_MEMORY = -2,
+    CUBO_HIP_ERR_EXCEEDS_CAP = -3,
+    CUBO_HIP_ERR_INVALID_ARGUMENT = -4,
+    CUBO_HIP_ERR_HIP = -5,
+    CUBO_HIP_ERR_UNSUPPORTED = -6,
+    CUBO_HIP_ERR_INCOMPLETE = -7,
+    CUBO_HIP_ERR_EXCEEDS_BUFFER_LIMIT = -8
+};
+
+/* Element types held on the device: the native detector counts (uint16) or Float32, the type every
+ * other element type becomes on read (Sources/CuboCore/MappedCube.swift:56-83). */
+enum { CUBO_HIP_U16 = 1, CUBO_HIP_F32 = 2 };
+
+typedef struct cubo_hip_cube cubo_hip_cube;
+
+/* ---- devices (replaces MetalContext.shared, Sources/CuboVirtual/MetalReducers.swift:21-58) ---- */
+typedef struct {
+    uint64_t hbm_bytes;        /* total device memory */
+    uint64_t free_bytes;       /* free device memory now (hipMemGetInfo) */
+    uint64_t max_alloc_bytes;  /* largest single allocation the library will attempt */
+    int32_t compute_units;
+    int32_t wavefront_size;
+    char name[64];
+    char arch[32];             /* e.g. "gfx950" */
+} cubo_hip_device_info;
+
+int cubo_hip_device_count(int *count);
+int cubo_hip_get_device_info(int device, cubo_hip_device_info *info);
+
+/* Message of the last failure on the calling thread (always NUL-terminated). */
+int cubo_hip_last_error(char *buffer, size_t size);
+
+/* ---- resident cube (replaces ResidentCube, Sources/CuboVirtual/ResidentCube.swift:31-128) ---- */
+
+/* Reasons for cubo_hip_availability, in the reference's order of checks
+ * (ResidentCube.swift:93-103): the memory cap first, then the device, then the largest buffer. */
+enum {
+    CUBO_HIP_AVAILABLE = 0,
+    CUBO_HIP_UNAVAILABLE_NO_DEVICE = 1,
+    CUBO_HIP_UNAVAILABLE_EXCEEDS_MEMORY_CAP = 2,
+    CUBO_HIP_UNAVAILABLE_EXCEEDS_BUFFER_LIMIT = 3
+};
+
+/* Fraction of the device's HBM one resident cube may take (ResidentCube.memoryCapFraction is 1/2 of
+ * host memory on Apple's unified memory; an MI355X cube lives in its own 288 GB of HBM). */
+#define CUBO_HIP_MEMORY_CAP_FRACTION 0.9
+
+/* availability(of:physicalMemory:) (ResidentCube.swift:93-103). hbm_bytes_override = 0 uses the
+ * device's HBM; a non-zero value injects a memory size (the reference's injected physicalMemory,
+ * Tests/CuboVirtualTests/ResidentCubeAvailabilityTests.swift:21-37). Writes the reason, the cube's
+ * bytes and the cap. */
+int cubo_hip_availability(int device, const int64_t shape[4], int dtype, uint64_t hbm_bytes_override,
+                          int *reason, uint64_t *bytes, uint64_t *cap);
+
+/* init(loading:) (ResidentCube.swift:43-58): allocates one device buffer for the cube in its native
+ * dtype. shape = [scanRows, scanColumns, detectorRows, detectorColumns] (CubeShape.swift:11-58). */
+int cubo_hip_cube_create(int device, const int64_t shape[4], int dtype, cubo_hip_cube **out);
+/* Copies scan rows [row0, row0 + nrows) from host memory in the cube's dtype (the tile-by-tile load,
+ * ResidentCube.swift:49-55). */
+int cubo_hip_cube_upload_rows(cubo_hip_cube *cube, int64_t row0, int64_t nrows, const void *host);
+/* readScanTile (ResidentCube.swift:113-117): copies rows out to host memory. */
+int cubo_hip_cube_read_rows(cubo_hip_cube *cube, int64_t row0, int64_t nrows, void *host);
+int cubo_hip_cube_destroy(cubo_hip_cube *cube);
+/* byteCount (ResidentCube.swift:109). */
+int cubo_hip_cube_byte_count(const cubo_hip_cube *cube, uint64_t *bytes);
+
+/* ---- the fused reduction: VirtualImaging.image, VirtualDiffraction.mean/max/meanAndMax and
+ * CenterOfMass.measure in one pass over the cube (VirtualImaging.swift:20-66,
+ * CenterOfMass.swift:27-47). ---- */
+
+typedef struct {
+    const float *masks;        /* n_masks x detectorCount detector masks (DetectorMask.make output), or NULL */
+    int32_t n_masks;
+    int32_t want_mean_max;     /* column statistics: mean and max patterns */
+    int32_t want_com;          /* centre of mass: S, Mr, Mc per position */
+    const float *com_weights;  /* detectorCount CoM weights, or NULL for all ones (CenterOfMass.swift:31-33) */
+} cubo_hip_reduce_request;
+
+typedef struct {
+    double *images;            /* n_masks x scanCount, row-major per mask, or NULL */
+    double *mean;              /* detectorCount, or NULL */
+    double *max;               /* detectorCount, or NULL */
+    double *com_intensity;     /* scanCount: S = sum w P */
+    double *com_moment_row;    /* scanCount: sum (w i) P */
+    double *com_moment_column; /* scanCount: sum (w j) P */
+    double *com_row;           /* scanCount: Mr / S (NaN where S = 0) */
+    double *com_column;        /* scanCount: Mc / S */
+} cubo_hip_reduce_result;
+
+typedef struct cubo_hip_reduction cubo_hip_reduction;
+
+/* A reduction is begun, fed scan-row tiles in scan order (the caller asks isCancelled before each
+ * and reports progress after each, Sources/CuboVirtual/Interruption.swift:15-25), and ended, which
+ * waits for the device and writes the caller's buffers. Every row must be fed exactly once before
+ * end; cancel frees a reduction without results. */
+int cubo_hip_reduce_begin(cubo_hip_cube *cube, const cubo_hip_reduce_request *request, cubo_hip_reduction **out);
+int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows);
+int cubo_hip_reduce_end(cubo_hip_reduction *r, cubo_hip_reduce_result *result);
+int cubo_hip_reduce_cancel(cubo_hip_reduction *r);
+/* Reuse without reallocation, for callers that run the same request on the same cube repeatedly:
+ * results waits and writes the caller's buffers but keeps the reduction; reset starts a new pass
+ * (every row again); release frees it. reduce_end = results + release. */
+int cubo_hip_reduce_results(cubo_hip_reduction *r, cubo_hip_reduce_result *result);
+int cubo_hip_reduce_reset(cubo_hip_reduction *r);
+int cubo_hip_reduce_release(cubo_hip_reduction *r);
+/* One call: begin, all rows as one tile, end. */
+int cubo_hip_reduce(cubo_hip_cube *cube, const cubo_hip_reduce_request *request, cubo_hip_reduce_result *result);
+
+/* Which kernel family a request runs on: 1 = exact integer (uint16 cube, non-negative integral
+ * weights), 2 = FP64 accumulation. Lets tests assert the path they exercise. */
+int cubo_hip_reduce_path(cubo_hip_cube *cube, const cubo_hip_reduce_request *request, int *path);
+
+/* Device time of the fused streaming kernel over the last `cubo_hip_reduce_rows` calls since the
+ * previous query, from HIP events on the cube's stream: total milliseconds and launch count. */
+int cubo_hip_cube_kernel_time(cubo_hip_cube *cube, double *total_ms, int64_t *launches);
+
+/* ---- synthetic cubes for benchmarks and tests (no reference counterpart: the reference's bench
+ * cubes are generated in Python, tools/benchmarks/bench.py:17-27). A disk probe with a sigmoid edge,
+ * Gaussian rings, a per-position deflection and Poisson counts, generated on the device from a
+ * counter-based hash of (seed, element index). ---- */
+typedef struct {
+    uint64_t seed;
+    double center_row, center_column;  /* undeflected probe centre (detector pixels) */
+    double radius;                     /* disk radius */
+    double edge;                       /* sigmoid edge width (0: hard edge) */
+    double counts;                     /* mean counts per pixel inside the disk */
+    double background;                 /* mean counts per pixel everywhere */
+    int32_t n_rings;                   /* up to 4 Gaussian rings */
+    double ring_q[4], ring_amplitude[4], ring_sigma[4];
+    int32_t n_waves;                   /* up to 8 deflection waves: d = a sin(2 pi (fr x + fc y) + phase) */
+    double wave_row_amp[8], wave_col_amp[8], wave_fr[8], wave_fc[8], wave_phase[8];
+    int32_t poisson;                   /* 1: Poisson counts, 0: the expected value */
+} cubo_hip_synth_params;
+
+int cubo_hip_cube_fill_synthetic(cubo_hip_cube *cube, const cubo_hip_synth_params *params);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,267 @@
+"""GPU parity of the fused streaming reduction against the oracle and CuboKit's golden vectors.
+
+Tolerances (stated per assertion):
+- virtual images, mean, CoM sums: relative 1e-10 against the float64 definition with the reference's floor
+  (1e-6 * max|want|, Tests/CuboKitTestSupport/TestSupport.swift:135-151) — stricter than the 1e-5 the
+  reference holds its Metal path to (PinsTests.swift:88-107);
+- max: bit-identical (PinsTests.swift:96);
+- integer cubes (uint16, and uint32 counts read as Float32): images, mean and CoM bit-identical
+  (PinsTests.swift:62-64, CenterOfMassProbeSizeTests.swift:45-50, 165-179);
+- CoM quotient: 1e-9 px where |S| >= 1e-6 sum|wP| (CenterOfMassProbeSizeTests.swift:57-69).
+"""
+import numpy as np
+import pytest
+
+import cubokit_amd as C
+from cubokit_amd import synthetic
+from oracle import oracle as O
+from tests.support import (assert_identical, assert_relative, com_pins, doubles, geometry_kwargs, is_integer_dtype,
+                           lcg_values, load_cube, virtual_pins)
+
+pytestmark = pytest.mark.gpu
+
+TILINGS = [C.ScanTiling.automatic, C.ScanTiling.rows(1), C.ScanTiling.rows(2)]
+
+
+def _geo(g):
+    kind, kw = geometry_kwargs(g)
+    if kind == "point":
+        return C.DetectorGeometry.point(kw["row"], kw["column"])
+    if kind == "circle":
+        return C.DetectorGeometry.circle(kw["centerRow"], kw["centerColumn"], kw["radius"])
+    if kind == "annulus":
+        return C.DetectorGeometry.annulus(kw["centerRow"], kw["centerColumn"], kw["inner"], kw["outer"])
+    if kind == "rectangle":
+        return C.DetectorGeometry.rectangle(kw["rowMin"], kw["rowMax"], kw["columnMin"], kw["columnMax"])
+    return C.DetectorGeometry.mask(kw["weights"])
+
+
+def _resident(arr):
+    return C.HIPResidentCube(arr)
+
+
+def test_device_is_gfx950():
+    info = C.device_info(0)
+    print("device", info.name, info.arch, info.compute_units, info.hbm_bytes)
+    assert info.arch.decode().startswith("gfx950")
+    assert info.wavefront_size == 64
+
+
+def test_pins_virtual_images():
+    for c in virtual_pins()["cubes"]:
+        arr = load_cube(c["file"])
+        cube = _resident(arr)
+        for g in c["geometries"]:
+            geo = _geo(g["geometry"])
+            ref = None
+            for t in TILINGS:
+                img = C.VirtualImaging.image(cube, geo, tiling=t)
+                assert img.path == C.ComputePath.hip
+                if ref is None:
+                    ref = img.values
+                    assert_relative(img.values, doubles(g["image_f64"]), 1e-10, f"{c['name']}/{g['name']}")
+                    if is_integer_dtype(c["dtype"]) and c["exact_in_float32"]:
+                        assert_identical(img.values, doubles(g["image_py4dstem_native"]), f"{c['name']}/{g['name']}")
+                else:
+                    assert_identical(img.values, ref, f"{c['name']}/{g['name']} tiling {t}")
+
+
+def test_pins_mean_and_max():
+    for c in virtual_pins()["cubes"]:
+        cube = _resident(load_cube(c["file"]))
+        for t in TILINGS:
+            mean, mx = C.VirtualDiffraction.mean_and_max(cube, tiling=t)
+            assert_identical(mx.values, doubles(c["max"]), f"{c['name']} max {t}")
+            assert_relative(mean.values, doubles(c["mean_f64"]), 1e-10, f"{c['name']} mean {t}")
+            if is_integer_dtype(c["dtype"]) and c["exact_in_float32"]:
+                assert_identical(mean.values, doubles(c["mean_py4dstem_native"]), f"{c['name']} mean {t}")
+
+
+def test_pins_center_of_mass():
+    for e in com_pins()["com"]:
+        name = e["cube"]
+        arr = load_cube(name)
+        cube = _resident(arr)
+        w = e.get("weights")
+        geo = None if w is None else C.DetectorGeometry.mask(np.array([np.float32(float(s)) for s in w]))
+        s, r, col = doubles(e["intensity"]), doubles(e["row"]), doubles(e["column"])
+        for t in TILINGS[:2]:
+            com = C.CenterOfMass.measure(cube, mask=geo, tiling=t)
+            if name in ("counts16", "counts32"):
+                assert_identical(com.intensity.values, s, f"{name} intensity")
+                assert_identical(com.moment_row.values, doubles(e["moment_row"]), f"{name} moment row")
+                assert_identical(com.row.values, r, f"{name} row")
+                assert_identical(com.column.values, col, f"{name} column")
+            else:
+                assert_relative(com.intensity.values, s, 1e-10, f"{name} intensity")
+                assert_relative(com.moment_row.values, doubles(e["moment_row"]), 1e-10, f"{name} Mr")
+                assert_relative(com.moment_column.values, doubles(e["moment_column"]), 1e-10, f"{name} Mc")
+            x = arr.reshape(-1, arr.shape[2] * arr.shape[3]).astype(np.float64)
+            wd = np.ones(x.shape[1]) if w is None else np.array([float(v) for v in w])
+            cond = np.abs(s) >= 1e-6 * np.abs(x * wd).sum(axis=1)
+            for k in np.nonzero(cond)[0]:
+                if np.isnan(r[k]):
+                    assert np.isnan(com.row.values[k])
+                else:
+                    assert abs(com.row.values[k] - r[k]) <= 1e-9 and abs(com.column.values[k] - col[k]) <= 1e-9
+
+
+def test_planted_shift_and_zero_pattern():
+    t = com_pins()["shift_truth"]
+    com = C.CenterOfMass.measure(_resident(load_cube("shift")))
+    assert_relative(com.row.values, doubles(t["row"]), 1e-12, "shift row")
+    assert_relative(com.column.values, doubles(t["column"]), 1e-12, "shift column")
+    z = np.array([0] * 9 + [0, 0, 0, 0, 5, 0, 0, 0, 0], np.float32).reshape(1, 2, 3, 3)
+    com = C.CenterOfMass.measure(_resident(z))
+    assert np.isnan(com.row.values[0]) and np.isnan(com.column.values[0])
+    assert com.row.values[1] == 1 and com.column.values[1] == 1
+
+
+def test_non_finite_and_signed_zero():
+    """RefuterRegressionTests.swift:101-112."""
+    cube = np.array([np.inf, -0.0, np.nan, 1, 2, -0.0, 3, -np.inf], np.float32).reshape(2, 1, 1, 4)
+    mean, mx = C.VirtualDiffraction.mean_and_max(_resident(cube))
+    assert mean.values[0] == np.inf
+    assert mean.values[1] == 0 and np.signbit(mean.values[1])
+    assert np.isnan(mean.values[2]) and np.isnan(mx.values[2])
+    assert mean.values[3] == -np.inf
+    assert mx.values[0] == np.inf and mx.values[3] == 1
+
+
+def _hot_cube():
+    """MetalNonFiniteTests.swift:11-27: infinities in selected patterns."""
+    rows, cols, qr, qc = 6, 5, 8, 10
+    n = qr * qc
+    p = np.arange(rows * cols)[:, None]
+    i = np.arange(n)[None, :]
+    v = (np.sin((p * 31 + i * 7).astype(np.float64)) * 50 + (i % 9) - 3).astype(np.float32).reshape(-1)
+    plus, minus, both = 3 * qc + 4, 5 * qc + 1, 6 * qc + 7
+    v[7 * n + plus] = np.inf
+    v[22 * n + minus] = -np.inf
+    v[3 * n + both] = np.inf
+    v[18 * n + both] = -np.inf
+    return v.reshape(rows, cols, qr, qc)
+
+
+def _same_special(got, want, tol, msg):
+    got, want = np.asarray(got), np.asarray(want)
+    fin = np.isfinite(want)
+    assert np.all(np.isnan(got[np.isnan(want)])), msg
+    inf = np.isinf(want)
+    assert np.array_equal(got[inf], want[inf]), msg
+    assert_relative(got[fin], want[fin], tol, msg)
+
+
+def test_infinities_match_the_oracle():
+    cube = _hot_cube()
+    res = _resident(cube)
+    mean, mx = C.VirtualDiffraction.mean_and_max(res)
+    om, ox = O.mean_max(cube)
+    _same_special(mean.values, om, 1e-10, "mean")
+    assert_identical(mx.values, ox, "max")
+    disk = C.DetectorGeometry.circle(4, 4, 3)
+    img = C.VirtualImaging.image(res, disk)
+    _same_special(img.values, O.virtual_image(cube, C.DetectorMask.make(disk, res.shape)), 1e-10, "image")
+    for mask in (None, disk):
+        com = C.CenterOfMass.measure(res, mask=mask)
+        w = None if mask is None else C.DetectorMask.make(mask, res.shape)
+        ref = O.com(cube, w)
+        for k in ("intensity", "moment_row", "moment_column", "row", "column"):
+            _same_special(getattr(com, k).values, ref[k], 1e-10, f"com {k}")
+
+
+def test_large_integer_cube_center_of_mass_is_exact():
+    """CenterOfMassProbeSizeTests.swift:165-179 on the exact integer kernel and on the FP64 kernel."""
+    vals = lcg_values(7, 4 * 1024 * 1024, 4001).reshape(2, 2, 1024, 1024)
+    ref = O.com(vals)
+    for arr in (vals.astype(np.uint16), vals):
+        res = _resident(arr)
+        com = C.CenterOfMass.measure(res)
+        assert_identical(com.intensity.values, ref["intensity"], "1024^2 intensity")
+        assert_identical(com.moment_row.values, ref["moment_row"], "1024^2 row moment")
+        assert_identical(com.moment_column.values, ref["moment_column"], "1024^2 column moment")
+        assert com.moment_row.values.max() > 3e10
+
+
+def test_kernel_families_are_the_intended_ones():
+    res = _resident(np.zeros((2, 2, 16, 16), np.uint16))
+    bf = C.DetectorMask.make(C.DetectorGeometry.circle(8, 8, 4), res.shape)
+    assert C.reduce_path(res, bf[None], want_com=True) == 1
+    assert C.reduce_path(res, (bf * 0.5)[None]) == 2
+    res32 = _resident(np.zeros((2, 2, 16, 16), np.float32))
+    assert C.reduce_path(res32, bf[None]) == 2
+
+
+def test_progress_and_cancellation():
+    res = _resident(np.ones((5, 3, 8, 8), np.uint16))
+    seen = []
+    C.VirtualDiffraction.mean_and_max(res, tiling=C.ScanTiling.rows(2), progress=seen.append)
+    assert seen == sorted(seen) and seen[-1] == 1.0 and len(seen) == 3
+    calls = []
+
+    def cancel_at_second():
+        calls.append(1)
+        return len(calls) >= 3          # the up-front check, tile 1, then cancel before tile 2
+
+    with pytest.raises(C.CancellationError):
+        C.VirtualDiffraction.mean_and_max(res, tiling=C.ScanTiling.rows(2), is_cancelled=cancel_at_second)
+    with pytest.raises(C.CancellationError):
+        C.VirtualImaging.image(res, C.DetectorGeometry.point(1, 1), is_cancelled=lambda: True)
+
+
+def test_small_float32_workload_matches_the_oracle():
+    """20x20x512x512 Float32 (disk r=30, 2 px sigmoid edge, Poisson): images, mean, max and CoM."""
+    shape, dtype, params = synthetic.small_f32()
+    res = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
+    res.fill_synthetic(params)
+    host = res.read_scan_tile(range(0, shape[0]))
+    bf = C.DetectorGeometry.circle(256, 256, 32)
+    adf = C.DetectorGeometry.annulus(256, 256, 40, 200)
+    out = C.first_look(res, [bf, adf])
+    for geo, img in zip((bf, adf), out.images):
+        assert_relative(img.values, O.virtual_image(host, C.DetectorMask.make(geo, res.shape)), 1e-10, "image")
+    om, ox = O.mean_max(host)
+    assert_relative(out.mean.values, om, 1e-10, "mean")
+    assert_identical(out.max.values, ox, "max")
+    ref = O.com(host)
+    for k in ("intensity", "moment_row", "moment_column"):
+        assert_relative(getattr(out.com, k).values, ref[k], 1e-10, k)
+    assert np.max(np.abs(out.com.row.values - ref["row"])) <= 1e-9
+
+
+def test_bandwidth_workload_is_bit_exact():
+    """256x256x128x128 uint16 (2 GiB): one fused pass gives BF r=14, ADF 20-60, mean, max and CoM; every
+    output is an exact integer result, so it must equal the oracle's bit for bit."""
+    shape, dtype, params = synthetic.bandwidth_u16()
+    res = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
+    res.fill_synthetic(params)
+    bf = C.DetectorGeometry.circle(64, 64, 14)
+    adf = C.DetectorGeometry.annulus(64, 64, 20, 60)
+    assert C.reduce_path(res, np.stack([C.DetectorMask.make(g, res.shape) for g in (bf, adf)]), want_com=True) == 1
+    out = C.first_look(res, [bf, adf])
+    host = res.read_scan_tile(range(0, shape[0]))
+    ref = O.cpu_first_processing_u16(host, 128, 128, shape[1], C.DetectorMask.make(bf, res.shape),
+                                     C.DetectorMask.make(adf, res.shape), threads=8)
+    assert_identical(out.images[0].values, ref["vi_bf"], "BF")
+    assert_identical(out.images[1].values, ref["vi_adf"], "ADF")
+    assert_identical(out.mean.values, ref["mean"], "mean")
+    assert_identical(out.max.values, ref["max"], "max")
+    for k in ("intensity", "moment_row", "moment_column"):
+        assert_identical(getattr(out.com, k).values, ref[k], k)
+    # the synthetic data are what the workload describes: a bright disk and real counts
+    assert out.max.values.max() > 400 and out.mean.values[64 * 128 + 64] > 400
+
+
+def test_tilings_give_the_same_bits_on_the_integer_kernel():
+    shape, dtype, params = synthetic.bandwidth_u16(scan=(24, 20))
+    res = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
+    res.fill_synthetic(params)
+    geos = [C.DetectorGeometry.circle(64, 64, 14), C.DetectorGeometry.annulus(64, 64, 20, 60)]
+    a = C.first_look(res, geos)
+    for t in (C.ScanTiling.rows(1), C.ScanTiling.rows(5)):
+        b = C.first_look(res, geos, tiling=t)
+        for x, y in zip(a.images, b.images):
+            assert_identical(y.values, x.values, f"image {t}")
+        assert_identical(b.mean.values, a.mean.values, f"mean {t}")
+        assert_identical(b.max.values, a.max.values, f"max {t}")
+        assert_identical(b.com.moment_column.values, a.com.moment_column.values, f"com {t}")
