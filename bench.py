@@ -119,6 +119,28 @@ def main() -> int:
     log(f"step {ms_per_step:.3f} ms ({value:.0f} GB/s); fused kernel {kernel_ms:.3f} ms ({achieved:.0f} GB/s)")
     assert out["max"].max() > 400 and np.all(np.isfinite(out["images"]))
 
+    # The K = 64 mask stack on the same cube: split-FP16 matrix cores against the plain VALU kernel.
+    stack_masks = synthetic.stack_masks(128, 128)
+    stack = {}
+    ys = {}
+    for name, method in (("plain", C.MaskStack.PLAIN), ("mfma_split_fp16", C.MaskStack.MFMA_SPLIT)):
+        st = C.MaskStack(cube, stack_masks, method)
+        y = st.run()                                    # warm-up
+        cube.kernel_time()
+        reps = 3 if name == "plain" else 10
+        for _ in range(reps):
+            st.run(y)
+        tms, nl = cube.kernel_time()
+        st.release()
+        ys[name] = y
+        stack[name + "_ms"] = round(tms / nl, 4)
+        log(f"mask stack {name}: {tms / nl:.3f} ms per pass")
+    stack["speedup"] = round(stack["plain_ms"] / stack["mfma_split_fp16_ms"], 2)
+    den = np.maximum(np.abs(ys["plain"]), 1e-6 * np.abs(ys["plain"]).max(axis=0))
+    stack["max_rel_diff_vs_plain"] = float(np.max(np.abs(ys["mfma_split_fp16"] - ys["plain"]) / den))
+    stack["k"] = int(stack_masks.shape[0])
+    stack["mfma_gbs"] = round(cube_bytes / (stack["mfma_split_fp16_ms"] / 1e3) / 1e9, 1)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -136,6 +158,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "mask_stack_k64": stack,
     }
     print(json.dumps(line), flush=True)
     return 0

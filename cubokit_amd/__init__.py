@@ -4,11 +4,13 @@ The product is the C-ABI library libcubo_hip.so (include/cubo_hip.h); this packa
 CuboKit's CuboVirtual API over it.
 """
 from .virtual import (CancellationError, CenterOfMass, CenterOfMassMaps, ComputePath, CubeShape, CuboError,
-                      CuboVirtualError, DetectorGeometry, DetectorImage, DetectorMask, FirstLook, HIPResidentCube,
+                      CuboVirtualError, DetectorGeometry, DetectorImage, DetectorMask, FirstLook, HIPResidentCube, MaskStack,
+                      center_of_mass_from_stack,
                       ResidentCubeError, ScanImage, ScanTiling, UnavailableReason, VirtualDiffraction, VirtualImaging,
                       device_count, device_info, first_look, hip_is_available, reduce_path)
 
 __all__ = ["CancellationError", "CenterOfMass", "CenterOfMassMaps", "ComputePath", "CubeShape", "CuboError",
            "CuboVirtualError", "DetectorGeometry", "DetectorImage", "DetectorMask", "FirstLook", "HIPResidentCube",
+           "MaskStack", "center_of_mass_from_stack",
            "ResidentCubeError", "ScanImage", "ScanTiling", "UnavailableReason", "VirtualDiffraction",
            "VirtualImaging", "device_count", "device_info", "first_look", "hip_is_available", "reduce_path"]

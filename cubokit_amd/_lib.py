@@ -24,6 +24,10 @@ ERR_EXCEEDS_BUFFER_LIMIT = -8
 U16 = 1
 F32 = 2
 
+STACK_AUTO = 0
+STACK_PLAIN = 1
+STACK_MFMA_SPLIT = 2
+
 AVAILABLE = 0
 UNAVAILABLE_NO_DEVICE = 1
 UNAVAILABLE_EXCEEDS_MEMORY_CAP = 2
@@ -77,6 +81,12 @@ SIGNATURES = {
     "cubo_hip_reduce": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ReduceResult)]),
     "cubo_hip_reduce_path": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ct.c_int)]),
     "cubo_hip_cube_kernel_time": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),
+    "cubo_hip_stack_plan_create": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32,
+                                              ct.POINTER(ct.c_void_p)]),
+    "cubo_hip_stack_plan_method": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32)]),
+    "cubo_hip_stack_plan_run": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "cubo_hip_stack_plan_destroy": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_mask_stack": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_void_p]),
     "cubo_hip_cube_fill_synthetic": (ct.c_int, [ct.c_void_p, ct.POINTER(SynthParams)]),
 }
 

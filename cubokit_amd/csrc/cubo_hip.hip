@@ -17,45 +17,16 @@
 
 #include "../../include/cubo_hip.h"
 #include "cube.hpp"
+#include "errors.hpp"
 #include "reduce_kernels.hpp"
 
+using cubo::DeviceGuard;
+using cubo::fail;
+using cubo::g_last_error;
+using cubo::hip_fail;
+#define HIP_TRY CUBO_HIP_TRY
+
 namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int status, const char *fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_last_error = buf;
-    return status;
-}
-
-int hip_fail(hipError_t e, const char *what) {
-    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
-        return fail(CUBO_HIP_ERR_OUT_OF_MEMORY, "%s: %s", what, hipGetErrorString(e));
-    return fail(CUBO_HIP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
-}
-
-#define HIP_TRY(expr, what)                          \
-    do {                                             \
-        hipError_t _e = (expr);                      \
-        if (_e != hipSuccess) return hip_fail(_e, what); \
-    } while (0)
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        if (prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-    }
-};
 
 int device_count() {
     int n = 0;

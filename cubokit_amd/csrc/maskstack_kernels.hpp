@@ -1,0 +1,33 @@
+// maskstack_kernels.hpp — launch interface of the mask-stack contraction (maskstack_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cubo {
+
+struct StackLaunch {
+    const uint16_t *cube;     // first position
+    int64_t npos;
+    int64_t ndet;             // multiple of 8
+    int32_t nchunks;          // ceil(ndet / 64)
+    int32_t k;                // masks
+    const void *wb;           // FP16 hi/lo weights in MFMA B-operand order [chunk][ntile][kstep][hi,lo][lane][8]
+    const double *inv_scale;  // [k] 1 / (power-of-two column scale)
+    double *y;                // [npos][k]
+};
+
+struct PlainStackLaunch {
+    const void *cube;         // uint16_t or float, first position
+    int64_t npos;
+    int64_t ndet;
+    int32_t k;                // masks in the whole stack (the row stride of y)
+    int32_t k0, kn;           // this pass: masks [k0, k0 + kn)
+    const float *w;           // [ndet][kpad] FP32 weights of this pass, pixel-major
+    double *y;                // [npos][k]
+};
+
+hipError_t launch_mask_stack_mfma(const StackLaunch &a, int ntiles, hipStream_t s);
+hipError_t launch_mask_stack_plain(const PlainStackLaunch &a, bool is_u16, int kpad, hipStream_t s);
+
+}  // namespace cubo

@@ -55,3 +55,23 @@ def random_waves(rng: np.random.Generator, n: int, amplitude: float, scan):
         fr, fc = rng.integers(1, 4, 2) / np.array(scan, float)
         out.append((float(ar), float(ac), float(fr), float(fc), float(rng.uniform(0, 2 * math.pi))))
     return out
+
+
+def stack_masks(qy: int, qx: int, seed: int = 20261103, n_sectors: int = 29, n_dense: int = 32) -> np.ndarray:
+    """The K = 3 + n_sectors + n_dense mask stack: {1, q_row, q_col}, annular sectors with seeded radii in
+    [0, 64) and angles, and dense uniform [0, 1) masks. Float32, K x (qy * qx)."""
+    rng = np.random.default_rng(seed)
+    ii, jj = np.indices((qy, qx), dtype=np.float64)
+    cy, cx = qy / 2.0, qx / 2.0
+    d = np.hypot(ii - cy, jj - cx)
+    ang = np.arctan2(ii - cy, jj - cx)
+    rows = [np.ones(qy * qx), ii.reshape(-1), jj.reshape(-1)]
+    for _ in range(n_sectors):
+        r0, r1 = np.sort(rng.uniform(0, 64, 2))
+        a0 = rng.uniform(-math.pi, math.pi)
+        width = rng.uniform(0.3, 2 * math.pi)
+        da = np.mod(ang - a0, 2 * math.pi)
+        rows.append(((d >= r0) & (d < r1) & (da < width)).astype(np.float64).reshape(-1))
+    for _ in range(n_dense):
+        rows.append(rng.uniform(0, 1, qy * qx))
+    return np.ascontiguousarray(np.stack(rows).astype(np.float32))

@@ -598,3 +598,47 @@ def first_look(source: Source, geometries: Sequence[DetectorGeometry] = (), com:
     images = [ScanImage(shape.scan_rows, shape.scan_columns, v, ComputePath.hip) for v in out.get("images", [])]
     return FirstLook(images, DetectorImage(qy, qx, out["mean"], ComputePath.hip),
                      DetectorImage(qy, qx, out["max"], ComputePath.hip), _com_maps(shape, out) if com else None)
+
+
+class MaskStack:
+    """Y = X W for a stack of weight rows, in one pass (include/cubo_hip.h, "mask stack")."""
+
+    AUTO, PLAIN, MFMA_SPLIT = L.STACK_AUTO, L.STACK_PLAIN, L.STACK_MFMA_SPLIT
+
+    def __init__(self, cube: HIPResidentCube, masks: np.ndarray, method: int = L.STACK_AUTO):
+        self.cube = cube
+        self.masks = np.ascontiguousarray(masks, np.float32).reshape(-1, cube.shape.detector_count)
+        self.k = self.masks.shape[0]
+        h = ct.c_void_p()
+        _check(L.lib().cubo_hip_stack_plan_create(cube.handle, self.masks.ctypes.data, self.k, method, ct.byref(h)),
+               "preparing the mask stack")
+        self._h = h
+        m = ct.c_int32()
+        _check(L.lib().cubo_hip_stack_plan_method(h, ct.byref(m)), "stack method")
+        self.method = m.value
+
+    def run(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """(scanCount, K) Double."""
+        y = out if out is not None else np.empty((self.cube.shape.scan_count, self.k))
+        _check(L.lib().cubo_hip_stack_plan_run(self._h, y.ctypes.data), "running the mask stack")
+        return y
+
+    def release(self):
+        if getattr(self, "_h", None) is not None:
+            L.lib().cubo_hip_stack_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def center_of_mass_from_stack(cube_shape: CubeShape, y: np.ndarray, columns=(0, 1, 2)) -> CenterOfMassMaps:
+    """The centre of mass from the stack's {w, w*q_row, w*q_col} columns; the division in FP64."""
+    s, mr, mc = (np.ascontiguousarray(y[:, c], np.float64) for c in columns)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        out = {"com_intensity": s, "com_moment_row": mr, "com_moment_column": mc, "com_row": mr / s,
+               "com_column": mc / s}
+    return _com_maps(cube_shape, out)

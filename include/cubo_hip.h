@@ -143,6 +143,25 @@ int cubo_hip_reduce_path(cubo_hip_cube *cube, const cubo_hip_reduce_request *req
  * previous query, from HIP events on the cube's stream: total milliseconds and launch count. */
 int cubo_hip_cube_kernel_time(cubo_hip_cube *cube, double *total_ms, int64_t *launches);
 
+/* ---- the mask stack: Y = X W for K weight rows in one pass, X the (scanCount x detectorCount) cube.
+ * {1, q_row, q_col, masks...} gives K virtual images and the centre-of-mass sums together. No reference
+ * counterpart (the reference runs VirtualImaging.image once per mask, CPUReducers.swift:13-39). ---- */
+enum {
+    CUBO_HIP_STACK_AUTO = 0,        /* the matrix-core method where it applies, else plain */
+    CUBO_HIP_STACK_PLAIN = 1,       /* VALU: FP32 FMAs, FP64 across 32-pixel chunks; any cube */
+    CUBO_HIP_STACK_MFMA_SPLIT = 2   /* uint16 cube, split-FP16 operands on MFMA, FP32 accumulators flushed to
+                                       FP64 every 256 pixels; at most 128 masks */
+};
+typedef struct cubo_hip_stack_plan cubo_hip_stack_plan;
+/* masks: k x detectorCount FP32 weights, row-major. */
+int cubo_hip_stack_plan_create(cubo_hip_cube *cube, const float *masks, int32_t k, int32_t method,
+                               cubo_hip_stack_plan **out);
+int cubo_hip_stack_plan_method(const cubo_hip_stack_plan *plan, int32_t *method);
+/* y: scanCount x k, row-major, caller-owned host memory. */
+int cubo_hip_stack_plan_run(cubo_hip_stack_plan *plan, double *y);
+int cubo_hip_stack_plan_destroy(cubo_hip_stack_plan *plan);
+int cubo_hip_mask_stack(cubo_hip_cube *cube, const float *masks, int32_t k, int32_t method, double *y);
+
 /* ---- synthetic cubes for benchmarks and tests (no reference counterpart: the reference's bench
  * cubes are generated in Python, tools/benchmarks/bench.py:17-27). A disk probe with a sigmoid edge,
  * Gaussian rings, a per-position deflection and Poisson counts, generated on the device from a
