@@ -63,6 +63,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="also run one device copy of 2 GiB (read 2 GiB, write 2 GiB) for counter calibration")
+    ap.add_argument("--no-stack", action="store_true", help="skip the K = 64 mask-stack comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload runs on one MI355X "
@@ -97,9 +100,12 @@ def main() -> int:
     if lib.cubo_hip_reduce_begin(cube.handle, ct.byref(req), ct.byref(h)) != 0:
         raise RuntimeError(L.last_error())
 
+    view = L.ReduceView()
+
     def step():
+        # One pass: reset, every row as one tile, and the results in (pinned) host memory.
         if lib.cubo_hip_reduce_reset(h) or lib.cubo_hip_reduce_rows(h, 0, s.scan_rows) or \
-                lib.cubo_hip_reduce_results(h, ct.byref(res)):
+                lib.cubo_hip_reduce_view(h, ct.byref(view)):
             raise RuntimeError(L.last_error())
 
     for _ in range(args.warmup):
@@ -111,6 +117,8 @@ def main() -> int:
         step()                                          # results() waits for the device: each step is complete
     el = time.perf_counter() - t0
     kms, launches = cube.kernel_time()
+    if lib.cubo_hip_reduce_results(h, ct.byref(res)):
+        raise RuntimeError(L.last_error())
     lib.cubo_hip_reduce_release(h)
     ms_per_step = el / args.steps * 1e3
     value = cube_bytes / (el / args.steps) / 1e9
@@ -119,11 +127,17 @@ def main() -> int:
     log(f"step {ms_per_step:.3f} ms ({value:.0f} GB/s); fused kernel {kernel_ms:.3f} ms ({achieved:.0f} GB/s)")
     assert out["max"].max() > 400 and np.all(np.isfinite(out["images"]))
 
+    if args.calibration_copy:
+        twin = cube.clone()                              # one copy kernel: 2 GiB read, 2 GiB written
+        twin.release()
+        log("calibration copy of 2 GiB done")
+
     # The K = 64 mask stack on the same cube: split-FP16 matrix cores against the plain VALU kernel.
     stack_masks = synthetic.stack_masks(128, 128)
     stack = {}
     ys = {}
-    for name, method in (("plain", C.MaskStack.PLAIN), ("mfma_split_fp16", C.MaskStack.MFMA_SPLIT)):
+    for name, method in (() if args.no_stack else
+                         (("plain", C.MaskStack.PLAIN), ("mfma_split_fp16", C.MaskStack.MFMA_SPLIT))):
         st = C.MaskStack(cube, stack_masks, method)
         y = st.run()                                    # warm-up
         cube.kernel_time()
@@ -135,11 +149,12 @@ def main() -> int:
         ys[name] = y
         stack[name + "_ms"] = round(tms / nl, 4)
         log(f"mask stack {name}: {tms / nl:.3f} ms per pass")
-    stack["speedup"] = round(stack["plain_ms"] / stack["mfma_split_fp16_ms"], 2)
-    den = np.maximum(np.abs(ys["plain"]), 1e-6 * np.abs(ys["plain"]).max(axis=0))
-    stack["max_rel_diff_vs_plain"] = float(np.max(np.abs(ys["mfma_split_fp16"] - ys["plain"]) / den))
-    stack["k"] = int(stack_masks.shape[0])
-    stack["mfma_gbs"] = round(cube_bytes / (stack["mfma_split_fp16_ms"] / 1e3) / 1e9, 1)
+    if ys:
+        stack["speedup"] = round(stack["plain_ms"] / stack["mfma_split_fp16_ms"], 2)
+        den = np.maximum(np.abs(ys["plain"]), 1e-6 * np.abs(ys["plain"]).max(axis=0))
+        stack["max_rel_diff_vs_plain"] = float(np.max(np.abs(ys["mfma_split_fp16"] - ys["plain"]) / den))
+        stack["k"] = int(stack_masks.shape[0])
+        stack["mfma_gbs"] = round(cube_bytes / (stack["mfma_split_fp16_ms"] / 1e3) / 1e9, 1)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -50,6 +50,11 @@ class ReduceResult(ct.Structure):
                                            "com_moment_column", "com_row", "com_column")]
 
 
+class ReduceView(ct.Structure):
+    _fields_ = [(n, ct.POINTER(ct.c_double)) for n in ("images", "mean", "max", "com_intensity", "com_moment_row",
+                                                       "com_moment_column", "com_row", "com_column")]
+
+
 class SynthParams(ct.Structure):
     _fields_ = [("seed", ct.c_uint64), ("center_row", ct.c_double), ("center_column", ct.c_double),
                 ("radius", ct.c_double), ("edge", ct.c_double), ("counts", ct.c_double),
@@ -70,6 +75,7 @@ SIGNATURES = {
     "cubo_hip_cube_upload_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p]),
     "cubo_hip_cube_read_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p]),
     "cubo_hip_cube_destroy": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_cube_clone": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_void_p)]),
     "cubo_hip_cube_byte_count": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_uint64)]),
     "cubo_hip_reduce_begin": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ct.c_void_p)]),
     "cubo_hip_reduce_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64]),
@@ -77,6 +83,7 @@ SIGNATURES = {
     "cubo_hip_reduce_cancel": (ct.c_int, [ct.c_void_p]),
     "cubo_hip_reduce_results": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceResult)]),
     "cubo_hip_reduce_reset": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_reduce_view": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceView)]),
     "cubo_hip_reduce_release": (ct.c_int, [ct.c_void_p]),
     "cubo_hip_reduce": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ReduceResult)]),
     "cubo_hip_reduce_path": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ct.c_int)]),

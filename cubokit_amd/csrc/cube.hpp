@@ -17,3 +17,24 @@ struct cubo_hip_cube {
     hipStream_t stream;     // every launch on this cube goes to this stream
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;   // fused-kernel launches not yet queried
 };
+
+#include "errors.hpp"
+
+namespace cubo {
+
+inline int element_size(int dtype) { return dtype == CUBO_HIP_U16 ? 2 : dtype == CUBO_HIP_F32 ? 4 : 0; }
+
+inline int rows_range(const cubo_hip_cube *c, int64_t row0, int64_t nrows) {
+    if (row0 < 0 || nrows < 0 || row0 + nrows > c->shape[0])
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "rows [%lld, %lld) outside the scan's %lld rows", (long long)row0,
+                    (long long)(row0 + nrows), (long long)c->shape[0]);
+    return CUBO_HIP_OK;
+}
+
+inline int compute_units(int dev) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    return cus;
+}
+
+}  // namespace cubo

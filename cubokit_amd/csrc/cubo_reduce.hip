@@ -1,0 +1,451 @@
+// cubo_reduce.hip — host side of the fused reduction (include/cubo_hip.h, "the fused reduction").
+//
+// Chooses the kernel family, prepares the weight rows, plans position groups from the kernel's occupancy,
+// feeds scan-row tiles in order (Sources/CuboVirtual/MetalReducers.swift:439-490: one pass per tile, the
+// caller checks cancellation between tiles) and returns the Double results. All results of a pass live in
+// one device buffer that comes back in one copy into pinned host memory; cubo_hip_reduce_view exposes it
+// without a further copy, cubo_hip_reduce_results copies it into the caller's buffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/cubo_hip.h"
+#include "cube.hpp"
+#include "errors.hpp"
+#include "reduce_kernels.hpp"
+
+using cubo::DeviceGuard;
+using cubo::fail;
+using cubo::hip_fail;
+
+struct cubo_hip_reduction {
+    cubo_hip_cube *cube;
+    int path;               // 1 exact integer, 2 FP64
+    int n_masks;
+    bool want_cols, want_com;
+    bool com_ones;          // integer path: arithmetic CoM rows
+    int nw;                 // weight rows in the kernel (padded)
+    int nv;                 // row outputs per position
+    int chunks;             // integer kernel: 8-pixel chunks per lane
+    int slice_px, nslices;
+    int blocks_per_cu;
+    int64_t rows_done, next_row;
+    bool viewed;
+    // device
+    void *weights;          // packed uint16 pairs (integer) or Double rows (FP64)
+    void *rowpart;          // [nslices][npos][nv]
+    void *colsum_part;      // [groups][ndet] uint32 (integer) or double (FP64)
+    void *colmax_part;      // [groups][ndet] uint32 (integer) or float (FP64)
+    void *col_sum;          // uint64 or double [ndet]
+    void *col_max;          // uint32 or float [ndet]
+    double *res_dev;        // [nv][npos] rows, then mean [ndet], max [ndet]
+    int max_groups;
+    // pinned host
+    double *res_host;       // mirror of res_dev, then com row [npos], com column [npos]
+    size_t res_count;       // doubles in res_dev
+};
+
+namespace {
+
+int64_t result_doubles(const cubo_hip_reduction *r) {
+    return (int64_t)std::max(r->nv, 1) * r->cube->npos + 2 * r->cube->ndet;
+}
+
+void free_reduction(cubo_hip_reduction *r) {
+    if (!r) return;
+    DeviceGuard g(r->cube->device);
+    void *bufs[] = {r->weights, r->rowpart, r->colsum_part, r->colmax_part, r->col_sum, r->col_max, r->res_dev};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (r->res_host) (void)hipHostFree(r->res_host);
+    delete r;
+}
+
+// Integral, non-negative and at most `limit`.
+bool integral_in(const float *w, int64_t n, double limit, double *maxw) {
+    double m = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        const double v = w[k];
+        if (!(v >= 0) || v > limit || v != std::floor(v)) return false;
+        m = std::max(m, v);
+    }
+    *maxw = m;
+    return true;
+}
+
+int padded_int_nw(int nw) {
+    for (int c : {0, 1, 2, 4, 8, 16})
+        if (nw <= c) return c;
+    return -1;
+}
+
+int padded_f64_nw(int nw) {
+    for (int c : {1, 2, 3, 4, 5, 6, 8, 11, 16})
+        if (nw <= c) return c;
+    return -1;
+}
+
+// Position groups of one launch: one wave of blocks that fills every compute unit at the kernel's
+// occupancy, at most 65536 positions per group (the integer column partials are uint32 before the
+// 64-bit add: 65535 * 65536 < 2^32).
+void plan_groups(int64_t npos, int nslices, int cus, int blocks_per_cu, int *ngroups, int *ppg, int *ppw) {
+    const int64_t target_blocks = (int64_t)cus * std::max(1, blocks_per_cu);
+    int64_t g = std::max<int64_t>(1, (target_blocks + nslices - 1) / nslices);
+    g = std::min<int64_t>(g, std::max<int64_t>(1, npos / cubo::kWavesPerBlock));
+    int64_t per = (npos + g - 1) / g;
+    per = std::min<int64_t>(per, 65536);
+    per = ((per + cubo::kWavesPerBlock - 1) / cubo::kWavesPerBlock) * cubo::kWavesPerBlock;
+    *ppg = (int)per;
+    *ngroups = (int)((npos + per - 1) / per);
+    *ppw = (int)(per / cubo::kWavesPerBlock);
+}
+
+int choose_path(const cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_hip_reduction *r) {
+    const int qy = (int)c->shape[2], qx = (int)c->shape[3];
+    const int64_t ndet = c->ndet;
+    r->n_masks = q->n_masks;
+    r->want_cols = q->want_mean_max != 0;
+    r->want_com = q->want_com != 0;
+    r->path = 2;
+    if (c->dtype != CUBO_HIP_U16 || ndet % 8 != 0) return CUBO_HIP_OK;
+    double maxw = 0, m = 0;
+    for (int k = 0; k < q->n_masks; ++k) {
+        if (!integral_in(q->masks + (size_t)k * ndet, ndet, 65535, &m)) return CUBO_HIP_OK;
+        maxw = std::max(maxw, m);
+    }
+    bool com_ones = false;
+    int nw = q->n_masks;
+    double comw = 0;
+    if (r->want_com) {
+        if (!q->com_weights && qx % 8 == 0 && qx <= 4096 && qy <= 4096) {
+            com_ones = true;
+            comw = (double)std::max(qy - 1, qx - 1);
+        } else {
+            double wm = 1;
+            if (q->com_weights && !integral_in(q->com_weights, ndet, 65535, &wm)) return CUBO_HIP_OK;
+            if (wm * std::max(qy - 1, qx - 1) > 65535) return CUBO_HIP_OK;
+            comw = wm * std::max(qy - 1, qx - 1);
+            nw += 3;
+        }
+    }
+    const int pnw = padded_int_nw(nw);
+    if (pnw < 0) return CUBO_HIP_OK;
+    // Largest chunk count whose per-lane uint32 partials cannot overflow: 8*chunks * 65535 * weight.
+    const double wbound = std::max({maxw, comw, 1.0});
+    for (int ch : {4, 2, 1}) {
+        if ((double)ch * 8.0 * 65535.0 * wbound >= 4294967296.0) continue;
+        if (!cubo::supported_int(pnw, ch, com_ones)) continue;
+        r->path = 1;
+        r->com_ones = com_ones;
+        r->nw = pnw;
+        r->nv = pnw + (com_ones ? 3 : 0);
+        r->chunks = ch;
+        r->slice_px = cubo::int_slice_pixels(ch);
+        return CUBO_HIP_OK;
+    }
+    return CUBO_HIP_OK;
+}
+
+hipError_t init_cols(cubo_hip_reduction *r) {
+    cubo_hip_cube *c = r->cube;
+    return r->path == 1 ? cubo::launch_fill_cols_init((uint64_t *)r->col_sum, (uint32_t *)r->col_max, nullptr, nullptr,
+                                                      (int)c->ndet, c->stream)
+                        : cubo::launch_fill_cols_init(nullptr, nullptr, (double *)r->col_sum, (float *)r->col_max,
+                                                      (int)c->ndet, c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cubo_hip_reduce_path(cubo_hip_cube *c, const cubo_hip_reduce_request *q, int *path) {
+    if (!c || !q || !path) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    cubo_hip_reduction tmp{};
+    tmp.cube = c;
+    if (int st = choose_path(c, q, &tmp)) return st;
+    *path = tmp.path;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_reduce_begin(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_hip_reduction **out) {
+    if (!c || !q || !out) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (q->n_masks < 0 || (q->n_masks > 0 && !q->masks))
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "masks: %d rows but no data", q->n_masks);
+    if (q->n_masks == 0 && !q->want_mean_max && !q->want_com)
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "nothing requested");
+    const int64_t ndet = c->ndet;
+    for (int64_t k = 0; k < (int64_t)q->n_masks * ndet; ++k)
+        if (!std::isfinite(q->masks[k])) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "a mask needs finite weights");
+    if (q->com_weights)
+        for (int64_t k = 0; k < ndet; ++k)
+            if (!std::isfinite(q->com_weights[k]))
+                return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the centre-of-mass weights must be finite");
+
+    auto *r = new (std::nothrow) cubo_hip_reduction();
+    if (!r) return fail(CUBO_HIP_ERR_OUT_OF_MEMORY, "host allocation");
+    r->cube = c;
+    if (int st = choose_path(c, q, r)) { delete r; return st; }
+    const int qx = (int)c->shape[3];
+    DeviceGuard g(c->device);
+    if (r->path == 2) {
+        const int nw = q->n_masks + (r->want_com ? 3 : 0);
+        const int pnw = padded_f64_nw(std::max(nw, 1));
+        if (pnw < 0) {
+            delete r;
+            return fail(CUBO_HIP_ERR_UNSUPPORTED, "%d weight rows in one pass (at most 16); split the masks", nw);
+        }
+        r->nw = pnw;
+        r->nv = pnw;
+        r->slice_px = cubo::kF64Slice;
+        r->blocks_per_cu = cubo::blocks_per_cu_f64(c->dtype == CUBO_HIP_U16, pnw);
+    } else {
+        r->blocks_per_cu = cubo::blocks_per_cu_int(r->nw, r->chunks, r->com_ones);
+    }
+    // Blocks per compute unit of the launch plan: the kernel's occupancy, or a tuned multiple of it.
+    if (const char *env = getenv("CUBO_HIP_BLOCKS_PER_CU")) r->blocks_per_cu = std::max(1, atoi(env));
+    r->nslices = (int)((ndet + r->slice_px - 1) / r->slice_px);
+
+    hipError_t e = hipSuccess;
+    if (r->path == 1) {
+        const int64_t pairs = ndet / 2;
+        std::vector<uint32_t> pk((size_t)std::max(r->nw, 1) * pairs, 0u);
+        auto put = [&](int row, int64_t px, uint32_t v) {
+            uint32_t &word = pk[(size_t)row * pairs + px / 2];
+            word |= (v & 0xffffu) << (16 * (px & 1));
+        };
+        for (int k = 0; k < q->n_masks; ++k)
+            for (int64_t px = 0; px < ndet; ++px) put(k, px, (uint32_t)q->masks[(size_t)k * ndet + px]);
+        if (r->want_com && !r->com_ones) {
+            for (int64_t px = 0; px < ndet; ++px) {
+                const uint32_t w = q->com_weights ? (uint32_t)q->com_weights[px] : 1u;
+                put(q->n_masks, px, w);
+                put(q->n_masks + 1, px, w * (uint32_t)(px / qx));
+                put(q->n_masks + 2, px, w * (uint32_t)(px % qx));
+            }
+        }
+        e = hipMalloc(&r->weights, pk.size() * 4);
+        if (e == hipSuccess) e = hipMemcpy(r->weights, pk.data(), pk.size() * 4, hipMemcpyHostToDevice);
+    } else {
+        std::vector<double> wd((size_t)r->nw * ndet, 0.0);
+        for (int k = 0; k < q->n_masks; ++k)
+            for (int64_t px = 0; px < ndet; ++px) wd[(size_t)k * ndet + px] = (double)q->masks[(size_t)k * ndet + px];
+        if (r->want_com) {
+            for (int64_t px = 0; px < ndet; ++px) {
+                const double w = q->com_weights ? (double)q->com_weights[px] : 1.0;
+                wd[(size_t)q->n_masks * ndet + px] = w;                             // w, w*i, w*j in Double
+                wd[(size_t)(q->n_masks + 1) * ndet + px] = w * (double)(px / qx);   // (CenterOfMass.swift:58-59)
+                wd[(size_t)(q->n_masks + 2) * ndet + px] = w * (double)(px % qx);
+            }
+        }
+        e = hipMalloc(&r->weights, wd.size() * 8);
+        if (e == hipSuccess) e = hipMemcpy(r->weights, wd.data(), wd.size() * 8, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) { free_reduction(r); return hip_fail(e, "uploading the weight rows"); }
+
+    // Scratch for the largest tile (the whole cube).
+    int ngroups = 0, ppg = 0, ppw = 0;
+    plan_groups(c->npos, r->nslices, cubo::compute_units(c->device), r->blocks_per_cu, &ngroups, &ppg, &ppw);
+    r->max_groups = ngroups;
+    r->res_count = (size_t)result_doubles(r);
+    struct { void **p; size_t n; } allocs[] = {
+        {&r->rowpart, (size_t)r->nslices * c->npos * std::max(r->nv, 1) * 8},
+        {&r->colsum_part, (size_t)ngroups * ndet * (r->path == 2 ? 8 : 4)},
+        {&r->colmax_part, (size_t)ngroups * ndet * 4},
+        {&r->col_sum, (size_t)ndet * 8}, {&r->col_max, (size_t)ndet * 4},
+        {(void **)&r->res_dev, r->res_count * 8}};
+    for (auto &a : allocs) {
+        if (a.n == 0) continue;
+        e = hipMalloc(a.p, a.n);
+        if (e != hipSuccess) { free_reduction(r); return hip_fail(e, "allocating reduction scratch"); }
+    }
+    e = hipHostMalloc((void **)&r->res_host, (r->res_count + 2 * (size_t)c->npos) * 8, hipHostMallocDefault);
+    if (e != hipSuccess) { free_reduction(r); return hip_fail(e, "allocating pinned result memory"); }
+    e = init_cols(r);
+    if (e != hipSuccess) { free_reduction(r); return hip_fail(e, "initialising the column sums"); }
+    *out = r;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows) {
+    if (!r) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL reduction");
+    cubo_hip_cube *c = r->cube;
+    if (int st = cubo::rows_range(c, row0, nrows)) return st;
+    if (row0 != r->next_row)
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "tiles must come in scan order: expected row %lld, got %lld",
+                    (long long)r->next_row, (long long)row0);
+    if (nrows == 0) return CUBO_HIP_OK;
+    DeviceGuard g(c->device);
+    const int64_t pos0 = row0 * c->shape[1];
+    const int64_t npos = nrows * c->shape[1];
+    const int ndet = (int)c->ndet;
+    int ngroups = 0, ppg = 0, ppw = 0;
+    plan_groups(npos, r->nslices, cubo::compute_units(c->device), r->blocks_per_cu, &ngroups, &ppg, &ppw);
+    if (ngroups > r->max_groups) return fail(CUBO_HIP_ERR_HIP, "internal: group plan exceeds scratch");
+    hipEvent_t t0, t1;
+    CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
+    CUBO_HIP_TRY(hipEventCreate(&t1), "hipEventCreate");
+    CUBO_HIP_TRY(hipEventRecord(t0, c->stream), "hipEventRecord");
+    hipError_t e;
+    if (r->path == 1) {
+        cubo::IntLaunch a{};
+        a.cube = (const uint16_t *)c->data + pos0 * c->ndet;
+        a.npos = npos;
+        a.ndet = ndet;
+        a.qx = (int)c->shape[3];
+        a.nslices = r->nslices;
+        a.ngroups = ngroups;
+        a.ppg = ppg;
+        a.ppw = ppw;
+        a.wpk = (const uint32_t *)r->weights;
+        a.rowpart = (uint64_t *)r->rowpart;
+        a.colsum = (uint32_t *)r->colsum_part;
+        a.colmax = (uint32_t *)r->colmax_part;
+        e = cubo::launch_int(a, r->nw, r->chunks, r->com_ones, c->stream);
+    } else {
+        cubo::F64Launch a{};
+        a.cube = (const char *)c->data + (size_t)pos0 * c->ndet * cubo::element_size(c->dtype);
+        a.npos = npos;
+        a.ndet = ndet;
+        a.nslices = r->nslices;
+        a.ngroups = ngroups;
+        a.ppg = ppg;
+        a.ppw = ppw;
+        a.w = (const double *)r->weights;
+        a.rowpart = (double *)r->rowpart;
+        a.colsum = (double *)r->colsum_part;
+        a.colmax = (float *)r->colmax_part;
+        e = cubo::launch_f64(a, c->dtype == CUBO_HIP_U16, r->nw, c->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "launching the fused reduction");
+    CUBO_HIP_TRY(hipEventRecord(t1, c->stream), "hipEventRecord");
+    c->timed.emplace_back(t0, t1);
+    if (r->nv > 0) {
+        e = r->path == 1 ? cubo::launch_finalize_rows_u64((const uint64_t *)r->rowpart, r->nslices, npos, r->nv,
+                                                          r->res_dev, pos0, c->npos, c->stream)
+                         : cubo::launch_finalize_rows_f64((const double *)r->rowpart, r->nslices, npos, r->nv,
+                                                          r->res_dev, pos0, c->npos, c->stream);
+        if (e != hipSuccess) return hip_fail(e, "launching the row totals");
+    }
+    e = r->path == 1 ? cubo::launch_accumulate_cols_int((const uint32_t *)r->colsum_part,
+                                                        (const uint32_t *)r->colmax_part, ngroups, ndet,
+                                                        (uint64_t *)r->col_sum, (uint32_t *)r->col_max, c->stream)
+                     : cubo::launch_accumulate_cols_f64((const double *)r->colsum_part, (const float *)r->colmax_part,
+                                                        ngroups, ndet, (double *)r->col_sum, (float *)r->col_max,
+                                                        c->stream);
+    if (e != hipSuccess) return hip_fail(e, "launching the column totals");
+    r->next_row = row0 + nrows;
+    r->rows_done += nrows;
+    r->viewed = false;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_reduce_view(cubo_hip_reduction *r, cubo_hip_result_view *v) {
+    if (!r || !v) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    cubo_hip_cube *c = r->cube;
+    if (r->rows_done != c->shape[0])
+        return fail(CUBO_HIP_ERR_INCOMPLETE, "the reduction covered %lld of %lld scan rows", (long long)r->rows_done,
+                    (long long)c->shape[0]);
+    const int64_t npos = c->npos, ndet = c->ndet;
+    double *rows = r->res_host;
+    double *mean = rows + (size_t)std::max(r->nv, 1) * npos, *max = mean + ndet;
+    double *crow = r->res_host + r->res_count, *ccol = crow + npos;
+    // The integer kernel's arithmetic CoM rows follow its weight rows; in every other layout the three CoM
+    // rows directly follow the masks.
+    const int base = (r->path == 1 && r->com_ones) ? r->nw : r->n_masks;
+    if (!r->viewed) {
+        DeviceGuard g(c->device);
+        hipError_t e = hipSuccess;
+        if (r->want_cols) {
+            double *dmean = r->res_dev + (size_t)std::max(r->nv, 1) * npos;
+            e = r->path == 1 ? cubo::launch_finish_cols_int((const uint64_t *)r->col_sum, (const uint32_t *)r->col_max,
+                                                            (int)ndet, (double)npos, dmean, dmean + ndet, c->stream)
+                             : cubo::launch_finish_cols_f64((const double *)r->col_sum, (const float *)r->col_max,
+                                                            (int)ndet, (double)npos, dmean, dmean + ndet, c->stream);
+            if (e != hipSuccess) return hip_fail(e, "launching the column finish");
+        }
+        CUBO_HIP_TRY(hipMemcpyAsync(r->res_host, r->res_dev, r->res_count * 8, hipMemcpyDeviceToHost, c->stream),
+                     "reading results");
+        CUBO_HIP_TRY(hipStreamSynchronize(c->stream), "waiting for the reduction");
+        if (r->want_com) {
+            const double *s = rows + (size_t)base * npos, *mr = s + npos, *mc = mr + npos;
+            for (int64_t p = 0; p < npos; ++p) {
+                crow[p] = mr[p] / s[p];       // NaN where S = 0 (CenterOfMass.swift:44-45)
+                ccol[p] = mc[p] / s[p];
+            }
+        }
+        r->viewed = true;
+    }
+    memset(v, 0, sizeof *v);
+    if (r->n_masks > 0) v->images = rows;
+    if (r->want_cols) { v->mean = mean; v->max = max; }
+    if (r->want_com) {
+        v->com_intensity = rows + (size_t)base * npos;
+        v->com_moment_row = rows + (size_t)(base + 1) * npos;
+        v->com_moment_column = rows + (size_t)(base + 2) * npos;
+        v->com_row = crow;
+        v->com_column = ccol;
+    }
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_reduce_results(cubo_hip_reduction *r, cubo_hip_reduce_result *res) {
+    if (!r || !res) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    cubo_hip_result_view v;
+    if (int st = cubo_hip_reduce_view(r, &v)) return st;
+    const int64_t npos = r->cube->npos, ndet = r->cube->ndet;
+    auto put = [](double *dst, const double *src, int64_t n) {
+        if (dst && src) memcpy(dst, src, n * sizeof(double));
+    };
+    put(res->images, v.images, (int64_t)r->n_masks * npos);
+    put(res->mean, v.mean, ndet);
+    put(res->max, v.max, ndet);
+    put(res->com_intensity, v.com_intensity, npos);
+    put(res->com_moment_row, v.com_moment_row, npos);
+    put(res->com_moment_column, v.com_moment_column, npos);
+    put(res->com_row, v.com_row, npos);
+    put(res->com_column, v.com_column, npos);
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_reduce_reset(cubo_hip_reduction *r) {
+    if (!r) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL reduction");
+    DeviceGuard g(r->cube->device);
+    r->rows_done = 0;
+    r->next_row = 0;
+    r->viewed = false;
+    hipError_t e = init_cols(r);
+    if (e != hipSuccess) return hip_fail(e, "initialising the column sums");
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_reduce_cancel(cubo_hip_reduction *r) {
+    if (!r) return CUBO_HIP_OK;
+    DeviceGuard g(r->cube->device);
+    (void)hipStreamSynchronize(r->cube->stream);
+    free_reduction(r);
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_reduce_release(cubo_hip_reduction *r) { return cubo_hip_reduce_cancel(r); }
+
+int cubo_hip_reduce_end(cubo_hip_reduction *r, cubo_hip_reduce_result *res) {
+    if (!r) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL reduction");
+    const int st = cubo_hip_reduce_results(r, res);
+    cubo_hip_reduce_cancel(r);
+    return st;
+}
+
+int cubo_hip_reduce(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_hip_reduce_result *res) {
+    cubo_hip_reduction *r = nullptr;
+    if (int st = cubo_hip_reduce_begin(c, q, &r)) return st;
+    if (int st = cubo_hip_reduce_rows(r, 0, c->shape[0])) { cubo_hip_reduce_cancel(r); return st; }
+    return cubo_hip_reduce_end(r, res);
+}
+
+}  // extern "C"

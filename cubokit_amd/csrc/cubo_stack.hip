@@ -24,18 +24,20 @@ struct cubo_hip_stack_plan {
     cubo_hip_cube *cube;
     int k;
     int method;          // CUBO_HIP_STACK_PLAIN or CUBO_HIP_STACK_MFMA_SPLIT
-    int ntiles;          // MFMA: 32-mask tiles
+    int ntiles;          // MFMA: 32-mask tiles per pass (1 or 2)
+    int passes;          // MFMA: passes of up to 64 masks
     int nchunks;         // MFMA: 64-pixel chunks
     int kpad;            // plain: masks per pass (padded)
     void *w;             // device weights
     double *inv_scale;   // MFMA: [k]
+    int tile_lo[8];      // MFMA: a 32-mask tile with a non-zero lo part (4 passes x 2 tiles)
     double *y;           // device output [npos][k]
 };
 
 namespace {
 
-// Masks per pass of the plain kernel: at most 32 keeps it at four waves per SIMD (64 FP32 + 64 FP64
-// accumulators would need every register of a wave); more masks take more passes over the cube.
+// Masks per pass of the plain kernel: at most 32 keeps two waves per SIMD (64 FP32 + 64 FP64 accumulators
+// would need every register of a wave); more masks take more passes over the cube.
 int padded_plain(int k) {
     for (int c : {8, 16, 32})
         if (k <= c) return c;
@@ -51,7 +53,7 @@ void free_plan(cubo_hip_stack_plan *p) {
     delete p;
 }
 
-bool mfma_ok(const cubo_hip_cube *c, int k) { return c->dtype == CUBO_HIP_U16 && c->ndet % 8 == 0 && k <= 128; }
+bool mfma_ok(const cubo_hip_cube *c, int k) { return c->dtype == CUBO_HIP_U16 && c->ndet % 8 == 0 && k <= 256; }
 
 }  // namespace
 
@@ -69,7 +71,7 @@ int cubo_hip_stack_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
     if (method == CUBO_HIP_STACK_MFMA_SPLIT && !mfma_ok(c, k))
         return fail(CUBO_HIP_ERR_UNSUPPORTED,
                     "the split-FP16 matrix-core stack needs a uint16 cube, a pattern length divisible by 8 and at "
-                    "most 128 masks");
+                    "most 256 masks");
     if (method != CUBO_HIP_STACK_MFMA_SPLIT && method != CUBO_HIP_STACK_PLAIN)
         return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "unknown mask-stack method %d", method);
 
@@ -83,7 +85,8 @@ int cubo_hip_stack_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
     if (e != hipSuccess) { free_plan(p); return hip_fail(e, "allocating the stack output"); }
 
     if (method == CUBO_HIP_STACK_MFMA_SPLIT) {
-        p->ntiles = k <= 32 ? 1 : k <= 64 ? 2 : 4;
+        p->ntiles = k <= 32 ? 1 : 2;
+        p->passes = (k + 63) / 64;
         p->nchunks = (int)((ndet + 63) / 64);
         // Column scale 2^(10 - floor(log2 max|w|)) puts every column's largest weight in [1024, 2048).
         std::vector<double> scale(k, 1.0), inv(k, 1.0);
@@ -97,23 +100,27 @@ int cubo_hip_stack_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
             }
         }
         // B operand order: [chunk][tile][kstep][hi, lo][lane][8 halves]; lane l holds mask 32*tile + l%32 and
-        // pixels 64*chunk + 16*kstep + 8*(l/32) + e, the same pixels the A operand's lane holds for its row.
+        // pixels 64*chunk + 32*(l/32) + 8*kstep + e, the same pixels the A operand's lane holds for its row.
         const size_t per_chunk = (size_t)p->ntiles * 4 * 2 * 64 * 8;
-        std::vector<_Float16> wb(per_chunk * p->nchunks);
+        const size_t per_pass = per_chunk * p->nchunks;
+        std::vector<_Float16> wb(per_pass * p->passes);
+        for (int pass = 0; pass < p->passes; ++pass)
         for (int ch = 0; ch < p->nchunks; ++ch)
             for (int t = 0; t < p->ntiles; ++t)
                 for (int s = 0; s < 4; ++s)
                     for (int l = 0; l < 64; ++l)
                         for (int el = 0; el < 8; ++el) {
-                            const int m = t * 32 + (l & 31);
-                            const int64_t px = (int64_t)ch * 64 + s * 16 + 8 * (l >> 5) + el;
+                            const int m = pass * 64 + t * 32 + (l & 31);
+                            const int64_t px = (int64_t)ch * 64 + 32 * (l >> 5) + 8 * s + el;
                             double v = 0;
                             if (m < k && px < ndet) v = (double)masks[(size_t)m * ndet + px] * scale[m];
                             const _Float16 hi = (_Float16)(float)v;
                             const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
-                            const size_t base = (((((size_t)ch * p->ntiles + t) * 4 + s) * 2) * 64 + l) * 8 + el;
+                            const size_t base =
+                                pass * per_pass + (((((size_t)ch * p->ntiles + t) * 4 + s) * 2) * 64 + l) * 8 + el;
                             wb[base] = hi;
                             wb[base + 64 * 8] = lo;
+                            if ((float)lo != 0.f) p->tile_lo[pass * 2 + t] = 1;
                         }
         e = hipMalloc(&p->w, wb.size() * sizeof(_Float16));
         if (e == hipSuccess) e = hipMemcpy(p->w, wb.data(), wb.size() * sizeof(_Float16), hipMemcpyHostToDevice);
@@ -153,16 +160,23 @@ int cubo_hip_stack_plan_run(cubo_hip_stack_plan *p, double *y) {
     CUBO_HIP_TRY(hipEventRecord(t0, c->stream), "hipEventRecord");
     hipError_t e = hipSuccess;
     if (p->method == CUBO_HIP_STACK_MFMA_SPLIT) {
-        cubo::StackLaunch a{};
-        a.cube = (const uint16_t *)c->data;
-        a.npos = c->npos;
-        a.ndet = c->ndet;
-        a.nchunks = p->nchunks;
-        a.k = p->k;
-        a.wb = p->w;
-        a.inv_scale = p->inv_scale;
-        a.y = p->y;
-        e = cubo::launch_mask_stack_mfma(a, p->ntiles, c->stream);
+        const size_t per_pass = (size_t)p->ntiles * 4 * 2 * 64 * 8 * p->nchunks;
+        for (int pass = 0; pass < p->passes && e == hipSuccess; ++pass) {
+            cubo::StackLaunch a{};
+            a.cube = (const uint16_t *)c->data;
+            a.npos = c->npos;
+            a.ndet = c->ndet;
+            a.nchunks = p->nchunks;
+            a.k = p->k;
+            a.k0 = pass * 64;
+            a.kn = std::min(64, p->k - pass * 64);
+            a.wb = (const _Float16 *)p->w + pass * per_pass;
+            a.inv_scale = p->inv_scale + pass * 64;
+            a.tile_lo[0] = p->tile_lo[pass * 2];
+            a.tile_lo[1] = p->tile_lo[pass * 2 + 1];
+            a.y = p->y;
+            e = cubo::launch_mask_stack_mfma(a, p->ntiles, c->stream);
+        }
     } else {
         const int passes = (p->k + p->kpad - 1) / p->kpad;
         for (int pass = 0; pass < passes && e == hipSuccess; ++pass) {

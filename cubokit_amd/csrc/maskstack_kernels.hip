@@ -8,10 +8,10 @@
 //  - split-FP16 MFMA (uint16 cubes): x = (x & 0xF800) + (x & 0x07FF), both parts exact in FP16; each
 //    mask column is scaled by a power of two and split into FP16 hi + lo (22 significant bits). Products
 //    are exact in the FP32 MFMA accumulators, which are flushed into FP64 every 256 pixels so FP32
-//    rounding cannot build up. The high data part is multiplied only for 64-pixel chunks whose block
-//    holds a count of 2048 or more.
-//  - plain VALU (uint16 or Float32 cubes): one position per lane, FP32 FMAs over 32-pixel chunks
-//    staged in LDS, weights as wave-uniform scalar loads, FP64 across 32-pixel chunks.
+//    rounding cannot build up. The high data part is multiplied only for 64-pixel chunks where the wave
+//    holds a count of 2048 or more, the lo weight part only for 32-mask tiles that have one.
+//  - plain VALU (uint16 or Float32 cubes): one position per lane, FP32 FMAs over 64-pixel chunks staged
+//    in LDS with their weights (read as LDS broadcasts), FP64 across chunks.
 #include "maskstack_kernels.hpp"
 
 namespace cubo {
@@ -22,10 +22,8 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
-constexpr int kStackBM = 128;          // positions per block (32 per wave)
 constexpr int kStackBK = 64;           // pixels per chunk
 constexpr int kFlushChunks = 4;        // FP32 -> FP64 every 4 chunks (256 pixels)
-constexpr int kRowHalves = kStackBK + 8;   // LDS row stride in halves (16-byte pad)
 
 __device__ __forceinline__ uint32_t cvt_pair(uint32_t w) {
     // two uint16 below 2^16 whose values are exact in FP16 (callers mask them to <= 2047 or to multiples
@@ -35,24 +33,50 @@ __device__ __forceinline__ uint32_t cvt_pair(uint32_t w) {
     return __builtin_bit_cast(uint32_t, h);
 }
 
-template <int NT>
-__global__ __launch_bounds__(256) void mask_stack_mfma(StackLaunch a) {
-    __shared__ _Float16 lo_t[kStackBM * kRowHalves];
-    __shared__ _Float16 hi_t[kStackBM * kRowHalves];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t p0 = (int64_t)blockIdx.x * kStackBM;
-    const int nchunks = a.nchunks;
+// Block: 8 waves x 32 positions. Each lane loads its A operand straight from HBM (16 bytes: 8 pixels of
+// one position per k-step), two 64-pixel chunks ahead of the matrix cores; the block shares one LDS copy
+// of the weight chunk (double-buffered, one barrier per chunk).
+constexpr int kStackWaves = 8;
+constexpr int kStackBM = 32 * kStackWaves;
 
-    // Loader mapping: thread t covers positions t/8 + 32j (j = 0..3), 16 bytes at pixel (t % 8) * 8.
-    const int lrow = tid >> 3, lcol = (tid & 7) * 8;
+template <int NT>
+__global__ __launch_bounds__(64 * kStackWaves) void mask_stack_mfma(StackLaunch a) {
+    constexpr int WCH = NT * 4 * 2 * 64;                 // half8 entries of one weight chunk
+    __shared__ half8 wlds[2][WCH];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t prow = (int64_t)blockIdx.x * kStackBM + wave * 32 + (lane & 31);
+    const bool live = prow < a.npos;
+    const int nchunks = a.nchunks;
+    const half8 *wsrc = reinterpret_cast<const half8 *>(a.wb);
+    // Within a 64-pixel chunk, k-step s of lane half h holds pixels 32h + 8s + (0..7): each lane reads 64
+    // contiguous bytes over the four k-steps, the two halves one full 128-byte line per position. Plain
+    // (cached) loads keep the line in L2 between the four k-step loads.
+    const uint16_t *xrow = a.cube + (live ? prow : 0) * a.ndet + 32 * (lane >> 5);
+
     auto load_x = [&](u32x4 (&x)[4], int chunk) {
-        const int px = chunk * kStackBK + lcol;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t p = p0 + lrow + 32 * j;
-            x[j] = (p < a.npos && px < a.ndet)
-                       ? __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.cube + p * a.ndet + px))
-                       : u32x4{0u, 0u, 0u, 0u};
+        for (int s = 0; s < 4; ++s) {
+            const int64_t px = (int64_t)chunk * kStackBK + 32 * (lane >> 5) + 8 * s;
+            x[s] = (live && px < a.ndet) ? *reinterpret_cast<const u32x4 *>(xrow + (int64_t)chunk * kStackBK + 8 * s)
+                                         : u32x4{0u, 0u, 0u, 0u};
+        }
+    };
+    // The block's weight chunk: each thread carries WPT 16-byte entries from HBM (or L2) to LDS. They are
+    // loaded before the data prefetch of the same step, so waiting for them never waits for that prefetch.
+    constexpr int WPT = (WCH + 64 * kStackWaves - 1) / (64 * kStackWaves);
+    half8 wreg[WPT];
+    auto fetch_w = [&](int chunk) {
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+            const int e = tid + j * 64 * kStackWaves;
+            if (e < WCH) wreg[j] = wsrc[(size_t)chunk * WCH + e];
+        }
+    };
+    auto put_w = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+            const int e = tid + j * 64 * kStackWaves;
+            if (e < WCH) wlds[buf][e] = wreg[j];
         }
     };
 
@@ -64,51 +88,43 @@ __global__ __launch_bounds__(256) void mask_stack_mfma(StackLaunch a) {
         for (int r = 0; r < 16; ++r) { acc[n][r] = 0.f; accd[n][r] = 0.0; }
     }
 
-    u32x4 x[4];
-    load_x(x, 0);
-    for (int chunk = 0; chunk < nchunks; ++chunk) {
-        // 1. convert the chunk held in registers and stage it in LDS
-        uint32_t any_hi = 0;
-        __syncthreads();                               // previous chunk's LDS reads are done
+    auto compute = [&](const u32x4 (&x)[4], int chunk) {
+        uint32_t any = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint32_t w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
-            uint32_t lo[4], hi[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                lo[k] = cvt_pair(w[k] & 0x07ff07ffu);
-                const uint32_t h = w[k] & 0xf800f800u;
-                any_hi |= h;
-                hi[k] = cvt_pair(h);
-            }
-            const int row = lrow + 32 * j;
-            *reinterpret_cast<u32x4 *>(&lo_t[row * kRowHalves + lcol]) = u32x4{lo[0], lo[1], lo[2], lo[3]};
-            *reinterpret_cast<u32x4 *>(&hi_t[row * kRowHalves + lcol]) = u32x4{hi[0], hi[1], hi[2], hi[3]};
-        }
-        const bool use_hi = __syncthreads_or(any_hi != 0);
-        // 2. prefetch the next chunk while the matrix cores work on this one
-        if (chunk + 1 < nchunks) load_x(x, chunk + 1);
-        // 3. MFMA over the chunk: 4 k-steps of 16 pixels
-        const half8 *wb = reinterpret_cast<const half8 *>(a.wb) + (size_t)chunk * NT * 4 * 2 * 64;
+        for (int s = 0; s < 4; ++s) any |= (x[s].x | x[s].y | x[s].z | x[s].w) & 0xf800f800u;
+        const bool use_hi = __builtin_amdgcn_ballot_w64(any != 0) != 0;   // wave-uniform
+        const half8 *wl = wlds[chunk & 1];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const int arow = wave * 32 + (lane & 31);
-            const int acol = s * 16 + (lane >> 5) * 8;
-            const half8 alo = *reinterpret_cast<const half8 *>(&lo_t[arow * kRowHalves + acol]);
-            const half8 ahi = *reinterpret_cast<const half8 *>(&hi_t[arow * kRowHalves + acol]);
+            const uint32_t w[4] = {x[s].x, x[s].y, x[s].z, x[s].w};
+            u32x4 lo, hi;
+            lo.x = cvt_pair(w[0] & 0x07ff07ffu); lo.y = cvt_pair(w[1] & 0x07ff07ffu);
+            lo.z = cvt_pair(w[2] & 0x07ff07ffu); lo.w = cvt_pair(w[3] & 0x07ff07ffu);
+            const half8 alo = __builtin_bit_cast(half8, lo);
 #pragma unroll
             for (int n = 0; n < NT; ++n) {
-                const half8 bh = wb[((n * 4 + s) * 2 + 0) * 64 + lane];
-                const half8 bl = wb[((n * 4 + s) * 2 + 1) * 64 + lane];
+                const half8 bh = wl[((n * 4 + s) * 2 + 0) * 64 + lane];
                 acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bh, acc[n], 0, 0, 0);
-                acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bl, acc[n], 0, 0, 0);
-                if (use_hi) {
+                if (a.tile_lo[n]) {
+                    const half8 bl = wl[((n * 4 + s) * 2 + 1) * 64 + lane];
+                    acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bl, acc[n], 0, 0, 0);
+                }
+            }
+            if (use_hi) {
+                hi.x = cvt_pair(w[0] & 0xf800f800u); hi.y = cvt_pair(w[1] & 0xf800f800u);
+                hi.z = cvt_pair(w[2] & 0xf800f800u); hi.w = cvt_pair(w[3] & 0xf800f800u);
+                const half8 ahi = __builtin_bit_cast(half8, hi);
+#pragma unroll
+                for (int n = 0; n < NT; ++n) {
+                    const half8 bh = wl[((n * 4 + s) * 2 + 0) * 64 + lane];
                     acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bh, acc[n], 0, 0, 0);
-                    acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bl, acc[n], 0, 0, 0);
+                    if (a.tile_lo[n]) {
+                        const half8 bl = wl[((n * 4 + s) * 2 + 1) * 64 + lane];
+                        acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bl, acc[n], 0, 0, 0);
+                    }
                 }
             }
         }
-        // 4. flush FP32 into FP64
         if ((chunk + 1) % kFlushChunks == 0 || chunk + 1 == nchunks) {
 #pragma unroll
             for (int n = 0; n < NT; ++n) {
@@ -116,18 +132,44 @@ __global__ __launch_bounds__(256) void mask_stack_mfma(StackLaunch a) {
                 for (int r = 0; r < 16; ++r) { accd[n][r] += (double)acc[n][r]; acc[n][r] = 0.f; }
             }
         }
+    };
+
+    // Chunk c's weights are in wlds[c & 1] before the barrier that precedes compute(c); each step fetches
+    // the next chunk's weights, then the data two chunks ahead, multiplies, and stores the weights into the
+    // other buffer (read last by compute(c - 1), which every wave finished before the previous barrier).
+    u32x4 X0[4], X1[4], X2[4];
+    fetch_w(0);
+    load_x(X0, 0);
+    if (nchunks > 1) load_x(X1, 1);
+    put_w(0);
+    __syncthreads();
+    auto step = [&](const u32x4 (&xc)[4], u32x4 (&xn)[4], int c) {
+        if (c + 1 < nchunks) fetch_w(c + 1);
+        if (c + 2 < nchunks) load_x(xn, c + 2);
+        compute(xc, c);
+        if (c + 1 < nchunks) put_w((c + 1) & 1);
+        __syncthreads();
+    };
+    int c = 0;
+    for (; c + 3 <= nchunks; c += 3) {
+        step(X0, X2, c);
+        step(X1, X0, c + 1);
+        step(X2, X1, c + 2);
     }
+    if (c < nchunks) step(X0, X2, c);
+    if (c + 1 < nchunks) step(X1, X0, c + 1);
+
     // 32x32 accumulator layout: register r of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+    const int64_t pbase = (int64_t)blockIdx.x * kStackBM + wave * 32;
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
         const int col = n * 32 + (lane & 31);
-        if (col >= a.k) continue;
+        if (col >= a.kn) continue;
         const double inv = a.inv_scale[col];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int row = wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int64_t p = p0 + row;
-            if (p < a.npos) a.y[p * a.k + col] = accd[n][r] * inv;
+            const int64_t p = pbase + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (p < a.npos) a.y[p * a.k + a.k0 + col] = accd[n][r] * inv;
         }
     }
 }
@@ -136,37 +178,94 @@ __global__ __launch_bounds__(256) void mask_stack_mfma(StackLaunch a) {
 // Plain VALU path: a lane per position.
 // ------------------------------------------------------------------------------------------------
 constexpr int kPlainBM = 256;
-constexpr int kPlainBK = 32;
-constexpr int kPlainRow = kPlainBK + 1;     // floats per LDS row (odd word stride: lanes hit distinct banks)
+constexpr int kPlainBK = 64;                 // one 128-byte line of a uint16 row per chunk
+
+template <typename T>
+struct PlainRow { static constexpr int value = kPlainBK + (sizeof(T) == 2 ? 2 : 1); };   // pad: lanes hit distinct banks
 
 template <typename T, int K>
 __global__ __launch_bounds__(256) void mask_stack_plain(PlainStackLaunch a) {
-    __shared__ float tile[kPlainBM * kPlainRow];
+    constexpr int ROW = PlainRow<T>::value;
+    constexpr int PIECES = kPlainBK * sizeof(T) / 16;          // 16-byte pieces per position per chunk
+    constexpr int XPT = kPlainBM * PIECES / 256;                // per thread
+    constexpr int ELEMS = 16 / sizeof(T);
+    constexpr int WPT = (kPlainBK * K / 4 + 255) / 256;         // float4 weights per thread
+    __shared__ T tile[kPlainBM * ROW];                          // the chunk in the cube's element type
+    __shared__ float4 wt[kPlainBK * K / 4];                     // its weights, read as broadcasts
     const int tid = threadIdx.x;
     const int64_t p0 = (int64_t)blockIdx.x * kPlainBM;
     const T *cube = reinterpret_cast<const T *>(a.cube);
+    const float4 *w4src = reinterpret_cast<const float4 *>(a.w);
+    const bool vec = a.ndet % ELEMS == 0;                       // 16-byte loads stay aligned and in the row
+
+    u32x4 xr[XPT];
+    float4 wr[WPT];
+    auto fetch = [&](int64_t c0) {
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+            const int e = tid + 256 * j;
+            const int64_t px = c0 + (e * 4) / K;
+            if (e < kPlainBK * K / 4) wr[j] = px < a.ndet ? w4src[c0 * (K / 4) + e] : float4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int j = 0; j < XPT; ++j) {
+            const int q = tid + 256 * j, r = q / PIECES, pc = q % PIECES;
+            const int64_t p = p0 + r, px = c0 + pc * ELEMS;
+            u32x4 v{0u, 0u, 0u, 0u};
+            if (p < a.npos && px < a.ndet) {
+                if (vec) {
+                    v = *reinterpret_cast<const u32x4 *>(cube + p * a.ndet + px);
+                } else {
+                    T t[ELEMS];
+#pragma unroll
+                    for (int e = 0; e < ELEMS; ++e) t[e] = px + e < a.ndet ? cube[p * a.ndet + px + e] : (T)0;
+                    v = __builtin_bit_cast(u32x4, t);
+                }
+            }
+            xr[j] = v;
+        }
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+            const int e = tid + 256 * j;
+            if (e < kPlainBK * K / 4) wt[e] = wr[j];
+        }
+#pragma unroll
+        for (int j = 0; j < XPT; ++j) {
+            const int q = tid + 256 * j, r = q / PIECES, pc = q % PIECES;
+            const T *t = reinterpret_cast<const T *>(&xr[j]);
+#pragma unroll
+            for (int e = 0; e < ELEMS; ++e) tile[r * ROW + pc * ELEMS + e] = t[e];
+        }
+    };
+
     float acc[K];
     double accd[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { acc[k] = 0.f; accd[k] = 0.0; }
-    for (int64_t c0 = 0; c0 < a.ndet; c0 += kPlainBK) {
+    const int nchunks = (int)((a.ndet + kPlainBK - 1) / kPlainBK);
+    fetch(0);
+    for (int ch = 0; ch < nchunks; ++ch) {
+        put();                                                   // chunk ch: registers -> LDS
         __syncthreads();
-        // stage 256 positions x 32 pixels: each pass of the block covers 8 positions x 32 pixels
-        for (int e = tid; e < kPlainBM * kPlainBK; e += 256) {
-            const int r = e / kPlainBK, c = e % kPlainBK;
-            const int64_t p = p0 + r, px = c0 + c;
-            tile[r * kPlainRow + c] = (p < a.npos && px < a.ndet) ? (float)cube[p * a.ndet + px] : 0.f;
-        }
-        __syncthreads();
-        const int n = (int)min((int64_t)kPlainBK, a.ndet - c0);
-        for (int c = 0; c < n; ++c) {
-            const float x = tile[tid * kPlainRow + c];
-            const float *w = a.w + (c0 + c) * K;          // wave-uniform: scalar loads
+        if (ch + 1 < nchunks) fetch((int64_t)(ch + 1) * kPlainBK);   // chunk ch + 1 in flight meanwhile
+#pragma unroll 2
+        for (int c = 0; c < kPlainBK; ++c) {
+            const float x = (float)tile[tid * ROW + c];
+            const float4 *w4 = &wt[c * K / 4];
 #pragma unroll
-            for (int k = 0; k < K; ++k) acc[k] = __builtin_fmaf(x, w[k], acc[k]);
+            for (int k = 0; k < K / 4; ++k) {
+                const float4 w = w4[k];
+                acc[4 * k] = __builtin_fmaf(x, w.x, acc[4 * k]);
+                acc[4 * k + 1] = __builtin_fmaf(x, w.y, acc[4 * k + 1]);
+                acc[4 * k + 2] = __builtin_fmaf(x, w.z, acc[4 * k + 2]);
+                acc[4 * k + 3] = __builtin_fmaf(x, w.w, acc[4 * k + 3]);
+            }
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) { accd[k] += (double)acc[k]; acc[k] = 0.f; }
+        __syncthreads();                                         // every lane is done with the chunk in LDS
     }
     const int64_t p = p0 + tid;
     if (p < a.npos) {
@@ -177,11 +276,10 @@ __global__ __launch_bounds__(256) void mask_stack_plain(PlainStackLaunch a) {
 }
 
 hipError_t launch_mask_stack_mfma(const StackLaunch &a, int nt, hipStream_t s) {
-    const dim3 grid((unsigned)((a.npos + kStackBM - 1) / kStackBM)), block(256);
+    const dim3 grid((unsigned)((a.npos + kStackBM - 1) / kStackBM)), block(64 * kStackWaves);
     switch (nt) {
         case 1: hipLaunchKernelGGL(mask_stack_mfma<1>, grid, block, 0, s, a); break;
         case 2: hipLaunchKernelGGL(mask_stack_mfma<2>, grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL(mask_stack_mfma<4>, grid, block, 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -195,7 +293,7 @@ hipError_t launch_mask_stack_plain(const PlainStackLaunch &a, bool is_u16, int k
         else hipLaunchKernelGGL((mask_stack_plain<float, K>), grid, block, 0, s, a); \
         return hipGetLastError(); \
     }
-    CUBO_PLAIN(8) CUBO_PLAIN(16) CUBO_PLAIN(32) CUBO_PLAIN(64)
+    CUBO_PLAIN(8) CUBO_PLAIN(16) CUBO_PLAIN(32)
 #undef CUBO_PLAIN
     return hipErrorInvalidValue;
 }

@@ -11,9 +11,11 @@ struct StackLaunch {
     int64_t npos;
     int64_t ndet;             // multiple of 8
     int32_t nchunks;          // ceil(ndet / 64)
-    int32_t k;                // masks
+    int32_t k;                // masks in the whole stack (the row stride of y)
+    int32_t k0, kn;           // this pass: masks [k0, k0 + kn), kn <= 32 * ntiles
     const void *wb;           // FP16 hi/lo weights in MFMA B-operand order [chunk][ntile][kstep][hi,lo][lane][8]
-    const double *inv_scale;  // [k] 1 / (power-of-two column scale)
+    const double *inv_scale;  // [kn] 1 / (power-of-two column scale) of this pass's masks
+    int32_t tile_lo[2];       // per 32-mask tile: 1 if any weight needs the FP16 lo part
     double *y;                // [npos][k]
 };
 
@@ -23,7 +25,7 @@ struct PlainStackLaunch {
     int64_t ndet;
     int32_t k;                // masks in the whole stack (the row stride of y)
     int32_t k0, kn;           // this pass: masks [k0, k0 + kn)
-    const float *w;           // [ndet][kpad] FP32 weights of this pass, pixel-major
+    const float *w;           // [ndet rounded up to 64][kpad] FP32 weights of this pass, pixel-major
     double *y;                // [npos][k]
 };
 

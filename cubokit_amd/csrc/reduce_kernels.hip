@@ -15,7 +15,8 @@
 //
 // Column partials of the four waves of a block are combined in LDS and written once per block (one
 // partial per position group); row partials are written once per (slice, position). Two small kernels
-// then sum slices (rows) and groups (columns) in a fixed order, so every result is deterministic.
+// then sum groups (columns) and slices (rows) in a fixed order, so every result is deterministic.
+// (Device-scope atomics for the column partials measured slower: their cost grew with the block count.)
 #include "reduce_kernels.hpp"
 
 namespace cubo {
@@ -356,19 +357,31 @@ __global__ void finalize_rows(const Acc *rowpart, int nslices, int64_t npos, int
     rows_out[(int64_t)v * total_pos + pos0 + p] = (double)s;
 }
 
-__global__ void accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups, int ndet,
-                                    uint64_t *sum, uint32_t *max) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ndet) return;
-    uint64_t s = sum[i];
-    uint32_t m = max[i];
-    for (int g = 0; g < ngroups; ++g) {
-        s += colsum[(size_t)g * ndet + i];
-        uint32_t v = colmax[(size_t)g * ndet + i];
-        m = v > m ? v : m;
+// 64 pixels x 4 group phases per block: each thread sums every fourth group of one pixel, LDS adds the four.
+__global__ __launch_bounds__(256) void accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups,
+                                                           int ndet, uint64_t *sum, uint32_t *max) {
+    __shared__ uint64_t ps[4][64];
+    __shared__ uint32_t pm[4][64];
+    const int lx = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lx;
+    uint64_t s = 0;
+    uint32_t m = 0;
+    if (i < ndet)
+        for (int g = q; g < ngroups; g += 4) {
+            s += colsum[(size_t)g * ndet + i];
+            const uint32_t v = colmax[(size_t)g * ndet + i];
+            m = v > m ? v : m;
+        }
+    ps[q][lx] = s;
+    pm[q][lx] = m;
+    __syncthreads();
+    if (q == 0 && i < ndet) {
+        s = sum[i] + ps[0][lx] + ps[1][lx] + ps[2][lx] + ps[3][lx];
+        m = max[i];
+        for (int k = 0; k < 4; ++k) m = pm[k][lx] > m ? pm[k][lx] : m;
+        sum[i] = s;
+        max[i] = m;
     }
-    sum[i] = s;
-    max[i] = m;
 }
 
 __global__ void accumulate_cols_f64(const double *colsum, const float *colmax, int ngroups, int ndet, double *sum,
@@ -399,6 +412,14 @@ __global__ void finish_cols_f64(const double *sum, const float *max, int ndet, d
     if (i >= ndet) return;
     mean[i] = sum[i] / count;
     maxd[i] = (double)max[i];
+}
+
+__global__ void copy_bytes(const u32x4 *src, u32x4 *dst, uint64_t n16, const uint8_t *tail_src, uint8_t *tail_dst,
+                           uint32_t tail) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    if (blockIdx.x == 0 && threadIdx.x < tail) tail_dst[threadIdx.x] = tail_src[threadIdx.x];
 }
 
 __global__ void fill_cols_init(uint64_t *isum, uint32_t *imax, double *fsum, float *fmax, int ndet) {
@@ -440,6 +461,16 @@ hipError_t launch_int(const IntLaunch &a, int nw, int chunks, bool com_ones, hip
     return hipErrorInvalidValue;
 }
 
+int blocks_per_cu_int(int nw, int chunks, bool com_ones) {
+    int n = 0;
+#define X(CH, NW, C1) \
+    if (chunks == CH && nw == NW && com_ones == C1) \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_u16_int<CH, NW, C1>, kBlock, 0) != hipSuccess) n = 1;
+    CUBO_INT_CASES(X)
+#undef X
+    return n;
+}
+
 #define CUBO_F64_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(11) X(16)
 
 bool supported_f64(int nw) {
@@ -462,6 +493,19 @@ hipError_t launch_f64(const F64Launch &a, bool is_u16, int nw, hipStream_t s) {
     return hipErrorInvalidValue;
 }
 
+int blocks_per_cu_f64(bool is_u16, int nw) {
+    int n = 0;
+#define X(NW) \
+    if (nw == NW) { \
+        hipError_t e = is_u16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_f64<uint16_t, NW>, kBlock, 0) \
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_f64<float, NW>, kBlock, 0); \
+        if (e != hipSuccess) n = 1; \
+    }
+    CUBO_F64_CASES(X)
+#undef X
+    return n;
+}
+
 static unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 hipError_t launch_finalize_rows_u64(const uint64_t *rowpart, int nslices, int64_t npos, int nv, double *rows_out,
@@ -480,8 +524,8 @@ hipError_t launch_finalize_rows_f64(const double *rowpart, int nslices, int64_t 
 
 hipError_t launch_accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups, int ndet,
                                       uint64_t *sum, uint32_t *max, hipStream_t s) {
-    hipLaunchKernelGGL(accumulate_cols_int, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, colsum, colmax, ngroups,
-                       ndet, sum, max);
+    hipLaunchKernelGGL(accumulate_cols_int, dim3(blocks_for(ndet, 64)), dim3(256), 0, s, colsum, colmax, ngroups, ndet,
+                       sum, max);
     return hipGetLastError();
 }
 
@@ -503,6 +547,14 @@ hipError_t launch_finish_cols_f64(const double *sum, const float *max, int ndet,
                                   double *maxd, hipStream_t s) {
     hipLaunchKernelGGL(finish_cols_f64, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, sum, max, ndet, count, mean,
                        maxd);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_bytes(const void *src, void *dst, uint64_t n, hipStream_t s) {
+    const uint64_t n16 = n / 16;
+    const uint32_t tail = (uint32_t)(n % 16);
+    hipLaunchKernelGGL(copy_bytes, dim3(4096), dim3(256), 0, s, (const u32x4 *)src, (u32x4 *)dst, n16,
+                       (const uint8_t *)src + n16 * 16, (uint8_t *)dst + n16 * 16, tail);
     return hipGetLastError();
 }
 

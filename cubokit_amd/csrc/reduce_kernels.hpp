@@ -56,6 +56,9 @@ constexpr int kF64Slice = kWave * kF64Chunks;
 // that (nw, chunks) is one of the compiled combinations with supported_int().
 bool supported_int(int nw, int chunks, bool com_ones);
 hipError_t launch_int(const IntLaunch &a, int nw, int chunks, bool com_ones, hipStream_t s);
+// Resident blocks per compute unit of an instantiation (0 if it is not compiled).
+int blocks_per_cu_int(int nw, int chunks, bool com_ones);
+int blocks_per_cu_f64(bool is_u16, int nw);
 bool supported_f64(int nw);
 hipError_t launch_f64(const F64Launch &a, bool is_u16, int nw, hipStream_t s);
 
@@ -74,6 +77,8 @@ hipError_t launch_finish_cols_int(const uint64_t *sum, const uint32_t *max, int 
                                   double *maxd, hipStream_t s);
 hipError_t launch_finish_cols_f64(const double *sum, const float *max, int ndet, double count, double *mean,
                                   double *maxd, hipStream_t s);
+// dst = src for n bytes (16-byte aligned buffers): 16-byte loads and stores, grid-stride.
+hipError_t launch_copy_bytes(const void *src, void *dst, uint64_t n, hipStream_t s);
 hipError_t launch_fill_cols_init(uint64_t *isum, uint32_t *imax, double *fsum, float *fmax, int ndet, hipStream_t s);
 
 }  // namespace cubo

@@ -397,6 +397,14 @@ class HIPResidentCube:
             raise CuboError(f"position ({row}, {column}) outside the scan")
         return self.read_scan_tile(range(row, row + 1))[0, column].copy()
 
+    def clone(self) -> "HIPResidentCube":
+        """A second resident cube with the same contents, copied on the device."""
+        h = ct.c_void_p()
+        _check(L.lib().cubo_hip_cube_clone(self.handle, ct.byref(h)), "cloning the cube")
+        c = HIPResidentCube.__new__(HIPResidentCube)
+        c.shape, c.dtype, c.device, c._h = self.shape, self.dtype, self.device, h
+        return c
+
     def fill_synthetic(self, params: L.SynthParams) -> None:
         _check(L.lib().cubo_hip_cube_fill_synthetic(self.handle, ct.byref(params)), "synthesising the cube")
 

@@ -90,6 +90,9 @@ int cubo_hip_cube_upload_rows(cubo_hip_cube *cube, int64_t row0, int64_t nrows, 
 /* readScanTile (ResidentCube.swift:113-117): copies rows out to host memory. */
 int cubo_hip_cube_read_rows(cubo_hip_cube *cube, int64_t row0, int64_t nrows, void *host);
 int cubo_hip_cube_destroy(cubo_hip_cube *cube);
+/* init(loading:) from another resident cube: a new cube on the same device with the same shape, dtype and
+ * contents, copied on the device by one streaming kernel (every byte read once and written once). */
+int cubo_hip_cube_clone(const cubo_hip_cube *source, cubo_hip_cube **out);
 /* byteCount (ResidentCube.swift:109). */
 int cubo_hip_cube_byte_count(const cubo_hip_cube *cube, uint64_t *bytes);
 
@@ -130,6 +133,12 @@ int cubo_hip_reduce_cancel(cubo_hip_reduction *r);
  * results waits and writes the caller's buffers but keeps the reduction; reset starts a new pass
  * (every row again); release frees it. reduce_end = results + release. */
 int cubo_hip_reduce_results(cubo_hip_reduction *r, cubo_hip_reduce_result *result);
+/* The same results without a copy: pointers into the reduction's pinned host memory, valid until the next
+ * reset, rows or release call on it (NULL for what was not requested). */
+typedef struct {
+    const double *images, *mean, *max, *com_intensity, *com_moment_row, *com_moment_column, *com_row, *com_column;
+} cubo_hip_result_view;
+int cubo_hip_reduce_view(cubo_hip_reduction *r, cubo_hip_result_view *view);
 int cubo_hip_reduce_reset(cubo_hip_reduction *r);
 int cubo_hip_reduce_release(cubo_hip_reduction *r);
 /* One call: begin, all rows as one tile, end. */
@@ -150,7 +159,7 @@ enum {
     CUBO_HIP_STACK_AUTO = 0,        /* the matrix-core method where it applies, else plain */
     CUBO_HIP_STACK_PLAIN = 1,       /* VALU: FP32 FMAs, FP64 across 32-pixel chunks; any cube */
     CUBO_HIP_STACK_MFMA_SPLIT = 2   /* uint16 cube, split-FP16 operands on MFMA, FP32 accumulators flushed to
-                                       FP64 every 256 pixels; at most 128 masks */
+                                       FP64 every 256 pixels; at most 256 masks (passes of 64) */
 };
 typedef struct cubo_hip_stack_plan cubo_hip_stack_plan;
 /* masks: k x detectorCount FP32 weights, row-major. */

@@ -52,7 +52,7 @@ def test_counts_above_2048_take_the_high_part():
     _check_columns(y, O.mask_stack(host, masks.astype(np.float64)), 1e-6, "high counts")
 
 
-@pytest.mark.parametrize("k", [1, 5, 33, 100])
+@pytest.mark.parametrize("k", [1, 5, 33, 100, 200])
 def test_odd_mask_counts_and_ragged_tiles(k):
     res = _cube((5, 7), 96, 300.0, 13)                 # 35 positions: a partial 128-position tile
     rng = np.random.default_rng(k)

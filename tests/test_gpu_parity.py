@@ -265,3 +265,52 @@ def test_tilings_give_the_same_bits_on_the_integer_kernel():
         assert_identical(b.mean.values, a.mean.values, f"mean {t}")
         assert_identical(b.max.values, a.max.values, f"max {t}")
         assert_identical(b.com.moment_column.values, a.com.moment_column.values, f"com {t}")
+
+
+def test_view_and_results_agree_and_reset_reuses_the_reduction():
+    """cubo_hip_reduce_view (pinned, no copy) and cubo_hip_reduce_results give the same bits; reset runs the same
+    request again without reallocation and gives the same bits again."""
+    import ctypes as ct
+    from cubokit_amd import _lib as L
+    shape, dtype, params = synthetic.bandwidth_u16(scan=(16, 12))
+    res = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
+    res.fill_synthetic(params)
+    masks = np.stack([C.DetectorMask.make(C.DetectorGeometry.circle(64, 64, 14), res.shape)])
+    req = L.ReduceRequest(masks=masks.ctypes.data, n_masks=1, want_mean_max=1, want_com=1, com_weights=None)
+    h = ct.c_void_p()
+    assert L.lib().cubo_hip_reduce_begin(res.handle, ct.byref(req), ct.byref(h)) == 0
+    npos = 16 * 12
+    outs = []
+    for rep in range(2):
+        if rep:
+            assert L.lib().cubo_hip_reduce_reset(h) == 0
+        for r0 in range(0, 16, 5):
+            assert L.lib().cubo_hip_reduce_rows(h, r0, min(5, 16 - r0)) == 0
+        v = L.ReduceView()
+        assert L.lib().cubo_hip_reduce_view(h, ct.byref(v)) == 0
+        got = {k: np.ctypeslib.as_array(getattr(v, k), (n,)).copy() for k, n in
+               (("images", npos), ("mean", 128 * 128), ("max", 128 * 128), ("com_moment_row", npos))}
+        r = {k: np.empty(n) for k, n in (("images", npos), ("mean", 128 * 128), ("max", 128 * 128),
+                                          ("com_moment_row", npos))}
+        rr = L.ReduceResult(**{k: a.ctypes.data for k, a in r.items()})
+        assert L.lib().cubo_hip_reduce_results(h, ct.byref(rr)) == 0
+        for k in r:
+            assert_identical(r[k], got[k], f"view vs results {k}")
+        outs.append(got)
+    L.lib().cubo_hip_reduce_release(h)
+    for k in outs[0]:
+        assert_identical(outs[1][k], outs[0][k], f"reset {k}")
+    ref = C.first_look(res, [C.DetectorGeometry.circle(64, 64, 14)])
+    assert_identical(outs[0]["images"], ref.images[0].values, "images vs first_look")
+    assert_identical(outs[0]["mean"], ref.mean.values, "mean vs first_look")
+
+
+def test_clone_copies_the_cube_on_the_device():
+    shape, dtype, params = synthetic.bandwidth_u16(scan=(6, 5))
+    res = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
+    res.fill_synthetic(params)
+    twin = res.clone()
+    assert twin.byte_count == res.byte_count
+    assert np.array_equal(twin.read_scan_tile(range(0, 6)), res.read_scan_tile(range(0, 6)))
+    odd = C.HIPResidentCube(np.arange(3 * 1 * 3 * 3, dtype=np.float32).reshape(3, 1, 3, 3))   # 108 bytes: a tail
+    assert np.array_equal(odd.clone().read_scan_tile(range(0, 3)), odd.read_scan_tile(range(0, 3)))
