@@ -56,6 +56,55 @@ def cpu_baseline(cube, bf, adf, seconds: float):
                       f".cpu restatement (VI serial, mean+max and CoM on {threads} threads), {el:.1f} s"}
 
 
+def batch_section(C):
+    """48 seeded uneven uint16 cubes with 8..64 masks each: one tuned graph replay of the persistent batch
+    kernel against a loop of single-cube split-FP16 stacks and a loop of plain stacks (each with its results
+    copied to host memory). GB/s are cube bytes per second of wall time."""
+    from cubokit_amd import synthetic
+    items = []
+    for shape, p, k in synthetic.batch_items():
+        c = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=np.uint16)
+        c.fill_synthetic(p)
+        items.append((c, np.ascontiguousarray(synthetic.stack_masks(shape[2], shape[3])[:k])))
+    total = sum(c.byte_count for c, _ in items)
+    log(f"batch: {len(items)} cubes, {total / 2**30:.1f} GiB, {sum(m.shape[0] for _, m in items)} masks")
+    b = C.StackBatch(items)
+    shape = b.tune()
+    ys = b.run()
+    reps = 10
+    b.kernel_time()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run(ys)
+    graph_s = (time.perf_counter() - t0) / reps
+    gms, gn = b.kernel_time()
+    res = {"cubes": len(items), "cube_bytes": total, "masks": int(sum(m.shape[0] for _, m in items)),
+           "tuned": shape, "graph_ms": round(graph_s * 1e3, 3), "graph_device_ms": round(gms / gn, 3),
+           "graph_gbs": round(total / graph_s / 1e9, 1)}
+    for name, method, r in (("mfma_loop", C.MaskStack.MFMA_SPLIT, 5), ("plain_loop", C.MaskStack.PLAIN, 2)):
+        plans = [C.MaskStack(c, m, method) for c, m in items]
+        outs = [p.run() for p in plans]
+        t0 = time.perf_counter()
+        for _ in range(r):
+            for p, y in zip(plans, outs):
+                p.run(y)
+        el = (time.perf_counter() - t0) / r
+        for p in plans:
+            p.release()
+        res[name + "_ms"] = round(el * 1e3, 3)
+        res[name + "_gbs"] = round(total / el / 1e9, 1)
+        if method == C.MaskStack.MFMA_SPLIT:
+            res["identical_to_single_stacks"] = all(np.array_equal(x, y) for x, y in zip(ys, outs))
+    res["speedup_vs_mfma_loop"] = round(res["mfma_loop_ms"] / res["graph_ms"], 2)
+    res["speedup_vs_plain_loop"] = round(res["plain_loop_ms"] / res["graph_ms"], 2)
+    log(f"batch: graph {res['graph_ms']} ms ({res['graph_gbs']} GB/s), single-stack loop {res['mfma_loop_ms']} ms, "
+        f"plain loop {res['plain_loop_ms']} ms; tuned {shape}")
+    b.release()
+    for c, _ in items:
+        c.release()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +115,7 @@ def main() -> int:
     ap.add_argument("--calibration-copy", action="store_true",
                     help="also run one device copy of 2 GiB (read 2 GiB, write 2 GiB) for counter calibration")
     ap.add_argument("--no-stack", action="store_true", help="skip the K = 64 mask-stack comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 48-cube batched-stack comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload runs on one MI355X "
@@ -156,6 +206,8 @@ def main() -> int:
         stack["k"] = int(stack_masks.shape[0])
         stack["mfma_gbs"] = round(cube_bytes / (stack["mfma_split_fp16_ms"] / 1e3) / 1e9, 1)
 
+    batch = None if args.no_batch else batch_section(C)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -174,6 +226,7 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "mask_stack_k64": stack,
+        "batch_48": batch,
     }
     print(json.dumps(line), flush=True)
     return 0

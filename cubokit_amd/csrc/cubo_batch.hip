@@ -1,0 +1,314 @@
+// cubo_batch.hip — C ABI of batched mask stacks (include/cubo_hip.h, "batches").
+//
+// Many resident uint16 cubes of uneven scan and detector size, each with its own stack of up to 64 masks,
+// run in ONE persistent launch: every block takes (cube, tile) items from a work counter until none is
+// left. The counter reset (a one-thread kernel, never a memset), the persistent kernel and the copies of
+// every cube's results into pinned host memory are captured once in a hipGraph and replayed per run.
+//
+// The launch shape (waves per block, resident blocks per compute unit, item order) is chosen by an
+// autotuner that times every candidate on the device and keeps the fastest per batch signature for the
+// life of the process.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <numeric>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cubo_hip.h"
+#include "cube.hpp"
+#include "errors.hpp"
+#include "maskstack_kernels.hpp"
+#include "stack_prep.hpp"
+
+using cubo::DeviceGuard;
+using cubo::fail;
+using cubo::hip_fail;
+
+namespace {
+
+struct Shape {
+    int waves = 8;          // 8 (256-position tiles) or 4 (128-position tiles)
+    int blocks_per_cu = 1;
+    int order = 0;          // 0: given order; 1: largest detector first (longest items first)
+    bool operator<(const Shape &o) const {
+        return std::tie(waves, blocks_per_cu, order) < std::tie(o.waves, o.blocks_per_cu, o.order);
+    }
+};
+
+std::mutex g_tune_mutex;
+std::map<std::string, Shape> g_tuned;    // batch signature -> chosen shape
+
+}  // namespace
+
+struct cubo_hip_batch {
+    int device;
+    int n;
+    std::vector<cubo_hip_cube *> cubes;
+    std::vector<int> k;
+    std::vector<int64_t> npos, ndet;
+    std::vector<void *> wb;              // per cube: device FP16 hi/lo weights
+    std::vector<double *> inv;           // per cube: device 1/scale
+    std::vector<double *> ydev;          // per cube: device results [npos][k]
+    std::vector<size_t> yoff;            // per cube: offset in the pinned result buffer (doubles)
+    std::vector<std::array<int, 2>> tile_lo;
+    double *yhost = nullptr;             // pinned
+    size_t ytotal = 0;
+    cubo::StackLaunch *launches = nullptr;   // device [n], in item order
+    int64_t *first = nullptr;                // device [n]
+    unsigned long long *counter = nullptr;
+    hipStream_t stream = nullptr;
+    Shape shape;
+    int64_t total_tiles = 0;
+    std::string signature;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;
+};
+
+namespace {
+
+void drop_graph(cubo_hip_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void free_batch(cubo_hip_batch *b) {
+    if (!b) return;
+    DeviceGuard g(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (auto &ev : b->timed) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    for (void *p : b->wb) if (p) (void)hipFree(p);
+    for (double *p : b->inv) if (p) (void)hipFree(p);
+    for (double *p : b->ydev) if (p) (void)hipFree(p);
+    if (b->launches) (void)hipFree(b->launches);
+    if (b->first) (void)hipFree(b->first);
+    if (b->counter) (void)hipFree(b->counter);
+    if (b->yhost) (void)hipHostFree(b->yhost);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// Item order and per-cube descriptors for a shape; uploads them and sets total_tiles.
+hipError_t upload_plan(cubo_hip_batch *b, const Shape &s) {
+    std::vector<int> order(b->n);
+    std::iota(order.begin(), order.end(), 0);
+    if (s.order == 1)
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->ndet[x] > b->ndet[y]; });
+    const int64_t bm = 32 * s.waves;
+    std::vector<cubo::StackLaunch> desc(b->n);
+    std::vector<int64_t> first(b->n);
+    int64_t t = 0;
+    for (int j = 0; j < b->n; ++j) {
+        const int i = order[j];
+        cubo::StackLaunch &a = desc[j];
+        a.cube = (const uint16_t *)b->cubes[i]->data;
+        a.npos = b->npos[i];
+        a.ndet = b->ndet[i];
+        a.nchunks = (int)((b->ndet[i] + 63) / 64);
+        a.k = b->k[i];
+        a.k0 = 0;
+        a.kn = b->k[i];
+        a.wb = b->wb[i];
+        a.inv_scale = b->inv[i];
+        a.tile_lo[0] = b->tile_lo[i][0];
+        a.tile_lo[1] = b->tile_lo[i][1];
+        a.y = b->ydev[i];
+        first[j] = t;
+        t += (b->npos[i] + bm - 1) / bm;
+    }
+    b->total_tiles = t;
+    hipError_t e = hipMemcpyAsync(b->launches, desc.data(), desc.size() * sizeof(desc[0]), hipMemcpyHostToDevice,
+                                  b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->first, first.data(), first.size() * sizeof(int64_t), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    return e;
+}
+
+hipError_t launch(cubo_hip_batch *b, const Shape &s) {
+    const int blocks = cubo::compute_units(b->device) * s.blocks_per_cu;
+    return cubo::launch_batch_stack(b->launches, b->first, b->n, b->total_tiles, b->counter, s.waves, blocks,
+                                    b->stream);
+}
+
+int build_graph(cubo_hip_batch *b) {
+    drop_graph(b);
+    if (hipError_t e = upload_plan(b, b->shape); e != hipSuccess) return hip_fail(e, "uploading the batch plan");
+    CUBO_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipError_t e = launch(b, b->shape);
+    for (int i = 0; i < b->n && e == hipSuccess; ++i)
+        e = hipMemcpyAsync(b->yhost + b->yoff[i], b->ydev[i], (size_t)b->npos[i] * b->k[i] * sizeof(double),
+                           hipMemcpyDeviceToHost, b->stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(e != hipSuccess ? e : e2, "capturing the batch graph");
+    }
+    b->graph = graph;
+    CUBO_HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    return CUBO_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cubo_hip_batch_create(int32_t n, cubo_hip_cube *const *cubes, const float *const *masks, const int32_t *k,
+                          cubo_hip_batch **out) {
+    if (!cubes || !masks || !k || !out || n <= 0) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "empty batch");
+    *out = nullptr;
+    const int dev = cubes[0] ? cubes[0]->device : -1;
+    for (int i = 0; i < n; ++i) {
+        const cubo_hip_cube *c = cubes[i];
+        if (!c || !masks[i]) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "batch item %d is NULL", i);
+        if (c->device != dev) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "a batch runs on one device");
+        if (c->dtype != CUBO_HIP_U16 || c->ndet % 8 != 0)
+            return fail(CUBO_HIP_ERR_UNSUPPORTED, "batch item %d: the batched stack needs a uint16 cube whose "
+                        "pattern length is a multiple of 8", i);
+        if (k[i] < 1 || k[i] > 64) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "batch item %d: %d masks (1..64)", i, k[i]);
+        for (int64_t j = 0; j < (int64_t)k[i] * c->ndet; ++j)
+            if (!std::isfinite(masks[i][j])) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "a mask needs finite weights");
+    }
+    auto *b = new (std::nothrow) cubo_hip_batch();
+    if (!b) return fail(CUBO_HIP_ERR_OUT_OF_MEMORY, "host allocation");
+    b->device = dev;
+    b->n = n;
+    DeviceGuard g(dev);
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { free_batch(b); return hip_fail(e, "hipStreamCreate"); }
+    for (int i = 0; i < n; ++i) {
+        cubo_hip_cube *c = cubes[i];
+        b->cubes.push_back(c);
+        b->k.push_back(k[i]);
+        b->npos.push_back(c->npos);
+        b->ndet.push_back(c->ndet);
+        b->signature += std::to_string(c->npos) + "x" + std::to_string(c->ndet) + "x" + std::to_string(k[i]) + ";";
+        cubo::SplitWeights sw = cubo::split_weights(masks[i], k[i], c->ndet, 2);
+        (void)hipStreamSynchronize(c->stream);       // the cube's own uploads are done
+        void *wb = nullptr;
+        double *inv = nullptr, *y = nullptr;
+        e = hipMalloc(&wb, sw.wb.size() * sizeof(_Float16));
+        if (e == hipSuccess) e = hipMemcpy(wb, sw.wb.data(), sw.wb.size() * sizeof(_Float16), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc((void **)&inv, k[i] * sizeof(double));
+        if (e == hipSuccess) e = hipMemcpy(inv, sw.inv.data(), k[i] * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc((void **)&y, (size_t)c->npos * k[i] * sizeof(double));
+        b->wb.push_back(wb);
+        b->inv.push_back(inv);
+        b->ydev.push_back(y);
+        b->tile_lo.push_back({sw.tile_lo[0], sw.tile_lo[1]});
+        b->yoff.push_back(b->ytotal);
+        b->ytotal += (size_t)c->npos * k[i];
+        if (e != hipSuccess) { free_batch(b); return hip_fail(e, "preparing the batch weights"); }
+    }
+    e = hipMalloc((void **)&b->launches, n * sizeof(cubo::StackLaunch));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->first, n * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->counter, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b->yhost, std::max<size_t>(b->ytotal, 1) * sizeof(double),
+                                           hipHostMallocDefault);
+    if (e != hipSuccess) { free_batch(b); return hip_fail(e, "allocating the batch"); }
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tuned.find(b->signature);
+        if (it != g_tuned.end()) b->shape = it->second;
+        else b->shape = Shape{8, cubo::batch_blocks_per_cu(8), 0};
+    }
+    if (int st = build_graph(b)) { free_batch(b); return st; }
+    *out = b;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_batch_tune(cubo_hip_batch *b, int32_t *waves, int32_t *blocks_per_cu, int32_t *order) {
+    if (!b) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL batch");
+    DeviceGuard g(b->device);
+    std::vector<Shape> cand;
+    for (int w : {8, 4})
+        for (int bpc = 1; bpc <= cubo::batch_blocks_per_cu(w); ++bpc)
+            for (int o : {0, 1}) cand.push_back(Shape{w, bpc, o});
+    hipEvent_t t0, t1;
+    CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
+    CUBO_HIP_TRY(hipEventCreate(&t1), "hipEventCreate");
+    float best = 1e30f;
+    Shape best_s = b->shape;
+    int status = CUBO_HIP_OK;
+    for (const Shape &s : cand) {
+        hipError_t e = upload_plan(b, s);
+        if (e == hipSuccess) e = launch(b, s);                  // warm-up
+        float total = 0;
+        for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+            e = hipEventRecord(t0, b->stream);
+            if (e == hipSuccess) e = launch(b, s);
+            if (e == hipSuccess) e = hipEventRecord(t1, b->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(t1);
+            float ms = 0;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+            total += ms;
+        }
+        if (e != hipSuccess) { status = hip_fail(e, "timing a batch shape"); break; }
+        if (total < best) { best = total; best_s = s; }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (status) return status;
+    b->shape = best_s;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tuned[b->signature] = best_s;
+    }
+    if (int st = build_graph(b)) return st;
+    if (waves) *waves = best_s.waves;
+    if (blocks_per_cu) *blocks_per_cu = best_s.blocks_per_cu;
+    if (order) *order = best_s.order;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_batch_run(cubo_hip_batch *b, double *const *y) {
+    if (!b) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL batch");
+    DeviceGuard g(b->device);
+    hipEvent_t t0, t1;
+    CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
+    CUBO_HIP_TRY(hipEventCreate(&t1), "hipEventCreate");
+    CUBO_HIP_TRY(hipEventRecord(t0, b->stream), "hipEventRecord");
+    CUBO_HIP_TRY(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
+    CUBO_HIP_TRY(hipEventRecord(t1, b->stream), "hipEventRecord");
+    b->timed.emplace_back(t0, t1);
+    CUBO_HIP_TRY(hipStreamSynchronize(b->stream), "waiting for the batch");
+    if (y)
+        for (int i = 0; i < b->n; ++i)
+            if (y[i]) memcpy(y[i], b->yhost + b->yoff[i], (size_t)b->npos[i] * b->k[i] * sizeof(double));
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_batch_time(cubo_hip_batch *b, double *total_ms, int64_t *runs) {
+    if (!b || !total_ms || !runs) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    DeviceGuard g(b->device);
+    double t = 0;
+    for (auto &ev : b->timed) {
+        float ms = 0;
+        CUBO_HIP_TRY(hipEventElapsedTime(&ms, ev.first, ev.second), "hipEventElapsedTime");
+        t += ms;
+    }
+    *total_ms = t;
+    *runs = (int64_t)b->timed.size();
+    for (auto &ev : b->timed) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    b->timed.clear();
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_batch_destroy(cubo_hip_batch *b) {
+    free_batch(b);
+    return CUBO_HIP_OK;
+}
+
+}  // extern "C"

@@ -15,6 +15,7 @@
 #include "cube.hpp"
 #include "errors.hpp"
 #include "maskstack_kernels.hpp"
+#include "stack_prep.hpp"
 
 using cubo::DeviceGuard;
 using cubo::fail;
@@ -85,43 +86,13 @@ int cubo_hip_stack_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
     if (e != hipSuccess) { free_plan(p); return hip_fail(e, "allocating the stack output"); }
 
     if (method == CUBO_HIP_STACK_MFMA_SPLIT) {
-        p->ntiles = k <= 32 ? 1 : 2;
-        p->passes = (k + 63) / 64;
-        p->nchunks = (int)((ndet + 63) / 64);
-        // Column scale 2^(10 - floor(log2 max|w|)) puts every column's largest weight in [1024, 2048).
-        std::vector<double> scale(k, 1.0), inv(k, 1.0);
-        for (int m = 0; m < k; ++m) {
-            double mx = 0;
-            for (int64_t i = 0; i < ndet; ++i) mx = std::max(mx, (double)std::fabs(masks[(size_t)m * ndet + i]));
-            if (mx > 0) {
-                const int ex = (int)std::floor(std::log2(mx));
-                scale[m] = std::ldexp(1.0, 10 - ex);
-                inv[m] = std::ldexp(1.0, ex - 10);
-            }
-        }
-        // B operand order: [chunk][tile][kstep][hi, lo][lane][8 halves]; lane l holds mask 32*tile + l%32 and
-        // pixels 64*chunk + 32*(l/32) + 8*kstep + e, the same pixels the A operand's lane holds for its row.
-        const size_t per_chunk = (size_t)p->ntiles * 4 * 2 * 64 * 8;
-        const size_t per_pass = per_chunk * p->nchunks;
-        std::vector<_Float16> wb(per_pass * p->passes);
-        for (int pass = 0; pass < p->passes; ++pass)
-        for (int ch = 0; ch < p->nchunks; ++ch)
-            for (int t = 0; t < p->ntiles; ++t)
-                for (int s = 0; s < 4; ++s)
-                    for (int l = 0; l < 64; ++l)
-                        for (int el = 0; el < 8; ++el) {
-                            const int m = pass * 64 + t * 32 + (l & 31);
-                            const int64_t px = (int64_t)ch * 64 + 32 * (l >> 5) + 8 * s + el;
-                            double v = 0;
-                            if (m < k && px < ndet) v = (double)masks[(size_t)m * ndet + px] * scale[m];
-                            const _Float16 hi = (_Float16)(float)v;
-                            const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
-                            const size_t base =
-                                pass * per_pass + (((((size_t)ch * p->ntiles + t) * 4 + s) * 2) * 64 + l) * 8 + el;
-                            wb[base] = hi;
-                            wb[base + 64 * 8] = lo;
-                            if ((float)lo != 0.f) p->tile_lo[pass * 2 + t] = 1;
-                        }
+        cubo::SplitWeights sw = cubo::split_weights(masks, k, ndet);
+        p->ntiles = sw.ntiles;
+        p->passes = sw.passes;
+        p->nchunks = sw.nchunks;
+        for (size_t t = 0; t < sw.tile_lo.size() && t < 8; ++t) p->tile_lo[t] = sw.tile_lo[t];
+        const std::vector<_Float16> &wb = sw.wb;
+        const std::vector<double> &inv = sw.inv;
         e = hipMalloc(&p->w, wb.size() * sizeof(_Float16));
         if (e == hipSuccess) e = hipMemcpy(p->w, wb.data(), wb.size() * sizeof(_Float16), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc((void **)&p->inv_scale, k * sizeof(double));

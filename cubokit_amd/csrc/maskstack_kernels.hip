@@ -33,18 +33,22 @@ __device__ __forceinline__ uint32_t cvt_pair(uint32_t w) {
     return __builtin_bit_cast(uint32_t, h);
 }
 
-// Block: 8 waves x 32 positions. Each lane loads its A operand straight from HBM (16 bytes: 8 pixels of
-// one position per k-step), two 64-pixel chunks ahead of the matrix cores; the block shares one LDS copy
-// of the weight chunk (double-buffered, one barrier per chunk).
-constexpr int kStackWaves = 8;
-constexpr int kStackBM = 32 * kStackWaves;
-
+// One tile: WAVES waves x 32 positions starting at p0. Each lane loads its A operand straight from HBM
+// (16 bytes: 8 pixels of one position per k-step), two 64-pixel chunks ahead of the matrix cores; the
+// block shares one LDS copy of the weight chunk (double-buffered, one barrier per chunk). Every thread of
+// the block calls it; it ends after a barrier, so the block may start another tile at once.
 template <int NT>
-__global__ __launch_bounds__(64 * kStackWaves) void mask_stack_mfma(StackLaunch a) {
-    constexpr int WCH = NT * 4 * 2 * 64;                 // half8 entries of one weight chunk
-    __shared__ half8 wlds[2][WCH];
+struct StackLds {
+    static constexpr int WCH = NT * 4 * 2 * 64;          // half8 entries of one weight chunk
+    half8 w[2][WCH];
+};
+
+template <int NT, int WAVES>
+__device__ __forceinline__ void stack_tile(const StackLaunch &a, int64_t p0, StackLds<NT> &sh) {
+    constexpr int WCH = StackLds<NT>::WCH;
+    auto &wlds = sh.w;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t prow = (int64_t)blockIdx.x * kStackBM + wave * 32 + (lane & 31);
+    const int64_t prow = p0 + wave * 32 + (lane & 31);
     const bool live = prow < a.npos;
     const int nchunks = a.nchunks;
     const half8 *wsrc = reinterpret_cast<const half8 *>(a.wb);
@@ -63,19 +67,19 @@ __global__ __launch_bounds__(64 * kStackWaves) void mask_stack_mfma(StackLaunch 
     };
     // The block's weight chunk: each thread carries WPT 16-byte entries from HBM (or L2) to LDS. They are
     // loaded before the data prefetch of the same step, so waiting for them never waits for that prefetch.
-    constexpr int WPT = (WCH + 64 * kStackWaves - 1) / (64 * kStackWaves);
+    constexpr int WPT = (WCH + 64 * WAVES - 1) / (64 * WAVES);
     half8 wreg[WPT];
     auto fetch_w = [&](int chunk) {
 #pragma unroll
         for (int j = 0; j < WPT; ++j) {
-            const int e = tid + j * 64 * kStackWaves;
+            const int e = tid + j * 64 * WAVES;
             if (e < WCH) wreg[j] = wsrc[(size_t)chunk * WCH + e];
         }
     };
     auto put_w = [&](int buf) {
 #pragma unroll
         for (int j = 0; j < WPT; ++j) {
-            const int e = tid + j * 64 * kStackWaves;
+            const int e = tid + j * 64 * WAVES;
             if (e < WCH) wlds[buf][e] = wreg[j];
         }
     };
@@ -103,6 +107,7 @@ __global__ __launch_bounds__(64 * kStackWaves) void mask_stack_mfma(StackLaunch 
             const half8 alo = __builtin_bit_cast(half8, lo);
 #pragma unroll
             for (int n = 0; n < NT; ++n) {
+                if (n * 32 >= a.kn) continue;                  // a tile with no mask of this stack
                 const half8 bh = wl[((n * 4 + s) * 2 + 0) * 64 + lane];
                 acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bh, acc[n], 0, 0, 0);
                 if (a.tile_lo[n]) {
@@ -116,6 +121,7 @@ __global__ __launch_bounds__(64 * kStackWaves) void mask_stack_mfma(StackLaunch 
                 const half8 ahi = __builtin_bit_cast(half8, hi);
 #pragma unroll
                 for (int n = 0; n < NT; ++n) {
+                    if (n * 32 >= a.kn) continue;
                     const half8 bh = wl[((n * 4 + s) * 2 + 0) * 64 + lane];
                     acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bh, acc[n], 0, 0, 0);
                     if (a.tile_lo[n]) {
@@ -160,7 +166,7 @@ __global__ __launch_bounds__(64 * kStackWaves) void mask_stack_mfma(StackLaunch 
     if (c + 1 < nchunks) step(X1, X0, c + 1);
 
     // 32x32 accumulator layout: register r of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
-    const int64_t pbase = (int64_t)blockIdx.x * kStackBM + wave * 32;
+    const int64_t pbase = p0 + wave * 32;
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
         const int col = n * 32 + (lane & 31);
@@ -173,6 +179,36 @@ __global__ __launch_bounds__(64 * kStackWaves) void mask_stack_mfma(StackLaunch 
         }
     }
 }
+
+template <int NT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void mask_stack_mfma(StackLaunch a) {
+    __shared__ StackLds<NT> sh;
+    stack_tile<NT, WAVES>(a, (int64_t)blockIdx.x * 32 * WAVES, sh);
+}
+
+// Persistent batch: every block takes tiles from one work counter until all cubes are done. A tile is
+// (cube, 32 * WAVES positions); first[c] is cube c's first tile number.
+template <int NT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void batch_stack_mfma(const StackLaunch *cubes, const int64_t *first, int ncubes,
+                                                              int64_t total, unsigned long long *counter) {
+    __shared__ StackLds<NT> sh;
+    __shared__ int64_t item_s;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = (int64_t)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const int64_t item = item_s;
+        __syncthreads();                                   // every thread has read item_s
+        if (item >= total) break;                          // block-uniform: every block leaves
+        int lo = 0, hi = ncubes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (first[mid] <= item) lo = mid; else hi = mid - 1;
+        }
+        stack_tile<NT, WAVES>(cubes[lo], (item - first[lo]) * 32 * WAVES, sh);
+    }
+}
+
+__global__ void reset_counter(unsigned long long *c) { *c = 0ull; }
 
 // ------------------------------------------------------------------------------------------------
 // Plain VALU path: a lane per position.
@@ -276,12 +312,32 @@ __global__ __launch_bounds__(256) void mask_stack_plain(PlainStackLaunch a) {
 }
 
 hipError_t launch_mask_stack_mfma(const StackLaunch &a, int nt, hipStream_t s) {
-    const dim3 grid((unsigned)((a.npos + kStackBM - 1) / kStackBM)), block(64 * kStackWaves);
+    constexpr int W = 8;
+    const dim3 grid((unsigned)((a.npos + 32 * W - 1) / (32 * W))), block(64 * W);
     switch (nt) {
-        case 1: hipLaunchKernelGGL(mask_stack_mfma<1>, grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL(mask_stack_mfma<2>, grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((mask_stack_mfma<1, W>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((mask_stack_mfma<2, W>), grid, block, 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+int batch_blocks_per_cu(int waves) {
+    int n = 0;
+    hipError_t e = waves == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_stack_mfma<2, 8>, 512, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_stack_mfma<2, 4>, 256, 0);
+    return e == hipSuccess && n > 0 ? n : 1;
+}
+
+hipError_t launch_batch_stack(const StackLaunch *cubes, const int64_t *first, int ncubes, int64_t total,
+                              unsigned long long *counter, int waves, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, counter);
+    if (waves == 8)
+        hipLaunchKernelGGL((batch_stack_mfma<2, 8>), dim3(blocks), dim3(512), 0, s, cubes, first, ncubes, total, counter);
+    else if (waves == 4)
+        hipLaunchKernelGGL((batch_stack_mfma<2, 4>), dim3(blocks), dim3(256), 0, s, cubes, first, ncubes, total, counter);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

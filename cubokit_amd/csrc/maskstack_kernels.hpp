@@ -30,6 +30,11 @@ struct PlainStackLaunch {
 };
 
 hipError_t launch_mask_stack_mfma(const StackLaunch &a, int ntiles, hipStream_t s);
+// Persistent batch of uint16 cubes (at most 64 masks each): resets the work counter with a kernel, then
+// launches `blocks` blocks of `waves` waves that take (cube, tile) items until none is left.
+int batch_blocks_per_cu(int waves);
+hipError_t launch_batch_stack(const StackLaunch *cubes, const int64_t *first, int ncubes, int64_t total,
+                              unsigned long long *counter, int waves, int blocks, hipStream_t s);
 hipError_t launch_mask_stack_plain(const PlainStackLaunch &a, bool is_u16, int kpad, hipStream_t s);
 
 }  // namespace cubo

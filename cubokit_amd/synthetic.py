@@ -75,3 +75,28 @@ def stack_masks(qy: int, qx: int, seed: int = 20261103, n_sectors: int = 29, n_d
     for _ in range(n_dense):
         rows.append(rng.uniform(0, 1, qy * qx))
     return np.ascontiguousarray(np.stack(rows).astype(np.float32))
+
+
+BATCH_SCANS = [(32, 32), (64, 64), (96, 128), (128, 128), (256, 256)]
+BATCH_DETECTORS = [64, 96, 128, 256]
+BATCH_CUBE_LIMIT = 2 << 30
+
+
+def batch_items(n: int = 48, seed: int = 20261104):
+    """A seeded screening batch of n uneven uint16 cubes: scan in {32x32, 64x64, 96x128, 128x128, 256x256},
+    square detectors of side {64, 96, 128, 256}, cubes of at most 2 GiB, 8..64 masks each (the first k rows of
+    stack_masks for that detector). Returns [(shape, SynthParams, k)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        scan = BATCH_SCANS[rng.integers(len(BATCH_SCANS))]
+        det = BATCH_DETECTORS[rng.integers(len(BATCH_DETECTORS))]
+        shape = (scan[0], scan[1], det, det)
+        if 2 * scan[0] * scan[1] * det * det > BATCH_CUBE_LIMIT:
+            continue
+        k = int(rng.integers(8, 65))
+        p = params(int(rng.integers(1 << 62)), (det / 2, det / 2), det / 10, float(rng.uniform(100, 3000)),
+                   edge=0.5, background=2.0, rings=[(det / 4, 20.0, 1.5)],
+                   waves=random_waves(rng, 2, 2.0, scan))
+        out.append((shape, p, k))
+    return out

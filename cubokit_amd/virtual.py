@@ -643,6 +643,64 @@ class MaskStack:
             pass
 
 
+class StackBatch:
+    """Mask stacks over many resident uint16 cubes in one persistent launch (include/cubo_hip.h, "batches").
+
+    items: (HIPResidentCube, masks (k, detectorCount)) pairs, k in 1..64, all cubes on one device. The
+    launch is captured once in a hipGraph; `run` replays it. `tune` times the launch shapes on the device
+    and keeps the fastest for every batch of the same shapes in this process."""
+
+    def __init__(self, items: Sequence[tuple]):
+        if not items:
+            raise CuboError("an empty batch")
+        self.cubes = [c for c, _ in items]
+        self.masks = [np.ascontiguousarray(m, np.float32).reshape(-1, c.shape.detector_count) for c, m in items]
+        self.k = [m.shape[0] for m in self.masks]
+        n = len(items)
+        cubes = (ct.c_void_p * n)(*[c.handle.value if isinstance(c.handle, ct.c_void_p) else c.handle
+                                     for c in self.cubes])
+        masks = (ct.c_void_p * n)(*[m.ctypes.data for m in self.masks])
+        ks = (ct.c_int32 * n)(*self.k)
+        h = ct.c_void_p()
+        _check(L.lib().cubo_hip_batch_create(n, cubes, masks, ks, ct.byref(h)), "preparing the batch")
+        self._h = h
+        self.shape = None
+
+    def tune(self) -> dict:
+        w, b, o = ct.c_int32(), ct.c_int32(), ct.c_int32()
+        _check(L.lib().cubo_hip_batch_tune(self._h, ct.byref(w), ct.byref(b), ct.byref(o)), "tuning the batch")
+        self.shape = {"waves": w.value, "blocks_per_cu": b.value,
+                      "order": "largest detector first" if o.value else "given"}
+        return self.shape
+
+    def run(self, out: Optional[Sequence[np.ndarray]] = None, copy_out: bool = True) -> list:
+        """[(scanCount_i, k_i) Double] per item; copy_out=False replays the graph without the host copy."""
+        if not copy_out:
+            _check(L.lib().cubo_hip_batch_run(self._h, None), "running the batch")
+            return []
+        ys = list(out) if out is not None else [np.empty((c.shape.scan_count, k)) for c, k in zip(self.cubes, self.k)]
+        ptrs = (ct.c_void_p * len(ys))(*[y.ctypes.data for y in ys])
+        _check(L.lib().cubo_hip_batch_run(self._h, ptrs), "running the batch")
+        return ys
+
+    def kernel_time(self):
+        """(total device milliseconds, replays) since the previous call."""
+        t, n = ct.c_double(), ct.c_int64()
+        _check(L.lib().cubo_hip_batch_time(self._h, ct.byref(t), ct.byref(n)), "batch time")
+        return t.value, n.value
+
+    def release(self):
+        if getattr(self, "_h", None) is not None:
+            L.lib().cubo_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
 def center_of_mass_from_stack(cube_shape: CubeShape, y: np.ndarray, columns=(0, 1, 2)) -> CenterOfMassMaps:
     """The centre of mass from the stack's {w, w*q_row, w*q_col} columns; the division in FP64."""
     s, mr, mc = (np.ascontiguousarray(y[:, c], np.float64) for c in columns)

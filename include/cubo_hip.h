@@ -171,6 +171,26 @@ int cubo_hip_stack_plan_run(cubo_hip_stack_plan *plan, double *y);
 int cubo_hip_stack_plan_destroy(cubo_hip_stack_plan *plan);
 int cubo_hip_mask_stack(cubo_hip_cube *cube, const float *masks, int32_t k, int32_t method, double *y);
 
+/* ---- batches: mask stacks over many resident uint16 cubes of uneven scan and detector size in ONE
+ * persistent launch, replayed from a hipGraph. No reference counterpart: the reference reduces one cube
+ * per call (CuboVirtual/VirtualImaging.swift:13-30); a session that screens many acquisitions calls it
+ * in a loop. Each item: a uint16 cube whose pattern length is a multiple of 8, 1..64 masks (FP32,
+ * k x detectorCount row-major), results as in cubo_hip_mask_stack (split-FP16 matrix-core method). All
+ * cubes on one device; they must stay alive and unchanged in shape while the batch exists. ---- */
+typedef struct cubo_hip_batch cubo_hip_batch;
+int cubo_hip_batch_create(int32_t n, cubo_hip_cube *const *cubes, const float *const *masks, const int32_t *k,
+                          cubo_hip_batch **out);
+/* Times every launch shape (waves per block 8 or 4, resident blocks per compute unit, items in the given
+ * order or largest detector first) on the device, keeps the fastest for this batch and for every later
+ * batch of the same shapes in this process, and rebuilds the graph. Any output pointer may be NULL. */
+int cubo_hip_batch_tune(cubo_hip_batch *batch, int32_t *waves, int32_t *blocks_per_cu, int32_t *order);
+/* One graph replay: counter reset, the persistent launch, copies of every result. y[i]: scanCount_i x k_i
+ * Doubles, caller-owned host memory (NULL entries or y == NULL skip the copy out). */
+int cubo_hip_batch_run(cubo_hip_batch *batch, double *const *y);
+/* Device time of the graph replays since the previous query: total milliseconds and replay count. */
+int cubo_hip_batch_time(cubo_hip_batch *batch, double *total_ms, int64_t *runs);
+int cubo_hip_batch_destroy(cubo_hip_batch *batch);
+
 /* ---- synthetic cubes for benchmarks and tests (no reference counterpart: the reference's bench
  * cubes are generated in Python, tools/benchmarks/bench.py:17-27). A disk probe with a sigmoid edge,
  * Gaussian rings, a per-position deflection and Poisson counts, generated on the device from a
