@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -127,6 +128,7 @@ hipError_t upload_plan(cubo_hip_batch *b, const Shape &s) {
         first[j] = t;
         t += (b->npos[i] + bm - 1) / bm;
     }
+    if (t >= INT32_MAX) return hipErrorInvalidValue;      // the kernel numbers items in 32 bits
     b->total_tiles = t;
     hipError_t e = hipMemcpyAsync(b->launches, desc.data(), desc.size() * sizeof(desc[0]), hipMemcpyHostToDevice,
                                   b->stream);

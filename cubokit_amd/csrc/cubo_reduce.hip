@@ -137,7 +137,11 @@ int choose_path(const cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_h
     if (pnw < 0) return CUBO_HIP_OK;
     // Largest chunk count whose per-lane uint32 partials cannot overflow: 8*chunks * 65535 * weight.
     const double wbound = std::max({maxw, comw, 1.0});
+    // CUBO_HIP_MAX_CHUNKS caps the chunk count (a tuning knob: fewer chunks, fewer registers per lane).
+    int max_chunks = 4;
+    if (const char *env = getenv("CUBO_HIP_MAX_CHUNKS")) max_chunks = std::max(1, atoi(env));
     for (int ch : {4, 2, 1}) {
+        if (ch > max_chunks) continue;
         if ((double)ch * 8.0 * 65535.0 * wbound >= 4294967296.0) continue;
         if (!cubo::supported_int(pnw, ch, com_ones)) continue;
         r->path = 1;

@@ -12,11 +12,16 @@
 //    holds a count of 2048 or more, the lo weight part only for 32-mask tiles that have one.
 //  - plain VALU (uint16 or Float32 cubes): one position per lane, FP32 FMAs over 64-pixel chunks staged
 //    in LDS with their weights (read as LDS broadcasts), FP64 across chunks.
+#include "device_sync.hpp"
 #include "maskstack_kernels.hpp"
 
 namespace cubo {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+// Global-memory views of pointers that were read from memory (the batch descriptors): without the
+// address space the compiler emits flat loads, whose completion it can only track together with LDS
+// traffic, and every wait then drains both.
+#define CUBO_GLOBAL __attribute__((address_space(1)))
 typedef float float16v __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short us2 __attribute__((ext_vector_type(2)));
@@ -51,37 +56,36 @@ __device__ __forceinline__ void stack_tile(const StackLaunch &a, int64_t p0, Sta
     const int64_t prow = p0 + wave * 32 + (lane & 31);
     const bool live = prow < a.npos;
     const int nchunks = a.nchunks;
-    const half8 *wsrc = reinterpret_cast<const half8 *>(a.wb);
+    const CUBO_GLOBAL half8 *wsrc = (const CUBO_GLOBAL half8 *)a.wb;
     // Within a 64-pixel chunk, k-step s of lane half h holds pixels 32h + 8s + (0..7): each lane reads 64
     // contiguous bytes over the four k-steps, the two halves one full 128-byte line per position. Plain
     // (cached) loads keep the line in L2 between the four k-step loads.
-    const uint16_t *xrow = a.cube + (live ? prow : 0) * a.ndet + 32 * (lane >> 5);
+    // Every load is unconditional (loads under a lane mask make the compiler wait for all loads in flight,
+    // which drains the prefetch): a position past the cube reads the first row and a pixel past the pattern
+    // reads its row's start. Neither needs masking: the weights of pixels past the pattern are zero and
+    // the split parts of any uint16 are finite in FP16, so they add exact zeros, and the rows of positions
+    // past the cube are never written.
+    const CUBO_GLOBAL uint16_t *xrow = (const CUBO_GLOBAL uint16_t *)a.cube + (live ? prow : 0) * a.ndet;
 
     auto load_x = [&](u32x4 (&x)[4], int chunk) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int64_t px = (int64_t)chunk * kStackBK + 32 * (lane >> 5) + 8 * s;
-            x[s] = (live && px < a.ndet) ? *reinterpret_cast<const u32x4 *>(xrow + (int64_t)chunk * kStackBK + 8 * s)
-                                         : u32x4{0u, 0u, 0u, 0u};
+            x[s] = *(const CUBO_GLOBAL u32x4 *)(xrow + (px < a.ndet ? px : 0));
         }
     };
     // The block's weight chunk: each thread carries WPT 16-byte entries from HBM (or L2) to LDS. They are
     // loaded before the data prefetch of the same step, so waiting for them never waits for that prefetch.
     constexpr int WPT = (WCH + 64 * WAVES - 1) / (64 * WAVES);
     half8 wreg[WPT];
+    static_assert(WCH % (64 * WAVES) == 0, "every thread carries the same number of weight entries");
     auto fetch_w = [&](int chunk) {
 #pragma unroll
-        for (int j = 0; j < WPT; ++j) {
-            const int e = tid + j * 64 * WAVES;
-            if (e < WCH) wreg[j] = wsrc[(size_t)chunk * WCH + e];
-        }
+        for (int j = 0; j < WPT; ++j) wreg[j] = wsrc[(size_t)chunk * WCH + tid + j * 64 * WAVES];
     };
     auto put_w = [&](int buf) {
 #pragma unroll
-        for (int j = 0; j < WPT; ++j) {
-            const int e = tid + j * 64 * WAVES;
-            if (e < WCH) wlds[buf][e] = wreg[j];
-        }
+        for (int j = 0; j < WPT; ++j) wlds[buf][tid + j * 64 * WAVES] = wreg[j];
     };
 
     float16v acc[NT];
@@ -92,6 +96,10 @@ __device__ __forceinline__ void stack_tile(const StackLaunch &a, int64_t p0, Sta
         for (int r = 0; r < 16; ++r) { acc[n][r] = 0.f; accd[n][r] = 0.0; }
     }
 
+    // Both tiles' weight fragments of a k-step are read from LDS before its MFMAs, so the reads overlap;
+    // a tile with no mask of this stack multiplies zero weights (exact zeros), and the lo parts are
+    // multiplied when any tile has one (any_lo is uniform over the launch).
+    const bool any_lo = (a.tile_lo[0] | (NT > 1 ? a.tile_lo[1] : 0)) != 0;
     auto compute = [&](const u32x4 (&x)[4], int chunk) {
         uint32_t any = 0;
 #pragma unroll
@@ -100,34 +108,33 @@ __device__ __forceinline__ void stack_tile(const StackLaunch &a, int64_t p0, Sta
         const half8 *wl = wlds[chunk & 1];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
+            half8 bh[NT], bl[NT];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                bh[n] = wl[((n * 4 + s) * 2 + 0) * 64 + lane];
+                bl[n] = wl[((n * 4 + s) * 2 + 1) * 64 + lane];
+            }
             const uint32_t w[4] = {x[s].x, x[s].y, x[s].z, x[s].w};
             u32x4 lo, hi;
             lo.x = cvt_pair(w[0] & 0x07ff07ffu); lo.y = cvt_pair(w[1] & 0x07ff07ffu);
             lo.z = cvt_pair(w[2] & 0x07ff07ffu); lo.w = cvt_pair(w[3] & 0x07ff07ffu);
             const half8 alo = __builtin_bit_cast(half8, lo);
 #pragma unroll
-            for (int n = 0; n < NT; ++n) {
-                if (n * 32 >= a.kn) continue;                  // a tile with no mask of this stack
-                const half8 bh = wl[((n * 4 + s) * 2 + 0) * 64 + lane];
-                acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bh, acc[n], 0, 0, 0);
-                if (a.tile_lo[n]) {
-                    const half8 bl = wl[((n * 4 + s) * 2 + 1) * 64 + lane];
-                    acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bl, acc[n], 0, 0, 0);
-                }
+            for (int n = 0; n < NT; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bh[n], acc[n], 0, 0, 0);
+            if (any_lo) {
+#pragma unroll
+                for (int n = 0; n < NT; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bl[n], acc[n], 0, 0, 0);
             }
             if (use_hi) {
                 hi.x = cvt_pair(w[0] & 0xf800f800u); hi.y = cvt_pair(w[1] & 0xf800f800u);
                 hi.z = cvt_pair(w[2] & 0xf800f800u); hi.w = cvt_pair(w[3] & 0xf800f800u);
                 const half8 ahi = __builtin_bit_cast(half8, hi);
 #pragma unroll
-                for (int n = 0; n < NT; ++n) {
-                    if (n * 32 >= a.kn) continue;
-                    const half8 bh = wl[((n * 4 + s) * 2 + 0) * 64 + lane];
-                    acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bh, acc[n], 0, 0, 0);
-                    if (a.tile_lo[n]) {
-                        const half8 bl = wl[((n * 4 + s) * 2 + 1) * 64 + lane];
-                        acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bl, acc[n], 0, 0, 0);
-                    }
+                for (int n = 0; n < NT; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bh[n], acc[n], 0, 0, 0);
+                if (any_lo) {
+#pragma unroll
+                    for (int n = 0; n < NT; ++n)
+                        acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bl[n], acc[n], 0, 0, 0);
                 }
             }
         }
@@ -145,16 +152,21 @@ __device__ __forceinline__ void stack_tile(const StackLaunch &a, int64_t p0, Sta
     // other buffer (read last by compute(c - 1), which every wave finished before the previous barrier).
     u32x4 X0[4], X1[4], X2[4];
     fetch_w(0);
+    __builtin_amdgcn_sched_barrier(0);
     load_x(X0, 0);
-    if (nchunks > 1) load_x(X1, 1);
+    load_x(X1, min(1, nchunks - 1));
     put_w(0);
-    __syncthreads();
+    lds_barrier();
+    // The loads are unconditional (past the last chunk they repeat it, and the extra weight copy lands in
+    // the buffer no wave reads any more), and the weight loads stay ahead of the data loads in issue order:
+    // the wait for the weights before put_w then leaves the data prefetch in flight.
     auto step = [&](const u32x4 (&xc)[4], u32x4 (&xn)[4], int c) {
-        if (c + 1 < nchunks) fetch_w(c + 1);
-        if (c + 2 < nchunks) load_x(xn, c + 2);
+        fetch_w(min(c + 1, nchunks - 1));
+        __builtin_amdgcn_sched_barrier(0);
+        load_x(xn, min(c + 2, nchunks - 1));
         compute(xc, c);
-        if (c + 1 < nchunks) put_w((c + 1) & 1);
-        __syncthreads();
+        put_w((c + 1) & 1);
+        lds_barrier();
     };
     int c = 0;
     for (; c + 3 <= nchunks; c += 3) {
@@ -171,11 +183,11 @@ __device__ __forceinline__ void stack_tile(const StackLaunch &a, int64_t p0, Sta
     for (int n = 0; n < NT; ++n) {
         const int col = n * 32 + (lane & 31);
         if (col >= a.kn) continue;
-        const double inv = a.inv_scale[col];
+        const double inv = ((const CUBO_GLOBAL double *)a.inv_scale)[col];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int64_t p = pbase + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (p < a.npos) a.y[p * a.k + a.k0 + col] = accd[n][r] * inv;
+            if (p < a.npos) ((CUBO_GLOBAL double *)a.y)[p * a.k + a.k0 + col] = accd[n][r] * inv;
         }
     }
 }
@@ -189,14 +201,16 @@ __global__ __launch_bounds__(64 * WAVES) void mask_stack_mfma(StackLaunch a) {
 // Persistent batch: every block takes tiles from one work counter until all cubes are done. A tile is
 // (cube, 32 * WAVES positions); first[c] is cube c's first tile number.
 template <int NT, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void batch_stack_mfma(const StackLaunch *cubes, const int64_t *first, int ncubes,
+__global__ __launch_bounds__(64 * WAVES) void batch_stack_mfma(const StackLaunch *__restrict__ cubes,
+                                                              const int64_t *__restrict__ first, int ncubes,
                                                               int64_t total, unsigned long long *counter) {
     __shared__ StackLds<NT> sh;
-    __shared__ int64_t item_s;
+    __shared__ int item_s;
     for (;;) {
-        if (threadIdx.x == 0) item_s = (int64_t)atomicAdd(counter, 1ull);
+        if (threadIdx.x == 0) item_s = (int)atomicAdd(counter, 1ull);     // total < 2^31 (checked on the host)
         __syncthreads();
-        const int64_t item = item_s;
+        // uniform from here: the item, the cube it falls in and that cube's descriptor live in scalar registers
+        const int item = __builtin_amdgcn_readfirstlane(item_s);
         __syncthreads();                                   // every thread has read item_s
         if (item >= total) break;                          // block-uniform: every block leaves
         int lo = 0, hi = ncubes - 1;
@@ -204,7 +218,8 @@ __global__ __launch_bounds__(64 * WAVES) void batch_stack_mfma(const StackLaunch
             const int mid = (lo + hi + 1) >> 1;
             if (first[mid] <= item) lo = mid; else hi = mid - 1;
         }
-        stack_tile<NT, WAVES>(cubes[lo], (item - first[lo]) * 32 * WAVES, sh);
+        const StackLaunch a = cubes[lo];
+        stack_tile<NT, WAVES>(a, (item - first[lo]) * 32 * WAVES, sh);
     }
 }
 
@@ -284,7 +299,7 @@ __global__ __launch_bounds__(256) void mask_stack_plain(PlainStackLaunch a) {
     fetch(0);
     for (int ch = 0; ch < nchunks; ++ch) {
         put();                                                   // chunk ch: registers -> LDS
-        __syncthreads();
+        lds_barrier();
         if (ch + 1 < nchunks) fetch((int64_t)(ch + 1) * kPlainBK);   // chunk ch + 1 in flight meanwhile
 #pragma unroll 2
         for (int c = 0; c < kPlainBK; ++c) {
@@ -301,7 +316,7 @@ __global__ __launch_bounds__(256) void mask_stack_plain(PlainStackLaunch a) {
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) { accd[k] += (double)acc[k]; acc[k] = 0.f; }
-        __syncthreads();                                         // every lane is done with the chunk in LDS
+        lds_barrier();                                           // every lane is done with the chunk in LDS
     }
     const int64_t p = p0 + tid;
     if (p < a.npos) {

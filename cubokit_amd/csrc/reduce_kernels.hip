@@ -89,6 +89,11 @@ struct RowBatch {
 // Exact integer kernel: uint16 cube, NW weight rows of uint16 (packed in pairs), optional arithmetic
 // centre-of-mass rows (w = 1: S, sum i*x, sum j*x) when the detector rows are multiples of 8 pixels.
 // ------------------------------------------------------------------------------------------------
+// Position buffers per wave of fused_u16_int (NB - 1 positions in flight).
+#ifndef CUBO_FUSED_NB
+#define CUBO_FUSED_NB(CH) (8 / (CH) + 1)
+#endif
+
 template <int CH, int NW, bool COM1>
 __global__ __launch_bounds__(kBlock) void fused_u16_int(IntLaunch a) {
     constexpr int NV0 = NW + (COM1 ? 3 : 0);
@@ -99,17 +104,25 @@ __global__ __launch_bounds__(kBlock) void fused_u16_int(IntLaunch a) {
     constexpr int LDS_COLS = (kWavesPerBlock - 1) * CH * 12 * kWave;
     __shared__ uint32_t lds[LDS_ROWS > LDS_COLS ? LDS_ROWS : LDS_COLS];
 
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    // the wave index is uniform; readfirstlane lets the compiler keep the position run in scalar registers
+    const int lane = threadIdx.x & (kWave - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int slice = blockIdx.x % a.nslices, group = blockIdx.x / a.nslices;
     const int s0 = slice * SLICE;
     const int pairs = a.ndet >> 1;
 
-    int px[CH];
+    // Pixels past the pattern (a partial last slice) are loaded from the row start instead, so that every
+    // load is unconditional: loads under a lane mask make the compiler wait for all loads in flight, which
+    // drains the prefetch. Their weights are zero and their column sums are never written; only the
+    // arithmetic centre-of-mass rows mask them out (vmask).
+    int px[CH], pxl[CH];
     bool valid[CH];
+    uint32_t vmask[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
         px[c] = s0 + (c * kWave + lane) * 8;
         valid[c] = px[c] < a.ndet;
+        pxl[c] = valid[c] ? px[c] : 0;
+        vmask[c] = valid[c] ? ~0u : 0u;
     }
     uint32_t w[NW > 0 ? NW : 1][CH][4];
 #pragma unroll
@@ -148,8 +161,7 @@ __global__ __launch_bounds__(kBlock) void fused_u16_int(IntLaunch a) {
         const uint16_t *row = a.cube + p * a.ndet;
 #pragma unroll
         for (int c = 0; c < CH; ++c)
-            d[c] = valid[c] ? __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(row + px[c]))
-                            : u32x4{0u, 0u, 0u, 0u};
+            d[c] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(row + pxl[c]));
     };
 
     int b = 0;
@@ -178,6 +190,8 @@ __global__ __launch_bounds__(kBlock) void fused_u16_int(IntLaunch a) {
                 jj = udot2(x[1], 0x00030002u, jj);                 // (2, 3)
                 jj = udot2(x[2], 0x00050004u, jj);                 // (4, 5)
                 jj = udot2(x[3], 0x00070006u, jj);                 // (6, 7)
+                cs &= vmask[c];
+                jj &= vmask[c];
                 acc[NW] += cs;
                 acc[NW + 1] += ic[c] * cs;
                 acc[NW + 2] += jc[c] * cs + jj;
@@ -193,21 +207,29 @@ __global__ __launch_bounds__(kBlock) void fused_u16_int(IntLaunch a) {
         }
     };
 
-    // Three position buffers in rotation: two positions in flight while one is reduced.
-    u32x4 A[CH], Bf[CH], Cf[CH];
+    // NB position buffers in rotation: NB - 1 positions in flight while one is reduced. The prefetches are
+    // unconditional
+    // (past the run they repeat its last position, whose copy is never reduced), so the compiler can count
+    // them and wait only for the position it reduces.
+    constexpr int NB = CUBO_FUSED_NB(CH);
+    u32x4 buf[NB][CH];
     int64_t p = pbeg;
-    if (p < pend) load(A, p);
-    if (p + 1 < pend) load(Bf, p + 1);
-    for (; p + 3 <= pend; p += 3) {
-        load(Cf, p + 2);
-        process(A);
-        if (p + 3 < pend) load(A, p + 3);
-        process(Bf);
-        if (p + 4 < pend) load(Bf, p + 4);
-        process(Cf);
+    if (pbeg < pend) {
+        const int64_t last = pend - 1;
+#pragma unroll
+        for (int j = 0; j < NB - 1; ++j) load(buf[j], min(p + j, last));
+        for (; p + NB <= pend; p += NB) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                load(buf[(j + NB - 1) % NB], min(p + j + NB - 1, last));
+                process(buf[j]);
+            }
+        }
+        const int64_t rest = pend - p;                      // < NB; held in buf[0 .. rest - 1]
+#pragma unroll
+        for (int j = 0; j < NB - 1; ++j)
+            if (j < rest) process(buf[j]);
     }
-    if (p < pend) process(A);
-    if (p + 1 < pend) process(Bf);
     if (b > 0) rb.flush(b, first, rowslice, lane);
 
     // Combine the four waves' column partials (same pixels, consecutive position runs) and write one
@@ -267,11 +289,13 @@ __global__ __launch_bounds__(kBlock) void fused_f64(F64Launch a) {
     const T *cube = reinterpret_cast<const T *>(a.cube);
 
     int px[CH];
+    int pxl[CH];
     bool valid[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
         px[c] = s0 + c * kWave + lane;
         valid[c] = px[c] < a.ndet;
+        pxl[c] = valid[c] ? px[c] : 0;
     }
     double csum[CH];
     float cmax[CH];
@@ -290,21 +314,23 @@ __global__ __launch_bounds__(kBlock) void fused_f64(F64Launch a) {
     int64_t first = pbeg;
     for (int64_t p = pbeg; p < pend; ++p) {
         const T *row = cube + p * a.ndet;
+        // unconditional loads (pixels past the pattern read the row start; see fused_u16_int), and the
+        // value of such a pixel is replaced by 0 where it is used, so it adds nothing to the rows; its
+        // column sums are never written
         T xv[CH];
 #pragma unroll
-        for (int c = 0; c < CH; ++c) xv[c] = valid[c] ? __builtin_nontemporal_load(row + px[c]) : (T)0;
+        for (int c = 0; c < CH; ++c) xv[c] = __builtin_nontemporal_load(row + pxl[c]);
         double acc[NW];
 #pragma unroll
         for (int v = 0; v < NW; ++v) acc[v] = 0.0;
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-            if (!valid[c]) continue;
-            const float xf = (float)xv[c];
+            const float xf = valid[c] ? (float)xv[c] : 0.f;
             const double x = (double)xf;
             csum[c] += x;
             cmax[c] = max_step(cmax[c], xf);
 #pragma unroll
-            for (int v = 0; v < NW; ++v) acc[v] = fma(x, a.w[(size_t)v * a.ndet + px[c]], acc[v]);
+            for (int v = 0; v < NW; ++v) acc[v] = fma(x, a.w[(size_t)v * a.ndet + pxl[c]], acc[v]);
         }
         rb.put(b, acc, lane);
         if (++b == rb.B) {
@@ -366,12 +392,25 @@ __global__ __launch_bounds__(256) void accumulate_cols_int(const uint32_t *colsu
     const int i = blockIdx.x * 64 + lx;
     uint64_t s = 0;
     uint32_t m = 0;
-    if (i < ndet)
-        for (int g = q; g < ngroups; g += 4) {
+    if (i < ndet) {
+        // eight groups' loads in flight per thread before any of them is added
+        int g = q;
+        for (; g + 28 < ngroups; g += 32) {
+            uint32_t vs[8], vm[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                vs[u] = colsum[(size_t)(g + 4 * u) * ndet + i];
+                vm[u] = colmax[(size_t)(g + 4 * u) * ndet + i];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) { s += vs[u]; m = vm[u] > m ? vm[u] : m; }
+        }
+        for (; g < ngroups; g += 4) {
             s += colsum[(size_t)g * ndet + i];
             const uint32_t v = colmax[(size_t)g * ndet + i];
             m = v > m ? v : m;
         }
+    }
     ps[q][lx] = s;
     pm[q][lx] = m;
     __syncthreads();
@@ -390,7 +429,19 @@ __global__ void accumulate_cols_f64(const double *colsum, const float *colmax, i
     if (i >= ndet) return;
     double s = sum[i];
     float m = max[i];
-    for (int g = 0; g < ngroups; ++g) {
+    int g = 0;
+    for (; g + 8 <= ngroups; g += 8) {                   // loads batched, adds still in group order
+        double vs[8];
+        float vm[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            vs[u] = colsum[(size_t)(g + u) * ndet + i];
+            vm[u] = colmax[(size_t)(g + u) * ndet + i];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { s += vs[u]; m = max_step(m, vm[u]); }
+    }
+    for (; g < ngroups; ++g) {
         s += colsum[(size_t)g * ndet + i];
         m = max_step(m, colmax[(size_t)g * ndet + i]);
     }

@@ -97,6 +97,6 @@ def batch_items(n: int = 48, seed: int = 20261104):
         k = int(rng.integers(8, 65))
         p = params(int(rng.integers(1 << 62)), (det / 2, det / 2), det / 10, float(rng.uniform(100, 3000)),
                    edge=0.5, background=2.0, rings=[(det / 4, 20.0, 1.5)],
-                   waves=random_waves(rng, 2, 2.0, scan))
+                   waves=random_waves(rng, 2, 3.0, scan))
         out.append((shape, p, k))
     return out
