@@ -41,13 +41,15 @@ struct cubo_hip_reduction {
     void *rowpart;          // [nslices][npos][nv]
     void *colsum_part;      // [groups][ndet] uint32 (integer) or double (FP64)
     void *colmax_part;      // [groups][ndet] uint32 (integer) or float (FP64)
-    void *col_sum;          // uint64 or double [ndet]
+    void *col_sum;          // uint64 or double [ndet], running over tiles
     void *col_max;          // uint32 or float [ndet]
-    double *res_dev;        // [nv][npos] rows, then mean [ndet], max [ndet]
+    bool full;              // integer kernel in the FULL layout (one block per group, rows written in-kernel)
     int max_groups;
-    // pinned host
-    double *res_host;       // mirror of res_dev, then com row [npos], com column [npos]
-    size_t res_count;       // doubles in res_dev
+    // Mapped pinned host memory, written by the kernels: rows [nv][npos], mean [ndet], max [ndet], then com
+    // row [npos], com column [npos]. res_hd is its device address.
+    double *res_host;
+    double *res_hd;
+    size_t res_count;       // doubles before the com rows
 };
 
 namespace {
@@ -59,7 +61,7 @@ int64_t result_doubles(const cubo_hip_reduction *r) {
 void free_reduction(cubo_hip_reduction *r) {
     if (!r) return;
     DeviceGuard g(r->cube->device);
-    void *bufs[] = {r->weights, r->rowpart, r->colsum_part, r->colmax_part, r->col_sum, r->col_max, r->res_dev};
+    void *bufs[] = {r->weights, r->rowpart, r->colsum_part, r->colmax_part, r->col_sum, r->col_max};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (r->res_host) (void)hipHostFree(r->res_host);
@@ -93,6 +95,15 @@ int padded_f64_nw(int nw) {
 // Position groups of one launch: one wave of blocks that fills every compute unit at the kernel's
 // occupancy, at most 65536 positions per group (the integer column partials are uint32 before the
 // 64-bit add: 65535 * 65536 < 2^32).
+// FULL layout: one block per group, every wave walks the whole group.
+void plan_groups_full(int64_t npos, int cus, int blocks_per_cu, int *ngroups, int *ppg, int *ppw) {
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * std::max(1, blocks_per_cu), npos));
+    const int64_t per = std::min<int64_t>((npos + g - 1) / g, 65536);
+    *ppg = (int)per;
+    *ngroups = (int)((npos + per - 1) / per);
+    *ppw = (int)per;
+}
+
 void plan_groups(int64_t npos, int nslices, int cus, int blocks_per_cu, int *ngroups, int *ppg, int *ppw) {
     const int64_t target_blocks = (int64_t)cus * std::max(1, blocks_per_cu);
     int64_t g = std::max<int64_t>(1, (target_blocks + nslices - 1) / nslices);
@@ -155,12 +166,25 @@ int choose_path(const cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_h
     return CUBO_HIP_OK;
 }
 
-hipError_t init_cols(cubo_hip_reduction *r) {
-    cubo_hip_cube *c = r->cube;
-    return r->path == 1 ? cubo::launch_fill_cols_init((uint64_t *)r->col_sum, (uint32_t *)r->col_max, nullptr, nullptr,
-                                                      (int)c->ndet, c->stream)
-                        : cubo::launch_fill_cols_init(nullptr, nullptr, (double *)r->col_sum, (float *)r->col_max,
-                                                      (int)c->ndet, c->stream);
+
+void plan(const cubo_hip_reduction *r, int64_t npos, int *ngroups, int *ppg, int *ppw) {
+    const int cus = cubo::compute_units(r->cube->device);
+    if (r->full) plan_groups_full(npos, cus, r->blocks_per_cu, ngroups, ppg, ppw);
+    else plan_groups(npos, r->nslices, cus, r->blocks_per_cu, ngroups, ppg, ppw);
+}
+
+// Where a launch starting at position pos0 writes its row totals and centre of mass (mapped host memory).
+cubo::RowsOut rows_out(const cubo_hip_reduction *r, int64_t pos0) {
+    cubo::RowsOut o{};
+    o.rows = r->res_hd;
+    o.com = r->res_hd + r->res_count;
+    o.pos0 = pos0;
+    o.total_pos = r->cube->npos;
+    o.nv = r->nv;
+    // The integer kernel's arithmetic CoM rows follow its weight rows; in every other layout the three CoM
+    // rows directly follow the masks.
+    o.com_base = !r->want_com ? -1 : (r->path == 1 && r->com_ones) ? r->nw : r->n_masks;
+    return o;
 }
 
 }  // namespace
@@ -208,12 +232,17 @@ int cubo_hip_reduce_begin(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cu
         r->nv = pnw;
         r->slice_px = cubo::kF64Slice;
         r->blocks_per_cu = cubo::blocks_per_cu_f64(c->dtype == CUBO_HIP_U16, pnw);
-    } else {
-        r->blocks_per_cu = cubo::blocks_per_cu_int(r->nw, r->chunks, r->com_ones);
+    }
+    r->nslices = (int)((ndet + r->slice_px - 1) / r->slice_px);
+    if (r->path == 1) {
+        // CUBO_HIP_INT_LAYOUT=split forces the (slice, group) layout (a tuning and test knob).
+        const char *layout = getenv("CUBO_HIP_INT_LAYOUT");
+        r->full = r->nslices <= cubo::kFullSlices && !(layout && strcmp(layout, "split") == 0) &&
+                  cubo::full_without_spills(r->nw, r->chunks, r->com_ones);
+        r->blocks_per_cu = cubo::blocks_per_cu_int(r->nw, r->chunks, r->com_ones, r->full, r->nslices);
     }
     // Blocks per compute unit of the launch plan: the kernel's occupancy, or a tuned multiple of it.
     if (const char *env = getenv("CUBO_HIP_BLOCKS_PER_CU")) r->blocks_per_cu = std::max(1, atoi(env));
-    r->nslices = (int)((ndet + r->slice_px - 1) / r->slice_px);
 
     hipError_t e = hipSuccess;
     if (r->path == 1) {
@@ -254,24 +283,25 @@ int cubo_hip_reduce_begin(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cu
 
     // Scratch for the largest tile (the whole cube).
     int ngroups = 0, ppg = 0, ppw = 0;
-    plan_groups(c->npos, r->nslices, cubo::compute_units(c->device), r->blocks_per_cu, &ngroups, &ppg, &ppw);
+    plan(r, c->npos, &ngroups, &ppg, &ppw);
     r->max_groups = ngroups;
     r->res_count = (size_t)result_doubles(r);
     struct { void **p; size_t n; } allocs[] = {
-        {&r->rowpart, (size_t)r->nslices * c->npos * std::max(r->nv, 1) * 8},
+        {&r->rowpart, r->full ? 0 : (size_t)r->nslices * c->npos * std::max(r->nv, 1) * 8},
         {&r->colsum_part, (size_t)ngroups * ndet * (r->path == 2 ? 8 : 4)},
         {&r->colmax_part, (size_t)ngroups * ndet * 4},
-        {&r->col_sum, (size_t)ndet * 8}, {&r->col_max, (size_t)ndet * 4},
-        {(void **)&r->res_dev, r->res_count * 8}};
+        {&r->col_sum, (size_t)ndet * 8}, {&r->col_max, (size_t)ndet * 4}};
     for (auto &a : allocs) {
         if (a.n == 0) continue;
         e = hipMalloc(a.p, a.n);
         if (e != hipSuccess) { free_reduction(r); return hip_fail(e, "allocating reduction scratch"); }
     }
-    e = hipHostMalloc((void **)&r->res_host, (r->res_count + 2 * (size_t)c->npos) * 8, hipHostMallocDefault);
+    // Results go straight from the kernels to mapped, coherent pinned memory: no copy after the pass.
+    e = hipHostMalloc((void **)&r->res_host, (r->res_count + 2 * (size_t)c->npos) * 8,
+                      hipHostMallocMapped | hipHostMallocCoherent);
     if (e != hipSuccess) { free_reduction(r); return hip_fail(e, "allocating pinned result memory"); }
-    e = init_cols(r);
-    if (e != hipSuccess) { free_reduction(r); return hip_fail(e, "initialising the column sums"); }
+    e = hipHostGetDevicePointer((void **)&r->res_hd, r->res_host, 0);
+    if (e != hipSuccess) { free_reduction(r); return hip_fail(e, "mapping the result memory"); }
     *out = r;
     return CUBO_HIP_OK;
 }
@@ -289,7 +319,7 @@ int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows) {
     const int64_t npos = nrows * c->shape[1];
     const int ndet = (int)c->ndet;
     int ngroups = 0, ppg = 0, ppw = 0;
-    plan_groups(npos, r->nslices, cubo::compute_units(c->device), r->blocks_per_cu, &ngroups, &ppg, &ppw);
+    plan(r, npos, &ngroups, &ppg, &ppw);
     if (ngroups > r->max_groups) return fail(CUBO_HIP_ERR_HIP, "internal: group plan exceeds scratch");
     hipEvent_t t0, t1;
     CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
@@ -310,7 +340,8 @@ int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows) {
         a.rowpart = (uint64_t *)r->rowpart;
         a.colsum = (uint32_t *)r->colsum_part;
         a.colmax = (uint32_t *)r->colmax_part;
-        e = cubo::launch_int(a, r->nw, r->chunks, r->com_ones, c->stream);
+        a.out = rows_out(r, pos0);
+        e = cubo::launch_int(a, r->nw, r->chunks, r->com_ones, r->full, c->stream);
     } else {
         cubo::F64Launch a{};
         a.cube = (const char *)c->data + (size_t)pos0 * c->ndet * cubo::element_size(c->dtype);
@@ -329,20 +360,27 @@ int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows) {
     if (e != hipSuccess) return hip_fail(e, "launching the fused reduction");
     CUBO_HIP_TRY(hipEventRecord(t1, c->stream), "hipEventRecord");
     c->timed.emplace_back(t0, t1);
-    if (r->nv > 0) {
-        e = r->path == 1 ? cubo::launch_finalize_rows_u64((const uint64_t *)r->rowpart, r->nslices, npos, r->nv,
-                                                          r->res_dev, pos0, c->npos, c->stream)
-                         : cubo::launch_finalize_rows_f64((const double *)r->rowpart, r->nslices, npos, r->nv,
-                                                          r->res_dev, pos0, c->npos, c->stream);
+    // The integer kernel in the FULL layout writes its row totals itself.
+    if (r->nv > 0 && !r->full) {
+        e = r->path == 1 ? cubo::launch_finalize_rows_u64((const uint64_t *)r->rowpart, r->nslices, npos,
+                                                          rows_out(r, pos0), c->stream)
+                         : cubo::launch_finalize_rows_f64((const double *)r->rowpart, r->nslices, npos,
+                                                          rows_out(r, pos0), c->stream);
         if (e != hipSuccess) return hip_fail(e, "launching the row totals");
     }
-    e = r->path == 1 ? cubo::launch_accumulate_cols_int((const uint32_t *)r->colsum_part,
-                                                        (const uint32_t *)r->colmax_part, ngroups, ndet,
-                                                        (uint64_t *)r->col_sum, (uint32_t *)r->col_max, c->stream)
-                     : cubo::launch_accumulate_cols_f64((const double *)r->colsum_part, (const float *)r->colmax_part,
-                                                        ngroups, ndet, (double *)r->col_sum, (float *)r->col_max,
-                                                        c->stream);
-    if (e != hipSuccess) return hip_fail(e, "launching the column totals");
+    if (r->want_cols) {
+        const bool first = r->rows_done == 0, finish = row0 + nrows == c->shape[0];
+        double *mean = r->res_hd + (size_t)std::max(r->nv, 1) * c->npos, *max = mean + c->ndet;
+        e = r->path == 1 ? cubo::launch_accumulate_cols_int((const uint32_t *)r->colsum_part,
+                                                            (const uint32_t *)r->colmax_part, ngroups, ndet,
+                                                            (uint64_t *)r->col_sum, (uint32_t *)r->col_max, first,
+                                                            finish, (double)c->npos, mean, max, c->stream)
+                         : cubo::launch_accumulate_cols_f64((const double *)r->colsum_part,
+                                                            (const float *)r->colmax_part, ngroups, ndet,
+                                                            (double *)r->col_sum, (float *)r->col_max, first, finish,
+                                                            (double)c->npos, mean, max, c->stream);
+        if (e != hipSuccess) return hip_fail(e, "launching the column totals");
+    }
     r->next_row = row0 + nrows;
     r->rows_done += nrows;
     r->viewed = false;
@@ -364,25 +402,7 @@ int cubo_hip_reduce_view(cubo_hip_reduction *r, cubo_hip_result_view *v) {
     const int base = (r->path == 1 && r->com_ones) ? r->nw : r->n_masks;
     if (!r->viewed) {
         DeviceGuard g(c->device);
-        hipError_t e = hipSuccess;
-        if (r->want_cols) {
-            double *dmean = r->res_dev + (size_t)std::max(r->nv, 1) * npos;
-            e = r->path == 1 ? cubo::launch_finish_cols_int((const uint64_t *)r->col_sum, (const uint32_t *)r->col_max,
-                                                            (int)ndet, (double)npos, dmean, dmean + ndet, c->stream)
-                             : cubo::launch_finish_cols_f64((const double *)r->col_sum, (const float *)r->col_max,
-                                                            (int)ndet, (double)npos, dmean, dmean + ndet, c->stream);
-            if (e != hipSuccess) return hip_fail(e, "launching the column finish");
-        }
-        CUBO_HIP_TRY(hipMemcpyAsync(r->res_host, r->res_dev, r->res_count * 8, hipMemcpyDeviceToHost, c->stream),
-                     "reading results");
         CUBO_HIP_TRY(hipStreamSynchronize(c->stream), "waiting for the reduction");
-        if (r->want_com) {
-            const double *s = rows + (size_t)base * npos, *mr = s + npos, *mc = mr + npos;
-            for (int64_t p = 0; p < npos; ++p) {
-                crow[p] = mr[p] / s[p];       // NaN where S = 0 (CenterOfMass.swift:44-45)
-                ccol[p] = mc[p] / s[p];
-            }
-        }
         r->viewed = true;
     }
     memset(v, 0, sizeof *v);
@@ -423,8 +443,6 @@ int cubo_hip_reduce_reset(cubo_hip_reduction *r) {
     r->rows_done = 0;
     r->next_row = 0;
     r->viewed = false;
-    hipError_t e = init_cols(r);
-    if (e != hipSuccess) return hip_fail(e, "initialising the column sums");
     return CUBO_HIP_OK;
 }
 

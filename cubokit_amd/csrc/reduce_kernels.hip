@@ -63,6 +63,22 @@ struct RowBatch {
         for (int v = 0; v < NV; ++v) buf[(v * B + b) * kLdsStride + lane] = acc[v];
     }
 
+    // The wave total of row r = lane % ROWS (v = r / B, b = r % B) in the lanes of group 0 (lane < ROWS);
+    // ends with the batch region free for the next batch.
+    template <typename Out>
+    __device__ __forceinline__ Out total(int lane) {
+        wave_sync();
+        const int r = lane % ROWS, g = lane / ROWS;
+        const Acc *row = buf + r * kLdsStride + g * PER;
+        Out s = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) s += (Out)row[k];
+#pragma unroll
+        for (int off = ROWS; off < kWave; off <<= 1) s += shfl_down(s, off);
+        wave_sync();
+        return s;
+    }
+
     template <typename Out>
     __device__ __forceinline__ void flush(int nb, int64_t first, Out *rowpart_slice, int lane) {
         wave_sync();
@@ -89,24 +105,56 @@ struct RowBatch {
 // Exact integer kernel: uint16 cube, NW weight rows of uint16 (packed in pairs), optional arithmetic
 // centre-of-mass rows (w = 1: S, sum i*x, sum j*x) when the detector rows are multiples of 8 pixels.
 // ------------------------------------------------------------------------------------------------
+// Row totals of positions [p_begin, p_end) of a launch: slices summed in slice order, then the centre of
+// mass from the totals. Threads tid, tid + nthreads, ... take one position each.
+template <typename Acc>
+__device__ __forceinline__ void finalize_positions(const Acc *rowpart, int nslices, int64_t npos, const RowsOut &o,
+                                                   int64_t p_begin, int64_t p_end, int tid, int nthreads) {
+    for (int64_t p = p_begin + tid; p < p_end; p += nthreads) {
+        double com3[3] = {0.0, 0.0, 0.0};
+        for (int v = 0; v < o.nv; ++v) {
+            Acc acc = 0;
+            for (int sl = 0; sl < nslices; ++sl) acc += rowpart[((int64_t)sl * npos + p) * o.nv + v];
+            const double d = (double)acc;
+            o.rows[(int64_t)v * o.total_pos + o.pos0 + p] = d;
+            const int k = v - o.com_base;
+            if (o.com_base >= 0 && k >= 0 && k < 3) com3[k] = d;
+        }
+        if (o.com_base >= 0) {
+            o.com[o.pos0 + p] = com3[1] / com3[0];                    // NaN where S = 0 (CenterOfMass.swift:44-45)
+            o.com[o.total_pos + o.pos0 + p] = com3[2] / com3[0];
+        }
+    }
+}
+
 // Position buffers per wave of fused_u16_int (NB - 1 positions in flight).
 #ifndef CUBO_FUSED_NB
 #define CUBO_FUSED_NB(CH) (8 / (CH) + 1)
 #endif
 
-template <int CH, int NW, bool COM1>
-__global__ __launch_bounds__(kBlock) void fused_u16_int(IntLaunch a) {
+// Two block layouts:
+//  - FULL = false: a block is (slice, position group); its 4 waves cover the same slice over consecutive
+//    position runs; per-slice row partials go to rowpart and a second kernel adds the slices.
+//  - FULL = true (at most kFullSlices slices): a block is a position group and has one wave per slice; all
+//    waves walk the same positions, so each batch of row partials is added across the waves in LDS and
+//    the block writes the row totals and the centre of mass itself (to mapped host memory: they reach the
+//    host while the pass still streams). One block barrier per batch.
+template <int CH, int NW, bool COM1, bool FULL>
+__global__ __launch_bounds__(FULL ? kWave * kFullSlices : kBlock) void fused_u16_int(IntLaunch a) {
     constexpr int NV0 = NW + (COM1 ? 3 : 0);
     constexpr int NV = NV0 > 0 ? NV0 : 1;       // row outputs (one unused row when only columns are asked)
     constexpr int ROWS = 32;
     constexpr int SLICE = kWave * CH * 8;
-    constexpr int LDS_ROWS = kWavesPerBlock * ROWS * kLdsStride;
-    constexpr int LDS_COLS = (kWavesPerBlock - 1) * CH * 12 * kWave;
+    constexpr int WAVES = FULL ? kFullSlices : kWavesPerBlock;
+    constexpr int LDS_ROWS = WAVES * ROWS * kLdsStride;
+    constexpr int LDS_COLS = FULL ? 0 : (kWavesPerBlock - 1) * CH * 12 * kWave;
     __shared__ uint32_t lds[LDS_ROWS > LDS_COLS ? LDS_ROWS : LDS_COLS];
+    __shared__ uint64_t part[FULL ? 2 * kFullSlices * ROWS : 1];    // FULL: [batch parity][wave][b * NV + v]
 
     // the wave index is uniform; readfirstlane lets the compiler keep the position run in scalar registers
     const int lane = threadIdx.x & (kWave - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int slice = blockIdx.x % a.nslices, group = blockIdx.x / a.nslices;
+    const int slice = FULL ? wave : (int)(blockIdx.x % a.nslices);
+    const int group = FULL ? (int)blockIdx.x : (int)(blockIdx.x / a.nslices);
     const int s0 = slice * SLICE;
     const int pairs = a.ndet >> 1;
 
@@ -150,12 +198,43 @@ __global__ __launch_bounds__(kBlock) void fused_u16_int(IntLaunch a) {
     }
 
     RowBatch<uint32_t, NV, ROWS> rb{lds + wave * ROWS * kLdsStride};
-    uint64_t *rowslice = a.rowpart + (size_t)slice * a.npos * NV;
+    uint64_t *rowslice = FULL ? nullptr : a.rowpart + (size_t)slice * a.npos * NV;
 
     const int64_t gbeg = (int64_t)group * a.ppg;
     const int64_t gend = min(a.npos, gbeg + a.ppg);
-    const int64_t pbeg = min(gend, gbeg + (int64_t)wave * a.ppw);
-    const int64_t pend = min(gend, pbeg + a.ppw);
+    const int64_t pbeg = FULL ? gbeg : min(gend, gbeg + (int64_t)wave * a.ppw);
+    const int64_t pend = FULL ? gend : min(gend, pbeg + a.ppw);
+
+    // FULL: the wave totals of a batch of nb positions go to part[parity]; after the barrier the first nb
+    // threads add the slices of one position each and write its totals. The parity alternates, so the
+    // next batch never overwrites a buffer that is still being read (its readers pass the next barrier
+    // first).
+    int parity = 0;
+    auto combine = [&](int nb, int64_t first_pos) {
+        const uint64_t t = rb.template total<uint64_t>(lane);
+        const int r = lane % ROWS, v = r / rb.B, bb = r % rb.B;
+        if (lane < ROWS && v < NV && bb < nb) part[(parity * kFullSlices + wave) * ROWS + bb * NV + v] = t;
+        __syncthreads();
+        if ((int)threadIdx.x < nb) {
+            const int pb = threadIdx.x;
+            const RowsOut &o = a.out;
+            double com3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int vv = 0; vv < NV; ++vv) {
+                uint64_t sum = 0;
+                for (int ww = 0; ww < a.nslices; ++ww) sum += part[(parity * kFullSlices + ww) * ROWS + pb * NV + vv];
+                const double d = (double)sum;
+                o.rows[(int64_t)vv * o.total_pos + o.pos0 + first_pos + pb] = d;
+                const int k = vv - o.com_base;
+                if (o.com_base >= 0 && k >= 0 && k < 3) com3[k] = d;
+            }
+            if (o.com_base >= 0) {
+                o.com[o.pos0 + first_pos + pb] = com3[1] / com3[0];      // NaN where S = 0
+                o.com[o.total_pos + o.pos0 + first_pos + pb] = com3[2] / com3[0];
+            }
+        }
+        parity ^= 1;
+    };
 
     auto load = [&](u32x4 (&d)[CH], int64_t p) {
         const uint16_t *row = a.cube + p * a.ndet;
@@ -200,7 +279,8 @@ __global__ __launch_bounds__(kBlock) void fused_u16_int(IntLaunch a) {
         if constexpr (NV0 > 0) {
             rb.put(b, acc, lane);
             if (++b == rb.B) {
-                rb.flush(b, first, rowslice, lane);
+                if constexpr (FULL) combine(b, first);
+                else rb.flush(b, first, rowslice, lane);
                 first += b;
                 b = 0;
             }
@@ -230,7 +310,29 @@ __global__ __launch_bounds__(kBlock) void fused_u16_int(IntLaunch a) {
         for (int j = 0; j < NB - 1; ++j)
             if (j < rest) process(buf[j]);
     }
-    if (b > 0) rb.flush(b, first, rowslice, lane);
+    if constexpr (NV0 > 0) {
+        if (b > 0) {
+            if constexpr (FULL) combine(b, first);
+            else rb.flush(b, first, rowslice, lane);
+        }
+    }
+
+    if constexpr (FULL) {
+        // one wave per slice: the column partials need no combining
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            if (valid[c]) {
+                uint32_t *cs = a.colsum + (size_t)group * a.ndet + px[c];
+                uint32_t *cm = a.colmax + (size_t)group * a.ndet + px[c];
+                reinterpret_cast<uint4 *>(cs)[0] = make_uint4(csum[c][0], csum[c][1], csum[c][2], csum[c][3]);
+                reinterpret_cast<uint4 *>(cs)[1] = make_uint4(csum[c][4], csum[c][5], csum[c][6], csum[c][7]);
+                reinterpret_cast<uint4 *>(cm)[0] = make_uint4(cmax[c][0] & 0xffffu, cmax[c][0] >> 16,
+                                                              cmax[c][1] & 0xffffu, cmax[c][1] >> 16);
+                reinterpret_cast<uint4 *>(cm)[1] = make_uint4(cmax[c][2] & 0xffffu, cmax[c][2] >> 16,
+                                                              cmax[c][3] & 0xffffu, cmax[c][3] >> 16);
+            }
+        return;
+    }
 
     // Combine the four waves' column partials (same pixels, consecutive position runs) and write one
     // partial per group.
@@ -372,20 +474,16 @@ __global__ __launch_bounds__(kBlock) void fused_f64(F64Launch a) {
 // Second-stage kernels.
 // ------------------------------------------------------------------------------------------------
 template <typename Acc>
-__global__ void finalize_rows(const Acc *rowpart, int nslices, int64_t npos, int nv, double *rows_out, int64_t pos0,
-                              int64_t total_pos) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npos * nv) return;
-    const int64_t p = i / nv;
-    const int v = (int)(i % nv);
-    Acc s = 0;
-    for (int sl = 0; sl < nslices; ++sl) s += rowpart[((int64_t)sl * npos + p) * nv + v];
-    rows_out[(int64_t)v * total_pos + pos0 + p] = (double)s;
+__global__ void finalize_rows(const Acc *rowpart, int nslices, int64_t npos, RowsOut o) {
+    finalize_positions<Acc>(rowpart, nslices, npos, o, (int64_t)blockIdx.x * blockDim.x,
+                            min(npos, (int64_t)(blockIdx.x + 1) * blockDim.x), threadIdx.x, blockDim.x);
 }
 
 // 64 pixels x 4 group phases per block: each thread sums every fourth group of one pixel, LDS adds the four.
 __global__ __launch_bounds__(256) void accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups,
-                                                           int ndet, uint64_t *sum, uint32_t *max) {
+                                                           int ndet, uint64_t *sum, uint32_t *max, bool first,
+                                                           bool finish, double count, double *mean_out,
+                                                           double *max_out) {
     __shared__ uint64_t ps[4][64];
     __shared__ uint32_t pm[4][64];
     const int lx = threadIdx.x & 63, q = threadIdx.x >> 6;
@@ -415,20 +513,31 @@ __global__ __launch_bounds__(256) void accumulate_cols_int(const uint32_t *colsu
     pm[q][lx] = m;
     __syncthreads();
     if (q == 0 && i < ndet) {
-        s = sum[i] + ps[0][lx] + ps[1][lx] + ps[2][lx] + ps[3][lx];
-        m = max[i];
+        s = ps[0][lx] + ps[1][lx] + ps[2][lx] + ps[3][lx];
+        m = 0;
         for (int k = 0; k < 4; ++k) m = pm[k][lx] > m ? pm[k][lx] : m;
-        sum[i] = s;
-        max[i] = m;
+        if (!first) {
+            s += sum[i];
+            m = max[i] > m ? max[i] : m;
+        }
+        if (finish) {
+            mean_out[i] = (double)s / count;
+            max_out[i] = (double)m;
+        } else {
+            sum[i] = s;
+            max[i] = m;
+        }
     }
 }
 
 __global__ void accumulate_cols_f64(const double *colsum, const float *colmax, int ngroups, int ndet, double *sum,
-                                    float *max) {
+                                    float *max, bool first, bool finish, double count, double *mean_out,
+                                    double *max_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndet) return;
-    double s = sum[i];
-    float m = max[i];
+    // the running sum starts at -0 (an all -0 column keeps its sign) and the maximum at -inf
+    double s = first ? -0.0 : sum[i];
+    float m = first ? -__builtin_inff() : max[i];
     int g = 0;
     for (; g + 8 <= ngroups; g += 8) {                   // loads batched, adds still in group order
         double vs[8];
@@ -445,24 +554,13 @@ __global__ void accumulate_cols_f64(const double *colsum, const float *colmax, i
         s += colsum[(size_t)g * ndet + i];
         m = max_step(m, colmax[(size_t)g * ndet + i]);
     }
-    sum[i] = s;
-    max[i] = m;
-}
-
-__global__ void finish_cols_int(const uint64_t *sum, const uint32_t *max, int ndet, double count, double *mean,
-                                double *maxd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ndet) return;
-    mean[i] = (double)sum[i] / count;
-    maxd[i] = (double)max[i];
-}
-
-__global__ void finish_cols_f64(const double *sum, const float *max, int ndet, double count, double *mean,
-                                double *maxd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ndet) return;
-    mean[i] = sum[i] / count;
-    maxd[i] = (double)max[i];
+    if (finish) {
+        mean_out[i] = s / count;
+        max_out[i] = (double)m;
+    } else {
+        sum[i] = s;
+        max[i] = m;
+    }
 }
 
 __global__ void copy_bytes(const u32x4 *src, u32x4 *dst, uint64_t n16, const uint8_t *tail_src, uint8_t *tail_dst,
@@ -473,12 +571,6 @@ __global__ void copy_bytes(const u32x4 *src, u32x4 *dst, uint64_t n16, const uin
     if (blockIdx.x == 0 && threadIdx.x < tail) tail_dst[threadIdx.x] = tail_src[threadIdx.x];
 }
 
-__global__ void fill_cols_init(uint64_t *isum, uint32_t *imax, double *fsum, float *fmax, int ndet) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ndet) return;
-    if (isum) { isum[i] = 0; imax[i] = 0; }
-    if (fsum) { fsum[i] = -0.0; fmax[i] = -__builtin_inff(); }
-}
 
 // ------------------------------------------------------------------------------------------------
 // Host launchers.
@@ -500,11 +592,13 @@ bool supported_int(int nw, int chunks, bool com_ones) {
     return false;
 }
 
-hipError_t launch_int(const IntLaunch &a, int nw, int chunks, bool com_ones, hipStream_t s) {
-    const dim3 grid((unsigned)(a.nslices * a.ngroups)), block(kBlock);
+hipError_t launch_int(const IntLaunch &a, int nw, int chunks, bool com_ones, bool full, hipStream_t s) {
+    if (full && a.nslices > kFullSlices) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(full ? a.ngroups : a.nslices * a.ngroups)), block(full ? kWave * a.nslices : kBlock);
 #define X(CH, NW, C1) \
     if (chunks == CH && nw == NW && com_ones == C1) { \
-        hipLaunchKernelGGL((fused_u16_int<CH, NW, C1>), grid, block, 0, s, a); \
+        if (full) hipLaunchKernelGGL((fused_u16_int<CH, NW, C1, true>), grid, block, 0, s, a); \
+        else hipLaunchKernelGGL((fused_u16_int<CH, NW, C1, false>), grid, block, 0, s, a); \
         return hipGetLastError(); \
     }
     CUBO_INT_CASES(X)
@@ -512,11 +606,28 @@ hipError_t launch_int(const IntLaunch &a, int nw, int chunks, bool com_ones, hip
     return hipErrorInvalidValue;
 }
 
-int blocks_per_cu_int(int nw, int chunks, bool com_ones) {
-    int n = 0;
+bool full_without_spills(int nw, int chunks, bool com_ones) {
+    hipFuncAttributes at{};
+    hipError_t e = hipErrorInvalidValue;
 #define X(CH, NW, C1) \
     if (chunks == CH && nw == NW && com_ones == C1) \
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_u16_int<CH, NW, C1>, kBlock, 0) != hipSuccess) n = 1;
+        e = hipFuncGetAttributes(&at, reinterpret_cast<const void *>(fused_u16_int<CH, NW, C1, true>));
+    CUBO_INT_CASES(X)
+#undef X
+    return e == hipSuccess && at.localSizeBytes == 0;
+}
+
+int blocks_per_cu_int(int nw, int chunks, bool com_ones, bool full, int nslices) {
+    int n = 0;
+    const int threads = full ? kWave * nslices : kBlock;
+#define X(CH, NW, C1) \
+    if (chunks == CH && nw == NW && com_ones == C1) { \
+        const hipError_t e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_u16_int<CH, NW, C1, true>, \
+                                                                                 threads, 0) \
+                                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_u16_int<CH, NW, C1, false>, \
+                                                                                 threads, 0); \
+        if (e != hipSuccess) n = 1; \
+    }
     CUBO_INT_CASES(X)
 #undef X
     return n;
@@ -559,45 +670,33 @@ int blocks_per_cu_f64(bool is_u16, int nw) {
 
 static unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
-hipError_t launch_finalize_rows_u64(const uint64_t *rowpart, int nslices, int64_t npos, int nv, double *rows_out,
-                                    int64_t pos0, int64_t total_pos, hipStream_t s) {
-    hipLaunchKernelGGL(finalize_rows<uint64_t>, dim3(blocks_for(npos * nv, 256)), dim3(256), 0, s, rowpart, nslices,
-                       npos, nv, rows_out, pos0, total_pos);
+hipError_t launch_finalize_rows_u64(const uint64_t *rowpart, int nslices, int64_t npos, const RowsOut &out,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL(finalize_rows<uint64_t>, dim3(blocks_for(npos, 256)), dim3(256), 0, s, rowpart, nslices, npos,
+                       out);
     return hipGetLastError();
 }
 
-hipError_t launch_finalize_rows_f64(const double *rowpart, int nslices, int64_t npos, int nv, double *rows_out,
-                                    int64_t pos0, int64_t total_pos, hipStream_t s) {
-    hipLaunchKernelGGL(finalize_rows<double>, dim3(blocks_for(npos * nv, 256)), dim3(256), 0, s, rowpart, nslices,
-                       npos, nv, rows_out, pos0, total_pos);
+hipError_t launch_finalize_rows_f64(const double *rowpart, int nslices, int64_t npos, const RowsOut &out,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL(finalize_rows<double>, dim3(blocks_for(npos, 256)), dim3(256), 0, s, rowpart, nslices, npos,
+                       out);
     return hipGetLastError();
 }
 
 hipError_t launch_accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups, int ndet,
-                                      uint64_t *sum, uint32_t *max, hipStream_t s) {
+                                      uint64_t *sum, uint32_t *max, bool first, bool finish, double count,
+                                      double *mean_out, double *max_out, hipStream_t s) {
     hipLaunchKernelGGL(accumulate_cols_int, dim3(blocks_for(ndet, 64)), dim3(256), 0, s, colsum, colmax, ngroups, ndet,
-                       sum, max);
+                       sum, max, first, finish, count, mean_out, max_out);
     return hipGetLastError();
 }
 
 hipError_t launch_accumulate_cols_f64(const double *colsum, const float *colmax, int ngroups, int ndet,
-                                      double *sum, float *max, hipStream_t s) {
+                                      double *sum, float *max, bool first, bool finish, double count,
+                                      double *mean_out, double *max_out, hipStream_t s) {
     hipLaunchKernelGGL(accumulate_cols_f64, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, colsum, colmax, ngroups,
-                       ndet, sum, max);
-    return hipGetLastError();
-}
-
-hipError_t launch_finish_cols_int(const uint64_t *sum, const uint32_t *max, int ndet, double count, double *mean,
-                                  double *maxd, hipStream_t s) {
-    hipLaunchKernelGGL(finish_cols_int, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, sum, max, ndet, count, mean,
-                       maxd);
-    return hipGetLastError();
-}
-
-hipError_t launch_finish_cols_f64(const double *sum, const float *max, int ndet, double count, double *mean,
-                                  double *maxd, hipStream_t s) {
-    hipLaunchKernelGGL(finish_cols_f64, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, sum, max, ndet, count, mean,
-                       maxd);
+                       ndet, sum, max, first, finish, count, mean_out, max_out);
     return hipGetLastError();
 }
 
@@ -609,9 +708,5 @@ hipError_t launch_copy_bytes(const void *src, void *dst, uint64_t n, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_fill_cols_init(uint64_t *isum, uint32_t *imax, double *fsum, float *fmax, int ndet, hipStream_t s) {
-    hipLaunchKernelGGL(fill_cols_init, dim3(blocks_for(ndet, 256)), dim3(256), 0, s, isum, imax, fsum, fmax, ndet);
-    return hipGetLastError();
-}
 
 }  // namespace cubo

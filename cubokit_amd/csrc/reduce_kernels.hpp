@@ -17,6 +17,20 @@ namespace cubo {
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
+// Most slices per pattern for the integer kernel's FULL layout (one wave per slice in one block).
+constexpr int kFullSlices = 8;
+
+// Row totals of a launch: rows[v][pos0 + p] for v < nv and, when com_base >= 0, the centre of mass
+// com[0][pos0 + p] = rows[com_base + 1] / rows[com_base] and com[1][...] = rows[com_base + 2] / rows[com_base]
+// (correctly rounded Double divisions; NaN where the intensity is 0). The pointers may be device
+// pointers of mapped host memory, so the totals reach the host while the kernel still streams.
+struct RowsOut {
+    double *rows;
+    double *com;
+    int64_t pos0, total_pos;
+    int32_t nv;
+    int32_t com_base;
+};
 
 struct IntLaunch {
     const uint16_t *cube;   // first position of this launch
@@ -31,6 +45,7 @@ struct IntLaunch {
     uint64_t *rowpart;      // [nslices][npos][nv]
     uint32_t *colsum;       // [ngroups][ndet]
     uint32_t *colmax;       // [ngroups][ndet]
+    RowsOut out;            // row totals (written by the kernel itself in the FULL layout)
 };
 
 struct F64Launch {
@@ -55,30 +70,32 @@ constexpr int kF64Slice = kWave * kF64Chunks;
 // Returns hipSuccess or the launch error. nw / com_ones select the instantiation; the caller checks
 // that (nw, chunks) is one of the compiled combinations with supported_int().
 bool supported_int(int nw, int chunks, bool com_ones);
-hipError_t launch_int(const IntLaunch &a, int nw, int chunks, bool com_ones, hipStream_t s);
+// full: one block per position group with one wave per slice (nslices <= kFullSlices), row totals written
+// by the kernel; otherwise blocks of 4 waves per (slice, group) and launch_finalize_rows_u64 afterwards.
+hipError_t launch_int(const IntLaunch &a, int nw, int chunks, bool com_ones, bool full, hipStream_t s);
+// The FULL instantiation keeps everything in registers (its 512-thread blocks cap a lane at 256 VGPRs).
+bool full_without_spills(int nw, int chunks, bool com_ones);
 // Resident blocks per compute unit of an instantiation (0 if it is not compiled).
-int blocks_per_cu_int(int nw, int chunks, bool com_ones);
+int blocks_per_cu_int(int nw, int chunks, bool com_ones, bool full, int nslices);
 int blocks_per_cu_f64(bool is_u16, int nw);
 bool supported_f64(int nw);
 hipError_t launch_f64(const F64Launch &a, bool is_u16, int nw, hipStream_t s);
 
-// Row totals over slices into rows_out[v][pos0 + p] (Double), v < nv.
-hipError_t launch_finalize_rows_u64(const uint64_t *rowpart, int nslices, int64_t npos, int nv, double *rows_out,
-                                    int64_t pos0, int64_t total_pos, hipStream_t s);
-hipError_t launch_finalize_rows_f64(const double *rowpart, int nslices, int64_t npos, int nv, double *rows_out,
-                                    int64_t pos0, int64_t total_pos, hipStream_t s);
-// Column partials of one launch, in group order, into the reduction's running sums and maxima.
+// Row totals over slices, one thread per position, into `out`.
+hipError_t launch_finalize_rows_u64(const uint64_t *rowpart, int nslices, int64_t npos, const RowsOut &out,
+                                    hipStream_t s);
+hipError_t launch_finalize_rows_f64(const double *rowpart, int nslices, int64_t npos, const RowsOut &out,
+                                    hipStream_t s);
+// Column partials of one launch, in group order, into the running sums and maxima. first: the sums start
+// from this launch (nothing is read from sum/max); finish: the launch is the last one, so mean = sum / count
+// and max go to mean_out / max_out (possibly mapped host memory) instead of sum / max.
 hipError_t launch_accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups, int ndet,
-                                      uint64_t *sum, uint32_t *max, hipStream_t s);
+                                      uint64_t *sum, uint32_t *max, bool first, bool finish, double count,
+                                      double *mean_out, double *max_out, hipStream_t s);
 hipError_t launch_accumulate_cols_f64(const double *colsum, const float *colmax, int ngroups, int ndet,
-                                      double *sum, float *max, hipStream_t s);
-// mean = sum / count, max widened; CoM quotients.
-hipError_t launch_finish_cols_int(const uint64_t *sum, const uint32_t *max, int ndet, double count, double *mean,
-                                  double *maxd, hipStream_t s);
-hipError_t launch_finish_cols_f64(const double *sum, const float *max, int ndet, double count, double *mean,
-                                  double *maxd, hipStream_t s);
+                                      double *sum, float *max, bool first, bool finish, double count,
+                                      double *mean_out, double *max_out, hipStream_t s);
 // dst = src for n bytes (16-byte aligned buffers): 16-byte loads and stores, grid-stride.
 hipError_t launch_copy_bytes(const void *src, void *dst, uint64_t n, hipStream_t s);
-hipError_t launch_fill_cols_init(uint64_t *isum, uint32_t *imax, double *fsum, float *fmax, int ndet, hipStream_t s);
 
 }  // namespace cubo

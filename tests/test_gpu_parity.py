@@ -267,6 +267,34 @@ def test_tilings_give_the_same_bits_on_the_integer_kernel():
         assert_identical(b.com.moment_column.values, a.com.moment_column.values, f"com {t}")
 
 
+@pytest.mark.parametrize("masks", [2, 4])
+def test_both_integer_block_layouts_give_the_same_bits(monkeypatch, masks):
+    """The integer kernel's two block layouts (one block per position group with a wave per slice, rows
+    written by the kernel; or blocks per (slice, group) with a separate slice sum) give the same bits, also
+    when the scan is not a multiple of the group size and over several tiles."""
+    shape, dtype, params = synthetic.bandwidth_u16(scan=(37, 23))
+    res = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
+    res.fill_synthetic(params)
+    geos = [C.DetectorGeometry.circle(64, 64, 14), C.DetectorGeometry.annulus(64, 64, 20, 60),
+            C.DetectorGeometry.circle(40, 70, 9), C.DetectorGeometry.rectangle(10, 30, 50, 100)][:masks]
+    runs = {}
+    for layout in ("full", "split"):
+        monkeypatch.setenv("CUBO_HIP_INT_LAYOUT", layout)
+        for t in (None, C.ScanTiling.rows(7)):
+            runs[(layout, t is None)] = C.first_look(res, geos, tiling=t)
+    ref = runs[("full", True)]
+    host = res.read_scan_tile(range(0, 37)).astype(np.float32)
+    want = [O.virtual_image(host, C.DetectorMask.make(g, res.shape)) for g in geos]
+    for k, r in runs.items():
+        for i, (x, y) in enumerate(zip(ref.images, r.images)):
+            assert_identical(y.values, x.values, f"image {i} {k}")
+            assert_identical(y.values, want[i], f"image {i} {k} vs oracle")
+        assert_identical(r.mean.values, ref.mean.values, f"mean {k}")
+        assert_identical(r.max.values, ref.max.values, f"max {k}")
+        for f in ("row", "column", "intensity", "moment_row", "moment_column"):
+            assert_identical(getattr(r.com, f).values, getattr(ref.com, f).values, f"com {f} {k}")
+
+
 def test_view_and_results_agree_and_reset_reuses_the_reduction():
     """cubo_hip_reduce_view (pinned, no copy) and cubo_hip_reduce_results give the same bits; reset runs the same
     request again without reallocation and gives the same bits again."""
