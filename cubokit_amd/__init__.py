@@ -5,12 +5,12 @@ CuboKit's CuboVirtual API over it.
 """
 from .virtual import (CancellationError, CenterOfMass, CenterOfMassMaps, ComputePath, CubeShape, CuboError,
                       CuboVirtualError, DetectorGeometry, DetectorImage, DetectorMask, FirstLook, HIPResidentCube, MaskStack,
-                      StackBatch, center_of_mass_from_stack,
+                      ShiftedStack, StackBatch, center_of_mass_from_stack,
                       ResidentCubeError, ScanImage, ScanTiling, UnavailableReason, VirtualDiffraction, VirtualImaging,
                       device_count, device_info, first_look, hip_is_available, reduce_path)
 
 __all__ = ["CancellationError", "CenterOfMass", "CenterOfMassMaps", "ComputePath", "CubeShape", "CuboError",
            "CuboVirtualError", "DetectorGeometry", "DetectorImage", "DetectorMask", "FirstLook", "HIPResidentCube",
-           "MaskStack", "StackBatch", "center_of_mass_from_stack",
+           "MaskStack", "ShiftedStack", "StackBatch", "center_of_mass_from_stack",
            "ResidentCubeError", "ScanImage", "ScanTiling", "UnavailableReason", "VirtualDiffraction",
            "VirtualImaging", "device_count", "device_info", "first_look", "hip_is_available", "reduce_path"]

@@ -1,0 +1,395 @@
+// cubo_shift.hip — C ABI of the shifted mask stack and its adjoint (include/cubo_hip.h, "shifted stack").
+//
+// The host side of the bilinear shift (Sources/CuboVirtual/ShiftedImaging.swift:43-73, 109-122): the origin
+// map's mean (NumPy's pairwise order), each position's integer shift wrapped into the detector and its
+// fractions, the positions sorted by integer shift into tiles, and the rolled weight tables the tiles need.
+// The device does the rest: the matrix-core pass for Y or for the origin adjoint, and the pixel-major pass
+// for the mask adjoint, both re-reading the resident cube.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "../../include/cubo_hip.h"
+#include "cube.hpp"
+#include "errors.hpp"
+#include "shift_kernels.hpp"
+#include "stack_prep.hpp"
+
+using cubo::DeviceGuard;
+using cubo::fail;
+using cubo::hip_fail;
+
+struct cubo_hip_shift_plan {
+    cubo_hip_cube *cube;
+    int k, kp;
+    int nchunks;
+    int any_lo;
+    void *base = nullptr;            // split weights, one 32-mask tile [chunk][kstep][hi, lo][lane][8]
+    double *inv = nullptr;           // [k]
+    int64_t slot_stride;             // 16-byte entries per rolled table
+    // shift-class preparation of the last origin map
+    std::vector<double> o_row, o_col;
+    bool prepared = false;
+    int ntiles = 0, nslots = 0;
+    std::vector<int32_t> h_shift;    // [npos][2]
+    std::vector<float> h_wts;        // [npos][4]
+    std::vector<double> h_wr, h_wc;
+    void *wroll = nullptr; size_t wroll_slots = 0;
+    int32_t *perm = nullptr, *shift = nullptr, *slot_shifts = nullptr;
+    cubo::ShiftTile *tiles = nullptr; size_t tiles_cap = 0;
+    double *wr = nullptr, *wc = nullptr;
+    float *wts = nullptr;
+    // per-call buffers
+    double *y = nullptr, *dy = nullptr, *g = nullptr;    // g: [2][npos]
+    float *dy32 = nullptr;
+    double *partial = nullptr, *dmask = nullptr;
+    int ngroups = 0;
+    hipEvent_t ev[6] = {};
+    double ms[3] = {0, 0, 0};        // forward stack, origin adjoint, mask adjoint
+    int64_t calls = 0;
+};
+
+namespace {
+
+void free_plan(cubo_hip_shift_plan *p) {
+    if (!p) return;
+    DeviceGuard g(p->cube->device);
+    (void)hipStreamSynchronize(p->cube->stream);
+    void *bufs[] = {p->base, p->inv, p->wroll, p->perm, p->shift, p->slot_shifts, p->tiles, p->wr, p->wc, p->wts,
+                    p->y, p->dy, p->g, p->dy32, p->partial, p->dmask};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (hipEvent_t e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete p;
+}
+
+// NumPy's pairwise float64 sum (numpy/core/src/umath/loops_utils.h.src pairwise_sum): fewer than 8 values
+// in order; up to 128 in eight interleaved partials; above, split at a multiple of 8 and recurse.
+double pairwise_sum(const double *a, int64_t n) {
+    if (n < 8) {
+        double s = 0.0;
+        for (int64_t i = 0; i < n; ++i) s += a[i];
+        return s;
+    }
+    if (n <= 128) {
+        double r[8];
+        for (int j = 0; j < 8; ++j) r[j] = a[j];
+        int64_t i = 8;
+        for (; i < n - (n % 8); i += 8)
+            for (int j = 0; j < 8; ++j) r[j] += a[i + j];
+        double s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        for (; i < n; ++i) s += a[i];
+        return s;
+    }
+    int64_t n2 = n / 2;
+    n2 -= n2 % 8;
+    return pairwise_sum(a, n2) + pairwise_sum(a + n2, n - n2);
+}
+
+// The integer shift wrapped into [0, n) (ShiftedImaging.swift:109-113: a floor shift modulo the extent).
+int wrap_shift(double v, int n) {
+    double m = std::fmod(v, (double)n);
+    if (m < 0) m += (double)n;
+    return ((int)m) % n;
+}
+
+template <typename T>
+hipError_t grow(T **p, size_t *cap, size_t n) {
+    if (*p && *cap >= n) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) *cap = n;
+    return e;
+}
+
+// Shift classes, tiles and rolled weight tables for an origin map (skipped when it is the previous one).
+int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol) {
+    cubo_hip_cube *c = p->cube;
+    const int64_t npos = c->npos;
+    const int qy = (int)c->shape[2], qx = (int)c->shape[3];
+    if (p->prepared && std::equal(orow, orow + npos, p->o_row.begin()) && std::equal(ocol, ocol + npos, p->o_col.begin()))
+        return CUBO_HIP_OK;
+    for (int64_t i = 0; i < npos; ++i)
+        if (!std::isfinite(orow[i]) || !std::isfinite(ocol[i]))
+            return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the origin map must be finite (position %lld)", (long long)i);
+    const double mr = pairwise_sum(orow, npos) / (double)npos, mc = pairwise_sum(ocol, npos) / (double)npos;
+    p->h_shift.resize(2 * npos);
+    p->h_wts.resize(4 * npos);
+    p->h_wr.resize(npos);
+    p->h_wc.resize(npos);
+    std::vector<int32_t> key(npos);
+    for (int64_t i = 0; i < npos; ++i) {
+        const double dr = orow[i] - mr, dc = ocol[i] - mc;
+        const double fr = std::floor(dr), fc = std::floor(dc);
+        const double wr = dr - fr, wc = dc - fc;
+        const int ir = wrap_shift(fr, qy), ic = wrap_shift(fc, qx);
+        p->h_shift[2 * i] = ir;
+        p->h_shift[2 * i + 1] = ic;
+        p->h_wr[i] = wr;
+        p->h_wc[i] = wc;
+        p->h_wts[4 * i] = (float)((1 - wr) * (1 - wc));
+        p->h_wts[4 * i + 1] = (float)(wr * (1 - wc));
+        p->h_wts[4 * i + 2] = (float)((1 - wr) * wc);
+        p->h_wts[4 * i + 3] = (float)(wr * wc);
+        key[i] = ir * qx + ic;
+    }
+    // counting sort by class
+    std::vector<int64_t> count((size_t)qy * qx + 1, 0);
+    for (int64_t i = 0; i < npos; ++i) ++count[key[i] + 1];
+    for (size_t i = 1; i < count.size(); ++i) count[i] += count[i - 1];
+    std::vector<int32_t> perm(npos);
+    {
+        std::vector<int64_t> at(count.begin(), count.end() - 1);
+        for (int64_t i = 0; i < npos; ++i) perm[at[key[i]]++] = (int32_t)i;
+    }
+    constexpr int TP = 32 * cubo::kShiftWaves;
+    std::vector<cubo::ShiftTile> tiles;
+    std::map<int, int> slot_of;
+    std::vector<int32_t> slot_shifts;
+    auto slot = [&](int tr, int tc) {
+        const int kk = tr * qx + tc;
+        auto it = slot_of.find(kk);
+        if (it != slot_of.end()) return it->second;
+        const int s = (int)slot_shifts.size() / 2;
+        slot_of[kk] = s;
+        slot_shifts.push_back(tr);
+        slot_shifts.push_back(tc);
+        return s;
+    };
+    for (int kk = 0; kk < qy * qx; ++kk) {
+        const int64_t b = count[kk], e = count[kk + 1];
+        if (b == e) continue;
+        const int ir = kk / qx, ic = kk % qx;
+        const int s0 = slot(ir, ic), s1 = slot((ir + 1) % qy, ic), s2 = slot(ir, (ic + 1) % qx),
+                  s3 = slot((ir + 1) % qy, (ic + 1) % qx);
+        for (int64_t t = b; t < e; t += TP) {
+            cubo::ShiftTile tl{};
+            tl.start = (int32_t)t;
+            tl.count = (int32_t)std::min<int64_t>(TP, e - t);
+            tl.slot[0] = s0; tl.slot[1] = s1; tl.slot[2] = s2; tl.slot[3] = s3;
+            tiles.push_back(tl);
+        }
+    }
+    const int nslots = (int)slot_shifts.size() / 2;
+    DeviceGuard g(c->device);
+    hipError_t e = hipSuccess;
+    size_t cap_tiles = p->tiles_cap;
+    if (!p->perm) {
+        e = hipMalloc((void **)&p->perm, npos * 4);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->shift, npos * 8);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->wr, npos * 8);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->wc, npos * 8);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->wts, npos * 16);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->slot_shifts, (size_t)qy * qx * 8);
+        if (e != hipSuccess) return hip_fail(e, "allocating the shift classes");
+    }
+    e = grow(&p->tiles, &cap_tiles, tiles.size());
+    p->tiles_cap = cap_tiles;
+    if (e == hipSuccess) e = grow((char **)&p->wroll, &p->wroll_slots, (size_t)nslots * p->slot_stride * 16);
+    if (e != hipSuccess) return hip_fail(e, "allocating the rolled weights");
+    hipStream_t s = c->stream;
+    e = hipMemcpyAsync(p->perm, perm.data(), npos * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->shift, p->h_shift.data(), npos * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->wr, p->h_wr.data(), npos * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->wc, p->h_wc.data(), npos * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->wts, p->h_wts.data(), npos * 16, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->tiles, tiles.data(), tiles.size() * sizeof(cubo::ShiftTile), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->slot_shifts, slot_shifts.data(), slot_shifts.size() * 4,
+                                            hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = cubo::launch_roll_weights(p->base, p->wroll, p->slot_shifts, nslots, p->nchunks, c->ndet, qy,
+                                                       qx, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);      // the host vectors above may change on the next call
+    if (e != hipSuccess) return hip_fail(e, "preparing the shift classes");
+    p->ntiles = (int)tiles.size();
+    p->nslots = nslots;
+    p->o_row.assign(orow, orow + npos);
+    p->o_col.assign(ocol, ocol + npos);
+    p->prepared = true;
+    return CUBO_HIP_OK;
+}
+
+cubo::ShiftLaunch launch_args(const cubo_hip_shift_plan *p) {
+    const cubo_hip_cube *c = p->cube;
+    cubo::ShiftLaunch a{};
+    a.cube = (const uint16_t *)c->data;
+    a.npos = c->npos;
+    a.ndet = c->ndet;
+    a.nchunks = p->nchunks;
+    a.k = p->k;
+    a.any_lo = p->any_lo;
+    a.wroll = p->wroll;
+    a.slot_stride = p->slot_stride;
+    a.inv_scale = p->inv;
+    a.tiles = p->tiles;
+    a.perm = p->perm;
+    a.wr = p->wr;
+    a.wc = p->wc;
+    return a;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms : 0.f;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cubo_hip_shift_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, cubo_hip_shift_plan **out) {
+    if (!c || !masks || !out) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (k < 1 || k > 32) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "%d masks (1..32)", k);
+    if (c->dtype != CUBO_HIP_U16 || c->ndet % 8 != 0)
+        return fail(CUBO_HIP_ERR_UNSUPPORTED, "the shifted stack needs a uint16 cube whose pattern length is a multiple of 8");
+    for (int64_t i = 0; i < (int64_t)k * c->ndet; ++i)
+        if (!std::isfinite(masks[i])) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "a mask needs finite weights");
+    auto *p = new (std::nothrow) cubo_hip_shift_plan();
+    if (!p) return fail(CUBO_HIP_ERR_OUT_OF_MEMORY, "host allocation");
+    p->cube = c;
+    p->k = k;
+    p->kp = k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : 32;
+    DeviceGuard g(c->device);
+    cubo::SplitWeights sw = cubo::split_weights(masks, k, c->ndet);
+    p->nchunks = sw.nchunks;
+    p->any_lo = sw.tile_lo[0];
+    p->slot_stride = (int64_t)sw.nchunks * 4 * 2 * 64;
+    hipError_t e = hipMalloc(&p->base, sw.wb.size() * sizeof(_Float16));
+    if (e == hipSuccess) e = hipMemcpy(p->base, sw.wb.data(), sw.wb.size() * sizeof(_Float16), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->inv, k * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(p->inv, sw.inv.data(), k * sizeof(double), hipMemcpyHostToDevice);
+    for (hipEvent_t &ev : p->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e != hipSuccess) { free_plan(p); return hip_fail(e, "preparing the shifted stack"); }
+    *out = p;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_shift_forward(cubo_hip_shift_plan *p, const double *origin_row, const double *origin_col, double *y) {
+    if (!p || !origin_row || !origin_col || !y) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (int st = prepare(p, origin_row, origin_col)) return st;
+    cubo_hip_cube *c = p->cube;
+    DeviceGuard g(c->device);
+    const size_t ny = (size_t)c->npos * p->k;
+    if (!p->y) CUBO_HIP_TRY(hipMalloc((void **)&p->y, ny * 8), "allocating the stack output");
+    cubo::ShiftLaunch a = launch_args(p);
+    a.y = p->y;
+    CUBO_HIP_TRY(hipEventRecord(p->ev[0], c->stream), "hipEventRecord");
+    CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, false, c->stream), "launching the shifted stack");
+    CUBO_HIP_TRY(hipEventRecord(p->ev[1], c->stream), "hipEventRecord");
+    CUBO_HIP_TRY(hipMemcpyAsync(y, p->y, ny * 8, hipMemcpyDeviceToHost, c->stream), "reading the stack");
+    CUBO_HIP_TRY(hipStreamSynchronize(c->stream), "waiting for the shifted stack");
+    p->ms[0] += elapsed(p->ev[0], p->ev[1]);
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, const double *origin_col, const double *dy,
+                            double *dmasks, double *d_origin_row, double *d_origin_col) {
+    if (!p || !origin_row || !origin_col || !dy) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if ((d_origin_row == nullptr) != (d_origin_col == nullptr))
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "ask for both origin gradients or neither");
+    cubo_hip_cube *c = p->cube;
+    const int64_t npos = c->npos, ndet = c->ndet;
+    for (int64_t i = 0; i < npos * p->k; ++i)
+        if (!std::isfinite(dy[i])) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "dL/dY must be finite");
+    if (int st = prepare(p, origin_row, origin_col)) return st;
+    DeviceGuard g(c->device);
+    hipStream_t s = c->stream;
+    const size_t ny = (size_t)npos * p->k;
+    if (!p->dy) CUBO_HIP_TRY(hipMalloc((void **)&p->dy, ny * 8), "allocating dL/dY");
+    CUBO_HIP_TRY(hipMemcpyAsync(p->dy, dy, ny * 8, hipMemcpyHostToDevice, s), "uploading dL/dY");
+    if (d_origin_row) {
+        if (!p->g) CUBO_HIP_TRY(hipMalloc((void **)&p->g, (size_t)npos * 16), "allocating the origin gradient");
+        cubo::ShiftLaunch a = launch_args(p);
+        a.dy = p->dy;
+        a.gr = p->g;
+        a.gc = p->g + npos;
+        CUBO_HIP_TRY(hipEventRecord(p->ev[2], s), "hipEventRecord");
+        CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, true, s), "launching the origin adjoint");
+        CUBO_HIP_TRY(hipEventRecord(p->ev[3], s), "hipEventRecord");
+    }
+    if (dmasks) {
+        // dL/dY in FP32, padded to the kernel's mask count
+        std::vector<float> d32((size_t)npos * p->kp, 0.f);
+        for (int64_t i = 0; i < npos; ++i)
+            for (int kk = 0; kk < p->k; ++kk) d32[(size_t)i * p->kp + kk] = (float)dy[(size_t)i * p->k + kk];
+        const int cus = cubo::compute_units(c->device);
+        const int64_t pixel_blocks = (ndet + 255) / 256;
+        const int ngroups = (int)std::max<int64_t>(1, std::min<int64_t>({(4 * cus + pixel_blocks - 1) / pixel_blocks,
+                                                                         256, npos}));
+        if (!p->dy32) CUBO_HIP_TRY(hipMalloc((void **)&p->dy32, d32.size() * 4), "allocating dL/dY");
+        if (ngroups > p->ngroups) {
+            if (p->partial) (void)hipFree(p->partial);
+            p->partial = nullptr;
+            CUBO_HIP_TRY(hipMalloc((void **)&p->partial, (size_t)ngroups * p->kp * ndet * 8), "allocating partials");
+            p->ngroups = ngroups;
+        }
+        if (!p->dmask) CUBO_HIP_TRY(hipMalloc((void **)&p->dmask, (size_t)p->kp * ndet * 8), "allocating dL/dmasks");
+        CUBO_HIP_TRY(hipMemcpyAsync(p->dy32, d32.data(), d32.size() * 4, hipMemcpyHostToDevice, s), "uploading dL/dY");
+        cubo::ShiftGradW gw{};
+        gw.cube = (const uint16_t *)c->data;
+        gw.npos = npos;
+        gw.qy = (int32_t)c->shape[2];
+        gw.qx = (int32_t)c->shape[3];
+        gw.kp = p->kp;
+        gw.shift = p->shift;
+        gw.wts = p->wts;
+        gw.dy = p->dy32;
+        gw.partial = p->partial;
+        gw.ngroups = ngroups;
+        gw.ppg = (npos + ngroups - 1) / ngroups;
+        CUBO_HIP_TRY(hipEventRecord(p->ev[4], s), "hipEventRecord");
+        CUBO_HIP_TRY(cubo::launch_shift_grad_w(gw, s), "launching the mask adjoint");
+        CUBO_HIP_TRY(cubo::launch_sum_groups(p->partial, ngroups, (int64_t)p->kp * ndet, p->dmask, s),
+                     "launching the mask adjoint sum");
+        CUBO_HIP_TRY(hipEventRecord(p->ev[5], s), "hipEventRecord");
+        CUBO_HIP_TRY(hipMemcpyAsync(dmasks, p->dmask, (size_t)p->k * ndet * 8, hipMemcpyDeviceToHost, s),
+                     "reading dL/dmasks");
+    }
+    std::vector<double> hg;
+    if (d_origin_row) {
+        hg.resize(2 * npos);
+        CUBO_HIP_TRY(hipMemcpyAsync(hg.data(), p->g, (size_t)npos * 16, hipMemcpyDeviceToHost, s),
+                     "reading the origin gradient");
+    }
+    CUBO_HIP_TRY(hipStreamSynchronize(s), "waiting for the adjoint");
+    if (d_origin_row) {
+        // d = origin - mean(origin): dL/d origin_q = g_q - mean(g)
+        double mr = 0, mc = 0;
+        for (int64_t i = 0; i < npos; ++i) { mr += hg[i]; mc += hg[npos + i]; }
+        mr /= (double)npos;
+        mc /= (double)npos;
+        for (int64_t i = 0; i < npos; ++i) {
+            d_origin_row[i] = hg[i] - mr;
+            d_origin_col[i] = hg[npos + i] - mc;
+        }
+        p->ms[1] += elapsed(p->ev[2], p->ev[3]);
+    }
+    if (dmasks) p->ms[2] += elapsed(p->ev[4], p->ev[5]);
+    ++p->calls;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_shift_kernel_time(cubo_hip_shift_plan *p, double *ms3, int64_t *calls) {
+    if (!p || !ms3 || !calls) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int i = 0; i < 3; ++i) { ms3[i] = p->ms[i]; p->ms[i] = 0; }
+    *calls = p->calls;
+    p->calls = 0;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_shift_plan_destroy(cubo_hip_shift_plan *p) {
+    free_plan(p);
+    return CUBO_HIP_OK;
+}
+
+}  // extern "C"

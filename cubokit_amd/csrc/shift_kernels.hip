@@ -1,0 +1,336 @@
+// shift_kernels.hip — the shifted mask stack and its adjoint (the bilinear shift of
+// Sources/CuboVirtual/ShiftedImaging.swift:43-73, 109-122, differentiated by hand).
+//
+// Forward and origin adjoint: one matrix-core pass over the cube. A tile holds positions of one integer
+// shift class, so the four corner contractions C_c = X * roll(W, s_c) share their weights across the tile:
+// the data are the exact uint16 split (x & 0xF800) + (x & 0x07FF), the rolled weights the split-FP16 hi/lo
+// of the power-of-two scaled masks, the products exact in FP32, the corner sums flushed to FP64 every 256
+// pixels and combined there:
+//   forward   Y   = sum_c w_c C_c
+//   backward  dY/dwr = (1 - wc)(C_1 - C_0) + wc (C_3 - C_2),  dY/dwc = (1 - wr)(C_2 - C_0) + wr (C_3 - C_1),
+//             g_r(p) = sum_k dL/dY[p, k] dY/dwr[p, k]   (d wr / d origin_r = 1 inside a bilinear cell).
+// Mask adjoint: a second pass, pixel-major on the vector ALUs (see shift_grad_w).
+#include "device_sync.hpp"
+#include "shift_kernels.hpp"
+
+namespace cubo {
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float float16v __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+#define CUBO_GLOBAL __attribute__((address_space(1)))
+
+constexpr int kBK = 64;            // pixels per chunk
+constexpr int kFlush = 4;          // chunks per FP64 flush (256 pixels)
+constexpr int kCorners = 4;
+constexpr int kTileEntries = 4 * 2 * 64;   // 16-byte weight entries of one chunk of one 32-mask tile
+
+__device__ __forceinline__ uint32_t cvt_pair(uint32_t w) {
+    const us2 u = __builtin_bit_cast(us2, w);
+    const h2 h = __builtin_convertvector(u, h2);
+    return __builtin_bit_cast(uint32_t, h);
+}
+
+template <int WAVES, bool BWD>
+__global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
+    constexpr int WCH = kCorners * kTileEntries;             // entries of one chunk, four corners
+    constexpr int TP = 32 * WAVES;
+    constexpr int WPT = WCH / (64 * WAVES);
+    static_assert(WCH % (64 * WAVES) == 0, "every thread carries the same number of weight entries");
+    __shared__ half8 wlds[2][WCH];
+    __shared__ double wts[2][TP];
+
+    const ShiftTile t = a.tiles[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int q = wave * 32 + (lane & 31);                     // this lane's position slot (A-operand row)
+    const bool live = q < t.count;
+    const int64_t p = a.perm[t.start + (live ? q : 0)];
+    const int nchunks = a.nchunks;
+    const CUBO_GLOBAL uint16_t *xrow = (const CUBO_GLOBAL uint16_t *)a.cube + p * a.ndet;
+    const CUBO_GLOBAL half8 *wsrc = (const CUBO_GLOBAL half8 *)a.wroll;
+
+    for (int i = tid; i < TP; i += 64 * WAVES) {
+        const int64_t pp = a.perm[t.start + min(i, t.count - 1)];
+        wts[0][i] = a.wr[pp];
+        wts[1][i] = a.wc[pp];
+    }
+
+    // Unconditional loads (a pixel past the pattern reads the row start; its weights are zero), as in the
+    // mask-stack kernel: a load under a lane mask would make every wait drain the prefetch.
+    auto load_x = [&](u32x4 (&x)[4], int chunk) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int64_t px = (int64_t)chunk * kBK + 32 * (lane >> 5) + 8 * s;
+            x[s] = *(const CUBO_GLOBAL u32x4 *)(xrow + (px < a.ndet ? px : 0));
+        }
+    };
+    // Entry e of a chunk's block copy belongs to corner e / kTileEntries (a compile-time constant per j).
+    half8 wreg[WPT];
+    auto fetch_w = [&](int chunk) {
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+            const int e = tid + j * 64 * WAVES;
+            const int corner = (j * 64 * WAVES) / kTileEntries + (64 * WAVES >= kTileEntries ? 0 : tid / kTileEntries);
+            wreg[j] = wsrc[(size_t)t.slot[corner] * a.slot_stride + (size_t)chunk * kTileEntries + (e % kTileEntries)];
+        }
+    };
+    auto put_w = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) wlds[buf][tid + j * 64 * WAVES] = wreg[j];
+    };
+
+    float16v acc[kCorners];
+    double accd[BWD ? 2 : 1][16];
+#pragma unroll
+    for (int c = 0; c < kCorners; ++c)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[c][r] = 0.f;
+#pragma unroll
+    for (int m = 0; m < (BWD ? 2 : 1); ++m)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) accd[m][r] = 0.0;
+
+    const bool any_lo = a.any_lo != 0;
+    // Register r of lane l holds row 8 (r / 4) + 4 (l / 32) + r % 4 of the wave's 32 positions.
+    auto flush = [&]() {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const double wr = wts[0][row], wc = wts[1][row];
+            const double c0 = acc[0][r], c1 = acc[1][r], c2 = acc[2][r], c3 = acc[3][r];
+            if constexpr (BWD) {
+                accd[0][r] += (1.0 - wc) * (c1 - c0) + wc * (c3 - c2);
+                accd[1][r] += (1.0 - wr) * (c2 - c0) + wr * (c3 - c1);
+            } else {
+                accd[0][r] += (1.0 - wr) * (1.0 - wc) * c0 + wr * (1.0 - wc) * c1 + (1.0 - wr) * wc * c2 + wr * wc * c3;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kCorners; ++c)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[c][r] = 0.f;
+    };
+    auto compute = [&](const u32x4 (&x)[4], int chunk) {
+        uint32_t any = 0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) any |= (x[s].x | x[s].y | x[s].z | x[s].w) & 0xf800f800u;
+        const bool use_hi = __builtin_amdgcn_ballot_w64(any != 0) != 0;
+        const half8 *wl = wlds[chunk & 1];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            half8 bh[kCorners], bl[kCorners];
+#pragma unroll
+            for (int c = 0; c < kCorners; ++c) {
+                bh[c] = wl[c * kTileEntries + (s * 2 + 0) * 64 + lane];
+                bl[c] = wl[c * kTileEntries + (s * 2 + 1) * 64 + lane];
+            }
+            const uint32_t w[4] = {x[s].x, x[s].y, x[s].z, x[s].w};
+            u32x4 lo, hi;
+            lo.x = cvt_pair(w[0] & 0x07ff07ffu); lo.y = cvt_pair(w[1] & 0x07ff07ffu);
+            lo.z = cvt_pair(w[2] & 0x07ff07ffu); lo.w = cvt_pair(w[3] & 0x07ff07ffu);
+            const half8 alo = __builtin_bit_cast(half8, lo);
+#pragma unroll
+            for (int c = 0; c < kCorners; ++c) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bh[c], acc[c], 0, 0, 0);
+            if (any_lo) {
+#pragma unroll
+                for (int c = 0; c < kCorners; ++c) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bl[c], acc[c], 0, 0, 0);
+            }
+            if (use_hi) {
+                hi.x = cvt_pair(w[0] & 0xf800f800u); hi.y = cvt_pair(w[1] & 0xf800f800u);
+                hi.z = cvt_pair(w[2] & 0xf800f800u); hi.w = cvt_pair(w[3] & 0xf800f800u);
+                const half8 ahi = __builtin_bit_cast(half8, hi);
+#pragma unroll
+                for (int c = 0; c < kCorners; ++c)
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bh[c], acc[c], 0, 0, 0);
+                if (any_lo) {
+#pragma unroll
+                    for (int c = 0; c < kCorners; ++c)
+                        acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bl[c], acc[c], 0, 0, 0);
+                }
+            }
+        }
+        if ((chunk + 1) % kFlush == 0 || chunk + 1 == nchunks) flush();
+    };
+
+    // Same pipeline as the mask stack: weights one chunk ahead through LDS (double-buffered, one barrier per
+    // chunk), data two chunks ahead in registers; weight loads issued before data loads.
+    u32x4 X0[4], X1[4], X2[4];
+    fetch_w(0);
+    __builtin_amdgcn_sched_barrier(0);
+    load_x(X0, 0);
+    load_x(X1, min(1, nchunks - 1));
+    put_w(0);
+    __syncthreads();                                   // also publishes wts
+    auto step = [&](const u32x4 (&xc)[4], u32x4 (&xn)[4], int c) {
+        fetch_w(min(c + 1, nchunks - 1));
+        __builtin_amdgcn_sched_barrier(0);
+        load_x(xn, min(c + 2, nchunks - 1));
+        compute(xc, c);
+        put_w((c + 1) & 1);
+        lds_barrier();
+    };
+    int c = 0;
+    for (; c + 3 <= nchunks; c += 3) {
+        step(X0, X2, c);
+        step(X1, X0, c + 1);
+        step(X2, X1, c + 2);
+    }
+    if (c < nchunks) step(X0, X2, c);
+    if (c + 1 < nchunks) step(X1, X0, c + 1);
+
+    const int col = lane & 31;
+    const double inv = col < a.k ? ((const CUBO_GLOBAL double *)a.inv_scale)[col] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const bool rl = row < t.count;
+        const int64_t pr = a.perm[t.start + (rl ? row : 0)];
+        if constexpr (BWD) {
+            double g0 = 0.0, g1 = 0.0;
+            if (rl && col < a.k) {
+                const double d = ((const CUBO_GLOBAL double *)a.dy)[pr * a.k + col] * inv;
+                g0 = d * accd[0][r];
+                g1 = d * accd[1][r];
+            }
+            // sum over the 32 mask columns (the lanes of this half-wave), in a fixed tree order
+#pragma unroll
+            for (int off = 1; off < 32; off <<= 1) {
+                g0 += __shfl_xor(g0, off, 64);
+                g1 += __shfl_xor(g1, off, 64);
+            }
+            if (rl && col == 0) {
+                ((CUBO_GLOBAL double *)a.gr)[pr] = g0;
+                ((CUBO_GLOBAL double *)a.gc)[pr] = g1;
+            }
+        } else {
+            if (rl && col < a.k) ((CUBO_GLOBAL double *)a.y)[pr * a.k + col] = accd[0][r] * inv;
+        }
+    }
+}
+
+__global__ void roll_weights(const half8 *base, half8 *out, const int32_t *shifts, int nchunks, int64_t ndet, int qy,
+                             int qx) {
+    // one thread per (slot, 16-byte entry); the entry's 8 halves are gathered one by one
+    const int64_t per_slot = (int64_t)nchunks * kTileEntries;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int slot = blockIdx.y;
+    if (e >= per_slot) return;
+    const int tr = shifts[2 * slot], tc = shifts[2 * slot + 1];
+    const int chunk = (int)(e / kTileEntries), rem = (int)(e % kTileEntries);
+    const int st = rem / 128, hl = (rem / 64) & 1, l = rem & 63;
+    const _Float16 *b16 = reinterpret_cast<const _Float16 *>(base);
+    half8 v;
+#pragma unroll
+    for (int el = 0; el < 8; ++el) {
+        const int64_t u = (int64_t)chunk * kBK + 32 * (l >> 5) + 8 * st + el;
+        _Float16 h = (_Float16)0.f;
+        if (u < ndet) {
+            const int ui = (int)(u / qx), uj = (int)(u % qx);
+            int si = ui - tr, sj = uj - tc;
+            si += si < 0 ? qy : 0;
+            sj += sj < 0 ? qx : 0;
+            const int64_t su = (int64_t)si * qx + sj;
+            const int sch = (int)(su / kBK), sh = (int)((su % kBK) / 32), sst = (int)((su % 32) / 8), sel = (int)(su % 8);
+            const int sl = sh * 32 + (l & 31);
+            h = b16[((((int64_t)sch * 4 + sst) * 2 + hl) * 64 + sl) * 8 + sel];
+        }
+        v[el] = h;
+    }
+    out[(int64_t)slot * per_slot + e] = v;
+}
+
+// One thread per pixel v = (a, b); the block's positions are uniform, so their shift, corner weights and
+// dL/dY come in scalar registers.
+template <int KP>
+__global__ __launch_bounds__(256) void shift_grad_w(ShiftGradW g) {
+    const int qy = g.qy, qx = g.qx;
+    const int64_t ndet = (int64_t)qy * qx;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int group = blockIdx.y;
+    const bool live = v < ndet;
+    const int va = live ? (int)(v / qx) : 0, vb = live ? (int)(v % qx) : 0;
+    const int64_t p0 = (int64_t)group * g.ppg;
+    const int64_t p1 = min(g.npos, p0 + g.ppg);
+    const int32_t *__restrict__ shift = g.shift;
+    const float *__restrict__ wts = g.wts;
+    const float *__restrict__ dy = g.dy;
+
+    float acc[KP];
+    double accd[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) { acc[k] = 0.f; accd[k] = 0.0; }
+    int n = 0;
+    for (int64_t p = p0; p < p1; ++p) {
+        const int ir = shift[2 * p], ic = shift[2 * p + 1];
+        int r0 = va + ir; r0 -= r0 >= qy ? qy : 0;
+        int r1 = r0 + 1;  r1 -= r1 >= qy ? qy : 0;
+        int c0 = vb + ic; c0 -= c0 >= qx ? qx : 0;
+        int c1 = c0 + 1;  c1 -= c1 >= qx ? qx : 0;
+        const uint16_t *x = g.cube + p * ndet;
+        const float x00 = x[(int64_t)r0 * qx + c0], x10 = x[(int64_t)r1 * qx + c0];
+        const float x01 = x[(int64_t)r0 * qx + c1], x11 = x[(int64_t)r1 * qx + c1];
+        const float xs = wts[4 * p] * x00 + wts[4 * p + 1] * x10 + wts[4 * p + 2] * x01 + wts[4 * p + 3] * x11;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) acc[k] = __builtin_fmaf(xs, dy[p * KP + k], acc[k]);
+        if (++n == 16) {
+#pragma unroll
+            for (int k = 0; k < KP; ++k) { accd[k] += (double)acc[k]; acc[k] = 0.f; }
+            n = 0;
+        }
+    }
+    if (!live) return;
+#pragma unroll
+    for (int k = 0; k < KP; ++k) g.partial[((int64_t)group * KP + k) * ndet + v] = accd[k] + (double)acc[k];
+}
+
+__global__ void sum_groups(const double *partial, int ngroups, int64_t n, double *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int gi = 0; gi < ngroups; ++gi) s += partial[(int64_t)gi * n + i];
+    out[i] = s;
+}
+
+}  // namespace
+
+hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    if (backward)
+        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, true>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
+    else
+        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_roll_weights(const void *base, void *out, const int32_t *shifts, int nslots, int nchunks, int64_t ndet,
+                               int qy, int qx, hipStream_t s) {
+    if (nslots <= 0) return hipSuccess;
+    const int64_t per_slot = (int64_t)nchunks * kTileEntries;
+    const dim3 grid((unsigned)((per_slot + 255) / 256), (unsigned)nslots);
+    hipLaunchKernelGGL(roll_weights, grid, dim3(256), 0, s, (const half8 *)base, (half8 *)out, shifts, nchunks, ndet, qy,
+                       qx);
+    return hipGetLastError();
+}
+
+hipError_t launch_shift_grad_w(const ShiftGradW &a, hipStream_t s) {
+    const int64_t ndet = (int64_t)a.qy * a.qx;
+    const dim3 grid((unsigned)((ndet + 255) / 256), (unsigned)a.ngroups);
+    switch (a.kp) {
+        case 4: hipLaunchKernelGGL(shift_grad_w<4>, grid, dim3(256), 0, s, a); break;
+        case 8: hipLaunchKernelGGL(shift_grad_w<8>, grid, dim3(256), 0, s, a); break;
+        case 16: hipLaunchKernelGGL(shift_grad_w<16>, grid, dim3(256), 0, s, a); break;
+        case 32: hipLaunchKernelGGL(shift_grad_w<32>, grid, dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_groups(const double *partial, int ngroups, int64_t n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(sum_groups, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, ngroups, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace cubo

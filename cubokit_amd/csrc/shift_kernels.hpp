@@ -1,0 +1,70 @@
+// shift_kernels.hpp — launch interface of the shifted mask stack and its adjoint (shift_kernels.hip).
+//
+// Per position p the masks are moved by the bilinear shift of MaskShift.bilinear
+// (Sources/CuboVirtual/ShiftedImaging.swift:43-73): with d = origin_p - mean(origin), integer parts
+// (ir, ic) = floor(d) wrapped into the detector and fractions (wr, wc), the shifted mask is the blend of the
+// four periodic rolls of the mask by the corner shifts s0 = (ir, ic), s1 = (ir + 1, ic), s2 = (ir, ic + 1),
+// s3 = (ir + 1, ic + 1) with weights (1-wr)(1-wc), wr(1-wc), (1-wr)wc, wr wc. So
+//     Y[p, k] = sum_c w_c(p) C_c[p, k],   C_c[p, k] = sum_u x_p[u] m_k[(u - s_c) mod (qy, qx)],
+// and every C_c is an ordinary mask-stack contraction with a rolled mask. Positions are sorted by (ir, ic)
+// into tiles of one class, so a tile multiplies the exact uint16 data by four rolled copies of the split-FP16
+// weights, on the matrix cores.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cubo {
+
+// A tile: up to 32 * WAVES positions of one shift class, perm[start .. start + count).
+struct ShiftTile {
+    int32_t start, count;
+    int32_t slot[4];          // rolled weight tables of the four corners
+};
+
+struct ShiftLaunch {
+    const uint16_t *cube;
+    int64_t npos, ndet;       // ndet: multiple of 8
+    int32_t nchunks;          // ceil(ndet / 64)
+    int32_t k;                // masks (at most 32)
+    int32_t any_lo;           // some weight needs its FP16 lo part
+    const void *wroll;        // [slot][chunk][kstep][hi, lo][lane][8] FP16, the split weights rolled by the slot's shift
+    int64_t slot_stride;      // 16-byte entries per slot
+    const double *inv_scale;  // [k] 1 / (power-of-two column scale)
+    const ShiftTile *tiles;
+    const int32_t *perm;      // positions sorted by shift class
+    const double *wr, *wc;    // [npos] bilinear fractions
+    double *y;                // forward: [npos][k]
+    const double *dy;         // backward: [npos][k] dL/dY
+    double *gr, *gc;          // backward: [npos] dL/d(shift row), dL/d(shift column), before the mean removal
+};
+
+// Tiles of 32 * kShiftWaves positions.
+constexpr int kShiftWaves = 8;
+hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, hipStream_t s);
+
+// out[slot] = base rolled by shift[slot] = (tr, tc): the entry of pixel u holds the base weight of pixel
+// ((ui - tr) mod qy, (uj - tc) mod qx). base is split_weights' one-tile layout.
+hipError_t launch_roll_weights(const void *base, void *out, const int32_t *shifts, int nslots, int nchunks, int64_t ndet,
+                               int qy, int qx, hipStream_t s);
+
+// dL/dm[k][v] = sum_p dY[p, k] xs_p[v], xs_p[v] = sum_c w_c(p) x_p[(v + s_c) mod (qy, qx)] (the pattern moved back
+// by the shift, bilinearly): one thread per pixel, position groups in parallel, FP32 within 16 positions, FP64
+// across them; partial[g][k][v] per group g.
+struct ShiftGradW {
+    const uint16_t *cube;
+    int64_t npos;
+    int32_t qy, qx;
+    int32_t kp;               // masks padded to the instantiation (4, 8, 16 or 32)
+    const int32_t *shift;     // [npos][2] (ir, ic)
+    const float *wts;         // [npos][4] corner weights
+    const float *dy;          // [npos][kp] dL/dY in FP32
+    double *partial;          // [ngroups][kp][ndet]
+    int32_t ngroups;
+    int64_t ppg;              // positions per group
+};
+hipError_t launch_shift_grad_w(const ShiftGradW &a, hipStream_t s);
+// out[i] = sum_g partial[g][i] in group order, i < n.
+hipError_t launch_sum_groups(const double *partial, int ngroups, int64_t n, double *out, hipStream_t s);
+
+}  // namespace cubo

@@ -701,6 +701,65 @@ class StackBatch:
             pass
 
 
+class ShiftedStack:
+    """The mask stack with every position's masks moved by MaskShift.bilinear, and its adjoint
+    (include/cubo_hip.h, "shifted stack"; Sources/CuboVirtual/ShiftedImaging.swift:43-73).
+
+    forward(origin_row, origin_col) -> Y (scanCount, K); backward(...) -> (dL/dmasks (K, detectorCount),
+    dL/dorigin_row, dL/dorigin_col) for a given dL/dY, re-reading the resident cube."""
+
+    def __init__(self, cube: HIPResidentCube, masks: np.ndarray):
+        self.cube = cube
+        self.masks = np.ascontiguousarray(masks, np.float32).reshape(-1, cube.shape.detector_count)
+        self.k = self.masks.shape[0]
+        h = ct.c_void_p()
+        _check(L.lib().cubo_hip_shift_plan_create(cube.handle, self.masks.ctypes.data, self.k, ct.byref(h)),
+               "preparing the shifted stack")
+        self._h = h
+
+    def _origins(self, origin_row, origin_col):
+        n = self.cube.shape.scan_count
+        r = np.ascontiguousarray(origin_row, np.float64).reshape(n)
+        c = np.ascontiguousarray(origin_col, np.float64).reshape(n)
+        return r, c
+
+    def forward(self, origin_row, origin_col) -> np.ndarray:
+        r, c = self._origins(origin_row, origin_col)
+        y = np.empty((self.cube.shape.scan_count, self.k))
+        _check(L.lib().cubo_hip_shift_forward(self._h, r.ctypes.data, c.ctypes.data, y.ctypes.data), "shifted stack")
+        return y
+
+    def backward(self, origin_row, origin_col, dy, masks: bool = True, origin: bool = True):
+        r, c = self._origins(origin_row, origin_col)
+        n = self.cube.shape.scan_count
+        d = np.ascontiguousarray(dy, np.float64).reshape(n, self.k)
+        dm = np.empty((self.k, self.cube.shape.detector_count)) if masks else None
+        dr = np.empty(n) if origin else None
+        dc = np.empty(n) if origin else None
+        ptr = (lambda a: a.ctypes.data if a is not None else None)
+        _check(L.lib().cubo_hip_shift_backward(self._h, r.ctypes.data, c.ctypes.data, d.ctypes.data, ptr(dm), ptr(dr),
+                                               ptr(dc)), "shifted stack adjoint")
+        return dm, dr, dc
+
+    def kernel_time(self):
+        """([forward, origin adjoint, mask adjoint] device ms, backward calls) since the previous call."""
+        ms = (ct.c_double * 3)()
+        n = ct.c_int64()
+        _check(L.lib().cubo_hip_shift_kernel_time(self._h, ms, ct.byref(n)), "shift kernel time")
+        return list(ms), n.value
+
+    def release(self):
+        if getattr(self, "_h", None) is not None:
+            L.lib().cubo_hip_shift_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
 def center_of_mass_from_stack(cube_shape: CubeShape, y: np.ndarray, columns=(0, 1, 2)) -> CenterOfMassMaps:
     """The centre of mass from the stack's {w, w*q_row, w*q_col} columns; the division in FP64."""
     s, mr, mc = (np.ascontiguousarray(y[:, c], np.float64) for c in columns)

@@ -191,6 +191,26 @@ int cubo_hip_batch_run(cubo_hip_batch *batch, double *const *y);
 int cubo_hip_batch_time(cubo_hip_batch *batch, double *total_ms, int64_t *runs);
 int cubo_hip_batch_destroy(cubo_hip_batch *batch);
 
+/* ---- the shifted mask stack and its adjoint (differentiation). Forward: per position p the k masks are
+ * moved by MaskShift.bilinear (Sources/CuboVirtual/ShiftedImaging.swift:43-73, 109-122) with the shift
+ * origin_p - mean(origin) (row and column), and Y[p][j] = sum_u x_p[u] shifted_mask_j,p[u]. The reference
+ * builds the shifted mask per position on the CPU (ShiftedImaging.swift:64-73) and has no adjoint; the
+ * centre of mass relative to the origin is the stack {w, w*q_row, w*q_col} shifted the same way.
+ * Backward (the vector-Jacobian product of Y, re-reading the cube): given dL/dY, dmasks = dL/dmasks
+ * (k x detectorCount, Double) and dL/dorigin (row and column, scanCount each; inside a bilinear cell, through
+ * the mean). Any output pointer of backward may be NULL (origin rows: both or neither). uint16 cubes whose
+ * pattern length is a multiple of 8, 1..32 masks (FP32, k x detectorCount), finite origins. ---- */
+typedef struct cubo_hip_shift_plan cubo_hip_shift_plan;
+int cubo_hip_shift_plan_create(cubo_hip_cube *cube, const float *masks, int32_t k, cubo_hip_shift_plan **out);
+/* y: scanCount x k Double, row-major. */
+int cubo_hip_shift_forward(cubo_hip_shift_plan *plan, const double *origin_row, const double *origin_col, double *y);
+int cubo_hip_shift_backward(cubo_hip_shift_plan *plan, const double *origin_row, const double *origin_col,
+                            const double *dy, double *dmasks, double *d_origin_row, double *d_origin_col);
+/* Device milliseconds since the previous query: [forward kernel, origin-adjoint kernel, mask-adjoint kernels],
+ * and the number of backward calls. */
+int cubo_hip_shift_kernel_time(cubo_hip_shift_plan *plan, double *ms3, int64_t *calls);
+int cubo_hip_shift_plan_destroy(cubo_hip_shift_plan *plan);
+
 /* ---- synthetic cubes for benchmarks and tests (no reference counterpart: the reference's bench
  * cubes are generated in Python, tools/benchmarks/bench.py:17-27). A disk probe with a sigmoid edge,
  * Gaussian rings, a per-position deflection and Poisson counts, generated on the device from a
