@@ -1,0 +1,392 @@
+// cubo_dpc.hip — C ABI of the iterative-DPC phase reconstruction and its adjoint (include/cubo_hip.h, "DPC").
+//
+// DPC.reconstruct (Sources/CuboPhase/DPC.swift:26-102): the field is embedded in a grid padded to
+// round(R * padding) x round(C * padding) (half-even), and each iteration computes the central differences of
+// the phase plus the field, their mean square against the field's, and the update
+// Re(IFFT(FFT(dx) i gx + FFT(dy) i gy)) with gx = -k_r / (4 |k|^2) (0 at the origin), halving the step when
+// the error grew. On the device: one complex FFT of dx + i dy, the spectrum split by symmetry, one inverse.
+// The adjoint of a step (and of a chain of fixed-step iterations from phi = 0) runs the transposed operators:
+// one FFT of the incoming phase gradient and one inverse.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/cubo_hip.h"
+#include "dpc_kernels.hpp"
+#include "errors.hpp"
+
+using cubo::DeviceGuard;
+using cubo::fail;
+using cubo::hip_fail;
+
+struct cubo_hip_dpc {
+    int device;
+    hipStream_t stream = nullptr;
+    int R, C, Rp, Cp;
+    double sr, sc;
+    double *gx = nullptr, *gy = nullptr;
+    double2 *tw_r = nullptr, *tw_c = nullptr;
+    double2 *z = nullptr, *f = nullptr, *tmp = nullptr;
+    double *partial = nullptr, *sum = nullptr;
+    double *phi = nullptr, *phi2 = nullptr;      // padded grid
+    double *fx = nullptr, *fy = nullptr;          // scan region
+    double *g_fx = nullptr, *g_fy = nullptr;
+    std::vector<double *> chain;                  // stored iterates of a chain (backward)
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    double ms = 0;
+    int64_t calls = 0;
+};
+
+namespace {
+
+void free_dpc(cubo_hip_dpc *d) {
+    if (!d) return;
+    DeviceGuard g(d->device);
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    void *bufs[] = {d->gx, d->gy, d->tw_r, d->tw_c, d->z, d->f, d->tmp, d->partial, d->sum, d->phi, d->phi2,
+                    d->fx, d->fy, d->g_fx, d->g_fy};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (double *b : d->chain) (void)hipFree(b);
+    if (d->t0) (void)hipEventDestroy(d->t0);
+    if (d->t1) (void)hipEventDestroy(d->t1);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+// numpy.fft.fftfreq(n, d)
+void fftfreq(int n, double d, double *out) {
+    const double scale = 1.0 / ((double)n * d);
+    const int positive = (n - 1) / 2 + 1;
+    for (int k = 0; k < n; ++k) out[k] = (double)(k < positive ? k : k - n) * scale;
+}
+
+std::vector<double2> twiddles(int n) {
+    std::vector<double2> t(n);
+    for (int m = 0; m < n; ++m) {
+        const double a = -2.0 * M_PI * (double)m / (double)n;
+        t[m] = make_double2(std::cos(a), std::sin(a));
+    }
+    return t;
+}
+
+cubo::DpcGrid grid(const cubo_hip_dpc *d) {
+    cubo::DpcGrid g{};
+    g.R = d->R; g.C = d->C; g.Rp = d->Rp; g.Cp = d->Cp;
+    g.sr = d->sr; g.sc = d->sc;
+    g.gx = d->gx; g.gy = d->gy;
+    return g;
+}
+
+// Field checks and the denominator mean(fx^2 + fy^2) (NumPy's pairwise order, as DPC.swift:49-54).
+double pairwise(const double *a, int64_t n) {
+    if (n < 8) {
+        double s = 0.0;
+        for (int64_t i = 0; i < n; ++i) s += a[i];
+        return s;
+    }
+    if (n <= 128) {
+        double r[8];
+        for (int j = 0; j < 8; ++j) r[j] = a[j];
+        int64_t i = 8;
+        for (; i < n - (n % 8); i += 8)
+            for (int j = 0; j < 8; ++j) r[j] += a[i + j];
+        double s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        for (; i < n; ++i) s += a[i];
+        return s;
+    }
+    int64_t n2 = n / 2;
+    n2 -= n2 % 8;
+    return pairwise(a, n2) + pairwise(a + n2, n - n2);
+}
+
+int field(cubo_hip_dpc *d, const double *fx, const double *fy, double *denom) {
+    const int64_t N = (int64_t)d->R * d->C;
+    std::vector<double> t(N);
+    for (int64_t k = 0; k < N; ++k) {
+        if (!std::isfinite(fx[k]) || !std::isfinite(fy[k]))
+            return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the field must be finite (entry %lld)", (long long)k);
+        t[k] = fx[k] * fx[k] + fy[k] * fy[k];
+    }
+    *denom = pairwise(t.data(), N) / (double)N;
+    if (!(*denom > 0)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the field is zero: DPC has nothing to reconstruct");
+    CUBO_HIP_TRY(hipMemcpyAsync(d->fx, fx, N * 8, hipMemcpyHostToDevice, d->stream), "uploading the field");
+    CUBO_HIP_TRY(hipMemcpyAsync(d->fy, fy, N * 8, hipMemcpyHostToDevice, d->stream), "uploading the field");
+    return CUBO_HIP_OK;
+}
+
+// dx, dy of phi into z and their mean square over the scan region into d->sum (device).
+hipError_t forward(cubo_hip_dpc *d, const double *phi) {
+    int nb = 0;
+    hipError_t e = cubo::launch_dpc_forward(grid(d), phi, d->fx, d->fy, d->z, d->partial, &nb, d->stream);
+    if (e == hipSuccess) e = cubo::launch_sum_partials(d->partial, nb, d->sum, d->stream);
+    return e;
+}
+
+// phi_out = phi + step Re(IFFT(i gx FFT(dx) + i gy FFT(dy))), with z = dx + i dy from forward().
+hipError_t update(cubo_hip_dpc *d, const double *phi, double step, double *phi_out) {
+    hipError_t e = cubo::launch_fft2(d->z, d->tmp, d->Rp, d->Cp, d->tw_r, d->tw_c, false, d->stream);
+    if (e == hipSuccess) e = cubo::launch_dpc_spectrum(grid(d), d->z, d->f, d->stream);
+    if (e == hipSuccess) e = cubo::launch_fft2(d->f, d->tmp, d->Rp, d->Cp, d->tw_r, d->tw_c, true, d->stream);
+    if (e == hipSuccess) e = cubo::launch_dpc_update(grid(d), phi, d->f, step, phi_out, d->stream);
+    return e;
+}
+
+// One step's vector-Jacobian product: g_phi_in (device, padded) -> g_phi_out (device), g_fx/g_fy += (device).
+hipError_t vjp(cubo_hip_dpc *d, const double *phi, double step, const double *g_in, double g_err, double denom,
+               double msq, double *g_out) {
+    const int64_t n = (int64_t)d->Rp * d->Cp;
+    hipError_t e = cubo::launch_real_to_complex(g_in, d->z, n, d->stream);
+    if (e == hipSuccess) e = cubo::launch_fft2(d->z, d->tmp, d->Rp, d->Cp, d->tw_r, d->tw_c, false, d->stream);
+    if (e == hipSuccess) e = cubo::launch_dpc_adjoint_spectrum(grid(d), d->z, d->f, d->stream);
+    if (e == hipSuccess) e = cubo::launch_fft2(d->f, d->tmp, d->Rp, d->Cp, d->tw_r, d->tw_c, true, d->stream);
+    // error = msq / denom with denom = mean(fx^2 + fy^2): the denominator's share of the field gradient
+    const double dd = -g_err * msq / (denom * denom);
+    if (e == hipSuccess)
+        e = cubo::launch_dpc_vjp(grid(d), phi, d->fx, d->fy, d->f, step, g_err, denom, dd, g_in, g_out, d->g_fx,
+                                 d->g_fy, d->stream);
+    return e;
+}
+
+double read_sum(cubo_hip_dpc *d, hipError_t *e) {
+    double s = 0;
+    *e = hipMemcpyAsync(&s, d->sum, 8, hipMemcpyDeviceToHost, d->stream);
+    if (*e == hipSuccess) *e = hipStreamSynchronize(d->stream);
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cubo_hip_dpc_create(int device, int32_t rows, int32_t cols, double padding, double sampling_row,
+                        double sampling_col, cubo_hip_dpc **out) {
+    if (!out) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (rows < 1 || cols < 1) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "empty scan %d x %d", rows, cols);
+    if (!(sampling_row > 0) || !(sampling_col > 0) || !std::isfinite(sampling_row) || !std::isfinite(sampling_col))
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the sampling must be positive and finite");
+    if (!(padding >= 1) || !std::isfinite(padding)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "padding below 1");
+    const int Rp = (int)std::nearbyint((double)rows * padding), Cp = (int)std::nearbyint((double)cols * padding);
+    if (Rp > cubo::kFftMax || Cp > cubo::kFftMax)
+        return fail(CUBO_HIP_ERR_UNSUPPORTED, "padded grid %d x %d: at most %d per dimension", Rp, Cp, cubo::kFftMax);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return fail(CUBO_HIP_ERR_NO_DEVICE, "no HIP device %d", device);
+    auto *d = new (std::nothrow) cubo_hip_dpc();
+    if (!d) return fail(CUBO_HIP_ERR_OUT_OF_MEMORY, "host allocation");
+    d->device = device;
+    d->R = rows; d->C = cols; d->Rp = Rp; d->Cp = Cp;
+    d->sr = sampling_row; d->sc = sampling_col;
+    DeviceGuard g(device);
+    const size_t n = (size_t)Rp * Cp, N = (size_t)rows * cols;
+    // operators (DPC.swift:40-47): k from numpy.fft.fftfreq, kden = 1 / (kx^2 + ky^2) with infinity at 0
+    std::vector<double> kx(Rp), ky(Cp), gx(n), gy(n);
+    fftfreq(Rp, sampling_row, kx.data());
+    fftfreq(Cp, sampling_col, ky.data());
+    for (int r = 0; r < Rp; ++r)
+        for (int c = 0; c < Cp; ++c) {
+            double den = kx[r] * kx[r] + ky[c] * ky[c];
+            if (r == 0 && c == 0) den = INFINITY;
+            const double kden = 1.0 / den;
+            gx[(int64_t)r * Cp + c] = -0.25 * kx[r] * kden;
+            gy[(int64_t)r * Cp + c] = -0.25 * ky[c] * kden;
+        }
+    const std::vector<double2> twr = twiddles(Rp), twc = twiddles(Cp);
+    hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    struct { void **p; size_t bytes; } a[] = {
+        {(void **)&d->gx, n * 8}, {(void **)&d->gy, n * 8}, {(void **)&d->tw_r, (size_t)Rp * 16}, {(void **)&d->tw_c, (size_t)Cp * 16},
+        {(void **)&d->z, n * 16}, {(void **)&d->f, n * 16}, {(void **)&d->tmp, n * 16},
+        {(void **)&d->partial, ((n + 255) / 256) * 8}, {(void **)&d->sum, 8}, {(void **)&d->phi, n * 8},
+        {(void **)&d->phi2, n * 8}, {(void **)&d->fx, N * 8}, {(void **)&d->fy, N * 8}, {(void **)&d->g_fx, N * 8},
+        {(void **)&d->g_fy, N * 8}};
+    for (auto &x : a)
+        if (e == hipSuccess) e = hipMalloc(x.p, x.bytes);
+    if (e == hipSuccess) e = hipMemcpy(d->gx, gx.data(), n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d->gy, gy.data(), n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d->tw_r, twr.data(), Rp * 16, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d->tw_c, twc.data(), Cp * 16, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&d->t0);
+    if (e == hipSuccess) e = hipEventCreate(&d->t1);
+    if (e != hipSuccess) { free_dpc(d); return hip_fail(e, "preparing DPC"); }
+    *out = d;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_dpc_padded_shape(const cubo_hip_dpc *d, int32_t *rows, int32_t *cols) {
+    if (!d || !rows || !cols) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *rows = d->Rp;
+    *cols = d->Cp;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_dpc_step(cubo_hip_dpc *d, const double *phi, const double *fx, const double *fy, double step,
+                      double *phi_out, double *error) {
+    if (!d || !phi || !fx || !fy || !phi_out || !error) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    DeviceGuard g(d->device);
+    double denom = 0;
+    if (int st = field(d, fx, fy, &denom)) return st;
+    const int64_t n = (int64_t)d->Rp * d->Cp;
+    CUBO_HIP_TRY(hipMemcpyAsync(d->phi, phi, n * 8, hipMemcpyHostToDevice, d->stream), "uploading the phase");
+    CUBO_HIP_TRY(hipEventRecord(d->t0, d->stream), "hipEventRecord");
+    CUBO_HIP_TRY(forward(d, d->phi), "launching the DPC differences");
+    CUBO_HIP_TRY(update(d, d->phi, step, d->phi2), "launching the DPC update");
+    CUBO_HIP_TRY(hipEventRecord(d->t1, d->stream), "hipEventRecord");
+    CUBO_HIP_TRY(hipMemcpyAsync(phi_out, d->phi2, n * 8, hipMemcpyDeviceToHost, d->stream), "reading the phase");
+    hipError_t e;
+    const double s = read_sum(d, &e);
+    if (e != hipSuccess) return hip_fail(e, "waiting for the DPC step");
+    *error = s / (double)((int64_t)d->R * d->C) / denom;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, d->t0, d->t1) == hipSuccess) d->ms += ms;
+    ++d->calls;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_dpc_step_backward(cubo_hip_dpc *d, const double *phi, const double *fx, const double *fy, double step,
+                               const double *g_phi_out, double g_error, double *g_phi, double *g_fx, double *g_fy) {
+    if (!d || !phi || !fx || !fy || !g_phi_out || !g_phi || !g_fx || !g_fy)
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    DeviceGuard g(d->device);
+    double denom = 0;
+    if (int st = field(d, fx, fy, &denom)) return st;
+    const int64_t n = (int64_t)d->Rp * d->Cp, N = (int64_t)d->R * d->C;
+    CUBO_HIP_TRY(hipMemcpyAsync(d->phi, phi, n * 8, hipMemcpyHostToDevice, d->stream), "uploading the phase");
+    CUBO_HIP_TRY(hipMemcpyAsync(d->phi2, g_phi_out, n * 8, hipMemcpyHostToDevice, d->stream), "uploading dL/dphi");
+    std::vector<double> zero(N, 0.0);
+    CUBO_HIP_TRY(hipMemcpyAsync(d->g_fx, zero.data(), N * 8, hipMemcpyHostToDevice, d->stream), "clearing");
+    CUBO_HIP_TRY(hipMemcpyAsync(d->g_fy, zero.data(), N * 8, hipMemcpyHostToDevice, d->stream), "clearing");
+    CUBO_HIP_TRY(forward(d, d->phi), "launching the DPC differences");
+    hipError_t e;
+    const double msq = read_sum(d, &e) / (double)N;
+    if (e != hipSuccess) return hip_fail(e, "waiting for the DPC differences");
+    double *gout = nullptr;
+    CUBO_HIP_TRY(hipMalloc((void **)&gout, n * 8), "allocating dL/dphi");
+    e = vjp(d, d->phi, step, d->phi2, g_error, denom, msq, gout);
+    if (e == hipSuccess) e = hipMemcpyAsync(g_phi, gout, n * 8, hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(g_fx, d->g_fx, N * 8, hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(g_fy, d->g_fy, N * 8, hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    (void)hipFree(gout);
+    if (e != hipSuccess) return hip_fail(e, "the DPC step adjoint");
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_dpc_reconstruct(cubo_hip_dpc *d, const double *fx, const double *fy, int32_t max_iter, double step,
+                             double stop, int32_t backtrack, double *phase, double *errors, int32_t *accepted,
+                             int32_t *n_iter, double *final_step) {
+    if (!d || !fx || !fy || !phase || !n_iter || !final_step) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (max_iter < 0 || !(step > 0) || !std::isfinite(step))
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "max_iter must be >= 0 and the step positive");
+    if (max_iter > 0 && (!errors || !accepted)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    DeviceGuard g(d->device);
+    double denom = 0;
+    if (int st = field(d, fx, fy, &denom)) return st;
+    const int64_t n = (int64_t)d->Rp * d->Cp, N = (int64_t)d->R * d->C;
+    std::vector<double> zero(n, 0.0);
+    CUBO_HIP_TRY(hipMemcpyAsync(d->phi, zero.data(), n * 8, hipMemcpyHostToDevice, d->stream), "clearing the phase");
+    double error = INFINITY;
+    int ne = 0, ni = 0;
+    double *cur = d->phi, *nxt = d->phi2;
+    CUBO_HIP_TRY(hipEventRecord(d->t0, d->stream), "hipEventRecord");
+    for (int it = 0; it < max_iter; ++it) {
+        if (step < stop) break;
+        CUBO_HIP_TRY(forward(d, cur), "launching the DPC differences");
+        hipError_t e;
+        const double new_error = read_sum(d, &e) / (double)N / denom;
+        if (e != hipSuccess) return hip_fail(e, "waiting for the DPC differences");
+        if (new_error > error && backtrack) {               // DPC.swift:83-90
+            step /= 2;
+            accepted[ni++] = 0;
+            continue;
+        }
+        error = new_error;
+        CUBO_HIP_TRY(update(d, cur, step, nxt), "launching the DPC update");
+        std::swap(cur, nxt);
+        errors[ne++] = error;
+        accepted[ni++] = 1;
+    }
+    CUBO_HIP_TRY(hipEventRecord(d->t1, d->stream), "hipEventRecord");
+    CUBO_HIP_TRY(cubo::launch_crop(cur, d->R, d->C, d->Cp, d->g_fx, d->stream), "launching the crop");
+    CUBO_HIP_TRY(hipMemcpyAsync(phase, d->g_fx, N * 8, hipMemcpyDeviceToHost, d->stream), "reading the phase");
+    CUBO_HIP_TRY(hipStreamSynchronize(d->stream), "waiting for DPC");
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, d->t0, d->t1) == hipSuccess) d->ms += ms;
+    ++d->calls;
+    *n_iter = ni;
+    *final_step = step;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_dpc_chain_backward(cubo_hip_dpc *d, const double *fx, const double *fy, int32_t nsteps, double step,
+                                const double *g_phase, const double *g_errors, double *phase, double *errors,
+                                double *g_fx, double *g_fy) {
+    if (!d || !fx || !fy || !g_phase || !g_fx || !g_fy) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (nsteps < 1 || !(step > 0) || !std::isfinite(step))
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "at least one step, with a positive step size");
+    DeviceGuard g(d->device);
+    double denom = 0;
+    if (int st = field(d, fx, fy, &denom)) return st;
+    const int64_t n = (int64_t)d->Rp * d->Cp, N = (int64_t)d->R * d->C;
+    while ((int)d->chain.size() < nsteps + 1) {
+        double *b = nullptr;
+        CUBO_HIP_TRY(hipMalloc((void **)&b, n * 8), "allocating the stored iterates");
+        d->chain.push_back(b);
+    }
+    std::vector<double> zero(std::max(n, N), 0.0), msq(nsteps);
+    CUBO_HIP_TRY(hipMemcpyAsync(d->chain[0], zero.data(), n * 8, hipMemcpyHostToDevice, d->stream), "clearing");
+    CUBO_HIP_TRY(hipEventRecord(d->t0, d->stream), "hipEventRecord");
+    // forward: phi_{i+1} = phi_i + step * update(phi_i), keeping every iterate
+    for (int i = 0; i < nsteps; ++i) {
+        CUBO_HIP_TRY(forward(d, d->chain[i]), "launching the DPC differences");
+        hipError_t e;
+        msq[i] = read_sum(d, &e) / (double)N;
+        if (e != hipSuccess) return hip_fail(e, "waiting for the DPC differences");
+        if (errors) errors[i] = msq[i] / denom;
+        CUBO_HIP_TRY(update(d, d->chain[i], step, d->chain[i + 1]), "launching the DPC update");
+    }
+    if (phase) {
+        CUBO_HIP_TRY(cubo::launch_crop(d->chain[nsteps], d->R, d->C, d->Cp, d->g_fx, d->stream), "launching the crop");
+        CUBO_HIP_TRY(hipMemcpyAsync(phase, d->g_fx, N * 8, hipMemcpyDeviceToHost, d->stream), "reading the phase");
+        CUBO_HIP_TRY(hipStreamSynchronize(d->stream), "waiting for DPC");
+    }
+    // backward: dL/dphase of the cropped phase into the padded grid, then each step's adjoint in reverse
+    CUBO_HIP_TRY(hipMemcpyAsync(d->g_fx, zero.data(), N * 8, hipMemcpyHostToDevice, d->stream), "clearing");
+    CUBO_HIP_TRY(hipMemcpyAsync(d->g_fy, zero.data(), N * 8, hipMemcpyHostToDevice, d->stream), "clearing");
+    CUBO_HIP_TRY(hipMemcpyAsync(d->tmp, g_phase, N * 8, hipMemcpyHostToDevice, d->stream), "uploading dL/dphase");
+    CUBO_HIP_TRY(cubo::launch_embed((const double *)d->tmp, d->R, d->C, d->Rp, d->Cp, d->phi, d->stream),
+                 "launching the embed");
+    double *gin = d->phi, *gout = d->phi2;
+    for (int i = nsteps - 1; i >= 0; --i) {
+        CUBO_HIP_TRY(vjp(d, d->chain[i], step, gin, g_errors ? g_errors[i] : 0.0, denom, msq[i], gout),
+                     "launching the DPC step adjoint");
+        std::swap(gin, gout);
+    }
+    CUBO_HIP_TRY(hipEventRecord(d->t1, d->stream), "hipEventRecord");
+    CUBO_HIP_TRY(hipMemcpyAsync(g_fx, d->g_fx, N * 8, hipMemcpyDeviceToHost, d->stream), "reading dL/dfx");
+    CUBO_HIP_TRY(hipMemcpyAsync(g_fy, d->g_fy, N * 8, hipMemcpyDeviceToHost, d->stream), "reading dL/dfy");
+    CUBO_HIP_TRY(hipStreamSynchronize(d->stream), "waiting for the DPC adjoint");
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, d->t0, d->t1) == hipSuccess) d->ms += ms;
+    ++d->calls;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_dpc_time(cubo_hip_dpc *d, double *ms, int64_t *calls) {
+    if (!d || !ms || !calls) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *ms = d->ms;
+    *calls = d->calls;
+    d->ms = 0;
+    d->calls = 0;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_dpc_destroy(cubo_hip_dpc *d) {
+    free_dpc(d);
+    return CUBO_HIP_OK;
+}
+
+}  // extern "C"

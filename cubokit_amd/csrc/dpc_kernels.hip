@@ -1,0 +1,291 @@
+// dpc_kernels.hip — FP64 FFT and the iterative-DPC phase step (Sources/CuboPhase/DPC.swift:26-102,
+// Sources/CuboCore/DoubleFFT.swift:119-195 for the FFT conventions) and its vector-Jacobian product.
+//
+// FFT: one block per row, the row in LDS. Lengths that are powers of two run Stockham radix-2 stages
+// (no bit reversal: each stage reads one LDS buffer and writes the other in natural order); other lengths a
+// direct DFT from the twiddle table. 2-D transforms are rows, a tiled transpose, rows, a transpose back.
+// Twiddles exp(-2 pi i m / n) come from the host in Double.
+#include "dpc_kernels.hpp"
+
+namespace cubo {
+namespace {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <bool INV>
+__global__ __launch_bounds__(256) void fft_rows(double2 *data, int n, const double2 *tw) {
+    extern __shared__ double2 sh[];
+    double2 *a = sh, *b = sh + n;
+    double2 *x = data + (int64_t)blockIdx.x * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = x[i];
+    __syncthreads();
+    if ((n & (n - 1)) == 0) {
+        const int half = n >> 1;
+        for (int ns = 1; ns < n; ns <<= 1) {
+            for (int j = threadIdx.x; j < half; j += blockDim.x) {
+                const int k = j & (ns - 1);
+                double2 w = tw[k * (half / ns)];               // exp(-2 pi i k / (2 ns))
+                if (INV) w.y = -w.y;
+                const double2 u = a[j], v = cmul(a[j + half], w);
+                const int o = ((j - k) << 1) + k;
+                b[o] = make_double2(u.x + v.x, u.y + v.y);
+                b[o + ns] = make_double2(u.x - v.x, u.y - v.y);
+            }
+            __syncthreads();
+            double2 *t = a; a = b; b = t;
+        }
+    } else {
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            double re = 0.0, im = 0.0;
+            int m = 0;
+            for (int j = 0; j < n; ++j) {
+                double2 w = tw[m];
+                if (INV) w.y = -w.y;
+                re += a[j].x * w.x - a[j].y * w.y;
+                im += a[j].x * w.y + a[j].y * w.x;
+                m += k;
+                if (m >= n) m -= n;
+            }
+            b[k] = make_double2(re, im);
+        }
+        __syncthreads();
+        double2 *t = a; a = b; b = t;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = a[i];
+}
+
+// out[c][r] = scale * in[r][c]
+__global__ __launch_bounds__(256) void transpose(const double2 *in, double2 *out, int rows, int cols, double scale) {
+    __shared__ double2 t[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int i = ty; i < 32; i += 8) {
+        const int r = by + i, c = bx + tx;
+        if (r < rows && c < cols) t[i][tx] = in[(int64_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int c = bx + i, r = by + tx;
+        if (c < cols && r < rows) {
+            const double2 v = t[tx][i];
+            out[(int64_t)c * rows + r] = make_double2(v.x * scale, v.y * scale);
+        }
+    }
+}
+
+// dx, dy of the scan position (r, c): the central differences of phi plus the field (DPC.swift:59-69).
+__device__ __forceinline__ void dpc_d(const DpcGrid &g, const double *phi, const double *fx, const double *fy, int r,
+                                      int c, double *dx, double *dy) {
+    const int Rp = g.Rp, Cp = g.Cp;
+    const double up = phi[(int64_t)((r - 1 + Rp) % Rp) * Cp + c], down = phi[(int64_t)((r + 1) % Rp) * Cp + c];
+    const double left = phi[(int64_t)r * Cp + (c - 1 + Cp) % Cp], right = phi[(int64_t)r * Cp + (c + 1) % Cp];
+    *dx = (up - down) / (2 * g.sr) + fx[(int64_t)r * g.C + c];
+    *dy = (left - right) / (2 * g.sc) + fy[(int64_t)r * g.C + c];
+}
+
+__global__ __launch_bounds__(256) void dpc_forward(DpcGrid g, const double *phi, const double *fx, const double *fy,
+                                                   double2 *z, double *partial) {
+    __shared__ double red[256];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t n = (int64_t)g.Rp * g.Cp;
+    double sq = 0.0;
+    if (i < n) {
+        const int r = (int)(i / g.Cp), c = (int)(i % g.Cp);
+        double dx = 0.0, dy = 0.0;
+        if (r < g.R && c < g.C) {
+            dpc_d(g, phi, fx, fy, r, c, &dx, &dy);
+            sq = dx * dx + dy * dy;
+        }
+        z[i] = make_double2(dx, dy);
+    }
+    red[threadIdx.x] = sq;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// F = FFT(dx + i dy): FFT(dx)[k] = (F[k] + conj F[-k]) / 2, FFT(dy)[k] = (F[k] - conj F[-k]) / (2i);
+// out = i (gx FFT(dx) + gy FFT(dy)).
+__global__ __launch_bounds__(256) void dpc_spectrum(DpcGrid g, const double2 *f, double2 *out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)g.Rp * g.Cp) return;
+    const int r = (int)(i / g.Cp), c = (int)(i % g.Cp);
+    const int rm = (g.Rp - r) % g.Rp, cm = (g.Cp - c) % g.Cp;
+    const double2 fk = f[i], fm = f[(int64_t)rm * g.Cp + cm];
+    const double2 X = make_double2(0.5 * (fk.x + fm.x), 0.5 * (fk.y - fm.y));
+    const double2 Y = make_double2(0.5 * (fk.y + fm.y), -0.5 * (fk.x - fm.x));
+    const double gx = g.gx[i], gy = g.gy[i];
+    const double sx = gx * X.x + gy * Y.x, sy = gx * X.y + gy * Y.y;
+    out[i] = make_double2(-sy, sx);
+}
+
+__global__ __launch_bounds__(256) void dpc_update(DpcGrid g, const double *phi, const double2 *u, double step,
+                                                  double *phi_out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < (int64_t)g.Rp * g.Cp) phi_out[i] = phi[i] + step * u[i].x;
+}
+
+__global__ __launch_bounds__(256) void real_to_complex(const double *in, double2 *z, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) z[i] = make_double2(in[i], 0.0);
+}
+
+// G = FFT(g), g real. The adjoint needs Re(IFFT(P)) and Re(IFFT(Q)) for P = G conj(i gx), Q = G conj(i gy).
+// Re(IFFT(P)) = IFFT of P's Hermitian part (P[k] + conj P[-k]) / 2 (P itself is not Hermitian on the Nyquist
+// row and column of an even length, where gx[-k] = gx[k]), so out = Herm(P) + i Herm(Q) gives both real parts
+// from one inverse transform.
+__global__ __launch_bounds__(256) void dpc_adjoint_spectrum(DpcGrid g, const double2 *G, double2 *out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)g.Rp * g.Cp) return;
+    const int r = (int)(i / g.Cp), c = (int)(i % g.Cp);
+    const int64_t m = (int64_t)((g.Rp - r) % g.Rp) * g.Cp + (g.Cp - c) % g.Cp;
+    const double2 vk = G[i], vm = G[m];
+    // P[k] = -i gx[k] G[k] = (gx G.y, -gx G.x); conj P[-k] = (gx[-k] G[-k].y, gx[-k] G[-k].x)
+    const double gxk = g.gx[i], gxm = g.gx[m], gyk = g.gy[i], gym = g.gy[m];
+    const double2 P = make_double2(0.5 * (gxk * vk.y + gxm * vm.y), 0.5 * (-gxk * vk.x + gxm * vm.x));
+    const double2 Q = make_double2(0.5 * (gyk * vk.y + gym * vm.y), 0.5 * (-gyk * vk.x + gym * vm.x));
+    out[i] = make_double2(P.x - Q.y, P.y + Q.x);
+}
+
+__device__ __forceinline__ void gd(const DpcGrid &g, const double *phi, const double *fx, const double *fy,
+                                   const double2 *w, double step, double g_err, double denom, int r, int c,
+                                   double *a, double *b) {
+    // gdx, gdy of scan entry (r, c) scaled by the stencil's 1 / (2 sampling); zero off the scan region
+    if (r >= g.R || c >= g.C) { *a = 0.0; *b = 0.0; return; }
+    double dx, dy;
+    dpc_d(g, phi, fx, fy, r, c, &dx, &dy);
+    const double2 wv = w[(int64_t)r * g.Cp + c];
+    const double N = (double)g.R * g.C;
+    const double gdx = wv.x * step + g_err * 2.0 * dx / (N * denom);
+    const double gdy = wv.y * step + g_err * 2.0 * dy / (N * denom);
+    *a = gdx / (2 * g.sr);
+    *b = gdy / (2 * g.sc);
+}
+
+__global__ __launch_bounds__(256) void dpc_vjp(DpcGrid g, const double *phi, const double *fx, const double *fy,
+                                               const double2 *w, double step, double g_err, double denom, double dd,
+                                               const double *g_phi_out, double *g_phi, double *g_fx, double *g_fy) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)g.Rp * g.Cp) return;
+    const int Rp = g.Rp, Cp = g.Cp;
+    const int r = (int)(i / Cp), c = (int)(i % Cp);
+    double a_dn, b_dn, a_up, b_up, a_rt, b_rt, a_lf, b_lf;
+    // dx at (r + 1, c) pushes +a into phi (r, c); dx at (r - 1, c) pushes -a; dy at (r, c + 1) pushes +b, at (r, c - 1) -b
+    gd(g, phi, fx, fy, w, step, g_err, denom, (r + 1) % Rp, c, &a_dn, &b_dn);
+    gd(g, phi, fx, fy, w, step, g_err, denom, (r - 1 + Rp) % Rp, c, &a_up, &b_up);
+    gd(g, phi, fx, fy, w, step, g_err, denom, r, (c + 1) % Cp, &a_rt, &b_rt);
+    gd(g, phi, fx, fy, w, step, g_err, denom, r, (c - 1 + Cp) % Cp, &a_lf, &b_lf);
+    g_phi[i] = g_phi_out[i] + a_dn - a_up + b_rt - b_lf;
+    if (r < g.R && c < g.C) {
+        double dx, dy;
+        dpc_d(g, phi, fx, fy, r, c, &dx, &dy);
+        const double2 wv = w[i];
+        const double N = (double)g.R * g.C;
+        const int64_t k = (int64_t)r * g.C + c;
+        const double gdx = wv.x * step + g_err * 2.0 * dx / (N * denom);
+        const double gdy = wv.y * step + g_err * 2.0 * dy / (N * denom);
+        g_fx[k] += gdx + dd * 2.0 * fx[k] / N;
+        g_fy[k] += gdy + dd * 2.0 * fy[k] / N;
+    }
+}
+
+__global__ __launch_bounds__(256) void sum_partials(const double *partial, int n, double *out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__global__ void crop(const double *padded, int R, int C, int Cp, double *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (int64_t)R * C) out[i] = padded[(i / C) * Cp + i % C];
+}
+
+__global__ void embed(const double *scan, int R, int C, int Rp, int Cp, double *padded) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)Rp * Cp) return;
+    const int r = (int)(i / Cp), c = (int)(i % Cp);
+    padded[i] = (r < R && c < C) ? scan[(int64_t)r * C + c] : 0.0;
+}
+
+unsigned blocks256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+hipError_t launch_fft2(double2 *data, double2 *tmp, int rows, int cols, const double2 *tw_r, const double2 *tw_c,
+                       bool inverse, hipStream_t s) {
+    if (rows > kFftMax || cols > kFftMax || rows < 1 || cols < 1) return hipErrorInvalidValue;
+    const size_t lds_c = 2 * (size_t)cols * sizeof(double2), lds_r = 2 * (size_t)rows * sizeof(double2);
+    if (inverse) hipLaunchKernelGGL(fft_rows<true>, dim3(rows), dim3(256), lds_c, s, data, cols, tw_c);
+    else hipLaunchKernelGGL(fft_rows<false>, dim3(rows), dim3(256), lds_c, s, data, cols, tw_c);
+    hipLaunchKernelGGL(transpose, dim3((cols + 31) / 32, (rows + 31) / 32), dim3(256), 0, s, data, tmp, rows, cols, 1.0);
+    if (inverse) hipLaunchKernelGGL(fft_rows<true>, dim3(cols), dim3(256), lds_r, s, tmp, rows, tw_r);
+    else hipLaunchKernelGGL(fft_rows<false>, dim3(cols), dim3(256), lds_r, s, tmp, rows, tw_r);
+    const double scale = inverse ? 1.0 / ((double)rows * cols) : 1.0;
+    hipLaunchKernelGGL(transpose, dim3((rows + 31) / 32, (cols + 31) / 32), dim3(256), 0, s, tmp, data, cols, rows, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_dpc_forward(const DpcGrid &g, const double *phi, const double *fx, const double *fy, double2 *z,
+                              double *partial, int *nblocks, hipStream_t s) {
+    const int64_t n = (int64_t)g.Rp * g.Cp;
+    *nblocks = (int)blocks256(n);
+    hipLaunchKernelGGL(dpc_forward, dim3(*nblocks), dim3(256), 0, s, g, phi, fx, fy, z, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_dpc_spectrum(const DpcGrid &g, const double2 *f, double2 *out, hipStream_t s) {
+    hipLaunchKernelGGL(dpc_spectrum, dim3(blocks256((int64_t)g.Rp * g.Cp)), dim3(256), 0, s, g, f, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_dpc_update(const DpcGrid &g, const double *phi, const double2 *u, double step, double *phi_out,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(dpc_update, dim3(blocks256((int64_t)g.Rp * g.Cp)), dim3(256), 0, s, g, phi, u, step, phi_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_real_to_complex(const double *in, double2 *z, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(real_to_complex, dim3(blocks256(n)), dim3(256), 0, s, in, z, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_dpc_adjoint_spectrum(const DpcGrid &g, const double2 *G, double2 *out, hipStream_t s) {
+    hipLaunchKernelGGL(dpc_adjoint_spectrum, dim3(blocks256((int64_t)g.Rp * g.Cp)), dim3(256), 0, s, g, G, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_dpc_vjp(const DpcGrid &g, const double *phi, const double *fx, const double *fy, const double2 *w,
+                          double step, double g_err, double denom, double dd, const double *g_phi_out, double *g_phi,
+                          double *g_fx, double *g_fy, hipStream_t s) {
+    hipLaunchKernelGGL(dpc_vjp, dim3(blocks256((int64_t)g.Rp * g.Cp)), dim3(256), 0, s, g, phi, fx, fy, w, step, g_err,
+                       denom, dd, g_phi_out, g_phi, g_fx, g_fy);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const double *partial, int n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, s, partial, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_crop(const double *padded, int R, int C, int Cp, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(crop, dim3(blocks256((int64_t)R * C)), dim3(256), 0, s, padded, R, C, Cp, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_embed(const double *scan, int R, int C, int Rp, int Cp, double *padded, hipStream_t s) {
+    hipLaunchKernelGGL(embed, dim3(blocks256((int64_t)Rp * Cp)), dim3(256), 0, s, scan, R, C, Rp, Cp, padded);
+    return hipGetLastError();
+}
+
+}  // namespace cubo

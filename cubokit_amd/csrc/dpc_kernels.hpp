@@ -1,0 +1,48 @@
+// dpc_kernels.hpp — FP64 FFT and the iterative-DPC phase step on the device (dpc_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cubo {
+
+// Largest FFT length per dimension (one row of complex doubles, twice, in LDS).
+constexpr int kFftMax = 2048;
+
+// In-place 2-D complex FFT of a rows x cols array of (re, im) pairs, NumPy's convention (forward
+// unnormalised, inverse divided by rows * cols), rows then columns. tw_r / tw_c: exp(-2 pi i m / n) for
+// m < n of each length; tmp: rows * cols pairs.
+hipError_t launch_fft2(double2 *data, double2 *tmp, int rows, int cols, const double2 *tw_r, const double2 *tw_c,
+                       bool inverse, hipStream_t s);
+
+struct DpcGrid {
+    int32_t R, C, Rp, Cp;     // scan region and padded grid
+    double sr, sc;            // sampling
+    const double *gx, *gy;    // [Rp][Cp] -0.25 k kden (the update operator is i g)
+};
+
+// z = dx + i dy on the padded grid (zero outside the scan region; DPC.swift:59-69) and sq = dx^2 + dy^2
+// on the scan region; partial[b] = this block's sum of sq (blocks of 256 scan entries).
+hipError_t launch_dpc_forward(const DpcGrid &g, const double *phi, const double *fx, const double *fy, double2 *z,
+                              double *partial, int *nblocks, hipStream_t s);
+// spectrum of the update: F = FFT(dx + i dy) -> i (gx FFT(dx) + gy FFT(dy)) (DPC.swift:71-79), in out.
+hipError_t launch_dpc_spectrum(const DpcGrid &g, const double2 *f, double2 *out, hipStream_t s);
+// phi_out = phi + step * Re(u).
+hipError_t launch_dpc_update(const DpcGrid &g, const double *phi, const double2 *u, double step, double *phi_out,
+                             hipStream_t s);
+// z = g_phi_out + 0 i (the adjoint's input), padded grid.
+hipError_t launch_real_to_complex(const double *in, double2 *z, int64_t n, hipStream_t s);
+// spectrum of the adjoint: G = FFT(g) -> G (gy - i gx), whose inverse is A_x^T g + i A_y^T g.
+hipError_t launch_dpc_adjoint_spectrum(const DpcGrid &g, const double2 *G, double2 *out, hipStream_t s);
+// The rest of the step's vector-Jacobian product (gdx = step Re(w) + 2 g_err dx / (N denom) on the scan region,
+// gdy likewise from Im(w)); g_phi = g_phi_out + the stencil adjoint; g_fx += gdx + dd 2 fx / N, g_fy likewise.
+hipError_t launch_dpc_vjp(const DpcGrid &g, const double *phi, const double *fx, const double *fy, const double2 *w,
+                          double step, double g_err, double denom, double dd, const double *g_phi_out, double *g_phi,
+                          double *g_fx, double *g_fy, hipStream_t s);
+// sum of partial[0 .. n) into out[0] (one block, fixed order).
+hipError_t launch_sum_partials(const double *partial, int n, double *out, hipStream_t s);
+// out = in[0 .. R) x [0 .. C) of the padded grid (the phase crop), or the reverse into a zeroed padded grid.
+hipError_t launch_crop(const double *padded, int R, int C, int Cp, double *out, hipStream_t s);
+hipError_t launch_embed(const double *scan, int R, int C, int Rp, int Cp, double *padded, hipStream_t s);
+
+}  // namespace cubo

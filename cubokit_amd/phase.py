@@ -1,0 +1,103 @@
+"""Mirror of CuboPhase's DPC over the C ABI (include/cubo_hip.h, "iterative DPC").
+
+DPC.reconstruct (Sources/CuboPhase/DPC.swift:26-102) on the device in Double, plus the adjoints of one
+iteration and of a fixed-step chain of iterations, which the reference does not have.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+from .virtual import CuboError, _check
+
+
+class DPC:
+    """One scan shape, padding and sampling: DPC(rows, cols, padding=2, sampling=(1, 1))."""
+
+    def __init__(self, rows: int, cols: int, padding: float = 2.0, sampling: Sequence[float] = (1.0, 1.0),
+                 device: int = 0):
+        self.rows, self.cols = int(rows), int(cols)
+        h = ct.c_void_p()
+        _check(L.lib().cubo_hip_dpc_create(device, self.rows, self.cols, float(padding), float(sampling[0]),
+                                           float(sampling[1]), ct.byref(h)), "preparing DPC")
+        self._h = h
+        r, c = ct.c_int32(), ct.c_int32()
+        _check(L.lib().cubo_hip_dpc_padded_shape(h, ct.byref(r), ct.byref(c)), "DPC shape")
+        self.padded_shape = (r.value, c.value)
+
+    def _field(self, fx, fy):
+        n = self.rows * self.cols
+        x = np.ascontiguousarray(fx, np.float64).reshape(n)
+        y = np.ascontiguousarray(fy, np.float64).reshape(n)
+        return x, y
+
+    def reconstruct(self, fx, fy, max_iter: int = 64, step: float = 0.5, stop: float = 1e-6,
+                    backtrack: bool = True) -> dict:
+        """phase (rows x cols, flattened), errors of the accepted iterations, the accepted flags, final step."""
+        x, y = self._field(fx, fy)
+        phase = np.zeros(self.rows * self.cols)
+        errors = np.zeros(max(max_iter, 1))
+        acc = np.zeros(max(max_iter, 1), np.int32)
+        n, fs = ct.c_int32(), ct.c_double()
+        _check(L.lib().cubo_hip_dpc_reconstruct(self._h, x.ctypes.data, y.ctypes.data, int(max_iter), float(step),
+                                                float(stop), int(backtrack), phase.ctypes.data, errors.ctypes.data,
+                                                acc.ctypes.data, ct.byref(n), ct.byref(fs)), "DPC")
+        accepted = [bool(a) for a in acc[: n.value]]
+        return dict(phase=phase, errors=errors[: sum(accepted)], accepted=accepted, final_step=fs.value)
+
+    def step(self, phi, fx, fy, step: float = 0.5):
+        """One accepted iteration on the padded grid: (phi_out, error of phi)."""
+        x, y = self._field(fx, fy)
+        p = np.ascontiguousarray(phi, np.float64).reshape(-1)
+        out = np.zeros_like(p)
+        err = ct.c_double()
+        _check(L.lib().cubo_hip_dpc_step(self._h, p.ctypes.data, x.ctypes.data, y.ctypes.data, float(step),
+                                         out.ctypes.data, ct.byref(err)), "DPC step")
+        return out, err.value
+
+    def step_backward(self, phi, fx, fy, g_phi_out, g_error: float = 0.0, step: float = 0.5):
+        """(dL/dphi, dL/dfx, dL/dfy) of one iteration."""
+        x, y = self._field(fx, fy)
+        p = np.ascontiguousarray(phi, np.float64).reshape(-1)
+        g = np.ascontiguousarray(g_phi_out, np.float64).reshape(-1)
+        g_phi, g_fx, g_fy = np.zeros_like(p), np.zeros_like(x), np.zeros_like(y)
+        _check(L.lib().cubo_hip_dpc_step_backward(self._h, p.ctypes.data, x.ctypes.data, y.ctypes.data, float(step),
+                                                  g.ctypes.data, float(g_error), g_phi.ctypes.data, g_fx.ctypes.data,
+                                                  g_fy.ctypes.data), "DPC step adjoint")
+        return g_phi, g_fx, g_fy
+
+    def chain_backward(self, fx, fy, nsteps: int, g_phase, g_errors: Optional[np.ndarray] = None, step: float = 0.5):
+        """nsteps fixed-step iterations from phase 0 and their adjoint: (phase, errors, dL/dfx, dL/dfy)."""
+        x, y = self._field(fx, fy)
+        gp = np.ascontiguousarray(g_phase, np.float64).reshape(self.rows * self.cols)
+        ge = None if g_errors is None else np.ascontiguousarray(g_errors, np.float64).reshape(nsteps)
+        phase = np.zeros(self.rows * self.cols)
+        errors = np.zeros(nsteps)
+        g_fx, g_fy = np.zeros_like(x), np.zeros_like(y)
+        _check(L.lib().cubo_hip_dpc_chain_backward(self._h, x.ctypes.data, y.ctypes.data, int(nsteps), float(step),
+                                                   gp.ctypes.data, ge.ctypes.data if ge is not None else None,
+                                                   phase.ctypes.data, errors.ctypes.data, g_fx.ctypes.data,
+                                                   g_fy.ctypes.data), "DPC chain adjoint")
+        return phase, errors, g_fx, g_fy
+
+    def kernel_time(self):
+        ms, n = ct.c_double(), ct.c_int64()
+        _check(L.lib().cubo_hip_dpc_time(self._h, ct.byref(ms), ct.byref(n)), "DPC time")
+        return ms.value, n.value
+
+    def release(self):
+        if getattr(self, "_h", None) is not None:
+            L.lib().cubo_hip_dpc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+__all__ = ["DPC", "CuboError"]

@@ -211,6 +211,37 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *plan, const double *origin_row,
 int cubo_hip_shift_kernel_time(cubo_hip_shift_plan *plan, double *ms3, int64_t *calls);
 int cubo_hip_shift_plan_destroy(cubo_hip_shift_plan *plan);
 
+/* ---- iterative DPC (DPC.reconstruct, Sources/CuboPhase/DPC.swift:26-102) on the device, in Double, and
+ * its adjoint. A plan fixes the scan (rows x cols), the padding (the grid is round(rows * padding) x
+ * round(cols * padding), half-even, at most 2048 per dimension) and the sampling. Fields fx, fy are
+ * rows x cols; phases on the padded grid are padded_rows x padded_cols, row-major. ---- */
+typedef struct cubo_hip_dpc cubo_hip_dpc;
+int cubo_hip_dpc_create(int device, int32_t rows, int32_t cols, double padding, double sampling_row,
+                        double sampling_col, cubo_hip_dpc **out);
+int cubo_hip_dpc_padded_shape(const cubo_hip_dpc *dpc, int32_t *rows, int32_t *cols);
+/* DPC.reconstruct: from phase 0, at most max_iter iterations; an iteration whose error grew halves the step
+ * instead when backtrack is set (DPC.swift:83-90); stops when the step falls below stop. phase: rows x cols;
+ * errors: the accepted iterations' errors; accepted: 1/0 per iteration (max_iter entries each). */
+int cubo_hip_dpc_reconstruct(cubo_hip_dpc *dpc, const double *fx, const double *fy, int32_t max_iter, double step,
+                             double stop, int32_t backtrack, double *phase, double *errors, int32_t *accepted,
+                             int32_t *n_iter, double *final_step);
+/* One accepted iteration from phi (padded grid): phi_out = phi + step * update, and the error of phi. */
+int cubo_hip_dpc_step(cubo_hip_dpc *dpc, const double *phi, const double *fx, const double *fy, double step,
+                      double *phi_out, double *error);
+/* Its vector-Jacobian product: from dL/dphi_out and dL/derror, dL/dphi, dL/dfx, dL/dfy. No reference
+ * counterpart (the reference has no adjoint). */
+int cubo_hip_dpc_step_backward(cubo_hip_dpc *dpc, const double *phi, const double *fx, const double *fy, double step,
+                               const double *g_phi_out, double g_error, double *g_phi, double *g_fx, double *g_fy);
+/* nsteps accepted iterations with a fixed step from phase 0 (the phase after them, cropped to the scan, and
+ * their errors: phase and errors may be NULL), then the adjoint of the whole chain: given dL/dphase
+ * (rows x cols) and dL/derrors (nsteps, or NULL for none), dL/dfx and dL/dfy. The iterates stay on the device. */
+int cubo_hip_dpc_chain_backward(cubo_hip_dpc *dpc, const double *fx, const double *fy, int32_t nsteps, double step,
+                                const double *g_phase, const double *g_errors, double *phase, double *errors,
+                                double *g_fx, double *g_fy);
+/* Device milliseconds of reconstruct / step / chain calls since the previous query, and their count. */
+int cubo_hip_dpc_time(cubo_hip_dpc *dpc, double *ms, int64_t *calls);
+int cubo_hip_dpc_destroy(cubo_hip_dpc *dpc);
+
 /* ---- synthetic cubes for benchmarks and tests (no reference counterpart: the reference's bench
  * cubes are generated in Python, tools/benchmarks/bench.py:17-27). A disk probe with a sigmoid edge,
  * Gaussian rings, a per-position deflection and Poisson counts, generated on the device from a
