@@ -105,6 +105,74 @@ def batch_section(C):
     return res
 
 
+def diff_section(C):
+    """The 512x512x128x128 uint16 differentiation workload (8 GiB): origins from one fused CoM pass, the K = 16
+    + CoM shifted stack forward, 20 fixed-step DPC iterations on the 512x512 centre-of-mass field and their
+    adjoint, then the stack's adjoint with respect to the masks and the origins (both re-reading the cube).
+    Kernel times from HIP events; GB/s are cube bytes per kernel second."""
+    from cubokit_amd import synthetic
+    from cubokit_amd.phase import DPC
+    shape, dtype, params = synthetic.differentiation_u16()
+    cube = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
+    cube.fill_synthetic(params)
+    nb = cube.byte_count
+    npos = shape[0] * shape[1]
+    log(f"differentiation: cube {shape} {nb / 2**30:.1f} GiB")
+    fl = C.first_look(cube, [], com=True)
+    orow, ocol = fl.com.row.values.copy(), fl.com.column.values.copy()
+    masks = synthetic.differentiation_masks()
+    st = C.ShiftedStack(cube, masks)
+    y = st.forward(orow, ocol)                              # prepares the shift classes
+    st.kernel_time()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        y = st.forward(orow, ocol)
+    fwd_wall = (time.perf_counter() - t0) / 3
+    ms, _ = st.kernel_time()
+    fwd_ms = ms[0] / 3
+    S, Mr, Mc = y[:, 16], y[:, 17], y[:, 18]
+    comr, comc = Mr / S, Mc / S
+    fx, fy = comr - comr.mean(), comc - comc.mean()
+    dpc = DPC(shape[0], shape[1])
+    rng = np.random.default_rng(5)
+    g_phase = rng.normal(size=npos)
+    dpc.chain_backward(fx, fy, 20, g_phase)                 # warm-up
+    dpc.kernel_time()
+    t0 = time.perf_counter()
+    phase, errors, gfx, gfy = dpc.chain_backward(fx, fy, 20, g_phase)
+    dpc_wall = time.perf_counter() - t0
+    dpc_ms, _ = dpc.kernel_time()
+    # through the mean removal and the quotients com = M / S
+    gcr, gcc = gfx - gfx.mean(), gfy - gfy.mean()
+    dy = np.zeros((npos, 19))
+    dy[:, :16] = rng.normal(size=(npos, 16))
+    dy[:, 17], dy[:, 18] = gcr / S, gcc / S
+    dy[:, 16] = -(gcr * Mr + gcc * Mc) / (S * S)
+    st.backward(orow, ocol, dy)                            # warm-up
+    st.kernel_time()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        dm, dr, dc = st.backward(orow, ocol, dy)
+    bwd_wall = (time.perf_counter() - t0) / 2
+    ms, calls = st.kernel_time()
+    org_ms, msk_ms = ms[1] / calls, ms[2] / calls
+    ok = bool(np.all(np.isfinite(dm)) and np.all(np.isfinite(dr)) and np.all(np.isfinite(phase)))
+    res = {"workload": "512x512x128x128 uint16 (8 GiB), disk r=12, 8-sinusoid +-2 px deflection; K=16 masks + CoM "
+                       "rows, bilinear per-position origin; 20 fixed-step DPC iterations on the 512x512 field",
+           "cube_bytes": nb, "forward_kernel_ms": round(fwd_ms, 3), "forward_gbs": round(nb / fwd_ms / 1e6, 1),
+           "origin_adjoint_ms": round(org_ms, 3), "origin_adjoint_gbs": round(nb / org_ms / 1e6, 1),
+           "mask_adjoint_ms": round(msk_ms, 3), "mask_adjoint_gbs": round(nb / msk_ms / 1e6, 1),
+           "dpc_20_steps_and_adjoint_device_ms": round(dpc_ms, 3), "dpc_20_steps_and_adjoint_wall_ms": round(dpc_wall * 1e3, 3),
+           "forward_call_wall_ms": round(fwd_wall * 1e3, 3), "backward_call_wall_ms": round(bwd_wall * 1e3, 3),
+           "dpc_final_error": float(errors[-1]), "finite": ok}
+    log(f"differentiation: forward {fwd_ms:.3f} ms, origin adjoint {org_ms:.3f} ms, mask adjoint {msk_ms:.3f} ms, "
+        f"DPC 20 steps + adjoint {dpc_ms:.3f} ms device ({dpc_wall * 1e3:.1f} ms wall)")
+    st.release()
+    dpc.release()
+    cube.release()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +184,7 @@ def main() -> int:
                     help="also run one device copy of 2 GiB (read 2 GiB, write 2 GiB) for counter calibration")
     ap.add_argument("--no-stack", action="store_true", help="skip the K = 64 mask-stack comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-cube batched-stack comparison")
+    ap.add_argument("--no-diff", action="store_true", help="skip the 8 GiB differentiation workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload runs on one MI355X "
@@ -207,6 +276,7 @@ def main() -> int:
         stack["mfma_gbs"] = round(cube_bytes / (stack["mfma_split_fp16_ms"] / 1e3) / 1e9, 1)
 
     batch = None if args.no_batch else batch_section(C)
+    diff = None if args.no_diff else diff_section(C)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -227,6 +297,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "mask_stack_k64": stack,
         "batch_48": batch,
+        "differentiation": diff,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -100,3 +100,19 @@ def batch_items(n: int = 48, seed: int = 20261104):
                    waves=random_waves(rng, 2, 3.0, scan))
         out.append((shape, p, k))
     return out
+
+
+def differentiation_u16(seed: int = 20261105, scan=(512, 512)):
+    """The 512x512x128x128 uint16 differentiation workload (8 GiB): disk r=12 at 500 counts/px, background
+    2/px, and a smooth deflection field, the sum of 8 seeded sinusoids within +-2 px."""
+    rng = np.random.default_rng(seed)
+    shape = (scan[0], scan[1], 128, 128)
+    waves = random_waves(rng, 8, 2.0, scan)
+    return shape, np.uint16, params(seed, (64.0, 64.0), 12.0, 500.0, edge=0.5, background=2.0, waves=waves)
+
+
+def differentiation_masks(qy: int = 128, qx: int = 128, seed: int = 20261106) -> np.ndarray:
+    """K = 16 masks (8 seeded annular sectors, 8 dense uniform [0, 1)) followed by the centre-of-mass rows
+    {1, q_row, q_col}: 19 rows, Float32."""
+    m = stack_masks(qy, qx, seed=seed, n_sectors=8, n_dense=8)       # {1, q_row, q_col, 8 sectors, 8 dense}
+    return np.ascontiguousarray(np.concatenate([m[3:], m[:3]]), np.float32)
