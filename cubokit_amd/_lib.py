@@ -106,6 +106,12 @@ SIGNATURES = {
                                            ct.c_void_p, ct.c_void_p]),
     "cubo_hip_shift_kernel_time": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_int64)]),
     "cubo_hip_shift_plan_destroy": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_phase_normalize": (ct.c_int, [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_int32,
+                                            ct.c_double, ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                            ct.c_void_p]),
+    "cubo_hip_phase_rotation": (ct.c_int, [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_void_p,
+                                           ct.c_int32, ct.c_int32, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int32),
+                                           ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
     "cubo_hip_dpc_create": (ct.c_int, [ct.c_int, ct.c_int32, ct.c_int32, ct.c_double, ct.c_double, ct.c_double,
                                        ct.POINTER(ct.c_void_p)]),
     "cubo_hip_dpc_padded_shape": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32)]),

@@ -1,0 +1,301 @@
+// cubo_phase.hip — C ABI of the CuboPhase field preparation on the device, in Double
+// (include/cubo_hip.h, "centre-of-mass reference and rotation").
+//
+// CenterOfMassReference.normalize (Sources/CuboPhase/CenterOfMassReference.swift:27-84): a least-squares plane
+// a + b u + c v over the finite positions (normal equations on centred coordinates, degenerate when
+// det <= tol * max(suu svv, 1)), the constant mean, or nothing, subtracted; NaN -> 0, +-inf -> +-max finite;
+// times the reciprocal sampling. RotationSolver.solve (Sources/CuboPhase/RotationSolver.swift:48-150): for each
+// angle and both orientations (components swapped first when transposed) the mean over the scan's interior of
+// |d_row y' - d_col x'| (or of the divergence when maximising); the first best angle wins and the plain
+// orientation wins ties. Each (angle, orientation) is one block group on the device; the sums of the blocks
+// are added on the host in block order.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "../../include/cubo_hip.h"
+#include "errors.hpp"
+
+using cubo::DeviceGuard;
+using cubo::fail;
+using cubo::hip_fail;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kSplit = 16;            // blocks per (angle, orientation)
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double out = red[0];
+    __syncthreads();
+    return out;
+}
+
+// mode 0: count, sum u, sum v of the finite entries; mode 1: suu, suv, svv, sum m; mode 2: sum u (m - a), sum v (m - a)
+__global__ __launch_bounds__(kThreads) void plane_sums(const double *m, int64_t n, int cols, int mode, double rmean,
+                                                       double cmean, double a, double *partial) {
+    __shared__ double red[kThreads];
+    double s[4] = {0, 0, 0, 0};
+    for (int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const double v = m[k];
+        if (!isfinite(v)) continue;
+        const double r = (double)(k / cols), c = (double)(k % cols);
+        if (mode == 0) { s[0] += 1; s[1] += r; s[2] += c; }
+        else {
+            const double u = r - rmean, w = c - cmean;
+            if (mode == 1) { s[0] += u * u; s[1] += u * w; s[2] += w * w; s[3] += v; }
+            else { s[0] += u * (v - a); s[1] += w * (v - a); }
+        }
+    }
+    for (int i = 0; i < 4; ++i) {
+        const double t = block_sum(s[i], red);
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * 4 + i] = t;
+    }
+}
+
+__global__ void plane_eval(double *out, int64_t n, int cols, double a, double b, double c, double rmean,
+                           double cmean) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = a + b * ((double)(k / cols) - rmean) + c * ((double)(k % cols) - cmean);
+}
+
+__device__ __forceinline__ double clean(double v) {
+    if (isnan(v)) return 0.0;
+    if (v == INFINITY) return DBL_MAX;
+    if (v == -INFINITY) return -DBL_MAX;
+    return v;
+}
+
+__global__ void normalize_apply(const double *mx, const double *my, const double *fx, const double *fy, int64_t n,
+                                double sampling, double *nx, double *ny) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    nx[k] = clean(mx[k] - fx[k]) * sampling;
+    ny[k] = clean(my[k] - fy[k]) * sampling;
+}
+
+// blockIdx.x = split, blockIdx.y = angle, blockIdx.z = orientation (1: components swapped)
+__global__ __launch_bounds__(kThreads) void rotation_sums(const double *nx, const double *ny, int R, int C,
+                                                          const double *cosv, const double *sinv, int maximize,
+                                                          double *partial) {
+    __shared__ double red[kThreads];
+    const int angle = blockIdx.y, t = blockIdx.z;
+    const double co = cosv[angle], si = sinv[angle];
+    const double *A = t ? ny : nx, *B = t ? nx : ny;
+    auto xr = [&](int64_t k) { return co * A[k] - si * B[k]; };   // rotated components (RotationSolver.swift:56-62)
+    auto yr = [&](int64_t k) { return si * A[k] + co * B[k]; };
+    const int64_t ni = (int64_t)(R - 2) * (C - 2);
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < ni; i += (int64_t)kSplit * kThreads) {
+        const int r = 1 + (int)(i / (C - 2)), c = 1 + (int)(i % (C - 2));
+        const int64_t k = (int64_t)r * C + c;
+        double v;
+        if (maximize) v = fabs((xr(k + C) - xr(k - C)) + (yr(k + 1) - yr(k - 1)));
+        else v = fabs((yr(k + C) - yr(k - C)) - (xr(k + 1) - xr(k - 1)));
+        s += v;
+    }
+    const double tot = block_sum(s, red);
+    if (threadIdx.x == 0) partial[((int64_t)t * gridDim.y + angle) * kSplit + blockIdx.x] = tot;
+}
+
+__global__ void rotate_apply(const double *nx, const double *ny, int64_t n, double co, double si, int t, double *cx,
+                             double *cy) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double a = t ? ny[k] : nx[k], b = t ? nx[k] : ny[k];
+    cx[k] = co * a - si * b;
+    cy[k] = si * a + co * b;
+}
+
+unsigned blocks(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+struct Dev {
+    int device;
+    hipStream_t s = nullptr;
+    std::vector<void *> bufs;
+    ~Dev() {
+        DeviceGuard g(device);
+        if (s) (void)hipStreamSynchronize(s);
+        for (void *b : bufs) (void)hipFree(b);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    template <typename T>
+    hipError_t alloc(T **p, size_t n) {
+        hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) bufs.push_back(*p);
+        return e;
+    }
+};
+
+int device_ok(int device) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return fail(CUBO_HIP_ERR_NO_DEVICE, "no HIP device %d", device);
+    return CUBO_HIP_OK;
+}
+
+// The plane of m (device) into out (device); CUBO_HIP_ERR_INVALID_ARGUMENT when degenerate.
+int plane(Dev &d, const double *m, int R, int C, double tol, double *out, double *partial, int nb) {
+    const int64_t n = (int64_t)R * C;
+    std::vector<double> h((size_t)nb * 4);
+    auto sums = [&](int mode, double rm, double cm, double a, double (&tot)[4]) -> hipError_t {
+        hipLaunchKernelGGL(plane_sums, dim3(nb), dim3(kThreads), 0, d.s, m, n, C, mode, rm, cm, a, partial);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), partial, h.size() * 8, hipMemcpyDeviceToHost, d.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(d.s);
+        for (int i = 0; i < 4; ++i) {
+            tot[i] = 0;
+            for (int b = 0; b < nb; ++b) tot[i] += h[(size_t)b * 4 + i];
+        }
+        return e;
+    };
+    double t0[4], t1[4], t2[4];
+    CUBO_HIP_TRY(sums(0, 0, 0, 0, t0), "plane sums");
+    if (t0[0] < 3) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the plane fit needs 3 finite positions");
+    const double cnt = t0[0], rm = t0[1] / cnt, cm = t0[2] / cnt;
+    CUBO_HIP_TRY(sums(1, rm, cm, 0, t1), "plane sums");
+    const double suu = t1[0], suv = t1[1], svv = t1[2], a = t1[3] / cnt;
+    CUBO_HIP_TRY(sums(2, rm, cm, a, t2), "plane sums");
+    const double det = suu * svv - suv * suv;
+    if (!(det > tol * std::fmax(suu * svv, 1.0)))
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the plane fit is degenerate (the positions are collinear)");
+    const double b = (svv * t2[0] - suv * t2[1]) / det, c = (suu * t2[1] - suv * t2[0]) / det;
+    hipLaunchKernelGGL(plane_eval, dim3(blocks(n)), dim3(kThreads), 0, d.s, out, n, C, a, b, c, rm, cm);
+    CUBO_HIP_TRY(hipGetLastError(), "plane evaluation");
+    return CUBO_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cubo_hip_phase_normalize(int device, const double *mx, const double *my, int32_t rows, int32_t cols,
+                             int32_t reference, double sampling, double tol, double *fx, double *fy, double *nx,
+                             double *ny) {
+    if (!mx || !my || !fx || !fy || !nx || !ny) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (rows < 1 || cols < 1) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "empty scan");
+    if (reference < 0 || reference > 2) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "reference %d (0, 1, 2)", reference);
+    if (int st = device_ok(device)) return st;
+    DeviceGuard g(device);
+    Dev d{device};
+    CUBO_HIP_TRY(hipStreamCreateWithFlags(&d.s, hipStreamNonBlocking), "hipStreamCreate");
+    const int64_t n = (int64_t)rows * cols;
+    const int nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + kThreads - 1) / kThreads));
+    double *dmx, *dmy, *dfx, *dfy, *dnx, *dny, *partial;
+    for (double **p : {&dmx, &dmy, &dfx, &dfy, &dnx, &dny}) CUBO_HIP_TRY(d.alloc(p, n), "allocating the field");
+    CUBO_HIP_TRY(d.alloc(&partial, (size_t)nb * 4), "allocating partial sums");
+    CUBO_HIP_TRY(hipMemcpyAsync(dmx, mx, n * 8, hipMemcpyHostToDevice, d.s), "uploading the field");
+    CUBO_HIP_TRY(hipMemcpyAsync(dmy, my, n * 8, hipMemcpyHostToDevice, d.s), "uploading the field");
+    if (reference == 1) {
+        if (int st = plane(d, dmx, rows, cols, tol, dfx, partial, nb)) return st;
+        if (int st = plane(d, dmy, rows, cols, tol, dfy, partial, nb)) return st;
+    } else {
+        double a = 0, b = 0;
+        if (reference == 2) {                        // NumPy means of the raw components (no finite filter)
+            std::vector<double> t(mx, mx + n), u(my, my + n);
+            auto pw = [](const double *v, int64_t m, auto &&self) -> double {
+                if (m < 8) { double s = 0; for (int64_t i = 0; i < m; ++i) s += v[i]; return s; }
+                if (m <= 128) {
+                    double r[8];
+                    for (int j = 0; j < 8; ++j) r[j] = v[j];
+                    int64_t i = 8;
+                    for (; i < m - (m % 8); i += 8)
+                        for (int j = 0; j < 8; ++j) r[j] += v[i + j];
+                    double s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+                    for (; i < m; ++i) s += v[i];
+                    return s;
+                }
+                int64_t m2 = m / 2;
+                m2 -= m2 % 8;
+                return self(v, m2, self) + self(v + m2, m - m2, self);
+            };
+            a = pw(t.data(), n, pw) / (double)n;
+            b = pw(u.data(), n, pw) / (double)n;
+        }
+        std::vector<double> ca(n, a), cb(n, b);
+        CUBO_HIP_TRY(hipMemcpyAsync(dfx, ca.data(), n * 8, hipMemcpyHostToDevice, d.s), "uploading the reference");
+        CUBO_HIP_TRY(hipMemcpyAsync(dfy, cb.data(), n * 8, hipMemcpyHostToDevice, d.s), "uploading the reference");
+        CUBO_HIP_TRY(hipStreamSynchronize(d.s), "waiting");
+    }
+    hipLaunchKernelGGL(normalize_apply, dim3(blocks(n)), dim3(kThreads), 0, d.s, dmx, dmy, dfx, dfy, n, sampling, dnx,
+                       dny);
+    CUBO_HIP_TRY(hipGetLastError(), "normalising");
+    CUBO_HIP_TRY(hipMemcpyAsync(fx, dfx, n * 8, hipMemcpyDeviceToHost, d.s), "reading the reference");
+    CUBO_HIP_TRY(hipMemcpyAsync(fy, dfy, n * 8, hipMemcpyDeviceToHost, d.s), "reading the reference");
+    CUBO_HIP_TRY(hipMemcpyAsync(nx, dnx, n * 8, hipMemcpyDeviceToHost, d.s), "reading the field");
+    CUBO_HIP_TRY(hipMemcpyAsync(ny, dny, n * 8, hipMemcpyDeviceToHost, d.s), "reading the field");
+    CUBO_HIP_TRY(hipStreamSynchronize(d.s), "waiting for the normalisation");
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_phase_rotation(int device, const double *nx, const double *ny, int32_t rows, int32_t cols,
+                            const double *angles_deg, int32_t nangles, int32_t maximize, double *degrees,
+                            int32_t *transposed, double *curve, double *curve_t, double *cx, double *cy) {
+    if (!nx || !ny || !angles_deg || !degrees || !transposed || !curve || !curve_t || !cx || !cy)
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (rows < 3 || cols < 3) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the scan needs at least 3 x 3 positions");
+    if (nangles < 1 || nangles > 65535) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "%d angles", nangles);
+    const int64_t n = (int64_t)rows * cols, ni = (int64_t)(rows - 2) * (cols - 2);
+    for (int64_t k = 0; k < n; ++k)
+        if (!std::isfinite(nx[k]) || !std::isfinite(ny[k]))
+            return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the field must be finite (entry %lld)", (long long)k);
+    if (int st = device_ok(device)) return st;
+    DeviceGuard g(device);
+    Dev d{device};
+    CUBO_HIP_TRY(hipStreamCreateWithFlags(&d.s, hipStreamNonBlocking), "hipStreamCreate");
+    std::vector<double> co(nangles), si(nangles);
+    for (int a = 0; a < nangles; ++a) {
+        const double rad = angles_deg[a] * (M_PI / 180.0);
+        co[a] = std::cos(rad);
+        si[a] = std::sin(rad);
+    }
+    double *dnx, *dny, *dco, *dsi, *partial, *dcx, *dcy;
+    for (double **p : {&dnx, &dny, &dcx, &dcy}) CUBO_HIP_TRY(d.alloc(p, n), "allocating the field");
+    CUBO_HIP_TRY(d.alloc(&dco, nangles), "allocating angles");
+    CUBO_HIP_TRY(d.alloc(&dsi, nangles), "allocating angles");
+    const size_t np = (size_t)2 * nangles * kSplit;
+    CUBO_HIP_TRY(d.alloc(&partial, np), "allocating partial sums");
+    CUBO_HIP_TRY(hipMemcpyAsync(dnx, nx, n * 8, hipMemcpyHostToDevice, d.s), "uploading the field");
+    CUBO_HIP_TRY(hipMemcpyAsync(dny, ny, n * 8, hipMemcpyHostToDevice, d.s), "uploading the field");
+    CUBO_HIP_TRY(hipMemcpyAsync(dco, co.data(), nangles * 8, hipMemcpyHostToDevice, d.s), "uploading angles");
+    CUBO_HIP_TRY(hipMemcpyAsync(dsi, si.data(), nangles * 8, hipMemcpyHostToDevice, d.s), "uploading angles");
+    hipLaunchKernelGGL(rotation_sums, dim3(kSplit, nangles, 2), dim3(kThreads), 0, d.s, dnx, dny, rows, cols, dco, dsi,
+                       (int)(maximize != 0), partial);
+    CUBO_HIP_TRY(hipGetLastError(), "launching the rotation curves");
+    std::vector<double> h(np);
+    CUBO_HIP_TRY(hipMemcpyAsync(h.data(), partial, np * 8, hipMemcpyDeviceToHost, d.s), "reading the curves");
+    CUBO_HIP_TRY(hipStreamSynchronize(d.s), "waiting for the rotation curves");
+    for (int t = 0; t < 2; ++t)
+        for (int a = 0; a < nangles; ++a) {
+            double s = 0;
+            for (int b = 0; b < kSplit; ++b) s += h[((size_t)t * nangles + a) * kSplit + b];
+            (t ? curve_t : curve)[a] = s / (double)ni;
+        }
+    int i = 0, j = 0;
+    for (int k = 1; k < nangles; ++k) {
+        if (maximize ? curve[k] > curve[i] : curve[k] < curve[i]) i = k;
+        if (maximize ? curve_t[k] > curve_t[j] : curve_t[k] < curve_t[j]) j = k;
+    }
+    const int t = maximize ? curve_t[j] > curve[i] : curve_t[j] < curve[i];   // the plain orientation wins ties
+    *transposed = t;
+    *degrees = angles_deg[t ? j : i];
+    const double rad = *degrees * (M_PI / 180.0);
+    hipLaunchKernelGGL(rotate_apply, dim3(blocks(n)), dim3(kThreads), 0, d.s, dnx, dny, n, std::cos(rad), std::sin(rad),
+                       t, dcx, dcy);
+    CUBO_HIP_TRY(hipGetLastError(), "rotating the field");
+    CUBO_HIP_TRY(hipMemcpyAsync(cx, dcx, n * 8, hipMemcpyDeviceToHost, d.s), "reading the field");
+    CUBO_HIP_TRY(hipMemcpyAsync(cy, dcy, n * 8, hipMemcpyDeviceToHost, d.s), "reading the field");
+    CUBO_HIP_TRY(hipStreamSynchronize(d.s), "waiting for the rotation");
+    return CUBO_HIP_OK;
+}
+
+}  // extern "C"

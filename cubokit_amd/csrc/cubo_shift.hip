@@ -257,7 +257,7 @@ int cubo_hip_shift_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
     if (!p) return fail(CUBO_HIP_ERR_OUT_OF_MEMORY, "host allocation");
     p->cube = c;
     p->k = k;
-    p->kp = k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : 32;
+    p->kp = k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : k <= 24 ? 24 : 32;
     DeviceGuard g(c->device);
     cubo::SplitWeights sw = cubo::split_weights(masks, k, c->ndet);
     p->nchunks = sw.nchunks;

@@ -262,24 +262,45 @@ __global__ __launch_bounds__(256) void shift_grad_w(ShiftGradW g) {
     double accd[KP];
 #pragma unroll
     for (int k = 0; k < KP; ++k) { acc[k] = 0.f; accd[k] = 0.0; }
-    int n = 0;
-    for (int64_t p = p0; p < p1; ++p) {
+    // U positions per round: their 4 U corner loads are issued before any is used, so a wave waits for memory
+    // once per U positions; FP32 sums go to FP64 every 16 positions.
+    constexpr int U = 8;
+    auto corners = [&](int64_t p, float (&x)[4]) {
         const int ir = shift[2 * p], ic = shift[2 * p + 1];
         int r0 = va + ir; r0 -= r0 >= qy ? qy : 0;
         int r1 = r0 + 1;  r1 -= r1 >= qy ? qy : 0;
         int c0 = vb + ic; c0 -= c0 >= qx ? qx : 0;
         int c1 = c0 + 1;  c1 -= c1 >= qx ? qx : 0;
-        const uint16_t *x = g.cube + p * ndet;
-        const float x00 = x[(int64_t)r0 * qx + c0], x10 = x[(int64_t)r1 * qx + c0];
-        const float x01 = x[(int64_t)r0 * qx + c1], x11 = x[(int64_t)r1 * qx + c1];
-        const float xs = wts[4 * p] * x00 + wts[4 * p + 1] * x10 + wts[4 * p + 2] * x01 + wts[4 * p + 3] * x11;
+        const uint16_t *xp = g.cube + p * ndet;
+        x[0] = xp[r0 * qx + c0];
+        x[1] = xp[r1 * qx + c0];
+        x[2] = xp[r0 * qx + c1];
+        x[3] = xp[r1 * qx + c1];
+    };
+    auto accumulate = [&](int64_t p, const float (&x)[4]) {
+        const float xs = wts[4 * p] * x[0] + wts[4 * p + 1] * x[1] + wts[4 * p + 2] * x[2] + wts[4 * p + 3] * x[3];
 #pragma unroll
         for (int k = 0; k < KP; ++k) acc[k] = __builtin_fmaf(xs, dy[p * KP + k], acc[k]);
-        if (++n == 16) {
+    };
+    auto flush = [&]() {
 #pragma unroll
-            for (int k = 0; k < KP; ++k) { accd[k] += (double)acc[k]; acc[k] = 0.f; }
-            n = 0;
-        }
+        for (int k = 0; k < KP; ++k) { accd[k] += (double)acc[k]; acc[k] = 0.f; }
+    };
+    int64_t p = p0;
+    int n = 0;
+    for (; p + U <= p1; p += U) {
+        float x[U][4];
+#pragma unroll
+        for (int q = 0; q < U; ++q) corners(p + q, x[q]);
+#pragma unroll
+        for (int q = 0; q < U; ++q) accumulate(p + q, x[q]);
+        n += U;
+        if (n >= 16) { flush(); n = 0; }
+    }
+    for (; p < p1; ++p) {
+        float x[4];
+        corners(p, x);
+        accumulate(p, x);
     }
     if (!live) return;
 #pragma unroll
@@ -322,6 +343,7 @@ hipError_t launch_shift_grad_w(const ShiftGradW &a, hipStream_t s) {
         case 4: hipLaunchKernelGGL(shift_grad_w<4>, grid, dim3(256), 0, s, a); break;
         case 8: hipLaunchKernelGGL(shift_grad_w<8>, grid, dim3(256), 0, s, a); break;
         case 16: hipLaunchKernelGGL(shift_grad_w<16>, grid, dim3(256), 0, s, a); break;
+        case 24: hipLaunchKernelGGL(shift_grad_w<24>, grid, dim3(256), 0, s, a); break;
         case 32: hipLaunchKernelGGL(shift_grad_w<32>, grid, dim3(256), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }

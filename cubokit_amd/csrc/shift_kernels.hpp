@@ -55,7 +55,7 @@ struct ShiftGradW {
     const uint16_t *cube;
     int64_t npos;
     int32_t qy, qx;
-    int32_t kp;               // masks padded to the instantiation (4, 8, 16 or 32)
+    int32_t kp;               // masks padded to the instantiation (4, 8, 16, 24 or 32)
     const int32_t *shift;     // [npos][2] (ir, ic)
     const float *wts;         // [npos][4] corner weights
     const float *dy;          // [npos][kp] dL/dY in FP32

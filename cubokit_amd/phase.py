@@ -14,6 +14,37 @@ from . import _lib as L
 from .virtual import CuboError, _check
 
 
+ANGLES = np.arange(179, dtype=np.float64) - 89.0      # RotationSolver's default angles, -89 ... 89 degrees
+_REFERENCE = {"none": 0, "plane": 1, "constant": 2}
+
+
+def normalize(mx, my, rows: int, cols: int, reference: str = "plane", sampling: float = 1.0, tol: float = 1e-12,
+              device: int = 0):
+    """CenterOfMassReference.normalize: (reference row, reference column, normalised row, normalised column)."""
+    n = rows * cols
+    x = np.ascontiguousarray(mx, np.float64).reshape(n)
+    y = np.ascontiguousarray(my, np.float64).reshape(n)
+    fx, fy, nx, ny = (np.zeros(n) for _ in range(4))
+    _check(L.lib().cubo_hip_phase_normalize(device, x.ctypes.data, y.ctypes.data, rows, cols, _REFERENCE[reference],
+                                            float(sampling), float(tol), fx.ctypes.data, fy.ctypes.data,
+                                            nx.ctypes.data, ny.ctypes.data), "normalize")
+    return fx, fy, nx, ny
+
+
+def rotation_solve(nx, ny, rows: int, cols: int, angles=ANGLES, maximize: bool = False, device: int = 0) -> dict:
+    """RotationSolver.solve: the chosen angle and orientation, both curves and the corrected field."""
+    n = rows * cols
+    x = np.ascontiguousarray(nx, np.float64).reshape(n)
+    y = np.ascontiguousarray(ny, np.float64).reshape(n)
+    a = np.ascontiguousarray(angles, np.float64)
+    deg, t = ct.c_double(), ct.c_int32()
+    curve, curve_t, cx, cy = np.zeros(a.size), np.zeros(a.size), np.zeros(n), np.zeros(n)
+    _check(L.lib().cubo_hip_phase_rotation(device, x.ctypes.data, y.ctypes.data, rows, cols, a.ctypes.data, a.size,
+                                           int(maximize), ct.byref(deg), ct.byref(t), curve.ctypes.data,
+                                           curve_t.ctypes.data, cx.ctypes.data, cy.ctypes.data), "rotation")
+    return dict(degrees=deg.value, transposed=bool(t.value), curve=curve, curve_transposed=curve_t, cx=cx, cy=cy)
+
+
 class DPC:
     """One scan shape, padding and sampling: DPC(rows, cols, padding=2, sampling=(1, 1))."""
 
@@ -100,4 +131,4 @@ class DPC:
             pass
 
 
-__all__ = ["DPC", "CuboError"]
+__all__ = ["ANGLES", "DPC", "CuboError", "normalize", "rotation_solve"]

@@ -211,6 +211,22 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *plan, const double *origin_row,
 int cubo_hip_shift_kernel_time(cubo_hip_shift_plan *plan, double *ms3, int64_t *calls);
 int cubo_hip_shift_plan_destroy(cubo_hip_shift_plan *plan);
 
+/* ---- the centre-of-mass field's reference and rotation (CuboPhase), on the device in Double ----
+ * CenterOfMassReference.normalize (Sources/CuboPhase/CenterOfMassReference.swift:27-84): reference 0 = none,
+ * 1 = least-squares plane over the finite positions (refused when degenerate: det <= tol * max(suu svv, 1)),
+ * 2 = the constant mean; fx/fy get the reference, nx/ny = (measured - reference) with NaN -> 0 and
+ * +-inf -> +-max finite, times sampling. All arrays rows x cols. */
+int cubo_hip_phase_normalize(int device, const double *mx, const double *my, int32_t rows, int32_t cols,
+                             int32_t reference, double sampling, double tol, double *fx, double *fy, double *nx,
+                             double *ny);
+/* RotationSolver.solve (Sources/CuboPhase/RotationSolver.swift:48-150): curve[a] and curve_t[a] (components
+ * swapped first) = mean over the interior of |d_row y' - d_col x'| (the divergence when maximize) for the
+ * field rotated by angles_deg[a]; the first best angle wins, the plain orientation wins ties; cx/cy = the
+ * field rotated by the chosen angle and orientation. */
+int cubo_hip_phase_rotation(int device, const double *nx, const double *ny, int32_t rows, int32_t cols,
+                            const double *angles_deg, int32_t nangles, int32_t maximize, double *degrees,
+                            int32_t *transposed, double *curve, double *curve_t, double *cx, double *cy);
+
 /* ---- iterative DPC (DPC.reconstruct, Sources/CuboPhase/DPC.swift:26-102) on the device, in Double, and
  * its adjoint. A plan fixes the scan (rows x cols), the padding (the grid is round(rows * padding) x
  * round(cols * padding), half-even, at most 2048 per dimension) and the sampling. Fields fx, fy are
