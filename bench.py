@@ -173,6 +173,37 @@ def diff_section(C):
     return res
 
 
+def pattern_section(C, cube):
+    """Selected-area patterns on the bandwidth cube: a disk of half the scan positions (exact integer kernel), every
+    position weighted (FP64 kernel) and the disk moved by the rounded centre of mass. GB/s over the bytes of the
+    patterns each pass reads."""
+    s = cube.shape
+    npos = s.scan_count
+    r, c = np.divmod(np.arange(npos), s.scan_columns)
+    disk = (r - s.scan_rows / 2) ** 2 + (c - s.scan_columns / 2) ** 2 < 0.5 * npos / np.pi
+    weights = 0.5 + (r + c) / (s.scan_rows + s.scan_columns)
+    com = C.CenterOfMass.measure(cube)
+    shift = C.PatternShift.rounded(C.OriginMap(com.row.values, com.column.values))
+    pbytes = s.detector_count * 2
+    cases = (("mask_mean", C.DiffractionStatistic.mean, C.ScanSelection.mask(disk), None, int(disk.sum())),
+             ("mask_max", C.DiffractionStatistic.max, C.ScanSelection.mask(disk), None, int(disk.sum())),
+             ("weights_mean", C.DiffractionStatistic.mean, C.ScanSelection.weights(weights), None, npos),
+             ("shifted_mask_mean", C.DiffractionStatistic.mean, C.ScanSelection.mask(disk), shift, int(disk.sum())))
+    out = {"selected_positions": int(disk.sum())}
+    for name, stat, scan, sh, n in cases:
+        one = C.ScanTiling.rows(s.scan_rows)                            # the whole scan as one tile: one launch
+        C.VirtualDiffraction.pattern(cube, stat, scan, shift=sh, tiling=one)        # warm-up
+        cube.kernel_time()
+        for _ in range(5):
+            C.VirtualDiffraction.pattern(cube, stat, scan, shift=sh, tiling=one)
+        tms, nl = cube.kernel_time()
+        ms = tms / max(1, nl)
+        out[name + "_ms"] = round(ms, 4)
+        out[name + "_gbs"] = round(n * pbytes / (ms / 1e3) / 1e9, 1)
+        log(f"pattern {name}: {ms:.3f} ms ({n * pbytes / (ms / 1e3) / 1e9:.0f} GB/s over the selected patterns)")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +216,7 @@ def main() -> int:
     ap.add_argument("--no-stack", action="store_true", help="skip the K = 64 mask-stack comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-cube batched-stack comparison")
     ap.add_argument("--no-diff", action="store_true", help="skip the 8 GiB differentiation workload")
+    ap.add_argument("--no-pattern", action="store_true", help="skip the selected-area pattern timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload runs on one MI355X "
@@ -275,6 +307,7 @@ def main() -> int:
         stack["k"] = int(stack_masks.shape[0])
         stack["mfma_gbs"] = round(cube_bytes / (stack["mfma_split_fp16_ms"] / 1e3) / 1e9, 1)
 
+    patterns = None if args.no_pattern else pattern_section(C, cube)
     batch = None if args.no_batch else batch_section(C)
     diff = None if args.no_diff else diff_section(C)
 
@@ -296,6 +329,7 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "mask_stack_k64": stack,
+        "selected_area": patterns,
         "batch_48": batch,
         "differentiation": diff,
     }

@@ -55,6 +55,17 @@ class ReduceView(ct.Structure):
                                                        "com_moment_column", "com_row", "com_column")]
 
 
+PATTERN_MEAN = 0
+PATTERN_MAX = 1
+TIES_FIRST = 0
+TIES_LATER = 1
+
+
+class PatternRequest(ct.Structure):
+    _fields_ = [("statistic", ct.c_int32), ("take", ct.c_void_p), ("weights", ct.c_void_p),
+                ("origin_row", ct.c_void_p), ("origin_col", ct.c_void_p), ("max_ties", ct.c_int32)]
+
+
 class SynthParams(ct.Structure):
     _fields_ = [("seed", ct.c_uint64), ("center_row", ct.c_double), ("center_column", ct.c_double),
                 ("radius", ct.c_double), ("edge", ct.c_double), ("counts", ct.c_double),
@@ -88,6 +99,15 @@ SIGNATURES = {
     "cubo_hip_reduce": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ReduceResult)]),
     "cubo_hip_reduce_path": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ct.c_int)]),
     "cubo_hip_cube_kernel_time": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),
+    "cubo_hip_pattern_begin": (ct.c_int, [ct.c_void_p, ct.POINTER(PatternRequest), ct.POINTER(ct.c_void_p)]),
+    "cubo_hip_pattern_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64]),
+    "cubo_hip_pattern_results": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "cubo_hip_pattern_reset": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_pattern_release": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_pattern_end": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "cubo_hip_pattern_cancel": (ct.c_int, [ct.c_void_p]),
+    "cubo_hip_pattern": (ct.c_int, [ct.c_void_p, ct.POINTER(PatternRequest), ct.c_void_p]),
+    "cubo_hip_pattern_path": (ct.c_int, [ct.c_void_p, ct.POINTER(PatternRequest), ct.POINTER(ct.c_int)]),
     "cubo_hip_stack_plan_create": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32,
                                               ct.POINTER(ct.c_void_p)]),
     "cubo_hip_stack_plan_method": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32)]),

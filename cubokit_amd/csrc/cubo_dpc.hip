@@ -17,6 +17,7 @@
 #include "../../include/cubo_hip.h"
 #include "dpc_kernels.hpp"
 #include "errors.hpp"
+#include "host_math.hpp"
 
 using cubo::DeviceGuard;
 using cubo::fail;
@@ -82,27 +83,6 @@ cubo::DpcGrid grid(const cubo_hip_dpc *d) {
 }
 
 // Field checks and the denominator mean(fx^2 + fy^2) (NumPy's pairwise order, as DPC.swift:49-54).
-double pairwise(const double *a, int64_t n) {
-    if (n < 8) {
-        double s = 0.0;
-        for (int64_t i = 0; i < n; ++i) s += a[i];
-        return s;
-    }
-    if (n <= 128) {
-        double r[8];
-        for (int j = 0; j < 8; ++j) r[j] = a[j];
-        int64_t i = 8;
-        for (; i < n - (n % 8); i += 8)
-            for (int j = 0; j < 8; ++j) r[j] += a[i + j];
-        double s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
-        for (; i < n; ++i) s += a[i];
-        return s;
-    }
-    int64_t n2 = n / 2;
-    n2 -= n2 % 8;
-    return pairwise(a, n2) + pairwise(a + n2, n - n2);
-}
-
 int field(cubo_hip_dpc *d, const double *fx, const double *fy, double *denom) {
     const int64_t N = (int64_t)d->R * d->C;
     std::vector<double> t(N);
@@ -111,7 +91,7 @@ int field(cubo_hip_dpc *d, const double *fx, const double *fy, double *denom) {
             return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the field must be finite (entry %lld)", (long long)k);
         t[k] = fx[k] * fx[k] + fy[k] * fy[k];
     }
-    *denom = pairwise(t.data(), N) / (double)N;
+    *denom = cubo::pairwise_sum(t.data(), N) / (double)N;
     if (!(*denom > 0)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the field is zero: DPC has nothing to reconstruct");
     CUBO_HIP_TRY(hipMemcpyAsync(d->fx, fx, N * 8, hipMemcpyHostToDevice, d->stream), "uploading the field");
     CUBO_HIP_TRY(hipMemcpyAsync(d->fy, fy, N * 8, hipMemcpyHostToDevice, d->stream), "uploading the field");

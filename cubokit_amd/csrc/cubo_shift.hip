@@ -17,12 +17,15 @@
 #include "../../include/cubo_hip.h"
 #include "cube.hpp"
 #include "errors.hpp"
+#include "host_math.hpp"
 #include "shift_kernels.hpp"
 #include "stack_prep.hpp"
 
 using cubo::DeviceGuard;
 using cubo::fail;
 using cubo::hip_fail;
+using cubo::pairwise_sum;
+using cubo::wrap_shift;
 
 struct cubo_hip_shift_plan {
     cubo_hip_cube *cube;
@@ -67,36 +70,6 @@ void free_plan(cubo_hip_shift_plan *p) {
     for (hipEvent_t e : p->ev)
         if (e) (void)hipEventDestroy(e);
     delete p;
-}
-
-// NumPy's pairwise float64 sum (numpy/core/src/umath/loops_utils.h.src pairwise_sum): fewer than 8 values
-// in order; up to 128 in eight interleaved partials; above, split at a multiple of 8 and recurse.
-double pairwise_sum(const double *a, int64_t n) {
-    if (n < 8) {
-        double s = 0.0;
-        for (int64_t i = 0; i < n; ++i) s += a[i];
-        return s;
-    }
-    if (n <= 128) {
-        double r[8];
-        for (int j = 0; j < 8; ++j) r[j] = a[j];
-        int64_t i = 8;
-        for (; i < n - (n % 8); i += 8)
-            for (int j = 0; j < 8; ++j) r[j] += a[i + j];
-        double s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
-        for (; i < n; ++i) s += a[i];
-        return s;
-    }
-    int64_t n2 = n / 2;
-    n2 -= n2 % 8;
-    return pairwise_sum(a, n2) + pairwise_sum(a + n2, n - n2);
-}
-
-// The integer shift wrapped into [0, n) (ShiftedImaging.swift:109-113: a floor shift modulo the extent).
-int wrap_shift(double v, int n) {
-    double m = std::fmod(v, (double)n);
-    if (m < 0) m += (double)n;
-    return ((int)m) % n;
 }
 
 template <typename T>

@@ -1,0 +1,409 @@
+// pattern_kernels.hip — selected-area, weighted and centre-shifted mean and maximum patterns for gfx950.
+//
+// Replaces the Metal kernel cubo_weighted_pattern (Sources/CuboVirtual/MetalReducers.swift:268-302: one thread
+// per pixel striding the positions of a tile) and the .cpu loops it mirrors (SelectedDiffraction.swift:110-179,
+// 225-271; ShiftedImaging.swift:125-167).
+//
+// Layout. Block b.x owns 64 * VEC consecutive pixels (64 lanes of VEC pixels, one 16-byte load per lane and
+// position, so a wave reads 1 KiB contiguous per position); block b.y is a position group. The four waves of a
+// block take the same pixels and consecutive quarters of the group's entries, in scan order, and are combined in
+// LDS in that order before one partial per (group, pixel) is written. A second kernel combines the groups in
+// order. Every entry's pattern is read once; positions outside the selection are not read.
+//
+// A rolled pattern (PatternShift.rounded): the output pixel (r, c) takes the pattern's pixel
+// ((r - sr) mod qy, (c - sc) mod qx). With 16-byte vectors the lane loads the two aligned vectors that hold its
+// VEC source pixels and picks them with the position's alignment offset, which is the same for every lane.
+#include "pattern_kernels.hpp"
+
+// TwoSum and the Double products round as written: no multiply-add contraction in this file.
+#pragma clang fp contract(off)
+
+namespace cubo {
+
+namespace {
+
+#define CUBO_GLOBAL __attribute__((address_space(1)))
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+
+constexpr int kPatWaves = 4;
+constexpr int kPatBlock = 64 * kPatWaves;
+
+__device__ __forceinline__ u32x4 load16(const char *p) {
+    return __builtin_nontemporal_load((const CUBO_GLOBAL u32x4 *)p);
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t load1(const char *p) {
+    if constexpr (sizeof(T) == 2)
+        return (uint32_t)__builtin_nontemporal_load((const CUBO_GLOBAL uint16_t *)p);
+    else
+        return __builtin_nontemporal_load((const CUBO_GLOBAL uint32_t *)p);
+}
+
+__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(us2, a), __builtin_bit_cast(us2, b)));
+}
+
+// Elements O .. O + VEC - 1 of the 32 bytes A:B, as a 16-byte vector (uint16: 8 elements, float: 4).
+template <int O, bool U16>
+__device__ __forceinline__ u32x4 take(const u32x4 &A, const u32x4 &B) {
+    const uint32_t S[8] = {A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w};
+    u32x4 r;
+    if constexpr (U16) {
+        constexpr int t = O / 2;
+        if constexpr (O % 2 == 0) {
+            r.x = S[t]; r.y = S[t + 1]; r.z = S[t + 2]; r.w = S[t + 3];
+        } else {
+            r.x = __builtin_amdgcn_alignbit(S[t + 1], S[t], 16);
+            r.y = __builtin_amdgcn_alignbit(S[t + 2], S[t + 1], 16);
+            r.z = __builtin_amdgcn_alignbit(S[t + 3], S[t + 2], 16);
+            r.w = __builtin_amdgcn_alignbit(S[t + 4], S[t + 3], 16);
+        }
+    } else {
+        r.x = S[O]; r.y = S[O + 1]; r.z = S[O + 2]; r.w = S[O + 3];
+    }
+    return r;
+}
+
+// o is uniform across the wave (it depends on the position's column shift only), so this is a scalar branch.
+template <bool U16>
+__device__ __forceinline__ u32x4 pick(const u32x4 &A, const u32x4 &B, int o) {
+    switch (o) {
+    case 0: return A;
+    case 1: return take<1, U16>(A, B);
+    case 2: return take<2, U16>(A, B);
+    case 3: return take<3, U16>(A, B);
+    case 4: if constexpr (U16) return take<4, U16>(A, B); else return A;
+    case 5: if constexpr (U16) return take<5, U16>(A, B); else return A;
+    case 6: if constexpr (U16) return take<6, U16>(A, B); else return A;
+    default: if constexpr (U16) return take<7, U16>(A, B); else return A;
+    }
+}
+
+// Element k of a loaded vector (VEC > 1) or the loaded element (VEC == 1), as Double.
+template <typename T, int VEC>
+__device__ __forceinline__ double elem(const u32x4 &v, int k) {
+    uint32_t bits;
+    if constexpr (VEC == 1) bits = v.x;
+    else if constexpr (sizeof(T) == 2) bits = (v[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+    else bits = v[k];
+    if constexpr (sizeof(T) == 2) return (double)bits;
+    else return (double)__builtin_bit_cast(float, bits);
+}
+
+// The maximum step: NaN wins; of equal values the first (m) or the later (x) is kept.
+template <bool FIRST, bool NAN_POSSIBLE>
+__device__ __forceinline__ double max_step(double m, double x) {
+    if constexpr (NAN_POSSIBLE) {
+        if constexpr (FIRST) return (x > m || __builtin_isnan(x)) ? x : m;
+        else return (x >= m || __builtin_isnan(x)) ? x : m;
+    } else {
+        if constexpr (FIRST) return x > m ? x : m;
+        else return x >= m ? x : m;
+    }
+}
+
+// (s, c) += x with Knuth's TwoSum: s the running sum, c the sum of the exact rounding errors.
+__device__ __forceinline__ void two_sum(double &s, double &c, double x) {
+    const double t = s + x;
+    const double bb = t - s;
+    c += (s - (t - bb)) + (x - bb);
+    s = t;
+}
+
+template <typename T, int VEC, bool INT, int STAT, bool SHIFT, bool FIRST>
+__global__ __launch_bounds__(kPatBlock) void pattern_kernel(PatternLaunch a) {
+    constexpr bool U16 = sizeof(T) == 2;
+    constexpr bool MEAN = STAT == kPatternMean, MAX = STAT == kPatternMax;
+    constexpr int U = SHIFT ? 4 : 8;                  // positions in flight per lane
+    constexpr int ES = (int)sizeof(T);
+    constexpr bool NAN_POSSIBLE = !U16;               // uint16 times a finite weight is never NaN
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t ndet = a.ndet;
+    const int64_t pix = ((int64_t)blockIdx.x * 64 + lane) * VEC;
+    const bool live = pix < ndet;
+    const int64_t lp = live ? pix : 0;                // lanes past the detector load pixel 0 and write nothing
+    const int qy = a.qy, qx = a.qx;
+    const int orow = SHIFT ? (int)(lp / qx) : 0, ocol = SHIFT ? (int)(lp % qx) : 0;
+
+    const int group = blockIdx.y;
+    const int64_t g0 = (int64_t)group * a.ppg, g1 = min(a.count, g0 + a.ppg);
+    const int64_t len = g1 > g0 ? g1 - g0 : 0;
+    const int64_t q = (len + kPatWaves - 1) / kPatWaves;
+    const int64_t j0 = g0 + min(len, (int64_t)wave * q), j1 = g0 + min(len, (int64_t)(wave + 1) * q);
+
+    const char *cube = (const char *)a.cube;
+    const int32_t *list = a.list;
+    const double *wts = a.w;
+    const uint32_t *shp = a.shift;
+    const int64_t pbytes = ndet * ES;
+
+    // per-lane state
+    uint32_t isum[VEC], imax[VEC];
+    double s[VEC], c[VEC], m[VEC];
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        isum[k] = 0u; imax[k] = 0u;
+        s[k] = 0.0; c[k] = 0.0; m[k] = -__builtin_inf();
+    }
+
+    // Entry j: its position, weight (1 without weights: x * 1 is x, bit for bit) and the lane's vector; with a
+    // roll the two aligned vectors that hold it and its alignment offset, picked only when consumed so that the
+    // loads of all U entries are in flight first.
+    struct In {
+        u32x4 A, B;
+        double w;
+        int o;
+    };
+    auto fetch = [&](int64_t j, In &in) {
+        const int64_t p = list ? (int64_t)list[a.first + j] : a.first + j;
+        in.w = wts ? wts[p] : 1.0;
+        in.o = 0;
+        const char *base = cube + p * pbytes;
+        if constexpr (!SHIFT) {
+            if constexpr (VEC == 1) in.A.x = load1<T>(base + lp * ES);
+            else in.A = load16(base + lp * ES);
+        } else {
+            const uint32_t sh = shp[p];
+            const int sr = (int)(sh >> 16), sc = (int)(sh & 0xffffu);
+            int r = orow - sr;
+            r += r < 0 ? qy : 0;
+            int u = ocol - sc;
+            u += u < 0 ? qx : 0;
+            if constexpr (VEC == 1) {
+                in.A.x = load1<T>(base + ((int64_t)r * qx + u) * ES);
+            } else {
+                in.o = (VEC - sc % VEC) % VEC;            // uniform: ocol and qx are multiples of VEC
+                const int c0 = u - in.o;
+                int c1 = c0 + VEC;
+                c1 = c1 == qx ? 0 : c1;
+                in.A = load16(base + ((int64_t)r * qx + c0) * ES);
+                in.B = load16(base + ((int64_t)r * qx + c1) * ES);
+            }
+        }
+    };
+    auto consume = [&](const In &in) {
+        u32x4 v;
+        if constexpr (SHIFT && VEC > 1) v = pick<U16>(in.A, in.B, in.o);
+        else v = in.A;
+        if constexpr (INT) {
+            if constexpr (VEC == 1) {
+                if constexpr (MEAN) isum[0] += v.x;
+                if constexpr (MAX) imax[0] = max(imax[0], v.x);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    // whole words accumulate (the low halves with the high ones carried in) and the high halves
+                    // apart; the low sums are recovered at the end
+                    if constexpr (MEAN) { isum[2 * k] += v[k]; isum[2 * k + 1] += v[k] >> 16; }
+                    if constexpr (MAX) imax[k] = pk_max_u16(imax[k], v[k]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                const double x = elem<T, VEC>(v, k) * in.w;
+                if constexpr (MEAN) two_sum(s[k], c[k], x);
+                if constexpr (MAX) m[k] = max_step<FIRST, NAN_POSSIBLE>(m[k], x);
+            }
+        }
+    };
+
+    int64_t j = j0;
+    for (; j + U <= j1; j += U) {
+        In in[U];
+#pragma unroll
+        for (int uu = 0; uu < U; ++uu) fetch(j + uu, in[uu]);
+#pragma unroll
+        for (int uu = 0; uu < U; ++uu) consume(in[uu]);
+    }
+    for (; j < j1; ++j) {
+        In in;
+        fetch(j, in);
+        consume(in);
+    }
+    if constexpr (INT && VEC > 1) {
+        if constexpr (MEAN) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t whole = isum[2 * k], hi = isum[2 * k + 1];
+                isum[2 * k] = whole - (hi << 16);         // exact: the low sum stays below 2^32
+            }
+        }
+        if constexpr (MAX) {
+#pragma unroll
+            for (int k = 3; k >= 0; --k) {
+                const uint32_t pk = imax[k];
+                imax[2 * k] = pk & 0xffffu;
+                imax[2 * k + 1] = pk >> 16;
+            }
+        }
+    }
+
+    // the four waves in order: waves 1..3 hand their state to wave 0 through LDS
+    constexpr int PER = INT ? ((MEAN ? VEC : 0) + (MAX ? VEC : 0)) : ((MEAN ? 2 * VEC : 0) + (MAX ? VEC : 0));
+    __shared__ uint64_t lds[(kPatWaves - 1) * PER * 64];
+    if (wave > 0) {
+        uint64_t *dst = lds + (size_t)(wave - 1) * PER * 64 + lane;
+        int e = 0;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            if constexpr (INT) {
+                if constexpr (MEAN) dst[64 * e++] = isum[k];
+                if constexpr (MAX) dst[64 * e++] = imax[k];
+            } else {
+                if constexpr (MEAN) { dst[64 * e++] = __builtin_bit_cast(uint64_t, s[k]); dst[64 * e++] = __builtin_bit_cast(uint64_t, c[k]); }
+                if constexpr (MAX) dst[64 * e++] = __builtin_bit_cast(uint64_t, m[k]);
+            }
+        }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    for (int w = 1; w < kPatWaves; ++w) {
+        const uint64_t *src = lds + (size_t)(w - 1) * PER * 64 + lane;
+        int e = 0;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            if constexpr (INT) {
+                if constexpr (MEAN) isum[k] += (uint32_t)src[64 * e++];
+                if constexpr (MAX) imax[k] = max(imax[k], (uint32_t)src[64 * e++]);
+            } else {
+                if constexpr (MEAN) {
+                    const double ps = __builtin_bit_cast(double, src[64 * e++]);
+                    const double pc = __builtin_bit_cast(double, src[64 * e++]);
+                    two_sum(s[k], c[k], ps);
+                    c[k] += pc;
+                }
+                if constexpr (MAX) m[k] = max_step<FIRST, NAN_POSSIBLE>(m[k], __builtin_bit_cast(double, src[64 * e++]));
+            }
+        }
+    }
+    if (!live) return;
+    const int64_t at = (int64_t)group * ndet + pix;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        if constexpr (INT) {
+            if constexpr (MEAN) ((uint32_t *)a.sum)[at + k] = isum[k];
+            if constexpr (MAX) ((uint32_t *)a.max)[at + k] = imax[k];
+        } else {
+            if constexpr (MEAN) ((double2 *)a.sum)[at + k] = make_double2(s[k], c[k]);
+            if constexpr (MAX) ((double *)a.max)[at + k] = m[k];
+        }
+    }
+}
+
+template <bool INT, int STAT, bool FIRST>
+__global__ void pattern_accumulate(const void *part_sum, const void *part_max, int ngroups, int64_t ndet, void *run_sum,
+                                   void *run_max, bool first, bool finish, double count, double *out) {
+    constexpr bool MEAN = STAT == kPatternMean, MAX = STAT == kPatternMax;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndet) return;
+    if constexpr (INT) {
+        if constexpr (MEAN) {
+            uint64_t sum = first ? 0u : ((const uint64_t *)run_sum)[i];
+            for (int g = 0; g < ngroups; ++g) sum += ((const uint32_t *)part_sum)[(int64_t)g * ndet + i];
+            if (finish) out[i] = (double)sum / count;
+            else ((uint64_t *)run_sum)[i] = sum;
+        }
+        if constexpr (MAX) {
+            uint32_t mx = first ? 0u : ((const uint32_t *)run_max)[i];
+            for (int g = 0; g < ngroups; ++g) mx = max(mx, ((const uint32_t *)part_max)[(int64_t)g * ndet + i]);
+            if (finish) out[i] = (double)mx;
+            else ((uint32_t *)run_max)[i] = mx;
+        }
+    } else {
+        if constexpr (MEAN) {
+            double2 sc = first ? make_double2(0.0, 0.0) : ((const double2 *)run_sum)[i];
+            for (int g = 0; g < ngroups; ++g) {
+                const double2 p = ((const double2 *)part_sum)[(int64_t)g * ndet + i];
+                two_sum(sc.x, sc.y, p.x);
+                sc.y += p.y;
+            }
+            if (finish) {
+                // CompensatedSum.value (CompensatedSum.swift:157-161): a non-finite sum as it is, -0 kept
+                double v;
+                if (!__builtin_isfinite(sc.x)) v = sc.x;
+                else if (sc.x == 0.0 && __builtin_signbit(sc.x) && sc.y == 0.0) v = sc.x;
+                else v = sc.x + sc.y;
+                out[i] = v / count;
+            } else {
+                ((double2 *)run_sum)[i] = sc;
+            }
+        }
+        if constexpr (MAX) {
+            double mx = first ? -__builtin_inf() : ((const double *)run_max)[i];
+            for (int g = 0; g < ngroups; ++g) mx = max_step<FIRST, true>(mx, ((const double *)part_max)[(int64_t)g * ndet + i]);
+            if (finish) out[i] = __builtin_isnan(mx) ? __builtin_nan("") : mx;
+            else ((double *)run_max)[i] = mx;
+        }
+    }
+}
+
+template <typename T, int VEC, bool INT, int STAT>
+hipError_t launch_t(const PatternLaunch &a, const PatternKind &k, dim3 grid, hipStream_t s) {
+    if (k.shift) {
+        if (k.first_wins) hipLaunchKernelGGL((pattern_kernel<T, VEC, INT, STAT, true, true>), grid, dim3(kPatBlock), 0, s, a);
+        else hipLaunchKernelGGL((pattern_kernel<T, VEC, INT, STAT, true, false>), grid, dim3(kPatBlock), 0, s, a);
+    } else {
+        if (k.first_wins) hipLaunchKernelGGL((pattern_kernel<T, VEC, INT, STAT, false, true>), grid, dim3(kPatBlock), 0, s, a);
+        else hipLaunchKernelGGL((pattern_kernel<T, VEC, INT, STAT, false, false>), grid, dim3(kPatBlock), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int VEC>
+hipError_t launch_v(const PatternLaunch &a, const PatternKind &k, dim3 grid, hipStream_t s) {
+    if (k.integer) {
+        if constexpr (sizeof(T) == 2)
+            return k.stat == kPatternMean ? launch_t<T, VEC, true, kPatternMean>(a, k, grid, s)
+                                          : launch_t<T, VEC, true, kPatternMax>(a, k, grid, s);
+        else
+            return hipErrorInvalidValue;
+    }
+    return k.stat == kPatternMean ? launch_t<T, VEC, false, kPatternMean>(a, k, grid, s)
+                                  : launch_t<T, VEC, false, kPatternMax>(a, k, grid, s);
+}
+
+}  // namespace
+
+int pattern_vec(bool u16, int64_t ndet, int qx, bool shift) {
+    const int v = u16 ? 8 : 4;
+    if ((ndet * (u16 ? 2 : 4)) % 16 != 0) return 1;
+    if (shift && qx % v != 0) return 1;
+    return v;
+}
+
+int pattern_block_pixels(int vec) { return 64 * vec; }
+
+hipError_t launch_pattern(const PatternLaunch &a, const PatternKind &k, hipStream_t s) {
+    if (a.count <= 0 || a.ngroups <= 0) return hipSuccess;
+    const int bp = pattern_block_pixels(k.vec);
+    const dim3 grid((unsigned)((a.ndet + bp - 1) / bp), (unsigned)a.ngroups);
+    if (k.u16) return k.vec == 8 ? launch_v<uint16_t, 8>(a, k, grid, s) : launch_v<uint16_t, 1>(a, k, grid, s);
+    return k.vec == 4 ? launch_v<float, 4>(a, k, grid, s) : launch_v<float, 1>(a, k, grid, s);
+}
+
+hipError_t launch_pattern_accumulate(const PatternKind &k, const void *part_sum, const void *part_max, int ngroups,
+                                     int64_t ndet, void *run_sum, void *run_max, bool first, bool finish, double count,
+                                     double *out, hipStream_t s) {
+    const dim3 grid((unsigned)((ndet + 255) / 256));
+#define CUBO_ACC(INT, STAT, FIRST)                                                                                     \
+    hipLaunchKernelGGL((pattern_accumulate<INT, STAT, FIRST>), grid, dim3(256), 0, s, part_sum, part_max, ngroups,    \
+                       ndet, run_sum, run_max, first, finish, count, out)
+    if (k.integer) {
+        if (k.stat == kPatternMean) CUBO_ACC(true, kPatternMean, true);
+        else CUBO_ACC(true, kPatternMax, true);
+    } else if (k.stat == kPatternMean) {
+        CUBO_ACC(false, kPatternMean, true);
+    } else if (k.first_wins) {
+        CUBO_ACC(false, kPatternMax, true);
+    } else {
+        CUBO_ACC(false, kPatternMax, false);
+    }
+#undef CUBO_ACC
+    return hipGetLastError();
+}
+
+}  // namespace cubo

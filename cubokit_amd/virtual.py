@@ -562,6 +562,160 @@ class VirtualDiffraction:
     def hip_is_available() -> bool:
         return hip_is_available()
 
+    @staticmethod
+    def pattern(source: Source, statistic: "DiffractionStatistic", scan: "ScanSelection" = None,
+                shift: "PatternShift" = None, path: ComputePath = ComputePath.hip, tiling: ScanTiling = None,
+                median_working_bytes: int = 256 << 20, progress: Progress = None,
+                is_cancelled: IsCancelled = None) -> DetectorImage:
+        """A virtual diffraction pattern over a selection of positions, optionally centre-shifted
+        (SelectedDiffraction.swift:46-108). The `.hip` path runs the mean and the max over `.all`, `.mask` and
+        `.weights`, each pattern optionally moved by PatternShift.rounded; the median and the Fourier shift are
+        refused, as the reference's `.metal` path refuses them."""
+        _require_hip(path)
+        statistic = DiffractionStatistic(statistic)
+        scan = scan if scan is not None else ScanSelection.all()
+        if median_working_bytes <= 0:
+            raise CuboVirtualError("invalidInput", "the median's working memory must be positive")
+        shape = _shape_of(source)
+        take, weights = scan.arrays(shape)
+        if statistic == DiffractionStatistic.median:
+            raise CuboVirtualError("invalidInput", "the .hip path covers the mean and the max over a selection")
+        if shift is not None and shift.fourier:
+            raise CuboVirtualError("invalidInput", "the .hip path moves patterns by the rounded shift, not the Fourier one")
+        if scan.kind == "all" and shift is None:
+            # VirtualDiffraction.mean / max (SelectedDiffraction.swift:89-92)
+            mm = VirtualDiffraction.mean_and_max(source, path, tiling, progress, is_cancelled)
+            return mm[0] if statistic == DiffractionStatistic.mean else mm[1]
+        return _pattern(source, statistic, take, weights, shift, L.TIES_FIRST, tiling, progress, is_cancelled)
+
+    @staticmethod
+    def mean_and_max_shifted(source: Source, shift: "PatternShift", tiling: ScanTiling = None,
+                             progress: Progress = None, is_cancelled: IsCancelled = None):
+        """meanAndMax(of:shift:) with a rounded shift (ShiftedImaging.swift:125-167): every pattern moved by the
+        map's mean minus its origin; the maximum keeps the later of equal values. Two passes, one per statistic."""
+        if shift.fourier:
+            raise CuboVirtualError("invalidInput", "the .hip path moves patterns by the rounded shift, not the Fourier one")
+        half = None if progress is None else (lambda f: progress(0.5 * f))
+        mean = _pattern(source, DiffractionStatistic.mean, None, None, shift, L.TIES_LATER, tiling, half, is_cancelled)
+        rest = None if progress is None else (lambda f: progress(0.5 + 0.5 * f))
+        mx = _pattern(source, DiffractionStatistic.max, None, None, shift, L.TIES_LATER, tiling, rest, is_cancelled)
+        return mean, mx
+
+
+class DiffractionStatistic(str, enum.Enum):
+    """How the patterns of a selection are combined (SelectedDiffraction.swift:14-17)."""
+    mean = "mean"
+    max = "max"
+    median = "median"
+
+
+class ScanSelection:
+    """Which scan positions a virtual diffraction pattern combines (SelectedDiffraction.swift:19-27)."""
+
+    def __init__(self, kind: str, values=None):
+        self.kind = kind
+        self.values = values
+
+    @staticmethod
+    def all() -> "ScanSelection":
+        return ScanSelection("all")
+
+    @staticmethod
+    def mask(flags) -> "ScanSelection":
+        return ScanSelection("mask", np.asarray(flags, bool).reshape(-1))
+
+    @staticmethod
+    def weights(values) -> "ScanSelection":
+        return ScanSelection("weights", np.asarray(values, np.float64).reshape(-1))
+
+    def arrays(self, shape: CubeShape):
+        """(take, weights) for the C ABI, with the reference's refusals (SelectedDiffraction.swift:64-79)."""
+        if self.kind == "all":
+            return None, None
+        if self.values.size != shape.scan_count:
+            raise CuboVirtualError("maskLengthMismatch", f"expected {shape.scan_count}, received {self.values.size}")
+        if self.kind == "mask":
+            if not self.values.any():
+                raise CuboVirtualError("invalidInput", "the selection holds no scan position")
+            return np.ascontiguousarray(self.values.astype(np.uint8)), None
+        if not np.all(np.isfinite(self.values)):
+            raise CuboVirtualError("invalidInput", "the weights must be finite")
+        return None, np.ascontiguousarray(self.values)
+
+
+class OriginMap:
+    """An origin per scan position (OriginMeasurement.swift:14-28): row and column maps and their NumPy means."""
+
+    def __init__(self, row, column):
+        r, c = np.asarray(getattr(row, "values", row), np.float64), np.asarray(getattr(column, "values", column), np.float64)
+        if r.shape != c.shape:
+            raise CuboVirtualError("originMapMismatch", "the row and column maps differ in shape")
+        self.row, self.column = np.ascontiguousarray(r.reshape(-1)), np.ascontiguousarray(c.reshape(-1))
+
+
+class PatternShift:
+    """How each pattern is moved before the mean and maximum (ShiftedImaging.swift:23-29)."""
+
+    def __init__(self, origin: OriginMap, fourier: bool):
+        self.origin, self.fourier = origin, fourier
+
+    @staticmethod
+    def rounded(origin: OriginMap) -> "PatternShift":
+        return PatternShift(origin, False)
+
+    @staticmethod
+    def fourier_shift(origin: OriginMap) -> "PatternShift":
+        return PatternShift(origin, True)
+
+
+def _pattern(source: Source, statistic: DiffractionStatistic, take, weights, shift: Optional[PatternShift], ties: int,
+             tiling: ScanTiling, progress: Progress, is_cancelled: IsCancelled) -> DetectorImage:
+    tiling = tiling or ScanTiling.automatic
+    if is_cancelled is not None and is_cancelled():
+        raise CancellationError()
+    cube, temporary = _resident(source, tiling)
+    try:
+        shape = cube.shape
+        if shift is not None and shift.origin.row.size != shape.scan_count:
+            raise CuboVirtualError("originMapMismatch", f"the origin map has {shift.origin.row.size} positions, "
+                                   f"the scan {shape.scan_count}")
+        req = L.PatternRequest(statistic=L.PATTERN_MEAN if statistic == DiffractionStatistic.mean else L.PATTERN_MAX,
+                               take=None if take is None else take.ctypes.data,
+                               weights=None if weights is None else weights.ctypes.data,
+                               origin_row=None if shift is None else shift.origin.row.ctypes.data,
+                               origin_col=None if shift is None else shift.origin.column.ctypes.data,
+                               max_ties=ties)
+        out = np.empty(shape.detector_count)
+        h = ct.c_void_p()
+        _check(L.lib().cubo_hip_pattern_begin(cube.handle, ct.byref(req), ct.byref(h)), "starting the pattern")
+        try:
+            for rg in tiling.ranges(shape):
+                if is_cancelled is not None and is_cancelled():
+                    raise CancellationError()
+                _check(L.lib().cubo_hip_pattern_rows(h, rg.start, len(rg)), "pattern rows")
+                if progress is not None:
+                    progress(rg.stop / shape.scan_rows)
+        except BaseException:
+            L.lib().cubo_hip_pattern_cancel(h)
+            raise
+        _check(L.lib().cubo_hip_pattern_end(h, out.ctypes.data), "finishing the pattern")
+        return DetectorImage(shape.detector_rows, shape.detector_columns, out, ComputePath.hip)
+    finally:
+        if temporary:
+            cube.release()
+
+
+def pattern_path(source: "HIPResidentCube", take=None, weights=None) -> int:
+    """1 = exact integer kernel, 2 = FP64 kernel, for a selected pattern on this cube."""
+    t = None if take is None else np.ascontiguousarray(np.asarray(take, bool).astype(np.uint8))
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    req = L.PatternRequest(statistic=L.PATTERN_MEAN, take=None if t is None else t.ctypes.data,
+                           weights=None if w is None else w.ctypes.data, origin_row=None, origin_col=None,
+                           max_ties=L.TIES_FIRST)
+    p = ct.c_int()
+    _check(L.lib().cubo_hip_pattern_path(source.handle, ct.byref(req), ct.byref(p)), "path")
+    return p.value
+
 
 class CenterOfMass:
     @staticmethod

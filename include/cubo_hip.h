@@ -152,6 +152,46 @@ int cubo_hip_reduce_path(cubo_hip_cube *cube, const cubo_hip_reduce_request *req
  * previous query, from HIP events on the cube's stream: total milliseconds and launch count. */
 int cubo_hip_cube_kernel_time(cubo_hip_cube *cube, double *total_ms, int64_t *launches);
 
+/* ---- selected-area and centre-shifted patterns: VirtualDiffraction.pattern(of:statistic:scan:shift:) for the
+ * mean and the max over a .mask or .weights selection or every position, each pattern optionally moved by
+ * PatternShift.rounded (Sources/CuboVirtual/SelectedDiffraction.swift:46-179, 225-271; the Metal path
+ * MetalReducers.weightedPattern, MetalReducers.swift:575-617), and VirtualDiffraction.meanAndMax(of:shift:)
+ * with .rounded, one statistic per pass (ShiftedImaging.swift:125-167). The median and PatternShift.fourier
+ * are not on the device path. ---- */
+enum { CUBO_HIP_PATTERN_MEAN = 0, CUBO_HIP_PATTERN_MAX = 1 };
+enum {
+    CUBO_HIP_TIES_FIRST = 0,   /* of equal maxima the first in scan order is kept (np.max; pattern(of:...),
+                                  SelectedDiffraction.swift:161, 260) */
+    CUBO_HIP_TIES_LATER = 1    /* the later is kept (Swift.max; meanAndMax(of:shift:), ShiftedImaging.swift:160) */
+};
+typedef struct {
+    int32_t statistic;         /* CUBO_HIP_PATTERN_MEAN or CUBO_HIP_PATTERN_MAX */
+    const uint8_t *take;       /* scanCount flags, non-zero = selected (ScanSelection.mask), or NULL */
+    const double *weights;     /* scanCount finite weights (ScanSelection.weights), or NULL; not with take */
+    const double *origin_row;  /* PatternShift.rounded: the origin map, scanCount each, or both NULL: each */
+    const double *origin_col;  /*   pattern is rolled by round_half_even(mean - origin) before it is combined */
+    int32_t max_ties;          /* CUBO_HIP_TIES_FIRST or CUBO_HIP_TIES_LATER */
+} cubo_hip_pattern_request;
+/* Mean: sum of value (times weight) / divisor, the divisor being the selected count, the Neumaier sum of the
+ * weights in scan order, or scanCount. Max: of value (times weight), seeded at -infinity, NaN winning.
+ * Refusals (CUBO_HIP_ERR_INVALID_ARGUMENT), as SelectedDiffraction.swift:60-102 and ShiftedImaging.swift:99-106:
+ * non-finite weights, an empty mask, weights summing to zero for the mean, a non-finite origin map.
+ * A session takes scan-row tiles in order like the fused reduction and writes detectorCount Doubles. */
+typedef struct cubo_hip_pattern_session cubo_hip_pattern_session;
+int cubo_hip_pattern_begin(cubo_hip_cube *cube, const cubo_hip_pattern_request *request, cubo_hip_pattern_session **out);
+int cubo_hip_pattern_rows(cubo_hip_pattern_session *pattern, int64_t row0, int64_t nrows);
+/* results waits and copies (the session is kept), reset starts a new pass, release frees, end = results + release,
+ * cancel frees without results. */
+int cubo_hip_pattern_results(cubo_hip_pattern_session *pattern, double *out);
+int cubo_hip_pattern_reset(cubo_hip_pattern_session *pattern);
+int cubo_hip_pattern_release(cubo_hip_pattern_session *pattern);
+int cubo_hip_pattern_end(cubo_hip_pattern_session *pattern, double *out);
+int cubo_hip_pattern_cancel(cubo_hip_pattern_session *pattern);
+/* One call: begin, all rows as one tile, end. */
+int cubo_hip_pattern(cubo_hip_cube *cube, const cubo_hip_pattern_request *request, double *out);
+/* 1 = exact integer kernel (uint16 cube, no weights), 2 = FP64 (compensated sums). */
+int cubo_hip_pattern_path(cubo_hip_cube *cube, const cubo_hip_pattern_request *request, int *path);
+
 /* ---- the mask stack: Y = X W for K weight rows in one pass, X the (scanCount x detectorCount) cube.
  * {1, q_row, q_col, masks...} gives K virtual images and the centre-of-mass sums together. No reference
  * counterpart (the reference runs VirtualImaging.image once per mask, CPUReducers.swift:13-39). ---- */

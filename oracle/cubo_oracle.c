@@ -227,6 +227,95 @@ static double mean_plain(const double *a, int64_t n) {   /* the origin map mean 
     return oc_pairwise_sum(a, n) / (double)n;
 }
 
+/* ---------------------------------------------------------------------------------------------- */
+/* Selected-area and centre-shifted patterns.                                                        */
+/*  - pattern over .mask / .weights without a shift: SelectedDiffraction.swift:110-179 — per pixel a */
+/*    TwoSum over the picked positions in scan order from +0 (CompensatedSum.add: Double(x) * w      */
+/*    formed first), value() / count or / Neumaier(weights); the max of Double(x) * w seeded -inf,    */
+/*    NaN wins, of equal values the first kept (v > m ? v : m);                                       */
+/*  - pattern with PatternShift.rounded: SelectedDiffraction.swift:225-271 — each picked pattern      */
+/*    rolled by wrap(round_half_even(meanRow - row[k])), ..., Neumaier per pixel, first kept;          */
+/*  - meanAndMax(of:shift: .rounded): ShiftedImaging.swift:125-167 — every position, Neumaier per     */
+/*    pixel / scanCount, the max with Swift.max (the later of equal values kept).                     */
+/* ---------------------------------------------------------------------------------------------- */
+
+static inline double max_first(double m, double v) {
+    if (isnan(m) || isnan(v)) return NAN;
+    return v > m ? v : m;
+}
+static inline double max_later(double m, double v) {
+    if (isnan(m) || isnan(v)) return NAN;
+    return v >= m ? v : m;
+}
+
+int oc_pattern(const float *cube, int64_t npos, int qy, int qx, int stat, const uint8_t *take, const double *w,
+               const double *orow, const double *ocol, int ties_later, double *out) {
+    const int64_t ndet = (int64_t)qy * qx;
+    if (take && w) return OC_INVALID_PARAMETER;
+    if (w)
+        for (int64_t k = 0; k < npos; ++k)
+            if (!isfinite(w[k])) return OC_NON_FINITE;
+    int64_t selected = 0;
+    for (int64_t k = 0; k < npos; ++k) selected += take ? (take[k] != 0) : 1;
+    if (selected == 0) return OC_INVALID_PARAMETER;
+    const int shifted = orow != NULL;
+    double mr = 0, mc = 0;
+    if (shifted) {
+        for (int64_t k = 0; k < npos; ++k)
+            if (!isfinite(orow[k]) || !isfinite(ocol[k])) return OC_NON_FINITE;
+        mr = mean_plain(orow, npos);
+        mc = mean_plain(ocol, npos);
+    }
+    double *sum = (double *)malloc(sizeof(double) * ndet), *corr = (double *)malloc(sizeof(double) * ndet);
+    double *top = (double *)malloc(sizeof(double) * ndet), *moved = (double *)malloc(sizeof(double) * ndet);
+    for (int64_t i = 0; i < ndet; ++i) { sum[i] = 0.0; corr[i] = 0.0; top[i] = -INFINITY; }
+    neumaier total = {0, 0};
+    for (int64_t k = 0; k < npos; ++k) {
+        if (take && !take[k]) continue;
+        const double wk = w ? w[k] : 1.0;
+        nm_add(&total, wk);
+        const float *x = cube + k * ndet;
+        if (shifted) {
+            /* np.roll: out[(i + sr) % qy, (j + sc) % qx] = p[i, j] */
+            const int sr = wrap_shift(nearbyint(mr - orow[k]), qy), sc = wrap_shift(nearbyint(mc - ocol[k]), qx);
+            for (int i = 0; i < qy; ++i)
+                for (int j = 0; j < qx; ++j) moved[(int64_t)((i + sr) % qy) * qx + (j + sc) % qx] = (double)x[(int64_t)i * qx + j];
+        } else {
+            for (int64_t i = 0; i < ndet; ++i) moved[i] = (double)x[i];
+        }
+        for (int64_t i = 0; i < ndet; ++i) {
+            const double v = w ? moved[i] * wk : moved[i];
+            if (stat == 0) {
+                if (shifted) {               /* Neumaier (Neumaier.add, CenterOfMass.swift:86-90) */
+                    neumaier n = {sum[i], corr[i]};
+                    nm_add(&n, v);
+                    sum[i] = n.sum; corr[i] = n.corr;
+                } else {                     /* TwoSum (CompensatedSum.scalar, CompensatedSum.swift:146-152) */
+                    const double a = sum[i], t = a + v, bb = t - a;
+                    corr[i] += (a - (t - bb)) + (v - bb);
+                    sum[i] = t;
+                }
+            } else {
+                top[i] = ties_later ? max_later(top[i], v) : max_first(top[i], v);
+            }
+        }
+    }
+    int st = OC_OK;
+    if (stat == 0) {
+        const double norm = w ? nm_value(&total) : (double)selected;
+        if (norm == 0) st = OC_ZERO_FIELD;
+        for (int64_t i = 0; i < ndet && st == OC_OK; ++i) {
+            const double v = shifted ? sum[i] + corr[i] : comp_value(sum[i], corr[i]);
+            out[i] = v / norm;
+        }
+    } else {
+        for (int64_t i = 0; i < ndet; ++i) out[i] = top[i];
+    }
+    free(sum); free(corr); free(top); free(moved);
+    return st;
+}
+
+
 /* value of roll(mask, sr, sc) at (i, j) = mask[(i - sr) mod qy, (j - sc) mod qx] */
 static inline double rolled(const double *m, int qy, int qx, int sr, int sc, int i, int j) {
     int a = i - sr; a %= qy; if (a < 0) a += qy;

@@ -63,6 +63,13 @@ int oc_vi_bilinear_grad_origin(const float *cube, int64_t npos, int qy, int qx, 
                                const double *origin_r, const double *origin_c, const double *dy,
                                double *d_origin_r, double *d_origin_c);
 
+/* Selected-area and centre-shifted patterns: stat 0 = mean, 1 = max; take (selection flags) or w (weights) or
+ * neither; orow/ocol an origin map for PatternShift.rounded or NULL; ties_later: Swift.max's tie rule
+ * (meanAndMax(of:shift:)) instead of the first-kept one. OC_INVALID_PARAMETER for an empty selection or take with
+ * w, OC_NON_FINITE for non-finite weights or origins, OC_ZERO_FIELD for a mean over weights summing to zero. */
+int oc_pattern(const float *cube, int64_t npos, int qy, int qx, int stat, const uint8_t *take, const double *w,
+               const double *orow, const double *ocol, int ties_later, double *out);
+
 /* NumPy pairwise float64 sum. */
 double oc_pairwise_sum(const double *a, int64_t n);
 

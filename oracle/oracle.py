@@ -46,6 +46,8 @@ def _load():
         "oc_vi_bilinear": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]),
         "oc_vi_bilinear_grad_w": (None, [_fp, _i64, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]),
         "oc_vi_bilinear_grad_origin": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "oc_pattern": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                 ct.c_void_p, ct.c_int, _dp]),
         "oc_pairwise_sum": (ct.c_double, [_dp, _i64]),
         "oc_fft2d": (None, [_dp, _dp, ct.c_int, ct.c_int, ct.c_int]),
         "oc_plane_fit": (ct.c_int, [_dp, ct.c_int, ct.c_int, ct.c_double, _dp]),
@@ -138,6 +140,34 @@ def mean_max(cube):
     mx = np.zeros(qy * qx)
     lib().oc_mean_max(x, x.shape[0], x.shape[1], mean, mx)
     return mean, mx
+
+
+def pattern(cube, statistic: str, take=None, weights=None, origin=None, ties: str = "first") -> np.ndarray:
+    """VirtualDiffraction.pattern / meanAndMax(of:shift:) for 'mean' or 'max' (SelectedDiffraction.swift:110-179,
+    225-271; ShiftedImaging.swift:125-167). take: flags per position; weights: Doubles per position; origin:
+    (row map, column map) for PatternShift.rounded; ties 'first' (np.max) or 'later' (Swift.max)."""
+    x, ry, rx, qy, qx = _flat(cube)
+    npos = ry * rx
+    keep = []
+
+    def ptr(a, dt):
+        if a is None:
+            return None
+        arr = np.ascontiguousarray(np.asarray(a).reshape(-1), dt)
+        if arr.size != npos:
+            raise OracleError("selection length mismatch")
+        keep.append(arr)
+        return arr.ctypes.data
+    t = ptr(None if take is None else np.asarray(take, bool).astype(np.uint8), np.uint8)
+    w = ptr(weights, np.float64)
+    orow = ocol = None
+    if origin is not None:
+        orow, ocol = ptr(origin[0], np.float64), ptr(origin[1], np.float64)
+    out = np.zeros(qy * qx)
+    st = lib().oc_pattern(x, npos, qy, qx, 0 if statistic == "mean" else 1, t, w, orow, ocol,
+                          1 if ties == "later" else 0, out)
+    _check(st, f"pattern {statistic}")
+    return out
 
 
 def com_sums(cube, weights=None):
