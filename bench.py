@@ -204,6 +204,40 @@ def pattern_section(C, cube):
     return out
 
 
+def ingest_section(C, cube):
+    """Host to HBM on the bandwidth cube (2 GiB of uint16): an .npy file in the page cache loaded by load_npy, the
+    same array fed from host memory into a resident cube, and a mean-and-max pass over the host array streamed in
+    16-row tiles (PCIe included; the copies overlap the reduction)."""
+    import tempfile
+    s = cube.shape
+    host = cube.read_scan_tile(range(0, s.scan_rows))
+    nbytes = host.nbytes
+    out = {"bytes": int(nbytes)}
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "cube.npy")
+        np.save(path, host)
+        C.HIPResidentCube.load_npy(path).release()                     # warm the page cache and the path
+        t0 = time.perf_counter()
+        loaded = C.HIPResidentCube.load_npy(path)
+        el = time.perf_counter() - t0
+        loaded.release()
+    out["load_npy_gbs"] = round(nbytes / el / 1e9, 2)
+    t0 = time.perf_counter()
+    fed = C.HIPResidentCube(host, tiling=C.ScanTiling.rows(16))
+    fed.read_scan_tile(range(0, 1))                                     # waits for the copies
+    el = time.perf_counter() - t0
+    fed.release()
+    out["feed_resident_gbs"] = round(nbytes / el / 1e9, 2)
+    C.VirtualDiffraction.mean_and_max(host, tiling=C.ScanTiling.rows(16))
+    t0 = time.perf_counter()
+    C.VirtualDiffraction.mean_and_max(host, tiling=C.ScanTiling.rows(16))
+    el = time.perf_counter() - t0
+    out["streamed_mean_max_gbs"] = round(nbytes / el / 1e9, 2)
+    log(f"ingest: load_npy {out['load_npy_gbs']} GB/s, feed {out['feed_resident_gbs']} GB/s, streamed mean+max "
+        f"{out['streamed_mean_max_gbs']} GB/s (PCIe included)")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +251,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-cube batched-stack comparison")
     ap.add_argument("--no-diff", action="store_true", help="skip the 8 GiB differentiation workload")
     ap.add_argument("--no-pattern", action="store_true", help="skip the selected-area pattern timings")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the host-to-HBM ingestion timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload runs on one MI355X "
@@ -308,6 +343,7 @@ def main() -> int:
         stack["mfma_gbs"] = round(cube_bytes / (stack["mfma_split_fp16_ms"] / 1e3) / 1e9, 1)
 
     patterns = None if args.no_pattern else pattern_section(C, cube)
+    ingest = None if args.no_ingest else ingest_section(C, cube)
     batch = None if args.no_batch else batch_section(C)
     diff = None if args.no_diff else diff_section(C)
 
@@ -330,6 +366,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "mask_stack_k64": stack,
         "selected_area": patterns,
+        "ingest": ingest,
         "batch_48": batch,
         "differentiation": diff,
     }

@@ -8,11 +8,13 @@ from .virtual import (CancellationError, CenterOfMass, CenterOfMassMaps, Compute
                       ShiftedStack, StackBatch, center_of_mass_from_stack,
                       ResidentCubeError, ScanImage, ScanTiling, UnavailableReason, VirtualDiffraction, VirtualImaging,
                       device_count, device_info, first_look, hip_is_available, reduce_path,
-                      DiffractionStatistic, OriginMap, PatternShift, ScanSelection, pattern_path)
+                      DiffractionStatistic, OriginMap, PatternShift, ScanSelection, pattern_path,
+                      HIPStreamedCube, NPYFile, NPYHeader)
 
 __all__ = ["CancellationError", "CenterOfMass", "CenterOfMassMaps", "ComputePath", "CubeShape", "CuboError",
            "CuboVirtualError", "DetectorGeometry", "DetectorImage", "DetectorMask", "FirstLook", "HIPResidentCube",
            "MaskStack", "ShiftedStack", "StackBatch", "center_of_mass_from_stack",
            "ResidentCubeError", "ScanImage", "ScanTiling", "UnavailableReason", "VirtualDiffraction",
            "VirtualImaging", "device_count", "device_info", "first_look", "hip_is_available", "reduce_path",
-           "DiffractionStatistic", "OriginMap", "PatternShift", "ScanSelection", "pattern_path"]
+           "DiffractionStatistic", "OriginMap", "PatternShift", "ScanSelection", "pattern_path",
+           "HIPStreamedCube", "NPYFile", "NPYHeader"]

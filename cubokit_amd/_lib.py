@@ -20,6 +20,7 @@ ERR_HIP = -5
 ERR_UNSUPPORTED = -6
 ERR_INCOMPLETE = -7
 ERR_EXCEEDS_BUFFER_LIMIT = -8
+ERR_FILE = -9
 
 U16 = 1
 F32 = 2
@@ -53,6 +54,15 @@ class ReduceResult(ct.Structure):
 class ReduceView(ct.Structure):
     _fields_ = [(n, ct.POINTER(ct.c_double)) for n in ("images", "mean", "max", "com_intensity", "com_moment_row",
                                                        "com_moment_column", "com_row", "com_column")]
+
+
+# stored element types, in ElementType's order
+ELEMENT_TYPES = ["float32", "float64", "uint8", "int8", "uint16", "int16", "uint32", "int32", "uint64", "int64"]
+
+
+class NpyHeader(ct.Structure):
+    _fields_ = [("major_version", ct.c_int32), ("minor_version", ct.c_int32), ("element_type", ct.c_int32),
+                ("rank", ct.c_int32), ("shape", ct.c_int64 * 32), ("data_offset", ct.c_int64)]
 
 
 PATTERN_MEAN = 0
@@ -99,6 +109,13 @@ SIGNATURES = {
     "cubo_hip_reduce": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ReduceResult)]),
     "cubo_hip_reduce_path": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ct.c_int)]),
     "cubo_hip_cube_kernel_time": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),
+    "cubo_hip_npy_read_header": (ct.c_int, [ct.c_char_p, ct.POINTER(NpyHeader)]),
+    "cubo_hip_cube_load_file": (ct.c_int, [ct.c_int, ct.c_char_p, ct.c_int64, ct.c_int32, ct.POINTER(ct.c_int64),
+                                           ct.POINTER(ct.c_void_p)]),
+    "cubo_hip_cube_load_npy": (ct.c_int, [ct.c_int, ct.c_char_p, ct.POINTER(ct.c_void_p)]),
+    "cubo_hip_cube_create_streamed": (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int64), ct.c_int, ct.c_int64,
+                                                 ct.POINTER(ct.c_void_p)]),
+    "cubo_hip_cube_feed_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p, ct.c_int32]),
     "cubo_hip_pattern_begin": (ct.c_int, [ct.c_void_p, ct.POINTER(PatternRequest), ct.POINTER(ct.c_void_p)]),
     "cubo_hip_pattern_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64]),
     "cubo_hip_pattern_results": (ct.c_int, [ct.c_void_p, ct.c_void_p]),

@@ -7,6 +7,10 @@
 #include <utility>
 #include <vector>
 
+namespace cubo {
+struct Feed;
+}
+
 struct cubo_hip_cube {
     int device;
     int64_t shape[4];       // [scanRows, scanColumns, detectorRows, detectorColumns]
@@ -16,6 +20,11 @@ struct cubo_hip_cube {
     void *data;             // one HBM buffer, storage order
     hipStream_t stream;     // every launch on this cube goes to this stream
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;   // fused-kernel launches not yet queried
+    // A streamed cube holds no whole-cube buffer: `data` is NULL and the scan-row tiles fed by
+    // cubo_hip_cube_feed_rows live in the feed's two-slot device ring. Every cube that has been fed owns a feed
+    // (pinned double buffers, a copy stream).
+    bool streamed = false;
+    cubo::Feed *feed = nullptr;
 };
 
 #include "errors.hpp"
@@ -30,6 +39,18 @@ inline int rows_range(const cubo_hip_cube *c, int64_t row0, int64_t nrows) {
                     (long long)(row0 + nrows), (long long)c->shape[0]);
     return CUBO_HIP_OK;
 }
+
+// The device address of scan rows [row0, row0 + nrows): in the resident buffer, or in the ring slot a streamed
+// cube was fed them into (an error when they are not staged). Defined in cubo_io.hip.
+int rows_data(const cubo_hip_cube *c, int64_t row0, int64_t nrows, const void **out);
+// Refuses a streamed cube for calls that read the whole cube.
+inline int require_resident(const cubo_hip_cube *c, const char *what) {
+    if (c->streamed)
+        return fail(CUBO_HIP_ERR_UNSUPPORTED, "%s needs a resident cube, not a streamed one", what);
+    return CUBO_HIP_OK;
+}
+// Releases a cube's feed (waits for its copies). Defined in cubo_io.hip.
+void free_feed(cubo_hip_cube *c);
 
 inline int compute_units(int dev) {
     int cus = 0;

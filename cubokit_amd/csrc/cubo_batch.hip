@@ -176,6 +176,7 @@ int cubo_hip_batch_create(int32_t n, cubo_hip_cube *const *cubes, const float *c
         const cubo_hip_cube *c = cubes[i];
         if (!c || !masks[i]) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "batch item %d is NULL", i);
         if (c->device != dev) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "a batch runs on one device");
+        if (int st = cubo::require_resident(c, "a batch")) return st;
         if (c->dtype != CUBO_HIP_U16 || c->ndet % 8 != 0)
             return fail(CUBO_HIP_ERR_UNSUPPORTED, "batch item %d: the batched stack needs a uint16 cube whose "
                         "pattern length is a multiple of 8", i);

@@ -155,8 +155,9 @@ int cubo_hip_cube_destroy(cubo_hip_cube *c) {
     if (!c) return CUBO_HIP_OK;
     DeviceGuard g(c->device);
     (void)hipStreamSynchronize(c->stream);
+    cubo::free_feed(c);
     for (auto &ev : c->timed) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
-    (void)hipFree(c->data);
+    if (c->data) (void)hipFree(c->data);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return CUBO_HIP_OK;
@@ -164,6 +165,7 @@ int cubo_hip_cube_destroy(cubo_hip_cube *c) {
 
 int cubo_hip_cube_clone(const cubo_hip_cube *src, cubo_hip_cube **out) {
     if (!src || !out) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (int st = cubo::require_resident(src, "cloning")) return st;
     if (int st = cubo_hip_cube_create(src->device, src->shape, src->dtype, out)) return st;
     cubo_hip_cube *d = *out;
     DeviceGuard g(src->device);
@@ -185,6 +187,7 @@ int cubo_hip_cube_byte_count(const cubo_hip_cube *c, uint64_t *bytes) {
 
 int cubo_hip_cube_upload_rows(cubo_hip_cube *c, int64_t row0, int64_t nrows, const void *host) {
     if (!c || (!host && nrows > 0)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (int st = cubo::require_resident(c, "uploading rows (a streamed cube is fed)")) return st;
     if (int st = rows_range(c, row0, nrows)) return st;
     DeviceGuard g(c->device);
     const uint64_t row_bytes = (uint64_t)c->shape[1] * c->ndet * element_size(c->dtype);
@@ -196,6 +199,7 @@ int cubo_hip_cube_upload_rows(cubo_hip_cube *c, int64_t row0, int64_t nrows, con
 
 int cubo_hip_cube_read_rows(cubo_hip_cube *c, int64_t row0, int64_t nrows, void *host) {
     if (!c || (!host && nrows > 0)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (int st = cubo::require_resident(c, "reading rows")) return st;
     if (int st = rows_range(c, row0, nrows)) return st;
     DeviceGuard g(c->device);
     const uint64_t row_bytes = (uint64_t)c->shape[1] * c->ndet * element_size(c->dtype);

@@ -217,11 +217,16 @@ int cubo_hip_pattern_rows(cubo_hip_pattern_session *p, int64_t row0, int64_t nro
     const bool finish = row0 + nrows == c->shape[0];
     hipStream_t s = c->stream;
     int ngroups = 0;
+    // The kernel addresses position p at base + p * pattern bytes; for a streamed cube the base is the ring slot
+    // moved back by the tile's first position (only positions of this tile are read).
+    const void *tile = nullptr;
+    if (int st = cubo::rows_data(c, row0, nrows, &tile)) return st;
+    const void *cube_base = (const void *)((uintptr_t)tile - (uintptr_t)pos0 * (uintptr_t)(c->ndet * cubo::element_size(c->dtype)));
     if (count > 0) {
         int64_t ppg = 0;
         plan(p, count, &ngroups, &ppg);
         cubo::PatternLaunch a{};
-        a.cube = c->data;
+        a.cube = cube_base;
         a.ndet = c->ndet;
         a.qy = (int)c->shape[2];
         a.qx = (int)c->shape[3];

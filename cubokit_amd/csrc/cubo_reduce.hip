@@ -318,6 +318,8 @@ int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows) {
     const int64_t pos0 = row0 * c->shape[1];
     const int64_t npos = nrows * c->shape[1];
     const int ndet = (int)c->ndet;
+    const void *tile = nullptr;   // the tile's first element: in the resident buffer or a streamed cube's ring
+    if (int st = cubo::rows_data(c, row0, nrows, &tile)) return st;
     int ngroups = 0, ppg = 0, ppw = 0;
     plan(r, npos, &ngroups, &ppg, &ppw);
     if (ngroups > r->max_groups) return fail(CUBO_HIP_ERR_HIP, "internal: group plan exceeds scratch");
@@ -328,7 +330,7 @@ int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows) {
     hipError_t e;
     if (r->path == 1) {
         cubo::IntLaunch a{};
-        a.cube = (const uint16_t *)c->data + pos0 * c->ndet;
+        a.cube = (const uint16_t *)tile;
         a.npos = npos;
         a.ndet = ndet;
         a.qx = (int)c->shape[3];
@@ -344,7 +346,7 @@ int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows) {
         e = cubo::launch_int(a, r->nw, r->chunks, r->com_ones, r->full, c->stream);
     } else {
         cubo::F64Launch a{};
-        a.cube = (const char *)c->data + (size_t)pos0 * c->ndet * cubo::element_size(c->dtype);
+        a.cube = tile;
         a.npos = npos;
         a.ndet = ndet;
         a.nslices = r->nslices;

@@ -222,6 +222,7 @@ int cubo_hip_shift_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
     if (!c || !masks || !out) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     if (k < 1 || k > 32) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "%d masks (1..32)", k);
+    if (int st = cubo::require_resident(c, "the shifted stack")) return st;
     if (c->dtype != CUBO_HIP_U16 || c->ndet % 8 != 0)
         return fail(CUBO_HIP_ERR_UNSUPPORTED, "the shifted stack needs a uint16 cube whose pattern length is a multiple of 8");
     for (int64_t i = 0; i < (int64_t)k * c->ndet; ++i)

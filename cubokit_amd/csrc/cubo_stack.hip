@@ -64,6 +64,7 @@ int cubo_hip_stack_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
                                cubo_hip_stack_plan **out) {
     if (!c || !masks || !out) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
+    if (int st = cubo::require_resident(c, "the mask stack")) return st;
     if (k <= 0) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the stack needs at least one mask");
     const int64_t ndet = c->ndet, npos = c->npos;
     for (int64_t i = 0; i < (int64_t)k * ndet; ++i)

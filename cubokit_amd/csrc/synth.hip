@@ -102,6 +102,7 @@ __global__ void synth_kernel(T *out, int64_t first, int64_t count, SynthDev s) {
 
 extern "C" int cubo_hip_cube_fill_synthetic(cubo_hip_cube *c, const cubo_hip_synth_params *p) {
     if (!c || !p) return CUBO_HIP_ERR_INVALID_ARGUMENT;
+    if (int st = cubo::require_resident(c, "a synthetic fill")) return st;
     if (p->n_rings < 0 || p->n_rings > 4 || p->n_waves < 0 || p->n_waves > 8) return CUBO_HIP_ERR_INVALID_ARGUMENT;
     for (int k = 0; k < p->n_rings; ++k)
         if (!(p->ring_sigma[k] > 0)) return CUBO_HIP_ERR_INVALID_ARGUMENT;

@@ -18,6 +18,7 @@ from __future__ import annotations
 import ctypes as ct
 import enum
 import math
+import os
 from dataclasses import dataclass
 from typing import Callable, Optional, Sequence, Union
 
@@ -31,7 +32,13 @@ from . import _lib as L
 
 
 class CuboError(Exception):
-    """CuboCore errors (Sources/CuboCore/CuboError.swift)."""
+    """CuboCore errors (Sources/CuboCore/CuboError.swift); `case` names the reference's case (cannotOpen, notNPY,
+    fortranOrderNotSupported, bigEndianNotSupported, unsupportedElementType, wrongRank, fileTooShort,
+    misalignedOffset, invalidShape) where the library reports one."""
+
+    def __init__(self, message: str = "", case: str = ""):
+        super().__init__(message)
+        self.case = case
 
 
 class CuboVirtualError(Exception):
@@ -63,6 +70,8 @@ def _check(status: int, what: str) -> None:
         raise CuboVirtualError("noHIPDevice", msg or what)
     if status == L.ERR_INVALID_ARGUMENT:
         raise CuboVirtualError("invalidInput", f"{what}: {msg}")
+    if status == L.ERR_FILE:
+        raise CuboError(f"{what}: {msg}", case=msg.split(":")[0].strip())
     raise CuboVirtualError("hip", f"{what}: {msg} (status {status})")
 
 
@@ -352,10 +361,36 @@ class HIPResidentCube:
         _check(L.lib().cubo_hip_cube_create(device, ext, _DTYPES[self.dtype], ct.byref(h)), "allocating the cube")
         self._h = h
         if loading is not None:
+            # through the pinned double-buffered feed; other element types are converted on the device
             for rg in (tiling or ScanTiling.automatic).ranges(shape):
-                tile = np.ascontiguousarray(arr[rg.start:rg.stop].astype(self.dtype, copy=False))
-                _check(L.lib().cubo_hip_cube_upload_rows(self._h, rg.start, len(rg),
-                                                         tile.ctypes.data_as(ct.c_void_p)), "uploading")
+                _feed(self._h, arr, rg)
+
+    @staticmethod
+    def _adopt(h, shape: "CubeShape", dtype, device: int) -> "HIPResidentCube":
+        c = HIPResidentCube.__new__(HIPResidentCube)
+        c.shape, c.dtype, c.device, c._h = shape, np.dtype(dtype), device, h
+        return c
+
+    @staticmethod
+    def load_npy(path: str, device: int = 0) -> "HIPResidentCube":
+        """MappedCube(npy:) loaded into HBM (NPYFile.swift:28-71, MappedCube.swift:130-136): uint16 stays uint16,
+        every other element type becomes Float32 on the device."""
+        hd = NPYFile.header(path)
+        if len(hd.shape) != 4:
+            raise CuboError(f"wrongRank: expected 4, received {len(hd.shape)}", case="wrongRank")
+        h = ct.c_void_p()
+        _check(L.lib().cubo_hip_cube_load_npy(device, os.fsencode(path), ct.byref(h)), f"loading {path}")
+        return HIPResidentCube._adopt(h, CubeShape.of(hd.shape), native_dtype(hd.element_type), device)
+
+    @staticmethod
+    def load_file(path: str, data_offset: int, element_type: str, shape: "CubeShape", device: int = 0):
+        """MappedCube(url:dataOffset:elementType:shape:) loaded into HBM (MappedCube.swift:98-127)."""
+        ext = (ct.c_int64 * 4)(*shape.extents)
+        h = ct.c_void_p()
+        _check(L.lib().cubo_hip_cube_load_file(device, os.fsencode(path), int(data_offset),
+                                               L.ELEMENT_TYPES.index(element_type), ext, ct.byref(h)),
+               f"loading {path}")
+        return HIPResidentCube._adopt(h, shape, native_dtype(element_type), device)
 
     @staticmethod
     def availability(shape: CubeShape, dtype=np.uint16, hbm_bytes: int = 0, device: int = 0):
@@ -435,12 +470,90 @@ class HIPResidentCube:
 Source = Union[HIPResidentCube, np.ndarray]
 
 
+@dataclass(frozen=True)
+class NPYHeader:
+    """What an .npy header says (NPYFile.swift:11-22)."""
+    major_version: int
+    minor_version: int
+    element_type: str
+    shape: list
+    data_offset: int
+
+
+class NPYFile:
+    @staticmethod
+    def header(path: str) -> NPYHeader:
+        """NPYFile.header(of:) (NPYFile.swift:28-71), parsed by the library."""
+        h = L.NpyHeader()
+        _check(L.lib().cubo_hip_npy_read_header(os.fsencode(path), ct.byref(h)), f"reading the header of {path}")
+        return NPYHeader(h.major_version, h.minor_version, L.ELEMENT_TYPES[h.element_type],
+                         [int(h.shape[k]) for k in range(h.rank)], int(h.data_offset))
+
+
+def _element_type(dtype: np.dtype) -> Optional[int]:
+    """The stored element type of a native little-endian NumPy dtype, or None."""
+    dt = np.dtype(dtype)
+    if dt.byteorder == ">" or dt.name not in L.ELEMENT_TYPES:
+        return None
+    return L.ELEMENT_TYPES.index(dt.name)
+
+
+def _feed(h, arr: np.ndarray, rg: range) -> None:
+    """Scan rows `rg` of `arr` to the cube `h` through the library's pinned feed."""
+    t = _element_type(arr.dtype)
+    tile = arr[rg.start:rg.stop]
+    if t is None:                           # float16, bool, big-endian: Float32 on the host first
+        tile, t = np.asarray(tile, np.float32), L.ELEMENT_TYPES.index("float32")
+    tile = np.ascontiguousarray(tile)
+    _check(L.lib().cubo_hip_cube_feed_rows(h, rg.start, len(rg), tile.ctypes.data_as(ct.c_void_p), t), "feeding rows")
+
+
+class HIPStreamedCube:
+    """A cube fed tile by tile from host memory (a NumPy array or np.memmap of a file) into a two-slot device ring:
+    the staging case of the reference's TileFeed (MetalReducers.swift:494-573), with the copies double-buffered
+    through pinned memory and overlapped with the reductions."""
+
+    def __init__(self, source: np.ndarray, tiling: ScanTiling, device: int = 0):
+        self.source = source
+        self.shape = CubeShape.of(source.shape)
+        self.dtype = native_dtype(source.dtype)
+        self.device = device
+        self.tile_rows = tiling.rows_per_tile(self.shape)
+        ext = (ct.c_int64 * 4)(*self.shape.extents)
+        h = ct.c_void_p()
+        _check(L.lib().cubo_hip_cube_create_streamed(device, ext, _DTYPES[self.dtype], self.tile_rows, ct.byref(h)),
+               "creating the streamed cube")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def feed(self, rg: range) -> None:
+        _feed(self._h, self.source, rg)
+
+    def release(self):
+        if getattr(self, "_h", None) is not None:
+            L.lib().cubo_hip_cube_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
 def _resident(source: Source, tiling: ScanTiling):
-    """A resident cube for `source` and whether it is a temporary staging copy (the TileFeed staging case,
-    MetalReducers.swift:504-573)."""
+    """(cube, temporary, feed): a resident cube as it is, or a host array as a streamed cube whose tiles `feed`
+    copies to the device before each tile's reduction (the TileFeed staging case, MetalReducers.swift:504-573)."""
     if isinstance(source, HIPResidentCube):
-        return source, False
-    return HIPResidentCube(np.asarray(source), tiling=tiling), True
+        return source, False, None
+    arr = np.asarray(source)
+    if arr.ndim != 4:
+        raise CuboError(f"invalid shape {list(arr.shape)}")
+    cube = HIPStreamedCube(arr, tiling)
+    return cube, True, cube.feed
 
 
 # ------------------------------------------------------------------------------------------------
@@ -461,7 +574,7 @@ def _reduce(source: Source, *, masks: Optional[np.ndarray], want_cols: bool, wan
     tiling = tiling or ScanTiling.automatic
     if is_cancelled is not None and is_cancelled():
         raise CancellationError()
-    cube, temporary = _resident(source, tiling)
+    cube, temporary, feed = _resident(source, tiling)
     try:
         shape = cube.shape
         n, npos = shape.detector_count, shape.scan_count
@@ -488,6 +601,8 @@ def _reduce(source: Source, *, masks: Optional[np.ndarray], want_cols: bool, wan
             for rg in tiling.ranges(shape):
                 if is_cancelled is not None and is_cancelled():
                     raise CancellationError()
+                if feed is not None:
+                    feed(rg)
                 _check(L.lib().cubo_hip_reduce_rows(h, rg.start, len(rg)), "reducing rows")
                 if progress is not None:
                     progress(rg.stop / shape.scan_rows)
@@ -673,7 +788,7 @@ def _pattern(source: Source, statistic: DiffractionStatistic, take, weights, shi
     tiling = tiling or ScanTiling.automatic
     if is_cancelled is not None and is_cancelled():
         raise CancellationError()
-    cube, temporary = _resident(source, tiling)
+    cube, temporary, feed = _resident(source, tiling)
     try:
         shape = cube.shape
         if shift is not None and shift.origin.row.size != shape.scan_count:
@@ -692,6 +807,8 @@ def _pattern(source: Source, statistic: DiffractionStatistic, take, weights, shi
             for rg in tiling.ranges(shape):
                 if is_cancelled is not None and is_cancelled():
                     raise CancellationError()
+                if feed is not None:
+                    feed(rg)
                 _check(L.lib().cubo_hip_pattern_rows(h, rg.start, len(rg)), "pattern rows")
                 if progress is not None:
                     progress(rg.stop / shape.scan_rows)

@@ -33,7 +33,11 @@ enum {
     CUBO_HIP_ERR_HIP = -5,
     CUBO_HIP_ERR_UNSUPPORTED = -6,
     CUBO_HIP_ERR_INCOMPLETE = -7,
-    CUBO_HIP_ERR_EXCEEDS_BUFFER_LIMIT = -8
+    CUBO_HIP_ERR_EXCEEDS_BUFFER_LIMIT = -8,
+    CUBO_HIP_ERR_FILE = -9     /* a CuboError of the file readers (Sources/CuboCore/CuboError.swift); the message
+                                  starts with the case: cannotOpen, notNPY, fortranOrderNotSupported,
+                                  bigEndianNotSupported, unsupportedElementType, wrongRank, fileTooShort,
+                                  misalignedOffset, invalidShape */
 };
 
 /* Element types held on the device: the native detector counts (uint16) or Float32, the type every
@@ -95,6 +99,49 @@ int cubo_hip_cube_destroy(cubo_hip_cube *cube);
 int cubo_hip_cube_clone(const cubo_hip_cube *source, cubo_hip_cube **out);
 /* byteCount (ResidentCube.swift:109). */
 int cubo_hip_cube_byte_count(const cubo_hip_cube *cube, uint64_t *bytes);
+
+/* ---- ingestion: .npy and raw files, and host tiles, into HBM (NPYFile.header,
+ * Sources/CuboCore/NPYFile.swift:28-130; MappedCube, Sources/CuboCore/MappedCube.swift:98-164; the staging case of
+ * TileFeed, Sources/CuboVirtual/MetalReducers.swift:494-573) ---- */
+
+/* Stored element types, in ElementType's order (MappedCube.swift:15-17). uint16 is kept as uint16 on the device;
+ * every other type becomes Float32 there, converted on the device as ElementType.convert does (float64 and the
+ * integers round to nearest even, MappedCube.swift:56-83). */
+enum {
+    CUBO_HIP_ELEMENT_F32 = 0, CUBO_HIP_ELEMENT_F64 = 1, CUBO_HIP_ELEMENT_U8 = 2, CUBO_HIP_ELEMENT_I8 = 3,
+    CUBO_HIP_ELEMENT_U16 = 4, CUBO_HIP_ELEMENT_I16 = 5, CUBO_HIP_ELEMENT_U32 = 6, CUBO_HIP_ELEMENT_I32 = 7,
+    CUBO_HIP_ELEMENT_U64 = 8, CUBO_HIP_ELEMENT_I64 = 9
+};
+typedef struct {
+    int32_t major_version, minor_version;
+    int32_t element_type;      /* CUBO_HIP_ELEMENT_* */
+    int32_t rank;              /* at most 32 */
+    int64_t shape[32];
+    int64_t data_offset;       /* byte offset of the first element */
+} cubo_hip_npy_header;
+/* NPYFile.header(of:) (NPYFile.swift:28-71): versions 1.0, 2.0 and 3.0, the reference's dict grammar and refusals.
+ * Host only: no device is needed. */
+int cubo_hip_npy_read_header(const char *path, cubo_hip_npy_header *header);
+/* MappedCube(url:dataOffset:elementType:shape:) + ResidentCube(loading:): the file region
+ * [data_offset, data_offset + cube bytes) into a new resident cube. Worker threads read the next chunk into one
+ * pinned buffer while the previous chunk's copy to HBM (and conversion) runs from the other. The file checks
+ * (misalignedOffset, cannotOpen, invalidShape, fileTooShort) come before any device call. */
+int cubo_hip_cube_load_file(int device, const char *path, int64_t data_offset, int32_t element_type,
+                            const int64_t shape[4], cubo_hip_cube **out);
+/* MappedCube(npy:) + ResidentCube(loading:): the header's rank must be 4 (wrongRank otherwise). */
+int cubo_hip_cube_load_npy(int device, const char *path, cubo_hip_cube **out);
+/* A cube whose scan rows are fed tile by tile from host memory instead of being held whole (the reference's
+ * staging buffer for sources that are not resident): a two-slot device ring of tile_rows scan rows each. Its
+ * tiles are reduced by cubo_hip_reduce_rows and cubo_hip_pattern_rows like a resident cube's; calls that read the
+ * whole cube refuse it. */
+int cubo_hip_cube_create_streamed(int device, const int64_t shape[4], int dtype, int64_t tile_rows,
+                                  cubo_hip_cube **out);
+/* Copies scan rows [row0, row0 + nrows) of `host` (stored as element_type, e.g. a mapped file) to the device,
+ * asynchronously: through two pinned buffers and a copy stream, converted on the device where the type is not the
+ * cube's. On a resident cube they land in place; on a streamed cube in the next ring slot (nrows <= tile_rows).
+ * Later launches on the cube wait for the copy; the call returns once `host` has been read, so the next tile's
+ * host read overlaps this tile's copy and reduction. */
+int cubo_hip_cube_feed_rows(cubo_hip_cube *cube, int64_t row0, int64_t nrows, const void *host, int32_t element_type);
 
 /* ---- the fused reduction: VirtualImaging.image, VirtualDiffraction.mean/max/meanAndMax and
  * CenterOfMass.measure in one pass over the cube (VirtualImaging.swift:20-66,
