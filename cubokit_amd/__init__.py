@@ -9,7 +9,7 @@ from .virtual import (CancellationError, CenterOfMass, CenterOfMassMaps, Compute
                       ResidentCubeError, ScanImage, ScanTiling, UnavailableReason, VirtualDiffraction, VirtualImaging,
                       device_count, device_info, first_look, hip_is_available, reduce_path,
                       DiffractionStatistic, OriginMap, PatternShift, ScanSelection, pattern_path,
-                      HIPStreamedCube, NPYFile, NPYHeader)
+                      HIPStreamedCube, NPYFile, NPYHeader, MaskShift, image_centered)
 
 __all__ = ["CancellationError", "CenterOfMass", "CenterOfMassMaps", "ComputePath", "CubeShape", "CuboError",
            "CuboVirtualError", "DetectorGeometry", "DetectorImage", "DetectorMask", "FirstLook", "HIPResidentCube",
@@ -17,4 +17,4 @@ __all__ = ["CancellationError", "CenterOfMass", "CenterOfMassMaps", "ComputePath
            "ResidentCubeError", "ScanImage", "ScanTiling", "UnavailableReason", "VirtualDiffraction",
            "VirtualImaging", "device_count", "device_info", "first_look", "hip_is_available", "reduce_path",
            "DiffractionStatistic", "OriginMap", "PatternShift", "ScanSelection", "pattern_path",
-           "HIPStreamedCube", "NPYFile", "NPYHeader"]
+           "HIPStreamedCube", "NPYFile", "NPYHeader", "MaskShift", "image_centered"]

@@ -139,6 +139,7 @@ SIGNATURES = {
     "cubo_hip_batch_destroy": (ct.c_int, [ct.c_void_p]),
     "cubo_hip_shift_plan_create": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_void_p)]),
     "cubo_hip_shift_forward": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "cubo_hip_shift_forward_rounded": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
     "cubo_hip_shift_backward": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                            ct.c_void_p, ct.c_void_p]),
     "cubo_hip_shift_kernel_time": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_int64)]),

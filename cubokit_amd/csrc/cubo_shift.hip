@@ -38,6 +38,7 @@ struct cubo_hip_shift_plan {
     // shift-class preparation of the last origin map
     std::vector<double> o_row, o_col;
     bool prepared = false;
+    bool prepared_rounded = false;   // the classes are of MaskShift.rounded (one roll, no blend)
     int ntiles = 0, nslots = 0;
     std::vector<int32_t> h_shift;    // [npos][2]
     std::vector<float> h_wts;        // [npos][4]
@@ -84,11 +85,14 @@ hipError_t grow(T **p, size_t *cap, size_t n) {
 }
 
 // Shift classes, tiles and rolled weight tables for an origin map (skipped when it is the previous one).
-int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol) {
+// rounded: MaskShift.rounded (ShiftedImaging.swift:66): the roll by wrap(round_half_even(d)), no blend; else
+// MaskShift.bilinear (:58-64): floor and fractions.
+int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol, bool rounded = false) {
     cubo_hip_cube *c = p->cube;
     const int64_t npos = c->npos;
     const int qy = (int)c->shape[2], qx = (int)c->shape[3];
-    if (p->prepared && std::equal(orow, orow + npos, p->o_row.begin()) && std::equal(ocol, ocol + npos, p->o_col.begin()))
+    if (p->prepared && p->prepared_rounded == rounded && std::equal(orow, orow + npos, p->o_row.begin()) &&
+        std::equal(ocol, ocol + npos, p->o_col.begin()))
         return CUBO_HIP_OK;
     for (int64_t i = 0; i < npos; ++i)
         if (!std::isfinite(orow[i]) || !std::isfinite(ocol[i]))
@@ -101,8 +105,8 @@ int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol) {
     std::vector<int32_t> key(npos);
     for (int64_t i = 0; i < npos; ++i) {
         const double dr = orow[i] - mr, dc = ocol[i] - mc;
-        const double fr = std::floor(dr), fc = std::floor(dc);
-        const double wr = dr - fr, wc = dc - fc;
+        const double fr = rounded ? std::nearbyint(dr) : std::floor(dr), fc = rounded ? std::nearbyint(dc) : std::floor(dc);
+        const double wr = rounded ? 0.0 : dr - fr, wc = rounded ? 0.0 : dc - fc;
         const int ir = wrap_shift(fr, qy), ic = wrap_shift(fc, qx);
         p->h_shift[2 * i] = ir;
         p->h_shift[2 * i + 1] = ic;
@@ -141,8 +145,9 @@ int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol) {
         const int64_t b = count[kk], e = count[kk + 1];
         if (b == e) continue;
         const int ir = kk / qx, ic = kk % qx;
-        const int s0 = slot(ir, ic), s1 = slot((ir + 1) % qy, ic), s2 = slot(ir, (ic + 1) % qx),
-                  s3 = slot((ir + 1) % qy, (ic + 1) % qx);
+        const int s0 = slot(ir, ic);
+        const int s1 = rounded ? s0 : slot((ir + 1) % qy, ic), s2 = rounded ? s0 : slot(ir, (ic + 1) % qx),
+                  s3 = rounded ? s0 : slot((ir + 1) % qy, (ic + 1) % qx);
         for (int64_t t = b; t < e; t += TP) {
             cubo::ShiftTile tl{};
             tl.start = (int32_t)t;
@@ -187,6 +192,7 @@ int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol) {
     p->o_row.assign(orow, orow + npos);
     p->o_col.assign(ocol, ocol + npos);
     p->prepared = true;
+    p->prepared_rounded = rounded;
     return CUBO_HIP_OK;
 }
 
@@ -248,9 +254,11 @@ int cubo_hip_shift_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
     return CUBO_HIP_OK;
 }
 
-int cubo_hip_shift_forward(cubo_hip_shift_plan *p, const double *origin_row, const double *origin_col, double *y) {
+namespace {
+
+int forward(cubo_hip_shift_plan *p, const double *origin_row, const double *origin_col, double *y, bool rounded) {
     if (!p || !origin_row || !origin_col || !y) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (int st = prepare(p, origin_row, origin_col)) return st;
+    if (int st = prepare(p, origin_row, origin_col, rounded)) return st;
     cubo_hip_cube *c = p->cube;
     DeviceGuard g(c->device);
     const size_t ny = (size_t)c->npos * p->k;
@@ -258,12 +266,23 @@ int cubo_hip_shift_forward(cubo_hip_shift_plan *p, const double *origin_row, con
     cubo::ShiftLaunch a = launch_args(p);
     a.y = p->y;
     CUBO_HIP_TRY(hipEventRecord(p->ev[0], c->stream), "hipEventRecord");
-    CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, false, c->stream), "launching the shifted stack");
+    CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, false, rounded, c->stream), "launching the shifted stack");
     CUBO_HIP_TRY(hipEventRecord(p->ev[1], c->stream), "hipEventRecord");
     CUBO_HIP_TRY(hipMemcpyAsync(y, p->y, ny * 8, hipMemcpyDeviceToHost, c->stream), "reading the stack");
     CUBO_HIP_TRY(hipStreamSynchronize(c->stream), "waiting for the shifted stack");
     p->ms[0] += elapsed(p->ev[0], p->ev[1]);
     return CUBO_HIP_OK;
+}
+
+}  // namespace
+
+int cubo_hip_shift_forward(cubo_hip_shift_plan *p, const double *origin_row, const double *origin_col, double *y) {
+    return forward(p, origin_row, origin_col, y, false);
+}
+
+int cubo_hip_shift_forward_rounded(cubo_hip_shift_plan *p, const double *origin_row, const double *origin_col,
+                                   double *y) {
+    return forward(p, origin_row, origin_col, y, true);
 }
 
 int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, const double *origin_col, const double *dy,
@@ -288,7 +307,7 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
         a.gr = p->g;
         a.gc = p->g + npos;
         CUBO_HIP_TRY(hipEventRecord(p->ev[2], s), "hipEventRecord");
-        CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, true, s), "launching the origin adjoint");
+        CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, true, false, s), "launching the origin adjoint");
         CUBO_HIP_TRY(hipEventRecord(p->ev[3], s), "hipEventRecord");
     }
     if (dmasks) {

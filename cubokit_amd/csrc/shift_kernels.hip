@@ -25,7 +25,6 @@ typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
 constexpr int kBK = 64;            // pixels per chunk
 constexpr int kFlush = 4;          // chunks per FP64 flush (256 pixels)
-constexpr int kCorners = 4;
 constexpr int kTileEntries = 4 * 2 * 64;   // 16-byte weight entries of one chunk of one 32-mask tile
 
 __device__ __forceinline__ uint32_t cvt_pair(uint32_t w) {
@@ -34,9 +33,12 @@ __device__ __forceinline__ uint32_t cvt_pair(uint32_t w) {
     return __builtin_bit_cast(uint32_t, h);
 }
 
-template <int WAVES, bool BWD>
+// NC corners: 4 for the bilinear shift, 1 for the rounded one (MaskShift.rounded: one roll, weight 1).
+template <int WAVES, bool BWD, int NC>
 __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
-    constexpr int WCH = kCorners * kTileEntries;             // entries of one chunk, four corners
+    static_assert(NC == 4 || (NC == 1 && !BWD), "the adjoint needs the four corners");
+    constexpr int kCorners = NC;
+    constexpr int WCH = kCorners * kTileEntries;             // entries of one chunk, NC corners
     constexpr int TP = 32 * WAVES;
     constexpr int WPT = WCH / (64 * WAVES);
     static_assert(WCH % (64 * WAVES) == 0, "every thread carries the same number of weight entries");
@@ -100,12 +102,17 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
         for (int r = 0; r < 16; ++r) {
             const int row = wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             const double wr = wts[0][row], wc = wts[1][row];
-            const double c0 = acc[0][r], c1 = acc[1][r], c2 = acc[2][r], c3 = acc[3][r];
-            if constexpr (BWD) {
-                accd[0][r] += (1.0 - wc) * (c1 - c0) + wc * (c3 - c2);
-                accd[1][r] += (1.0 - wr) * (c2 - c0) + wr * (c3 - c1);
+            if constexpr (NC == 1) {
+                accd[0][r] += (double)acc[0][r];
             } else {
-                accd[0][r] += (1.0 - wr) * (1.0 - wc) * c0 + wr * (1.0 - wc) * c1 + (1.0 - wr) * wc * c2 + wr * wc * c3;
+                const double c0 = acc[0][r], c1 = acc[1][r], c2 = acc[2][r], c3 = acc[3][r];
+                if constexpr (BWD) {
+                    accd[0][r] += (1.0 - wc) * (c1 - c0) + wc * (c3 - c2);
+                    accd[1][r] += (1.0 - wr) * (c2 - c0) + wr * (c3 - c1);
+                } else {
+                    accd[0][r] += (1.0 - wr) * (1.0 - wc) * c0 + wr * (1.0 - wc) * c1 + (1.0 - wr) * wc * c2 +
+                                  wr * wc * c3;
+                }
             }
         }
 #pragma unroll
@@ -317,12 +324,15 @@ __global__ void sum_groups(const double *partial, int ngroups, int64_t n, double
 
 }  // namespace
 
-hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, hipStream_t s) {
+hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, bool rounded, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
+    if (backward && rounded) return hipErrorInvalidValue;
     if (backward)
-        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, true>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
+        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, true, 4>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
+    else if (rounded)
+        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false, 1>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
     else
-        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
+        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false, 4>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
     return hipGetLastError();
 }
 

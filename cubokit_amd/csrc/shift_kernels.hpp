@@ -41,7 +41,8 @@ struct ShiftLaunch {
 
 // Tiles of 32 * kShiftWaves positions.
 constexpr int kShiftWaves = 8;
-hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, hipStream_t s);
+// rounded: MaskShift.rounded, one roll per class (tiles use slot[0] only); the forward pass only.
+hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, bool rounded, hipStream_t s);
 
 // out[slot] = base rolled by shift[slot] = (tr, tc): the entry of pixel u holds the base weight of pixel
 // ((ui - tr) mod qy, (uj - tc) mod qx). base is split_weights' one-tile layout.

@@ -652,6 +652,78 @@ class VirtualImaging:
         return [ScanImage(shape.scan_rows, shape.scan_columns, v, ComputePath.hip) for v in out["images"]]
 
 
+class MaskShift:
+    """How the detector mask follows the beam from position to position (ShiftedImaging.swift:14-21)."""
+
+    def __init__(self, kind: str, origin: Optional["OriginMap"] = None):
+        self.kind, self.origin = kind, origin
+
+    @staticmethod
+    def none() -> "MaskShift":
+        return MaskShift("none")
+
+    @staticmethod
+    def rounded(origin: "OriginMap") -> "MaskShift":
+        return MaskShift("rounded", origin)
+
+    @staticmethod
+    def bilinear(origin: "OriginMap") -> "MaskShift":
+        return MaskShift("bilinear", origin)
+
+
+def _transform(g: DetectorGeometry, origin, pixel_size) -> DetectorGeometry:
+    """get_calibrated_detector_geometry (ShiftedImaging.swift:77-97): the origin added, then every coordinate divided
+    by the pixel size; a point truncates toward zero."""
+    if origin is not None and pixel_size is not None:
+        raise CuboVirtualError("centeredAndCalibrated")
+    if pixel_size is not None and not (pixel_size > 0 and math.isfinite(pixel_size)):
+        raise CuboVirtualError("invalidPixelSize", str(pixel_size))
+    r0, c0 = (float(origin[0]), float(origin[1])) if origin is not None else (0.0, 0.0)
+    sc = float(pixel_size) if pixel_size is not None else 1.0
+    p = g.params
+    if g.kind == "mask":
+        return g
+    if g.kind == "point":
+        fr, fc = (p[0] + r0) / sc, (p[1] + c0) / sc
+        if not (math.isfinite(fr) and math.isfinite(fc)) or abs(math.trunc(fr)) >= 1e15 or abs(math.trunc(fc)) >= 1e15:
+            raise CuboVirtualError("nonFiniteGeometry")
+        return DetectorGeometry.point(math.trunc(fr), math.trunc(fc))
+    if g.kind == "circle":
+        return DetectorGeometry.circle((p[0] + r0) / sc, (p[1] + c0) / sc, p[2] / sc)
+    if g.kind == "annulus":
+        return DetectorGeometry.annulus((p[0] + r0) / sc, (p[1] + c0) / sc, p[2] / sc, p[3] / sc)
+    return DetectorGeometry.rectangle((p[0] + r0) / sc, (p[1] + r0) / sc, (p[2] + c0) / sc, (p[3] + c0) / sc)
+
+
+def image_centered(source: Source, geometry: DetectorGeometry, centered_on=None, pixel_size: Optional[float] = None,
+                   shift: Optional[MaskShift] = None, tiling: ScanTiling = None) -> ScanImage:
+    """VirtualImaging.image(of:geometry:centeredOn:pixelSize:shift:) (ShiftedImaging.swift:34-75): the geometry
+    centred on `centered_on` (normally the origin map's mean) or calibrated by `pixel_size`, then the mask fixed
+    (.none) or moved per position by the rounded or bilinear shift of the origin map, on the matrix cores."""
+    shape = _shape_of(source)
+    g = _transform(geometry, centered_on, pixel_size)
+    if shift is None or shift.kind == "none":
+        return VirtualImaging.image(source, g, tiling=tiling)
+    origin = shift.origin
+    if origin.row.size != shape.scan_count:
+        raise CuboVirtualError("originMapMismatch", f"the origin map has {origin.row.size} positions, the scan "
+                               f"{shape.scan_count}")
+    if not (np.all(np.isfinite(origin.row)) and np.all(np.isfinite(origin.column))):
+        raise CuboVirtualError("originMapMismatch", "the origin map has non-finite values")
+    mask = DetectorMask.make(g, shape)
+    cube, temporary = (source, False) if isinstance(source, HIPResidentCube) else (HIPResidentCube(source), True)
+    try:
+        st = ShiftedStack(cube, mask[None])
+        try:
+            y = st.forward(origin.row, origin.column, rounded=shift.kind == "rounded")
+        finally:
+            st.release()
+    finally:
+        if temporary:
+            cube.release()
+    return ScanImage(shape.scan_rows, shape.scan_columns, y[:, 0].copy(), ComputePath.hip)
+
+
 class VirtualDiffraction:
     @staticmethod
     def mean(source: Source, path: ComputePath = ComputePath.hip, tiling: ScanTiling = None,
@@ -994,10 +1066,12 @@ class ShiftedStack:
         c = np.ascontiguousarray(origin_col, np.float64).reshape(n)
         return r, c
 
-    def forward(self, origin_row, origin_col) -> np.ndarray:
+    def forward(self, origin_row, origin_col, rounded: bool = False) -> np.ndarray:
+        """MaskShift.bilinear, or with rounded=True MaskShift.rounded (one roll, no blend)."""
         r, c = self._origins(origin_row, origin_col)
         y = np.empty((self.cube.shape.scan_count, self.k))
-        _check(L.lib().cubo_hip_shift_forward(self._h, r.ctypes.data, c.ctypes.data, y.ctypes.data), "shifted stack")
+        f = L.lib().cubo_hip_shift_forward_rounded if rounded else L.lib().cubo_hip_shift_forward
+        _check(f(self._h, r.ctypes.data, c.ctypes.data, y.ctypes.data), "shifted stack")
         return y
 
     def backward(self, origin_row, origin_col, dy, masks: bool = True, origin: bool = True):

@@ -291,6 +291,10 @@ typedef struct cubo_hip_shift_plan cubo_hip_shift_plan;
 int cubo_hip_shift_plan_create(cubo_hip_cube *cube, const float *masks, int32_t k, cubo_hip_shift_plan **out);
 /* y: scanCount x k Double, row-major. */
 int cubo_hip_shift_forward(cubo_hip_shift_plan *plan, const double *origin_row, const double *origin_col, double *y);
+/* MaskShift.rounded (ShiftedImaging.swift:66): each position's masks rolled by wrap(round_half_even(origin - mean)),
+ * no blend; one contraction per shift class on the matrix cores. Exact for uint16 cubes and integral masks. */
+int cubo_hip_shift_forward_rounded(cubo_hip_shift_plan *plan, const double *origin_row, const double *origin_col,
+                                   double *y);
 int cubo_hip_shift_backward(cubo_hip_shift_plan *plan, const double *origin_row, const double *origin_col,
                             const double *dy, double *dmasks, double *d_origin_row, double *d_origin_col);
 /* Device milliseconds since the previous query: [forward kernel, origin-adjoint kernel, mask-adjoint kernels],

@@ -358,6 +358,27 @@ int oc_vi_bilinear(const float *cube, int64_t npos, int qy, int qx, const double
     return OC_OK;
 }
 
+/* MaskShift.rounded (ShiftedImaging.swift:65-71): the mask rolled by wrap(round_half_even(origin - mean)) per
+ * position, Neumaier over Double(x) * m in pixel order. */
+int oc_vi_rounded(const float *cube, int64_t npos, int qy, int qx, const double *mask, const double *origin_r,
+                  const double *origin_c, double *out) {
+    for (int64_t p = 0; p < npos; ++p)
+        if (!isfinite(origin_r[p]) || !isfinite(origin_c[p])) return OC_NON_FINITE;
+    int64_t ndet = (int64_t)qy * qx;
+    double mr = mean_plain(origin_r, npos), mc = mean_plain(origin_c, npos);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < npos; ++p) {
+        const int sr = wrap_shift(nearbyint(origin_r[p] - mr), qy), sc = wrap_shift(nearbyint(origin_c[p] - mc), qx);
+        neumaier a = {0, 0};
+        const float *x = cube + p * ndet;
+        for (int i = 0; i < qy; ++i)
+            for (int j = 0; j < qx; ++j)
+                nm_add(&a, (double)x[(int64_t)i * qx + j] * rolled(mask, qy, qx, sr, sc, i, j));
+        out[p] = nm_value(&a);
+    }
+    return OC_OK;
+}
+
 /* dW[u] = sum_p dY[p] * sum_corners w_corner * x[p, u + shift_corner]: the adjoint with respect to the
  * unshifted mask (x at pixel (i, j) meets mask[(i - sr) mod qy, (j - sc) mod qx]). */
 void oc_vi_bilinear_grad_w(const float *cube, int64_t npos, int qy, int qx, const double *origin_r,

@@ -55,6 +55,10 @@ void oc_mask_stack(const float *cube, int64_t npos, int64_t ndet, const double *
 int oc_vi_bilinear(const float *cube, int64_t npos, int qy, int qx, const double *mask,
                    const double *origin_r, const double *origin_c, double *out);
 
+/* Rounded centre-shifted virtual image (MaskShift.rounded). */
+int oc_vi_rounded(const float *cube, int64_t npos, int qy, int qx, const double *mask,
+                  const double *origin_r, const double *origin_c, double *out);
+
 /* Adjoints of the reductions above, in Double. */
 void oc_mask_stack_grad_w(const float *cube, int64_t npos, int64_t ndet, const double *dy, int k, double *dw);
 void oc_vi_bilinear_grad_w(const float *cube, int64_t npos, int qy, int qx, const double *origin_r,

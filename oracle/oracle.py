@@ -44,6 +44,7 @@ def _load():
         "oc_mask_stack": (None, [_fp, _i64, _i64, _dp, ct.c_int, _dp]),
         "oc_mask_stack_grad_w": (None, [_fp, _i64, _i64, _dp, ct.c_int, _dp]),
         "oc_vi_bilinear": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]),
+        "oc_vi_rounded": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]),
         "oc_vi_bilinear_grad_w": (None, [_fp, _i64, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]),
         "oc_vi_bilinear_grad_origin": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
         "oc_pattern": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
@@ -207,6 +208,15 @@ def vi_bilinear(cube, m, origin_r, origin_c) -> np.ndarray:
     _check(lib().oc_vi_bilinear(x, x.shape[0], qy, qx, np.ascontiguousarray(m, np.float64),
                                 np.ascontiguousarray(origin_r, np.float64),
                                 np.ascontiguousarray(origin_c, np.float64), out), "vi_bilinear")
+    return out
+
+
+def vi_rounded(cube, m, origin_r, origin_c) -> np.ndarray:
+    x, ry, rx, qy, qx = _flat(cube)
+    out = np.zeros(ry * rx)
+    _check(lib().oc_vi_rounded(x, x.shape[0], qy, qx, np.ascontiguousarray(m, np.float64),
+                               np.ascontiguousarray(origin_r, np.float64),
+                               np.ascontiguousarray(origin_c, np.float64), out), "vi_rounded")
     return out
 
 

@@ -35,6 +35,13 @@ def com_pins() -> dict:
     return _cache["c"]
 
 
+def origin_fit_pins() -> dict:
+    if "o" not in _cache:
+        with open(golden("origin-fit-pins.json")) as f:
+            _cache["o"] = json.load(f)
+    return _cache["o"]
+
+
 def doubles(v) -> np.ndarray:
     return np.array([float(s) for s in v], dtype=np.float64)
 
