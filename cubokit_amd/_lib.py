@@ -65,6 +65,21 @@ class NpyHeader(ct.Structure):
                 ("rank", ct.c_int32), ("shape", ct.c_int64 * 32), ("data_offset", ct.c_int64)]
 
 
+ORIGIN_PY4DSTEM = 0
+ORIGIN_ITERATED_WINDOW = 1
+
+
+class ProbeEstimate(ct.Structure):
+    _fields_ = [("radius", ct.c_double), ("center_row", ct.c_double), ("center_column", ct.c_double),
+                ("threshold", ct.c_double)]
+
+
+class OriginRequest(ct.Structure):
+    _fields_ = [("estimator", ct.c_int32), ("radius", ct.c_double), ("radius_scale", ct.c_double),
+                ("scan_mask", ct.c_void_p), ("fast_center", ct.c_int32), ("passes", ct.c_int32),
+                ("minimum_margin", ct.c_double), ("binned_coarse", ct.c_int32)]
+
+
 PATTERN_MEAN = 0
 PATTERN_MAX = 1
 TIES_FIRST = 0
@@ -125,6 +140,11 @@ SIGNATURES = {
     "cubo_hip_pattern_cancel": (ct.c_int, [ct.c_void_p]),
     "cubo_hip_pattern": (ct.c_int, [ct.c_void_p, ct.POINTER(PatternRequest), ct.c_void_p]),
     "cubo_hip_pattern_path": (ct.c_int, [ct.c_void_p, ct.POINTER(PatternRequest), ct.POINTER(ct.c_int)]),
+    "cubo_hip_probe_size": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_double, ct.c_double, ct.c_int32,
+                                       ct.POINTER(ProbeEstimate)]),
+    "cubo_hip_origins": (ct.c_int, [ct.c_void_p, ct.POINTER(OriginRequest), ct.c_int64, ct.c_int64, ct.c_void_p,
+                                    ct.c_void_p]),
+    "cubo_hip_gaussian_filter_pattern": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_double, ct.c_void_p]),
     "cubo_hip_stack_plan_create": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32,
                                               ct.POINTER(ct.c_void_p)]),
     "cubo_hip_stack_plan_method": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32)]),

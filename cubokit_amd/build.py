@@ -17,7 +17,7 @@ OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libcubo_hip.so")
 SOURCES = ["cubo_hip.hip", "cubo_reduce.hip", "reduce_kernels.hip", "synth.hip", "cubo_stack.hip", "maskstack_kernels.hip",
            "cubo_batch.hip", "shift_kernels.hip", "cubo_shift.hip",
-           "dpc_kernels.hip", "cubo_dpc.hip", "cubo_phase.hip", "pattern_kernels.hip", "cubo_pattern.hip", "cubo_io.hip"]
+           "dpc_kernels.hip", "cubo_dpc.hip", "cubo_phase.hip", "pattern_kernels.hip", "cubo_pattern.hip", "cubo_io.hip", "origin_kernels.hip", "cubo_origin.hip"]
 ARCH = os.environ.get("CUBO_HIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

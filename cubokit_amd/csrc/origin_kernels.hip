@@ -1,0 +1,295 @@
+// origin_kernels.hip — per-position origins for gfx950: py4DSTEM's get_origin in Double and the iterated-window
+// estimator in Float32 (the launch interface and the algorithms are summarised in origin_kernels.hpp).
+//
+// get_origin: one block per position at a time (blocks loop over the tile's positions). The axis-0 Gaussian pass
+// goes to the block's scratch pattern in HBM (L2-resident), the axis-1 pass is fused with the argmax, and the
+// centre of mass within r * rscale of it is a compensated (TwoSum) sum over the block. Every filtered value is the
+// reference's operation sequence (the centre weight times the pixel, then (p[i - t] + p[i + t]) * w for t = R down
+// to 1, no multiply-add contraction), so the argmax is the reference's.
+//
+// Iterated window: one wave per position. The coarse step gives each lane whole bin x bin blocks, summed in the
+// reference's order (so the block sums and the first-brightest block are its bits); each refinement pass gives each
+// lane whole window rows, and lane 0 adds the row sums in row order.
+#include "origin_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace cubo {
+
+namespace {
+
+constexpr int kPyBlock = 256;
+constexpr int kIWWaves = 4;
+
+template <bool U16>
+__device__ __forceinline__ double pix(const void *pat, int64_t i) {
+    if constexpr (U16) return (double)((const uint16_t *)pat)[i];
+    else return (double)((const float *)pat)[i];
+}
+
+// (s, c) += x, Knuth's TwoSum.
+__device__ __forceinline__ void two_sum(double &s, double &c, double x) {
+    const double t = s + x, bb = t - s;
+    c += (s - (t - bb)) + (x - bb);
+    s = t;
+}
+
+template <bool U16>
+__global__ __launch_bounds__(kPyBlock) void py4_origin(Py4Launch a) {
+    const int tid = threadIdx.x;
+    const int qy = a.qy, qx = a.qx, R = a.R;
+    const int64_t ndet = (int64_t)qy * qx;
+    const double *w = a.weights;
+    const double centre = w[R];
+    double *sa = a.scratch + (int64_t)blockIdx.x * ndet;
+    __shared__ double red_v[kPyBlock];
+    __shared__ int64_t red_i[kPyBlock], red_n[kPyBlock];
+    __shared__ double red_s[6][kPyBlock];
+    __shared__ int64_t best_at;
+    for (int64_t pos = blockIdx.x; pos < a.npos; pos += gridDim.x) {
+        if (a.take && !a.take[pos]) {
+            if (tid == 0) { a.out_row[pos] = __builtin_nan(""); a.out_col[pos] = __builtin_nan(""); }
+            continue;   // uniform across the block
+        }
+        const void *pat = (const char *)a.cube + pos * ndet * (U16 ? 2 : 4);
+        // axis 0 (rows), mode nearest
+        for (int64_t idx = tid; idx < ndet; idx += kPyBlock) {
+            const int i = (int)(idx / qx), j = (int)(idx % qx);
+            double acc = pix<U16>(pat, idx) * centre;
+            for (int t = R; t >= 1; --t) {
+                const int ia = i - t < 0 ? 0 : i - t, ib = i + t >= qy ? qy - 1 : i + t;
+                acc += (pix<U16>(pat, (int64_t)ia * qx + j) + pix<U16>(pat, (int64_t)ib * qx + j)) * w[R - t];
+            }
+            sa[idx] = acc;
+        }
+        __syncthreads();
+        // axis 1 (columns) and the argmax: the first NaN if any, else the first maximum
+        double bv = -__builtin_inf();
+        int64_t bi = INT64_MAX, bn = INT64_MAX;
+        for (int64_t idx = tid; idx < ndet; idx += kPyBlock) {
+            const int i = (int)(idx / qx), j = (int)(idx % qx);
+            const double *row = sa + (int64_t)i * qx;
+            double acc = row[j] * centre;
+            for (int t = R; t >= 1; --t) {
+                const int ja = j - t < 0 ? 0 : j - t, jb = j + t >= qx ? qx - 1 : j + t;
+                acc += (row[ja] + row[jb]) * w[R - t];
+            }
+            if (a.filtered && pos == 0) a.filtered[idx] = acc;
+            if (__builtin_isnan(acc)) {
+                if (bn == INT64_MAX) bn = idx;
+            } else if (bi == INT64_MAX || acc > bv) {
+                bv = acc;
+                bi = idx;
+            }
+        }
+        red_v[tid] = bv; red_i[tid] = bi; red_n[tid] = bn;
+        __syncthreads();
+        if (tid == 0) {
+            double v = red_v[0];
+            int64_t b = red_i[0], n = red_n[0];
+            for (int k = 1; k < kPyBlock; ++k) {
+                n = red_n[k] < n ? red_n[k] : n;
+                if (red_i[k] != INT64_MAX && (b == INT64_MAX || red_v[k] > v || (red_v[k] == v && red_i[k] < b))) {
+                    v = red_v[k];
+                    b = red_i[k];
+                }
+            }
+            best_at = n != INT64_MAX ? n : (b != INT64_MAX ? b : 0);
+        }
+        __syncthreads();
+        const int64_t at = best_at;
+        const double i0 = (double)(at / qx), j0 = (double)(at % qx);
+        if (a.fast_center) {
+            if (tid == 0) { a.out_row[pos] = i0; a.out_col[pos] = j0; }
+            __syncthreads();
+            continue;
+        }
+        // get_CoM(dp * mask): every pixel times its 0/1 weight, so a NaN anywhere gives NaN
+        double s = 0, sc = 0, r = 0, rc = 0, c = 0, cc = 0;
+        for (int64_t idx = tid; idx < ndet; idx += kPyBlock) {
+            const double fi = (double)(idx / qx), fj = (double)(idx % qx);
+            const double di = fi - i0, dj = fj - j0;
+            const double v = pix<U16>(pat, idx) * (__builtin_sqrt(di * di + dj * dj) < a.limit ? 1.0 : 0.0);
+            two_sum(s, sc, v);
+            two_sum(r, rc, fi * v);
+            two_sum(c, cc, fj * v);
+        }
+        red_s[0][tid] = s; red_s[1][tid] = sc; red_s[2][tid] = r; red_s[3][tid] = rc; red_s[4][tid] = c; red_s[5][tid] = cc;
+        __syncthreads();
+        if (tid == 0) {
+            double S = 0, SC = 0, MR = 0, MRC = 0, MC = 0, MCC = 0;
+            for (int k = 0; k < kPyBlock; ++k) {
+                two_sum(S, SC, red_s[0][k]); SC += red_s[1][k];
+                two_sum(MR, MRC, red_s[2][k]); MRC += red_s[3][k];
+                two_sum(MC, MCC, red_s[4][k]); MCC += red_s[5][k];
+            }
+            const double tot = __builtin_isfinite(S) ? S + SC : S;
+            const double mr = __builtin_isfinite(MR) ? MR + MRC : MR;
+            const double mc = __builtin_isfinite(MC) ? MC + MCC : MC;
+            a.out_row[pos] = mr / tot;
+            a.out_col[pos] = mc / tot;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- iterated window (Float32) ----
+
+__device__ __forceinline__ bool non_finite(float v) {
+    return (__builtin_bit_cast(uint32_t, v) & 0x7F800000u) == 0x7F800000u;
+}
+
+template <bool U16>
+__device__ __forceinline__ float raw(const void *pat, int64_t i) {
+    if constexpr (U16) return (float)((const uint16_t *)pat)[i];
+    else return ((const float *)pat)[i];
+}
+
+// The pixel, or for a non-finite one the median of its finite 8 neighbours of the original pattern (0 when none),
+// with the reference's insertion order (IteratedWindowOrigin.swift:48-70).
+template <bool U16>
+__device__ float filled(const void *pat, int row, int col, int qy, int qx) {
+    const float v = raw<U16>(pat, (int64_t)row * qx + col);
+    if constexpr (U16) {
+        return v;
+    } else {
+        if (!non_finite(v)) return v;
+        float nb[8];
+        int k = 0;
+        for (int dr = -1; dr <= 1; ++dr)
+            for (int dc = -1; dc <= 1; ++dc) {
+                if (dr == 0 && dc == 0) continue;
+                const int rr = row + dr, cc = col + dc;
+                if (rr < 0 || rr >= qy || cc < 0 || cc >= qx) continue;
+                const float w = raw<U16>(pat, (int64_t)rr * qx + cc);
+                if (non_finite(w)) continue;
+                int i = k++;
+                while (i > 0 && nb[i - 1] > w) { nb[i] = nb[i - 1]; --i; }
+                nb[i] = w;
+            }
+        if (k == 0) return 0.0f;
+        return (k % 2 == 1) ? nb[k / 2] : 0.5f * (nb[k / 2 - 1] + nb[k / 2]);
+    }
+}
+
+template <bool U16>
+__global__ __launch_bounds__(64 * kIWWaves) void iterated_window(IWLaunch a, int rows_per_wave, int waves) {
+    extern __shared__ float part[];            // [waves][3][rows_per_wave]
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t pos = (int64_t)blockIdx.x * waves + wave;
+    if (pos >= a.npos) return;                 // whole waves; no block barrier below
+    const int qy = a.qy, qx = a.qx;
+    const int64_t ndet = (int64_t)qy * qx;
+    const void *pat = (const char *)a.cube + pos * ndet * (U16 ? 2 : 4);
+    float *pI = part + (size_t)wave * 3 * rows_per_wave, *pX = pI + rows_per_wave, *pY = pX + rows_per_wave;
+
+    // coarse: the brightest bin x bin block (the first in raster order), its centre
+    const float r = a.radius;
+    const int bin = a.binned ? max(1, (int)__builtin_roundf(r)) : 1;
+    const int nby = (qy + bin - 1) / bin, nbx = (qx + bin - 1) / bin;
+    float best = -3.402823466e38f;
+    int bestb = INT32_MAX;
+    for (int b = lane; b < nby * nbx; b += 64) {
+        const int by = (b / nbx) * bin, bx = (b % nbx) * bin;
+        const int yEnd = min(by + bin, qy), xEnd = min(bx + bin, qx);
+        float s = 0.0f;
+        for (int y = by; y < yEnd; ++y)
+            for (int x = bx; x < xEnd; ++x) s += filled<U16>(pat, y, x, qy, qx);
+        if (s > best) { best = s; bestb = b; }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float ov = __shfl_xor(best, off, 64);
+        const int ob = __shfl_xor(bestb, off, 64);
+        if (ob != INT32_MAX && (bestb == INT32_MAX || ov > best || (ov == best && ob < bestb))) { best = ov; bestb = ob; }
+    }
+    float cr = 0.5f * (float)(qy - 1), cc = 0.5f * (float)(qx - 1);
+    if (bestb != INT32_MAX) {
+        const int by = (bestb / nbx) * bin, bx = (bestb % nbx) * bin;
+        const int yEnd = min(by + bin, qy), xEnd = min(bx + bin, qx);
+        cc = 0.5f * (float)(bx + xEnd - 1);
+        cr = 0.5f * (float)(by + yEnd - 1);
+    }
+    // refinement passes
+    const float rs = r * a.radius_scale, rm = r + a.margin;
+    const float win = rm >= rs ? rm : rs;
+    const float win2 = win * win;
+    for (int pass = 0; pass < a.passes; ++pass) {
+        const int y0 = max(0, (int)__builtin_floorf(cr - win));
+        const int y1 = min(qy - 1, (int)__builtin_ceilf(cr + win));
+        const int x0 = max(0, (int)__builtin_floorf(cc - win));
+        const int x1 = min(qx - 1, (int)__builtin_ceilf(cc + win));
+        const int nrows = min(y1 >= y0 && x1 >= x0 ? y1 - y0 + 1 : 0, rows_per_wave);
+        for (int k = lane; k < nrows; k += 64) {
+            const int y = y0 + k;
+            float sI = 0.0f, sX = 0.0f, sY = 0.0f;
+            const float dy = (float)y - cr;
+            const float dy2 = dy * dy;
+            if (dy2 <= win2) {
+                for (int x = x0; x <= x1; ++x) {
+                    const float dx = (float)x - cc;
+                    if (__builtin_fmaf(dx, dx, dy2) > win2) continue;
+                    const float f = filled<U16>(pat, y, x, qy, qx);
+                    const float v = 0.0f >= f ? 0.0f : f;
+                    sI += v;
+                    sX = __builtin_fmaf(v, (float)x, sX);
+                    sY = __builtin_fmaf(v, (float)y, sY);
+                }
+            }
+            pI[k] = sI; pX[k] = sX; pY[k] = sY;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        float sumI = 0.0f, sumIX = 0.0f, sumIY = 0.0f;
+        if (lane == 0)
+            for (int k = 0; k < nrows; ++k) { sumI += pI[k]; sumIX += pX[k]; sumIY += pY[k]; }
+        sumI = __shfl(sumI, 0, 64);
+        sumIX = __shfl(sumIX, 0, 64);
+        sumIY = __shfl(sumIY, 0, 64);
+        __builtin_amdgcn_wave_barrier();        // the partials are read before the next pass writes them
+        if (!(sumI > 0.0f)) break;
+        const float nc = sumIX / sumI, nr = sumIY / sumI;
+        const bool settled = __builtin_fabsf(nc - cc) < 1e-4f && __builtin_fabsf(nr - cr) < 1e-4f;
+        cc = nc;
+        cr = nr;
+        if (settled) break;
+    }
+    if (lane == 0) {
+        a.out_row[pos] = (double)cr;
+        a.out_col[pos] = (double)cc;
+    }
+}
+
+}  // namespace
+
+int py4_blocks(int cus, int64_t npos) {
+    const int64_t b = (int64_t)cus * 4;
+    return (int)(npos < b ? (npos > 0 ? npos : 1) : b);
+}
+
+hipError_t launch_py4_origin(const Py4Launch &a, int blocks, hipStream_t s) {
+    if (a.npos <= 0) return hipSuccess;
+    if (a.u16) hipLaunchKernelGGL(py4_origin<true>, dim3(blocks), dim3(kPyBlock), 0, s, a);
+    else hipLaunchKernelGGL(py4_origin<false>, dim3(blocks), dim3(kPyBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_iterated_window(const IWLaunch &a, hipStream_t s) {
+    if (a.npos <= 0) return hipSuccess;
+    // window rows: at most the detector's rows
+    const float rs = a.radius * a.radius_scale, rm = a.radius + a.margin;
+    const double win = rm >= rs ? rm : rs;
+    const int64_t span = (int64_t)(2.0 * win) + 3;
+    const int rows = (int)(span < a.qy ? span : a.qy);
+    const size_t per_wave = (size_t)3 * rows * sizeof(float);
+    const int waves = per_wave * kIWWaves <= 65536 ? kIWWaves : 1;
+    const dim3 grid((unsigned)((a.npos + waves - 1) / waves));
+    if (a.u16)
+        hipLaunchKernelGGL(iterated_window<true>, grid, dim3(64 * waves), per_wave * waves, s, a, rows, waves);
+    else
+        hipLaunchKernelGGL(iterated_window<false>, grid, dim3(64 * waves), per_wave * waves, s, a, rows, waves);
+    return hipGetLastError();
+}
+
+}  // namespace cubo

@@ -831,13 +831,31 @@ class ScanSelection:
 
 
 class OriginMap:
-    """An origin per scan position (OriginMeasurement.swift:14-28): row and column maps and their NumPy means."""
+    """An origin per scan position (OriginMeasurement.swift:14-40): row and column maps (flattened in scan order) and
+    their NumPy means."""
 
     def __init__(self, row, column):
         r, c = np.asarray(getattr(row, "values", row), np.float64), np.asarray(getattr(column, "values", column), np.float64)
         if r.shape != c.shape:
             raise CuboVirtualError("originMapMismatch", "the row and column maps differ in shape")
         self.row, self.column = np.ascontiguousarray(r.reshape(-1)), np.ascontiguousarray(c.reshape(-1))
+        self.scan_shape = (getattr(row, "scan_rows", None), getattr(row, "scan_columns", None))
+
+    @property
+    def mean_row(self) -> float:
+        return float(np.mean(self.row))          # NumPy's pairwise mean (NumPySum.mean)
+
+    @property
+    def mean_column(self) -> float:
+        return float(np.mean(self.column))
+
+    def replacing_nan_with_mean(self) -> "OriginMap":
+        """py4DSTEM's remove_NaN (OriginMeasurement.swift:30-39)."""
+        def filled(m):
+            keep = m[~np.isnan(m)]
+            fill = float(np.mean(keep)) if keep.size else np.nan
+            return np.where(np.isnan(m), fill, m)
+        return OriginMap(filled(self.row), filled(self.column))
 
 
 class PatternShift:

@@ -239,6 +239,36 @@ int cubo_hip_pattern(cubo_hip_cube *cube, const cubo_hip_pattern_request *reques
 /* 1 = exact integer kernel (uint16 cube, no weights), 2 = FP64 (compensated sums). */
 int cubo_hip_pattern_path(cubo_hip_cube *cube, const cubo_hip_pattern_request *request, int *path);
 
+/* ---- origins: py4DSTEM's get_origin (OriginMeasurement.measure, Sources/CuboVirtual/OriginMeasurement.swift:52-95)
+ * and the iterated-window estimator (OriginMeasurement.measure(of:radius:radiusScale:estimator:...),
+ * IteratedWindowOrigin.swift:136-241; the Metal kernel cubo_measure_origin, MetalReducers.swift:179-266), per scan
+ * position on the device, and ProbeSize.estimate (ProbeSize.swift:25-58) on the host ---- */
+typedef struct {
+    double radius, center_row, center_column, threshold;
+} cubo_hip_probe_estimate;
+/* py4DSTEM's get_probe_size step for step in Double (NumPy's linspace, gradient, median and pairwise sums): the
+ * radius of the bright disk of a pattern (normally the maximum pattern). Host only. "probeNotFound" refusals. */
+int cubo_hip_probe_size(const double *pattern, int32_t rows, int32_t cols, double lower_threshold,
+                        double upper_threshold, int32_t count, cubo_hip_probe_estimate *out);
+enum { CUBO_HIP_ORIGIN_PY4DSTEM = 0, CUBO_HIP_ORIGIN_ITERATED_WINDOW = 1 };
+typedef struct {
+    int32_t estimator;         /* CUBO_HIP_ORIGIN_* */
+    double radius;             /* the probe radius in pixels (sigma of the Gaussian, or the window's radius) */
+    double radius_scale;       /* 1.2 */
+    const uint8_t *scan_mask;  /* py4DSTEM: scanCount flags, origins only where set (NaN elsewhere), or NULL */
+    int32_t fast_center;       /* py4DSTEM: the filtered argmax itself, no centre of mass */
+    int32_t passes;            /* iterated window: 0 ... 1000 (4) */
+    double minimum_margin;     /* iterated window: >= 0 (1.5) */
+    int32_t binned_coarse;     /* iterated window: the brightest round(radius)-sized block (1) or pixel (0) */
+} cubo_hip_origin_request;
+/* The origins of scan rows [row0, row0 + nrows) into out_row / out_col (scanCount Doubles each; only the tile's
+ * entries are written), so a caller walks the tiles with cancellation and progress between them. */
+int cubo_hip_origins(cubo_hip_cube *cube, const cubo_hip_origin_request *request, int64_t row0, int64_t nrows,
+                     double *out_row, double *out_col);
+/* GaussianFilter.apply (OriginMeasurement.swift:127-135) of one pattern: scipy's gaussian_filter with mode nearest,
+ * in Double on the device (the filter the get_origin kernel runs). */
+int cubo_hip_gaussian_filter_pattern(cubo_hip_cube *cube, int64_t row, int64_t column, double sigma, double *out);
+
 /* ---- the mask stack: Y = X W for K weight rows in one pass, X the (scanCount x detectorCount) cube.
  * {1, q_row, q_col, masks...} gives K virtual images and the centre-of-mass sums together. No reference
  * counterpart (the reference runs VirtualImaging.image once per mask, CPUReducers.swift:13-39). ---- */

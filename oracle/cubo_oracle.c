@@ -911,3 +911,198 @@ void oc_cpu_first_processing_u16(const uint16_t *cube, int64_t npos, int qy, int
     }
     free(tile); free(sum); free(corr); free(mx); free(pat); free(bfd); free(adfd);
 }
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Origins.                                                                                          */
+/*  - py4DSTEM's get_origin (OriginMeasurement.swift:52-95): scipy's gaussian_filter (sigma = r,      */
+/*    truncate 4, mode nearest; weights exp(-0.5/sigma^2 x^2) over NumPy's pairwise sum, SciPyKernels */
+/*    .swift:66-78; axis 0 then axis 1 in NI_Correlate1D's order, :128-150), the first argmax (the   */
+/*    first NaN if any), the Neumaier centre of mass of p * [hypot < r * rscale];                    */
+/*  - ProbeSize.estimate (ProbeSize.swift:25-58);                                                    */
+/*  - the iterated-window estimator in Float32 (IteratedWindowOrigin.swift:48-70, 136-180).          */
+/* ---------------------------------------------------------------------------------------------- */
+
+int oc_gaussian_weights(double sigma, double *w, int max_len) {
+    const int R = (int)(4.0 * sigma + 0.5);
+    if (2 * R + 1 > max_len) return -1;
+    const double sigma2 = sigma * sigma;
+    for (int x = -R; x <= R; ++x) w[x + R] = exp(-0.5 / sigma2 * (double)(x * x));
+    const double s = oc_pairwise_sum(w, 2 * R + 1);
+    for (int k = 0; k <= 2 * R; ++k) w[k] = w[k] / s;
+    return R;
+}
+
+static inline int nearest(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
+
+void oc_gaussian_filter(const double *p, int qy, int qx, const double *w, int R, double *out) {
+    double *a = (double *)malloc(sizeof(double) * qy * qx);
+    const double centre = w[R];
+    for (int i = 0; i < qy; ++i)
+        for (int j = 0; j < qx; ++j) {
+            double acc = p[i * qx + j] * centre;
+            for (int t = R; t >= 1; --t) acc += (p[nearest(i - t, qy) * qx + j] + p[nearest(i + t, qy) * qx + j]) * w[R - t];
+            a[i * qx + j] = acc;
+        }
+    for (int i = 0; i < qy; ++i)
+        for (int j = 0; j < qx; ++j) {
+            double acc = a[i * qx + j] * centre;
+            for (int t = R; t >= 1; --t) acc += (a[i * qx + nearest(j - t, qx)] + a[i * qx + nearest(j + t, qx)]) * w[R - t];
+            out[i * qx + j] = acc;
+        }
+    free(a);
+}
+
+int oc_origin_py4dstem(const float *cube, int64_t npos, int qy, int qx, double radius, double rscale,
+                       const uint8_t *take, int fast_center, double *out_r, double *out_c) {
+    if (!(radius > 0) || !isfinite(radius) || radius * radius < 2.2250738585072014e-308 || !(4.0 * radius + 0.5 < 1e6))
+        return OC_INVALID_PARAMETER;
+    if (!(rscale > 0) || !isfinite(rscale)) return OC_INVALID_PARAMETER;
+    const int64_t n = (int64_t)qy * qx;
+    const int len = 2 * (int)(4.0 * radius + 0.5) + 1;
+    double *w = (double *)malloc(sizeof(double) * len);
+    const int R = oc_gaussian_weights(radius, w, len);
+    const double limit = radius * rscale;
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t k = 0; k < npos; ++k) {
+        if (take && !take[k]) { out_r[k] = NAN; out_c[k] = NAN; continue; }
+        double *p = (double *)malloc(sizeof(double) * n), *g = (double *)malloc(sizeof(double) * n);
+        for (int64_t i = 0; i < n; ++i) p[i] = (double)cube[k * n + i];
+        oc_gaussian_filter(p, qy, qx, w, R, g);
+        int64_t best = 0, nan = -1;
+        for (int64_t i = 0; i < n; ++i) if (isnan(g[i])) { nan = i; break; }
+        if (nan >= 0) best = nan; else for (int64_t i = 1; i < n; ++i) if (g[i] > g[best]) best = i;
+        const double i0 = (double)(best / qx), j0 = (double)(best % qx);
+        if (fast_center) { out_r[k] = i0; out_c[k] = j0; free(p); free(g); continue; }
+        neumaier s = {0, 0}, mr = {0, 0}, mc = {0, 0};
+        for (int64_t i = 0; i < n; ++i) {
+            const double di = (double)(i / qx) - i0, dj = (double)(i % qx) - j0;
+            const double v = p[i] * (hypot(di, dj) < limit ? 1.0 : 0.0);
+            nm_add(&s, v); nm_add(&mr, (double)(i / qx) * v); nm_add(&mc, (double)(i % qx) * v);
+        }
+        out_r[k] = nm_value(&mr) / nm_value(&s);
+        out_c[k] = nm_value(&mc) / nm_value(&s);
+        free(p); free(g);
+    }
+    free(w);
+    return OC_OK;
+}
+
+static int cmp_double(const void *a, const void *b) {
+    const double x = *(const double *)a, y = *(const double *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+int oc_probe_size(const double *v, int rows, int cols, double lower, double upper, int count, double out[4]) {
+    if (count < 2) return OC_INVALID_PARAMETER;
+    const int64_t n = (int64_t)rows * cols;
+    double *th = (double *)malloc(sizeof(double) * count), *radii = (double *)malloc(sizeof(double) * count);
+    double *grad = (double *)malloc(sizeof(double) * count), *sorted = (double *)malloc(sizeof(double) * count);
+    const double step = (upper - lower) / (double)(count - 1);
+    for (int k = 0; k < count; ++k) th[k] = (double)k * step + lower;
+    th[count - 1] = upper;
+    double maximum = -INFINITY;
+    for (int64_t i = 0; i < n; ++i) maximum = (isnan(maximum) || isnan(v[i])) ? NAN : (v[i] > maximum ? v[i] : maximum);
+    for (int k = 0; k < count; ++k) {
+        const double limit = maximum * th[k];
+        int64_t c = 0;
+        for (int64_t i = 0; i < n; ++i) c += v[i] > limit;
+        radii[k] = sqrt((double)c / M_PI);
+    }
+    grad[0] = radii[1] - radii[0];
+    grad[count - 1] = radii[count - 1] - radii[count - 2];
+    for (int k = 1; k < count - 1; ++k) grad[k] = (radii[k + 1] - radii[k - 1]) / 2.0;
+    memcpy(sorted, grad, sizeof(double) * count);
+    qsort(sorted, count, sizeof(double), cmp_double);
+    const double median = count % 2 ? sorted[count / 2] : (sorted[count / 2 - 1] + sorted[count / 2]) / 2;
+    double *tr = (double *)malloc(sizeof(double) * count), *tt = (double *)malloc(sizeof(double) * count);
+    int nt = 0;
+    for (int k = 0; k < count; ++k)
+        if (grad[k] <= 0 && grad[k] >= 2 * median) { tr[nt] = radii[k]; tt[nt] = th[k]; ++nt; }
+    const double radius = nt ? oc_pairwise_sum(tr, nt) / (double)nt : NAN;
+    const double threshold = nt ? oc_pairwise_sum(tt, nt) / (double)nt : NAN;
+    const double limit = maximum * threshold;
+    double *m = (double *)malloc(sizeof(double) * n), *mr = (double *)malloc(sizeof(double) * n),
+           *mc = (double *)malloc(sizeof(double) * n);
+    for (int64_t i = 0; i < n; ++i) {
+        m[i] = v[i] * (v[i] > limit ? 1.0 : 0.0);
+        mr[i] = (double)(i / cols) * m[i];
+        mc[i] = (double)(i % cols) * m[i];
+    }
+    const double total = oc_pairwise_sum(m, n);
+    out[0] = radius;
+    out[1] = oc_pairwise_sum(mr, n) / total;
+    out[2] = oc_pairwise_sum(mc, n) / total;
+    out[3] = threshold;
+    free(th); free(radii); free(grad); free(sorted); free(tr); free(tt); free(m); free(mr); free(mc);
+    return (isfinite(out[0]) && isfinite(out[1]) && isfinite(out[2])) ? OC_OK : OC_ZERO_FIELD;
+}
+
+static float iw_filled(const float *pat, int row, int col, int qy, int qx) {
+    const float v = pat[row * qx + col];
+    if (isfinite(v)) return v;
+    float nb[8];
+    int k = 0;
+    for (int dr = -1; dr <= 1; ++dr)
+        for (int dc = -1; dc <= 1; ++dc) {
+            if (dr == 0 && dc == 0) continue;
+            const int rr = row + dr, cc = col + dc;
+            if (rr < 0 || rr >= qy || cc < 0 || cc >= qx) continue;
+            const float w = pat[rr * qx + cc];
+            if (!isfinite(w)) continue;
+            int i = k++;
+            while (i > 0 && nb[i - 1] > w) { nb[i] = nb[i - 1]; --i; }
+            nb[i] = w;
+        }
+    if (k == 0) return 0.0f;
+    return (k % 2 == 1) ? nb[k / 2] : 0.5f * (nb[k / 2 - 1] + nb[k / 2]);
+}
+
+void oc_origin_iterated(const float *cube, int64_t npos, int qy, int qx, float r, float rscale, int passes, float margin,
+                        int binned, double *out_r, double *out_c) {
+    const int64_t n = (int64_t)qy * qx;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < npos; ++k) {
+        const float *pat = cube + k * n;
+        const int bin = binned ? (roundf(r) > 1 ? (int)roundf(r) : 1) : 1;
+        float best = -3.402823466e38f;
+        float cr = 0.5f * (float)(qy - 1), cc = 0.5f * (float)(qx - 1);
+        for (int by = 0; by < qy; by += bin) {
+            const int ye = by + bin < qy ? by + bin : qy;
+            for (int bx = 0; bx < qx; bx += bin) {
+                const int xe = bx + bin < qx ? bx + bin : qx;
+                float s = 0.0f;
+                for (int y = by; y < ye; ++y)
+                    for (int x = bx; x < xe; ++x) s += iw_filled(pat, y, x, qy, qx);
+                if (s > best) { best = s; cc = 0.5f * (float)(bx + xe - 1); cr = 0.5f * (float)(by + ye - 1); }
+            }
+        }
+        const float rs = r * rscale, rm = r + margin;
+        const float win = rm >= rs ? rm : rs, win2 = win * win;
+        for (int pass = 0; pass < passes; ++pass) {
+            int y0 = (int)floorf(cr - win), y1 = (int)ceilf(cr + win), x0 = (int)floorf(cc - win), x1 = (int)ceilf(cc + win);
+            y0 = y0 < 0 ? 0 : y0; x0 = x0 < 0 ? 0 : x0;
+            y1 = y1 > qy - 1 ? qy - 1 : y1; x1 = x1 > qx - 1 ? qx - 1 : x1;
+            float sI = 0.0f, sX = 0.0f, sY = 0.0f;
+            for (int y = y0; y <= y1; ++y) {
+                const float dy = (float)y - cr, dy2 = dy * dy;
+                if (dy2 > win2) continue;
+                for (int x = x0; x <= x1; ++x) {
+                    const float dx = (float)x - cc;
+                    if (fmaf(dx, dx, dy2) > win2) continue;
+                    const float f = iw_filled(pat, y, x, qy, qx);
+                    const float v = 0.0f >= f ? 0.0f : f;
+                    sI += v;
+                    sX = fmaf(v, (float)x, sX);
+                    sY = fmaf(v, (float)y, sY);
+                }
+            }
+            if (!(sI > 0.0f)) break;
+            const float nc = sX / sI, nr = sY / sI;
+            const int settled = fabsf(nc - cc) < 1e-4f && fabsf(nr - cr) < 1e-4f;
+            cc = nc; cr = nr;
+            if (settled) break;
+        }
+        out_r[k] = (double)cr;
+        out_c[k] = (double)cc;
+    }
+}

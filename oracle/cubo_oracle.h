@@ -74,6 +74,16 @@ int oc_vi_bilinear_grad_origin(const float *cube, int64_t npos, int qy, int qx, 
 int oc_pattern(const float *cube, int64_t npos, int qy, int qx, int stat, const uint8_t *take, const double *w,
                const double *orow, const double *ocol, int ties_later, double *out);
 
+/* Origins: py4DSTEM's get_origin (Double), ProbeSize.estimate (out: radius, centre row, centre column, threshold)
+ * and the iterated-window estimator (Float32). Gaussian weights return R (the kernel is 2R + 1 long), or -1. */
+int oc_gaussian_weights(double sigma, double *w, int max_len);
+void oc_gaussian_filter(const double *p, int qy, int qx, const double *w, int R, double *out);
+int oc_origin_py4dstem(const float *cube, int64_t npos, int qy, int qx, double radius, double rscale,
+                       const uint8_t *take, int fast_center, double *out_r, double *out_c);
+int oc_probe_size(const double *v, int rows, int cols, double lower, double upper, int count, double out[4]);
+void oc_origin_iterated(const float *cube, int64_t npos, int qy, int qx, float r, float rscale, int passes, float margin,
+                        int binned, double *out_r, double *out_c);
+
 /* NumPy pairwise float64 sum. */
 double oc_pairwise_sum(const double *a, int64_t n);
 

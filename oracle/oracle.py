@@ -49,6 +49,13 @@ def _load():
         "oc_vi_bilinear_grad_origin": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
         "oc_pattern": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                  ct.c_void_p, ct.c_int, _dp]),
+        "oc_gaussian_weights": (ct.c_int, [ct.c_double, _dp, ct.c_int]),
+        "oc_gaussian_filter": (None, [_dp, ct.c_int, ct.c_int, _dp, ct.c_int, _dp]),
+        "oc_origin_py4dstem": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_void_p,
+                                         ct.c_int, _dp, _dp]),
+        "oc_probe_size": (ct.c_int, [_dp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int, _dp]),
+        "oc_origin_iterated": (None, [_fp, _i64, ct.c_int, ct.c_int, ct.c_float, ct.c_float, ct.c_int, ct.c_float,
+                                     ct.c_int, _dp, _dp]),
         "oc_pairwise_sum": (ct.c_double, [_dp, _i64]),
         "oc_fft2d": (None, [_dp, _dp, ct.c_int, ct.c_int, ct.c_int]),
         "oc_plane_fit": (ct.c_int, [_dp, ct.c_int, ct.c_int, ct.c_double, _dp]),
@@ -169,6 +176,45 @@ def pattern(cube, statistic: str, take=None, weights=None, origin=None, ties: st
                           1 if ties == "later" else 0, out)
     _check(st, f"pattern {statistic}")
     return out
+
+
+def gaussian_filter(pattern, sigma: float) -> np.ndarray:
+    """scipy.ndimage.gaussian_filter(pattern, sigma, mode='nearest') in the reference's order."""
+    p = np.ascontiguousarray(pattern, np.float64)
+    qy, qx = p.shape
+    w = np.zeros(2 * int(4.0 * sigma + 0.5) + 1)
+    R = lib().oc_gaussian_weights(float(sigma), w, w.size)
+    out = np.zeros(qy * qx)
+    lib().oc_gaussian_filter(p.ravel(), qy, qx, w, R, out)
+    return out.reshape(qy, qx)
+
+
+def origin_py4dstem(cube, radius: float, radius_scale: float = 1.2, take=None, fast_center: bool = False):
+    """OriginMeasurement.measure (py4DSTEM get_origin): (row map, column map) per position."""
+    x, ry, rx, qy, qx = _flat(cube)
+    r, c = np.zeros(ry * rx), np.zeros(ry * rx)
+    t = None if take is None else np.ascontiguousarray(np.asarray(take, bool).astype(np.uint8))
+    _check(lib().oc_origin_py4dstem(x, ry * rx, qy, qx, float(radius), float(radius_scale),
+                                    None if t is None else t.ctypes.data, int(fast_center), r, c), "origin")
+    return r, c
+
+
+def probe_size(pattern, lower=0.01, upper=0.99, count=100):
+    """ProbeSize.estimate: (radius, centre row, centre column, threshold)."""
+    p = np.ascontiguousarray(pattern, np.float64)
+    out = np.zeros(4)
+    _check(lib().oc_probe_size(p.ravel(), p.shape[0], p.shape[1], lower, upper, count, out), "probe size")
+    return tuple(out)
+
+
+def origin_iterated(cube, radius: float, radius_scale: float = 1.2, passes: int = 4, margin: float = 1.5,
+                    binned: bool = True):
+    """The iterated-window estimator in Float32: (row map, column map)."""
+    x, ry, rx, qy, qx = _flat(cube)
+    r, c = np.zeros(ry * rx), np.zeros(ry * rx)
+    lib().oc_origin_iterated(x, ry * rx, qy, qx, float(radius), float(radius_scale), int(passes), float(margin),
+                             int(binned), r, c)
+    return r, c
 
 
 def com_sums(cube, weights=None):

@@ -1,0 +1,105 @@
+"""Origins on the device against CuboKit's origin-fit pins and the oracle.
+
+Bars: the default radius (the probe size of the device's max pattern) bit for bit; get_origin's origins within
+1e-9 px of the pinned float64 values (OriginFitSupport.swift:44-46) and within 1e-12 px of the oracle; the filtered
+pattern within 1e-12 (:48-50); the fast centre (the filtered argmax) exact; the iterated window within relative 1e-5
+of the oracle's Float32 loop, the reference's own bar between its CPU and Metal implementations
+(IteratedWindowOriginTests.swift:36-50).
+"""
+import numpy as np
+import pytest
+
+import cubokit_amd as C
+from cubokit_amd import origins
+from oracle import oracle as O
+from tests.support import assert_relative, doubles, load_cube, origin_fit_pins
+from tests.test_oracle_origin_pins import iterated_window_cube
+
+pytestmark = pytest.mark.gpu
+
+
+def test_pinned_radii_origins_and_filters():
+    for e in origin_fit_pins()["cubes"]:
+        print(f"case: origin pins {e['cube']}", flush=True)
+        arr = load_cube(e["cube"])
+        res = C.HIPResidentCube(arr)
+        omap, radius = C.OriginMeasurement.measure(res, tiling=C.ScanTiling.rows(2))
+        assert radius == float(e["radius"]), e["cube"]
+        assert np.max(np.abs(omap.row - doubles(e["origin_row"]))) <= 1e-9
+        assert np.max(np.abs(omap.column - doubles(e["origin_column"]))) <= 1e-9
+        r, c = O.origin_py4dstem(arr, radius, 1.2)
+        assert np.max(np.abs(omap.row - r)) <= 1e-12 and np.max(np.abs(omap.column - c)) <= 1e-12
+        f = origins.gaussian_filter(res, 0, 0, radius).ravel()
+        assert_relative(f, doubles(e["filtered_00"]), 1e-12, f"{e['cube']} filtered")
+        fast, _ = C.OriginMeasurement.measure(res, radius, fast_center=True)
+        fr, fc = O.origin_py4dstem(arr, radius, 1.2, fast_center=True)
+        np.testing.assert_array_equal(fast.row, fr)
+        np.testing.assert_array_equal(fast.column, fc)
+        n = arr.shape[0] * arr.shape[1]
+        mask = np.arange(n) % 3 != 1
+        part, _ = C.OriginMeasurement.measure(res, radius, scan_mask=mask)
+        assert np.all(np.isnan(part.row[~mask])) and np.max(np.abs(part.row[mask] - r[mask])) <= 1e-12
+        res.release()
+
+
+def test_argmax_ties_boundaries_and_nan():
+    """OriginFitTests.swift:50-73 and 88-105 on the device."""
+    q = 9
+    flat = C.HIPResidentCube(np.full((1, 1, q, q), 2.0, np.float32))
+    o, _ = C.OriginMeasurement.measure(flat, 3.0, 1.0)
+    ii, jj = np.meshgrid(np.arange(q), np.arange(q), indexing="ij")
+    inside = np.sqrt(ii ** 2 + jj ** 2) < 3
+    assert abs(o.row[0] - ii[inside].mean()) <= 1e-12 and abs(o.column[0] - jj[inside].mean()) <= 1e-12
+    i = np.arange(24)[:, None] - 11.5
+    j = np.arange(25)[None, :] - 12.5
+    v = np.where(i * i + j * j < 6.1 * 6.1, 1000.0, 0.0).astype(np.float32).reshape(1, 1, 24, 25)
+    o, _ = C.OriginMeasurement.measure(C.HIPResidentCube(v), 3.3, 1.2)
+    assert o.row[0] == 11 and o.column[0] == 12
+    v[0, 0, 0, 0] = np.nan
+    o, _ = C.OriginMeasurement.measure(C.HIPResidentCube(v), 3.3)
+    assert np.isnan(o.row[0])
+    with pytest.raises(C.CuboVirtualError):
+        C.OriginMeasurement.measure(flat, -1.0)
+
+
+@pytest.mark.parametrize("ring,nan", [(False, True), (True, False)])
+def test_iterated_window_matches_the_oracle(ring, nan):
+    cube, truth = iterated_window_cube(ring=ring, nan=nan)
+    res = C.HIPResidentCube(cube)
+    for est, kw in ((C.OriginEstimator.iterated_window(), {}),
+                    (C.OriginEstimator.iterated_window(passes=2, minimum_margin=0.5, binned_coarse=False),
+                     {"passes": 2, "margin": 0.5, "binned": False})):
+        print(f"case: iterated window ring={ring} nan={nan} {est}", flush=True)
+        m = C.OriginMeasurement.measure_with(res, 6.2, estimator=est, tiling=C.ScanTiling.rows(3))
+        r, c = O.origin_iterated(cube, 6.2, **kw)
+        for a, b in ((m.row, r), (m.column, c)):
+            assert np.all(np.abs(a - b) <= 1e-5 * np.maximum(np.abs(b), 1)), (a, b)
+    if not ring:
+        m = C.OriginMeasurement.measure_with(res, 6.2)
+        for k, (tr, tc) in enumerate(truth):
+            assert abs(m.row[k] - tr) < 0.1 and abs(m.column[k] - tc) < 0.1
+    with pytest.raises(C.CuboVirtualError):
+        C.OriginMeasurement.measure_with(res, 6.2, estimator=C.OriginEstimator.iterated_window(passes=1001))
+
+
+def test_bandwidth_cube_origins_against_the_oracle_on_two_scan_rows():
+    """The 256x256x128x128 uint16 cube: every position's origin on the device; two scan rows (512 positions)
+    against the oracle, and the recovered origins follow the planted deflection."""
+    from cubokit_amd import synthetic
+    shape, dtype, params = synthetic.bandwidth_u16()
+    res = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
+    res.fill_synthetic(params)
+    print("case: full-size get_origin", flush=True)
+    omap, radius = C.OriginMeasurement.measure(res, tiling=C.ScanTiling.rows(shape[0]))
+    rows = range(100, 102)
+    sub = res.read_scan_tile(rows)
+    r, c = O.origin_py4dstem(sub, radius, 1.2)
+    sl = slice(rows.start * shape[1], rows.stop * shape[1])
+    assert np.max(np.abs(omap.row[sl] - r)) <= 1e-12 and np.max(np.abs(omap.column[sl] - c)) <= 1e-12
+    print("case: full-size iterated window", flush=True)
+    iw = C.OriginMeasurement.measure_with(res, radius, tiling=C.ScanTiling.rows(shape[0]))
+    ir, ic = O.origin_iterated(sub, radius)
+    assert np.all(np.abs(iw.row[sl] - ir) <= 1e-5 * np.maximum(np.abs(ir), 1))
+    assert np.all(np.abs(iw.column[sl] - ic) <= 1e-5 * np.maximum(np.abs(ic), 1))
+    # both estimators see the same beam: their maps agree to a fraction of a pixel
+    assert np.median(np.abs(iw.row - omap.row)) < 0.5

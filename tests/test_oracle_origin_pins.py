@@ -55,3 +55,95 @@ def test_half_shifts_round_to_even():
     rows = (np.arange(30) % 2).astype(np.float64)
     m = O.mask("circle", 16, 16, centerRow=7, centerColumn=8, radius=4.5)
     assert_relative(O.vi_rounded(cube, m.astype(np.float64), rows, rows), O.virtual_image(cube, m), 1e-12, "half shifts")
+
+
+def _max_pattern(cube):
+    x = np.asarray(cube).astype(np.float32).reshape(-1, cube.shape[2] * cube.shape[3])
+    return x.max(axis=0).astype(np.float64).reshape(cube.shape[2:])
+
+
+def test_probe_radius_origins_and_filter_on_the_pins():
+    """OriginFitSupport.swift:36-49: the default radius (probe size of the max pattern) exact, the origins within
+    1e-9 px, the filtered first pattern within 1e-12; the planted cubes' truth within 1e-9 px
+    (OriginFitTests.swift:21-27)."""
+    import cubokit_amd as C
+    for e in origin_fit_pins()["cubes"]:
+        cube = load_cube(e["cube"])
+        mx = _max_pattern(cube)
+        radius = O.probe_size(mx)[0]
+        assert radius == float(e["radius"]), e["cube"]
+        # the library's host ProbeSize gives the same bits (no device needed)
+        lib_r = C.ProbeSize.estimate(C.DetectorImage(mx.shape[0], mx.shape[1], mx.ravel(), C.ComputePath.hip)).radius
+        assert lib_r == radius
+        r, c = O.origin_py4dstem(cube, radius, 1.2)
+        assert np.max(np.abs(r - doubles(e["origin_row"]))) <= 1e-9
+        assert np.max(np.abs(c - doubles(e["origin_column"]))) <= 1e-9
+        f = O.gaussian_filter(cube[0, 0].astype(np.float32).astype(np.float64), radius).ravel()
+        assert_relative(f, doubles(e["filtered_00"]), 1e-12, f"{e['cube']} filtered")
+        if e.get("truth_row"):
+            assert np.max(np.abs(r - doubles(e["truth_row"]))) <= 1e-9
+            assert np.max(np.abs(c - doubles(e["truth_column"]))) <= 1e-9
+
+
+def test_argmax_ties_and_the_strict_mask_boundary():
+    """OriginFitTests.swift:50-73 by hand: a constant pattern's argmax is pixel (0, 0) and the disk keeps
+    |q - argmax| < r * rscale strictly; a bright pixel on a gradient."""
+    q = 9
+    flat = np.full((1, 1, q, q), 2.0, np.float32)
+    r, c = O.origin_py4dstem(flat, 3.0, 1.0)
+    ii, jj = np.meshgrid(np.arange(q), np.arange(q), indexing="ij")
+    inside = np.sqrt(ii ** 2 + jj ** 2) < 3
+    assert abs(r[0] - ii[inside].mean()) <= 1e-12 and abs(c[0] - jj[inside].mean()) <= 1e-12
+    v = (1 + np.arange(q * q) // q).astype(np.float32)
+    v[4 * q + 4] = 1000
+    cube = v.reshape(1, 1, q, q)
+    r, c = O.origin_py4dstem(cube, 3.0, 1.0)
+    d = np.sqrt((ii - 4.0) ** 2 + (jj - 4.0) ** 2) < 3
+    w = v.reshape(q, q).astype(np.float64)
+    assert abs(r[0] - (ii * w)[d].sum() / w[d].sum()) <= 1e-12
+
+
+def test_symmetric_ties_keep_the_first_argmax():
+    """OriginFitTests.swift:88-105: a disk centred at (11.5, 12.5) filters to exactly symmetric values, so the first
+    of the tied maxima wins and the origin is (11, 12); a NaN anywhere makes the origin NaN."""
+    i = np.arange(24)[:, None] - 11.5
+    j = np.arange(25)[None, :] - 12.5
+    v = np.where(i * i + j * j < 6.1 * 6.1, 1000.0, 0.0).astype(np.float32).reshape(1, 1, 24, 25)
+    r, c = O.origin_py4dstem(v, 3.3, 1.2)
+    assert r[0] == 11 and c[0] == 12
+    v2 = v.copy()
+    v2[0, 0, 0, 0] = np.nan
+    r, _ = O.origin_py4dstem(v2, 3.3)
+    assert np.isnan(r[0])
+
+
+def iterated_window_cube(rows=4, columns=5, q=40, ring=False, nan=False):
+    """IteratedWindowOriginTests.swift:13-34: Gaussian-edged disks of radius 6 at per-position sub-pixel centres."""
+    values, truth = [], []
+    i = np.arange(q, dtype=np.float64)[:, None]
+    j = np.arange(q, dtype=np.float64)[None, :]
+    for sr in range(rows):
+        for sc in range(columns):
+            cr, cc = 19.3 + 0.21 * sr - 0.07 * sc, 20.6 - 0.13 * sr + 0.17 * sc
+            truth.append((cr, cc))
+            d = np.sqrt((i - cr) ** 2 + (j - cc) ** 2)
+            ii, jj = np.meshgrid(np.arange(q), np.arange(q), indexing="ij")
+            v = 100 / (1 + np.exp((d - 6) / 0.5)) + 1 + 0.01 * ((ii * 7 + jj * 3 + sr + sc) % 5)
+            if ring:
+                v = v + 40 / (1 + np.exp((np.abs(d - 14) - 1.5) / 0.4))
+            v = v.astype(np.float32).ravel()
+            if nan:
+                v[19 * q + 20] = np.nan
+                v[21 * q + 18] = np.inf
+                v[3] = -np.inf
+                v[0] = np.nan
+            values.append(v)
+    return np.array(values).reshape(rows, columns, q, q), truth
+
+
+def test_iterated_window_recovers_the_planted_origin():
+    """IteratedWindowOriginTests.swift:52-60 on the oracle: within 0.1 px with non-finite pixels."""
+    cube, truth = iterated_window_cube(nan=True)
+    r, c = O.origin_iterated(cube, 6.2)
+    for k, (tr, tc) in enumerate(truth):
+        assert abs(r[k] - tr) < 0.1 and abs(c[k] - tc) < 0.1, (k, r[k], c[k], tr, tc)
