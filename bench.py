@@ -238,6 +238,27 @@ def ingest_section(C, cube):
     return out
 
 
+def origins_section(C, cube):
+    """Origins of every position of the bandwidth cube: py4DSTEM's get_origin (Double Gaussian filter, argmax,
+    centre of mass) and the iterated window (Float32), one launch each over the whole scan."""
+    s = cube.shape
+    mx = C.VirtualDiffraction.max(cube)
+    radius = C.ProbeSize.estimate(mx).radius
+    one = C.ScanTiling.rows(s.scan_rows)
+    out = {"radius": radius}
+    for name, run in (("get_origin", lambda: C.OriginMeasurement.measure(cube, radius, tiling=one)),
+                      ("iterated_window", lambda: C.OriginMeasurement.measure_with(cube, radius, tiling=one))):
+        run()
+        cube.kernel_time()
+        for _ in range(3):
+            run()
+        ms, n = cube.kernel_time()
+        out[name + "_ms"] = round(ms / max(1, n), 3)
+        out[name + "_positions_per_s"] = round(s.scan_count / (ms / max(1, n) / 1e3), 0)
+        log(f"origins {name}: {ms / max(1, n):.3f} ms for {s.scan_count} positions")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,6 +273,7 @@ def main() -> int:
     ap.add_argument("--no-diff", action="store_true", help="skip the 8 GiB differentiation workload")
     ap.add_argument("--no-pattern", action="store_true", help="skip the selected-area pattern timings")
     ap.add_argument("--no-ingest", action="store_true", help="skip the host-to-HBM ingestion timings")
+    ap.add_argument("--no-origins", action="store_true", help="skip the origin timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload runs on one MI355X "
@@ -344,6 +366,7 @@ def main() -> int:
 
     patterns = None if args.no_pattern else pattern_section(C, cube)
     ingest = None if args.no_ingest else ingest_section(C, cube)
+    origins = None if args.no_origins else origins_section(C, cube)
     batch = None if args.no_batch else batch_section(C)
     diff = None if args.no_diff else diff_section(C)
 
@@ -367,6 +390,7 @@ def main() -> int:
         "mask_stack_k64": stack,
         "selected_area": patterns,
         "ingest": ingest,
+        "origins": origins,
         "batch_48": batch,
         "differentiation": diff,
     }
