@@ -30,6 +30,12 @@ using cubo::wrap_shift;
 struct cubo_hip_shift_plan {
     cubo_hip_cube *cube;
     int k, kp;
+    // A cube the matrix-core path does not take (Float32, or a pattern length not a multiple of 8) runs the forward
+    // pass on the vector ALUs in Double (shift_valu); its adjoint is not offered.
+    bool valu = false;
+    float *vmasks = nullptr;         // [k][ndet]
+    int32_t *vshift = nullptr;       // [npos][2]
+    double *vwr = nullptr, *vwc = nullptr;
     int nchunks;
     int any_lo;
     void *base = nullptr;            // split weights, one 32-mask tile [chunk][kstep][hi, lo][lane][8]
@@ -65,7 +71,7 @@ void free_plan(cubo_hip_shift_plan *p) {
     DeviceGuard g(p->cube->device);
     (void)hipStreamSynchronize(p->cube->stream);
     void *bufs[] = {p->base, p->inv, p->wroll, p->perm, p->shift, p->slot_shifts, p->tiles, p->wr, p->wc, p->wts,
-                    p->y, p->dy, p->g, p->dy32, p->partial, p->dmask};
+                    p->y, p->dy, p->g, p->dy32, p->partial, p->dmask, p->vmasks, p->vshift, p->vwr, p->vwc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : p->ev)
@@ -229,8 +235,8 @@ int cubo_hip_shift_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
     *out = nullptr;
     if (k < 1 || k > 32) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "%d masks (1..32)", k);
     if (int st = cubo::require_resident(c, "the shifted stack")) return st;
-    if (c->dtype != CUBO_HIP_U16 || c->ndet % 8 != 0)
-        return fail(CUBO_HIP_ERR_UNSUPPORTED, "the shifted stack needs a uint16 cube whose pattern length is a multiple of 8");
+    if (c->shape[2] > 65535 || c->shape[3] > 65535)
+        return fail(CUBO_HIP_ERR_UNSUPPORTED, "the shifted stack takes detectors up to 65535 pixels a side");
     for (int64_t i = 0; i < (int64_t)k * c->ndet; ++i)
         if (!std::isfinite(masks[i])) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "a mask needs finite weights");
     auto *p = new (std::nothrow) cubo_hip_shift_plan();
@@ -239,6 +245,16 @@ int cubo_hip_shift_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
     p->k = k;
     p->kp = k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : k <= 24 ? 24 : 32;
     DeviceGuard g(c->device);
+    if (c->dtype != CUBO_HIP_U16 || c->ndet % 8 != 0) {
+        p->valu = true;
+        hipError_t e = hipMalloc((void **)&p->vmasks, (size_t)k * c->ndet * 4);
+        if (e == hipSuccess) e = hipMemcpy(p->vmasks, masks, (size_t)k * c->ndet * 4, hipMemcpyHostToDevice);
+        for (hipEvent_t &ev : p->ev)
+            if (e == hipSuccess) e = hipEventCreate(&ev);
+        if (e != hipSuccess) { free_plan(p); return hip_fail(e, "preparing the shifted stack"); }
+        *out = p;
+        return CUBO_HIP_OK;
+    }
     cubo::SplitWeights sw = cubo::split_weights(masks, k, c->ndet);
     p->nchunks = sw.nchunks;
     p->any_lo = sw.tile_lo[0];
@@ -256,8 +272,64 @@ int cubo_hip_shift_plan_create(cubo_hip_cube *c, const float *masks, int32_t k, 
 
 namespace {
 
+// The vector-ALU forward pass (any cube): the shifts of MaskShift.rounded / .bilinear per position, then shift_valu.
+int forward_valu(cubo_hip_shift_plan *p, const double *orow, const double *ocol, double *y, bool rounded) {
+    cubo_hip_cube *c = p->cube;
+    const int64_t npos = c->npos;
+    const int qy = (int)c->shape[2], qx = (int)c->shape[3];
+    for (int64_t i = 0; i < npos; ++i)
+        if (!std::isfinite(orow[i]) || !std::isfinite(ocol[i]))
+            return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the origin map must be finite (position %lld)", (long long)i);
+    const double mr = pairwise_sum(orow, npos) / (double)npos, mc = pairwise_sum(ocol, npos) / (double)npos;
+    std::vector<int32_t> sh(2 * npos);
+    std::vector<double> wr(npos), wc(npos);
+    for (int64_t i = 0; i < npos; ++i) {
+        const double dr = orow[i] - mr, dc = ocol[i] - mc;
+        const double fr = rounded ? std::nearbyint(dr) : std::floor(dr), fc = rounded ? std::nearbyint(dc) : std::floor(dc);
+        wr[i] = rounded ? 0.0 : dr - fr;
+        wc[i] = rounded ? 0.0 : dc - fc;
+        sh[2 * i] = wrap_shift(fr, qy);
+        sh[2 * i + 1] = wrap_shift(fc, qx);
+    }
+    DeviceGuard g(c->device);
+    hipStream_t s = c->stream;
+    const size_t ny = (size_t)npos * p->k;
+    hipError_t e = hipSuccess;
+    if (!p->vshift) e = hipMalloc((void **)&p->vshift, (size_t)npos * 8);
+    if (e == hipSuccess && !p->vwr) e = hipMalloc((void **)&p->vwr, (size_t)npos * 8);
+    if (e == hipSuccess && !p->vwc) e = hipMalloc((void **)&p->vwc, (size_t)npos * 8);
+    if (e == hipSuccess && !p->y) e = hipMalloc((void **)&p->y, ny * 8);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->vshift, sh.data(), (size_t)npos * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->vwr, wr.data(), (size_t)npos * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->vwc, wc.data(), (size_t)npos * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventRecord(p->ev[0], s);
+    if (e == hipSuccess) {
+        cubo::ShiftValu a{};
+        a.cube = c->data;
+        a.u16 = c->dtype == CUBO_HIP_U16;
+        a.npos = npos;
+        a.qy = qy;
+        a.qx = qx;
+        a.k = p->k;
+        a.masks = p->vmasks;
+        a.shift = p->vshift;
+        a.wr = p->vwr;
+        a.wc = p->vwc;
+        a.y = p->y;
+        e = cubo::launch_shift_valu(a, rounded, s);
+    }
+    if (e == hipSuccess) e = hipEventRecord(p->ev[1], s);
+    if (e == hipSuccess) e = hipMemcpyAsync(y, p->y, ny * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);    // the host vectors go out of scope
+    else (void)hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "the shifted stack");
+    p->ms[0] += elapsed(p->ev[0], p->ev[1]);
+    return CUBO_HIP_OK;
+}
+
 int forward(cubo_hip_shift_plan *p, const double *origin_row, const double *origin_col, double *y, bool rounded) {
     if (!p || !origin_row || !origin_col || !y) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (p->valu) return forward_valu(p, origin_row, origin_col, y, rounded);
     if (int st = prepare(p, origin_row, origin_col, rounded)) return st;
     cubo_hip_cube *c = p->cube;
     DeviceGuard g(c->device);
@@ -290,6 +362,9 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
     if (!p || !origin_row || !origin_col || !dy) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     if ((d_origin_row == nullptr) != (d_origin_col == nullptr))
         return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "ask for both origin gradients or neither");
+    if (p->valu)
+        return fail(CUBO_HIP_ERR_UNSUPPORTED, "the adjoint of the shifted stack needs a uint16 cube whose pattern "
+                    "length is a multiple of 8");
     cubo_hip_cube *c = p->cube;
     const int64_t npos = c->npos, ndet = c->ndet;
     for (int64_t i = 0; i < npos * p->k; ++i)

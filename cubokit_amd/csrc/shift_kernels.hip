@@ -314,6 +314,51 @@ __global__ __launch_bounds__(256) void shift_grad_w(ShiftGradW g) {
     for (int k = 0; k < KP; ++k) g.partial[((int64_t)group * KP + k) * ndet + v] = accd[k] + (double)acc[k];
 }
 
+template <bool U16, bool ROUNDED>
+__global__ __launch_bounds__(256) void shift_valu(ShiftValu a) {
+#pragma clang fp contract(off)   // the blend, the products and TwoSum round as written
+    const int lane = threadIdx.x & 63;
+    const int64_t pos = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pos >= a.npos) return;                      // whole waves; no block barrier
+    const int qy = a.qy, qx = a.qx;
+    const int64_t ndet = (int64_t)qy * qx;
+    const int ir = a.shift[2 * pos], ic = a.shift[2 * pos + 1];
+    const double wr = a.wr[pos], wc = a.wc[pos];
+    const double w00 = (1 - wr) * (1 - wc), w10 = wr * (1 - wc), w01 = (1 - wr) * wc, w11 = wr * wc;
+    for (int k = 0; k < a.k; ++k) {
+        const float *m = a.masks + (int64_t)k * ndet;
+        double s = 0.0, c = 0.0;
+        for (int64_t u = lane; u < ndet; u += 64) {
+            const int i = (int)(u / qx), j = (int)(u % qx);
+            // roll(mask, dr, dc)[i, j] = mask[(i - dr) mod qy, (j - dc) mod qx]
+            int i0 = i - ir; i0 += i0 < 0 ? qy : 0;
+            int j0 = j - ic; j0 += j0 < 0 ? qx : 0;
+            double mv;
+            if constexpr (ROUNDED) {
+                mv = (double)m[(int64_t)i0 * qx + j0];
+            } else {
+                const int i1 = i0 == 0 ? qy - 1 : i0 - 1, j1 = j0 == 0 ? qx - 1 : j0 - 1;
+                mv = (double)m[(int64_t)i0 * qx + j0] * w00 + (double)m[(int64_t)i1 * qx + j0] * w10 +
+                     (double)m[(int64_t)i0 * qx + j1] * w01 + (double)m[(int64_t)i1 * qx + j1] * w11;
+            }
+            double x;
+            if constexpr (U16) x = (double)((const uint16_t *)a.cube)[pos * ndet + u];
+            else x = (double)((const float *)a.cube)[pos * ndet + u];
+            const double v = x * mv;
+            const double t = s + v, bb = t - s;
+            c += (s - (t - bb)) + (v - bb);
+            s = t;
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double os = __shfl_xor(s, off, 64), oc = __shfl_xor(c, off, 64);
+            const double t = s + os, bb = t - s;
+            c += ((s - (t - bb)) + (os - bb)) + oc;
+            s = t;
+        }
+        if (lane == 0) a.y[pos * a.k + k] = __builtin_isfinite(s) ? s + c : s;
+    }
+}
+
 __global__ void sum_groups(const double *partial, int ngroups, int64_t n, double *out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -333,6 +378,19 @@ hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, b
         hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false, 1>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
     else
         hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false, 4>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_shift_valu(const ShiftValu &a, bool rounded, hipStream_t s) {
+    if (a.npos <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((a.npos + 3) / 4));
+    if (a.u16) {
+        if (rounded) hipLaunchKernelGGL((shift_valu<true, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((shift_valu<true, false>), grid, dim3(256), 0, s, a);
+    } else {
+        if (rounded) hipLaunchKernelGGL((shift_valu<false, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((shift_valu<false, false>), grid, dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -44,6 +44,22 @@ constexpr int kShiftWaves = 8;
 // rounded: MaskShift.rounded, one roll per class (tiles use slot[0] only); the forward pass only.
 hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, bool rounded, hipStream_t s);
 
+// Any cube (uint16 or Float32, any detector shape): each position's masks moved by the bilinear or the rounded
+// shift and dotted with the pattern in Double, as MaskShift's CPU loop does (ShiftedImaging.swift:55-71): the blend
+// a*w00 + b*w10 + c*w01 + d*w11 of the four rolls, Double(x) * m, TwoSum per lane. A wave per position, the masks
+// one after another (the pattern is re-read from the cache). The forward pass only.
+struct ShiftValu {
+    const void *cube;
+    bool u16;
+    int64_t npos;
+    int32_t qy, qx, k;
+    const float *masks;       // [k][qy * qx]
+    const int32_t *shift;     // [npos][2] (ir, ic) in [0, qy) x [0, qx)
+    const double *wr, *wc;    // [npos] fractions (0 for the rounded shift)
+    double *y;                // [npos][k]
+};
+hipError_t launch_shift_valu(const ShiftValu &a, bool rounded, hipStream_t s);
+
 // out[slot] = base rolled by shift[slot] = (tr, tc): the entry of pixel u holds the base weight of pixel
 // ((ui - tr) mod qy, (uj - tc) mod qx). base is split_weights' one-tile layout.
 hipError_t launch_roll_weights(const void *base, void *out, const int32_t *shifts, int nslots, int nchunks, int64_t ndet,

@@ -1,6 +1,6 @@
 """The device path against CuboKit's origin-fit pins (origin-fit-pins.json, OriginFitSupport.swift:31-96):
-centred and calibrated images (the fused pass), rounded- and bilinear-shifted images (the shifted stack on the
-matrix cores, uint16 cubes) and the rounded-shift mean and max patterns (the pattern kernels), on the pinned
+centred and calibrated images (the fused pass), rounded- and bilinear-shifted images (the shifted stack: on the
+matrix cores for uint16 cubes, on the vector ALUs in Double for Float32 cubes) and the rounded-shift mean and max patterns (the pattern kernels), on the pinned
 float64 origin maps.
 
 Bars as the reference's: 1e-10 relative against the float64 definition; py4DSTEM's own values bit for bit on the
@@ -39,18 +39,15 @@ def test_images_and_patterns_on_the_pinned_origins():
             cal = C.image_centered(res, _geo(im["calibrated_geometry"]), pixel_size=im["pixel_size"]).values
             assert_relative(centred, doubles(im["centered"]), 1e-10, f"{name} centred")
             assert_relative(cal, doubles(im["calibrated"]), 1e-10, f"{name} calibrated")
+            # uint16 cubes: the matrix-core shifted stack; Float32 cubes: the vector-ALU pass in Double
+            rounded = C.image_centered(res, geo, centered_on=centre, shift=C.MaskShift.rounded(omap)).values
+            bilinear = C.image_centered(res, geo, centered_on=centre, shift=C.MaskShift.bilinear(omap)).values
+            assert_relative(rounded, doubles(im["rounded"]), 1e-10, f"{name} rounded")
+            assert_relative(bilinear, doubles(im["bilinear"]), 1e-10, f"{name} bilinear")
             if e["cube"] in INTEGER:
                 assert_identical(centred, doubles(im["centered_py4dstem"]), f"{name} centred vs py4DSTEM")
                 assert_identical(cal, doubles(im["calibrated_py4dstem"]), f"{name} calibrated vs py4DSTEM")
-                rounded = C.image_centered(res, geo, centered_on=centre, shift=C.MaskShift.rounded(omap)).values
-                bilinear = C.image_centered(res, geo, centered_on=centre, shift=C.MaskShift.bilinear(omap)).values
                 assert_identical(rounded, doubles(im["rounded_py4dstem"]), f"{name} rounded vs py4DSTEM")
-                assert_relative(rounded, doubles(im["rounded"]), 1e-10, f"{name} rounded")
-                assert_relative(bilinear, doubles(im["bilinear"]), 1e-10, f"{name} bilinear")
-            else:
-                # the shifted stack reads uint16 cubes; a Float32 cube is refused, never approximated
-                with pytest.raises(C.CuboVirtualError):
-                    C.image_centered(res, geo, centered_on=centre, shift=C.MaskShift.rounded(omap))
             n += 1
         pm = e["patterns"]["rounded"]
         mean, mx = C.VirtualDiffraction.mean_and_max_shifted(res, C.PatternShift.rounded(omap))
