@@ -175,8 +175,8 @@ def diff_section(C):
 
 def pattern_section(C, cube):
     """Selected-area patterns on the bandwidth cube: a disk of half the scan positions (exact integer kernel), every
-    position weighted (FP64 kernel) and the disk moved by the rounded centre of mass. GB/s over the bytes of the
-    patterns each pass reads."""
+    position weighted (FP64 kernel), the disk moved by the rounded centre of mass, and by the Fourier shift (FP64
+    FFTs of every pattern). GB/s over the bytes of the patterns each pass reads."""
     s = cube.shape
     npos = s.scan_count
     r, c = np.divmod(np.arange(npos), s.scan_columns)
@@ -188,7 +188,9 @@ def pattern_section(C, cube):
     cases = (("mask_mean", C.DiffractionStatistic.mean, C.ScanSelection.mask(disk), None, int(disk.sum())),
              ("mask_max", C.DiffractionStatistic.max, C.ScanSelection.mask(disk), None, int(disk.sum())),
              ("weights_mean", C.DiffractionStatistic.mean, C.ScanSelection.weights(weights), None, npos),
-             ("shifted_mask_mean", C.DiffractionStatistic.mean, C.ScanSelection.mask(disk), shift, int(disk.sum())))
+             ("shifted_mask_mean", C.DiffractionStatistic.mean, C.ScanSelection.mask(disk), shift, int(disk.sum())),
+             ("fourier_mask_mean", C.DiffractionStatistic.mean, C.ScanSelection.mask(disk),
+              C.PatternShift.fourier_shift(C.OriginMap(com.row.values, com.column.values)), int(disk.sum())))
     out = {"selected_positions": int(disk.sum())}
     for name, stat, scan, sh, n in cases:
         one = C.ScanTiling.rows(s.scan_rows)                            # the whole scan as one tile: one launch

@@ -88,7 +88,12 @@ TIES_LATER = 1
 
 class PatternRequest(ct.Structure):
     _fields_ = [("statistic", ct.c_int32), ("take", ct.c_void_p), ("weights", ct.c_void_p),
-                ("origin_row", ct.c_void_p), ("origin_col", ct.c_void_p), ("max_ties", ct.c_int32)]
+                ("origin_row", ct.c_void_p), ("origin_col", ct.c_void_p), ("max_ties", ct.c_int32),
+                ("shift_kind", ct.c_int32)]
+
+
+SHIFT_ROUNDED = 0
+SHIFT_FOURIER = 1
 
 
 class SynthParams(ct.Structure):

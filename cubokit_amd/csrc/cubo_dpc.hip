@@ -65,15 +65,6 @@ void fftfreq(int n, double d, double *out) {
     for (int k = 0; k < n; ++k) out[k] = (double)(k < positive ? k : k - n) * scale;
 }
 
-std::vector<double2> twiddles(int n) {
-    std::vector<double2> t(n);
-    for (int m = 0; m < n; ++m) {
-        const double a = -2.0 * M_PI * (double)m / (double)n;
-        t[m] = make_double2(std::cos(a), std::sin(a));
-    }
-    return t;
-}
-
 cubo::DpcGrid grid(const cubo_hip_dpc *d) {
     cubo::DpcGrid g{};
     g.R = d->R; g.C = d->C; g.Rp = d->Rp; g.Cp = d->Cp;
@@ -175,7 +166,7 @@ int cubo_hip_dpc_create(int device, int32_t rows, int32_t cols, double padding, 
             gx[(int64_t)r * Cp + c] = -0.25 * kx[r] * kden;
             gy[(int64_t)r * Cp + c] = -0.25 * ky[c] * kden;
         }
-    const std::vector<double2> twr = twiddles(Rp), twc = twiddles(Cp);
+    const std::vector<double2> twr = cubo::fft_twiddles(Rp), twc = cubo::fft_twiddles(Cp);
     hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     struct { void **p; size_t bytes; } a[] = {
         {(void **)&d->gx, n * 8}, {(void **)&d->gy, n * 8}, {(void **)&d->tw_r, (size_t)Rp * 16}, {(void **)&d->tw_c, (size_t)Cp * 16},

@@ -19,6 +19,7 @@
 #include "cube.hpp"
 #include "errors.hpp"
 #include "host_math.hpp"
+#include "dpc_kernels.hpp"
 #include "pattern_kernels.hpp"
 
 using cubo::DeviceGuard;
@@ -42,6 +43,11 @@ struct cubo_hip_pattern_session {
     void *part_sum = nullptr, *part_max = nullptr;
     void *run_sum = nullptr, *run_max = nullptr;
     double *out_host = nullptr, *out_dev = nullptr;   // mapped pinned: the pattern
+    // PatternShift.fourier: per-position shifts, fftfreq tables, twiddles and the batch's complex patterns
+    bool fourier = false;
+    int batch_max = 0;
+    double *sr = nullptr, *sc = nullptr, *fr = nullptr, *fc = nullptr;
+    double2 *tw_r = nullptr, *tw_c = nullptr, *fdata = nullptr, *ftmp = nullptr;
 };
 
 namespace {
@@ -50,7 +56,8 @@ void free_pattern(cubo_hip_pattern_session *p) {
     if (!p) return;
     DeviceGuard g(p->cube->device);
     (void)hipStreamSynchronize(p->cube->stream);
-    void *bufs[] = {p->list, p->w, p->shift, p->part_sum, p->part_max, p->run_sum, p->run_max};
+    void *bufs[] = {p->list, p->w, p->shift, p->part_sum, p->part_max, p->run_sum, p->run_max, p->sr, p->sc, p->fr,
+                    p->fc, p->tw_r, p->tw_c, p->fdata, p->ftmp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->out_host) (void)hipHostFree(p->out_host);
@@ -91,6 +98,11 @@ int check_request(const cubo_hip_cube *c, const cubo_hip_pattern_request *q) {
         return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "a shift needs both origin maps");
     if (q->max_ties != CUBO_HIP_TIES_FIRST && q->max_ties != CUBO_HIP_TIES_LATER)
         return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "max_ties %d is neither first nor later", q->max_ties);
+    if (q->shift_kind != CUBO_HIP_SHIFT_ROUNDED && q->shift_kind != CUBO_HIP_SHIFT_FOURIER)
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "shift_kind %d is neither rounded nor Fourier", q->shift_kind);
+    if (q->origin_row && q->shift_kind == CUBO_HIP_SHIFT_FOURIER &&
+        (c->shape[2] > cubo::kFftMax || c->shape[3] > cubo::kFftMax))
+        return fail(CUBO_HIP_ERR_UNSUPPORTED, "the Fourier shift takes detectors up to %d pixels a side", cubo::kFftMax);
     if (c->dtype != CUBO_HIP_U16 && c->dtype != CUBO_HIP_F32)
         return fail(CUBO_HIP_ERR_UNSUPPORTED, "the cube's element type is neither uint16 nor float");
     if (c->npos > INT32_MAX) return fail(CUBO_HIP_ERR_UNSUPPORTED, "more than 2^31 - 1 scan positions");
@@ -128,8 +140,9 @@ int cubo_hip_pattern_begin(cubo_hip_cube *c, const cubo_hip_pattern_request *q, 
     if (!p) return fail(CUBO_HIP_ERR_OUT_OF_MEMORY, "host allocation");
     p->cube = c;
     const bool u16 = c->dtype == CUBO_HIP_U16, shifted = q->origin_row != nullptr;
+    p->fourier = shifted && q->shift_kind == CUBO_HIP_SHIFT_FOURIER;
     p->kind.u16 = u16;
-    p->kind.integer = u16 && !q->weights;
+    p->kind.integer = u16 && !q->weights && !p->fourier;
     p->kind.stat = q->statistic == CUBO_HIP_PATTERN_MEAN ? cubo::kPatternMean : cubo::kPatternMax;
     p->kind.vec = cubo::pattern_vec(u16, ndet, qx, shifted);
     p->kind.shift = shifted;
@@ -157,9 +170,26 @@ int cubo_hip_pattern_begin(cubo_hip_cube *c, const cubo_hip_pattern_request *q, 
         free_pattern(p);
         return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the weights sum to zero");
     }
-    // the rolls: the map's mean minus the origin, rounded half to even and wrapped (ShiftedImaging.swift:143-156)
+    // the rolls: the map's mean minus the origin, rounded half to even and wrapped (ShiftedImaging.swift:143-156);
+    // the Fourier shift keeps the unrounded shift and NumPy's fftfreq (ShiftedImaging.swift:169-172)
     std::vector<uint32_t> rolls;
-    if (shifted) {
+    std::vector<double> vsr, vsc, vfr, vfc;
+    if (p->fourier) {
+        const double mr = cubo::pairwise_sum(q->origin_row, npos) / (double)npos;
+        const double mc = cubo::pairwise_sum(q->origin_col, npos) / (double)npos;
+        vsr.resize(npos);
+        vsc.resize(npos);
+        for (int64_t i = 0; i < npos; ++i) { vsr[i] = mr - q->origin_row[i]; vsc[i] = mc - q->origin_col[i]; }
+        auto fftfreq = [](int n) {
+            std::vector<double> f(n);
+            const double scale = 1.0 / (double)n;
+            const int positive = (n - 1) / 2 + 1;
+            for (int k = 0; k < n; ++k) f[k] = (double)(k < positive ? k : k - n) * scale;
+            return f;
+        };
+        vfr = fftfreq(qy);
+        vfc = fftfreq(qx);
+    } else if (shifted) {
         const double mr = cubo::pairwise_sum(q->origin_row, npos) / (double)npos;
         const double mc = cubo::pairwise_sum(q->origin_col, npos) / (double)npos;
         rolls.resize(npos);
@@ -185,7 +215,22 @@ int cubo_hip_pattern_begin(cubo_hip_cube *c, const cubo_hip_pattern_request *q, 
     hipError_t e = hipSuccess;
     if (p->masked) e = hipMalloc((void **)&p->list, list.size() * 4);
     if (e == hipSuccess && q->weights) e = hipMalloc((void **)&p->w, npos * 8);
-    if (e == hipSuccess && shifted) e = hipMalloc((void **)&p->shift, npos * 4);
+    if (e == hipSuccess && shifted && !p->fourier) e = hipMalloc((void **)&p->shift, npos * 4);
+    std::vector<double2> twr, twc;
+    if (p->fourier) {
+        // batches of at most 256 MiB of complex patterns (twice: the FFT's transpose buffer)
+        p->batch_max = (int)std::max<int64_t>(1, std::min<int64_t>(4096, ((int64_t)256 << 20) / (ndet * 16)));
+        twr = cubo::fft_twiddles(qy);
+        twc = cubo::fft_twiddles(qx);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->sr, npos * 8);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->sc, npos * 8);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->fr, qy * 8);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->fc, qx * 8);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->tw_r, qy * 16);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->tw_c, qx * 16);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->fdata, (size_t)p->batch_max * ndet * 16);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->ftmp, (size_t)p->batch_max * ndet * 16);
+    }
     if (e == hipSuccess && mean) e = hipMalloc(&p->part_sum, (size_t)p->max_groups * ndet * sum_b);
     if (e == hipSuccess && !mean) e = hipMalloc(&p->part_max, (size_t)p->max_groups * ndet * max_b);
     if (e == hipSuccess && mean) e = hipMalloc(&p->run_sum, ndet * run_sb);
@@ -195,7 +240,14 @@ int cubo_hip_pattern_begin(cubo_hip_cube *c, const cubo_hip_pattern_request *q, 
     if (e != hipSuccess) { free_pattern(p); return hip_fail(e, "allocating the pattern buffers"); }
     if (p->masked) e = hipMemcpyAsync(p->list, list.data(), list.size() * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess && q->weights) e = hipMemcpyAsync(p->w, q->weights, npos * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && shifted) e = hipMemcpyAsync(p->shift, rolls.data(), npos * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && shifted && !p->fourier)
+        e = hipMemcpyAsync(p->shift, rolls.data(), npos * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->sr, vsr.data(), npos * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->sc, vsc.data(), npos * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->fr, vfr.data(), qy * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->fc, vfc.data(), qx * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->tw_r, twr.data(), qy * 16, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->tw_c, twc.data(), qx * 16, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);   // the host vectors go out of scope
     if (e != hipSuccess) { free_pattern(p); return hip_fail(e, "uploading the selection"); }
     *out = p;
@@ -216,6 +268,57 @@ int cubo_hip_pattern_rows(cubo_hip_pattern_session *p, int64_t row0, int64_t nro
     const int64_t count = (p->masked ? p->prefix[pos1] : pos1) - first;
     const bool finish = row0 + nrows == c->shape[0];
     hipStream_t s = c->stream;
+    if (p->fourier) {
+        const void *tile = nullptr;
+        if (int st = cubo::rows_data(c, row0, nrows, &tile)) return st;
+        const void *base = (const void *)((uintptr_t)tile - (uintptr_t)pos0 * (uintptr_t)(c->ndet * cubo::element_size(c->dtype)));
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
+        if (hipEventCreate(&t1) != hipSuccess) { (void)hipEventDestroy(t0); return fail(CUBO_HIP_ERR_HIP, "hipEventCreate"); }
+        c->timed.emplace_back(t0, t1);
+        CUBO_HIP_TRY(hipEventRecord(t0, s), "hipEventRecord");
+        for (int64_t b0 = 0; b0 < count || (count == 0 && b0 == 0 && finish); b0 += p->batch_max) {
+            const int nb = (int)std::min<int64_t>(p->batch_max, count - b0);
+            if (nb > 0) {
+                cubo::FourierBatch fb{};
+                fb.cube = base;
+                fb.u16 = c->dtype == CUBO_HIP_U16;
+                fb.ndet = c->ndet;
+                fb.qy = (int)c->shape[2];
+                fb.qx = (int)c->shape[3];
+                fb.first = first + b0;
+                fb.batch = nb;
+                fb.list = p->list;
+                fb.w = p->w;
+                fb.sr = p->sr;
+                fb.sc = p->sc;
+                fb.fr = p->fr;
+                fb.fc = p->fc;
+                fb.data = p->fdata;
+                fb.part_sum = (double2 *)p->part_sum;
+                fb.part_max = (double *)p->part_max;
+                fb.stat = p->kind.stat;
+                fb.first_wins = p->kind.first_wins;
+                CUBO_HIP_TRY(cubo::launch_fourier_expand(fb, s), "expanding the patterns");
+                CUBO_HIP_TRY(cubo::launch_fft2_batched(p->fdata, p->ftmp, nb, fb.qy, fb.qx, p->tw_r, p->tw_c, false, s),
+                             "the forward FFT");
+                CUBO_HIP_TRY(cubo::launch_fourier_phase(fb, s), "the phase ramp");
+                CUBO_HIP_TRY(cubo::launch_fft2_batched(p->fdata, p->ftmp, nb, fb.qy, fb.qx, p->tw_r, p->tw_c, true, s),
+                             "the inverse FFT");
+                CUBO_HIP_TRY(cubo::launch_fourier_reduce(fb, s), "reducing the batch");
+            }
+            const bool last = b0 + p->batch_max >= count;
+            CUBO_HIP_TRY(cubo::launch_pattern_accumulate(p->kind, p->part_sum, p->part_max, nb > 0 ? 1 : 0, c->ndet,
+                                                         p->run_sum, p->run_max, !p->started, finish && last, p->count,
+                                                         p->out_dev, s),
+                         "launching the pattern totals");
+            p->started = true;
+            if (count == 0) break;
+        }
+        CUBO_HIP_TRY(hipEventRecord(t1, s), "hipEventRecord");
+        p->next_row = row0 + nrows;
+        return CUBO_HIP_OK;
+    }
     int ngroups = 0;
     // The kernel addresses position p at base + p * pattern bytes; for a streamed cube the base is the ring slot
     // moved back by the tile's first position (only positions of this tile are read).

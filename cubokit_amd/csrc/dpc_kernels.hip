@@ -56,10 +56,12 @@ __global__ __launch_bounds__(256) void fft_rows(double2 *data, int n, const doub
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = a[i];
 }
 
-// out[c][r] = scale * in[r][c]
+// out[c][r] = scale * in[r][c], for each of gridDim.z arrays of rows x cols
 __global__ __launch_bounds__(256) void transpose(const double2 *in, double2 *out, int rows, int cols, double scale) {
     __shared__ double2 t[32][33];
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    in += (int64_t)blockIdx.z * rows * cols;
+    out += (int64_t)blockIdx.z * rows * cols;
     for (int i = ty; i < 32; i += 8) {
         const int r = by + i, c = bx + tx;
         if (r < rows && c < cols) t[i][tx] = in[(int64_t)r * cols + c];
@@ -222,18 +224,27 @@ unsigned blocks256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
+hipError_t launch_fft2_batched(double2 *data, double2 *tmp, int batch, int rows, int cols, const double2 *tw_r,
+                               const double2 *tw_c, bool inverse, hipStream_t s) {
+    if (rows > kFftMax || cols > kFftMax || rows < 1 || cols < 1 || batch < 1 || batch > 65535)
+        return hipErrorInvalidValue;
+    const size_t lds_c = 2 * (size_t)cols * sizeof(double2), lds_r = 2 * (size_t)rows * sizeof(double2);
+    const unsigned nr = (unsigned)((int64_t)batch * rows), nc = (unsigned)((int64_t)batch * cols);
+    if (inverse) hipLaunchKernelGGL(fft_rows<true>, dim3(nr), dim3(256), lds_c, s, data, cols, tw_c);
+    else hipLaunchKernelGGL(fft_rows<false>, dim3(nr), dim3(256), lds_c, s, data, cols, tw_c);
+    hipLaunchKernelGGL(transpose, dim3((cols + 31) / 32, (rows + 31) / 32, batch), dim3(256), 0, s, data, tmp, rows, cols,
+                       1.0);
+    if (inverse) hipLaunchKernelGGL(fft_rows<true>, dim3(nc), dim3(256), lds_r, s, tmp, rows, tw_r);
+    else hipLaunchKernelGGL(fft_rows<false>, dim3(nc), dim3(256), lds_r, s, tmp, rows, tw_r);
+    const double scale = inverse ? 1.0 / ((double)rows * cols) : 1.0;
+    hipLaunchKernelGGL(transpose, dim3((rows + 31) / 32, (cols + 31) / 32, batch), dim3(256), 0, s, tmp, data, cols, rows,
+                       scale);
+    return hipGetLastError();
+}
+
 hipError_t launch_fft2(double2 *data, double2 *tmp, int rows, int cols, const double2 *tw_r, const double2 *tw_c,
                        bool inverse, hipStream_t s) {
-    if (rows > kFftMax || cols > kFftMax || rows < 1 || cols < 1) return hipErrorInvalidValue;
-    const size_t lds_c = 2 * (size_t)cols * sizeof(double2), lds_r = 2 * (size_t)rows * sizeof(double2);
-    if (inverse) hipLaunchKernelGGL(fft_rows<true>, dim3(rows), dim3(256), lds_c, s, data, cols, tw_c);
-    else hipLaunchKernelGGL(fft_rows<false>, dim3(rows), dim3(256), lds_c, s, data, cols, tw_c);
-    hipLaunchKernelGGL(transpose, dim3((cols + 31) / 32, (rows + 31) / 32), dim3(256), 0, s, data, tmp, rows, cols, 1.0);
-    if (inverse) hipLaunchKernelGGL(fft_rows<true>, dim3(cols), dim3(256), lds_r, s, tmp, rows, tw_r);
-    else hipLaunchKernelGGL(fft_rows<false>, dim3(cols), dim3(256), lds_r, s, tmp, rows, tw_r);
-    const double scale = inverse ? 1.0 / ((double)rows * cols) : 1.0;
-    hipLaunchKernelGGL(transpose, dim3((rows + 31) / 32, (cols + 31) / 32), dim3(256), 0, s, tmp, data, cols, rows, scale);
-    return hipGetLastError();
+    return launch_fft2_batched(data, tmp, 1, rows, cols, tw_r, tw_c, inverse, s);
 }
 
 hipError_t launch_dpc_forward(const DpcGrid &g, const double *phi, const double *fx, const double *fy, double2 *z,

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+#include <vector>
+
 namespace cubo {
 
 // Largest FFT length per dimension (one row of complex doubles, twice, in LDS).
@@ -14,6 +17,20 @@ constexpr int kFftMax = 2048;
 // m < n of each length; tmp: rows * cols pairs.
 hipError_t launch_fft2(double2 *data, double2 *tmp, int rows, int cols, const double2 *tw_r, const double2 *tw_c,
                        bool inverse, hipStream_t s);
+
+// exp(-2 pi i m / n) for m < n: the twiddle table of the FFT kernels, in Double on the host.
+inline std::vector<double2> fft_twiddles(int n) {
+    std::vector<double2> t(n);
+    for (int m = 0; m < n; ++m) {
+        const double a = -2.0 * M_PI * (double)m / (double)n;
+        t[m] = make_double2(std::cos(a), std::sin(a));
+    }
+    return t;
+}
+
+// The same for `batch` arrays of rows x cols stored one after another (batch <= 65535).
+hipError_t launch_fft2_batched(double2 *data, double2 *tmp, int batch, int rows, int cols, const double2 *tw_r,
+                               const double2 *tw_c, bool inverse, hipStream_t s);
 
 struct DpcGrid {
     int32_t R, C, Rp, Cp;     // scan region and padded grid

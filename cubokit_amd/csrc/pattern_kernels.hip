@@ -15,6 +15,8 @@
 // VEC source pixels and picks them with the position's alignment offset, which is the same for every lane.
 #include "pattern_kernels.hpp"
 
+#include <algorithm>
+
 // TwoSum and the Double products round as written: no multiply-add contraction in this file.
 #pragma clang fp contract(off)
 
@@ -341,6 +343,53 @@ __global__ void pattern_accumulate(const void *part_sum, const void *part_max, i
     }
 }
 
+__device__ __forceinline__ int64_t entry_position(const FourierBatch &b, int64_t entry) {
+    return b.list ? (int64_t)b.list[entry] : entry;
+}
+
+template <bool U16>
+__global__ void fourier_expand(FourierBatch b) {
+    const int64_t n = (int64_t)b.batch * b.ndet, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t k = i / b.ndet, u = i % b.ndet;
+        const int64_t p = entry_position(b, b.first + k);
+        double x;
+        if constexpr (U16) x = (double)((const uint16_t *)b.cube)[p * b.ndet + u];
+        else x = (double)((const float *)b.cube)[p * b.ndet + u];
+        b.data[i] = make_double2(x, 0.0);
+    }
+}
+
+// The reference's phase factor, operation for operation (SelectedDiffraction.swift:278-281).
+__global__ void fourier_phase(FourierBatch b) {
+    const int64_t n = (int64_t)b.batch * b.ndet, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t k = i / b.ndet, u = i % b.ndet;
+        const int r = (int)(u / b.qx), c = (int)(u % b.qx);
+        const int64_t p = entry_position(b, b.first + k);
+        const double angle = -2 * M_PI * (b.sc[p] * b.fc[c] + b.sr[p] * b.fr[r]);
+        const double co = cos(angle), si = sin(angle);
+        const double2 v = b.data[i];
+        b.data[i] = make_double2(v.x * co - v.y * si, v.x * si + v.y * co);
+    }
+}
+
+// One group partial over the batch, entries in order: the real part (times the weight), TwoSum and the maximum.
+template <int STAT, bool FIRST>
+__global__ void fourier_reduce(FourierBatch b) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= b.ndet) return;
+    double s = 0.0, c = 0.0, m = -__builtin_inf();
+    for (int k = 0; k < b.batch; ++k) {
+        double v = b.data[(int64_t)k * b.ndet + u].x;
+        if (b.w) v = v * b.w[entry_position(b, b.first + k)];
+        if constexpr (STAT == kPatternMean) two_sum(s, c, v);
+        else m = max_step<FIRST, true>(m, v);
+    }
+    if constexpr (STAT == kPatternMean) b.part_sum[u] = make_double2(s, c);
+    else b.part_max[u] = m;
+}
+
 template <typename T, int VEC, bool INT, int STAT>
 hipError_t launch_t(const PatternLaunch &a, const PatternKind &k, dim3 grid, hipStream_t s) {
     if (k.shift) {
@@ -383,6 +432,28 @@ hipError_t launch_pattern(const PatternLaunch &a, const PatternKind &k, hipStrea
     const dim3 grid((unsigned)((a.ndet + bp - 1) / bp), (unsigned)a.ngroups);
     if (k.u16) return k.vec == 8 ? launch_v<uint16_t, 8>(a, k, grid, s) : launch_v<uint16_t, 1>(a, k, grid, s);
     return k.vec == 4 ? launch_v<float, 4>(a, k, grid, s) : launch_v<float, 1>(a, k, grid, s);
+}
+
+hipError_t launch_fourier_expand(const FourierBatch &b, hipStream_t s) {
+    const int64_t n = (int64_t)b.batch * b.ndet;
+    const dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 16384));
+    if (b.u16) hipLaunchKernelGGL(fourier_expand<true>, grid, dim3(256), 0, s, b);
+    else hipLaunchKernelGGL(fourier_expand<false>, grid, dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_fourier_phase(const FourierBatch &b, hipStream_t s) {
+    const int64_t n = (int64_t)b.batch * b.ndet;
+    hipLaunchKernelGGL(fourier_phase, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 16384)), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_fourier_reduce(const FourierBatch &b, hipStream_t s) {
+    const dim3 grid((unsigned)((b.ndet + 255) / 256));
+    if (b.stat == kPatternMean) hipLaunchKernelGGL((fourier_reduce<kPatternMean, true>), grid, dim3(256), 0, s, b);
+    else if (b.first_wins) hipLaunchKernelGGL((fourier_reduce<kPatternMax, true>), grid, dim3(256), 0, s, b);
+    else hipLaunchKernelGGL((fourier_reduce<kPatternMax, false>), grid, dim3(256), 0, s, b);
+    return hipGetLastError();
 }
 
 hipError_t launch_pattern_accumulate(const PatternKind &k, const void *part_sum, const void *part_max, int ngroups,

@@ -55,4 +55,30 @@ hipError_t launch_pattern_accumulate(const PatternKind &k, const void *part_sum,
                                      int64_t ndet, void *run_sum, void *run_max, bool first, bool finish, double count,
                                      double *out, hipStream_t s);
 
+// PatternShift.fourier (SelectedDiffraction.swift:273-285, ShiftedImaging.swift:145-153): each pattern moved by
+// Re(IFFT2(FFT2(p) exp(-2 pi i (sc fc + sr fr)))), (sr, sc) the map's mean minus the origin, fr, fc NumPy's fftfreq.
+// A batch of entries is expanded to complex, transformed by the batched FP64 FFT (dpc_kernels.hpp), multiplied by
+// the phase ramp, transformed back and reduced, in entry order, into one group partial of the pattern kernels'
+// FP64 layout (sum and correction, maximum), which launch_pattern_accumulate folds into the running totals.
+struct FourierBatch {
+    const void *cube;         // position p's pattern at cube + p * ndet * element size
+    bool u16;
+    int64_t ndet;
+    int32_t qy, qx;
+    int64_t first;            // the batch's first entry
+    int32_t batch;
+    const int32_t *list;      // entry -> position, or NULL (the entry is the position)
+    const double *w;          // [npos] weights, or NULL
+    const double *sr, *sc;    // [npos] the shift in pixels
+    const double *fr, *fc;    // fftfreq(qy), fftfreq(qx)
+    double2 *data;            // [batch][ndet]
+    double2 *part_sum;        // [ndet]
+    double *part_max;         // [ndet]
+    int stat;
+    bool first_wins;
+};
+hipError_t launch_fourier_expand(const FourierBatch &b, hipStream_t s);
+hipError_t launch_fourier_phase(const FourierBatch &b, hipStream_t s);
+hipError_t launch_fourier_reduce(const FourierBatch &b, hipStream_t s);
+
 }  // namespace cubo

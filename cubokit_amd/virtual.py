@@ -756,8 +756,8 @@ class VirtualDiffraction:
                 is_cancelled: IsCancelled = None) -> DetectorImage:
         """A virtual diffraction pattern over a selection of positions, optionally centre-shifted
         (SelectedDiffraction.swift:46-108). The `.hip` path runs the mean and the max over `.all`, `.mask` and
-        `.weights`, each pattern optionally moved by PatternShift.rounded; the median and the Fourier shift are
-        refused, as the reference's `.metal` path refuses them."""
+        `.weights`, each pattern optionally moved by PatternShift.rounded or .fourier; the median is refused, as the
+        reference's `.metal` path refuses it."""
         _require_hip(path)
         statistic = DiffractionStatistic(statistic)
         scan = scan if scan is not None else ScanSelection.all()
@@ -767,8 +767,6 @@ class VirtualDiffraction:
         take, weights = scan.arrays(shape)
         if statistic == DiffractionStatistic.median:
             raise CuboVirtualError("invalidInput", "the .hip path covers the mean and the max over a selection")
-        if shift is not None and shift.fourier:
-            raise CuboVirtualError("invalidInput", "the .hip path moves patterns by the rounded shift, not the Fourier one")
         if scan.kind == "all" and shift is None:
             # VirtualDiffraction.mean / max (SelectedDiffraction.swift:89-92)
             mm = VirtualDiffraction.mean_and_max(source, path, tiling, progress, is_cancelled)
@@ -778,10 +776,9 @@ class VirtualDiffraction:
     @staticmethod
     def mean_and_max_shifted(source: Source, shift: "PatternShift", tiling: ScanTiling = None,
                              progress: Progress = None, is_cancelled: IsCancelled = None):
-        """meanAndMax(of:shift:) with a rounded shift (ShiftedImaging.swift:125-167): every pattern moved by the
-        map's mean minus its origin; the maximum keeps the later of equal values. Two passes, one per statistic."""
-        if shift.fourier:
-            raise CuboVirtualError("invalidInput", "the .hip path moves patterns by the rounded shift, not the Fourier one")
+        """meanAndMax(of:shift:) (ShiftedImaging.swift:125-167): every pattern moved by the map's mean minus its
+        origin (rounded, or through the Fourier shift theorem); the maximum keeps the later of equal values. Two
+        passes, one per statistic."""
         half = None if progress is None else (lambda f: progress(0.5 * f))
         mean = _pattern(source, DiffractionStatistic.mean, None, None, shift, L.TIES_LATER, tiling, half, is_cancelled)
         rest = None if progress is None else (lambda f: progress(0.5 + 0.5 * f))
@@ -889,7 +886,8 @@ def _pattern(source: Source, statistic: DiffractionStatistic, take, weights, shi
                                weights=None if weights is None else weights.ctypes.data,
                                origin_row=None if shift is None else shift.origin.row.ctypes.data,
                                origin_col=None if shift is None else shift.origin.column.ctypes.data,
-                               max_ties=ties)
+                               max_ties=ties, shift_kind=L.SHIFT_FOURIER if shift is not None and shift.fourier
+                               else L.SHIFT_ROUNDED)
         out = np.empty(shape.detector_count)
         h = ct.c_void_p()
         _check(L.lib().cubo_hip_pattern_begin(cube.handle, ct.byref(req), ct.byref(h)), "starting the pattern")

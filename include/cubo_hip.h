@@ -203,8 +203,8 @@ int cubo_hip_cube_kernel_time(cubo_hip_cube *cube, double *total_ms, int64_t *la
  * mean and the max over a .mask or .weights selection or every position, each pattern optionally moved by
  * PatternShift.rounded (Sources/CuboVirtual/SelectedDiffraction.swift:46-179, 225-271; the Metal path
  * MetalReducers.weightedPattern, MetalReducers.swift:575-617), and VirtualDiffraction.meanAndMax(of:shift:)
- * with .rounded, one statistic per pass (ShiftedImaging.swift:125-167). The median and PatternShift.fourier
- * are not on the device path. ---- */
+ * with .rounded or .fourier, one statistic per pass (ShiftedImaging.swift:125-167). The median is not on the
+ * device path. ---- */
 enum { CUBO_HIP_PATTERN_MEAN = 0, CUBO_HIP_PATTERN_MAX = 1 };
 enum {
     CUBO_HIP_TIES_FIRST = 0,   /* of equal maxima the first in scan order is kept (np.max; pattern(of:...),
@@ -218,7 +218,11 @@ typedef struct {
     const double *origin_row;  /* PatternShift.rounded: the origin map, scanCount each, or both NULL: each */
     const double *origin_col;  /*   pattern is rolled by round_half_even(mean - origin) before it is combined */
     int32_t max_ties;          /* CUBO_HIP_TIES_FIRST or CUBO_HIP_TIES_LATER */
+    int32_t shift_kind;        /* CUBO_HIP_SHIFT_ROUNDED (PatternShift.rounded) or CUBO_HIP_SHIFT_FOURIER
+                                  (PatternShift.fourier: Re(IFFT2(FFT2(p) exp(-2 pi i (sc fc + sr fr)))) with the
+                                  unrounded shift, FP64 on the device, detectors up to 2048 pixels a side) */
 } cubo_hip_pattern_request;
+enum { CUBO_HIP_SHIFT_ROUNDED = 0, CUBO_HIP_SHIFT_FOURIER = 1 };
 /* Mean: sum of value (times weight) / divisor, the divisor being the selected count, the Neumaier sum of the
  * weights in scan order, or scanCount. Max: of value (times weight), seeded at -infinity, NaN winning.
  * Refusals (CUBO_HIP_ERR_INVALID_ARGUMENT), as SelectedDiffraction.swift:60-102 and ShiftedImaging.swift:99-106:

@@ -249,7 +249,7 @@ static inline double max_later(double m, double v) {
 }
 
 int oc_pattern(const float *cube, int64_t npos, int qy, int qx, int stat, const uint8_t *take, const double *w,
-               const double *orow, const double *ocol, int ties_later, double *out) {
+               const double *orow, const double *ocol, int ties_later, int fourier, double *out) {
     const int64_t ndet = (int64_t)qy * qx;
     if (take && w) return OC_INVALID_PARAMETER;
     if (w)
@@ -268,6 +268,11 @@ int oc_pattern(const float *cube, int64_t npos, int qy, int qx, int stat, const 
     }
     double *sum = (double *)malloc(sizeof(double) * ndet), *corr = (double *)malloc(sizeof(double) * ndet);
     double *top = (double *)malloc(sizeof(double) * ndet), *moved = (double *)malloc(sizeof(double) * ndet);
+    double *im = (double *)malloc(sizeof(double) * ndet), *fr = (double *)malloc(sizeof(double) * qy),
+           *fc = (double *)malloc(sizeof(double) * qx);
+    /* NumPy's fftfreq (ShiftedImaging.swift:169-172) */
+    for (int i = 0; i < qy; ++i) fr[i] = (double)(i < (qy - 1) / 2 + 1 ? i : i - qy) * (1.0 / (double)qy);
+    for (int j = 0; j < qx; ++j) fc[j] = (double)(j < (qx - 1) / 2 + 1 ? j : j - qx) * (1.0 / (double)qx);
     for (int64_t i = 0; i < ndet; ++i) { sum[i] = 0.0; corr[i] = 0.0; top[i] = -INFINITY; }
     neumaier total = {0, 0};
     for (int64_t k = 0; k < npos; ++k) {
@@ -275,7 +280,20 @@ int oc_pattern(const float *cube, int64_t npos, int qy, int qx, int stat, const 
         const double wk = w ? w[k] : 1.0;
         nm_add(&total, wk);
         const float *x = cube + k * ndet;
-        if (shifted) {
+        if (shifted && fourier) {
+            /* get_shifted_ar: Re(IFFT2(FFT2(p) exp(-2 pi i (sc f_c + sr f_r)))) (SelectedDiffraction.swift:273-285) */
+            const double sr = mr - orow[k], sc = mc - ocol[k];
+            for (int64_t i = 0; i < ndet; ++i) { moved[i] = (double)x[i]; im[i] = 0.0; }
+            oc_fft2d(moved, im, qy, qx, 0);
+            for (int i = 0; i < qy; ++i)
+                for (int j = 0; j < qx; ++j) {
+                    const double angle = -2 * M_PI * (sc * fc[j] + sr * fr[i]);
+                    const double c = cos(angle), s = sin(angle), a = moved[(int64_t)i * qx + j], b = im[(int64_t)i * qx + j];
+                    moved[(int64_t)i * qx + j] = a * c - b * s;
+                    im[(int64_t)i * qx + j] = a * s + b * c;
+                }
+            oc_fft2d(moved, im, qy, qx, 1);
+        } else if (shifted) {
             /* np.roll: out[(i + sr) % qy, (j + sc) % qx] = p[i, j] */
             const int sr = wrap_shift(nearbyint(mr - orow[k]), qy), sc = wrap_shift(nearbyint(mc - ocol[k]), qx);
             for (int i = 0; i < qy; ++i)
@@ -311,7 +329,7 @@ int oc_pattern(const float *cube, int64_t npos, int qy, int qx, int stat, const 
     } else {
         for (int64_t i = 0; i < ndet; ++i) out[i] = top[i];
     }
-    free(sum); free(corr); free(top); free(moved);
+    free(sum); free(corr); free(top); free(moved); free(im); free(fr); free(fc);
     return st;
 }
 

@@ -68,11 +68,11 @@ int oc_vi_bilinear_grad_origin(const float *cube, int64_t npos, int qy, int qx, 
                                double *d_origin_r, double *d_origin_c);
 
 /* Selected-area and centre-shifted patterns: stat 0 = mean, 1 = max; take (selection flags) or w (weights) or
- * neither; orow/ocol an origin map for PatternShift.rounded or NULL; ties_later: Swift.max's tie rule
+ * neither; orow/ocol an origin map for PatternShift.rounded (fourier = 0) or .fourier (1), or NULL; ties_later: Swift.max's tie rule
  * (meanAndMax(of:shift:)) instead of the first-kept one. OC_INVALID_PARAMETER for an empty selection or take with
  * w, OC_NON_FINITE for non-finite weights or origins, OC_ZERO_FIELD for a mean over weights summing to zero. */
 int oc_pattern(const float *cube, int64_t npos, int qy, int qx, int stat, const uint8_t *take, const double *w,
-               const double *orow, const double *ocol, int ties_later, double *out);
+               const double *orow, const double *ocol, int ties_later, int fourier, double *out);
 
 /* Origins: py4DSTEM's get_origin (Double), ProbeSize.estimate (out: radius, centre row, centre column, threshold)
  * and the iterated-window estimator (Float32). Gaussian weights return R (the kernel is 2R + 1 long), or -1. */

@@ -48,7 +48,7 @@ def _load():
         "oc_vi_bilinear_grad_w": (None, [_fp, _i64, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]),
         "oc_vi_bilinear_grad_origin": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
         "oc_pattern": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
-                                 ct.c_void_p, ct.c_int, _dp]),
+                                 ct.c_void_p, ct.c_int, ct.c_int, _dp]),
         "oc_gaussian_weights": (ct.c_int, [ct.c_double, _dp, ct.c_int]),
         "oc_gaussian_filter": (None, [_dp, ct.c_int, ct.c_int, _dp, ct.c_int, _dp]),
         "oc_origin_py4dstem": (ct.c_int, [_fp, _i64, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_void_p,
@@ -150,7 +150,8 @@ def mean_max(cube):
     return mean, mx
 
 
-def pattern(cube, statistic: str, take=None, weights=None, origin=None, ties: str = "first") -> np.ndarray:
+def pattern(cube, statistic: str, take=None, weights=None, origin=None, ties: str = "first",
+            fourier: bool = False) -> np.ndarray:
     """VirtualDiffraction.pattern / meanAndMax(of:shift:) for 'mean' or 'max' (SelectedDiffraction.swift:110-179,
     225-271; ShiftedImaging.swift:125-167). take: flags per position; weights: Doubles per position; origin:
     (row map, column map) for PatternShift.rounded; ties 'first' (np.max) or 'later' (Swift.max)."""
@@ -173,7 +174,7 @@ def pattern(cube, statistic: str, take=None, weights=None, origin=None, ties: st
         orow, ocol = ptr(origin[0], np.float64), ptr(origin[1], np.float64)
     out = np.zeros(qy * qx)
     st = lib().oc_pattern(x, npos, qy, qx, 0 if statistic == "mean" else 1, t, w, orow, ocol,
-                          1 if ties == "later" else 0, out)
+                          1 if ties == "later" else 0, int(fourier), out)
     _check(st, f"pattern {statistic}")
     return out
 

@@ -54,6 +54,12 @@ def test_images_and_patterns_on_the_pinned_origins():
         assert_relative(mean.values, doubles(pm["mean"]), 1e-10, f"{e['cube']} rounded mean")
         assert_relative(mean.values, doubles(pm["mean_py4dstem"]), 1e-10, f"{e['cube']} rounded mean vs py4DSTEM")
         assert_identical(mx.values, doubles(pm["max"]), f"{e['cube']} rounded max")
+        # the Fourier shift (FP64 FFTs on the device): the reference's normwise 1e-12 against the pins
+        pf = e["patterns"]["fourier"]
+        fmean, fmax = C.VirtualDiffraction.mean_and_max_shifted(res, C.PatternShift.fourier_shift(omap))
+        for got, key in ((fmean.values, "mean"), (fmax.values, "max")):
+            want = doubles(pf[key])
+            assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), f"{e['cube']} fourier {key}"
         res.release()
     assert n == 8
 

@@ -123,9 +123,6 @@ def test_refusals_and_edges():
     all_ = C.ScanSelection.mask(np.ones(16, bool))
     with pytest.raises(C.CuboVirtualError):
         C.VirtualDiffraction.pattern(res, C.DiffractionStatistic.median, all_)
-    omap = C.OriginMap(np.full(16, 3.0), np.full(16, 3.0))
-    with pytest.raises(C.CuboVirtualError):
-        C.VirtualDiffraction.pattern(res, MEAN, all_, shift=C.PatternShift.fourier_shift(omap))
     with pytest.raises(C.CuboVirtualError):
         C.VirtualDiffraction.pattern(res, MEAN, C.ScanSelection.mask([True]))
     with pytest.raises(C.CuboVirtualError):
@@ -347,3 +344,37 @@ def test_bandwidth_cube_selected_and_shifted_patterns():
     got_max = C.VirtualDiffraction.pattern(res, MAX, C.ScanSelection.mask(disk), shift=shift).values
     assert_identical(got_mean, O.pattern(host, "mean", take=disk, origin=orig), "shifted disk mean")
     assert_identical(got_max, O.pattern(host, "max", take=disk, origin=orig), "shifted disk max")
+
+
+@pytest.mark.parametrize("dtype,det", [(np.float32, (16, 24)), (np.uint16, (15, 12))])
+def test_fourier_shift_matches_the_oracle(dtype, det):
+    """PatternShift.fourier over a mask, weights and every position (first-kept maxima) and meanAndMax(of:shift:)
+    (later-kept): FP64 FFTs on both sides, normwise within 1e-12 of max|oracle|; batches smaller than the selection
+    through several tiles."""
+    rng = np.random.default_rng(23)
+    qy, qx = det
+    R, Cs = 7, 6
+    npos = R * Cs
+    if dtype == np.uint16:
+        cube = rng.integers(0, 65536, size=(R, Cs, qy, qx)).astype(np.uint16)
+    else:
+        cube = rng.normal(30, 10, size=(R, Cs, qy, qx)).astype(np.float32)
+    orow, ocol = rng.normal(7, 2.5, npos), rng.normal(9, 2.5, npos)
+    mask = rng.random(npos) < 0.6
+    weights = rng.normal(size=npos)
+    res = C.HIPResidentCube(cube)
+    shift = C.PatternShift.fourier_shift(C.OriginMap(orow, ocol))
+
+    def close(got, want, msg):
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), msg
+    for t in (C.ScanTiling.rows(2), C.ScanTiling.automatic):
+        for kind, scan, kw in (("mask", C.ScanSelection.mask(mask), {"take": mask}),
+                               ("weights", C.ScanSelection.weights(weights), {"weights": weights}),
+                               ("all", C.ScanSelection.all(), {})):
+            print(f"case: fourier {dtype.__name__} {det} {kind} {t}", flush=True)
+            for stat in (MEAN, MAX):
+                got = C.VirtualDiffraction.pattern(res, stat, scan, shift=shift, tiling=t).values
+                close(got, O.pattern(cube, stat.value, origin=(orow, ocol), fourier=True, **kw), f"{kind} {stat}")
+        mean, mx = C.VirtualDiffraction.mean_and_max_shifted(res, shift, tiling=t)
+        close(mean.values, O.pattern(cube, "mean", origin=(orow, ocol), ties="later", fourier=True), "meanAndMax mean")
+        close(mx.values, O.pattern(cube, "max", origin=(orow, ocol), ties="later", fourier=True), "meanAndMax max")

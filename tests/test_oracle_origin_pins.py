@@ -1,5 +1,6 @@
 """The oracle against CuboKit's origin-fit pins (origin-fit-pins.json, OriginFitSupport.swift:31-96): centred,
-calibrated, rounded- and bilinear-shifted virtual images and the rounded-shift mean and max patterns of four cubes,
+calibrated, rounded- and bilinear-shifted virtual images and the rounded- and Fourier-shift mean and max patterns of
+four cubes,
 on the pinned float64 origin maps. Bars as the reference's: 1e-10 relative against the float64 definition,
 py4DSTEM's own values bit for bit on the integer cubes, the rounded max bit for bit."""
 import numpy as np
@@ -46,6 +47,12 @@ def test_images_and_patterns_on_the_pinned_origins():
         mx = O.pattern(cube, "max", origin=(orow, ocol), ties="later")
         assert_relative(mean, doubles(pm["mean"]), 1e-10, f"{e['cube']} rounded mean")
         assert_identical(mx, doubles(pm["max"]), f"{e['cube']} rounded max")
+        # the Fourier shift: its error is normwise, |d| <= 1e-12 max|want| (OriginFitSupport.swift:78-82, 89-91)
+        pf = e["patterns"]["fourier"]
+        for stat in ("mean", "max"):
+            got = O.pattern(cube, stat, origin=(orow, ocol), ties="later", fourier=True)
+            want = doubles(pf[stat])
+            assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), f"{e['cube']} fourier {stat}"
     assert n == 8
 
 
