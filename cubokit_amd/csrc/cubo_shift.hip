@@ -57,6 +57,11 @@ struct cubo_hip_shift_plan {
     // per-call buffers
     double *y = nullptr, *dy = nullptr, *g = nullptr;    // g: [2][npos]
     float *dy32 = nullptr;
+    void *dyw = nullptr;             // dL/dY split into FP16 hi/lo for the matrix-core mask adjoint
+    size_t dyw_bytes = 0;
+    double *dy_inv = nullptr;
+    double *rpartial = nullptr, *rdmask = nullptr;   // the matrix-core adjoint's [groups][32][ndet] and [32][ndet]
+    int rgroups = 0;
     double *partial = nullptr, *dmask = nullptr;
     int ngroups = 0;
     hipEvent_t ev[6] = {};
@@ -71,7 +76,8 @@ void free_plan(cubo_hip_shift_plan *p) {
     DeviceGuard g(p->cube->device);
     (void)hipStreamSynchronize(p->cube->stream);
     void *bufs[] = {p->base, p->inv, p->wroll, p->perm, p->shift, p->slot_shifts, p->tiles, p->wr, p->wc, p->wts,
-                    p->y, p->dy, p->g, p->dy32, p->partial, p->dmask, p->vmasks, p->vshift, p->vwr, p->vwc};
+                    p->y, p->dy, p->g, p->dy32, p->partial, p->dmask, p->vmasks, p->vshift, p->vwr, p->vwc,
+                    p->dyw, p->dy_inv, p->rpartial, p->rdmask};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : p->ev)
@@ -385,7 +391,82 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
         CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, true, false, s), "launching the origin adjoint");
         CUBO_HIP_TRY(hipEventRecord(p->ev[3], s), "hipEventRecord");
     }
-    if (dmasks) {
+    bool dy_finite = true;
+    for (int64_t i = 0; i < npos * p->k; ++i) dy_finite = dy_finite && std::isfinite(dy[i]);
+    if (dmasks && dy_finite && cubo::shift_grad_rows_supported((int)c->shape[3])) {
+        // dL/dY split for the matrix cores (a non-finite dL/dY takes the vector-ALU path, which propagates it): each column scaled by a power of two so its largest value lies in
+        // [1024, 2048), then FP16 hi + lo, in the B-operand order [k-step][hi, lo][lane][8]: lane l holds mask
+        // l % 32 and positions 16 s + 8 (l / 32) + e. Zero past the positions and the masks.
+        const int64_t nks = (npos + 15) / 16;
+        std::vector<double> inv(32, 0.0), scale(32, 0.0);
+        for (int kk = 0; kk < p->k; ++kk) {
+            double mx = 0;
+            for (int64_t i = 0; i < npos; ++i) mx = std::max(mx, std::fabs(dy[(size_t)i * p->k + kk]));
+            if (mx > 0) {
+                const int ex = (int)std::floor(std::log2(mx));
+                scale[kk] = std::ldexp(1.0, 10 - ex);
+                inv[kk] = std::ldexp(1.0, ex - 10);
+            }
+        }
+        std::vector<_Float16> dyw((size_t)nks * 2 * 64 * 8, (_Float16)0.f);
+        for (int64_t ks = 0; ks < nks; ++ks)
+            for (int l = 0; l < 64; ++l) {
+                const int kk = l & 31;
+                if (kk >= p->k) continue;
+                for (int e = 0; e < 8; ++e) {
+                    const int64_t pos = ks * 16 + 8 * (l >> 5) + e;
+                    if (pos >= npos) continue;
+                    const double v = dy[(size_t)pos * p->k + kk] * scale[kk];
+                    const _Float16 hi = (_Float16)(float)v;
+                    const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
+                    dyw[((ks * 2 + 0) * 64 + l) * 8 + e] = hi;
+                    dyw[((ks * 2 + 1) * 64 + l) * 8 + e] = lo;
+                }
+            }
+        const int qy = (int)c->shape[2];
+        const int cus = cubo::compute_units(c->device);
+        const int ngroups = (int)std::max<int64_t>(1, std::min<int64_t>({(4 * cus + qy - 1) / qy, 64, nks}));
+        const size_t dyw_bytes = dyw.size() * sizeof(_Float16);
+        if (p->dyw_bytes < dyw_bytes) {
+            if (p->dyw) (void)hipFree(p->dyw);
+            p->dyw = nullptr;
+            p->dyw_bytes = 0;
+            CUBO_HIP_TRY(hipMalloc(&p->dyw, dyw_bytes), "allocating dL/dY");
+            p->dyw_bytes = dyw_bytes;
+        }
+        if (!p->dy_inv) CUBO_HIP_TRY(hipMalloc((void **)&p->dy_inv, 32 * 8), "allocating dL/dY scales");
+        if (ngroups > p->rgroups) {
+            if (p->rpartial) (void)hipFree(p->rpartial);
+            p->rpartial = nullptr;
+            p->rgroups = 0;
+            CUBO_HIP_TRY(hipMalloc((void **)&p->rpartial, (size_t)ngroups * 32 * ndet * 8), "allocating partials");
+            p->rgroups = ngroups;
+        }
+        if (!p->rdmask) CUBO_HIP_TRY(hipMalloc((void **)&p->rdmask, (size_t)32 * ndet * 8), "allocating dL/dmasks");
+        CUBO_HIP_TRY(hipMemcpyAsync(p->dyw, dyw.data(), dyw_bytes, hipMemcpyHostToDevice, s), "uploading dL/dY");
+        CUBO_HIP_TRY(hipMemcpyAsync(p->dy_inv, inv.data(), 32 * 8, hipMemcpyHostToDevice, s), "uploading dL/dY");
+        cubo::ShiftGradRows gw{};
+        gw.cube = (const uint16_t *)c->data;
+        gw.npos = npos;
+        gw.qy = qy;
+        gw.qx = (int32_t)c->shape[3];
+        gw.shift = p->shift;
+        gw.wr = p->wr;
+        gw.wc = p->wc;
+        gw.dyw = p->dyw;
+        gw.dy_inv = p->dy_inv;
+        gw.partial = p->rpartial;
+        gw.ngroups = ngroups;
+        gw.ppg = ((nks + ngroups - 1) / ngroups) * 16;
+        CUBO_HIP_TRY(hipEventRecord(p->ev[4], s), "hipEventRecord");
+        CUBO_HIP_TRY(cubo::launch_shift_grad_rows(gw, s), "launching the mask adjoint");
+        CUBO_HIP_TRY(cubo::launch_sum_groups(p->rpartial, ngroups, (int64_t)32 * ndet, p->rdmask, s),
+                     "launching the mask adjoint sum");
+        CUBO_HIP_TRY(hipEventRecord(p->ev[5], s), "hipEventRecord");
+        CUBO_HIP_TRY(hipMemcpyAsync(dmasks, p->rdmask, (size_t)p->k * ndet * 8, hipMemcpyDeviceToHost, s),
+                     "reading dL/dmasks");
+        CUBO_HIP_TRY(hipStreamSynchronize(s), "waiting for the mask adjoint");   // the host vectors go out of scope
+    } else if (dmasks) {
         // dL/dY in FP32, padded to the kernel's mask count
         std::vector<float> d32((size_t)npos * p->kp, 0.f);
         for (int64_t i = 0; i < npos; ++i)

@@ -314,6 +314,120 @@ __global__ __launch_bounds__(256) void shift_grad_w(ShiftGradW g) {
     for (int k = 0; k < KP; ++k) g.partial[((int64_t)group * KP + k) * ndet + v] = accd[k] + (double)acc[k];
 }
 
+template <int W>
+__global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
+    constexpr int QX = 32 * W;                      // detector columns
+    constexpr int VPR = QX / 8;                     // 16-byte vectors per row
+    constexpr int VPB = 16 * 2 * VPR;               // vectors per batch of 16 positions
+    constexpr int T = 64 * W;
+    static_assert(VPB % T == 0, "every thread stages the same number of vectors");
+    constexpr int PER = VPB / T;
+    __shared__ __align__(16) uint16_t st[2][16][2][QX];
+    __shared__ int sic[2][16];
+    __shared__ float swr[2][16], swc[2][16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, hf = lane >> 5;
+    const int qy = g.qy;
+    const int64_t ndet = (int64_t)qy * QX;
+    const int arow = blockIdx.x, group = blockIdx.y;
+    const int64_t p_begin = (int64_t)group * g.ppg, p_end = min(g.npos, p_begin + g.ppg);
+    const CUBO_GLOBAL uint16_t *cube = (const CUBO_GLOBAL uint16_t *)g.cube;
+    const CUBO_GLOBAL half8 *dyw = (const CUBO_GLOBAL half8 *)g.dyw;
+
+    u32x4 reg[PER];
+    int reg_ic = 0;
+    float reg_wr = 0.f, reg_wc = 0.f;
+    // the batch's rows into registers; positions past the cube repeat the last one (their dL/dY is zero)
+    auto load = [&](int64_t p0) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int v = tid + k * T;
+            const int j = v / (2 * VPR), rem = v % (2 * VPR), row = rem / VPR, col8 = rem % VPR;
+            const int64_t p = min(p0 + j, g.npos - 1);
+            const int ir = g.shift[2 * p];
+            int r = arow + ir + row;
+            r -= r >= qy ? qy : 0;
+            r -= r >= qy ? qy : 0;
+            reg[k] = *(const CUBO_GLOBAL u32x4 *)(cube + p * ndet + (int64_t)r * QX + col8 * 8);
+        }
+        if (tid < 16) {
+            const int64_t p = min(p0 + tid, g.npos - 1);
+            reg_ic = g.shift[2 * p + 1];
+            reg_wr = (float)g.wr[p];
+            reg_wc = (float)g.wc[p];
+        }
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int v = tid + k * T;
+            const int j = v / (2 * VPR), rem = v % (2 * VPR), row = rem / VPR, col8 = rem % VPR;
+            *(u32x4 *)&st[b][j][row][col8 * 8] = reg[k];
+        }
+        if (tid < 16) { sic[b][tid] = reg_ic; swr[b][tid] = reg_wr; swc[b][tid] = reg_wc; }
+    };
+
+    float16v acc;
+    double accd[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) { acc[r] = 0.f; accd[r] = 0.0; }
+    const int col = 32 * wave + (lane & 31);
+    int nk = 0;
+    auto compute = [&](int b, int64_t p0) {
+        const int64_t ks = p0 >> 4;
+        const half8 bh = dyw[(ks * 2 + 0) * 64 + lane], bl = dyw[(ks * 2 + 1) * 64 + lane];
+        half8 ah, al;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int j = 8 * hf + e;
+            int c0 = col + sic[b][j];
+            c0 -= c0 >= QX ? QX : 0;
+            int c1 = c0 + 1;
+            c1 -= c1 >= QX ? QX : 0;
+            const float wr = swr[b][j], wc = swc[b][j];
+            const float x00 = st[b][j][0][c0], x01 = st[b][j][0][c1], x10 = st[b][j][1][c0], x11 = st[b][j][1][c1];
+            // the bilinear blend (ShiftedImaging.swift:58-64) as two column lerps and a row lerp, halved
+            const float av = x00 + wc * (x01 - x00), bv = x10 + wc * (x11 - x10);
+            const float xs = 0.5f * (av + wr * (bv - av));
+            const _Float16 hi = (_Float16)xs;
+            ah[e] = hi;
+            al[e] = (_Float16)(xs - (float)hi);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc, 0, 0, 0);
+        if (++nk == 2) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) { accd[r] += (double)acc[r]; acc[r] = 0.f; }
+            nk = 0;
+        }
+    };
+
+    if (p_begin < p_end) {
+        load(p_begin);
+        store(0);
+        __syncthreads();
+        int b = 0;
+        for (int64_t p0 = p_begin; p0 < p_end; p0 += 16) {
+            const bool more = p0 + 16 < p_end;          // uniform
+            if (more) load(p0 + 16);
+            compute(b, p0);
+            if (more) store(b ^ 1);
+            lds_barrier();
+            b ^= 1;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) accd[r] += (double)acc[r];
+    // register r of lane l: output pixel (arow, 32 wave + 8 (r / 4) + 4 (l / 32) + r % 4), mask l % 32
+    const int mask = lane & 31;
+    const double scale = 2.0 * g.dy_inv[mask];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int64_t pix = (int64_t)arow * QX + 32 * wave + 8 * (r >> 2) + 4 * hf + (r & 3);
+        g.partial[((int64_t)group * 32 + mask) * ndet + pix] = accd[r] * scale;
+    }
+}
+
 template <bool U16, bool ROUNDED>
 __global__ __launch_bounds__(256) void shift_valu(ShiftValu a) {
 #pragma clang fp contract(off)   // the blend, the products and TwoSum round as written
@@ -378,6 +492,24 @@ hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, b
         hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false, 1>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
     else
         hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false, 4>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
+    return hipGetLastError();
+}
+
+bool shift_grad_rows_supported(int qx) { return qx % 32 == 0 && qx >= 32 && qx <= 256; }
+
+hipError_t launch_shift_grad_rows(const ShiftGradRows &a, hipStream_t s) {
+    if (!shift_grad_rows_supported(a.qx) || a.ngroups <= 0) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)a.qy, (unsigned)a.ngroups);
+    switch (a.qx / 32) {
+    case 1: hipLaunchKernelGGL(shift_grad_rows<1>, grid, dim3(64), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(shift_grad_rows<2>, grid, dim3(128), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(shift_grad_rows<3>, grid, dim3(192), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(shift_grad_rows<4>, grid, dim3(256), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(shift_grad_rows<5>, grid, dim3(320), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(shift_grad_rows<6>, grid, dim3(384), 0, s, a); break;
+    case 7: hipLaunchKernelGGL(shift_grad_rows<7>, grid, dim3(448), 0, s, a); break;
+    default: hipLaunchKernelGGL(shift_grad_rows<8>, grid, dim3(512), 0, s, a); break;
+    }
     return hipGetLastError();
 }
 

@@ -81,6 +81,27 @@ struct ShiftGradW {
     int64_t ppg;              // positions per group
 };
 hipError_t launch_shift_grad_w(const ShiftGradW &a, hipStream_t s);
+
+// The same adjoint on the matrix cores, for detectors whose rows hold a multiple of 32 pixels, at most 256: block
+// (a, g) owns output row a and position group g, one wave per 32 output pixels. For each position the two input
+// rows the bilinear shift reads ((a + ir) and (a + ir + 1) mod qy) are staged in LDS in batches of 16 positions
+// (double-buffered), each lane blends its pixel's four neighbours for 8 positions from LDS (xs / 2, split into FP16
+// hi + lo), and the 32 x 16 A operand meets dL/dY split the same way (B operand, [k-step][hi, lo][lane][8], columns
+// scaled by powers of two) in three MFMAs per k-step, flushed to FP64 every 32 positions.
+struct ShiftGradRows {
+    const uint16_t *cube;
+    int64_t npos;
+    int32_t qy, qx;
+    const int32_t *shift;     // [npos][2] (ir, ic)
+    const double *wr, *wc;    // [npos]
+    const void *dyw;          // [ceil(npos / 16)][hi, lo][lane][8] FP16, zero past the positions and the masks
+    const double *dy_inv;     // [32] 1 / column scale (0 past the masks)
+    double *partial;          // [ngroups][32][ndet]
+    int32_t ngroups;
+    int64_t ppg;              // positions per group, a multiple of 16
+};
+bool shift_grad_rows_supported(int qx);
+hipError_t launch_shift_grad_rows(const ShiftGradRows &a, hipStream_t s);
 // out[i] = sum_g partial[g][i] in group order, i < n.
 hipError_t launch_sum_groups(const double *partial, int ngroups, int64_t n, double *out, hipStream_t s);
 

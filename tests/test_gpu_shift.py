@@ -74,9 +74,11 @@ def test_equal_origins_give_the_plain_stack_bit_for_bit():
     assert np.array_equal(y, C.MaskStack(res, masks, C.MaskStack.MFMA_SPLIT).run())
 
 
-def test_backward_matches_the_oracle():
-    res = _cube((9, 11), 32, 400.0, 6)          # 99 positions in many shift classes: partial tiles
-    masks = _masks(32, 6, 7, signed=True)
+@pytest.mark.parametrize("det", [32, 24])
+def test_backward_matches_the_oracle(det):
+    """32 px rows: the matrix-core mask adjoint (rows staged in LDS); 24 px rows: the vector-ALU one."""
+    res = _cube((9, 11), det, 400.0, 6)         # 99 positions in many shift classes: partial tiles
+    masks = _masks(det, 6, 7, signed=True)
     orow, ocol = _origins(99, 8)
     rng = np.random.default_rng(9)
     dy = rng.normal(size=(99, 6))
