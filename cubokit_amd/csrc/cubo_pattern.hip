@@ -45,6 +45,7 @@ struct cubo_hip_pattern_session {
     double *out_host = nullptr, *out_dev = nullptr;   // mapped pinned: the pattern
     // PatternShift.fourier: per-position shifts, fftfreq tables, twiddles and the batch's complex patterns
     bool fourier = false;
+    bool fused = false;              // the one-kernel path (power-of-two detectors); else batches of FFTs
     int batch_max = 0;
     double *sr = nullptr, *sc = nullptr, *fr = nullptr, *fc = nullptr;
     double2 *tw_r = nullptr, *tw_c = nullptr, *fdata = nullptr, *ftmp = nullptr;
@@ -207,6 +208,10 @@ int cubo_hip_pattern_begin(cubo_hip_cube *c, const cubo_hip_pattern_request *q, 
         const int64_t blocks = (ndet + bp - 1) / bp;
         int64_t gmax = std::max<int64_t>(1, ((int64_t)8 * p->cus + blocks - 1) / blocks);
         if (p->kind.integer) gmax = std::max<int64_t>(gmax, (entries + 65535) / 65536);
+        if (p->fourier && cubo::fourier_fused_supported(qy, qx)) {
+            p->fused = true;
+            gmax = std::max<int64_t>(gmax, p->cus);
+        }
         p->max_groups = gmax;
     }
     const bool mean = p->kind.stat == cubo::kPatternMean;
@@ -217,7 +222,14 @@ int cubo_hip_pattern_begin(cubo_hip_cube *c, const cubo_hip_pattern_request *q, 
     if (e == hipSuccess && q->weights) e = hipMalloc((void **)&p->w, npos * 8);
     if (e == hipSuccess && shifted && !p->fourier) e = hipMalloc((void **)&p->shift, npos * 4);
     std::vector<double2> twr, twc;
-    if (p->fourier) {
+    if (p->fused) {
+        // one twiddle table for both axes: exp(-2 pi i m / max(qy, qx)), m < max / 2
+        twr = cubo::fft_twiddles(std::max(qy, qx));
+        twr.resize(std::max(qy, qx) / 2);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->sr, npos * 8);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->sc, npos * 8);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->tw_r, twr.size() * 16);
+    } else if (p->fourier) {
         // batches of at most 256 MiB of complex patterns (twice: the FFT's transpose buffer)
         p->batch_max = (int)std::max<int64_t>(1, std::min<int64_t>(4096, ((int64_t)256 << 20) / (ndet * 16)));
         twr = cubo::fft_twiddles(qy);
@@ -244,10 +256,12 @@ int cubo_hip_pattern_begin(cubo_hip_cube *c, const cubo_hip_pattern_request *q, 
         e = hipMemcpyAsync(p->shift, rolls.data(), npos * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->sr, vsr.data(), npos * 8, hipMemcpyHostToDevice, s);
     if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->sc, vsc.data(), npos * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->fr, vfr.data(), qy * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->fc, vfc.data(), qx * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->tw_r, twr.data(), qy * 16, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && p->fourier) e = hipMemcpyAsync(p->tw_c, twc.data(), qx * 16, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && p->fused) e = hipMemcpyAsync(p->tw_r, twr.data(), twr.size() * 16, hipMemcpyHostToDevice, s);
+    const bool batched = p->fourier && !p->fused;
+    if (e == hipSuccess && batched) e = hipMemcpyAsync(p->fr, vfr.data(), qy * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && batched) e = hipMemcpyAsync(p->fc, vfc.data(), qx * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && batched) e = hipMemcpyAsync(p->tw_r, twr.data(), qy * 16, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && batched) e = hipMemcpyAsync(p->tw_c, twc.data(), qx * 16, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);   // the host vectors go out of scope
     if (e != hipSuccess) { free_pattern(p); return hip_fail(e, "uploading the selection"); }
     *out = p;
@@ -277,7 +291,40 @@ int cubo_hip_pattern_rows(cubo_hip_pattern_session *p, int64_t row0, int64_t nro
         if (hipEventCreate(&t1) != hipSuccess) { (void)hipEventDestroy(t0); return fail(CUBO_HIP_ERR_HIP, "hipEventCreate"); }
         c->timed.emplace_back(t0, t1);
         CUBO_HIP_TRY(hipEventRecord(t0, s), "hipEventRecord");
-        for (int64_t b0 = 0; b0 < count || (count == 0 && b0 == 0 && finish); b0 += p->batch_max) {
+        if (p->fused && (count > 0 || finish)) {
+            int64_t ngroups = 0;
+            if (count > 0) {
+                ngroups = std::min<int64_t>(p->max_groups, count);
+                const int64_t ppg = (count + ngroups - 1) / ngroups;
+                ngroups = (count + ppg - 1) / ppg;
+                cubo::FourierFused ff{};
+                ff.cube = base;
+                ff.u16 = c->dtype == CUBO_HIP_U16;
+                ff.qy = (int)c->shape[2];
+                ff.qx = (int)c->shape[3];
+                ff.ndet = c->ndet;
+                ff.first = first;
+                ff.count = count;
+                ff.list = p->list;
+                ff.w = p->w;
+                ff.sr = p->sr;
+                ff.sc = p->sc;
+                ff.tw = p->tw_r;
+                ff.ngroups = (int)ngroups;
+                ff.ppg = ppg;
+                ff.part_sum = (double2 *)p->part_sum;
+                ff.part_max = (double *)p->part_max;
+                ff.stat = p->kind.stat;
+                ff.first_wins = p->kind.first_wins;
+                CUBO_HIP_TRY(cubo::launch_fourier_fused(ff, s), "launching the Fourier-shifted patterns");
+            }
+            CUBO_HIP_TRY(cubo::launch_pattern_accumulate(p->kind, p->part_sum, p->part_max, (int)ngroups, c->ndet,
+                                                         p->run_sum, p->run_max, !p->started, finish, p->count,
+                                                         p->out_dev, s),
+                         "launching the pattern totals");
+            p->started = true;
+        }
+        for (int64_t b0 = 0; !p->fused && (b0 < count || (count == 0 && b0 == 0 && finish)); b0 += p->batch_max) {
             const int nb = (int)std::min<int64_t>(p->batch_max, count - b0);
             if (nb > 0) {
                 cubo::FourierBatch fb{};

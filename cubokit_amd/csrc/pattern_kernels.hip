@@ -390,6 +390,338 @@ __global__ void fourier_reduce(FourierBatch b) {
     else b.part_max[u] = m;
 }
 
+// ---- PatternShift.fourier in one kernel -------------------------------------------------------------------------
+//
+// LDS (doubles): the pattern region [qy / 2 pairs][2 (qx / 2 + 1) complex], then the ramp factors A [qy] and
+// B [qx / 2 + 1], the twiddles [max / 2] and the doubly-Nyquist factor. Pair q (rows 2q, 2q + 1) owns doubles
+// q * 2 (qx + 2) .. (q + 1) * 2 (qx + 2) - 1 in two layouts:
+//   - split, for the row transforms: the real parts of the packed row z = row 2q + i row 2q+1 at offset c, the
+//     imaginary parts at offset qx + 2 + c;
+//   - interleaved, for the column transforms: the half spectrum of row r, bin k, as the complex number
+//     (r (qx / 2 + 1) + k) of the region.
+// Wave w owns the pairs w, w + 8, ... (their row transforms, the pack and unpack between the layouts and the
+// pixels it adds up) and the columns w, w + 8, ...; two block barriers per pattern separate the row and column
+// phases. Within a wave, LDS operations complete in issue order, so a wave-level fence orders its stages.
+constexpr int kFfWaves = 8;
+constexpr int kFfBlock = 64 * kFfWaves;
+constexpr int kFfPix = 32;        // pixels per lane: qy * qx <= 64 * kFfWaves * kFfPix
+constexpr int kFfItems = 3;       // half-spectrum bins per lane and pair: qx / 2 + 1 <= 64 * kFfItems
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double2 cmul(double2 x, double2 y) {
+    return make_double2(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x);
+}
+
+__device__ __forceinline__ double2 cmul_conj(double2 x, double2 y) {   // x * conj(y)
+    return make_double2(x.x * y.x + x.y * y.y, x.y * y.x - x.x * y.y);
+}
+
+__device__ __forceinline__ int bit_reverse(int x, int bits) {
+    return (int)(__builtin_bitreverse32((unsigned)x) >> (32 - bits));
+}
+
+int fourier_fused_lds(int qy, int qx) {
+    const int h = qx / 2 + 1, l = qy > qx ? qy : qx;
+    return 8 * (qy * (qx + 2) + 2 * qy + 2 * h + l + 2);
+}
+
+template <bool U16, int STAT, bool FIRST, int LM>
+__global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
+    constexpr bool MEAN = STAT == kPatternMean;
+    constexpr int VE = U16 ? 8 : 4;                      // elements per 16-byte load
+    constexpr int CH = U16 ? 4 : 8;                      // loads per lane and pattern
+    constexpr int ES = U16 ? 2 : 4;
+    extern __shared__ double ff_lds[];
+    constexpr int M = 1 << LM, H = M / 2 + 1;
+    constexpr int R2 = M + 2;                            // split layout: real plane to imaginary plane
+    const int N = a.qy;
+    const int LN = __builtin_ctz((unsigned)N);
+    const int L = N > M ? N : M, LL = __builtin_ctz((unsigned)L);
+    double *S = ff_lds;
+    double2 *C = (double2 *)ff_lds;
+    double2 *A = (double2 *)(ff_lds + N * R2);
+    double2 *B = A + N;
+    double2 *TW = B + H;
+    double2 *NYQ = TW + L / 2;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int NP = N / 16;                               // pairs per wave
+    const int NC = wave < H ? (H - wave + kFfWaves - 1) / kFfWaves : 0;   // columns of this wave
+    const int nchunks = NP * 2 * M / VE;
+    const int J = NP * 2 * M / 64;                       // pixels per lane (qx >= 32: whole 64-pixel slices)
+    const int64_t e_begin = a.first + (int64_t)blockIdx.x * a.ppg;
+    const int64_t e_end = min(a.first + a.count, e_begin + a.ppg);
+    for (int i = tid; i < L / 2; i += kFfBlock) TW[i] = a.tw[i];
+
+    double acc_s[kFfPix], acc_c[kFfPix];
+#pragma unroll
+    for (int j = 0; j < kFfPix; ++j) {
+        acc_s[j] = MEAN ? 0.0 : -__builtin_inf();
+        acc_c[j] = 0.0;
+    }
+    u32x4 pre[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) pre[j] = u32x4{0u, 0u, 0u, 0u};
+    auto position = [&](int64_t e) { return a.list ? (int64_t)a.list[e] : e; };
+    auto prefetch = [&](int64_t e) {
+        const char *base = (const char *)a.cube + position(e) * a.ndet * ES;
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int u = lane + 64 * j;
+            if (u < nchunks) {
+                const int per = 2 * M / VE, t = u / per, cc = u - t * per;
+                const int q = wave + kFfWaves * t;
+                pre[j] = load16(base + ((int64_t)2 * q * M + cc * VE) * ES);
+            }
+        }
+    };
+    __syncthreads();                                     // the twiddles
+    if (e_begin < e_end) prefetch(e_begin);
+    for (int64_t e = e_begin; e < e_end; ++e) {
+        // the raw pattern into the split layout of this wave's pairs
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int u = lane + 64 * j;
+            if (u < nchunks) {
+                const int per = 2 * M / VE, t = u / per, cc = u - t * per;
+                const int q = wave + kFfWaves * t, off = cc * VE;
+                const int odd = off >= M ? 1 : 0;
+                double *dst = S + q * 2 * R2 + odd * R2 + (off - odd * M);
+#pragma unroll
+                for (int i = 0; i < VE; i += 2) {
+                    double x0, x1;
+                    if constexpr (U16) {
+                        x0 = (double)(pre[j][i >> 1] & 0xffffu);
+                        x1 = (double)(pre[j][i >> 1] >> 16);
+                    } else {
+                        // through scalars: __builtin_bit_cast of a vector element subscript reads element 0
+                        const uint32_t b0 = pre[j][i], b1 = pre[j][i + 1];
+                        x0 = (double)__builtin_bit_cast(float, b0);
+                        x1 = (double)__builtin_bit_cast(float, b1);
+                    }
+                    *(double2 *)(dst + i) = make_double2(x0, x1);
+                }
+            }
+        }
+        const int64_t pe = position(e);
+        if (e + 1 < e_end) prefetch(e + 1);
+        // the ramp: A[f] B[k] is exp(-2 pi i (sr fr + sc fc)) / (qy qx) with NumPy's fftfreq, the angles taken
+        // in half turns (sincospi); at the Nyquist row and column the ramp averaged with its conjugate mirror
+        // (the Hermitian part, whose inverse is the real part of the reference's inverse transform)
+        {
+            const double sr = a.sr[pe], sc = a.sc[pe];
+            const double scale = 1.0 / ((double)N * (double)M);
+            for (int i = tid; i < N + H; i += kFfBlock) {
+                if (i < N) {
+                    if (i == N / 2) {
+                        A[i] = make_double2(cospi(sr) * scale, 0.0);
+                    } else {
+                        const double fr = (double)(i < N / 2 ? i : i - N) / (double)N;
+                        double sn, cs;
+                        sincospi(-2.0 * (sr * fr), &sn, &cs);
+                        A[i] = make_double2(cs * scale, sn * scale);
+                    }
+                } else {
+                    const int k = i - N;
+                    if (k == M / 2) {
+                        B[k] = make_double2(cospi(sc), 0.0);
+                    } else {
+                        double sn, cs;
+                        sincospi(-2.0 * (sc * ((double)k / (double)M)), &sn, &cs);
+                        B[k] = make_double2(cs, sn);
+                    }
+                }
+            }
+            if (tid == 0) NYQ[0] = make_double2(cospi(sr + sc) * scale, 0.0);
+        }
+        wave_sync();
+        // forward row transforms of the packed pairs (decimation in frequency: natural in, bit-reversed out)
+        for (int lh = LM - 1; lh >= 0; --lh) {
+            const int h = 1 << lh;
+            for (int u = lane; u < NP * (M / 2); u += 64) {
+                const int t = u >> (LM - 1), b = u & (M / 2 - 1), j = b & (h - 1);
+                const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+                double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
+                const double2 x = make_double2(re[i0], im[i0]), y = make_double2(re[i1], im[i1]);
+                double2 d = make_double2(x.x - y.x, x.y - y.y);
+                if (h > 1) d = cmul(d, TW[j << (LL - 1 - lh)]);
+                re[i0] = x.x + y.x;
+                im[i0] = x.y + y.y;
+                re[i1] = d.x;
+                im[i1] = d.y;
+            }
+            wave_sync();
+        }
+        // unpack: Z = FFT(row 2q + i row 2q+1) gives both half spectra, X0[k] = (Z[k] + conj Z[-k]) / 2 and
+        // X1[k] = -i (Z[k] - conj Z[-k]) / 2, written interleaved
+        for (int t = 0; t < NP; ++t) {
+            double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
+            double2 z0[kFfItems], z1[kFfItems];
+#pragma unroll
+            for (int i = 0; i < kFfItems; ++i) {
+                const int k = lane + 64 * i;
+                if (k <= M / 2) {
+                    const int pk = bit_reverse(k, LM), pm = bit_reverse((M - k) & (M - 1), LM);
+                    z0[i] = make_double2(re[pk], im[pk]);
+                    z1[i] = make_double2(re[pm], im[pm]);
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < kFfItems; ++i) {
+                const int k = lane + 64 * i;
+                if (k <= M / 2) {
+                    ((double2 *)re)[k] = make_double2(0.5 * (z0[i].x + z1[i].x), 0.5 * (z0[i].y - z1[i].y));
+                    ((double2 *)im)[k] = make_double2(0.5 * (z0[i].y + z1[i].y), -0.5 * (z0[i].x - z1[i].x));
+                }
+            }
+            wave_sync();
+        }
+        __syncthreads();
+        // the columns: forward (decimation in frequency), the ramp on the last stage, inverse (decimation in time)
+        for (int lh = LN - 1; lh >= 0; --lh) {
+            const int h = 1 << lh;
+            for (int u = lane; u < NC * (N / 2); u += 64) {
+                const int i = u >> (LN - 1), b = u & (N / 2 - 1), j = b & (h - 1);
+                const int k = wave + kFfWaves * i;
+                const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+                const double2 x = C[i0 * H + k], y = C[i1 * H + k];
+                double2 s0 = make_double2(x.x + y.x, x.y + y.y), d = make_double2(x.x - y.x, x.y - y.y);
+                if (h > 1) {
+                    d = cmul(d, TW[j << (LL - 1 - lh)]);
+                } else {
+                    const int f0 = bit_reverse(i0, LN), f1 = bit_reverse(i1, LN);
+                    const bool kn = k == M / 2;
+                    s0 = cmul(s0, (kn && f0 == N / 2) ? NYQ[0] : cmul(A[f0], B[k]));
+                    d = cmul(d, (kn && f1 == N / 2) ? NYQ[0] : cmul(A[f1], B[k]));
+                }
+                C[i0 * H + k] = s0;
+                C[i1 * H + k] = d;
+            }
+            wave_sync();
+        }
+        for (int lh = 0; lh < LN; ++lh) {
+            const int h = 1 << lh;
+            for (int u = lane; u < NC * (N / 2); u += 64) {
+                const int i = u >> (LN - 1), b = u & (N / 2 - 1), j = b & (h - 1);
+                const int k = wave + kFfWaves * i;
+                const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+                const double2 x = C[i0 * H + k];
+                double2 y = C[i1 * H + k];
+                if (h > 1) y = cmul_conj(y, TW[j << (LL - 1 - lh)]);
+                C[i0 * H + k] = make_double2(x.x + y.x, x.y + y.y);
+                C[i1 * H + k] = make_double2(x.x - y.x, x.y - y.y);
+            }
+            wave_sync();
+        }
+        __syncthreads();
+        // pack: W = X0 + i X1 over the full row (W[-k] = conj X0[k] + i conj X1[k]), bit-reversed for the inverse
+        for (int t = 0; t < NP; ++t) {
+            double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
+            double2 z0[kFfItems], z1[kFfItems];
+#pragma unroll
+            for (int i = 0; i < kFfItems; ++i) {
+                const int k = lane + 64 * i;
+                if (k <= M / 2) {
+                    z0[i] = ((const double2 *)re)[k];
+                    z1[i] = ((const double2 *)im)[k];
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < kFfItems; ++i) {
+                const int k = lane + 64 * i;
+                if (k <= M / 2) {
+                    const int pk = bit_reverse(k, LM);
+                    re[pk] = z0[i].x - z1[i].y;
+                    im[pk] = z0[i].y + z1[i].x;
+                    if (k > 0 && k < M / 2) {
+                        const int pm = bit_reverse(M - k, LM);
+                        re[pm] = z0[i].x + z1[i].y;
+                        im[pm] = z1[i].x - z0[i].y;
+                    }
+                }
+            }
+            wave_sync();
+        }
+        // inverse row transforms (decimation in time: bit-reversed in, natural out): rows 2q and 2q+1 real
+        for (int lh = 0; lh < LM; ++lh) {
+            const int h = 1 << lh;
+            for (int u = lane; u < NP * (M / 2); u += 64) {
+                const int t = u >> (LM - 1), b = u & (M / 2 - 1), j = b & (h - 1);
+                const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+                double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
+                const double2 x = make_double2(re[i0], im[i0]);
+                double2 y = make_double2(re[i1], im[i1]);
+                if (h > 1) y = cmul_conj(y, TW[j << (LL - 1 - lh)]);
+                re[i0] = x.x + y.x;
+                im[i0] = x.y + y.y;
+                re[i1] = x.x - y.x;
+                im[i1] = x.y - y.y;
+            }
+            wave_sync();
+        }
+        // this lane's pixels: the shifted value (times the weight), TwoSum or the maximum, in entry order
+        const double wgt = a.w ? a.w[pe] : 1.0;
+#pragma unroll
+        for (int j = 0; j < kFfPix; ++j) {
+            const int u = lane + 64 * j;
+            if (j < J) {
+                const int t = u >> (LM + 1), rem = u & (2 * M - 1);
+                double v = S[(wave + kFfWaves * t) * 2 * R2 + (rem >= M ? R2 : 0) + (rem & (M - 1))];
+                if (a.w) v = v * wgt;
+                if constexpr (MEAN) two_sum(acc_s[j], acc_c[j], v);
+                else acc_s[j] = max_step<FIRST, true>(acc_s[j], v);
+            }
+            if (j % 8 == 7) __builtin_amdgcn_sched_barrier(0);   // at most 8 reads in flight
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int j = 0; j < kFfPix; ++j) {
+        const int u = lane + 64 * j;
+        if (j < J) {
+            const int t = u >> (LM + 1), rem = u & (2 * M - 1);
+            const int64_t pix = (int64_t)(wave + kFfWaves * t) * 2 * M + rem;
+            const int64_t at = (int64_t)blockIdx.x * a.ndet + pix;
+            if constexpr (MEAN) a.part_sum[at] = make_double2(acc_s[j], acc_c[j]);
+            else a.part_max[at] = acc_s[j];
+        }
+    }
+}
+
+template <bool U16, int STAT, bool FIRST, int LM>
+hipError_t launch_ff(const FourierFused &a, int lds, hipStream_t s) {
+    const void *fn = (const void *)fourier_fused<U16, STAT, FIRST, LM>;
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((fourier_fused<U16, STAT, FIRST, LM>), dim3((unsigned)a.ngroups), dim3(kFfBlock), lds, s, a);
+    return hipGetLastError();
+}
+
+template <bool U16, int STAT, bool FIRST>
+hipError_t launch_ff_m(const FourierFused &a, int lds, hipStream_t s) {
+    switch (a.qx) {
+    case 32: return launch_ff<U16, STAT, FIRST, 5>(a, lds, s);
+    case 64: return launch_ff<U16, STAT, FIRST, 6>(a, lds, s);
+    case 128: return launch_ff<U16, STAT, FIRST, 7>(a, lds, s);
+    case 256: return launch_ff<U16, STAT, FIRST, 8>(a, lds, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <bool U16>
+hipError_t launch_ff_u(const FourierFused &a, int lds, hipStream_t s) {
+    if (a.stat == kPatternMean) return launch_ff_m<U16, kPatternMean, true>(a, lds, s);
+    if (a.first_wins) return launch_ff_m<U16, kPatternMax, true>(a, lds, s);
+    return launch_ff_m<U16, kPatternMax, false>(a, lds, s);
+}
+
 template <typename T, int VEC, bool INT, int STAT>
 hipError_t launch_t(const PatternLaunch &a, const PatternKind &k, dim3 grid, hipStream_t s) {
     if (k.shift) {
@@ -446,6 +778,20 @@ hipError_t launch_fourier_phase(const FourierBatch &b, hipStream_t s) {
     const int64_t n = (int64_t)b.batch * b.ndet;
     hipLaunchKernelGGL(fourier_phase, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 16384)), dim3(256), 0, s, b);
     return hipGetLastError();
+}
+
+bool fourier_fused_supported(int qy, int qx) {
+    auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
+    return pow2(qy) && pow2(qx) && qy >= 16 && qx >= 32 && qx <= 256 && qx <= 2 * 64 * kFfItems - 2 &&
+           (int64_t)qy * qx <= (int64_t)64 * kFfWaves * kFfPix && fourier_fused_lds(qy, qx) <= 160 * 1024;
+}
+
+hipError_t launch_fourier_fused(const FourierFused &a, hipStream_t s) {
+    if (!fourier_fused_supported(a.qy, a.qx) || a.ngroups <= 0) return hipErrorInvalidValue;
+    if (a.count <= 0) return hipSuccess;
+    const int lds = fourier_fused_lds(a.qy, a.qx);
+    if (a.u16) return launch_ff_u<true>(a, lds, s);
+    return launch_ff_u<false>(a, lds, s);
 }
 
 hipError_t launch_fourier_reduce(const FourierBatch &b, hipStream_t s) {

@@ -78,6 +78,33 @@ struct FourierBatch {
     bool first_wins;
 };
 hipError_t launch_fourier_expand(const FourierBatch &b, hipStream_t s);
+
+// The same shift and reduction in one kernel for power-of-two detectors of at most 16384 pixels (rows >= 16,
+// 16 <= columns <= 256): block g takes the entries first + g ppg .. first + (g + 1) ppg - 1 in order, one pattern
+// at a time, with the pattern's half spectrum (rows x (columns / 2 + 1), FP64 complex) in LDS. Rows are
+// transformed two at a time as one complex FFT, the columns in place, the phase ramp made Hermitian (so the real
+// part of the inverse is the inverse of a real pattern) and the inverse taken the same way back; each lane adds
+// the shifted values of its own pixels (TwoSum, or the maximum) in registers and the block writes one group
+// partial of the layout launch_pattern_accumulate reads.
+struct FourierFused {
+    const void *cube;         // position p's pattern at cube + p * ndet * element size
+    bool u16;
+    int32_t qy, qx;
+    int64_t ndet;
+    int64_t first, count;     // entries first .. first + count - 1
+    const int32_t *list;      // entry -> position, or NULL
+    const double *w;          // [npos] weights, or NULL
+    const double *sr, *sc;    // [npos] the shift in pixels
+    const double2 *tw;        // [max(qy, qx) / 2] exp(-2 pi i m / max(qy, qx))
+    int32_t ngroups;
+    int64_t ppg;              // entries per group
+    double2 *part_sum;        // [ngroups][ndet]
+    double *part_max;         // [ngroups][ndet]
+    int stat;
+    bool first_wins;
+};
+bool fourier_fused_supported(int qy, int qx);
+hipError_t launch_fourier_fused(const FourierFused &a, hipStream_t s);
 hipError_t launch_fourier_phase(const FourierBatch &b, hipStream_t s);
 hipError_t launch_fourier_reduce(const FourierBatch &b, hipStream_t s);
 

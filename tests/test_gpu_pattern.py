@@ -346,11 +346,12 @@ def test_bandwidth_cube_selected_and_shifted_patterns():
     assert_identical(got_max, O.pattern(host, "max", take=disk, origin=orig), "shifted disk max")
 
 
-@pytest.mark.parametrize("dtype,det", [(np.float32, (16, 24)), (np.uint16, (15, 12))])
+@pytest.mark.parametrize("dtype,det", [(np.float32, (16, 24)), (np.uint16, (15, 12)),          # batched FFTs
+                                       (np.uint16, (32, 64)), (np.float32, (64, 32)), (np.uint16, (128, 128))])
 def test_fourier_shift_matches_the_oracle(dtype, det):
     """PatternShift.fourier over a mask, weights and every position (first-kept maxima) and meanAndMax(of:shift:)
-    (later-kept): FP64 FFTs on both sides, normwise within 1e-12 of max|oracle|; batches smaller than the selection
-    through several tiles."""
+    (later-kept): FP64 FFTs on both sides, normwise within 1e-12 of max|oracle|, through several tiles. Power-of-two
+    detectors take the one-kernel path (half spectrum in LDS), the others batches of 2-D FFTs."""
     rng = np.random.default_rng(23)
     qy, qx = det
     R, Cs = 7, 6
@@ -378,3 +379,29 @@ def test_fourier_shift_matches_the_oracle(dtype, det):
         mean, mx = C.VirtualDiffraction.mean_and_max_shifted(res, shift, tiling=t)
         close(mean.values, O.pattern(cube, "mean", origin=(orow, ocol), ties="later", fourier=True), "meanAndMax mean")
         close(mx.values, O.pattern(cube, "max", origin=(orow, ocol), ties="later", fourier=True), "meanAndMax max")
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.float32])
+def test_fourier_shift_groups_of_many_entries(dtype):
+    """The one-kernel path with more entries than blocks: each block adds a run of patterns in order before the
+    groups are combined; a signed weight map and a mask; within 1e-12 of max|oracle|."""
+    rng = np.random.default_rng(29)
+    R, Cs, qy, qx = 24, 23, 32, 32
+    npos = R * Cs
+    if dtype == np.uint16:
+        cube = rng.integers(0, 65536, size=(R, Cs, qy, qx)).astype(np.uint16)
+    else:
+        cube = rng.normal(30, 10, size=(R, Cs, qy, qx)).astype(np.float32)
+    orow, ocol = rng.normal(15, 4, npos), rng.normal(16, 4, npos)
+    mask = rng.random(npos) < 0.8
+    weights = rng.normal(size=npos)
+    res = C.HIPResidentCube(cube)
+    shift = C.PatternShift.fourier_shift(C.OriginMap(orow, ocol))
+    for t in (C.ScanTiling.automatic, C.ScanTiling.rows(11)):
+        for kind, scan, kw in (("mask", C.ScanSelection.mask(mask), {"take": mask}),
+                               ("weights", C.ScanSelection.weights(weights), {"weights": weights})):
+            print(f"case: fourier groups {dtype.__name__} {kind} {t}", flush=True)
+            for stat in (MEAN, MAX):
+                got = C.VirtualDiffraction.pattern(res, stat, scan, shift=shift, tiling=t).values
+                want = O.pattern(cube, stat.value, origin=(orow, ocol), fourier=True, **kw)
+                assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), f"{kind} {stat}"
