@@ -223,9 +223,8 @@ int cubo_hip_pattern_begin(cubo_hip_cube *c, const cubo_hip_pattern_request *q, 
     if (e == hipSuccess && shifted && !p->fourier) e = hipMalloc((void **)&p->shift, npos * 4);
     std::vector<double2> twr, twc;
     if (p->fused) {
-        // one twiddle table for both axes: exp(-2 pi i m / max(qy, qx)), m < max / 2
+        // one twiddle table for both axes: exp(-2 pi i m / max(qy, qx)), m < max
         twr = cubo::fft_twiddles(std::max(qy, qx));
-        twr.resize(std::max(qy, qx) / 2);
         if (e == hipSuccess) e = hipMalloc((void **)&p->sr, npos * 8);
         if (e == hipSuccess) e = hipMalloc((void **)&p->sc, npos * 8);
         if (e == hipSuccess) e = hipMalloc((void **)&p->tw_r, twr.size() * 16);

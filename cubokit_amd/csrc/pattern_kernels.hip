@@ -425,9 +425,53 @@ __device__ __forceinline__ int bit_reverse(int x, int bits) {
     return (int)(__builtin_bitreverse32((unsigned)x) >> (32 - bits));
 }
 
+__device__ __forceinline__ double2 cadd(double2 x, double2 y) { return make_double2(x.x + y.x, x.y + y.y); }
+__device__ __forceinline__ double2 csub(double2 x, double2 y) { return make_double2(x.x - y.x, x.y - y.y); }
+
+// One radix-4 stage of a decimation-in-frequency transform (exp(-2 pi i ...)): the elements j, j + h, j + 2h, j + 3h
+// of a block of 4h, w1..w3 = W^j, W^2j, W^3j with W = exp(-2 pi i / 4h). The results land where the two radix-2
+// stages of spans 2h and h would put them, so the output order stays bit-reversed.
+__device__ __forceinline__ void dif4(double2 &x0, double2 &x1, double2 &x2, double2 &x3, double2 w1, double2 w2,
+                                     double2 w3, bool twiddle) {
+    const double2 a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3), d = csub(x1, x3);
+    const double2 a3 = make_double2(d.y, -d.x);       // (x1 - x3) * -i
+    double2 y0 = cadd(a0, a2), y2 = csub(a0, a2), y1 = cadd(a1, a3), y3 = csub(a1, a3);
+    if (twiddle) {
+        y1 = cmul(y1, w1);
+        y2 = cmul(y2, w2);
+        y3 = cmul(y3, w3);
+    }
+    x0 = y0;   // position j
+    x1 = y2;   // j + h
+    x2 = y1;   // j + 2h
+    x3 = y3;   // j + 3h
+}
+
+// One radix-4 stage of a decimation-in-time inverse (exp(+2 pi i ...)): the radix-2 stages of spans h and 2h, with
+// c1 = W^j and c2 = W^2j (conjugated here).
+__device__ __forceinline__ void dit4(double2 &x0, double2 &x1, double2 &x2, double2 &x3, double2 c1, double2 c2,
+                                     bool twiddle) {
+    double2 t1 = x1, t3 = x3;
+    if (twiddle) {
+        t1 = cmul_conj(x1, c2);
+        t3 = cmul_conj(x3, c2);
+    }
+    const double2 p0 = cadd(x0, t1), p1 = csub(x0, t1), p2 = cadd(x2, t3), p3 = csub(x2, t3);
+    double2 q2 = p2, q3 = p3;
+    if (twiddle) {
+        q2 = cmul_conj(p2, c1);
+        q3 = cmul_conj(p3, c1);
+    }
+    const double2 q3i = make_double2(-q3.y, q3.x);    // * i
+    x0 = cadd(p0, q2);    // position j
+    x2 = csub(p0, q2);    // j + 2h
+    x1 = cadd(p1, q3i);   // j + h
+    x3 = csub(p1, q3i);   // j + 3h
+}
+
 int fourier_fused_lds(int qy, int qx) {
     const int h = qx / 2 + 1, l = qy > qx ? qy : qx;
-    return 8 * (qy * (qx + 2) + 2 * qy + 2 * h + l + 2);
+    return 8 * (qy * (qx + 2) + 2 * qy + 2 * h + 2 * l + 2);
 }
 
 template <bool U16, int STAT, bool FIRST, int LM>
@@ -447,7 +491,7 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
     double2 *A = (double2 *)(ff_lds + N * R2);
     double2 *B = A + N;
     double2 *TW = B + H;
-    double2 *NYQ = TW + L / 2;
+    double2 *NYQ = TW + L;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int NP = N / 16;                               // pairs per wave
@@ -456,7 +500,7 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
     const int J = NP * 2 * M / 64;                       // pixels per lane (qx >= 32: whole 64-pixel slices)
     const int64_t e_begin = a.first + (int64_t)blockIdx.x * a.ppg;
     const int64_t e_end = min(a.first + a.count, e_begin + a.ppg);
-    for (int i = tid; i < L / 2; i += kFfBlock) TW[i] = a.tw[i];
+    for (int i = tid; i < L; i += kFfBlock) TW[i] = a.tw[i];
 
     double acc_s[kFfPix], acc_c[kFfPix];
 #pragma unroll
@@ -480,6 +524,119 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             }
         }
     };
+    // Stages over this wave's packed pairs (split layout). Radix-2 span h = 2^lh, radix-4 quarter h = 2^lq.
+    auto row_r2_dif = [&](int lh) {
+        const int h = 1 << lh;
+        for (int u = lane; u < NP * (M / 2); u += 64) {
+            const int t = u >> (LM - 1), b = u & (M / 2 - 1), j = b & (h - 1);
+            const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+            double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
+            const double2 x = make_double2(re[i0], im[i0]), y = make_double2(re[i1], im[i1]);
+            double2 d = csub(x, y);
+            if (h > 1) d = cmul(d, TW[j << (LL - 1 - lh)]);
+            re[i0] = x.x + y.x;
+            im[i0] = x.y + y.y;
+            re[i1] = d.x;
+            im[i1] = d.y;
+        }
+        wave_sync();
+    };
+    auto row_r2_dit = [&](int lh) {
+        const int h = 1 << lh;
+        for (int u = lane; u < NP * (M / 2); u += 64) {
+            const int t = u >> (LM - 1), b = u & (M / 2 - 1), j = b & (h - 1);
+            const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+            double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
+            const double2 x = make_double2(re[i0], im[i0]);
+            double2 y = make_double2(re[i1], im[i1]);
+            if (h > 1) y = cmul_conj(y, TW[j << (LL - 1 - lh)]);
+            re[i0] = x.x + y.x;
+            im[i0] = x.y + y.y;
+            re[i1] = x.x - y.x;
+            im[i1] = x.y - y.y;
+        }
+        wave_sync();
+    };
+    auto row_r4 = [&](int lq, bool inverse) {
+        const int h = 1 << lq;
+        for (int u = lane; u < NP * (M / 4); u += 64) {
+            const int t = u >> (LM - 2), b = u & (M / 4 - 1), j = b & (h - 1);
+            const int i0 = ((b >> lq) << (lq + 2)) + j;
+            double *re = S + (wave + kFfWaves * t) * 2 * R2 + i0, *im = re + R2;
+            double2 x0 = make_double2(re[0], im[0]), x1 = make_double2(re[h], im[h]);
+            double2 x2 = make_double2(re[2 * h], im[2 * h]), x3 = make_double2(re[3 * h], im[3 * h]);
+            const int ts = LL - lq - 2;
+            if (inverse) dit4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], h > 1);
+            else dif4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], TW[(3 * j) << ts], h > 1);
+            re[0] = x0.x; im[0] = x0.y;
+            re[h] = x1.x; im[h] = x1.y;
+            re[2 * h] = x2.x; im[2 * h] = x2.y;
+            re[3 * h] = x3.x; im[3 * h] = x3.y;
+        }
+        wave_sync();
+    };
+    auto row_r4_dif = [&](int lq) { row_r4(lq, false); };
+    auto row_r4_dit = [&](int lq) { row_r4(lq, true); };
+    // Stages over this wave's columns (interleaved layout: bin k of row r is C[r H + k]).
+    auto col_r2_dif = [&](int lh) {            // never the last forward stage (that one is radix-4)
+        const int h = 1 << lh;
+        for (int u = lane; u < NC * (N / 2); u += 64) {
+            const int i = u >> (LN - 1), b = u & (N / 2 - 1), j = b & (h - 1);
+            const int k = wave + kFfWaves * i;
+            const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+            const double2 x = C[i0 * H + k], y = C[i1 * H + k];
+            double2 d = csub(x, y);
+            if (h > 1) d = cmul(d, TW[j << (LL - 1 - lh)]);
+            C[i0 * H + k] = cadd(x, y);
+            C[i1 * H + k] = d;
+        }
+        wave_sync();
+    };
+    auto col_r2_dit = [&](int lh) {
+        const int h = 1 << lh;
+        for (int u = lane; u < NC * (N / 2); u += 64) {
+            const int i = u >> (LN - 1), b = u & (N / 2 - 1), j = b & (h - 1);
+            const int k = wave + kFfWaves * i;
+            const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+            const double2 x = C[i0 * H + k];
+            double2 y = C[i1 * H + k];
+            if (h > 1) y = cmul_conj(y, TW[j << (LL - 1 - lh)]);
+            C[i0 * H + k] = cadd(x, y);
+            C[i1 * H + k] = csub(x, y);
+        }
+        wave_sync();
+    };
+    auto ramp = [&](int f, int k) { return (k == M / 2 && f == N / 2) ? NYQ[0] : cmul(A[f], B[k]); };
+    auto col_r4 = [&](int lq, bool inverse) {
+        const int h = 1 << lq;
+        for (int u = lane; u < NC * (N / 4); u += 64) {
+            const int i = u >> (LN - 2), b = u & (N / 4 - 1), j = b & (h - 1);
+            const int k = wave + kFfWaves * i;
+            const int i0 = ((b >> lq) << (lq + 2)) + j;
+            double2 *c = C + i0 * H + k;
+            double2 x0 = c[0], x1 = c[h * H], x2 = c[2 * h * H], x3 = c[3 * h * H];
+            const int ts = LL - lq - 2;
+            if (inverse) {
+                dit4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], h > 1);
+            } else {
+                dif4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], TW[(3 * j) << ts], h > 1);
+                if (lq == 0) {                         // the last forward stage: frequencies bit-reversed
+                    x0 = cmul(x0, ramp(bit_reverse(i0, LN), k));
+                    x1 = cmul(x1, ramp(bit_reverse(i0 + 1, LN), k));
+                    x2 = cmul(x2, ramp(bit_reverse(i0 + 2, LN), k));
+                    x3 = cmul(x3, ramp(bit_reverse(i0 + 3, LN), k));
+                }
+            }
+            c[0] = x0;
+            c[h * H] = x1;
+            c[2 * h * H] = x2;
+            c[3 * h * H] = x3;
+        }
+        wave_sync();
+    };
+    auto col_r4_dif = [&](int lq) { col_r4(lq, false); };
+    auto col_r4_dit = [&](int lq) { col_r4(lq, true); };
+
     __syncthreads();                                     // the twiddles
     if (e_begin < e_end) prefetch(e_begin);
     for (int64_t e = e_begin; e < e_end; ++e) {
@@ -540,22 +697,15 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             if (tid == 0) NYQ[0] = make_double2(cospi(sr + sc) * scale, 0.0);
         }
         wave_sync();
-        // forward row transforms of the packed pairs (decimation in frequency: natural in, bit-reversed out)
-        for (int lh = LM - 1; lh >= 0; --lh) {
-            const int h = 1 << lh;
-            for (int u = lane; u < NP * (M / 2); u += 64) {
-                const int t = u >> (LM - 1), b = u & (M / 2 - 1), j = b & (h - 1);
-                const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
-                double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
-                const double2 x = make_double2(re[i0], im[i0]), y = make_double2(re[i1], im[i1]);
-                double2 d = make_double2(x.x - y.x, x.y - y.y);
-                if (h > 1) d = cmul(d, TW[j << (LL - 1 - lh)]);
-                re[i0] = x.x + y.x;
-                im[i0] = x.y + y.y;
-                re[i1] = d.x;
-                im[i1] = d.y;
+        // forward row transforms of the packed pairs (decimation in frequency: natural in, bit-reversed out): a
+        // radix-2 stage first when log2 qx is odd, then radix-4
+        {
+            int top = LM - 1;
+            if (LM & 1) {
+                row_r2_dif(top);
+                top -= 1;
             }
-            wave_sync();
+            for (; top >= 1; top -= 2) row_r4_dif(top - 1);
         }
         // unpack: Z = FFT(row 2q + i row 2q+1) gives both half spectra, X0[k] = (Z[k] + conj Z[-k]) / 2 and
         // X1[k] = -i (Z[k] - conj Z[-k]) / 2, written interleaved
@@ -583,41 +733,18 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             wave_sync();
         }
         __syncthreads();
-        // the columns: forward (decimation in frequency), the ramp on the last stage, inverse (decimation in time)
-        for (int lh = LN - 1; lh >= 0; --lh) {
-            const int h = 1 << lh;
-            for (int u = lane; u < NC * (N / 2); u += 64) {
-                const int i = u >> (LN - 1), b = u & (N / 2 - 1), j = b & (h - 1);
-                const int k = wave + kFfWaves * i;
-                const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
-                const double2 x = C[i0 * H + k], y = C[i1 * H + k];
-                double2 s0 = make_double2(x.x + y.x, x.y + y.y), d = make_double2(x.x - y.x, x.y - y.y);
-                if (h > 1) {
-                    d = cmul(d, TW[j << (LL - 1 - lh)]);
-                } else {
-                    const int f0 = bit_reverse(i0, LN), f1 = bit_reverse(i1, LN);
-                    const bool kn = k == M / 2;
-                    s0 = cmul(s0, (kn && f0 == N / 2) ? NYQ[0] : cmul(A[f0], B[k]));
-                    d = cmul(d, (kn && f1 == N / 2) ? NYQ[0] : cmul(A[f1], B[k]));
-                }
-                C[i0 * H + k] = s0;
-                C[i1 * H + k] = d;
+        // the columns: forward (decimation in frequency; the last stage, radix-4, applies the ramp), then inverse
+        // (decimation in time: bit-reversed in, natural out)
+        {
+            int top = LN - 1;
+            if (LN & 1) {
+                col_r2_dif(top);
+                top -= 1;
             }
-            wave_sync();
-        }
-        for (int lh = 0; lh < LN; ++lh) {
-            const int h = 1 << lh;
-            for (int u = lane; u < NC * (N / 2); u += 64) {
-                const int i = u >> (LN - 1), b = u & (N / 2 - 1), j = b & (h - 1);
-                const int k = wave + kFfWaves * i;
-                const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
-                const double2 x = C[i0 * H + k];
-                double2 y = C[i1 * H + k];
-                if (h > 1) y = cmul_conj(y, TW[j << (LL - 1 - lh)]);
-                C[i0 * H + k] = make_double2(x.x + y.x, x.y + y.y);
-                C[i1 * H + k] = make_double2(x.x - y.x, x.y - y.y);
-            }
-            wave_sync();
+            for (; top >= 1; top -= 2) col_r4_dif(top - 1);
+            int lq = 0;
+            for (; lq + 2 <= LN; lq += 2) col_r4_dit(lq);
+            if (lq < LN) col_r2_dit(lq);
         }
         __syncthreads();
         // pack: W = X0 + i X1 over the full row (W[-k] = conj X0[k] + i conj X1[k]), bit-reversed for the inverse
@@ -650,21 +777,10 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             wave_sync();
         }
         // inverse row transforms (decimation in time: bit-reversed in, natural out): rows 2q and 2q+1 real
-        for (int lh = 0; lh < LM; ++lh) {
-            const int h = 1 << lh;
-            for (int u = lane; u < NP * (M / 2); u += 64) {
-                const int t = u >> (LM - 1), b = u & (M / 2 - 1), j = b & (h - 1);
-                const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
-                double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
-                const double2 x = make_double2(re[i0], im[i0]);
-                double2 y = make_double2(re[i1], im[i1]);
-                if (h > 1) y = cmul_conj(y, TW[j << (LL - 1 - lh)]);
-                re[i0] = x.x + y.x;
-                im[i0] = x.y + y.y;
-                re[i1] = x.x - y.x;
-                im[i1] = x.y - y.y;
-            }
-            wave_sync();
+        {
+            int lq = 0;
+            for (; lq + 2 <= LM; lq += 2) row_r4_dit(lq);
+            if (lq < LM) row_r2_dit(lq);
         }
         // this lane's pixels: the shifted value (times the weight), TwoSum or the maximum, in entry order
         const double wgt = a.w ? a.w[pe] : 1.0;

@@ -95,7 +95,7 @@ struct FourierFused {
     const int32_t *list;      // entry -> position, or NULL
     const double *w;          // [npos] weights, or NULL
     const double *sr, *sc;    // [npos] the shift in pixels
-    const double2 *tw;        // [max(qy, qx) / 2] exp(-2 pi i m / max(qy, qx))
+    const double2 *tw;        // [max(qy, qx)] exp(-2 pi i m / max(qy, qx))
     int32_t ngroups;
     int64_t ppg;              // entries per group
     double2 *part_sum;        // [ngroups][ndet]
