@@ -12,11 +12,17 @@
 // lane whole window rows, and lane 0 adds the row sums in row order.
 #include "origin_kernels.hpp"
 
+#include <climits>
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace cubo {
 
 namespace {
+
+#define CUBO_GLOBAL __attribute__((address_space(1)))
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kPyBlock = 256;
 constexpr int kIWWaves = 4;
@@ -131,6 +137,228 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin(Py4Launch a) {
         }
         __syncthreads();
     }
+}
+
+// ---- get_origin with the pattern in LDS -------------------------------------------------------------------
+//
+// For patterns whose elements fit in LDS beside a strip of the axis-0 pass: the raw pattern is staged once, the
+// axis-0 pass of S rows goes to a transposed strip T[j][i] (row stride S + 1 doubles, so lanes of consecutive
+// columns write different banks), and the axis-1 pass reads it along lanes of consecutive rows. Each lane
+// computes kPyV consecutive outputs along the filtered axis from two register windows of the line (the values
+// left and right of the outputs), which slide kPyV taps at a time: two loads per kPyV outputs and taps instead of
+// two per output and tap. The operation sequence of every output is the reference's.
+constexpr int kPyV = 8;
+
+// acc[v] = the filtered values at o .. o + kPyV - 1 of a line x(0 .. n - 1) with nearest-edge extension: x(o) times
+// the centre weight, then (x(o - t) + x(o + t)) * w[R - t] for t = R down to 1.
+template <typename X>
+__device__ __forceinline__ void filter_run(double (&acc)[kPyV], int o, int n, int R, const double *w, double centre,
+                                           X x) {
+    auto at = [&](int k) { return x(k < 0 ? 0 : (k >= n ? n - 1 : k)); };
+#pragma unroll
+    for (int v = 0; v < kPyV; ++v) acc[v] = at(o + v) * centre;
+    int t = R;
+    for (const int lead = R - R % kPyV; t > lead; --t) {       // the first R mod kPyV taps one by one
+        const double wt = w[R - t];
+#pragma unroll
+        for (int v = 0; v < kPyV; ++v) acc[v] += (at(o + v - t) + at(o + v + t)) * wt;
+    }
+    if (t < 1) return;
+    // taps t0 .. t0 - kPyV + 1: left value of (v, s) is lb[v + s] = x(o - t0 + v + s), right rb[v - s + kPyV - 1] =
+    // x(o + t0 + v - s)
+    double lb[2 * kPyV - 1], rb[2 * kPyV - 1];
+#pragma unroll
+    for (int k = 0; k < 2 * kPyV - 1; ++k) {
+        lb[k] = at(o - t + k);
+        rb[k] = at(o + t - kPyV + 1 + k);
+    }
+    for (;;) {
+#pragma unroll
+        for (int sidx = 0; sidx < kPyV; ++sidx) {
+            const double wt = w[R - t + sidx];
+#pragma unroll
+            for (int v = 0; v < kPyV; ++v) acc[v] += (lb[v + sidx] + rb[v - sidx + kPyV - 1]) * wt;
+        }
+        t -= kPyV;
+        if (t < 1) break;
+#pragma unroll
+        for (int k = 0; k < kPyV - 1; ++k) lb[k] = lb[k + kPyV];
+#pragma unroll
+        for (int k = 0; k < kPyV; ++k) lb[kPyV - 1 + k] = at(o - t + kPyV - 1 + k);
+#pragma unroll
+        for (int k = 2 * kPyV - 2; k >= kPyV; --k) rb[k] = rb[k - kPyV];
+#pragma unroll
+        for (int k = 0; k < kPyV; ++k) rb[k] = at(o + t - kPyV + 1 + k);
+    }
+}
+
+// (v, b, n) <- the merge of two argmax states: the smallest NaN index n; of the non-NaN maxima the larger value, on a
+// tie the smaller index b (INT_MAX: none).
+__device__ __forceinline__ void argmax_merge(double &v, int &b, int &n, double v2, int b2, int n2) {
+    n = n2 < n ? n2 : n;
+    if (b2 != INT_MAX && (b == INT_MAX || v2 > v || (v2 == v && b2 < b))) {
+        v = v2;
+        b = b2;
+    }
+}
+
+__device__ __forceinline__ void argmax_merge_wave(double &v, int &b, int &n) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const double v2 = __shfl_xor(v, m);
+        const int b2 = __shfl_xor(b, m), n2 = __shfl_xor(n, m);
+        argmax_merge(v, b, n, v2, b2, n2);
+    }
+}
+
+// (s, c) summed over the wave: s by TwoSum, the corrections and the rounding errors into c.
+__device__ __forceinline__ void two_sum_wave(double &s, double &c) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const double s2 = __shfl_xor(s, m), c2 = __shfl_xor(c, m);
+        two_sum(s, c, s2);
+        c += c2;
+    }
+}
+
+template <bool U16>
+__global__ __launch_bounds__(kPyBlock) void py4_origin_lds(Py4Launch a, int S) {
+    using T = typename std::conditional<U16, uint16_t, float>::type;
+    extern __shared__ double py_lds[];
+    const int tid = threadIdx.x;
+    const int qy = a.qy, qx = a.qx, R = a.R, ndet = qy * qx, SP = S + 1;
+    T *raw = (T *)py_lds;                                  // [ndet], padded to 16 bytes
+    double *tr = py_lds + ((ndet * (int)sizeof(T) + 15) & ~15) / 8;   // [qx][S + 1]: axis 0, transposed
+    double *wl = tr + qx * SP;                             // [2R + 1]
+    constexpr int kWaves = kPyBlock / 64;
+    __shared__ double red_v[kWaves];
+    __shared__ int red_i[kWaves], red_n[kWaves];
+    __shared__ double red_s[6][kWaves];
+    __shared__ int best_at;
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < 2 * R + 1; i += kPyBlock) wl[i] = a.weights[i];
+    const double centre = a.weights[R];
+    const bool vec = (ndet * (int)sizeof(T)) % 16 == 0;
+    for (int64_t pos = blockIdx.x; pos < a.npos; pos += gridDim.x) {
+        if (a.take && !a.take[pos]) {
+            if (tid == 0) { a.out_row[pos] = __builtin_nan(""); a.out_col[pos] = __builtin_nan(""); }
+            continue;   // uniform across the block
+        }
+        const T *pat = (const T *)((const char *)a.cube + pos * (int64_t)ndet * (int64_t)sizeof(T));
+        if (vec) {
+            const int nv = ndet * (int)sizeof(T) / 16;
+            for (int v = tid; v < nv; v += kPyBlock)
+                ((u32x4 *)raw)[v] = __builtin_nontemporal_load((const CUBO_GLOBAL u32x4 *)pat + v);
+        } else {
+            for (int v = tid; v < ndet; v += kPyBlock) raw[v] = pat[v];
+        }
+        __syncthreads();
+        double bv = -__builtin_inf();
+        int bi = INT_MAX, bn = INT_MAX;
+        for (int s0 = 0; s0 < qy; s0 += S) {
+            const int rows = qy - s0 < S ? qy - s0 : S;
+            // axis 0: item (row group g, column j), lanes along j
+            const int groups = (rows + kPyV - 1) / kPyV;
+            for (int it = tid; it < groups * qx; it += kPyBlock) {
+                const int g = it / qx, j = it - g * qx;
+                double acc[kPyV];
+                filter_run(acc, s0 + g * kPyV, qy, R, wl, centre, [&](int k) { return (double)raw[k * qx + j]; });
+#pragma unroll
+                for (int v = 0; v < kPyV; ++v)
+                    if (g * kPyV + v < rows) tr[j * SP + g * kPyV + v] = acc[v];
+            }
+            __syncthreads();
+            // axis 1 and the argmax: item (column group c, strip row il), lanes along il
+            const int cgroups = (qx + kPyV - 1) / kPyV;
+            for (int it = tid; it < cgroups * rows; it += kPyBlock) {
+                const int c = it / rows, il = it - c * rows;
+                double acc[kPyV];
+                filter_run(acc, c * kPyV, qx, R, wl, centre, [&](int k) { return tr[k * SP + il]; });
+#pragma unroll
+                for (int v = 0; v < kPyV; ++v) {
+                    const int j = c * kPyV + v;
+                    if (j < qx) {
+                        const int idx = (s0 + il) * qx + j;
+                        if (a.filtered && pos == 0) a.filtered[idx] = acc[v];
+                        if (__builtin_isnan(acc[v])) {
+                            bn = idx < bn ? idx : bn;
+                        } else if (bi == INT_MAX || acc[v] > bv || (acc[v] == bv && idx < bi)) {
+                            bv = acc[v];
+                            bi = idx;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // the first NaN, else the largest value with the smallest index: the same result in any merge order
+        argmax_merge_wave(bv, bi, bn);
+        if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; red_n[wave] = bn; }
+        __syncthreads();
+        if (tid == 0) {
+            double v = red_v[0];
+            int b = red_i[0], n = red_n[0];
+            for (int k = 1; k < kWaves; ++k) argmax_merge(v, b, n, red_v[k], red_i[k], red_n[k]);
+            best_at = n != INT_MAX ? n : (b != INT_MAX ? b : 0);
+        }
+        __syncthreads();
+        const int at = best_at;
+        const double i0 = (double)(at / qx), j0 = (double)(at % qx);
+        if (a.fast_center) {
+            if (tid == 0) { a.out_row[pos] = i0; a.out_col[pos] = j0; }
+            __syncthreads();
+            continue;
+        }
+        // get_CoM(dp * mask), pixels tid, tid + block, ... per thread as in py4_origin
+        double s = 0, sc = 0, r = 0, rc = 0, c = 0, cc = 0;
+        for (int idx = tid; idx < ndet; idx += kPyBlock) {
+            const int ii = idx / qx;
+            const double fi = (double)ii, fj = (double)(idx - ii * qx);
+            const double di = fi - i0, dj = fj - j0;
+            const double v = (double)raw[idx] * (__builtin_sqrt(di * di + dj * dj) < a.limit ? 1.0 : 0.0);
+            two_sum(s, sc, v);
+            two_sum(r, rc, fi * v);
+            two_sum(c, cc, fj * v);
+        }
+        // compensated pairs merged across the wave (TwoSum of the sums, corrections added), then the waves in order
+        two_sum_wave(s, sc);
+        two_sum_wave(r, rc);
+        two_sum_wave(c, cc);
+        if (lane == 0) {
+            red_s[0][wave] = s; red_s[1][wave] = sc; red_s[2][wave] = r;
+            red_s[3][wave] = rc; red_s[4][wave] = c; red_s[5][wave] = cc;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double S0 = 0, SC = 0, MR = 0, MRC = 0, MC = 0, MCC = 0;
+            for (int k = 0; k < kWaves; ++k) {
+                two_sum(S0, SC, red_s[0][k]); SC += red_s[1][k];
+                two_sum(MR, MRC, red_s[2][k]); MRC += red_s[3][k];
+                two_sum(MC, MCC, red_s[4][k]); MCC += red_s[5][k];
+            }
+            const double tot = __builtin_isfinite(S0) ? S0 + SC : S0;
+            const double mr = __builtin_isfinite(MR) ? MR + MRC : MR;
+            const double mc = __builtin_isfinite(MC) ? MC + MCC : MC;
+            a.out_row[pos] = mr / tot;
+            a.out_col[pos] = mc / tot;
+        }
+        __syncthreads();
+    }
+}
+
+// Strip rows and dynamic LDS bytes of py4_origin_lds for a detector (the largest strip that leaves room for two
+// blocks per CU, else for one), or 0 rows when the pattern does not fit.
+int py4_lds_strip(int qy, int qx, int R, bool u16, int *bytes) {
+    const int64_t raw = ((int64_t)qy * qx * (u16 ? 2 : 4) + 15) & ~(int64_t)15;
+    for (const int64_t budget : {(int64_t)78 * 1024, (int64_t)156 * 1024})
+        for (int S = 32; S >= 8; S /= 2) {
+            const int64_t b = (int64_t)qx * (S + 1) * 8 + (int64_t)(2 * R + 1) * 8 + raw;
+            if (b <= budget) {
+                *bytes = (int)b;
+                return S;
+            }
+        }
+    return 0;
 }
 
 // ---- iterated window (Float32) ----
@@ -270,6 +498,16 @@ int py4_blocks(int cus, int64_t npos) {
 
 hipError_t launch_py4_origin(const Py4Launch &a, int blocks, hipStream_t s) {
     if (a.npos <= 0) return hipSuccess;
+    int bytes = 0;
+    const int S = py4_lds_strip(a.qy, a.qx, a.R, a.u16, &bytes);
+    if (S > 0) {
+        const void *fn = a.u16 ? (const void *)py4_origin_lds<true> : (const void *)py4_origin_lds<false>;
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) return e;
+        if (a.u16) hipLaunchKernelGGL(py4_origin_lds<true>, dim3(blocks), dim3(kPyBlock), bytes, s, a, S);
+        else hipLaunchKernelGGL(py4_origin_lds<false>, dim3(blocks), dim3(kPyBlock), bytes, s, a, S);
+        return hipGetLastError();
+    }
     if (a.u16) hipLaunchKernelGGL(py4_origin<true>, dim3(blocks), dim3(kPyBlock), 0, s, a);
     else hipLaunchKernelGGL(py4_origin<false>, dim3(blocks), dim3(kPyBlock), 0, s, a);
     return hipGetLastError();
