@@ -147,25 +147,20 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin(Py4Launch a) {
 // computes kPyV consecutive outputs along the filtered axis from two register windows of the line (the values
 // left and right of the outputs), which slide kPyV taps at a time: two loads per kPyV outputs and taps instead of
 // two per output and tap. The operation sequence of every output is the reference's.
-constexpr int kPyV = 8;
+constexpr int kPyV = 16;
 
 // acc[v] = the filtered values at o .. o + kPyV - 1 of a line x(0 .. n - 1) with nearest-edge extension: x(o) times
 // the centre weight, then (x(o - t) + x(o + t)) * w[R - t] for t = R down to 1.
 template <typename X>
 __device__ __forceinline__ void filter_run(double (&acc)[kPyV], int o, int n, int R, const double *w, double centre,
                                            X x) {
-    auto at = [&](int k) { return x(k < 0 ? 0 : (k >= n ? n - 1 : k)); };
+    auto at = [&](int k) { return x(min(max(k, 0), n - 1)); };
 #pragma unroll
     for (int v = 0; v < kPyV; ++v) acc[v] = at(o + v) * centre;
     int t = R;
-    for (const int lead = R - R % kPyV; t > lead; --t) {       // the first R mod kPyV taps one by one
-        const double wt = w[R - t];
-#pragma unroll
-        for (int v = 0; v < kPyV; ++v) acc[v] += (at(o + v - t) + at(o + v + t)) * wt;
-    }
     if (t < 1) return;
-    // taps t0 .. t0 - kPyV + 1: left value of (v, s) is lb[v + s] = x(o - t0 + v + s), right rb[v - s + kPyV - 1] =
-    // x(o + t0 + v - s)
+    // the taps in groups t .. t - kPyV + 1 (the last group: t .. 1): left value of (output v, tap t - s) is
+    // lb[v + s] = x(o - t + v + s), right rb[v - s + kPyV - 1] = x(o + t + v - s)
     double lb[2 * kPyV - 1], rb[2 * kPyV - 1];
 #pragma unroll
     for (int k = 0; k < 2 * kPyV - 1; ++k) {
@@ -173,11 +168,14 @@ __device__ __forceinline__ void filter_run(double (&acc)[kPyV], int o, int n, in
         rb[k] = at(o + t - kPyV + 1 + k);
     }
     for (;;) {
+        const int count = t < kPyV ? t : kPyV;             // uniform
 #pragma unroll
         for (int sidx = 0; sidx < kPyV; ++sidx) {
-            const double wt = w[R - t + sidx];
+            if (sidx < count) {
+                const double wt = w[R - t + sidx];
 #pragma unroll
-            for (int v = 0; v < kPyV; ++v) acc[v] += (lb[v + sidx] + rb[v - sidx + kPyV - 1]) * wt;
+                for (int v = 0; v < kPyV; ++v) acc[v] += (lb[v + sidx] + rb[v - sidx + kPyV - 1]) * wt;
+            }
         }
         t -= kPyV;
         if (t < 1) break;
@@ -221,12 +219,15 @@ __device__ __forceinline__ void two_sum_wave(double &s, double &c) {
     }
 }
 
-template <bool U16>
-__global__ __launch_bounds__(kPyBlock) void py4_origin_lds(Py4Launch a, int S) {
+// POW2: qx = 1 << lqx (row offsets by shifts); S = 1 << lS rows per strip.
+template <bool U16, bool POW2>
+__global__ __launch_bounds__(kPyBlock) void py4_origin_lds(Py4Launch a, int lS, int lqx) {
     using T = typename std::conditional<U16, uint16_t, float>::type;
     extern __shared__ double py_lds[];
     const int tid = threadIdx.x;
-    const int qy = a.qy, qx = a.qx, R = a.R, ndet = qy * qx, SP = S + 1;
+    const int qy = a.qy, qx = a.qx, R = a.R, ndet = qy * qx, S = 1 << lS, SP = S + 1;
+    auto row_at = [&](int k) { return POW2 ? k << lqx : k * qx; };     // k * qx
+    auto col_at = [&](int k) { return (k << lS) + k; };                  // k * (S + 1)
     T *raw = (T *)py_lds;                                  // [ndet], padded to 16 bytes
     double *tr = py_lds + ((ndet * (int)sizeof(T) + 15) & ~15) / 8;   // [qx][S + 1]: axis 0, transposed
     double *wl = tr + qx * SP;                             // [2R + 1]
@@ -262,10 +263,10 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin_lds(Py4Launch a, int S) {
             for (int it = tid; it < groups * qx; it += kPyBlock) {
                 const int g = it / qx, j = it - g * qx;
                 double acc[kPyV];
-                filter_run(acc, s0 + g * kPyV, qy, R, wl, centre, [&](int k) { return (double)raw[k * qx + j]; });
+                filter_run(acc, s0 + g * kPyV, qy, R, wl, centre, [&](int k) { return (double)raw[row_at(k) + j]; });
 #pragma unroll
                 for (int v = 0; v < kPyV; ++v)
-                    if (g * kPyV + v < rows) tr[j * SP + g * kPyV + v] = acc[v];
+                    if (g * kPyV + v < rows) tr[col_at(j) + g * kPyV + v] = acc[v];
             }
             __syncthreads();
             // axis 1 and the argmax: item (column group c, strip row il), lanes along il
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin_lds(Py4Launch a, int S) {
             for (int it = tid; it < cgroups * rows; it += kPyBlock) {
                 const int c = it / rows, il = it - c * rows;
                 double acc[kPyV];
-                filter_run(acc, c * kPyV, qx, R, wl, centre, [&](int k) { return tr[k * SP + il]; });
+                filter_run(acc, c * kPyV, qx, R, wl, centre, [&](int k) { return tr[col_at(k) + il]; });
 #pragma unroll
                 for (int v = 0; v < kPyV; ++v) {
                     const int j = c * kPyV + v;
@@ -501,12 +502,17 @@ hipError_t launch_py4_origin(const Py4Launch &a, int blocks, hipStream_t s) {
     int bytes = 0;
     const int S = py4_lds_strip(a.qy, a.qx, a.R, a.u16, &bytes);
     if (S > 0) {
-        const void *fn = a.u16 ? (const void *)py4_origin_lds<true> : (const void *)py4_origin_lds<false>;
-        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        if (e != hipSuccess) return e;
-        if (a.u16) hipLaunchKernelGGL(py4_origin_lds<true>, dim3(blocks), dim3(kPyBlock), bytes, s, a, S);
-        else hipLaunchKernelGGL(py4_origin_lds<false>, dim3(blocks), dim3(kPyBlock), bytes, s, a, S);
-        return hipGetLastError();
+        const int lS = __builtin_ctz((unsigned)S);
+        const bool pow2 = (a.qx & (a.qx - 1)) == 0;
+        const int lqx = pow2 ? __builtin_ctz((unsigned)a.qx) : 0;
+        auto launch = [&](auto kernel) {
+            const hipError_t e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kPyBlock), bytes, s, a, lS, lqx);
+            return hipGetLastError();
+        };
+        if (a.u16) return pow2 ? launch(py4_origin_lds<true, true>) : launch(py4_origin_lds<true, false>);
+        return pow2 ? launch(py4_origin_lds<false, true>) : launch(py4_origin_lds<false, false>);
     }
     if (a.u16) hipLaunchKernelGGL(py4_origin<true>, dim3(blocks), dim3(kPyBlock), 0, s, a);
     else hipLaunchKernelGGL(py4_origin<false>, dim3(blocks), dim3(kPyBlock), 0, s, a);
