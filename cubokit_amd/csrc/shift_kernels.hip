@@ -70,18 +70,21 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
         }
     };
     // Entry e of a chunk's block copy belongs to corner e / kTileEntries (a compile-time constant per j).
-    half8 wreg[WPT];
-    auto fetch_w = [&](int chunk) {
+    // The forward keeps two chunks of weights in registers (fetched two chunks ahead); the adjoint, short of
+    // registers, one.
+    constexpr bool DEEP = !BWD;
+    half8 wreg[DEEP ? 2 : 1][WPT];
+    auto fetch_w = [&](int set, int chunk) {
 #pragma unroll
         for (int j = 0; j < WPT; ++j) {
             const int e = tid + j * 64 * WAVES;
             const int corner = (j * 64 * WAVES) / kTileEntries + (64 * WAVES >= kTileEntries ? 0 : tid / kTileEntries);
-            wreg[j] = wsrc[(size_t)t.slot[corner] * a.slot_stride + (size_t)chunk * kTileEntries + (e % kTileEntries)];
+            wreg[set][j] = wsrc[(size_t)t.slot[corner] * a.slot_stride + (size_t)chunk * kTileEntries + (e % kTileEntries)];
         }
     };
-    auto put_w = [&](int buf) {
+    auto put_w = [&](int set, int buf) {
 #pragma unroll
-        for (int j = 0; j < WPT; ++j) wlds[buf][tid + j * 64 * WAVES] = wreg[j];
+        for (int j = 0; j < WPT; ++j) wlds[buf][tid + j * 64 * WAVES] = wreg[set][j];
     };
 
     float16v acc[kCorners];
@@ -162,31 +165,42 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
         if ((chunk + 1) % kFlush == 0 || chunk + 1 == nchunks) flush();
     };
 
-    // Same pipeline as the mask stack: weights one chunk ahead through LDS (double-buffered, one barrier per
-    // chunk), data two chunks ahead in registers; weight loads issued before data loads.
+    // As the mask stack: weights through LDS (double-buffered, one barrier per chunk), data two chunks ahead in
+    // registers, weight loads issued before data loads. The weights of chunk c + 1 go to LDS at the end of step
+    // c; in the forward they were fetched during step c - 1 (a whole step of latency), in the adjoint during
+    // step c.
     u32x4 X0[4], X1[4], X2[4];
-    fetch_w(0);
+    fetch_w(0, 0);
     __builtin_amdgcn_sched_barrier(0);
     load_x(X0, 0);
     load_x(X1, min(1, nchunks - 1));
-    put_w(0);
+    put_w(0, 0);
+    if constexpr (DEEP) fetch_w(1, min(1, nchunks - 1));
     __syncthreads();                                   // also publishes wts
-    auto step = [&](const u32x4 (&xc)[4], u32x4 (&xn)[4], int c) {
-        fetch_w(min(c + 1, nchunks - 1));
+    // cur: the register set holding chunk c + 1 (the forward alternates: set (c + 1) & 1)
+    auto step = [&](const u32x4 (&xc)[4], u32x4 (&xn)[4], int c, int cur) {
+        if constexpr (DEEP) fetch_w(cur ^ 1, min(c + 2, nchunks - 1));
+        else fetch_w(0, min(c + 1, nchunks - 1));
         __builtin_amdgcn_sched_barrier(0);
         load_x(xn, min(c + 2, nchunks - 1));
         compute(xc, c);
-        put_w((c + 1) & 1);
+        put_w(DEEP ? cur : 0, (c + 1) & 1);
         lds_barrier();
     };
     int c = 0;
-    for (; c + 3 <= nchunks; c += 3) {
-        step(X0, X2, c);
-        step(X1, X0, c + 1);
-        step(X2, X1, c + 2);
+    for (; c + 6 <= nchunks; c += 6) {
+        step(X0, X2, c, 1);
+        step(X1, X0, c + 1, 0);
+        step(X2, X1, c + 2, 1);
+        step(X0, X2, c + 3, 0);
+        step(X1, X0, c + 4, 1);
+        step(X2, X1, c + 5, 0);
     }
-    if (c < nchunks) step(X0, X2, c);
-    if (c + 1 < nchunks) step(X1, X0, c + 1);
+    if (c < nchunks) step(X0, X2, c, 1);
+    if (c + 1 < nchunks) step(X1, X0, c + 1, 0);
+    if (c + 2 < nchunks) step(X2, X1, c + 2, 1);
+    if (c + 3 < nchunks) step(X0, X2, c + 3, 0);
+    if (c + 4 < nchunks) step(X1, X0, c + 4, 1);
 
     const int col = lane & 31;
     const double inv = col < a.k ? ((const CUBO_GLOBAL double *)a.inv_scale)[col] : 0.0;
