@@ -42,6 +42,37 @@ def test_pinned_radii_origins_and_filters():
         res.release()
 
 
+@pytest.mark.parametrize("dtype,det,radius", [(np.uint16, (13, 21), 2.6),     # 546 B patterns: staged per element
+                                              (np.float32, (70, 33), 6.4),    # several axis-0 strips, odd width
+                                              (np.float32, (64, 64), 9.9),    # power-of-two rows, Float32
+                                              (np.uint16, (40, 48), 17.3)])   # taps longer than the detector
+def test_filter_shapes_against_the_oracle(dtype, det, radius):
+    """get_origin on detector shapes the pins do not have: patterns staged element by element, several strips of
+    the axis-0 pass, odd and power-of-two rows, a filter longer than the pattern (every tap clamped); origins
+    within 1e-12 px of the oracle, the fast centre and the filtered first pattern as the oracle's."""
+    rng = np.random.default_rng(41)
+    qy, qx = det
+    R, Cs = 3, 4
+    ii, jj = np.meshgrid(np.arange(qy), np.arange(qx), indexing="ij")
+    cube = np.empty((R, Cs, qy, qx), dtype)
+    for p in range(R * Cs):
+        cy, cx = rng.uniform(0.3, 0.7) * qy, rng.uniform(0.3, 0.7) * qx
+        disk = np.where((ii - cy) ** 2 + (jj - cx) ** 2 < (0.25 * min(qy, qx)) ** 2, 900.0, 0.0)
+        cube.reshape(-1, qy, qx)[p] = (disk + rng.integers(0, 40, size=(qy, qx))).astype(dtype)
+    print(f"case: origin shape {dtype.__name__} {det} r={radius}", flush=True)
+    res = C.HIPResidentCube(cube)
+    omap, _ = C.OriginMeasurement.measure(res, radius, 1.2, tiling=C.ScanTiling.rows(2))
+    r, c = O.origin_py4dstem(cube, radius, 1.2)
+    assert np.max(np.abs(omap.row - r)) <= 1e-12 and np.max(np.abs(omap.column - c)) <= 1e-12
+    fast, _ = C.OriginMeasurement.measure(res, radius, fast_center=True)
+    fr, fc = O.origin_py4dstem(cube, radius, 1.2, fast_center=True)
+    np.testing.assert_array_equal(fast.row, fr)
+    np.testing.assert_array_equal(fast.column, fc)
+    got = origins.gaussian_filter(res, 0, 0, radius)
+    np.testing.assert_array_equal(got, O.gaussian_filter(cube[0, 0].astype(np.float64), radius))
+    res.release()
+
+
 def test_argmax_ties_boundaries_and_nan():
     """OriginFitTests.swift:50-73 and 88-105 on the device."""
     q = 9

@@ -405,3 +405,28 @@ def test_fourier_shift_groups_of_many_entries(dtype):
                 got = C.VirtualDiffraction.pattern(res, stat, scan, shift=shift, tiling=t).values
                 want = O.pattern(cube, stat.value, origin=(orow, ocol), fourier=True, **kw)
                 assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), f"{kind} {stat}"
+
+
+@pytest.mark.parametrize("det", [(32, 64), (16, 24)])          # the one-kernel path, batched FFTs
+def test_fourier_shift_with_a_nan_pixel(det):
+    """A NaN pixel spreads through its pattern's transform: selected, the Fourier-shifted mean and max are NaN
+    everywhere, as the oracle's; masked out, the others give the oracle's values within 1e-12 of max|oracle|."""
+    rng = np.random.default_rng(31)
+    qy, qx = det
+    R, Cs = 4, 5
+    npos = R * Cs
+    cube = rng.normal(30, 10, size=(R, Cs, qy, qx)).astype(np.float32)
+    cube[2, 3, qy // 2, 1] = np.nan
+    orow, ocol = rng.normal(qy / 2, 2, npos), rng.normal(qx / 2, 2, npos)
+    shift = C.PatternShift.fourier_shift(C.OriginMap(orow, ocol))
+    res = C.HIPResidentCube(cube)
+    bad = 2 * Cs + 3
+    for stat in (MEAN, MAX):
+        print(f"case: fourier nan {det} {stat}", flush=True)
+        got = C.VirtualDiffraction.pattern(res, stat, C.ScanSelection.all(), shift=shift).values
+        want = O.pattern(cube, stat.value, origin=(orow, ocol), fourier=True)
+        assert np.all(np.isnan(want)) and np.all(np.isnan(got)), stat
+        keep = np.arange(npos) != bad
+        got = C.VirtualDiffraction.pattern(res, stat, C.ScanSelection.mask(keep), shift=shift).values
+        want = O.pattern(cube, stat.value, take=keep, origin=(orow, ocol), fourier=True)
+        assert np.all(np.isfinite(want)) and np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), stat
