@@ -198,7 +198,8 @@ int cubo_hip_batch_create(int32_t n, cubo_hip_cube *const *cubes, const float *c
         b->npos.push_back(c->npos);
         b->ndet.push_back(c->ndet);
         b->signature += std::to_string(c->npos) + "x" + std::to_string(c->ndet) + "x" + std::to_string(k[i]) + ";";
-        cubo::SplitWeights sw = cubo::split_weights(masks[i], k[i], c->ndet, 2);
+        // at most 32 masks: the one-tile layout, and the kernel multiplies one 32-mask tile; else two tiles
+        cubo::SplitWeights sw = cubo::split_weights(masks[i], k[i], c->ndet);
         (void)hipStreamSynchronize(c->stream);       // the cube's own uploads are done
         void *wb = nullptr;
         double *inv = nullptr, *y = nullptr;

@@ -219,7 +219,11 @@ __global__ __launch_bounds__(64 * WAVES) void batch_stack_mfma(const StackLaunch
             if (first[mid] <= item) lo = mid; else hi = mid - 1;
         }
         const StackLaunch a = cubes[lo];
-        stack_tile<NT, WAVES>(a, (item - first[lo]) * 32 * WAVES, sh);
+        // a cube of at most 32 masks has its weights in the one-tile layout: half the matrix-core work
+        if (NT == 2 && a.kn <= 32)
+            stack_tile<1, WAVES>(a, (item - first[lo]) * 32 * WAVES, *reinterpret_cast<StackLds<1> *>(&sh));
+        else
+            stack_tile<NT, WAVES>(a, (item - first[lo]) * 32 * WAVES, sh);
     }
 }
 

@@ -19,11 +19,10 @@ struct SplitWeights {
     int ntiles = 0, passes = 0, nchunks = 0;
 };
 
-// min_tiles = 2 lays a stack of at most 32 masks out as two tiles (the second all zero), the layout of the
-// batched kernel, which always runs two tiles.
-inline SplitWeights split_weights(const float *masks, int k, int64_t ndet, int min_tiles = 1) {
+// One 32-mask tile for at most 32 masks, else two per pass of 64.
+inline SplitWeights split_weights(const float *masks, int k, int64_t ndet) {
     SplitWeights s;
-    s.ntiles = std::max(k <= 32 ? 1 : 2, min_tiles);
+    s.ntiles = k <= 32 ? 1 : 2;
     s.passes = (k + 63) / 64;
     s.nchunks = (int)((ndet + 63) / 64);
     std::vector<double> scale(k, 1.0);
