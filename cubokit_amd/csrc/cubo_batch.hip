@@ -26,6 +26,7 @@
 #include "../../include/cubo_hip.h"
 #include "cube.hpp"
 #include "errors.hpp"
+#include "host_math.hpp"
 #include "maskstack_kernels.hpp"
 #include "stack_prep.hpp"
 
@@ -288,9 +289,19 @@ int cubo_hip_batch_run(cubo_hip_batch *b, double *const *y) {
     CUBO_HIP_TRY(hipEventRecord(t1, b->stream), "hipEventRecord");
     b->timed.emplace_back(t0, t1);
     CUBO_HIP_TRY(hipStreamSynchronize(b->stream), "waiting for the batch");
-    if (y)
-        for (int i = 0; i < b->n; ++i)
-            if (y[i]) memcpy(y[i], b->yhost + b->yoff[i], (size_t)b->npos[i] * b->k[i] * sizeof(double));
+    if (y) {
+        // the pinned results (one contiguous range, cube after cube) into the caller's arrays, in pieces on
+        // several host threads
+        const size_t total = b->ytotal * sizeof(double);
+        cubo::parallel_pieces(total, cubo::reader_threads(), [&](size_t off, size_t len) {
+            for (int i = 0; i < b->n; ++i) {
+                const size_t lo = b->yoff[i] * sizeof(double), hi = lo + (size_t)b->npos[i] * b->k[i] * sizeof(double);
+                const size_t s0 = std::max(lo, off), s1 = std::min(hi, off + len);
+                if (y[i] && s0 < s1) memcpy((char *)y[i] + (s0 - lo), (const char *)b->yhost + s0, s1 - s0);
+            }
+            return true;
+        });
+    }
     return CUBO_HIP_OK;
 }
 

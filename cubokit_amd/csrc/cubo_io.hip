@@ -24,6 +24,7 @@
 #include "../../include/cubo_hip.h"
 #include "cube.hpp"
 #include "errors.hpp"
+#include "host_math.hpp"
 
 using cubo::DeviceGuard;
 using cubo::element_size;
@@ -93,27 +94,6 @@ hipError_t launch_to_f32(int t, const void *src, float *dst, int64_t n, hipStrea
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
-}
-
-int reader_threads() {
-    const unsigned h = std::thread::hardware_concurrency();
-    return (int)std::max(1u, std::min(8u, h ? h : 1u));
-}
-
-// Runs fn(offset, bytes) over [0, n) in `threads` contiguous pieces (4 KiB aligned), in parallel.
-template <typename Fn>
-bool parallel_pieces(size_t n, int threads, Fn fn) {
-    const size_t piece = std::max<size_t>(((n / threads) + 4095) & ~(size_t)4095, 4096);
-    std::vector<std::thread> pool;
-    std::vector<char> ok((n + piece - 1) / piece, 1);
-    size_t k = 0;
-    for (size_t off = 0; off < n; off += piece, ++k) {
-        const size_t len = std::min(piece, n - off);
-        if (off + len >= n && pool.empty()) { ok[k] = fn(off, len); break; }   // one piece: no thread
-        pool.emplace_back([&, off, len, k] { ok[k] = fn(off, len); });
-    }
-    for (auto &t : pool) t.join();
-    return std::all_of(ok.begin(), ok.end(), [](char v) { return v != 0; });
 }
 
 int make_feed(cubo_hip_cube *c) {

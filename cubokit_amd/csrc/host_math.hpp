@@ -1,10 +1,13 @@
-// host_math.hpp — host-side arithmetic shared by the C ABI translation units: NumPy's pairwise sum (the mean of
-// an origin map) and the wrap of an integral shift into a detector axis.
+// host_math.hpp — host-side helpers shared by the C ABI translation units: NumPy's pairwise sum (the mean of an
+// origin map), the wrap of an integral shift into a detector axis, and a pool for large host copies.
 #pragma once
 
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
+#include <thread>
+#include <vector>
 
 namespace cubo {
 
@@ -37,6 +40,28 @@ inline int wrap_shift(double v, int n) {
     double m = std::fmod(v, (double)n);
     if (m < 0) m += (double)n;
     return ((int)m) % n;
+}
+
+// Threads for large host copies and file reads: the hardware threads, at most 8.
+inline int reader_threads() {
+    const unsigned h = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(8u, h ? h : 1u));
+}
+
+// Runs fn(offset, bytes) over [0, n) in `threads` contiguous pieces (4 KiB aligned), in parallel.
+template <typename Fn>
+bool parallel_pieces(size_t n, int threads, Fn fn) {
+    const size_t piece = std::max<size_t>(((n / threads) + 4095) & ~(size_t)4095, 4096);
+    std::vector<std::thread> pool;
+    std::vector<char> ok((n + piece - 1) / piece, 1);
+    size_t k = 0;
+    for (size_t off = 0; off < n; off += piece, ++k) {
+        const size_t len = std::min(piece, n - off);
+        if (off + len >= n && pool.empty()) { ok[k] = fn(off, len); break; }   // one piece: no thread
+        pool.emplace_back([&, off, len, k] { ok[k] = fn(off, len); });
+    }
+    for (auto &t : pool) t.join();
+    return std::all_of(ok.begin(), ok.end(), [](char v) { return v != 0; });
 }
 
 }  // namespace cubo
