@@ -2,8 +2,9 @@
 //
 // Many resident uint16 cubes of uneven scan and detector size, each with its own stack of up to 64 masks,
 // run in ONE persistent launch: every block takes (cube, tile) items from a work counter until none is
-// left. The counter reset (a one-thread kernel, never a memset), the persistent kernel and the copies of
-// every cube's results into pinned host memory are captured once in a hipGraph and replayed per run.
+// left. The counter reset (a one-thread kernel, never a memset) and the persistent kernel, which writes every
+// cube's results into mapped pinned host memory as its tiles finish (over PCIe, while the rest still streams),
+// are captured once in a hipGraph and replayed per run.
 //
 // The launch shape (waves per block, resident blocks per compute unit, item order) is chosen by an
 // autotuner that times every candidate on the device and keeps the fastest per batch signature for the
@@ -58,10 +59,10 @@ struct cubo_hip_batch {
     std::vector<int64_t> npos, ndet;
     std::vector<void *> wb;              // per cube: device FP16 hi/lo weights
     std::vector<double *> inv;           // per cube: device 1/scale
-    std::vector<double *> ydev;          // per cube: device results [npos][k]
     std::vector<size_t> yoff;            // per cube: offset in the pinned result buffer (doubles)
     std::vector<std::array<int, 2>> tile_lo;
-    double *yhost = nullptr;             // pinned
+    double *yhost = nullptr;             // pinned and mapped: the kernel writes every cube's [npos][k] here
+    double *ymap = nullptr;              // its device address
     size_t ytotal = 0;
     cubo::StackLaunch *launches = nullptr;   // device [n], in item order
     int64_t *first = nullptr;                // device [n]
@@ -92,7 +93,6 @@ void free_batch(cubo_hip_batch *b) {
     for (auto &ev : b->timed) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
     for (void *p : b->wb) if (p) (void)hipFree(p);
     for (double *p : b->inv) if (p) (void)hipFree(p);
-    for (double *p : b->ydev) if (p) (void)hipFree(p);
     if (b->launches) (void)hipFree(b->launches);
     if (b->first) (void)hipFree(b->first);
     if (b->counter) (void)hipFree(b->counter);
@@ -125,7 +125,7 @@ hipError_t upload_plan(cubo_hip_batch *b, const Shape &s) {
         a.inv_scale = b->inv[i];
         a.tile_lo[0] = b->tile_lo[i][0];
         a.tile_lo[1] = b->tile_lo[i][1];
-        a.y = b->ydev[i];
+        a.y = b->ymap + b->yoff[i];
         first[j] = t;
         t += (b->npos[i] + bm - 1) / bm;
     }
@@ -149,10 +149,8 @@ int build_graph(cubo_hip_batch *b) {
     drop_graph(b);
     if (hipError_t e = upload_plan(b, b->shape); e != hipSuccess) return hip_fail(e, "uploading the batch plan");
     CUBO_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    // the kernel writes the results straight into the mapped host buffer: no copy nodes
     hipError_t e = launch(b, b->shape);
-    for (int i = 0; i < b->n && e == hipSuccess; ++i)
-        e = hipMemcpyAsync(b->yhost + b->yoff[i], b->ydev[i], (size_t)b->npos[i] * b->k[i] * sizeof(double),
-                           hipMemcpyDeviceToHost, b->stream);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -203,15 +201,13 @@ int cubo_hip_batch_create(int32_t n, cubo_hip_cube *const *cubes, const float *c
         cubo::SplitWeights sw = cubo::split_weights(masks[i], k[i], c->ndet);
         (void)hipStreamSynchronize(c->stream);       // the cube's own uploads are done
         void *wb = nullptr;
-        double *inv = nullptr, *y = nullptr;
+        double *inv = nullptr;
         e = hipMalloc(&wb, sw.wb.size() * sizeof(_Float16));
         if (e == hipSuccess) e = hipMemcpy(wb, sw.wb.data(), sw.wb.size() * sizeof(_Float16), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc((void **)&inv, k[i] * sizeof(double));
         if (e == hipSuccess) e = hipMemcpy(inv, sw.inv.data(), k[i] * sizeof(double), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMalloc((void **)&y, (size_t)c->npos * k[i] * sizeof(double));
         b->wb.push_back(wb);
         b->inv.push_back(inv);
-        b->ydev.push_back(y);
         b->tile_lo.push_back({sw.tile_lo[0], sw.tile_lo[1]});
         b->yoff.push_back(b->ytotal);
         b->ytotal += (size_t)c->npos * k[i];
@@ -221,7 +217,8 @@ int cubo_hip_batch_create(int32_t n, cubo_hip_cube *const *cubes, const float *c
     if (e == hipSuccess) e = hipMalloc((void **)&b->first, n * sizeof(int64_t));
     if (e == hipSuccess) e = hipMalloc((void **)&b->counter, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->yhost, std::max<size_t>(b->ytotal, 1) * sizeof(double),
-                                           hipHostMallocDefault);
+                                           hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&b->ymap, b->yhost, 0);
     if (e != hipSuccess) { free_batch(b); return hip_fail(e, "allocating the batch"); }
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
