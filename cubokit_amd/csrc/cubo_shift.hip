@@ -391,42 +391,24 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
         CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, true, false, s), "launching the origin adjoint");
         CUBO_HIP_TRY(hipEventRecord(p->ev[3], s), "hipEventRecord");
     }
-    bool dy_finite = true;
-    for (int64_t i = 0; i < npos * p->k; ++i) dy_finite = dy_finite && std::isfinite(dy[i]);
-    if (dmasks && dy_finite && cubo::shift_grad_rows_supported((int)c->shape[3])) {
-        // dL/dY split for the matrix cores (a non-finite dL/dY takes the vector-ALU path, which propagates it): each column scaled by a power of two so its largest value lies in
-        // [1024, 2048), then FP16 hi + lo, in the B-operand order [k-step][hi, lo][lane][8]: lane l holds mask
-        // l % 32 and positions 16 s + 8 (l / 32) + e. Zero past the positions and the masks.
+    if (dmasks && cubo::shift_grad_rows_supported((int)c->shape[3])) {
+        // dL/dY split for the matrix cores on the device (split_dy): each column scaled by a power of two so its
+        // largest value lies in [1024, 2048), then FP16 hi + lo in the B-operand order
         const int64_t nks = (npos + 15) / 16;
+        std::vector<double> mx(p->k, 0.0);
+        for (int64_t i = 0; i < npos; ++i)
+            for (int kk = 0; kk < p->k; ++kk) mx[kk] = std::max(mx[kk], std::fabs(dy[(size_t)i * p->k + kk]));
         std::vector<double> inv(32, 0.0), scale(32, 0.0);
-        for (int kk = 0; kk < p->k; ++kk) {
-            double mx = 0;
-            for (int64_t i = 0; i < npos; ++i) mx = std::max(mx, std::fabs(dy[(size_t)i * p->k + kk]));
-            if (mx > 0) {
-                const int ex = (int)std::floor(std::log2(mx));
+        for (int kk = 0; kk < p->k; ++kk)
+            if (mx[kk] > 0) {
+                const int ex = (int)std::floor(std::log2(mx[kk]));
                 scale[kk] = std::ldexp(1.0, 10 - ex);
                 inv[kk] = std::ldexp(1.0, ex - 10);
-            }
-        }
-        std::vector<_Float16> dyw((size_t)nks * 2 * 64 * 8, (_Float16)0.f);
-        for (int64_t ks = 0; ks < nks; ++ks)
-            for (int l = 0; l < 64; ++l) {
-                const int kk = l & 31;
-                if (kk >= p->k) continue;
-                for (int e = 0; e < 8; ++e) {
-                    const int64_t pos = ks * 16 + 8 * (l >> 5) + e;
-                    if (pos >= npos) continue;
-                    const double v = dy[(size_t)pos * p->k + kk] * scale[kk];
-                    const _Float16 hi = (_Float16)(float)v;
-                    const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
-                    dyw[((ks * 2 + 0) * 64 + l) * 8 + e] = hi;
-                    dyw[((ks * 2 + 1) * 64 + l) * 8 + e] = lo;
-                }
             }
         const int qy = (int)c->shape[2];
         const int cus = cubo::compute_units(c->device);
         const int ngroups = (int)std::max<int64_t>(1, std::min<int64_t>({(4 * cus + qy - 1) / qy, 64, nks}));
-        const size_t dyw_bytes = dyw.size() * sizeof(_Float16);
+        const size_t dyw_bytes = (size_t)nks * 2 * 64 * 8 * sizeof(_Float16);
         if (p->dyw_bytes < dyw_bytes) {
             if (p->dyw) (void)hipFree(p->dyw);
             p->dyw = nullptr;
@@ -434,7 +416,7 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
             CUBO_HIP_TRY(hipMalloc(&p->dyw, dyw_bytes), "allocating dL/dY");
             p->dyw_bytes = dyw_bytes;
         }
-        if (!p->dy_inv) CUBO_HIP_TRY(hipMalloc((void **)&p->dy_inv, 32 * 8), "allocating dL/dY scales");
+        if (!p->dy_inv) CUBO_HIP_TRY(hipMalloc((void **)&p->dy_inv, 64 * 8), "allocating dL/dY scales");
         if (ngroups > p->rgroups) {
             if (p->rpartial) (void)hipFree(p->rpartial);
             p->rpartial = nullptr;
@@ -443,8 +425,9 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
             p->rgroups = ngroups;
         }
         if (!p->rdmask) CUBO_HIP_TRY(hipMalloc((void **)&p->rdmask, (size_t)32 * ndet * 8), "allocating dL/dmasks");
-        CUBO_HIP_TRY(hipMemcpyAsync(p->dyw, dyw.data(), dyw_bytes, hipMemcpyHostToDevice, s), "uploading dL/dY");
-        CUBO_HIP_TRY(hipMemcpyAsync(p->dy_inv, inv.data(), 32 * 8, hipMemcpyHostToDevice, s), "uploading dL/dY");
+        inv.insert(inv.end(), scale.begin(), scale.end());          // dy_inv: [32] 1 / scale, then [32] scale
+        CUBO_HIP_TRY(hipMemcpyAsync(p->dy_inv, inv.data(), 64 * 8, hipMemcpyHostToDevice, s), "uploading dL/dY scales");
+        CUBO_HIP_TRY(cubo::launch_split_dy(p->dy, npos, p->k, p->dy_inv + 32, p->dyw, s), "splitting dL/dY");
         cubo::ShiftGradRows gw{};
         gw.cube = (const uint16_t *)c->data;
         gw.npos = npos;

@@ -511,6 +511,28 @@ hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, b
 
 bool shift_grad_rows_supported(int qx) { return qx % 32 == 0 && qx >= 32 && qx <= 256; }
 
+namespace {
+__global__ void split_dy(const double *dy, int64_t npos, int k, const double *scale, _Float16 *dyw, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;     // (k-step, lane, e)
+    if (i >= n) return;
+    const int64_t ks = i / 512;
+    const int l = (int)((i / 8) % 64), e = (int)(i % 8), kk = l & 31;
+    const int64_t pos = ks * 16 + 8 * (l >> 5) + e;
+    const double v = (kk < k && pos < npos) ? dy[pos * k + kk] * scale[kk] : 0.0;
+    const _Float16 hi = (_Float16)(float)v;
+    const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
+    dyw[((ks * 2 + 0) * 64 + l) * 8 + e] = hi;
+    dyw[((ks * 2 + 1) * 64 + l) * 8 + e] = lo;
+}
+}  // namespace
+
+hipError_t launch_split_dy(const double *dy, int64_t npos, int k, const double *scale, void *dyw, hipStream_t s) {
+    const int64_t n = (npos + 15) / 16 * 512;
+    hipLaunchKernelGGL(split_dy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dy, npos, k, scale,
+                       (_Float16 *)dyw, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_shift_grad_rows(const ShiftGradRows &a, hipStream_t s) {
     if (!shift_grad_rows_supported(a.qx) || a.ngroups <= 0) return hipErrorInvalidValue;
     const dim3 grid((unsigned)a.qy, (unsigned)a.ngroups);

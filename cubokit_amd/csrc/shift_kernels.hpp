@@ -101,6 +101,10 @@ struct ShiftGradRows {
     int64_t ppg;              // positions per group, a multiple of 16
 };
 bool shift_grad_rows_supported(int qx);
+// The B operand of shift_grad_rows from dL/dY on the device: column kk of dy [npos][k] times scale[kk] (a power of
+// two), rounded to Float then split into FP16 hi and lo, at [k-step][hi, lo][lane][8] (lane l: mask l % 32,
+// positions 16 s + 8 (l / 32) + e), zero past the positions and the masks.
+hipError_t launch_split_dy(const double *dy, int64_t npos, int k, const double *scale, void *dyw, hipStream_t s);
 hipError_t launch_shift_grad_rows(const ShiftGradRows &a, hipStream_t s);
 // out[i] = sum_g partial[g][i] in group order, i < n.
 hipError_t launch_sum_groups(const double *partial, int ngroups, int64_t n, double *out, hipStream_t s);
