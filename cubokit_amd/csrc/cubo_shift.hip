@@ -55,7 +55,8 @@ struct cubo_hip_shift_plan {
     double *wr = nullptr, *wc = nullptr;
     float *wts = nullptr;
     // per-call buffers
-    double *y = nullptr, *dy = nullptr, *g = nullptr;    // g: [2][npos]
+    double *yhost = nullptr, *y = nullptr;   // Y [npos][k]: pinned host memory the kernels write, its device address
+    double *dy = nullptr, *g = nullptr;      // g: [2][npos]
     float *dy32 = nullptr;
     void *dyw = nullptr;             // dL/dY split into FP16 hi/lo for the matrix-core mask adjoint
     size_t dyw_bytes = 0;
@@ -76,13 +77,30 @@ void free_plan(cubo_hip_shift_plan *p) {
     DeviceGuard g(p->cube->device);
     (void)hipStreamSynchronize(p->cube->stream);
     void *bufs[] = {p->base, p->inv, p->wroll, p->perm, p->shift, p->slot_shifts, p->tiles, p->wr, p->wc, p->wts,
-                    p->y, p->dy, p->g, p->dy32, p->partial, p->dmask, p->vmasks, p->vshift, p->vwr, p->vwc,
+                    p->dy, p->g, p->dy32, p->partial, p->dmask, p->vmasks, p->vshift, p->vwr, p->vwc,
                     p->dyw, p->dy_inv, p->rpartial, p->rdmask};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    if (p->yhost) (void)hipHostFree(p->yhost);
     for (hipEvent_t e : p->ev)
         if (e) (void)hipEventDestroy(e);
     delete p;
+}
+
+// Y in mapped pinned host memory: the forward kernels write it over PCIe while they stream the cube.
+hipError_t ensure_y(cubo_hip_shift_plan *p, size_t ny) {
+    if (p->yhost) return hipSuccess;
+    hipError_t e = hipHostMalloc((void **)&p->yhost, ny * 8, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&p->y, p->yhost, 0);
+    return e;
+}
+
+// The caller's copy of Y, on several host threads.
+void copy_y(const cubo_hip_shift_plan *p, double *y, size_t ny) {
+    cubo::parallel_pieces(ny * 8, cubo::reader_threads(), [&](size_t off, size_t len) {
+        memcpy((char *)y + off, (const char *)p->yhost + off, len);
+        return true;
+    });
 }
 
 template <typename T>
@@ -304,7 +322,7 @@ int forward_valu(cubo_hip_shift_plan *p, const double *orow, const double *ocol,
     if (!p->vshift) e = hipMalloc((void **)&p->vshift, (size_t)npos * 8);
     if (e == hipSuccess && !p->vwr) e = hipMalloc((void **)&p->vwr, (size_t)npos * 8);
     if (e == hipSuccess && !p->vwc) e = hipMalloc((void **)&p->vwc, (size_t)npos * 8);
-    if (e == hipSuccess && !p->y) e = hipMalloc((void **)&p->y, ny * 8);
+    if (e == hipSuccess) e = ensure_y(p, ny);
     if (e == hipSuccess) e = hipMemcpyAsync(p->vshift, sh.data(), (size_t)npos * 8, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(p->vwr, wr.data(), (size_t)npos * 8, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(p->vwc, wc.data(), (size_t)npos * 8, hipMemcpyHostToDevice, s);
@@ -325,10 +343,10 @@ int forward_valu(cubo_hip_shift_plan *p, const double *orow, const double *ocol,
         e = cubo::launch_shift_valu(a, rounded, s);
     }
     if (e == hipSuccess) e = hipEventRecord(p->ev[1], s);
-    if (e == hipSuccess) e = hipMemcpyAsync(y, p->y, ny * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);    // the host vectors go out of scope
     else (void)hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "the shifted stack");
+    copy_y(p, y, ny);
     p->ms[0] += elapsed(p->ev[0], p->ev[1]);
     return CUBO_HIP_OK;
 }
@@ -340,14 +358,14 @@ int forward(cubo_hip_shift_plan *p, const double *origin_row, const double *orig
     cubo_hip_cube *c = p->cube;
     DeviceGuard g(c->device);
     const size_t ny = (size_t)c->npos * p->k;
-    if (!p->y) CUBO_HIP_TRY(hipMalloc((void **)&p->y, ny * 8), "allocating the stack output");
+    CUBO_HIP_TRY(ensure_y(p, ny), "allocating the stack output");
     cubo::ShiftLaunch a = launch_args(p);
     a.y = p->y;
     CUBO_HIP_TRY(hipEventRecord(p->ev[0], c->stream), "hipEventRecord");
     CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, false, rounded, c->stream), "launching the shifted stack");
     CUBO_HIP_TRY(hipEventRecord(p->ev[1], c->stream), "hipEventRecord");
-    CUBO_HIP_TRY(hipMemcpyAsync(y, p->y, ny * 8, hipMemcpyDeviceToHost, c->stream), "reading the stack");
     CUBO_HIP_TRY(hipStreamSynchronize(c->stream), "waiting for the shifted stack");
+    copy_y(p, y, ny);
     p->ms[0] += elapsed(p->ev[0], p->ev[1]);
     return CUBO_HIP_OK;
 }
