@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/cubo_hip.h"
@@ -57,6 +58,7 @@ struct cubo_hip_shift_plan {
     // per-call buffers
     double *yhost = nullptr, *y = nullptr;   // Y [npos][k]: pinned host memory the kernels write, its device address
     double *dy = nullptr, *g = nullptr;      // g: [2][npos]
+    double *dyhost = nullptr;                // pinned staging of dL/dY
     float *dy32 = nullptr;
     void *dyw = nullptr;             // dL/dY split into FP16 hi/lo for the matrix-core mask adjoint
     size_t dyw_bytes = 0;
@@ -82,6 +84,7 @@ void free_plan(cubo_hip_shift_plan *p) {
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->yhost) (void)hipHostFree(p->yhost);
+    if (p->dyhost) (void)hipHostFree(p->dyhost);
     for (hipEvent_t e : p->ev)
         if (e) (void)hipEventDestroy(e);
     delete p;
@@ -93,6 +96,35 @@ hipError_t ensure_y(cubo_hip_shift_plan *p, size_t ny) {
     hipError_t e = hipHostMalloc((void **)&p->yhost, ny * 8, hipHostMallocMapped | hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&p->y, p->yhost, 0);
     return e;
+}
+
+// One pass over dL/dY [npos][k] on host threads: copied into the pinned staging buffer, checked finite, and each
+// column's largest magnitude. False if a value is not finite.
+bool stage_dy(cubo_hip_shift_plan *p, const double *dy, int64_t npos, std::vector<double> &colmax) {
+    const int k = p->k, T = (int)std::min<int64_t>(cubo::reader_threads(), std::max<int64_t>(1, npos / 4096));
+    std::vector<std::vector<double>> mx(T, std::vector<double>(k, 0.0));
+    std::vector<char> ok(T, 1);
+    auto run = [&](int t) {
+        const int64_t i0 = npos * t / T, i1 = npos * (t + 1) / T;
+        double *m = mx[t].data();
+        bool fin = true;
+        for (int64_t i = i0; i < i1; ++i)
+            for (int kk = 0; kk < k; ++kk) {
+                const double v = dy[(size_t)i * k + kk];
+                p->dyhost[(size_t)i * k + kk] = v;
+                fin = fin && std::isfinite(v);
+                m[kk] = std::max(m[kk], std::fabs(v));
+            }
+        ok[t] = fin;
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(run, t);
+    run(0);
+    for (auto &th : pool) th.join();
+    colmax.assign(k, 0.0);
+    for (int t = 0; t < T; ++t)
+        for (int kk = 0; kk < k; ++kk) colmax[kk] = std::max(colmax[kk], mx[t][kk]);
+    return std::all_of(ok.begin(), ok.end(), [](char v) { return v != 0; });
 }
 
 // The caller's copy of Y, on several host threads.
@@ -391,14 +423,15 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
                     "length is a multiple of 8");
     cubo_hip_cube *c = p->cube;
     const int64_t npos = c->npos, ndet = c->ndet;
-    for (int64_t i = 0; i < npos * p->k; ++i)
-        if (!std::isfinite(dy[i])) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "dL/dY must be finite");
-    if (int st = prepare(p, origin_row, origin_col)) return st;
-    DeviceGuard g(c->device);
-    hipStream_t s = c->stream;
     const size_t ny = (size_t)npos * p->k;
+    DeviceGuard g(c->device);
+    if (!p->dyhost) CUBO_HIP_TRY(hipHostMalloc((void **)&p->dyhost, ny * 8, hipHostMallocDefault), "allocating dL/dY");
+    std::vector<double> colmax;
+    if (!stage_dy(p, dy, npos, colmax)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "dL/dY must be finite");
+    if (int st = prepare(p, origin_row, origin_col)) return st;
+    hipStream_t s = c->stream;
     if (!p->dy) CUBO_HIP_TRY(hipMalloc((void **)&p->dy, ny * 8), "allocating dL/dY");
-    CUBO_HIP_TRY(hipMemcpyAsync(p->dy, dy, ny * 8, hipMemcpyHostToDevice, s), "uploading dL/dY");
+    CUBO_HIP_TRY(hipMemcpyAsync(p->dy, p->dyhost, ny * 8, hipMemcpyHostToDevice, s), "uploading dL/dY");
     if (d_origin_row) {
         if (!p->g) CUBO_HIP_TRY(hipMalloc((void **)&p->g, (size_t)npos * 16), "allocating the origin gradient");
         cubo::ShiftLaunch a = launch_args(p);
@@ -413,13 +446,10 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
         // dL/dY split for the matrix cores on the device (split_dy): each column scaled by a power of two so its
         // largest value lies in [1024, 2048), then FP16 hi + lo in the B-operand order
         const int64_t nks = (npos + 15) / 16;
-        std::vector<double> mx(p->k, 0.0);
-        for (int64_t i = 0; i < npos; ++i)
-            for (int kk = 0; kk < p->k; ++kk) mx[kk] = std::max(mx[kk], std::fabs(dy[(size_t)i * p->k + kk]));
         std::vector<double> inv(32, 0.0), scale(32, 0.0);
         for (int kk = 0; kk < p->k; ++kk)
-            if (mx[kk] > 0) {
-                const int ex = (int)std::floor(std::log2(mx[kk]));
+            if (colmax[kk] > 0) {
+                const int ex = (int)std::floor(std::log2(colmax[kk]));
                 scale[kk] = std::ldexp(1.0, 10 - ex);
                 inv[kk] = std::ldexp(1.0, ex - 10);
             }
