@@ -469,6 +469,19 @@ __device__ __forceinline__ void dit4(double2 &x0, double2 &x1, double2 &x2, doub
     x3 = csub(p1, q3i);   // j + 3h
 }
 
+// Bank-spreading permutations of the fused kernel's LDS (each an XOR of higher bits into lower ones, so a
+// bijection within aligned blocks): logical row r of the half spectrum is stored as row swz_row(r), and element i of
+// a packed row in the split layout at swz_el(i). With them the lanes of every radix-4 stage of a 128x128 pattern
+// reach distinct banks (rows 4b + m, 16 (b >> 2) + (b & 3) + 4m, ... of one column or packed row).
+__device__ __forceinline__ int swz_row(int r) {
+    const int g = (r >> 4) & 3;
+    return r ^ (g | (g << 2));
+}
+__device__ __forceinline__ int swz_el(int i) {
+    const int g = (i >> 5) & 3;
+    return i ^ (g | (g << 2)) ^ (((i >> 6) & 1) << 4);
+}
+
 int fourier_fused_lds(int qy, int qx) {
     const int h = qx / 2 + 1, l = qy > qx ? qy : qx;
     return 8 * (qy * (qx + 2) + 2 * qy + 2 * h + 2 * l + 2);
@@ -498,6 +511,10 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
     const int NC = wave < H ? (H - wave + kFfWaves - 1) / kFfWaves : 0;   // columns of this wave
     const int nchunks = NP * 2 * M / VE;
     const int J = NP * 2 * M / 64;                       // pixels per lane (qx >= 32: whole 64-pixel slices)
+    // packed pair q (logical rows 2q, 2q + 1) lives in the region of stored rows swz_row(2q) and its neighbour;
+    // flip: logical row 2q is the region's second stored row
+    auto region = [&](int q) { return (swz_row(2 * q) >> 1) * 2 * R2; };
+    auto flip = [&](int q) { return swz_row(2 * q) & 1; };
     const int64_t e_begin = a.first + (int64_t)blockIdx.x * a.ppg;
     const int64_t e_end = min(a.first + a.count, e_begin + a.ppg);
     for (int i = tid; i < L; i += kFfBlock) TW[i] = a.tw[i];
@@ -529,8 +546,8 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
         const int h = 1 << lh;
         for (int u = lane; u < NP * (M / 2); u += 64) {
             const int t = u >> (LM - 1), b = u & (M / 2 - 1), j = b & (h - 1);
-            const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
-            double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
+            const int i0 = swz_el(((b >> lh) << (lh + 1)) + j), i1 = swz_el(((b >> lh) << (lh + 1)) + j + h);
+            double *re = S + region(wave + kFfWaves * t), *im = re + R2;
             const double2 x = make_double2(re[i0], im[i0]), y = make_double2(re[i1], im[i1]);
             double2 d = csub(x, y);
             if (h > 1) d = cmul(d, TW[j << (LL - 1 - lh)]);
@@ -545,8 +562,8 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
         const int h = 1 << lh;
         for (int u = lane; u < NP * (M / 2); u += 64) {
             const int t = u >> (LM - 1), b = u & (M / 2 - 1), j = b & (h - 1);
-            const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
-            double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
+            const int i0 = swz_el(((b >> lh) << (lh + 1)) + j), i1 = swz_el(((b >> lh) << (lh + 1)) + j + h);
+            double *re = S + region(wave + kFfWaves * t), *im = re + R2;
             const double2 x = make_double2(re[i0], im[i0]);
             double2 y = make_double2(re[i1], im[i1]);
             if (h > 1) y = cmul_conj(y, TW[j << (LL - 1 - lh)]);
@@ -562,16 +579,17 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
         for (int u = lane; u < NP * (M / 4); u += 64) {
             const int t = u >> (LM - 2), b = u & (M / 4 - 1), j = b & (h - 1);
             const int i0 = ((b >> lq) << (lq + 2)) + j;
-            double *re = S + (wave + kFfWaves * t) * 2 * R2 + i0, *im = re + R2;
-            double2 x0 = make_double2(re[0], im[0]), x1 = make_double2(re[h], im[h]);
-            double2 x2 = make_double2(re[2 * h], im[2 * h]), x3 = make_double2(re[3 * h], im[3 * h]);
+            const int e0 = swz_el(i0), e1 = swz_el(i0 + h), e2 = swz_el(i0 + 2 * h), e3 = swz_el(i0 + 3 * h);
+            double *re = S + region(wave + kFfWaves * t), *im = re + R2;
+            double2 x0 = make_double2(re[e0], im[e0]), x1 = make_double2(re[e1], im[e1]);
+            double2 x2 = make_double2(re[e2], im[e2]), x3 = make_double2(re[e3], im[e3]);
             const int ts = LL - lq - 2;
             if (inverse) dit4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], h > 1);
             else dif4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], TW[(3 * j) << ts], h > 1);
-            re[0] = x0.x; im[0] = x0.y;
-            re[h] = x1.x; im[h] = x1.y;
-            re[2 * h] = x2.x; im[2 * h] = x2.y;
-            re[3 * h] = x3.x; im[3 * h] = x3.y;
+            re[e0] = x0.x; im[e0] = x0.y;
+            re[e1] = x1.x; im[e1] = x1.y;
+            re[e2] = x2.x; im[e2] = x2.y;
+            re[e3] = x3.x; im[e3] = x3.y;
         }
         wave_sync();
     };
@@ -583,12 +601,12 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
         for (int u = lane; u < NC * (N / 2); u += 64) {
             const int i = u >> (LN - 1), b = u & (N / 2 - 1), j = b & (h - 1);
             const int k = wave + kFfWaves * i;
-            const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
-            const double2 x = C[i0 * H + k], y = C[i1 * H + k];
+            const int r0 = swz_row(((b >> lh) << (lh + 1)) + j), r1 = swz_row(((b >> lh) << (lh + 1)) + j + h);
+            const double2 x = C[r0 * H + k], y = C[r1 * H + k];
             double2 d = csub(x, y);
             if (h > 1) d = cmul(d, TW[j << (LL - 1 - lh)]);
-            C[i0 * H + k] = cadd(x, y);
-            C[i1 * H + k] = d;
+            C[r0 * H + k] = cadd(x, y);
+            C[r1 * H + k] = d;
         }
         wave_sync();
     };
@@ -597,12 +615,12 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
         for (int u = lane; u < NC * (N / 2); u += 64) {
             const int i = u >> (LN - 1), b = u & (N / 2 - 1), j = b & (h - 1);
             const int k = wave + kFfWaves * i;
-            const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
-            const double2 x = C[i0 * H + k];
-            double2 y = C[i1 * H + k];
+            const int r0 = swz_row(((b >> lh) << (lh + 1)) + j), r1 = swz_row(((b >> lh) << (lh + 1)) + j + h);
+            const double2 x = C[r0 * H + k];
+            double2 y = C[r1 * H + k];
             if (h > 1) y = cmul_conj(y, TW[j << (LL - 1 - lh)]);
-            C[i0 * H + k] = cadd(x, y);
-            C[i1 * H + k] = csub(x, y);
+            C[r0 * H + k] = cadd(x, y);
+            C[r1 * H + k] = csub(x, y);
         }
         wave_sync();
     };
@@ -613,8 +631,9 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             const int i = u >> (LN - 2), b = u & (N / 4 - 1), j = b & (h - 1);
             const int k = wave + kFfWaves * i;
             const int i0 = ((b >> lq) << (lq + 2)) + j;
-            double2 *c = C + i0 * H + k;
-            double2 x0 = c[0], x1 = c[h * H], x2 = c[2 * h * H], x3 = c[3 * h * H];
+            double2 *c0 = C + swz_row(i0) * H + k, *c1 = C + swz_row(i0 + h) * H + k;
+            double2 *c2 = C + swz_row(i0 + 2 * h) * H + k, *c3 = C + swz_row(i0 + 3 * h) * H + k;
+            double2 x0 = *c0, x1 = *c1, x2 = *c2, x3 = *c3;
             const int ts = LL - lq - 2;
             if (inverse) {
                 dit4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], h > 1);
@@ -627,10 +646,10 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
                     x3 = cmul(x3, ramp(bit_reverse(i0 + 3, LN), k));
                 }
             }
-            c[0] = x0;
-            c[h * H] = x1;
-            c[2 * h * H] = x2;
-            c[3 * h * H] = x3;
+            *c0 = x0;
+            *c1 = x1;
+            *c2 = x2;
+            *c3 = x3;
         }
         wave_sync();
     };
@@ -648,7 +667,8 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
                 const int per = 2 * M / VE, t = u / per, cc = u - t * per;
                 const int q = wave + kFfWaves * t, off = cc * VE;
                 const int odd = off >= M ? 1 : 0;
-                double *dst = S + q * 2 * R2 + odd * R2 + (off - odd * M);
+                const int c0 = off - odd * M, x = swz_el(c0) ^ c0;   // the chunk's XOR (bits 5, 6 of c0)
+                double *dst = S + region(q) + odd * R2;
 #pragma unroll
                 for (int i = 0; i < VE; i += 2) {
                     double x0, x1;
@@ -661,7 +681,8 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
                         x0 = (double)__builtin_bit_cast(float, b0);
                         x1 = (double)__builtin_bit_cast(float, b1);
                     }
-                    *(double2 *)(dst + i) = make_double2(x0, x1);
+                    // elements c0 + i, c0 + i + 1 land on (c0 + i) ^ x and its neighbour: swapped when x is odd
+                    *(double2 *)(dst + ((c0 + i) ^ (x & ~1))) = (x & 1) ? make_double2(x1, x0) : make_double2(x0, x1);
                 }
             }
         }
@@ -710,13 +731,14 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
         // unpack: Z = FFT(row 2q + i row 2q+1) gives both half spectra, X0[k] = (Z[k] + conj Z[-k]) / 2 and
         // X1[k] = -i (Z[k] - conj Z[-k]) / 2, written interleaved
         for (int t = 0; t < NP; ++t) {
-            double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
+            const int q = wave + kFfWaves * t;
+            double *re = S + region(q), *im = re + R2;
             double2 z0[kFfItems], z1[kFfItems];
 #pragma unroll
             for (int i = 0; i < kFfItems; ++i) {
                 const int k = lane + 64 * i;
                 if (k <= M / 2) {
-                    const int pk = bit_reverse(k, LM), pm = bit_reverse((M - k) & (M - 1), LM);
+                    const int pk = swz_el(bit_reverse(k, LM)), pm = swz_el(bit_reverse((M - k) & (M - 1), LM));
                     z0[i] = make_double2(re[pk], im[pk]);
                     z1[i] = make_double2(re[pm], im[pm]);
                 }
@@ -726,8 +748,10 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             for (int i = 0; i < kFfItems; ++i) {
                 const int k = lane + 64 * i;
                 if (k <= M / 2) {
-                    ((double2 *)re)[k] = make_double2(0.5 * (z0[i].x + z1[i].x), 0.5 * (z0[i].y - z1[i].y));
-                    ((double2 *)im)[k] = make_double2(0.5 * (z0[i].y + z1[i].y), -0.5 * (z0[i].x - z1[i].x));
+                    // logical rows 2q, 2q + 1 are the region's stored rows swz_row(2q), swz_row(2q) ^ 1
+                    ((double2 *)(re + flip(q) * R2))[k] = make_double2(0.5 * (z0[i].x + z1[i].x), 0.5 * (z0[i].y - z1[i].y));
+                    ((double2 *)(re + (flip(q) ^ 1) * R2))[k] =
+                        make_double2(0.5 * (z0[i].y + z1[i].y), -0.5 * (z0[i].x - z1[i].x));
                 }
             }
             wave_sync();
@@ -749,14 +773,15 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
         __syncthreads();
         // pack: W = X0 + i X1 over the full row (W[-k] = conj X0[k] + i conj X1[k]), bit-reversed for the inverse
         for (int t = 0; t < NP; ++t) {
-            double *re = S + (wave + kFfWaves * t) * 2 * R2, *im = re + R2;
+            const int q = wave + kFfWaves * t;
+            double *re = S + region(q), *im = re + R2;
             double2 z0[kFfItems], z1[kFfItems];
 #pragma unroll
             for (int i = 0; i < kFfItems; ++i) {
                 const int k = lane + 64 * i;
                 if (k <= M / 2) {
-                    z0[i] = ((const double2 *)re)[k];
-                    z1[i] = ((const double2 *)im)[k];
+                    z0[i] = ((const double2 *)(re + flip(q) * R2))[k];
+                    z1[i] = ((const double2 *)(re + (flip(q) ^ 1) * R2))[k];
                 }
             }
             wave_sync();
@@ -764,11 +789,11 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             for (int i = 0; i < kFfItems; ++i) {
                 const int k = lane + 64 * i;
                 if (k <= M / 2) {
-                    const int pk = bit_reverse(k, LM);
+                    const int pk = swz_el(bit_reverse(k, LM));
                     re[pk] = z0[i].x - z1[i].y;
                     im[pk] = z0[i].y + z1[i].x;
                     if (k > 0 && k < M / 2) {
-                        const int pm = bit_reverse(M - k, LM);
+                        const int pm = swz_el(bit_reverse(M - k, LM));
                         re[pm] = z0[i].x + z1[i].y;
                         im[pm] = z1[i].x - z0[i].y;
                     }
@@ -782,14 +807,15 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             for (; lq + 2 <= LM; lq += 2) row_r4_dit(lq);
             if (lq < LM) row_r2_dit(lq);
         }
-        // this lane's pixels: the shifted value (times the weight), TwoSum or the maximum, in entry order
+        // this lane's slots: stored element rem & (M - 1) of plane rem >= M (the same logical pixel for every
+        // entry: swz_el is its own inverse); the shifted value (times the weight), TwoSum or the maximum
         const double wgt = a.w ? a.w[pe] : 1.0;
 #pragma unroll
         for (int j = 0; j < kFfPix; ++j) {
             const int u = lane + 64 * j;
             if (j < J) {
                 const int t = u >> (LM + 1), rem = u & (2 * M - 1);
-                double v = S[(wave + kFfWaves * t) * 2 * R2 + (rem >= M ? R2 : 0) + (rem & (M - 1))];
+                double v = S[region(wave + kFfWaves * t) + (rem >= M ? R2 : 0) + (rem & (M - 1))];
                 if (a.w) v = v * wgt;
                 if constexpr (MEAN) two_sum(acc_s[j], acc_c[j], v);
                 else acc_s[j] = max_step<FIRST, true>(acc_s[j], v);
@@ -803,7 +829,7 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
         const int u = lane + 64 * j;
         if (j < J) {
             const int t = u >> (LM + 1), rem = u & (2 * M - 1);
-            const int64_t pix = (int64_t)(wave + kFfWaves * t) * 2 * M + rem;
+            const int64_t pix = (int64_t)(wave + kFfWaves * t) * 2 * M + (rem & M) + swz_el(rem & (M - 1));
             const int64_t at = (int64_t)blockIdx.x * a.ndet + pix;
             if constexpr (MEAN) a.part_sum[at] = make_double2(acc_s[j], acc_c[j]);
             else a.part_max[at] = acc_s[j];
