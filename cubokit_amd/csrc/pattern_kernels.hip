@@ -519,11 +519,14 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
     const int64_t e_end = min(a.first + a.count, e_begin + a.ppg);
     for (int i = tid; i < L; i += kFfBlock) TW[i] = a.tw[i];
 
-    double acc_s[kFfPix], acc_c[kFfPix];
+    // per slot: the running sum (or maximum) in Double and the TwoSum compensation in Float (it gathers rounding
+    // errors far below the sum's last bit, so 24 bits of it are plenty; registers are short here)
+    double acc_s[kFfPix];
+    float acc_c[kFfPix];
 #pragma unroll
     for (int j = 0; j < kFfPix; ++j) {
         acc_s[j] = MEAN ? 0.0 : -__builtin_inf();
-        acc_c[j] = 0.0;
+        acc_c[j] = 0.f;
     }
     u32x4 pre[CH];
 #pragma unroll
@@ -817,8 +820,13 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
                 const int t = u >> (LM + 1), rem = u & (2 * M - 1);
                 double v = S[region(wave + kFfWaves * t) + (rem >= M ? R2 : 0) + (rem & (M - 1))];
                 if (a.w) v = v * wgt;
-                if constexpr (MEAN) two_sum(acc_s[j], acc_c[j], v);
-                else acc_s[j] = max_step<FIRST, true>(acc_s[j], v);
+                if constexpr (MEAN) {
+                    const double t = acc_s[j] + v, bb = t - acc_s[j];
+                    acc_c[j] += (float)((acc_s[j] - (t - bb)) + (v - bb));
+                    acc_s[j] = t;
+                } else {
+                    acc_s[j] = max_step<FIRST, true>(acc_s[j], v);
+                }
             }
             if (j % 8 == 7) __builtin_amdgcn_sched_barrier(0);   // at most 8 reads in flight
         }
@@ -831,7 +839,7 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             const int t = u >> (LM + 1), rem = u & (2 * M - 1);
             const int64_t pix = (int64_t)(wave + kFfWaves * t) * 2 * M + (rem & M) + swz_el(rem & (M - 1));
             const int64_t at = (int64_t)blockIdx.x * a.ndet + pix;
-            if constexpr (MEAN) a.part_sum[at] = make_double2(acc_s[j], acc_c[j]);
+            if constexpr (MEAN) a.part_sum[at] = make_double2(acc_s[j], (double)acc_c[j]);
             else a.part_max[at] = acc_s[j];
         }
     }
