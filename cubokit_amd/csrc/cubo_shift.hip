@@ -443,8 +443,8 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
         CUBO_HIP_TRY(hipEventRecord(p->ev[3], s), "hipEventRecord");
     }
     if (dmasks && cubo::shift_grad_rows_supported((int)c->shape[3])) {
-        // dL/dY split for the matrix cores on the device (split_dy): each column scaled by a power of two so its
-        // largest value lies in [1024, 2048), then FP16 hi + lo in the B-operand order
+        // dL/dY times each corner weight, split for the matrix cores on the device (split_dy): each column scaled
+        // by a power of two so its largest value lies in [1024, 2048), then FP16 hi + lo in the B-operand order
         const int64_t nks = (npos + 15) / 16;
         std::vector<double> inv(32, 0.0), scale(32, 0.0);
         for (int kk = 0; kk < p->k; ++kk)
@@ -456,7 +456,7 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
         const int qy = (int)c->shape[2];
         const int cus = cubo::compute_units(c->device);
         const int ngroups = (int)std::max<int64_t>(1, std::min<int64_t>({(4 * cus + qy - 1) / qy, 64, nks}));
-        const size_t dyw_bytes = (size_t)nks * 2 * 64 * 8 * sizeof(_Float16);
+        const size_t dyw_bytes = (size_t)nks * 4 * 2 * 64 * 8 * sizeof(_Float16);
         if (p->dyw_bytes < dyw_bytes) {
             if (p->dyw) (void)hipFree(p->dyw);
             p->dyw = nullptr;
@@ -475,15 +475,14 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
         if (!p->rdmask) CUBO_HIP_TRY(hipMalloc((void **)&p->rdmask, (size_t)32 * ndet * 8), "allocating dL/dmasks");
         inv.insert(inv.end(), scale.begin(), scale.end());          // dy_inv: [32] 1 / scale, then [32] scale
         CUBO_HIP_TRY(hipMemcpyAsync(p->dy_inv, inv.data(), 64 * 8, hipMemcpyHostToDevice, s), "uploading dL/dY scales");
-        CUBO_HIP_TRY(cubo::launch_split_dy(p->dy, npos, p->k, p->dy_inv + 32, p->dyw, s), "splitting dL/dY");
+        CUBO_HIP_TRY(cubo::launch_split_dy(p->dy, p->wr, p->wc, npos, p->k, p->dy_inv + 32, p->dyw, s),
+                     "splitting dL/dY");
         cubo::ShiftGradRows gw{};
         gw.cube = (const uint16_t *)c->data;
         gw.npos = npos;
         gw.qy = qy;
         gw.qx = (int32_t)c->shape[3];
         gw.shift = p->shift;
-        gw.wr = p->wr;
-        gw.wc = p->wc;
         gw.dyw = p->dyw;
         gw.dy_inv = p->dy_inv;
         gw.partial = p->rpartial;

@@ -331,14 +331,16 @@ __global__ __launch_bounds__(256) void shift_grad_w(ShiftGradW g) {
 template <int W>
 __global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
     constexpr int QX = 32 * W;                      // detector columns
+    constexpr int QXP = QX + 8;                     // a staged row: the row, then its first 8 pixels again
     constexpr int VPR = QX / 8;                     // 16-byte vectors per row
-    constexpr int VPB = 16 * 2 * VPR;               // vectors per batch of 16 positions
+    constexpr int VPB = 16 * 2 * VPR;               // row vectors per batch of 16 positions
     constexpr int T = 64 * W;
-    static_assert(VPB % T == 0, "every thread stages the same number of vectors");
-    constexpr int PER = VPB / T;
-    __shared__ __align__(16) uint16_t st[2][16][2][QX];
+    constexpr int BVB = 4 * 2 * 64;                 // 16-byte B-operand entries per batch (corner, hi/lo, lane)
+    static_assert(VPB % T == 0, "every thread stages the same number of row vectors");
+    constexpr int PER = VPB / T, BPER = (BVB + T - 1) / T;
+    __shared__ __align__(16) uint16_t st[2][16][2][QXP];
+    __shared__ half8 sb[2][BVB];
     __shared__ int sic[2][16];
-    __shared__ float swr[2][16], swc[2][16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, hf = lane >> 5;
     const int qy = g.qy;
     const int64_t ndet = (int64_t)qy * QX;
@@ -347,11 +349,13 @@ __global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
     const CUBO_GLOBAL uint16_t *cube = (const CUBO_GLOBAL uint16_t *)g.cube;
     const CUBO_GLOBAL half8 *dyw = (const CUBO_GLOBAL half8 *)g.dyw;
 
-    u32x4 reg[PER];
-    int reg_ic = 0;
-    float reg_wr = 0.f, reg_wc = 0.f;
-    // the batch's rows into registers; positions past the cube repeat the last one (their dL/dY is zero)
-    auto load = [&](int64_t p0) {
+    // two batches in registers (set 0, 1), so a batch's loads have a whole step to arrive
+    u32x4 reg[2][PER];
+    half8 breg[2][BPER];
+    int reg_ic[2] = {0, 0};
+    // the batch's rows and B operands into register set `set`; positions past the cube repeat the last one (their
+    // B operands are zero)
+    auto load = [&](int set, int64_t p0) {
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int v = tid + k * T;
@@ -361,23 +365,26 @@ __global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
             int r = arow + ir + row;
             r -= r >= qy ? qy : 0;
             r -= r >= qy ? qy : 0;
-            reg[k] = *(const CUBO_GLOBAL u32x4 *)(cube + p * ndet + (int64_t)r * QX + col8 * 8);
+            reg[set][k] = *(const CUBO_GLOBAL u32x4 *)(cube + p * ndet + (int64_t)r * QX + col8 * 8);
         }
-        if (tid < 16) {
-            const int64_t p = min(p0 + tid, g.npos - 1);
-            reg_ic = g.shift[2 * p + 1];
-            reg_wr = (float)g.wr[p];
-            reg_wc = (float)g.wc[p];
-        }
+        const int64_t ks = p0 >> 4;
+#pragma unroll
+        for (int k = 0; k < BPER; ++k)
+            if (BVB % T == 0 || tid + k * T < BVB) breg[set][k] = dyw[ks * BVB + tid + k * T];
+        if (tid < 16) reg_ic[set] = g.shift[2 * min(p0 + tid, g.npos - 1) + 1];
     };
-    auto store = [&](int b) {
+    auto store = [&](int set, int b) {
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int v = tid + k * T;
             const int j = v / (2 * VPR), rem = v % (2 * VPR), row = rem / VPR, col8 = rem % VPR;
-            *(u32x4 *)&st[b][j][row][col8 * 8] = reg[k];
+            *(u32x4 *)&st[b][j][row][col8 * 8] = reg[set][k];
+            if (col8 == 0) *(u32x4 *)&st[b][j][row][QX] = reg[set][k];   // the wrap: pixel QX reads pixel 0
         }
-        if (tid < 16) { sic[b][tid] = reg_ic; swr[b][tid] = reg_wr; swc[b][tid] = reg_wc; }
+#pragma unroll
+        for (int k = 0; k < BPER; ++k)
+            if (BVB % T == 0 || tid + k * T < BVB) sb[b][tid + k * T] = breg[set][k];
+        if (tid < 16) sic[b][tid] = reg_ic[set];
     };
 
     float16v acc;
@@ -386,29 +393,56 @@ __global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
     for (int r = 0; r < 16; ++r) { acc[r] = 0.f; accd[r] = 0.0; }
     const int col = 32 * wave + (lane & 31);
     int nk = 0;
-    auto compute = [&](int b, int64_t p0) {
-        const int64_t ks = p0 >> 4;
-        const half8 bh = dyw[(ks * 2 + 0) * 64 + lane], bl = dyw[(ks * 2 + 1) * 64 + lane];
-        half8 ah, al;
+    auto compute = [&](int b) {
+        // x[c][e / 2]: corner c's values for the lane's pixel and positions 8 hf + e, e + 1 (one per half); corners
+        // in the order (0,0), (1,0), (0,1), (1,1) of the B operands. Each position's pixel c0 = col + ic (mod qx)
+        // and its right neighbour come from one aligned dword pair of the staged row (padded for the wrap).
+        uint32_t x[4][4];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int j = 8 * hf + e;
-            int c0 = col + sic[b][j];
-            c0 -= c0 >= QX ? QX : 0;
-            int c1 = c0 + 1;
-            c1 -= c1 >= QX ? QX : 0;
-            const float wr = swr[b][j], wc = swc[b][j];
-            const float x00 = st[b][j][0][c0], x01 = st[b][j][0][c1], x10 = st[b][j][1][c0], x11 = st[b][j][1][c1];
-            // the bilinear blend (ShiftedImaging.swift:58-64) as two column lerps and a row lerp, halved
-            const float av = x00 + wc * (x01 - x00), bv = x10 + wc * (x11 - x10);
-            const float xs = 0.5f * (av + wr * (bv - av));
-            const _Float16 hi = (_Float16)xs;
-            ah[e] = hi;
-            al[e] = (_Float16)(xs - (float)hi);
+        for (int e = 0; e < 8; e += 2) {
+            uint32_t pr[2][2];                      // [position e, e + 1][row 0, 1]: (x[c0], x[c0 + 1])
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int j = 8 * hf + e + h;
+                int c0 = col + sic[b][j];
+                c0 -= c0 >= QX ? QX : 0;
+                const uint32_t sh = (uint32_t)(c0 & 1) * 16u;
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const uint32_t *w = (const uint32_t *)&st[b][j][r][0] + (c0 >> 1);
+                    pr[h][r] = __builtin_amdgcn_alignbit(w[1], w[0], sh);
+                }
+            }
+            // corner (r, 0): the low halves of positions e, e + 1; corner (r, 1): the high halves
+            x[0][e >> 1] = __builtin_amdgcn_perm(pr[1][0], pr[0][0], 0x05040100u);
+            x[1][e >> 1] = __builtin_amdgcn_perm(pr[1][1], pr[0][1], 0x05040100u);
+            x[2][e >> 1] = __builtin_amdgcn_perm(pr[1][0], pr[0][0], 0x07060302u);
+            x[3][e >> 1] = __builtin_amdgcn_perm(pr[1][1], pr[0][1], 0x07060302u);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc, 0, 0, 0);
+        uint32_t any = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) any |= x[c][q] & 0xf800f800u;
+        const bool use_hi = __builtin_amdgcn_ballot_w64(any != 0) != 0;   // wave-uniform
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const half8 bh = sb[b][(c * 2 + 0) * 64 + lane], bl = sb[b][(c * 2 + 1) * 64 + lane];
+            u32x4 lo;
+            lo.x = cvt_pair(x[c][0] & 0x07ff07ffu); lo.y = cvt_pair(x[c][1] & 0x07ff07ffu);
+            lo.z = cvt_pair(x[c][2] & 0x07ff07ffu); lo.w = cvt_pair(x[c][3] & 0x07ff07ffu);
+            const half8 alo = __builtin_bit_cast(half8, lo);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bl, acc, 0, 0, 0);
+            if (use_hi) {
+                u32x4 hi;
+                hi.x = cvt_pair(x[c][0] & 0xf800f800u); hi.y = cvt_pair(x[c][1] & 0xf800f800u);
+                hi.z = cvt_pair(x[c][2] & 0xf800f800u); hi.w = cvt_pair(x[c][3] & 0xf800f800u);
+                const half8 ahi = __builtin_bit_cast(half8, hi);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bh, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bl, acc, 0, 0, 0);
+            }
+        }
         if (++nk == 2) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) { accd[r] += (double)acc[r]; acc[r] = 0.f; }
@@ -417,24 +451,30 @@ __global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
     };
 
     if (p_begin < p_end) {
-        load(p_begin);
-        store(0);
+        load(0, p_begin);
+        store(0, 0);
+        if (p_begin + 16 < p_end) load(1, p_begin + 16);
         __syncthreads();
-        int b = 0;
-        for (int64_t p0 = p_begin; p0 < p_end; p0 += 16) {
-            const bool more = p0 + 16 < p_end;          // uniform
-            if (more) load(p0 + 16);
-            compute(b, p0);
-            if (more) store(b ^ 1);
+        // step(p0, b, cur): batch p0 is in LDS buffer b, batch p0 + 16 in register set cur; the loads of batch
+        // p0 + 32 go to the other set while batch p0 is multiplied
+        auto step = [&](int64_t p0, int b, int cur) {
+            if (p0 + 32 < p_end) load(cur ^ 1, p0 + 32);   // uniform
+            compute(b);
+            if (p0 + 16 < p_end) store(cur, b ^ 1);
             lds_barrier();
-            b ^= 1;
+        };
+        int64_t p0 = p_begin;
+        for (; p0 + 16 < p_end; p0 += 32) {
+            step(p0, 0, 1);
+            step(p0 + 16, 1, 0);
         }
+        if (p0 < p_end) step(p0, 0, 1);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) accd[r] += (double)acc[r];
     // register r of lane l: output pixel (arow, 32 wave + 8 (r / 4) + 4 (l / 32) + r % 4), mask l % 32
     const int mask = lane & 31;
-    const double scale = 2.0 * g.dy_inv[mask];
+    const double scale = g.dy_inv[mask];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int64_t pix = (int64_t)arow * QX + 32 * wave + 8 * (r >> 2) + 4 * hf + (r & 3);
@@ -512,23 +552,30 @@ hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, b
 bool shift_grad_rows_supported(int qx) { return qx % 32 == 0 && qx >= 32 && qx <= 256; }
 
 namespace {
-__global__ void split_dy(const double *dy, int64_t npos, int k, const double *scale, _Float16 *dyw, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;     // (k-step, lane, e)
+__global__ void split_dy(const double *dy, const double *wr, const double *wc, int64_t npos, int k,
+                         const double *scale, _Float16 *dyw, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;     // (k-step, corner, lane, e)
     if (i >= n) return;
-    const int64_t ks = i / 512;
-    const int l = (int)((i / 8) % 64), e = (int)(i % 8), kk = l & 31;
+    const int64_t ks = i / 2048;
+    const int c = (int)((i / 512) % 4), l = (int)((i / 8) % 64), e = (int)(i % 8), kk = l & 31;
     const int64_t pos = ks * 16 + 8 * (l >> 5) + e;
-    const double v = (kk < k && pos < npos) ? dy[pos * k + kk] * scale[kk] : 0.0;
+    double v = 0.0;
+    if (kk < k && pos < npos) {
+        const double r = wr[pos], q = wc[pos];
+        const double w = c == 0 ? (1.0 - r) * (1.0 - q) : c == 1 ? r * (1.0 - q) : c == 2 ? (1.0 - r) * q : r * q;
+        v = dy[pos * k + kk] * w * scale[kk];
+    }
     const _Float16 hi = (_Float16)(float)v;
     const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
-    dyw[((ks * 2 + 0) * 64 + l) * 8 + e] = hi;
-    dyw[((ks * 2 + 1) * 64 + l) * 8 + e] = lo;
+    dyw[(((ks * 4 + c) * 2 + 0) * 64 + l) * 8 + e] = hi;
+    dyw[(((ks * 4 + c) * 2 + 1) * 64 + l) * 8 + e] = lo;
 }
 }  // namespace
 
-hipError_t launch_split_dy(const double *dy, int64_t npos, int k, const double *scale, void *dyw, hipStream_t s) {
-    const int64_t n = (npos + 15) / 16 * 512;
-    hipLaunchKernelGGL(split_dy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dy, npos, k, scale,
+hipError_t launch_split_dy(const double *dy, const double *wr, const double *wc, int64_t npos, int k,
+                           const double *scale, void *dyw, hipStream_t s) {
+    const int64_t n = (npos + 15) / 16 * 2048;
+    hipLaunchKernelGGL(split_dy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dy, wr, wc, npos, k, scale,
                        (_Float16 *)dyw, n);
     return hipGetLastError();
 }

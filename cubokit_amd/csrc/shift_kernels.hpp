@@ -83,28 +83,31 @@ struct ShiftGradW {
 hipError_t launch_shift_grad_w(const ShiftGradW &a, hipStream_t s);
 
 // The same adjoint on the matrix cores, for detectors whose rows hold a multiple of 32 pixels, at most 256: block
-// (a, g) owns output row a and position group g, one wave per 32 output pixels. For each position the two input
-// rows the bilinear shift reads ((a + ir) and (a + ir + 1) mod qy) are staged in LDS in batches of 16 positions
-// (double-buffered), each lane blends its pixel's four neighbours for 8 positions from LDS (xs / 2, split into FP16
-// hi + lo), and the 32 x 16 A operand meets dL/dY split the same way (B operand, [k-step][hi, lo][lane][8], columns
-// scaled by powers of two) in three MFMAs per k-step, flushed to FP64 every 32 positions.
+// (a, g) owns output row a and position group g, one wave per 32 output pixels. dL/dm[k][v] =
+// sum_p sum_c (dL/dY[p, k] w_c(p)) x_p[v + s_c(p)] over the four corners c of the bilinear shift: for each corner
+// the A operand is the pattern value at that neighbour of pixel v (32 pixels x 16 positions, uint16 split exactly
+// into FP16 lo and hi parts) and the B operand dL/dY times the corner weight (FP16 hi + lo, columns scaled by
+// powers of two). Per batch of 16 positions the two input rows they read ((a + ir) and (a + ir + 1) mod qy) and the
+// batch's B operands are staged in LDS (double-buffered); two MFMAs per corner and k-step (four when the data have
+// high parts), flushed to FP64 every 32 positions.
 struct ShiftGradRows {
     const uint16_t *cube;
     int64_t npos;
     int32_t qy, qx;
     const int32_t *shift;     // [npos][2] (ir, ic)
-    const double *wr, *wc;    // [npos]
-    const void *dyw;          // [ceil(npos / 16)][hi, lo][lane][8] FP16, zero past the positions and the masks
+    const void *dyw;          // [ceil(npos / 16)][corner][hi, lo][lane][8] FP16, zero past the positions and masks
     const double *dy_inv;     // [32] 1 / column scale (0 past the masks)
     double *partial;          // [ngroups][32][ndet]
     int32_t ngroups;
     int64_t ppg;              // positions per group, a multiple of 16
 };
 bool shift_grad_rows_supported(int qx);
-// The B operand of shift_grad_rows from dL/dY on the device: column kk of dy [npos][k] times scale[kk] (a power of
-// two), rounded to Float then split into FP16 hi and lo, at [k-step][hi, lo][lane][8] (lane l: mask l % 32,
+// The B operands of shift_grad_rows on the device: column kk of dy [npos][k] times the position's corner weight
+// ((1-wr)(1-wc), wr(1-wc), (1-wr)wc, wr wc for the corners (0,0), (1,0), (0,1), (1,1)) times scale[kk] (a power of
+// two), rounded to Float then split into FP16 hi and lo, at [k-step][corner][hi, lo][lane][8] (lane l: mask l % 32,
 // positions 16 s + 8 (l / 32) + e), zero past the positions and the masks.
-hipError_t launch_split_dy(const double *dy, int64_t npos, int k, const double *scale, void *dyw, hipStream_t s);
+hipError_t launch_split_dy(const double *dy, const double *wr, const double *wc, int64_t npos, int k,
+                           const double *scale, void *dyw, hipStream_t s);
 hipError_t launch_shift_grad_rows(const ShiftGradRows &a, hipStream_t s);
 // out[i] = sum_g partial[g][i] in group order, i < n.
 hipError_t launch_sum_groups(const double *partial, int ngroups, int64_t n, double *out, hipStream_t s);
