@@ -455,7 +455,17 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
             }
         const int qy = (int)c->shape[2];
         const int cus = cubo::compute_units(c->device);
-        const int ngroups = (int)std::max<int64_t>(1, std::min<int64_t>({(4 * cus + qy - 1) / qy, 64, nks}));
+        // groups: of the counts that give at least one block per compute unit, the one whose blocks (one of them
+        // per compute unit at a time) fill their last round best
+        const int64_t bands = (qy + cubo::shift_grad_rows_band((int)c->shape[3]) - 1) /
+                              cubo::shift_grad_rows_band((int)c->shape[3]);
+        int ngroups = 1;
+        double best = -1;
+        for (int64_t ng = std::max<int64_t>(1, (cus + bands - 1) / bands); ng <= std::min<int64_t>(64, nks); ++ng) {
+            const int64_t blocks = bands * ng, rounds = (blocks + cus - 1) / cus;
+            const double fill = (double)blocks / (double)(rounds * cus);
+            if (fill > best + 0.02) { best = fill; ngroups = (int)ng; }
+        }
         const size_t dyw_bytes = (size_t)nks * 4 * 2 * 64 * 8 * sizeof(_Float16);
         if (p->dyw_bytes < dyw_bytes) {
             if (p->dyw) (void)hipFree(p->dyw);

@@ -328,23 +328,25 @@ __global__ __launch_bounds__(256) void shift_grad_w(ShiftGradW g) {
     for (int k = 0; k < KP; ++k) g.partial[((int64_t)group * KP + k) * ndet + v] = accd[k] + (double)acc[k];
 }
 
-template <int W>
-__global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
+// RB output rows per block (RB x W waves): the RB + 1 input rows each position's band reads are staged once.
+template <int W, int RB>
+__global__ __launch_bounds__(64 * W * RB) void shift_grad_rows(ShiftGradRows g) {
     constexpr int QX = 32 * W;                      // detector columns
     constexpr int QXP = QX + 8;                     // a staged row: the row, then its first 8 pixels again
     constexpr int VPR = QX / 8;                     // 16-byte vectors per row
-    constexpr int VPB = 16 * 2 * VPR;               // row vectors per batch of 16 positions
-    constexpr int T = 64 * W;
+    constexpr int SR = RB + 1;                      // staged rows per position
+    constexpr int VPB = 16 * SR * VPR;              // row vectors per batch of 16 positions
+    constexpr int T = 64 * W * RB;
     constexpr int BVB = 4 * 2 * 64;                 // 16-byte B-operand entries per batch (corner, hi/lo, lane)
-    static_assert(VPB % T == 0, "every thread stages the same number of row vectors");
-    constexpr int PER = VPB / T, BPER = (BVB + T - 1) / T;
-    __shared__ __align__(16) uint16_t st[2][16][2][QXP];
+    constexpr int PER = (VPB + T - 1) / T, BPER = (BVB + T - 1) / T;
+    __shared__ __align__(16) uint16_t st[2][16][SR][QXP];
     __shared__ half8 sb[2][BVB];
     __shared__ int sic[2][16];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, hf = lane >> 5;
+    const int tid = threadIdx.x, lane = tid & 63, hf = lane >> 5;
+    const int wv = tid >> 6, rw = wv / W, wave = wv % W;   // output row a0 + rw, columns 32 wave ..
     const int qy = g.qy;
     const int64_t ndet = (int64_t)qy * QX;
-    const int arow = blockIdx.x, group = blockIdx.y;
+    const int a0 = blockIdx.x * RB, arow = a0 + rw, group = blockIdx.y;
     const int64_t p_begin = (int64_t)group * g.ppg, p_end = min(g.npos, p_begin + g.ppg);
     const CUBO_GLOBAL uint16_t *cube = (const CUBO_GLOBAL uint16_t *)g.cube;
     const CUBO_GLOBAL half8 *dyw = (const CUBO_GLOBAL half8 *)g.dyw;
@@ -359,13 +361,15 @@ __global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int v = tid + k * T;
-            const int j = v / (2 * VPR), rem = v % (2 * VPR), row = rem / VPR, col8 = rem % VPR;
-            const int64_t p = min(p0 + j, g.npos - 1);
-            const int ir = g.shift[2 * p];
-            int r = arow + ir + row;
-            r -= r >= qy ? qy : 0;
-            r -= r >= qy ? qy : 0;
-            reg[set][k] = *(const CUBO_GLOBAL u32x4 *)(cube + p * ndet + (int64_t)r * QX + col8 * 8);
+            if (VPB % T == 0 || v < VPB) {
+                const int j = v / (SR * VPR), rem = v % (SR * VPR), row = rem / VPR, col8 = rem % VPR;
+                const int64_t p = min(p0 + j, g.npos - 1);
+                const int ir = g.shift[2 * p];
+                int r = a0 + ir + row;                  // < 3 qy: a0 < qy, ir < qy, row <= RB <= qy
+                r -= r >= qy ? qy : 0;
+                r -= r >= qy ? qy : 0;
+                reg[set][k] = *(const CUBO_GLOBAL u32x4 *)(cube + p * ndet + (int64_t)r * QX + col8 * 8);
+            }
         }
         const int64_t ks = p0 >> 4;
 #pragma unroll
@@ -377,9 +381,11 @@ __global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int v = tid + k * T;
-            const int j = v / (2 * VPR), rem = v % (2 * VPR), row = rem / VPR, col8 = rem % VPR;
-            *(u32x4 *)&st[b][j][row][col8 * 8] = reg[set][k];
-            if (col8 == 0) *(u32x4 *)&st[b][j][row][QX] = reg[set][k];   // the wrap: pixel QX reads pixel 0
+            if (VPB % T == 0 || v < VPB) {
+                const int j = v / (SR * VPR), rem = v % (SR * VPR), row = rem / VPR, col8 = rem % VPR;
+                *(u32x4 *)&st[b][j][row][col8 * 8] = reg[set][k];
+                if (col8 == 0) *(u32x4 *)&st[b][j][row][QX] = reg[set][k];   // the wrap: pixel QX reads pixel 0
+            }
         }
 #pragma unroll
         for (int k = 0; k < BPER; ++k)
@@ -409,7 +415,7 @@ __global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
                 const uint32_t sh = (uint32_t)(c0 & 1) * 16u;
 #pragma unroll
                 for (int r = 0; r < 2; ++r) {
-                    const uint32_t *w = (const uint32_t *)&st[b][j][r][0] + (c0 >> 1);
+                    const uint32_t *w = (const uint32_t *)&st[b][j][rw + r][0] + (c0 >> 1);
                     pr[h][r] = __builtin_amdgcn_alignbit(w[1], w[0], sh);
                 }
             }
@@ -473,6 +479,7 @@ __global__ __launch_bounds__(64 * W) void shift_grad_rows(ShiftGradRows g) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) accd[r] += (double)acc[r];
     // register r of lane l: output pixel (arow, 32 wave + 8 (r / 4) + 4 (l / 32) + r % 4), mask l % 32
+    if (arow >= qy) return;                          // the band's last row past the detector (odd qy)
     const int mask = lane & 31;
     const double scale = g.dy_inv[mask];
 #pragma unroll
@@ -551,6 +558,8 @@ hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, b
 
 bool shift_grad_rows_supported(int qx) { return qx % 32 == 0 && qx >= 32 && qx <= 256; }
 
+int shift_grad_rows_band(int qx) { return qx <= 128 ? 3 : 1; }
+
 namespace {
 __global__ void split_dy(const double *dy, const double *wr, const double *wc, int64_t npos, int k,
                          const double *scale, _Float16 *dyw, int64_t n) {
@@ -582,16 +591,19 @@ hipError_t launch_split_dy(const double *dy, const double *wr, const double *wc,
 
 hipError_t launch_shift_grad_rows(const ShiftGradRows &a, hipStream_t s) {
     if (!shift_grad_rows_supported(a.qx) || a.ngroups <= 0) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)a.qy, (unsigned)a.ngroups);
+    // three output rows per block up to 128-pixel rows (at most 12 waves, one block of them per compute unit),
+    // one above
+    const int rb = shift_grad_rows_band(a.qx);
+    const dim3 grid((unsigned)((a.qy + rb - 1) / rb), (unsigned)a.ngroups);
     switch (a.qx / 32) {
-    case 1: hipLaunchKernelGGL(shift_grad_rows<1>, grid, dim3(64), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(shift_grad_rows<2>, grid, dim3(128), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(shift_grad_rows<3>, grid, dim3(192), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(shift_grad_rows<4>, grid, dim3(256), 0, s, a); break;
-    case 5: hipLaunchKernelGGL(shift_grad_rows<5>, grid, dim3(320), 0, s, a); break;
-    case 6: hipLaunchKernelGGL(shift_grad_rows<6>, grid, dim3(384), 0, s, a); break;
-    case 7: hipLaunchKernelGGL(shift_grad_rows<7>, grid, dim3(448), 0, s, a); break;
-    default: hipLaunchKernelGGL(shift_grad_rows<8>, grid, dim3(512), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((shift_grad_rows<1, 3>), grid, dim3(192), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((shift_grad_rows<2, 3>), grid, dim3(384), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((shift_grad_rows<3, 3>), grid, dim3(576), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((shift_grad_rows<4, 3>), grid, dim3(768), 0, s, a); break;
+    case 5: hipLaunchKernelGGL((shift_grad_rows<5, 1>), grid, dim3(320), 0, s, a); break;
+    case 6: hipLaunchKernelGGL((shift_grad_rows<6, 1>), grid, dim3(384), 0, s, a); break;
+    case 7: hipLaunchKernelGGL((shift_grad_rows<7, 1>), grid, dim3(448), 0, s, a); break;
+    default: hipLaunchKernelGGL((shift_grad_rows<8, 1>), grid, dim3(512), 0, s, a); break;
     }
     return hipGetLastError();
 }

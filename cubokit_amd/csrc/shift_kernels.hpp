@@ -83,13 +83,14 @@ struct ShiftGradW {
 hipError_t launch_shift_grad_w(const ShiftGradW &a, hipStream_t s);
 
 // The same adjoint on the matrix cores, for detectors whose rows hold a multiple of 32 pixels, at most 256: block
-// (a, g) owns output row a and position group g, one wave per 32 output pixels. dL/dm[k][v] =
+// (band, g) owns a band of output rows (shift_grad_rows_band) and position group g, one wave per output row and
+// 32 output pixels. dL/dm[k][v] =
 // sum_p sum_c (dL/dY[p, k] w_c(p)) x_p[v + s_c(p)] over the four corners c of the bilinear shift: for each corner
 // the A operand is the pattern value at that neighbour of pixel v (32 pixels x 16 positions, uint16 split exactly
 // into FP16 lo and hi parts) and the B operand dL/dY times the corner weight (FP16 hi + lo, columns scaled by
-// powers of two). Per batch of 16 positions the two input rows they read ((a + ir) and (a + ir + 1) mod qy) and the
-// batch's B operands are staged in LDS (double-buffered); two MFMAs per corner and k-step (four when the data have
-// high parts), flushed to FP64 every 32 positions.
+// powers of two). Per batch of 16 positions the band + 1 input rows they read (from (a0 + ir) mod qy on) and the
+// batch's B operands are staged in LDS once for the band (double-buffered); two MFMAs per corner and k-step (four
+// when the data have high parts), flushed to FP64 every 32 positions.
 struct ShiftGradRows {
     const uint16_t *cube;
     int64_t npos;
@@ -102,6 +103,8 @@ struct ShiftGradRows {
     int64_t ppg;              // positions per group, a multiple of 16
 };
 bool shift_grad_rows_supported(int qx);
+// Output rows per block of shift_grad_rows for a row width (the grid is ceil(qy / band) x ngroups).
+int shift_grad_rows_band(int qx);
 // The B operands of shift_grad_rows on the device: column kk of dy [npos][k] times the position's corner weight
 // ((1-wr)(1-wc), wr(1-wc), (1-wr)wc, wr wc for the corners (0,0), (1,0), (0,1), (1,1)) times scale[kk] (a power of
 // two), rounded to Float then split into FP16 hi and lo, at [k-step][corner][hi, lo][lane][8] (lane l: mask l % 32,
