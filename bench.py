@@ -37,6 +37,19 @@ def traffic_from_profiles():
         return json.load(f).get("fused_u16_int_bandwidth_u16", {}).get("hbm_bytes_per_launch")
 
 
+def mfma_from_profiles():
+    """Matrix-core utilisation of the K = 64 split-FP16 stack from rocprofv3 counters (profiles/traffic.json)."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        e = json.load(f).get("mask_stack_mfma<2, 8>_bandwidth_u16_k64", {})
+    if "mfma_tflops" not in e:
+        return None
+    return {"tflops": e["mfma_tflops"], "frac_of_dense_fp16_peak": e["mfma_frac_of_dense_fp16_peak"],
+            "source": "rocprofv3 counters (SQ_INSTS_VALU_MFMA_MOPS_F16)"}
+
+
 def cpu_baseline(cube, bf, adf, seconds: float):
     """The oracle's restatement of CuboKit's .cpu pass, on a bounded sample of the same cube."""
     from oracle import oracle as O
@@ -365,6 +378,7 @@ def main() -> int:
         stack["max_rel_diff_vs_plain"] = float(np.max(np.abs(ys["mfma_split_fp16"] - ys["plain"]) / den))
         stack["k"] = int(stack_masks.shape[0])
         stack["mfma_gbs"] = round(cube_bytes / (stack["mfma_split_fp16_ms"] / 1e3) / 1e9, 1)
+        stack["mfma_utilisation"] = mfma_from_profiles()
 
     patterns = None if args.no_pattern else pattern_section(C, cube)
     ingest = None if args.no_ingest else ingest_section(C, cube)
