@@ -74,9 +74,10 @@ def test_equal_origins_give_the_plain_stack_bit_for_bit():
     assert np.array_equal(y, C.MaskStack(res, masks, C.MaskStack.MFMA_SPLIT).run())
 
 
-@pytest.mark.parametrize("det", [32, 24])
+@pytest.mark.parametrize("det", [32, 24, 96, 160])
 def test_backward_matches_the_oracle(det):
-    """32 px rows: the matrix-core mask adjoint (rows staged in LDS); 24 px rows: the vector-ALU one."""
+    """Rows of 32, 96 and 160 px: the matrix-core mask adjoint (bands of three output rows for 32 and 96, of one for
+    160; 96 px rows stage with 9 waves, not a divisor of the batch); 24 px rows: the vector-ALU one."""
     res = _cube((9, 11), det, 400.0, 6)         # 99 positions in many shift classes: partial tiles
     masks = _masks(det, 6, 7, signed=True)
     orow, ocol = _origins(99, 8)
