@@ -25,6 +25,7 @@
 #include "cube.hpp"
 #include "errors.hpp"
 #include "host_math.hpp"
+#include "reduce_kernels.hpp"
 
 using cubo::DeviceGuard;
 using cubo::element_size;
@@ -37,9 +38,9 @@ constexpr size_t kFeedSlotBytes = (size_t)64 << 20;   // pinned bytes per slot
 
 struct Feed {
     int device = 0;
-    void *pinned[2] = {nullptr, nullptr};
-    void *raw[2] = {nullptr, nullptr};         // device copies of raw chunks that need a conversion
-    hipEvent_t copied[2] = {nullptr, nullptr};  // copy stream: the slot's DMA (and conversion) is done
+    void *pinned[2] = {nullptr, nullptr};      // mapped: the device reads the slots over PCIe itself
+    void *pinned_dev[2] = {nullptr, nullptr};  // their device addresses
+    hipEvent_t copied[2] = {nullptr, nullptr};  // copy stream: the kernel that read the slot is done
     hipEvent_t consumed[2] = {nullptr, nullptr};
     hipStream_t copy = nullptr;
     int next = 0;
@@ -104,7 +105,8 @@ int make_feed(cubo_hip_cube *c) {
     f->device = c->device;
     hipError_t e = hipStreamCreateWithFlags(&f->copy, hipStreamNonBlocking);
     for (int s = 0; s < 2 && e == hipSuccess; ++s) {
-        e = hipHostMalloc(&f->pinned[s], kFeedSlotBytes, hipHostMallocDefault);
+        e = hipHostMalloc(&f->pinned[s], kFeedSlotBytes, hipHostMallocMapped);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&f->pinned_dev[s], f->pinned[s], 0);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&f->copied[s], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&f->consumed[s], hipEventDisableTiming);
     }
@@ -128,16 +130,17 @@ int feed_bytes(cubo_hip_cube *c, void *dst, size_t nbytes, int t, hipEvent_t aft
         const size_t len = std::min(chunk, nbytes - off);
         const int s = f->next;
         f->next ^= 1;
-        CUBO_HIP_TRY(hipEventSynchronize(f->copied[s]), "waiting for a pinned buffer");   // its last DMA is done
+        CUBO_HIP_TRY(hipEventSynchronize(f->copied[s]), "waiting for a pinned buffer");   // its last reader is done
         if (!fill(f->pinned[s], off, len)) return CUBO_HIP_ERR_FILE;                        // fill set the message
         char *out = (char *)dst + (off / eb) * out_b;
-        if (!conv) {
+        // a kernel pulls the slot over PCIe (many reads in flight), converting on the way when the element type
+        // is not the cube's
+        if (!conv && ((uintptr_t)out & 15) == 0)   // the copy kernel moves 16-byte words
+            CUBO_HIP_TRY(cubo::launch_copy_bytes(f->pinned_dev[s], out, len, f->copy), "copying to HBM");
+        else if (!conv)
             CUBO_HIP_TRY(hipMemcpyAsync(out, f->pinned[s], len, hipMemcpyHostToDevice, f->copy), "copying to HBM");
-        } else {
-            if (!f->raw[s]) CUBO_HIP_TRY(hipMalloc(&f->raw[s], kFeedSlotBytes), "allocating the conversion buffer");
-            CUBO_HIP_TRY(hipMemcpyAsync(f->raw[s], f->pinned[s], len, hipMemcpyHostToDevice, f->copy), "copying to HBM");
-            CUBO_HIP_TRY(launch_to_f32(t, f->raw[s], (float *)out, (int64_t)(len / eb), f->copy), "converting");
-        }
+        else
+            CUBO_HIP_TRY(launch_to_f32(t, f->pinned_dev[s], (float *)out, (int64_t)(len / eb), f->copy), "converting");
         CUBO_HIP_TRY(hipEventRecord(f->copied[s], f->copy), "hipEventRecord");
     }
     // later work on the cube's stream waits for every copy of this call
@@ -322,7 +325,6 @@ void free_feed(cubo_hip_cube *c) {
     if (f->copy) (void)hipStreamSynchronize(f->copy);
     for (int s = 0; s < 2; ++s) {
         if (f->pinned[s]) (void)hipHostFree(f->pinned[s]);
-        if (f->raw[s]) (void)hipFree(f->raw[s]);
         if (f->ring[s]) (void)hipFree(f->ring[s]);
         if (f->copied[s]) (void)hipEventDestroy(f->copied[s]);
         if (f->consumed[s]) (void)hipEventDestroy(f->consumed[s]);

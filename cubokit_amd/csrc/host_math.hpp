@@ -42,10 +42,10 @@ inline int wrap_shift(double v, int n) {
     return ((int)m) % n;
 }
 
-// Threads for large host copies and file reads: the hardware threads, at most 8.
+// Threads for large host copies and file reads: the hardware threads, at most 16.
 inline int reader_threads() {
     const unsigned h = std::thread::hardware_concurrency();
-    return (int)std::max(1u, std::min(8u, h ? h : 1u));
+    return (int)std::max(1u, std::min(16u, h ? h : 1u));
 }
 
 // Runs fn(offset, bytes) over [0, n) in `threads` contiguous pieces (4 KiB aligned), in parallel.
