@@ -15,6 +15,8 @@
 #include "device_sync.hpp"
 #include "maskstack_kernels.hpp"
 
+#include <type_traits>
+
 namespace cubo {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -48,7 +50,7 @@ struct StackLds {
     half8 w[2][WCH];
 };
 
-template <int NT, int WAVES>
+template <int NT, int WAVES, bool DEEP>
 __device__ __forceinline__ void stack_tile(const StackLaunch &a, int64_t p0, StackLds<NT> &sh) {
     constexpr int WCH = StackLds<NT>::WCH;
     auto &wlds = sh.w;
@@ -76,16 +78,20 @@ __device__ __forceinline__ void stack_tile(const StackLaunch &a, int64_t p0, Sta
     };
     // The block's weight chunk: each thread carries WPT 16-byte entries from HBM (or L2) to LDS. They are
     // loaded before the data prefetch of the same step, so waiting for them never waits for that prefetch.
+    // DEEP keeps two weight chunks and three data chunks ahead in registers (for blocks with the registers
+    // to spare): a step then waits on loads issued two steps earlier.
     constexpr int WPT = (WCH + 64 * WAVES - 1) / (64 * WAVES);
-    half8 wreg[WPT];
+    constexpr int NW = DEEP ? 2 : 1;       // weight chunks in registers
+    constexpr int NX = DEEP ? 4 : 3;       // data chunks in registers (the current one and NX - 1 ahead)
+    half8 wreg[NW][WPT];
     static_assert(WCH % (64 * WAVES) == 0, "every thread carries the same number of weight entries");
-    auto fetch_w = [&](int chunk) {
+    auto fetch_w = [&](int set, int chunk) {
 #pragma unroll
-        for (int j = 0; j < WPT; ++j) wreg[j] = wsrc[(size_t)chunk * WCH + tid + j * 64 * WAVES];
+        for (int j = 0; j < WPT; ++j) wreg[set][j] = wsrc[(size_t)chunk * WCH + tid + j * 64 * WAVES];
     };
-    auto put_w = [&](int buf) {
+    auto put_w = [&](int set, int buf) {
 #pragma unroll
-        for (int j = 0; j < WPT; ++j) wlds[buf][tid + j * 64 * WAVES] = wreg[j];
+        for (int j = 0; j < WPT; ++j) wlds[buf][tid + j * 64 * WAVES] = wreg[set][j];
     };
 
     float16v acc[NT];
@@ -147,35 +153,51 @@ __device__ __forceinline__ void stack_tile(const StackLaunch &a, int64_t p0, Sta
         }
     };
 
-    // Chunk c's weights are in wlds[c & 1] before the barrier that precedes compute(c); each step fetches
-    // the next chunk's weights, then the data two chunks ahead, multiplies, and stores the weights into the
-    // other buffer (read last by compute(c - 1), which every wave finished before the previous barrier).
-    u32x4 X0[4], X1[4], X2[4];
-    fetch_w(0);
-    __builtin_amdgcn_sched_barrier(0);
-    load_x(X0, 0);
-    load_x(X1, min(1, nchunks - 1));
-    put_w(0);
-    lds_barrier();
-    // The loads are unconditional (past the last chunk they repeat it, and the extra weight copy lands in
-    // the buffer no wave reads any more), and the weight loads stay ahead of the data loads in issue order:
-    // the wait for the weights before put_w then leaves the data prefetch in flight.
-    auto step = [&](const u32x4 (&xc)[4], u32x4 (&xn)[4], int c) {
-        fetch_w(min(c + 1, nchunks - 1));
+    // Chunk c's weights are in wlds[c & 1] before the barrier that precedes compute(c); step c fetches the
+    // weights NW chunks ahead into the register set chunk c's weights left, then the data NX - 1 chunks
+    // ahead, multiplies, and stores chunk c + 1's weights into the other LDS buffer (read last by
+    // compute(c - 1), which every wave finished before the previous barrier).
+    // The prologue issues the loads of the steps before chunk 0 in the order the steps issue them (step v:
+    // the weights of chunk v + NW, then the data of chunk v + NX - 1), so every wait in the loop counts the
+    // same loads on its first pass as on later ones.
+    u32x4 X[NX][4];
+#pragma unroll
+    for (int v = -(NX - 1); v < 0; ++v) {
+        if (v + NW >= 0) fetch_w(v + NW, min(v + NW, nchunks - 1));
         __builtin_amdgcn_sched_barrier(0);
-        load_x(xn, min(c + 2, nchunks - 1));
-        compute(xc, c);
-        put_w((c + 1) & 1);
+        load_x(X[v + NX - 1], min(v + NX - 1, nchunks - 1));
+    }
+    put_w(0, 0);
+    lds_barrier();
+    // The loads are unconditional (past the last chunk they repeat it, and the extra weight copies land in
+    // registers or in the buffer no wave reads any more), and each weight fetch is issued before the data
+    // loads of its step: the wait for the weights before put_w then leaves the data prefetch in flight.
+    // Step P of an unrolled group handles a chunk c with c = P modulo NW and NX, so the register sets are
+    // chosen at compile time.
+    auto step = [&](auto phase, int c) {
+        constexpr int P = decltype(phase)::value;
+        fetch_w(P % NW, min(c + NW, nchunks - 1));
+        __builtin_amdgcn_sched_barrier(0);
+        load_x(X[(P + NX - 1) % NX], min(c + NX - 1, nchunks - 1));
+        compute(X[P % NX], c);
+        put_w((P + 1) % NW, (c + 1) & 1);
         lds_barrier();
     };
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    using P2 = std::integral_constant<int, 2>;
+    using P3 = std::integral_constant<int, 3>;
+    static_assert(NX % NW == 0 && (NX == 3 || NX == 4), "one unrolled group of NX steps");
     int c = 0;
-    for (; c + 3 <= nchunks; c += 3) {
-        step(X0, X2, c);
-        step(X1, X0, c + 1);
-        step(X2, X1, c + 2);
+    for (; c + NX <= nchunks; c += NX) {
+        step(P0{}, c);
+        step(P1{}, c + 1);
+        step(P2{}, c + 2);
+        if constexpr (NX == 4) step(P3{}, c + 3);
     }
-    if (c < nchunks) step(X0, X2, c);
-    if (c + 1 < nchunks) step(X1, X0, c + 1);
+    if (c < nchunks) step(P0{}, c);
+    if (c + 1 < nchunks) step(P1{}, c + 1);
+    if constexpr (NX == 4) if (c + 2 < nchunks) step(P2{}, c + 2);
 
     // 32x32 accumulator layout: register r of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
     const int64_t pbase = p0 + wave * 32;
@@ -195,7 +217,7 @@ __device__ __forceinline__ void stack_tile(const StackLaunch &a, int64_t p0, Sta
 template <int NT, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void mask_stack_mfma(StackLaunch a) {
     __shared__ StackLds<NT> sh;
-    stack_tile<NT, WAVES>(a, (int64_t)blockIdx.x * 32 * WAVES, sh);
+    stack_tile<NT, WAVES, WAVES == 8>(a, (int64_t)blockIdx.x * 32 * WAVES, sh);
 }
 
 // Persistent batch: every block takes tiles from one work counter until all cubes are done. A tile is
@@ -221,9 +243,9 @@ __global__ __launch_bounds__(64 * WAVES) void batch_stack_mfma(const StackLaunch
         const StackLaunch a = cubes[lo];
         // a cube of at most 32 masks has its weights in the one-tile layout: half the matrix-core work
         if (NT == 2 && a.kn <= 32)
-            stack_tile<1, WAVES>(a, (item - first[lo]) * 32 * WAVES, *reinterpret_cast<StackLds<1> *>(&sh));
+            stack_tile<1, WAVES, WAVES == 8>(a, (item - first[lo]) * 32 * WAVES, *reinterpret_cast<StackLds<1> *>(&sh));
         else
-            stack_tile<NT, WAVES>(a, (item - first[lo]) * 32 * WAVES, sh);
+            stack_tile<NT, WAVES, WAVES == 8>(a, (item - first[lo]) * 32 * WAVES, sh);
     }
 }
 

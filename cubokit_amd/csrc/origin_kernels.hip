@@ -173,8 +173,15 @@ __device__ __forceinline__ void filter_run(double (&acc)[kPyV], int o, int n, in
         for (int sidx = 0; sidx < kPyV; ++sidx) {
             if (sidx < count) {
                 const double wt = w[R - t + sidx];
+                // the sums, products and accumulations of the kPyV outputs as three independent batches, so
+                // consecutive FP64 operations never wait on each other's results
+                double pr[kPyV];
 #pragma unroll
-                for (int v = 0; v < kPyV; ++v) acc[v] += (lb[v + sidx] + rb[v - sidx + kPyV - 1]) * wt;
+                for (int v = 0; v < kPyV; ++v) pr[v] = lb[v + sidx] + rb[v - sidx + kPyV - 1];
+#pragma unroll
+                for (int v = 0; v < kPyV; ++v) pr[v] = pr[v] * wt;
+#pragma unroll
+                for (int v = 0; v < kPyV; ++v) acc[v] += pr[v];
             }
         }
         t -= kPyV;
