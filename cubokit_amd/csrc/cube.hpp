@@ -7,6 +7,8 @@
 #include <utility>
 #include <vector>
 
+#include "timing.hpp"
+
 namespace cubo {
 struct Feed;
 }
@@ -19,7 +21,7 @@ struct cubo_hip_cube {
     uint64_t bytes;
     void *data;             // one HBM buffer, storage order
     hipStream_t stream;     // every launch on this cube goes to this stream
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;   // fused-kernel launches not yet queried
+    cubo::LaunchTimer timer;   // device time of the fused-kernel launches, until queried
     // A streamed cube holds no whole-cube buffer: `data` is NULL and the scan-row tiles fed by
     // cubo_hip_cube_feed_rows live in the feed's two-slot device ring. Every cube that has been fed owns a feed
     // (pinned double buffers, a copy stream).

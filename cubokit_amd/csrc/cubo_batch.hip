@@ -73,7 +73,7 @@ struct cubo_hip_batch {
     std::string signature;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;
+    cubo::LaunchTimer timer;     // device time of the graph replays, until queried
 };
 
 namespace {
@@ -90,7 +90,7 @@ void free_batch(cubo_hip_batch *b) {
     DeviceGuard g(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
-    for (auto &ev : b->timed) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    b->timer.release();
     for (void *p : b->wb) if (p) (void)hipFree(p);
     for (double *p : b->inv) if (p) (void)hipFree(p);
     if (b->launches) (void)hipFree(b->launches);
@@ -278,13 +278,9 @@ int cubo_hip_batch_tune(cubo_hip_batch *b, int32_t *waves, int32_t *blocks_per_c
 int cubo_hip_batch_run(cubo_hip_batch *b, double *const *y) {
     if (!b) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL batch");
     DeviceGuard g(b->device);
-    hipEvent_t t0, t1;
-    CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
-    CUBO_HIP_TRY(hipEventCreate(&t1), "hipEventCreate");
-    CUBO_HIP_TRY(hipEventRecord(t0, b->stream), "hipEventRecord");
+    CUBO_HIP_TRY(b->timer.start(b->stream), "hipEventRecord");
     CUBO_HIP_TRY(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
-    CUBO_HIP_TRY(hipEventRecord(t1, b->stream), "hipEventRecord");
-    b->timed.emplace_back(t0, t1);
+    CUBO_HIP_TRY(b->timer.stop(b->stream), "hipEventRecord");
     CUBO_HIP_TRY(hipStreamSynchronize(b->stream), "waiting for the batch");
     if (y) {
         // the pinned results (one contiguous range, cube after cube) into the caller's arrays, in pieces on
@@ -305,16 +301,7 @@ int cubo_hip_batch_run(cubo_hip_batch *b, double *const *y) {
 int cubo_hip_batch_time(cubo_hip_batch *b, double *total_ms, int64_t *runs) {
     if (!b || !total_ms || !runs) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     DeviceGuard g(b->device);
-    double t = 0;
-    for (auto &ev : b->timed) {
-        float ms = 0;
-        CUBO_HIP_TRY(hipEventElapsedTime(&ms, ev.first, ev.second), "hipEventElapsedTime");
-        t += ms;
-    }
-    *total_ms = t;
-    *runs = (int64_t)b->timed.size();
-    for (auto &ev : b->timed) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
-    b->timed.clear();
+    CUBO_HIP_TRY(b->timer.total(total_ms, runs), "timing the replays");
     return CUBO_HIP_OK;
 }
 

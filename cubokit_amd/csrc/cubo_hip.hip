@@ -156,7 +156,7 @@ int cubo_hip_cube_destroy(cubo_hip_cube *c) {
     DeviceGuard g(c->device);
     (void)hipStreamSynchronize(c->stream);
     cubo::free_feed(c);
-    for (auto &ev : c->timed) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    c->timer.release();
     if (c->data) (void)hipFree(c->data);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -212,17 +212,7 @@ int cubo_hip_cube_read_rows(cubo_hip_cube *c, int64_t row0, int64_t nrows, void 
 int cubo_hip_cube_kernel_time(cubo_hip_cube *c, double *total_ms, int64_t *launches) {
     if (!c || !total_ms || !launches) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     DeviceGuard g(c->device);
-    double t = 0;
-    for (auto &ev : c->timed) {
-        HIP_TRY(hipEventSynchronize(ev.second), "hipEventSynchronize");
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, ev.first, ev.second), "hipEventElapsedTime");
-        t += ms;
-    }
-    *total_ms = t;
-    *launches = (int64_t)c->timed.size();
-    for (auto &ev : c->timed) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
-    c->timed.clear();
+    HIP_TRY(c->timer.total(total_ms, launches), "timing the launches");
     return CUBO_HIP_OK;
 }
 

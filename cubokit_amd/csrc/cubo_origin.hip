@@ -119,9 +119,6 @@ int cubo_hip_origins(cubo_hip_cube *c, const cubo_hip_origin_request *q, int64_t
     const int64_t pos0 = row0 * c->shape[1], npos = nrows * c->shape[1];
     double *d_out = nullptr;
     CUBO_HIP_TRY(hipMalloc((void **)&d_out, (size_t)npos * 16), "allocating the origins");
-    hipEvent_t t0 = nullptr, t1 = nullptr;      // the launch's device time, reported by cubo_hip_cube_kernel_time
-    if (hipEventCreate(&t0) == hipSuccess && hipEventCreate(&t1) == hipSuccess) c->timed.emplace_back(t0, t1);
-    else { if (t0) (void)hipEventDestroy(t0); t0 = t1 = nullptr; }
     hipError_t e = hipSuccess;
     double *d_w = nullptr, *scratch = nullptr;
     uint8_t *d_take = nullptr;
@@ -151,9 +148,9 @@ int cubo_hip_origins(cubo_hip_cube *c, const cubo_hip_origin_request *q, int64_t
             a.out_row = d_out;
             a.out_col = d_out + npos;
             a.filtered = nullptr;
-            if (t0) e = hipEventRecord(t0, s);
+            e = c->timer.start(s);     // the launch's device time, reported by cubo_hip_cube_kernel_time
             if (e == hipSuccess) e = cubo::launch_py4_origin(a, blocks, s);
-            if (e == hipSuccess && t1) e = hipEventRecord(t1, s);
+            if (e == hipSuccess) e = c->timer.stop(s);
         }
     } else {
         cubo::IWLaunch a{};
@@ -169,9 +166,9 @@ int cubo_hip_origins(cubo_hip_cube *c, const cubo_hip_origin_request *q, int64_t
         a.binned = q->binned_coarse;
         a.out_row = d_out;
         a.out_col = d_out + npos;
-        if (t0) e = hipEventRecord(t0, s);
+        e = c->timer.start(s);
         if (e == hipSuccess) e = cubo::launch_iterated_window(a, s);
-        if (e == hipSuccess && t1) e = hipEventRecord(t1, s);
+        if (e == hipSuccess) e = c->timer.stop(s);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out_row + pos0, d_out, (size_t)npos * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(out_col + pos0, d_out + npos, (size_t)npos * 8, hipMemcpyDeviceToHost, s);

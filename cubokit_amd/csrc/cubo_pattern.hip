@@ -285,11 +285,7 @@ int cubo_hip_pattern_rows(cubo_hip_pattern_session *p, int64_t row0, int64_t nro
         const void *tile = nullptr;
         if (int st = cubo::rows_data(c, row0, nrows, &tile)) return st;
         const void *base = (const void *)((uintptr_t)tile - (uintptr_t)pos0 * (uintptr_t)(c->ndet * cubo::element_size(c->dtype)));
-        hipEvent_t t0 = nullptr, t1 = nullptr;
-        CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
-        if (hipEventCreate(&t1) != hipSuccess) { (void)hipEventDestroy(t0); return fail(CUBO_HIP_ERR_HIP, "hipEventCreate"); }
-        c->timed.emplace_back(t0, t1);
-        CUBO_HIP_TRY(hipEventRecord(t0, s), "hipEventRecord");
+        CUBO_HIP_TRY(c->timer.start(s), "hipEventRecord");
         if (p->fused && (count > 0 || finish)) {
             int64_t ngroups = 0;
             if (count > 0) {
@@ -361,7 +357,7 @@ int cubo_hip_pattern_rows(cubo_hip_pattern_session *p, int64_t row0, int64_t nro
             p->started = true;
             if (count == 0) break;
         }
-        CUBO_HIP_TRY(hipEventRecord(t1, s), "hipEventRecord");
+        CUBO_HIP_TRY(c->timer.stop(s), "hipEventRecord");
         p->next_row = row0 + nrows;
         return CUBO_HIP_OK;
     }
@@ -388,13 +384,9 @@ int cubo_hip_pattern_rows(cubo_hip_pattern_session *p, int64_t row0, int64_t nro
         a.ppg = ppg;
         a.sum = p->part_sum;
         a.max = p->part_max;
-        hipEvent_t t0 = nullptr, t1 = nullptr;
-        CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
-        if (hipEventCreate(&t1) != hipSuccess) { (void)hipEventDestroy(t0); return fail(CUBO_HIP_ERR_HIP, "hipEventCreate"); }
-        c->timed.emplace_back(t0, t1);
-        CUBO_HIP_TRY(hipEventRecord(t0, s), "hipEventRecord");
+        CUBO_HIP_TRY(c->timer.start(s), "hipEventRecord");
         CUBO_HIP_TRY(cubo::launch_pattern(a, p->kind, s), "launching the pattern kernel");
-        CUBO_HIP_TRY(hipEventRecord(t1, s), "hipEventRecord");
+        CUBO_HIP_TRY(c->timer.stop(s), "hipEventRecord");
     }
     if (count > 0 || finish) {
         CUBO_HIP_TRY(cubo::launch_pattern_accumulate(p->kind, p->part_sum, p->part_max, ngroups, c->ndet, p->run_sum,

@@ -323,10 +323,7 @@ int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows) {
     int ngroups = 0, ppg = 0, ppw = 0;
     plan(r, npos, &ngroups, &ppg, &ppw);
     if (ngroups > r->max_groups) return fail(CUBO_HIP_ERR_HIP, "internal: group plan exceeds scratch");
-    hipEvent_t t0, t1;
-    CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
-    CUBO_HIP_TRY(hipEventCreate(&t1), "hipEventCreate");
-    CUBO_HIP_TRY(hipEventRecord(t0, c->stream), "hipEventRecord");
+    CUBO_HIP_TRY(c->timer.start(c->stream), "hipEventRecord");
     hipError_t e;
     if (r->path == 1) {
         cubo::IntLaunch a{};
@@ -360,8 +357,7 @@ int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows) {
         e = cubo::launch_f64(a, c->dtype == CUBO_HIP_U16, r->nw, c->stream);
     }
     if (e != hipSuccess) return hip_fail(e, "launching the fused reduction");
-    CUBO_HIP_TRY(hipEventRecord(t1, c->stream), "hipEventRecord");
-    c->timed.emplace_back(t0, t1);
+    CUBO_HIP_TRY(c->timer.stop(c->stream), "hipEventRecord");
     // The integer kernel in the FULL layout writes its row totals itself.
     if (r->nv > 0 && !r->full) {
         e = r->path == 1 ? cubo::launch_finalize_rows_u64((const uint64_t *)r->rowpart, r->nslices, npos,

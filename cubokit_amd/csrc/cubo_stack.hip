@@ -126,10 +126,7 @@ int cubo_hip_stack_plan_run(cubo_hip_stack_plan *p, double *y) {
     if (!p || !y) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     cubo_hip_cube *c = p->cube;
     DeviceGuard g(c->device);
-    hipEvent_t t0, t1;
-    CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
-    CUBO_HIP_TRY(hipEventCreate(&t1), "hipEventCreate");
-    CUBO_HIP_TRY(hipEventRecord(t0, c->stream), "hipEventRecord");
+    CUBO_HIP_TRY(c->timer.start(c->stream), "hipEventRecord");
     hipError_t e = hipSuccess;
     if (p->method == CUBO_HIP_STACK_MFMA_SPLIT) {
         const size_t per_pass = (size_t)p->ntiles * 4 * 2 * 64 * 8 * p->nchunks;
@@ -165,8 +162,7 @@ int cubo_hip_stack_plan_run(cubo_hip_stack_plan *p, double *y) {
         }
     }
     if (e != hipSuccess) return hip_fail(e, "launching the mask stack");
-    CUBO_HIP_TRY(hipEventRecord(t1, c->stream), "hipEventRecord");
-    c->timed.emplace_back(t0, t1);
+    CUBO_HIP_TRY(c->timer.stop(c->stream), "hipEventRecord");
     CUBO_HIP_TRY(hipMemcpyAsync(y, p->y, (size_t)c->npos * p->k * sizeof(double), hipMemcpyDeviceToHost, c->stream),
                  "reading the stack");
     CUBO_HIP_TRY(hipStreamSynchronize(c->stream), "waiting for the stack");

@@ -333,6 +333,30 @@ def test_view_and_results_agree_and_reset_reuses_the_reduction():
     assert_identical(outs[0]["mean"], ref.mean.values, "mean vs first_look")
 
 
+def test_kernel_time_counts_launches_past_the_pending_limit():
+    """The timing events of launches nobody queries are recycled past 256 pending pairs (folded into running
+    totals), and the query still counts every launch since the previous one."""
+    import ctypes as ct
+    from cubokit_amd import _lib as L
+    shape, dtype, params = synthetic.bandwidth_u16(scan=(4, 4))
+    res = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
+    res.fill_synthetic(params)
+    masks = np.stack([C.DetectorMask.make(C.DetectorGeometry.circle(64, 64, 14), res.shape)])
+    req = L.ReduceRequest(masks=masks.ctypes.data, n_masks=1, want_mean_max=1, want_com=1, com_weights=None)
+    h = ct.c_void_p()
+    assert L.lib().cubo_hip_reduce_begin(res.handle, ct.byref(req), ct.byref(h)) == 0
+    res.kernel_time()
+    v = L.ReduceView()
+    for passes in (300, 3):
+        for _ in range(passes):
+            assert L.lib().cubo_hip_reduce_reset(h) == 0
+            assert L.lib().cubo_hip_reduce_rows(h, 0, 4) == 0
+            assert L.lib().cubo_hip_reduce_view(h, ct.byref(v)) == 0
+        ms, launches = res.kernel_time()
+        assert launches == passes and ms > 0
+    L.lib().cubo_hip_reduce_release(h)
+
+
 def test_clone_copies_the_cube_on_the_device():
     shape, dtype, params = synthetic.bandwidth_u16(scan=(6, 5))
     res = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=dtype)
