@@ -72,7 +72,8 @@ def cpu_baseline(cube, bf, adf, seconds: float):
 def batch_section(C):
     """48 seeded uneven uint16 cubes with 8..64 masks each: one tuned graph replay of the persistent batch
     kernel against a loop of single-cube split-FP16 stacks and a loop of plain stacks (each with its results
-    copied to host memory). GB/s are cube bytes per second of wall time."""
+    copied to host memory), and the replay with the results read in place in its pinned buffer (no copy).
+    GB/s are cube bytes per second of wall time."""
     from cubokit_amd import synthetic
     items = []
     for shape, p, k in synthetic.batch_items():
@@ -90,10 +91,16 @@ def batch_section(C):
     for _ in range(reps):
         b.run(ys)
     graph_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        views = b.run(view=True)                        # the results read in place in pinned host memory
+    view_s = (time.perf_counter() - t0) / reps
+    views_match = all(np.array_equal(v, y) for v, y in zip(views, ys))
     gms, gn = b.kernel_time()
     res = {"cubes": len(items), "cube_bytes": total, "masks": int(sum(m.shape[0] for _, m in items)),
            "tuned": shape, "graph_ms": round(graph_s * 1e3, 3), "graph_device_ms": round(gms / gn, 3),
-           "graph_gbs": round(total / graph_s / 1e9, 1)}
+           "graph_gbs": round(total / graph_s / 1e9, 1), "graph_view_ms": round(view_s * 1e3, 3),
+           "graph_view_gbs": round(total / view_s / 1e9, 1), "view_matches_copy": views_match}
     for name, method, r in (("mfma_loop", C.MaskStack.MFMA_SPLIT, 5), ("plain_loop", C.MaskStack.PLAIN, 2)):
         plans = [C.MaskStack(c, m, method) for c, m in items]
         outs = [p.run() for p in plans]

@@ -160,6 +160,7 @@ SIGNATURES = {
     "cubo_hip_batch_tune": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32),
                                        ct.POINTER(ct.c_int32)]),
     "cubo_hip_batch_run": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "cubo_hip_batch_view": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.POINTER(ct.POINTER(ct.c_double))]),
     "cubo_hip_batch_time": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),
     "cubo_hip_batch_destroy": (ct.c_int, [ct.c_void_p]),
     "cubo_hip_shift_plan_create": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_void_p)]),

@@ -40,7 +40,8 @@ namespace {
 struct Shape {
     int waves = 8;          // 8 (256-position tiles) or 4 (128-position tiles)
     int blocks_per_cu = 1;
-    int order = 0;          // 0: given order; 1: largest detector first (longest items first)
+    int order = 0;          // 0: given order; 1: largest detector first (longest items first); 2: interleaved
+                            // so that result writes to host memory overlap the cube reads
     bool operator<(const Shape &o) const {
         return std::tie(waves, blocks_per_cu, order) < std::tie(o.waves, o.blocks_per_cu, o.order);
     }
@@ -65,7 +66,8 @@ struct cubo_hip_batch {
     double *ymap = nullptr;              // its device address
     size_t ytotal = 0;
     cubo::StackLaunch *launches = nullptr;   // device [n], in item order
-    int64_t *first = nullptr;                // device [n]
+    cubo::BatchItem *items = nullptr;        // device: the items in the order of the shape
+    int64_t max_items = 0;                   // its capacity: the tiles of 128-position items
     unsigned long long *counter = nullptr;
     hipStream_t stream = nullptr;
     Shape shape;
@@ -74,6 +76,7 @@ struct cubo_hip_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     cubo::LaunchTimer timer;     // device time of the graph replays, until queried
+    bool ran = false;            // the pinned buffer holds the results of a run
 };
 
 namespace {
@@ -94,26 +97,58 @@ void free_batch(cubo_hip_batch *b) {
     for (void *p : b->wb) if (p) (void)hipFree(p);
     for (double *p : b->inv) if (p) (void)hipFree(p);
     if (b->launches) (void)hipFree(b->launches);
-    if (b->first) (void)hipFree(b->first);
+    if (b->items) (void)hipFree(b->items);
     if (b->counter) (void)hipFree(b->counter);
     if (b->yhost) (void)hipHostFree(b->yhost);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
-// Item order and per-cube descriptors for a shape; uploads them and sets total_tiles.
-hipError_t upload_plan(cubo_hip_batch *b, const Shape &s) {
+// Rates that set the interleaved order: the stack kernel reads the cubes from HBM at about 6.5 TB/s and
+// writes results into mapped host memory at about 52 GB/s (both measured on MI355X). A tile whose results
+// take longer to write than its positions take to read is write-heavy.
+constexpr double kReadBytesPerS = 6.5e12;
+constexpr double kHostWriteBytesPerS = 52e9;
+
+// The items of a shape: every cube's tiles, cube by cube in the given order or largest detector first, or
+// interleaved: write-heavy and read-heavy tiles merged (each kind keeping cube order) so that the time the
+// items so far spend writing to host memory keeps level with the time they spend reading, and the two overlap.
+std::vector<cubo::BatchItem> plan_items(const cubo_hip_batch *b, const Shape &s) {
+    const int64_t bm = 32 * s.waves;
     std::vector<int> order(b->n);
     std::iota(order.begin(), order.end(), 0);
     if (s.order == 1)
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->ndet[x] > b->ndet[y]; });
-    const int64_t bm = 32 * s.waves;
+    std::vector<cubo::BatchItem> items, heavy, light;
+    std::vector<double> heavy_t[2], light_t[2];      // per item: [0] read seconds, [1] write seconds
+    for (int i : order) {
+        const int64_t tiles = (b->npos[i] + bm - 1) / bm;
+        for (int64_t t = 0; t < tiles; ++t) {
+            const cubo::BatchItem it{i, (int32_t)t};
+            if (s.order != 2) { items.push_back(it); continue; }
+            const double rows = (double)std::min(bm, b->npos[i] - t * bm);
+            const double rd = rows * (double)b->ndet[i] * 2 / kReadBytesPerS;
+            const double wr = rows * (double)b->k[i] * 8 / kHostWriteBytesPerS;
+            if (wr > rd) { heavy.push_back(it); heavy_t[0].push_back(rd); heavy_t[1].push_back(wr); }
+            else { light.push_back(it); light_t[0].push_back(rd); light_t[1].push_back(wr); }
+        }
+    }
+    if (s.order != 2) return items;
+    double read_s = 0, write_s = 0;
+    size_t h = 0, l = 0;
+    while (h < heavy.size() || l < light.size()) {
+        const bool take_heavy = l == light.size() || (h < heavy.size() && write_s <= read_s);
+        if (take_heavy) { items.push_back(heavy[h]); read_s += heavy_t[0][h]; write_s += heavy_t[1][h]; ++h; }
+        else { items.push_back(light[l]); read_s += light_t[0][l]; write_s += light_t[1][l]; ++l; }
+    }
+    return items;
+}
+
+// Per-cube descriptors and the item order for a shape; uploads them and sets total_tiles.
+hipError_t upload_plan(cubo_hip_batch *b, const Shape &s) {
     std::vector<cubo::StackLaunch> desc(b->n);
-    std::vector<int64_t> first(b->n);
-    int64_t t = 0;
-    for (int j = 0; j < b->n; ++j) {
-        const int i = order[j];
-        cubo::StackLaunch &a = desc[j];
+    for (int i = 0; i < b->n; ++i) {
+        cubo::StackLaunch &a = desc[i];
         a.cube = (const uint16_t *)b->cubes[i]->data;
         a.npos = b->npos[i];
         a.ndet = b->ndet[i];
@@ -126,23 +161,23 @@ hipError_t upload_plan(cubo_hip_batch *b, const Shape &s) {
         a.tile_lo[0] = b->tile_lo[i][0];
         a.tile_lo[1] = b->tile_lo[i][1];
         a.y = b->ymap + b->yoff[i];
-        first[j] = t;
-        t += (b->npos[i] + bm - 1) / bm;
     }
-    if (t >= INT32_MAX) return hipErrorInvalidValue;      // the kernel numbers items in 32 bits
+    const std::vector<cubo::BatchItem> items = plan_items(b, s);
+    const int64_t t = (int64_t)items.size();
+    if (t >= INT32_MAX || t > b->max_items) return hipErrorInvalidValue;   // the kernel numbers items in 32 bits
     b->total_tiles = t;
     hipError_t e = hipMemcpyAsync(b->launches, desc.data(), desc.size() * sizeof(desc[0]), hipMemcpyHostToDevice,
                                   b->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->first, first.data(), first.size() * sizeof(int64_t), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess && t > 0)
+        e = hipMemcpyAsync(b->items, items.data(), items.size() * sizeof(items[0]), hipMemcpyHostToDevice,
+                           b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     return e;
 }
 
 hipError_t launch(cubo_hip_batch *b, const Shape &s) {
     const int blocks = cubo::compute_units(b->device) * s.blocks_per_cu;
-    return cubo::launch_batch_stack(b->launches, b->first, b->n, b->total_tiles, b->counter, s.waves, blocks,
-                                    b->stream);
+    return cubo::launch_batch_stack(b->launches, b->items, b->total_tiles, b->counter, s.waves, blocks, b->stream);
 }
 
 int build_graph(cubo_hip_batch *b) {
@@ -214,7 +249,8 @@ int cubo_hip_batch_create(int32_t n, cubo_hip_cube *const *cubes, const float *c
         if (e != hipSuccess) { free_batch(b); return hip_fail(e, "preparing the batch weights"); }
     }
     e = hipMalloc((void **)&b->launches, n * sizeof(cubo::StackLaunch));
-    if (e == hipSuccess) e = hipMalloc((void **)&b->first, n * sizeof(int64_t));
+    for (int i = 0; i < n; ++i) b->max_items += (b->npos[i] + 127) / 128;
+    if (e == hipSuccess) e = hipMalloc((void **)&b->items, std::max<int64_t>(b->max_items, 1) * sizeof(cubo::BatchItem));
     if (e == hipSuccess) e = hipMalloc((void **)&b->counter, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->yhost, std::max<size_t>(b->ytotal, 1) * sizeof(double),
                                            hipHostMallocMapped | hipHostMallocCoherent);
@@ -237,7 +273,7 @@ int cubo_hip_batch_tune(cubo_hip_batch *b, int32_t *waves, int32_t *blocks_per_c
     std::vector<Shape> cand;
     for (int w : {8, 4})
         for (int bpc = 1; bpc <= cubo::batch_blocks_per_cu(w); ++bpc)
-            for (int o : {0, 1}) cand.push_back(Shape{w, bpc, o});
+            for (int o : {0, 1, 2}) cand.push_back(Shape{w, bpc, o});
     hipEvent_t t0, t1;
     CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
     CUBO_HIP_TRY(hipEventCreate(&t1), "hipEventCreate");
@@ -282,6 +318,7 @@ int cubo_hip_batch_run(cubo_hip_batch *b, double *const *y) {
     CUBO_HIP_TRY(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
     CUBO_HIP_TRY(b->timer.stop(b->stream), "hipEventRecord");
     CUBO_HIP_TRY(hipStreamSynchronize(b->stream), "waiting for the batch");
+    b->ran = true;
     if (y) {
         // the pinned results (one contiguous range, cube after cube) into the caller's arrays, in pieces on
         // several host threads
@@ -302,6 +339,14 @@ int cubo_hip_batch_time(cubo_hip_batch *b, double *total_ms, int64_t *runs) {
     if (!b || !total_ms || !runs) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     DeviceGuard g(b->device);
     CUBO_HIP_TRY(b->timer.total(total_ms, runs), "timing the replays");
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_batch_view(const cubo_hip_batch *b, int32_t i, const double **y) {
+    if (!b || !y) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (i < 0 || i >= b->n) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "batch item %d of %d", i, b->n);
+    if (!b->ran) return fail(CUBO_HIP_ERR_INCOMPLETE, "the batch has not run");
+    *y = b->yhost + b->yoff[i];
     return CUBO_HIP_OK;
 }
 

@@ -220,11 +220,11 @@ __global__ __launch_bounds__(64 * WAVES) void mask_stack_mfma(StackLaunch a) {
     stack_tile<NT, WAVES, WAVES == 8>(a, (int64_t)blockIdx.x * 32 * WAVES, sh);
 }
 
-// Persistent batch: every block takes tiles from one work counter until all cubes are done. A tile is
-// (cube, 32 * WAVES positions); first[c] is cube c's first tile number.
+// Persistent batch: every block takes items from one work counter until all are done. Item i is tile
+// items[i].tile (32 * WAVES positions) of cube items[i].cube; the host chooses the order.
 template <int NT, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void batch_stack_mfma(const StackLaunch *__restrict__ cubes,
-                                                              const int64_t *__restrict__ first, int ncubes,
+                                                              const BatchItem *__restrict__ items,
                                                               int64_t total, unsigned long long *counter) {
     __shared__ StackLds<NT> sh;
     __shared__ int item_s;
@@ -235,17 +235,14 @@ __global__ __launch_bounds__(64 * WAVES) void batch_stack_mfma(const StackLaunch
         const int item = __builtin_amdgcn_readfirstlane(item_s);
         __syncthreads();                                   // every thread has read item_s
         if (item >= total) break;                          // block-uniform: every block leaves
-        int lo = 0, hi = ncubes - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (first[mid] <= item) lo = mid; else hi = mid - 1;
-        }
-        const StackLaunch a = cubes[lo];
+        const BatchItem it = items[item];
+        const StackLaunch a = cubes[it.cube];
+        const int64_t p0 = (int64_t)it.tile * 32 * WAVES;
         // a cube of at most 32 masks has its weights in the one-tile layout: half the matrix-core work
         if (NT == 2 && a.kn <= 32)
-            stack_tile<1, WAVES, WAVES == 8>(a, (item - first[lo]) * 32 * WAVES, *reinterpret_cast<StackLds<1> *>(&sh));
+            stack_tile<1, WAVES, WAVES == 8>(a, p0, *reinterpret_cast<StackLds<1> *>(&sh));
         else
-            stack_tile<NT, WAVES, WAVES == 8>(a, (item - first[lo]) * 32 * WAVES, sh);
+            stack_tile<NT, WAVES, WAVES == 8>(a, p0, sh);
     }
 }
 
@@ -370,13 +367,13 @@ int batch_blocks_per_cu(int waves) {
     return e == hipSuccess && n > 0 ? n : 1;
 }
 
-hipError_t launch_batch_stack(const StackLaunch *cubes, const int64_t *first, int ncubes, int64_t total,
+hipError_t launch_batch_stack(const StackLaunch *cubes, const BatchItem *items, int64_t total,
                               unsigned long long *counter, int waves, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, counter);
     if (waves == 8)
-        hipLaunchKernelGGL((batch_stack_mfma<2, 8>), dim3(blocks), dim3(512), 0, s, cubes, first, ncubes, total, counter);
+        hipLaunchKernelGGL((batch_stack_mfma<2, 8>), dim3(blocks), dim3(512), 0, s, cubes, items, total, counter);
     else if (waves == 4)
-        hipLaunchKernelGGL((batch_stack_mfma<2, 4>), dim3(blocks), dim3(256), 0, s, cubes, first, ncubes, total, counter);
+        hipLaunchKernelGGL((batch_stack_mfma<2, 4>), dim3(blocks), dim3(256), 0, s, cubes, items, total, counter);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -30,10 +30,14 @@ struct PlainStackLaunch {
 };
 
 hipError_t launch_mask_stack_mfma(const StackLaunch &a, int ntiles, hipStream_t s);
+// One work item of a persistent batch: tile `tile` (32 * waves positions) of cube `cube`.
+struct BatchItem {
+    int32_t cube, tile;
+};
 // Persistent batch of uint16 cubes (at most 64 masks each): resets the work counter with a kernel, then
-// launches `blocks` blocks of `waves` waves that take (cube, tile) items until none is left.
+// launches `blocks` blocks of `waves` waves that take items[0, total) in order until none is left.
 int batch_blocks_per_cu(int waves);
-hipError_t launch_batch_stack(const StackLaunch *cubes, const int64_t *first, int ncubes, int64_t total,
+hipError_t launch_batch_stack(const StackLaunch *cubes, const BatchItem *items, int64_t total,
                               unsigned long long *counter, int waves, int blocks, hipStream_t s);
 hipError_t launch_mask_stack_plain(const PlainStackLaunch &a, bool is_u16, int kpad, hipStream_t s);
 

@@ -1029,18 +1029,31 @@ class StackBatch:
         w, b, o = ct.c_int32(), ct.c_int32(), ct.c_int32()
         _check(L.lib().cubo_hip_batch_tune(self._h, ct.byref(w), ct.byref(b), ct.byref(o)), "tuning the batch")
         self.shape = {"waves": w.value, "blocks_per_cu": b.value,
-                      "order": "largest detector first" if o.value else "given"}
+                      "order": ("given", "largest detector first", "interleaved")[o.value]}
         return self.shape
 
-    def run(self, out: Optional[Sequence[np.ndarray]] = None, copy_out: bool = True) -> list:
-        """[(scanCount_i, k_i) Double] per item; copy_out=False replays the graph without the host copy."""
-        if not copy_out:
+    def run(self, out: Optional[Sequence[np.ndarray]] = None, copy_out: bool = True, view: bool = False) -> list:
+        """[(scanCount_i, k_i) Double] per item; copy_out=False replays the graph without the host copy.
+        view=True returns read-only arrays over the batch's pinned result buffer (no copy), valid until the next
+        run or release."""
+        if view or not copy_out:
             _check(L.lib().cubo_hip_batch_run(self._h, None), "running the batch")
-            return []
+            return self.view() if view else []
         ys = list(out) if out is not None else [np.empty((c.shape.scan_count, k)) for c, k in zip(self.cubes, self.k)]
         ptrs = (ct.c_void_p * len(ys))(*[y.ctypes.data for y in ys])
         _check(L.lib().cubo_hip_batch_run(self._h, ptrs), "running the batch")
         return ys
+
+    def view(self) -> list:
+        """The last run's results in place: read-only (scanCount_i, k_i) arrays over the pinned buffer."""
+        out = []
+        for i, (c, k) in enumerate(zip(self.cubes, self.k)):
+            p = ct.POINTER(ct.c_double)()
+            _check(L.lib().cubo_hip_batch_view(self._h, i, ct.byref(p)), "viewing the batch results")
+            a = np.ctypeslib.as_array(p, (c.shape.scan_count, k))
+            a.flags.writeable = False
+            out.append(a)
+        return out
 
     def kernel_time(self):
         """(total device milliseconds, replays) since the previous call."""

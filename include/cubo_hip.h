@@ -302,12 +302,16 @@ typedef struct cubo_hip_batch cubo_hip_batch;
 int cubo_hip_batch_create(int32_t n, cubo_hip_cube *const *cubes, const float *const *masks, const int32_t *k,
                           cubo_hip_batch **out);
 /* Times every launch shape (waves per block 8 or 4, resident blocks per compute unit, items in the given
- * order or largest detector first) on the device, keeps the fastest for this batch and for every later
- * batch of the same shapes in this process, and rebuilds the graph. Any output pointer may be NULL. */
+ * order (0), largest detector first (1) or interleaved so that writing results to host memory overlaps
+ * reading the cubes (2)) on the device, keeps the fastest for this batch and for every later batch of the
+ * same shapes in this process, and rebuilds the graph. Any output pointer may be NULL. */
 int cubo_hip_batch_tune(cubo_hip_batch *batch, int32_t *waves, int32_t *blocks_per_cu, int32_t *order);
 /* One graph replay: counter reset, the persistent launch, copies of every result. y[i]: scanCount_i x k_i
  * Doubles, caller-owned host memory (NULL entries or y == NULL skip the copy out). */
 int cubo_hip_batch_run(cubo_hip_batch *batch, double *const *y);
+/* Cube i's results of the last run in place, scanCount_i x k_i Doubles in the batch's pinned host buffer (no
+ * copy): valid until the next cubo_hip_batch_run or cubo_hip_batch_destroy. An error before the first run. */
+int cubo_hip_batch_view(const cubo_hip_batch *batch, int32_t i, const double **y);
 /* Device time of the graph replays since the previous query: total milliseconds and replay count. */
 int cubo_hip_batch_time(cubo_hip_batch *batch, double *total_ms, int64_t *runs);
 int cubo_hip_batch_destroy(cubo_hip_batch *batch);

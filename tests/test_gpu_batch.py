@@ -4,10 +4,13 @@ Each item must give exactly the bits of the same cube's single split-FP16 mask s
 accumulation order per position), and lie within 1e-6 of sum |x w| of the oracle's FP64 contraction
 (oc_mask_stack). Graph replays with other device work between them must repeat the first result bit for bit.
 """
+import ctypes as ct
+
 import numpy as np
 import pytest
 
 import cubokit_amd as C
+from cubokit_amd import _lib as L
 from cubokit_amd import synthetic
 from oracle import oracle as O
 
@@ -75,6 +78,23 @@ def test_graph_replays_repeat_the_first_result():
             assert np.array_equal(x, y), f"replay {rep} item {j} differs"
     b.run(copy_out=False)
     assert b.kernel_time()[1] == 6
+
+
+def test_view_reads_the_results_in_place():
+    """cubo_hip_batch_view: an error before the first run; afterwards each cube's results in the pinned buffer,
+    the same bits as the copy out, and read-only."""
+    items = _batch_items(seed=11)
+    b = C.StackBatch(items)
+    with pytest.raises(C.CuboVirtualError):
+        b.view()
+    copied = b.run()
+    viewed = b.run(view=True)
+    for j, (x, y) in enumerate(zip(copied, viewed)):
+        assert y.shape == x.shape and np.array_equal(x, y), f"item {j}: view differs from the copy"
+        assert not y.flags.writeable
+    ptr = ct.POINTER(ct.c_double)()
+    assert L.lib().cubo_hip_batch_view(b._h, len(items), ct.byref(ptr)) != 0       # no such item
+    assert L.lib().cubo_hip_batch_view(b._h, -1, ct.byref(ptr)) != 0
 
 
 def test_tuned_shape_gives_identical_results():
