@@ -133,8 +133,9 @@ def batch_section(C):
 def diff_section(C):
     """The 512x512x128x128 uint16 differentiation workload (8 GiB): origins from one fused CoM pass, the K = 16
     + CoM shifted stack forward, 20 fixed-step DPC iterations on the 512x512 centre-of-mass field and their
-    adjoint, then the stack's adjoint with respect to the masks and the origins (both re-reading the cube).
-    Kernel times from HIP events; GB/s are cube bytes per kernel second."""
+    adjoint, then the stack's adjoint with respect to the masks and the origins (both re-reading the cube), and
+    CuboPhase's plane reference and rotation solve on the same field. Kernel times from HIP events; GB/s are cube
+    bytes per kernel second; the phase calls are timed whole (host arrays in and out)."""
     from cubokit_amd import synthetic
     from cubokit_amd.phase import DPC
     shape, dtype, params = synthetic.differentiation_u16()
@@ -181,6 +182,19 @@ def diff_section(C):
     bwd_wall = (time.perf_counter() - t0) / 2
     ms, calls = st.kernel_time()
     org_ms, msk_ms = ms[1] / calls, ms[2] / calls
+    # CuboPhase's reference plane and rotation on the same 512x512 centre-of-mass field (host arrays in and out,
+    # so the call times include the copies over the host link)
+    from cubokit_amd.phase import normalize, rotation_solve
+    _, _, nx, ny = normalize(comr, comc, shape[0], shape[1])         # warm-up
+    t0 = time.perf_counter()
+    for _ in range(5):
+        _, _, nx, ny = normalize(comr, comc, shape[0], shape[1])
+    norm_ms = (time.perf_counter() - t0) / 5 * 1e3
+    rot = rotation_solve(nx, ny, shape[0], shape[1])                 # warm-up
+    t0 = time.perf_counter()
+    for _ in range(5):
+        rot = rotation_solve(nx, ny, shape[0], shape[1])
+    rot_ms = (time.perf_counter() - t0) / 5 * 1e3
     ok = bool(np.all(np.isfinite(dm)) and np.all(np.isfinite(dr)) and np.all(np.isfinite(phase)))
     res = {"workload": "512x512x128x128 uint16 (8 GiB), disk r=12, 8-sinusoid +-2 px deflection; K=16 masks + CoM "
                        "rows, bilinear per-position origin; 20 fixed-step DPC iterations on the 512x512 field",
@@ -189,9 +203,12 @@ def diff_section(C):
            "mask_adjoint_ms": round(msk_ms, 3), "mask_adjoint_gbs": round(nb / msk_ms / 1e6, 1),
            "dpc_20_steps_and_adjoint_device_ms": round(dpc_ms, 3), "dpc_20_steps_and_adjoint_wall_ms": round(dpc_wall * 1e3, 3),
            "forward_call_wall_ms": round(fwd_wall * 1e3, 3), "backward_call_wall_ms": round(bwd_wall * 1e3, 3),
-           "dpc_final_error": float(errors[-1]), "finite": ok}
+           "dpc_final_error": float(errors[-1]), "plane_normalize_call_ms": round(norm_ms, 3),
+           "rotation_solve_call_ms": round(rot_ms, 3), "rotation_degrees": rot["degrees"],
+           "rotation_transposed": rot["transposed"], "finite": ok}
     log(f"differentiation: forward {fwd_ms:.3f} ms, origin adjoint {org_ms:.3f} ms, mask adjoint {msk_ms:.3f} ms, "
-        f"DPC 20 steps + adjoint {dpc_ms:.3f} ms device ({dpc_wall * 1e3:.1f} ms wall)")
+        f"DPC 20 steps + adjoint {dpc_ms:.3f} ms device ({dpc_wall * 1e3:.1f} ms wall); plane reference "
+        f"{norm_ms:.3f} ms, rotation (358 scores) {rot_ms:.3f} ms per call")
     st.release()
     dpc.release()
     cube.release()
