@@ -149,12 +149,19 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin(Py4Launch a) {
 // two per output and tap. The operation sequence of every output is the reference's.
 constexpr int kPyV = 16;
 
+// min(max(k, 0), hi) for hi >= 0 in one instruction (the compiler emits a max and a min: it cannot know 0 <= hi).
+__device__ __forceinline__ int clamp_index(int k, int hi) {
+    int r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(k), "v"(hi));
+    return r;
+}
+
 // acc[v] = the filtered values at o .. o + kPyV - 1 of a line x(0 .. n - 1) with nearest-edge extension: x(o) times
 // the centre weight, then (x(o - t) + x(o + t)) * w[R - t] for t = R down to 1.
 template <typename X>
 __device__ __forceinline__ void filter_run(double (&acc)[kPyV], int o, int n, int R, const double *w, double centre,
                                            X x) {
-    auto at = [&](int k) { return x(min(max(k, 0), n - 1)); };
+    auto at = [&](int k) { return x(clamp_index(k, n - 1)); };
 #pragma unroll
     for (int v = 0; v < kPyV; ++v) acc[v] = at(o + v) * centre;
     int t = R;
@@ -235,6 +242,8 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin_lds(Py4Launch a, int lS, 
     const int qy = a.qy, qx = a.qx, R = a.R, ndet = qy * qx, S = 1 << lS, SP = S + 1;
     auto row_at = [&](int k) { return POW2 ? k << lqx : k * qx; };     // k * qx
     auto col_at = [&](int k) { return (k << lS) + k; };                  // k * (S + 1)
+    constexpr int lT = sizeof(T) == 2 ? 1 : 2;
+    const unsigned qxb = (unsigned)qx * sizeof(T), sp8 = (unsigned)SP * 8;   // row strides in bytes (< 2^24)
     T *raw = (T *)py_lds;                                  // [ndet], padded to 16 bytes
     double *tr = py_lds + ((ndet * (int)sizeof(T) + 15) & ~15) / 8;   // [qx][S + 1]: axis 0, transposed
     double *wl = tr + qx * SP;                             // [2R + 1]
@@ -270,7 +279,11 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin_lds(Py4Launch a, int lS, 
             for (int it = tid; it < groups * qx; it += kPyBlock) {
                 const int g = it / qx, j = it - g * qx;
                 double acc[kPyV];
-                filter_run(acc, s0 + g * kPyV, qy, R, wl, centre, [&](int k) { return (double)raw[row_at(k) + j]; });
+                // byte addresses in one shift-add (POW2) or multiply-add per load
+                const char *rj = (const char *)(raw + j);
+                filter_run(acc, s0 + g * kPyV, qy, R, wl, centre, [&](int k) {
+                    return (double)*(const T *)(rj + (POW2 ? k << (lqx + lT) : (int)__umul24(k, qxb)));
+                });
 #pragma unroll
                 for (int v = 0; v < kPyV; ++v)
                     if (g * kPyV + v < rows) tr[col_at(j) + g * kPyV + v] = acc[v];
@@ -281,7 +294,9 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin_lds(Py4Launch a, int lS, 
             for (int it = tid; it < cgroups * rows; it += kPyBlock) {
                 const int c = it / rows, il = it - c * rows;
                 double acc[kPyV];
-                filter_run(acc, c * kPyV, qx, R, wl, centre, [&](int k) { return tr[col_at(k) + il]; });
+                const char *ti = (const char *)(tr + il);
+                filter_run(acc, c * kPyV, qx, R, wl, centre,
+                           [&](int k) { return *(const double *)(ti + __umul24(k, sp8)); });
 #pragma unroll
                 for (int v = 0; v < kPyV; ++v) {
                     const int j = c * kPyV + v;
