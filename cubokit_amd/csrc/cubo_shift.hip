@@ -267,6 +267,7 @@ cubo::ShiftLaunch launch_args(const cubo_hip_shift_plan *p) {
     a.nchunks = p->nchunks;
     a.k = p->k;
     a.any_lo = p->any_lo;
+    a.row_chunks = c->shape[3] % 64 == 0 ? (int32_t)(c->shape[3] / 64) : 0;
     a.wroll = p->wroll;
     a.slot_stride = p->slot_stride;
     a.inv_scale = p->inv;

@@ -34,15 +34,24 @@ __device__ __forceinline__ uint32_t cvt_pair(uint32_t w) {
 }
 
 // NC corners: 4 for the bilinear shift, 1 for the rounded one (MaskShift.rounded: one roll, weight 1).
-template <int WAVES, bool BWD, int NC>
+// D > 0 (the detector's rows are D whole chunks): the corners one row down, (ir + 1, ic) and (ir + 1, ic + 1),
+// hold the weights of the corners above them D chunks earlier (a roll by one row moves every pixel qx = 64 D
+// places), so only corners 0 and 2 are loaded and LDS keeps a ring of D + 2 chunks of them; the lower corners
+// read the ring D chunks back. The ring's slots of chunks -D .. -1 hold the last D chunks (the roll wraps).
+template <int WAVES, bool BWD, int NC, int D>
 __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
-    static_assert(NC == 4 || (NC == 1 && !BWD), "the adjoint needs the four corners");
+    static_assert(NC == 4 || (NC == 1 && !BWD && D == 0), "the adjoint needs the four corners");
     constexpr int kCorners = NC;
-    constexpr int WCH = kCorners * kTileEntries;             // entries of one chunk, NC corners
+    constexpr bool RING = D > 0;
+    constexpr int RS = RING ? D + 2 : 2;                     // chunk buffers in LDS
+    constexpr int NCL = RING ? 2 : kCorners;                 // corners loaded per chunk
+    constexpr int WCH = NCL * kTileEntries;                  // entries of one chunk buffer
     constexpr int TP = 32 * WAVES;
     constexpr int WPT = WCH / (64 * WAVES);
     static_assert(WCH % (64 * WAVES) == 0, "every thread carries the same number of weight entries");
-    __shared__ half8 wlds[2][WCH];
+    __shared__ half8 wlds[RS][WCH];
+    // the buffer of chunk c (c >= -D)
+    auto buf_of = [](int c) { return RING ? (c + RS) % RS : c & 1; };
     __shared__ double wts[2][TP];
 
     const ShiftTile t = a.tiles[blockIdx.x];
@@ -78,7 +87,8 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
 #pragma unroll
         for (int j = 0; j < WPT; ++j) {
             const int e = tid + j * 64 * WAVES;
-            const int corner = (j * 64 * WAVES) / kTileEntries + (64 * WAVES >= kTileEntries ? 0 : tid / kTileEntries);
+            const int ci = (j * 64 * WAVES) / kTileEntries + (64 * WAVES >= kTileEntries ? 0 : tid / kTileEntries);
+            const int corner = RING ? 2 * ci : ci;
             wreg[set][j] = wsrc[(size_t)t.slot[corner] * a.slot_stride + (size_t)chunk * kTileEntries + (e % kTileEntries)];
         }
     };
@@ -128,14 +138,16 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) any |= (x[s].x | x[s].y | x[s].z | x[s].w) & 0xf800f800u;
         const bool use_hi = __builtin_amdgcn_ballot_w64(any != 0) != 0;
-        const half8 *wl = wlds[chunk & 1];
+        const half8 *wl = wlds[buf_of(chunk)];
+        const half8 *wd = RING ? wlds[buf_of(chunk - D)] : wl;    // the lower corners' weights
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             half8 bh[kCorners], bl[kCorners];
 #pragma unroll
             for (int c = 0; c < kCorners; ++c) {
-                bh[c] = wl[c * kTileEntries + (s * 2 + 0) * 64 + lane];
-                bl[c] = wl[c * kTileEntries + (s * 2 + 1) * 64 + lane];
+                const half8 *wc = RING ? ((c & 1) ? wd : wl) + (c >> 1) * kTileEntries : wl + c * kTileEntries;
+                bh[c] = wc[(s * 2 + 0) * 64 + lane];
+                bl[c] = wc[(s * 2 + 1) * 64 + lane];
             }
             const uint32_t w[4] = {x[s].x, x[s].y, x[s].z, x[s].w};
             u32x4 lo, hi;
@@ -170,11 +182,19 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
     // c; in the forward they were fetched during step c - 1 (a whole step of latency), in the adjoint during
     // step c.
     u32x4 X0[4], X1[4], X2[4];
+    if constexpr (RING) {
+        // the last D chunks into the slots of chunks -D .. -1, before the first data loads
+#pragma unroll
+        for (int v = -D; v < 0; ++v) {
+            fetch_w(0, ((v % nchunks) + nchunks) % nchunks);
+            put_w(0, buf_of(v));
+        }
+    }
     fetch_w(0, 0);
     __builtin_amdgcn_sched_barrier(0);
     load_x(X0, 0);
     load_x(X1, min(1, nchunks - 1));
-    put_w(0, 0);
+    put_w(0, buf_of(0));
     if constexpr (DEEP) fetch_w(1, min(1, nchunks - 1));
     __syncthreads();                                   // also publishes wts
     // cur: the register set holding chunk c + 1 (the forward alternates: set (c + 1) & 1)
@@ -184,7 +204,7 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
         __builtin_amdgcn_sched_barrier(0);
         load_x(xn, min(c + 2, nchunks - 1));
         compute(xc, c);
-        put_w(DEEP ? cur : 0, (c + 1) & 1);
+        put_w(DEEP ? cur : 0, buf_of(c + 1));
         lds_barrier();
     };
     int c = 0;
@@ -547,12 +567,23 @@ __global__ void sum_groups(const double *partial, int ngroups, int64_t n, double
 hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, bool rounded, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
     if (backward && rounded) return hipErrorInvalidValue;
-    if (backward)
-        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, true, 4>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
-    else if (rounded)
-        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false, 1>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
-    else
-        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false, 4>), dim3(ntiles), dim3(64 * kShiftWaves), 0, s, a);
+    const dim3 grid(ntiles), block(64 * kShiftWaves);
+    // the weight ring for detectors whose rows are 1, 2 or 4 whole chunks (64, 128 or 256 columns)
+#define CUBO_SHIFT_FOUR(BWD)                                                                            \
+    switch (a.row_chunks) {                                                                             \
+        case 1: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, BWD, 4, 1>), grid, block, 0, s, a); break; \
+        case 2: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, BWD, 4, 2>), grid, block, 0, s, a); break; \
+        case 4: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, BWD, 4, 4>), grid, block, 0, s, a); break; \
+        default: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, BWD, 4, 0>), grid, block, 0, s, a); break; \
+    }
+    if (backward) {
+        CUBO_SHIFT_FOUR(true)
+    } else if (rounded) {
+        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false, 1, 0>), grid, block, 0, s, a);
+    } else {
+        CUBO_SHIFT_FOUR(false)
+    }
+#undef CUBO_SHIFT_FOUR
     return hipGetLastError();
 }
 

@@ -28,6 +28,7 @@ struct ShiftLaunch {
     int32_t nchunks;          // ceil(ndet / 64)
     int32_t k;                // masks (at most 32)
     int32_t any_lo;           // some weight needs its FP16 lo part
+    int32_t row_chunks;       // qx / 64 when qx is a multiple of 64 (a roll by one row = row_chunks chunks), else 0
     const void *wroll;        // [slot][chunk][kstep][hi, lo][lane][8] FP16, the split weights rolled by the slot's shift
     int64_t slot_stride;      // 16-byte entries per slot
     const double *inv_scale;  // [k] 1 / (power-of-two column scale)

@@ -53,9 +53,12 @@ def _host(res):
     return res.read_scan_tile(range(0, res.shape.scan_rows))
 
 
-def test_forward_matches_the_oracle_per_mask():
-    res = _cube((6, 7), 32, 300.0, 1)
-    masks = _masks(32, 5, 2, signed=True)
+# 32 columns: every corner's weights loaded per chunk; 64, 128 and 256 columns (1, 2 and 4 chunks per row): the
+# lower corners' weights taken from the LDS ring of the upper corners' chunks
+@pytest.mark.parametrize("det", [32, 64, 128, 256])
+def test_forward_matches_the_oracle_per_mask(det):
+    res = _cube((6, 7), det, 300.0, 1)
+    masks = _masks(det, 5, 2, signed=True)
     orow, ocol = _origins(42, 3)
     y = C.ShiftedStack(res, masks).forward(orow, ocol)
     host = _host(res)
@@ -74,10 +77,11 @@ def test_equal_origins_give_the_plain_stack_bit_for_bit():
     assert np.array_equal(y, C.MaskStack(res, masks, C.MaskStack.MFMA_SPLIT).run())
 
 
-@pytest.mark.parametrize("det", [32, 24, 96, 160])
+@pytest.mark.parametrize("det", [32, 24, 96, 160, 64, 128, 256])
 def test_backward_matches_the_oracle(det):
     """Rows of 32, 96 and 160 px: the matrix-core mask adjoint (bands of three output rows for 32 and 96, of one for
-    160; 96 px rows stage with 9 waves, not a divisor of the batch); 24 px rows: the vector-ALU one."""
+    160; 96 px rows stage with 9 waves, not a divisor of the batch); 24 px rows: the vector-ALU one. Rows of 64,
+    128 and 256 px: the origin adjoint's lower corners from the LDS weight ring."""
     res = _cube((9, 11), det, 400.0, 6)         # 99 positions in many shift classes: partial tiles
     masks = _masks(det, 6, 7, signed=True)
     orow, ocol = _origins(99, 8)
