@@ -22,16 +22,40 @@ __global__ __launch_bounds__(256) void fft_rows(double2 *data, int n, const doub
     for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = x[i];
     __syncthreads();
     if ((n & (n - 1)) == 0) {
-        const int half = n >> 1;
-        for (int ns = 1; ns < n; ns <<= 1) {
+        // Stockham autosort stages: one radix-2 stage when log2 n is odd, then radix-4 stages (half the block
+        // barriers of radix-2 throughout). A stage of radix r and span ns takes, for j < n / r and k = j mod ns,
+        // the inputs a[j + m n / r] times w^(m k) (w = exp(-+2 pi i / (r ns))), their r-point DFT y_q, and writes
+        // b[r (j - k) + k + q ns]; ns grows by r. tw[i] = exp(-2 pi i i / n).
+        int ns = 1;
+        if (__builtin_ctz((unsigned)n) & 1) {
+            const int half = n >> 1;
             for (int j = threadIdx.x; j < half; j += blockDim.x) {
+                const double2 u = a[j], v = a[j + half];
+                b[2 * j] = make_double2(u.x + v.x, u.y + v.y);
+                b[2 * j + 1] = make_double2(u.x - v.x, u.y - v.y);
+            }
+            __syncthreads();
+            double2 *t = a; a = b; b = t;
+            ns = 2;
+        }
+        const int quarter = n >> 2;
+        for (; ns < n; ns <<= 2) {
+            const int step = quarter / ns;                      // tw index of w^1 = exp(-2 pi i / (4 ns))
+            for (int j = threadIdx.x; j < quarter; j += blockDim.x) {
                 const int k = j & (ns - 1);
-                double2 w = tw[k * (half / ns)];               // exp(-2 pi i k / (2 ns))
-                if (INV) w.y = -w.y;
-                const double2 u = a[j], v = cmul(a[j + half], w);
-                const int o = ((j - k) << 1) + k;
-                b[o] = make_double2(u.x + v.x, u.y + v.y);
-                b[o + ns] = make_double2(u.x - v.x, u.y - v.y);
+                double2 w1 = tw[k * step], w2 = tw[2 * k * step], w3 = tw[3 * k * step];
+                if (INV) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+                const double2 x0 = a[j], x1 = cmul(a[j + quarter], w1), x2 = cmul(a[j + 2 * quarter], w2),
+                              x3 = cmul(a[j + 3 * quarter], w3);
+                const double2 s02 = make_double2(x0.x + x2.x, x0.y + x2.y), d02 = make_double2(x0.x - x2.x, x0.y - x2.y);
+                const double2 s13 = make_double2(x1.x + x3.x, x1.y + x3.y), d13 = make_double2(x1.x - x3.x, x1.y - x3.y);
+                // -i d13 (forward) or +i d13 (inverse)
+                const double2 r13 = INV ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
+                const int o = ((j - k) << 2) + k;
+                b[o] = make_double2(s02.x + s13.x, s02.y + s13.y);
+                b[o + ns] = make_double2(d02.x + r13.x, d02.y + r13.y);
+                b[o + 2 * ns] = make_double2(s02.x - s13.x, s02.y - s13.y);
+                b[o + 3 * ns] = make_double2(d02.x - r13.x, d02.y - r13.y);
             }
             __syncthreads();
             double2 *t = a; a = b; b = t;
