@@ -36,6 +36,8 @@ struct cubo_hip_dpc {
     double *fx = nullptr, *fy = nullptr;          // scan region
     double *g_fx = nullptr, *g_fy = nullptr;
     std::vector<double *> chain;                  // stored iterates of a chain (backward)
+    double *chain_sums = nullptr;                 // a chain's squared-error sums, one per step
+    int chain_sums_cap = 0;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     double ms = 0;
     int64_t calls = 0;
@@ -52,6 +54,7 @@ void free_dpc(cubo_hip_dpc *d) {
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (double *b : d->chain) (void)hipFree(b);
+    if (d->chain_sums) (void)hipFree(d->chain_sums);
     if (d->t0) (void)hipEventDestroy(d->t0);
     if (d->t1) (void)hipEventDestroy(d->t1);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -90,10 +93,11 @@ int field(cubo_hip_dpc *d, const double *fx, const double *fy, double *denom) {
 }
 
 // dx, dy of phi into z and their mean square over the scan region into d->sum (device).
-hipError_t forward(cubo_hip_dpc *d, const double *phi) {
+// The differences of phi into z, and their squared-error sum into *sum (default: d->sum).
+hipError_t forward(cubo_hip_dpc *d, const double *phi, double *sum = nullptr) {
     int nb = 0;
     hipError_t e = cubo::launch_dpc_forward(grid(d), phi, d->fx, d->fy, d->z, d->partial, &nb, d->stream);
-    if (e == hipSuccess) e = cubo::launch_sum_partials(d->partial, nb, d->sum, d->stream);
+    if (e == hipSuccess) e = cubo::launch_sum_partials(d->partial, nb, sum ? sum : d->sum, d->stream);
     return e;
 }
 
@@ -307,17 +311,28 @@ int cubo_hip_dpc_chain_backward(cubo_hip_dpc *d, const double *fx, const double 
         CUBO_HIP_TRY(hipMalloc((void **)&b, n * 8), "allocating the stored iterates");
         d->chain.push_back(b);
     }
+    if (d->chain_sums_cap < nsteps) {
+        if (d->chain_sums) CUBO_HIP_TRY(hipFree(d->chain_sums), "freeing the error sums");
+        d->chain_sums = nullptr;
+        d->chain_sums_cap = 0;
+        CUBO_HIP_TRY(hipMalloc((void **)&d->chain_sums, (size_t)nsteps * 8), "allocating the error sums");
+        d->chain_sums_cap = nsteps;
+    }
     std::vector<double> zero(std::max(n, N), 0.0), msq(nsteps);
     CUBO_HIP_TRY(hipMemcpyAsync(d->chain[0], zero.data(), n * 8, hipMemcpyHostToDevice, d->stream), "clearing");
     CUBO_HIP_TRY(hipEventRecord(d->t0, d->stream), "hipEventRecord");
-    // forward: phi_{i+1} = phi_i + step * update(phi_i), keeping every iterate
+    // forward: phi_{i+1} = phi_i + step * update(phi_i), keeping every iterate and every step's squared-error
+    // sum on the device (the fixed steps need no decision on the host, so the steps run back to back)
     for (int i = 0; i < nsteps; ++i) {
-        CUBO_HIP_TRY(forward(d, d->chain[i]), "launching the DPC differences");
-        hipError_t e;
-        msq[i] = read_sum(d, &e) / (double)N;
-        if (e != hipSuccess) return hip_fail(e, "waiting for the DPC differences");
-        if (errors) errors[i] = msq[i] / denom;
+        CUBO_HIP_TRY(forward(d, d->chain[i], d->chain_sums + i), "launching the DPC differences");
         CUBO_HIP_TRY(update(d, d->chain[i], step, d->chain[i + 1]), "launching the DPC update");
+    }
+    CUBO_HIP_TRY(hipMemcpyAsync(msq.data(), d->chain_sums, (size_t)nsteps * 8, hipMemcpyDeviceToHost, d->stream),
+                 "reading the error sums");
+    CUBO_HIP_TRY(hipStreamSynchronize(d->stream), "waiting for the DPC steps");
+    for (int i = 0; i < nsteps; ++i) {
+        msq[i] /= (double)N;
+        if (errors) errors[i] = msq[i] / denom;
     }
     if (phase) {
         CUBO_HIP_TRY(cubo::launch_crop(d->chain[nsteps], d->R, d->C, d->Cp, d->g_fx, d->stream), "launching the crop");
