@@ -231,9 +231,9 @@ int cubo_hip_dpc_step_backward(cubo_hip_dpc *d, const double *phi, const double 
     const int64_t n = (int64_t)d->Rp * d->Cp, N = (int64_t)d->R * d->C;
     CUBO_HIP_TRY(hipMemcpyAsync(d->phi, phi, n * 8, hipMemcpyHostToDevice, d->stream), "uploading the phase");
     CUBO_HIP_TRY(hipMemcpyAsync(d->phi2, g_phi_out, n * 8, hipMemcpyHostToDevice, d->stream), "uploading dL/dphi");
-    std::vector<double> zero(N, 0.0);
-    CUBO_HIP_TRY(hipMemcpyAsync(d->g_fx, zero.data(), N * 8, hipMemcpyHostToDevice, d->stream), "clearing");
-    CUBO_HIP_TRY(hipMemcpyAsync(d->g_fy, zero.data(), N * 8, hipMemcpyHostToDevice, d->stream), "clearing");
+    // the buffers are cleared on the device (all-zero bytes are +0.0), with no host copy
+    CUBO_HIP_TRY(hipMemsetAsync(d->g_fx, 0, N * 8, d->stream), "clearing");
+    CUBO_HIP_TRY(hipMemsetAsync(d->g_fy, 0, N * 8, d->stream), "clearing");
     CUBO_HIP_TRY(forward(d, d->phi), "launching the DPC differences");
     hipError_t e;
     const double msq = read_sum(d, &e) / (double)N;
@@ -261,8 +261,7 @@ int cubo_hip_dpc_reconstruct(cubo_hip_dpc *d, const double *fx, const double *fy
     double denom = 0;
     if (int st = field(d, fx, fy, &denom)) return st;
     const int64_t n = (int64_t)d->Rp * d->Cp, N = (int64_t)d->R * d->C;
-    std::vector<double> zero(n, 0.0);
-    CUBO_HIP_TRY(hipMemcpyAsync(d->phi, zero.data(), n * 8, hipMemcpyHostToDevice, d->stream), "clearing the phase");
+    CUBO_HIP_TRY(hipMemsetAsync(d->phi, 0, n * 8, d->stream), "clearing the phase");
     double error = INFINITY;
     int ne = 0, ni = 0;
     double *cur = d->phi, *nxt = d->phi2;
@@ -318,8 +317,8 @@ int cubo_hip_dpc_chain_backward(cubo_hip_dpc *d, const double *fx, const double 
         CUBO_HIP_TRY(hipMalloc((void **)&d->chain_sums, (size_t)nsteps * 8), "allocating the error sums");
         d->chain_sums_cap = nsteps;
     }
-    std::vector<double> zero(std::max(n, N), 0.0), msq(nsteps);
-    CUBO_HIP_TRY(hipMemcpyAsync(d->chain[0], zero.data(), n * 8, hipMemcpyHostToDevice, d->stream), "clearing");
+    std::vector<double> msq(nsteps);
+    CUBO_HIP_TRY(hipMemsetAsync(d->chain[0], 0, n * 8, d->stream), "clearing");
     CUBO_HIP_TRY(hipEventRecord(d->t0, d->stream), "hipEventRecord");
     // forward: phi_{i+1} = phi_i + step * update(phi_i), keeping every iterate and every step's squared-error
     // sum on the device (the fixed steps need no decision on the host, so the steps run back to back)
@@ -340,8 +339,8 @@ int cubo_hip_dpc_chain_backward(cubo_hip_dpc *d, const double *fx, const double 
         CUBO_HIP_TRY(hipStreamSynchronize(d->stream), "waiting for DPC");
     }
     // backward: dL/dphase of the cropped phase into the padded grid, then each step's adjoint in reverse
-    CUBO_HIP_TRY(hipMemcpyAsync(d->g_fx, zero.data(), N * 8, hipMemcpyHostToDevice, d->stream), "clearing");
-    CUBO_HIP_TRY(hipMemcpyAsync(d->g_fy, zero.data(), N * 8, hipMemcpyHostToDevice, d->stream), "clearing");
+    CUBO_HIP_TRY(hipMemsetAsync(d->g_fx, 0, N * 8, d->stream), "clearing");
+    CUBO_HIP_TRY(hipMemsetAsync(d->g_fy, 0, N * 8, d->stream), "clearing");
     CUBO_HIP_TRY(hipMemcpyAsync(d->tmp, g_phase, N * 8, hipMemcpyHostToDevice, d->stream), "uploading dL/dphase");
     CUBO_HIP_TRY(cubo::launch_embed((const double *)d->tmp, d->R, d->C, d->Rp, d->Cp, d->phi, d->stream),
                  "launching the embed");
