@@ -143,34 +143,48 @@ int device_ok(int device) {
     return CUBO_HIP_OK;
 }
 
-// The plane of m (device) into out (device); CUBO_HIP_ERR_INVALID_ARGUMENT when degenerate.
-int plane(Dev &d, const double *m, int R, int C, double tol, double *out, double *partial, int nb) {
+// The planes of the two components m[0], m[1] (device) into out[0], out[1] (device);
+// CUBO_HIP_ERR_INVALID_ARGUMENT when degenerate. Each of the three reduction passes runs for both components
+// before the host reads their block sums (in block order) once. partial holds 2 x nb x 4 doubles.
+int planes(Dev &d, const double *const m[2], int R, int C, double tol, double *const out[2], double *partial, int nb) {
     const int64_t n = (int64_t)R * C;
-    std::vector<double> h((size_t)nb * 4);
-    auto sums = [&](int mode, double rm, double cm, double a, double (&tot)[4]) -> hipError_t {
-        hipLaunchKernelGGL(plane_sums, dim3(nb), dim3(kThreads), 0, d.s, m, n, C, mode, rm, cm, a, partial);
+    std::vector<double> h((size_t)2 * nb * 4);
+    auto sums = [&](int mode, const double (&rm)[2], const double (&cm)[2], const double (&a)[2],
+                    double (&tot)[2][4]) -> hipError_t {
+        for (int f = 0; f < 2; ++f)
+            hipLaunchKernelGGL(plane_sums, dim3(nb), dim3(kThreads), 0, d.s, m[f], n, C, mode, rm[f], cm[f], a[f],
+                               partial + (size_t)f * nb * 4);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(h.data(), partial, h.size() * 8, hipMemcpyDeviceToHost, d.s);
         if (e == hipSuccess) e = hipStreamSynchronize(d.s);
-        for (int i = 0; i < 4; ++i) {
-            tot[i] = 0;
-            for (int b = 0; b < nb; ++b) tot[i] += h[(size_t)b * 4 + i];
-        }
+        for (int f = 0; f < 2; ++f)
+            for (int i = 0; i < 4; ++i) {
+                tot[f][i] = 0;
+                for (int b = 0; b < nb; ++b) tot[f][i] += h[((size_t)f * nb + b) * 4 + i];
+            }
         return e;
     };
-    double t0[4], t1[4], t2[4];
-    CUBO_HIP_TRY(sums(0, 0, 0, 0, t0), "plane sums");
-    if (t0[0] < 3) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the plane fit needs 3 finite positions");
-    const double cnt = t0[0], rm = t0[1] / cnt, cm = t0[2] / cnt;
-    CUBO_HIP_TRY(sums(1, rm, cm, 0, t1), "plane sums");
-    const double suu = t1[0], suv = t1[1], svv = t1[2], a = t1[3] / cnt;
+    const double zero[2] = {0, 0};
+    double t0[2][4], t1[2][4], t2[2][4], rm[2], cm[2], a[2], cnt[2];
+    CUBO_HIP_TRY(sums(0, zero, zero, zero, t0), "plane sums");
+    for (int f = 0; f < 2; ++f) {
+        if (t0[f][0] < 3) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the plane fit needs 3 finite positions");
+        cnt[f] = t0[f][0];
+        rm[f] = t0[f][1] / cnt[f];
+        cm[f] = t0[f][2] / cnt[f];
+    }
+    CUBO_HIP_TRY(sums(1, rm, cm, zero, t1), "plane sums");
+    for (int f = 0; f < 2; ++f) a[f] = t1[f][3] / cnt[f];
     CUBO_HIP_TRY(sums(2, rm, cm, a, t2), "plane sums");
-    const double det = suu * svv - suv * suv;
-    if (!(det > tol * std::fmax(suu * svv, 1.0)))
-        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the plane fit is degenerate (the positions are collinear)");
-    const double b = (svv * t2[0] - suv * t2[1]) / det, c = (suu * t2[1] - suv * t2[0]) / det;
-    hipLaunchKernelGGL(plane_eval, dim3(blocks(n)), dim3(kThreads), 0, d.s, out, n, C, a, b, c, rm, cm);
-    CUBO_HIP_TRY(hipGetLastError(), "plane evaluation");
+    for (int f = 0; f < 2; ++f) {
+        const double suu = t1[f][0], suv = t1[f][1], svv = t1[f][2];
+        const double det = suu * svv - suv * suv;
+        if (!(det > tol * std::fmax(suu * svv, 1.0)))
+            return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the plane fit is degenerate (the positions are collinear)");
+        const double b = (svv * t2[f][0] - suv * t2[f][1]) / det, c = (suu * t2[f][1] - suv * t2[f][0]) / det;
+        hipLaunchKernelGGL(plane_eval, dim3(blocks(n)), dim3(kThreads), 0, d.s, out[f], n, C, a[f], b, c, rm[f], cm[f]);
+        CUBO_HIP_TRY(hipGetLastError(), "plane evaluation");
+    }
     return CUBO_HIP_OK;
 }
 
@@ -192,12 +206,13 @@ int cubo_hip_phase_normalize(int device, const double *mx, const double *my, int
     const int nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + kThreads - 1) / kThreads));
     double *dmx, *dmy, *dfx, *dfy, *dnx, *dny, *partial;
     for (double **p : {&dmx, &dmy, &dfx, &dfy, &dnx, &dny}) CUBO_HIP_TRY(d.alloc(p, n), "allocating the field");
-    CUBO_HIP_TRY(d.alloc(&partial, (size_t)nb * 4), "allocating partial sums");
+    CUBO_HIP_TRY(d.alloc(&partial, (size_t)2 * nb * 4), "allocating partial sums");
     CUBO_HIP_TRY(hipMemcpyAsync(dmx, mx, n * 8, hipMemcpyHostToDevice, d.s), "uploading the field");
     CUBO_HIP_TRY(hipMemcpyAsync(dmy, my, n * 8, hipMemcpyHostToDevice, d.s), "uploading the field");
     if (reference == 1) {
-        if (int st = plane(d, dmx, rows, cols, tol, dfx, partial, nb)) return st;
-        if (int st = plane(d, dmy, rows, cols, tol, dfy, partial, nb)) return st;
+        const double *const m[2] = {dmx, dmy};
+        double *const out[2] = {dfx, dfy};
+        if (int st = planes(d, m, rows, cols, tol, out, partial, nb)) return st;
     } else {
         double a = 0, b = 0;
         if (reference == 2) {                        // NumPy means of the raw components (no finite filter)
