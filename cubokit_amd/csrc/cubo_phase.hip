@@ -11,9 +11,13 @@
 // are added on the host in block order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "../../include/cubo_hip.h"
@@ -118,21 +122,56 @@ __global__ void rotate_apply(const double *nx, const double *ny, int64_t n, doub
 
 unsigned blocks(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
+// The stream and device buffers of the phase calls on one device, kept between calls: a buffer only grows, so
+// a warm call allocates and frees nothing (hipFree can wait for the whole device). Calls on one device take
+// turns; the workspace lives until the process ends.
+struct PhaseWorkspace {
+    std::mutex mu;
+    hipStream_t s = nullptr;
+    std::vector<std::pair<void *, size_t>> slots;    // the k-th buffer a call asks for, and its capacity in bytes
+};
+
+PhaseWorkspace &workspace(int device) {
+    static std::mutex map_mu;
+    static std::map<int, PhaseWorkspace *> spaces;
+    std::lock_guard<std::mutex> l(map_mu);
+    PhaseWorkspace *&w = spaces[device];
+    if (!w) w = new PhaseWorkspace();
+    return *w;
+}
+
+// One call's hold on its device's workspace (construct with the device current).
 struct Dev {
     int device;
+    PhaseWorkspace &w;
+    std::unique_lock<std::mutex> lock;
     hipStream_t s = nullptr;
-    std::vector<void *> bufs;
+    size_t next = 0;
+    explicit Dev(int dev) : device(dev), w(workspace(dev)), lock(w.mu) {}
     ~Dev() {
         DeviceGuard g(device);
-        if (s) (void)hipStreamSynchronize(s);
-        for (void *b : bufs) (void)hipFree(b);
-        if (s) (void)hipStreamDestroy(s);
+        if (s) (void)hipStreamSynchronize(s);        // the next call may reuse or regrow the buffers
+    }
+    hipError_t init() {
+        if (!w.s) {
+            if (hipError_t e = hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking)) return e;
+        }
+        s = w.s;
+        return hipSuccess;
     }
     template <typename T>
     hipError_t alloc(T **p, size_t n) {
-        hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
-        if (e == hipSuccess) bufs.push_back(*p);
-        return e;
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        if (next == w.slots.size()) w.slots.push_back({nullptr, 0});
+        std::pair<void *, size_t> &sl = w.slots[next++];
+        if (sl.second < bytes) {
+            if (sl.first) (void)hipFree(sl.first);
+            sl = {nullptr, 0};
+            if (hipError_t e = hipMalloc(&sl.first, bytes)) return e;
+            sl.second = bytes;
+        }
+        *p = (T *)sl.first;
+        return hipSuccess;
     }
 };
 
@@ -200,8 +239,8 @@ int cubo_hip_phase_normalize(int device, const double *mx, const double *my, int
     if (reference < 0 || reference > 2) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "reference %d (0, 1, 2)", reference);
     if (int st = device_ok(device)) return st;
     DeviceGuard g(device);
-    Dev d{device};
-    CUBO_HIP_TRY(hipStreamCreateWithFlags(&d.s, hipStreamNonBlocking), "hipStreamCreate");
+    Dev d(device);
+    CUBO_HIP_TRY(d.init(), "hipStreamCreate");
     const int64_t n = (int64_t)rows * cols;
     const int nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + kThreads - 1) / kThreads));
     double *dmx, *dmy, *dfx, *dfy, *dnx, *dny, *partial;
@@ -265,8 +304,8 @@ int cubo_hip_phase_rotation(int device, const double *nx, const double *ny, int3
             return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the field must be finite (entry %lld)", (long long)k);
     if (int st = device_ok(device)) return st;
     DeviceGuard g(device);
-    Dev d{device};
-    CUBO_HIP_TRY(hipStreamCreateWithFlags(&d.s, hipStreamNonBlocking), "hipStreamCreate");
+    Dev d(device);
+    CUBO_HIP_TRY(d.init(), "hipStreamCreate");
     std::vector<double> co(nangles), si(nangles);
     for (int a = 0; a < nangles; ++a) {
         const double rad = angles_deg[a] * (M_PI / 180.0);
