@@ -341,6 +341,8 @@ int cubo_hip_shift_kernel_time(cubo_hip_shift_plan *plan, double *ms3, int64_t *
 int cubo_hip_shift_plan_destroy(cubo_hip_shift_plan *plan);
 
 /* ---- the centre-of-mass field's reference and rotation (CuboPhase), on the device in Double ----
+ * Both calls are thread-safe: they share one stream and one set of device buffers per device (kept between
+ * calls), so concurrent calls on one device run one after another.
  * CenterOfMassReference.normalize (Sources/CuboPhase/CenterOfMassReference.swift:27-84): reference 0 = none,
  * 1 = least-squares plane over the finite positions (refused when degenerate: det <= tol * max(suu svv, 1)),
  * 2 = the constant mean; fx/fy get the reference, nx/ny = (measured - reference) with NaN -> 0 and
