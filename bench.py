@@ -55,12 +55,34 @@ def mfma_utilisation(kernel_ms: float):
                       "HIP events in this run"}
 
 
+def host_cpu():
+    """(threads the baseline uses, nproc, CPU model). The threads are every core this process may run on, capped by
+    OMP_NUM_THREADS where the host sets it (the GPU machines give one GPU's process a 16-core share)."""
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = nproc
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(usable, int(cap)) if cap.isdigit() and int(cap) > 0 else usable
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, nproc, usable, model
+
+
 def cpu_baseline(cube, bf, adf, seconds: float):
     """The oracle's restatement of CuboKit's .cpu pass, on a bounded sample of the same cube."""
     from oracle import oracle as O
     rows = 16
     sample = cube.read_scan_tile(range(0, rows))
-    threads = min(16, os.cpu_count() or 1)
+    threads, nproc, usable, model = host_cpu()
     nbytes = sample.nbytes
     passes, t0 = 0, time.perf_counter()
     while True:
@@ -70,18 +92,44 @@ def cpu_baseline(cube, bf, adf, seconds: float):
         if el >= seconds or passes >= 500:
             break
     return {"value": round(nbytes * passes / el / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "affinity_cores": usable, "cpu_model": model,
             "sample": f"{rows} of 256 scan rows ({nbytes / 2**20:.0f} MiB of uint16), {passes} passes of the oracle's "
                       f".cpu restatement (VI serial, mean+max and CoM on {threads} threads), {el:.1f} s"}
 
 
+def stack_error_vs_fp64(cube, masks, ys, rows=(0, 97, 255)):
+    """C3's error of every output against the FP64 oracle on sampled scan rows: per mask column the largest
+    |y - y64| / y64 (the masks are non-negative), for each method; and the centre of mass from {1, q_row, q_col}
+    (the FP64 division of the stack's columns 1, 2 by 0) against the oracle's, in pixels."""
+    from oracle import oracle as O
+    s = cube.shape
+    host = np.concatenate([cube.read_scan_tile(range(r, r + 1)) for r in rows])
+    ref = O.mask_stack(host, masks.astype(np.float64))
+    pos = np.concatenate([np.arange(r * s.scan_columns, (r + 1) * s.scan_columns) for r in rows])
+    out = {"sample": f"scan rows {list(rows)} ({pos.size} positions), every one of the {masks.shape[0]} outputs"}
+    tiny = np.finfo(np.float64).tiny
+    for name, y in ys.items():
+        err = np.max(np.abs(y[pos] - ref) / np.maximum(np.abs(ref), tiny), axis=0)
+        out[name] = {"max": float(err.max()), "per_output": [float(f"{e:.3g}") for e in err]}
+        cr, cc = y[pos, 1] / y[pos, 0], y[pos, 2] / y[pos, 0]
+        out[name]["com_px_vs_fp64"] = float(max(np.max(np.abs(cr - ref[:, 1] / ref[:, 0])),
+                                                np.max(np.abs(cc - ref[:, 2] / ref[:, 0]))))
+    return out
+
+
 def batch_section(C):
-    """48 seeded uneven uint16 cubes with 8..64 masks each: one tuned graph replay of the persistent batch
-    kernel against a loop of single-cube split-FP16 stacks and a loop of plain stacks (each with its results
-    copied to host memory), and the replay with the results read in place in its pinned buffer (no copy).
-    GB/s are cube bytes per second of wall time."""
+    """48 seeded uneven uint16 cubes (47.4 GiB, up to 256x256x256x256) with 8..64 masks each: one tuned graph replay
+    of the persistent batch kernel against a loop of single-cube split-FP16 stacks and a loop of plain stacks (each
+    with its results copied to host memory), and the replay with the results read in place in its pinned buffer (no
+    copy). GB/s are cube bytes per second of wall time. Per-cube latency: the device time of each cube's own launch
+    in the loops (HIP events on the cube's stream) against the batch's device time per cube."""
     from cubokit_amd import synthetic
+    specs = synthetic.batch_items()
+    why = C.StackBatch.availability([s for s, _, _ in specs])
+    if why is not None:
+        raise RuntimeError(f"the 48-cube batch does not fit the device: {why.message}")
     items = []
-    for shape, p, k in synthetic.batch_items():
+    for shape, p, k in specs:
         c = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=np.uint16)
         c.fill_synthetic(p)
         items.append((c, np.ascontiguousarray(synthetic.stack_masks(shape[2], shape[3])[:k])))
@@ -103,27 +151,40 @@ def batch_section(C):
     views_match = all(np.array_equal(v, y) for v, y in zip(views, ys))
     gms, gn = b.kernel_time()
     res = {"cubes": len(items), "cube_bytes": total, "masks": int(sum(m.shape[0] for _, m in items)),
+           "largest_cube": list(max((c.shape.extents for c, _ in items), key=lambda e: int(np.prod(e)))),
            "tuned": shape, "graph_ms": round(graph_s * 1e3, 3), "graph_device_ms": round(gms / gn, 3),
            "graph_gbs": round(total / graph_s / 1e9, 1), "graph_view_ms": round(view_s * 1e3, 3),
-           "graph_view_gbs": round(total / view_s / 1e9, 1), "view_matches_copy": views_match}
+           "graph_view_gbs": round(total / view_s / 1e9, 1), "view_matches_copy": views_match,
+           "per_cube_device_ms_batched": round(gms / gn / len(items), 4)}
     for name, method, r in (("mfma_loop", C.MaskStack.MFMA_SPLIT, 5), ("plain_loop", C.MaskStack.PLAIN, 2)):
         plans = [C.MaskStack(c, m, method) for c, m in items]
         outs = [p.run() for p in plans]
+        for c, _ in items:
+            c.kernel_time()                             # drop the warm-up launches
         t0 = time.perf_counter()
         for _ in range(r):
             for p, y in zip(plans, outs):
                 p.run(y)
         el = (time.perf_counter() - t0) / r
+        per_cube = []
+        for c, _ in items:
+            ms, n = c.kernel_time()
+            per_cube.append(ms / r)                     # every launch of the cube's passes, per run
         for p in plans:
             p.release()
         res[name + "_ms"] = round(el * 1e3, 3)
         res[name + "_gbs"] = round(total / el / 1e9, 1)
+        res[name + "_per_cube_device_ms"] = {"mean": round(float(np.mean(per_cube)), 4),
+                                             "median": round(float(np.median(per_cube)), 4),
+                                             "max": round(float(np.max(per_cube)), 4),
+                                             "sum": round(float(np.sum(per_cube)), 3)}
         if method == C.MaskStack.MFMA_SPLIT:
             res["identical_to_single_stacks"] = all(np.array_equal(x, y) for x, y in zip(ys, outs))
     res["speedup_vs_mfma_loop"] = round(res["mfma_loop_ms"] / res["graph_ms"], 2)
     res["speedup_vs_plain_loop"] = round(res["plain_loop_ms"] / res["graph_ms"], 2)
     log(f"batch: graph {res['graph_ms']} ms ({res['graph_gbs']} GB/s), single-stack loop {res['mfma_loop_ms']} ms, "
-        f"plain loop {res['plain_loop_ms']} ms; tuned {shape}")
+        f"plain loop {res['plain_loop_ms']} ms; per cube {res['per_cube_device_ms_batched']} ms batched vs "
+        f"{res['plain_loop_per_cube_device_ms']['mean']} ms plain; tuned {shape}")
     b.release()
     for c, _ in items:
         c.release()
@@ -405,6 +466,9 @@ def main() -> int:
         stack["speedup"] = round(stack["plain_ms"] / stack["mfma_split_fp16_ms"], 2)
         den = np.maximum(np.abs(ys["plain"]), 1e-6 * np.abs(ys["plain"]).max(axis=0))
         stack["max_rel_diff_vs_plain"] = float(np.max(np.abs(ys["mfma_split_fp16"] - ys["plain"]) / den))
+        stack["error_vs_fp64"] = stack_error_vs_fp64(cube, stack_masks, ys)
+        log(f"mask stack error vs FP64: plain {stack['error_vs_fp64']['plain']['max']:.2e}, split-FP16 "
+            f"{stack['error_vs_fp64']['mfma_split_fp16']['max']:.2e}")
         stack["k"] = int(stack_masks.shape[0])
         stack["mfma_gbs"] = round(cube_bytes / (stack["mfma_split_fp16_ms"] / 1e3) / 1e9, 1)
         stack["mfma_utilisation"] = mfma_utilisation(stack["mfma_split_fp16_ms"])

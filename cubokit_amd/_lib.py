@@ -21,6 +21,7 @@ ERR_UNSUPPORTED = -6
 ERR_INCOMPLETE = -7
 ERR_EXCEEDS_BUFFER_LIMIT = -8
 ERR_FILE = -9
+ERR_CANCELLED = -10
 
 U16 = 1
 F32 = 2
@@ -172,10 +173,12 @@ SIGNATURES = {
     "cubo_hip_shift_plan_destroy": (ct.c_int, [ct.c_void_p]),
     "cubo_hip_phase_normalize": (ct.c_int, [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_int32,
                                             ct.c_double, ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p,
-                                            ct.c_void_p]),
+                                            ct.c_void_p, ct.c_void_p, ct.c_void_p]),
     "cubo_hip_phase_rotation": (ct.c_int, [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_void_p,
-                                           ct.c_int32, ct.c_int32, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int32),
-                                           ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+                                           ct.c_int32, ct.c_int32, ct.c_void_p, ct.c_int32, ct.c_void_p,
+                                           ct.POINTER(ct.c_double), ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32),
+                                           ct.POINTER(ct.c_int32), ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                           ct.c_void_p]),
     "cubo_hip_dpc_create": (ct.c_int, [ct.c_int, ct.c_int32, ct.c_int32, ct.c_double, ct.c_double, ct.c_double,
                                        ct.POINTER(ct.c_void_p)]),
     "cubo_hip_dpc_padded_shape": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32)]),

@@ -274,9 +274,12 @@ int cubo_hip_batch_tune(cubo_hip_batch *b, int32_t *waves, int32_t *blocks_per_c
     for (int w : {8, 4})
         for (int bpc = 1; bpc <= cubo::batch_blocks_per_cu(w); ++bpc)
             for (int o : {0, 1, 2}) cand.push_back(Shape{w, bpc, o});
-    hipEvent_t t0, t1;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
     CUBO_HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
-    CUBO_HIP_TRY(hipEventCreate(&t1), "hipEventCreate");
+    if (hipError_t e = hipEventCreate(&t1); e != hipSuccess) {
+        (void)hipEventDestroy(t0);
+        return hip_fail(e, "hipEventCreate");
+    }
     float best = 1e30f;
     Shape best_s = b->shape;
     int status = CUBO_HIP_OK;
@@ -298,7 +301,13 @@ int cubo_hip_batch_tune(cubo_hip_batch *b, int32_t *waves, int32_t *blocks_per_c
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    if (status) return status;
+    if (status) {
+        // the candidates overwrote the device item list the graph was captured with: put the plan of the kept
+        // shape back (a failure here leaves no graph, and cubo_hip_batch_run refuses to run)
+        (void)hipStreamSynchronize(b->stream);
+        if (build_graph(b) != CUBO_HIP_OK) drop_graph(b);
+        return status;
+    }
     b->shape = best_s;
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
@@ -313,6 +322,7 @@ int cubo_hip_batch_tune(cubo_hip_batch *b, int32_t *waves, int32_t *blocks_per_c
 
 int cubo_hip_batch_run(cubo_hip_batch *b, double *const *y) {
     if (!b) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL batch");
+    if (!b->exec) return fail(CUBO_HIP_ERR_INCOMPLETE, "the batch has no graph (a failed tune could not restore it)");
     DeviceGuard g(b->device);
     CUBO_HIP_TRY(b->timer.start(b->stream), "hipEventRecord");
     CUBO_HIP_TRY(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");

@@ -87,12 +87,13 @@ __global__ void normalize_apply(const double *mx, const double *my, const double
     ny[k] = clean(my[k] - fy[k]) * sampling;
 }
 
-// blockIdx.x = split, blockIdx.y = angle, blockIdx.z = orientation (1: components swapped)
+// blockIdx.x = split, blockIdx.y = angle, blockIdx.z = the z-th orientation scored, t_first + z (1: components
+// swapped first)
 __global__ __launch_bounds__(kThreads) void rotation_sums(const double *nx, const double *ny, int R, int C,
                                                           const double *cosv, const double *sinv, int maximize,
-                                                          double *partial) {
+                                                          int t_first, double *partial) {
     __shared__ double red[kThreads];
-    const int angle = blockIdx.y, t = blockIdx.z;
+    const int angle = blockIdx.y, t = t_first + (int)blockIdx.z;
     const double co = cosv[angle], si = sinv[angle];
     const double *A = t ? ny : nx, *B = t ? nx : ny;
     auto xr = [&](int64_t k) { return co * A[k] - si * B[k]; };   // rotated components (RotationSolver.swift:56-62)
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void rotation_sums(const double *nx, cons
         s += v;
     }
     const double tot = block_sum(s, red);
-    if (threadIdx.x == 0) partial[((int64_t)t * gridDim.y + angle) * kSplit + blockIdx.x] = tot;
+    if (threadIdx.x == 0) partial[((int64_t)blockIdx.z * gridDim.y + angle) * kSplit + blockIdx.x] = tot;
 }
 
 __global__ void rotate_apply(const double *nx, const double *ny, int64_t n, double co, double si, int t, double *cx,
@@ -232,11 +233,16 @@ int planes(Dev &d, const double *const m[2], int R, int C, double tol, double *c
 extern "C" {
 
 int cubo_hip_phase_normalize(int device, const double *mx, const double *my, int32_t rows, int32_t cols,
-                             int32_t reference, double sampling, double tol, double *fx, double *fy, double *nx,
-                             double *ny) {
+                             int32_t reference, double sampling, double tol, const double *given_row,
+                             const double *given_col, double *fx, double *fy, double *nx, double *ny) {
     if (!mx || !my || !fx || !fy || !nx || !ny) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     if (rows < 1 || cols < 1) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "empty scan");
-    if (reference < 0 || reference > 2) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "reference %d (0, 1, 2)", reference);
+    if (reference < CUBO_HIP_REFERENCE_NONE || reference > CUBO_HIP_REFERENCE_GIVEN)
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "reference %d (0 none, 1 plane, 2 constant, 3 given)", reference);
+    if (reference == CUBO_HIP_REFERENCE_GIVEN && (!given_row || !given_col))
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the given reference needs both components");
+    if (!(tol >= 0) || !std::isfinite(tol))                  // CenterOfMassReference.swift:30-32
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the degeneracy tolerance must be finite and not negative (%g)", tol);
     if (int st = device_ok(device)) return st;
     DeviceGuard g(device);
     Dev d(device);
@@ -248,13 +254,16 @@ int cubo_hip_phase_normalize(int device, const double *mx, const double *my, int
     CUBO_HIP_TRY(d.alloc(&partial, (size_t)2 * nb * 4), "allocating partial sums");
     CUBO_HIP_TRY(hipMemcpyAsync(dmx, mx, n * 8, hipMemcpyHostToDevice, d.s), "uploading the field");
     CUBO_HIP_TRY(hipMemcpyAsync(dmy, my, n * 8, hipMemcpyHostToDevice, d.s), "uploading the field");
-    if (reference == 1) {
+    if (reference == CUBO_HIP_REFERENCE_PLANE) {
         const double *const m[2] = {dmx, dmy};
         double *const out[2] = {dfx, dfy};
         if (int st = planes(d, m, rows, cols, tol, out, partial, nb)) return st;
+    } else if (reference == CUBO_HIP_REFERENCE_GIVEN) {    // .given(field): the field is the reference (:45-49)
+        CUBO_HIP_TRY(hipMemcpyAsync(dfx, given_row, n * 8, hipMemcpyHostToDevice, d.s), "uploading the reference");
+        CUBO_HIP_TRY(hipMemcpyAsync(dfy, given_col, n * 8, hipMemcpyHostToDevice, d.s), "uploading the reference");
     } else {
         double a = 0, b = 0;
-        if (reference == 2) {                        // NumPy means of the raw components (no finite filter)
+        if (reference == CUBO_HIP_REFERENCE_CONSTANT) {      // NumPy means of the raw components (no finite filter)
             std::vector<double> t(mx, mx + n), u(my, my + n);
             auto pw = [](const double *v, int64_t m, auto &&self) -> double {
                 if (m < 8) { double s = 0; for (int64_t i = 0; i < m; ++i) s += v[i]; return s; }
@@ -292,57 +301,105 @@ int cubo_hip_phase_normalize(int device, const double *mx, const double *my, int
 }
 
 int cubo_hip_phase_rotation(int device, const double *nx, const double *ny, int32_t rows, int32_t cols,
-                            const double *angles_deg, int32_t nangles, int32_t maximize, double *degrees,
-                            int32_t *transposed, double *curve, double *curve_t, double *cx, double *cy) {
-    if (!nx || !ny || !angles_deg || !degrees || !transposed || !curve || !curve_t || !cx || !cy)
+                            const double *angles_deg, int32_t nangles, int32_t maximize, const double *force_degrees,
+                            int32_t force_transpose, const volatile int32_t *cancel, double *degrees,
+                            int32_t *transposed, int32_t *n_curve, int32_t *n_curve_t, double *curve, double *curve_t,
+                            double *cx, double *cy) {
+    if (!nx || !ny || !degrees || !transposed || !n_curve || !n_curve_t || !cx || !cy)
         return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (force_transpose < -1 || force_transpose > 1)
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "force_transpose %d (-1, 0, 1)", force_transpose);
+    // requireUsable (RotationSolver.swift:128-135) comes first on every branch
     if (rows < 3 || cols < 3) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the scan needs at least 3 x 3 positions");
-    if (nangles < 1 || nangles > 65535) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "%d angles", nangles);
     const int64_t n = (int64_t)rows * cols, ni = (int64_t)(rows - 2) * (cols - 2);
     for (int64_t k = 0; k < n; ++k)
         if (!std::isfinite(nx[k]) || !std::isfinite(ny[k]))
             return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the field must be finite (entry %lld)", (long long)k);
+    if (cancel && *cancel) return fail(CUBO_HIP_ERR_CANCELLED, "cancelled");
+    // what is scored (RotationSolver.swift:66-90): a forced angle with both orientations, a forced orientation over
+    // every angle, both orientations over every angle, or nothing when angle and orientation are both forced
+    const double *score_angles = angles_deg;
+    int na = nangles, t_first = 0, nt = 2;
+    if (force_degrees) {
+        if (!std::isfinite(*force_degrees)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "invalidAngles: the forced angle");
+        score_angles = force_degrees;
+        na = force_transpose >= 0 ? 0 : 1;
+    } else {
+        if (!angles_deg || nangles < 1 || nangles > 65535)
+            return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "invalidAngles: %d angles", nangles);
+        for (int a = 0; a < nangles; ++a)
+            if (!std::isfinite(angles_deg[a]))
+                return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "invalidAngles: angle %d is not finite", a);
+        if (force_transpose >= 0) { t_first = force_transpose; nt = 1; }
+    }
+    if (na > 0 && ((t_first == 0 && !curve) || ((t_first == 1 || nt == 2) && !curve_t)))
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL curve");
     if (int st = device_ok(device)) return st;
     DeviceGuard g(device);
     Dev d(device);
     CUBO_HIP_TRY(d.init(), "hipStreamCreate");
-    std::vector<double> co(nangles), si(nangles);
-    for (int a = 0; a < nangles; ++a) {
-        const double rad = angles_deg[a] * (M_PI / 180.0);
-        co[a] = std::cos(rad);
-        si[a] = std::sin(rad);
-    }
-    double *dnx, *dny, *dco, *dsi, *partial, *dcx, *dcy;
+    double *dnx, *dny, *dcx, *dcy;
     for (double **p : {&dnx, &dny, &dcx, &dcy}) CUBO_HIP_TRY(d.alloc(p, n), "allocating the field");
-    CUBO_HIP_TRY(d.alloc(&dco, nangles), "allocating angles");
-    CUBO_HIP_TRY(d.alloc(&dsi, nangles), "allocating angles");
-    const size_t np = (size_t)2 * nangles * kSplit;
-    CUBO_HIP_TRY(d.alloc(&partial, np), "allocating partial sums");
     CUBO_HIP_TRY(hipMemcpyAsync(dnx, nx, n * 8, hipMemcpyHostToDevice, d.s), "uploading the field");
     CUBO_HIP_TRY(hipMemcpyAsync(dny, ny, n * 8, hipMemcpyHostToDevice, d.s), "uploading the field");
-    CUBO_HIP_TRY(hipMemcpyAsync(dco, co.data(), nangles * 8, hipMemcpyHostToDevice, d.s), "uploading angles");
-    CUBO_HIP_TRY(hipMemcpyAsync(dsi, si.data(), nangles * 8, hipMemcpyHostToDevice, d.s), "uploading angles");
-    hipLaunchKernelGGL(rotation_sums, dim3(kSplit, nangles, 2), dim3(kThreads), 0, d.s, dnx, dny, rows, cols, dco, dsi,
-                       (int)(maximize != 0), partial);
-    CUBO_HIP_TRY(hipGetLastError(), "launching the rotation curves");
-    std::vector<double> h(np);
-    CUBO_HIP_TRY(hipMemcpyAsync(h.data(), partial, np * 8, hipMemcpyDeviceToHost, d.s), "reading the curves");
-    CUBO_HIP_TRY(hipStreamSynchronize(d.s), "waiting for the rotation curves");
-    for (int t = 0; t < 2; ++t)
-        for (int a = 0; a < nangles; ++a) {
-            double s = 0;
-            for (int b = 0; b < kSplit; ++b) s += h[((size_t)t * nangles + a) * kSplit + b];
-            (t ? curve_t : curve)[a] = s / (double)ni;
+    int t = force_transpose;
+    double deg = force_degrees ? *force_degrees : 0.0;
+    *n_curve = 0;
+    *n_curve_t = 0;
+    if (na > 0) {
+        std::vector<double> co(na), si(na);
+        for (int a = 0; a < na; ++a) {
+            const double rad = score_angles[a] * (M_PI / 180.0);   // deg * (pi / 180), RotationSolver.swift:93
+            co[a] = std::cos(rad);
+            si[a] = std::sin(rad);
         }
-    int i = 0, j = 0;
-    for (int k = 1; k < nangles; ++k) {
-        if (maximize ? curve[k] > curve[i] : curve[k] < curve[i]) i = k;
-        if (maximize ? curve_t[k] > curve_t[j] : curve_t[k] < curve_t[j]) j = k;
+        double *dco, *dsi, *partial;
+        CUBO_HIP_TRY(d.alloc(&dco, na), "allocating angles");
+        CUBO_HIP_TRY(d.alloc(&dsi, na), "allocating angles");
+        const size_t np = (size_t)nt * na * kSplit;
+        CUBO_HIP_TRY(d.alloc(&partial, np), "allocating partial sums");
+        CUBO_HIP_TRY(hipMemcpyAsync(dco, co.data(), na * 8, hipMemcpyHostToDevice, d.s), "uploading angles");
+        CUBO_HIP_TRY(hipMemcpyAsync(dsi, si.data(), na * 8, hipMemcpyHostToDevice, d.s), "uploading angles");
+        hipLaunchKernelGGL(rotation_sums, dim3(kSplit, na, nt), dim3(kThreads), 0, d.s, dnx, dny, rows, cols, dco, dsi,
+                           (int)(maximize != 0), t_first, partial);
+        CUBO_HIP_TRY(hipGetLastError(), "launching the rotation curves");
+        std::vector<double> h(np), sc((size_t)nt * na);
+        CUBO_HIP_TRY(hipMemcpyAsync(h.data(), partial, np * 8, hipMemcpyDeviceToHost, d.s), "reading the curves");
+        CUBO_HIP_TRY(hipStreamSynchronize(d.s), "waiting for the rotation curves");
+        // parallelScores asks once more after the scores: a cancelled solve returns no curve (ParallelScores.swift:37)
+        if (cancel && *cancel) return fail(CUBO_HIP_ERR_CANCELLED, "cancelled");
+        for (int z = 0; z < nt; ++z)
+            for (int a = 0; a < na; ++a) {
+                double s = 0;
+                for (int b = 0; b < kSplit; ++b) s += h[((size_t)z * na + a) * kSplit + b];
+                sc[(size_t)z * na + a] = s / (double)ni;
+            }
+        auto better = [&](double a, double b) { return maximize ? a > b : a < b; };
+        auto best = [&](const double *c) {
+            int k = 0;
+            for (int i = 1; i < na; ++i)
+                if (better(c[i], c[k])) k = i;
+            return k;
+        };
+        for (int z = 0; z < nt; ++z) {
+            double *dst = (t_first + z) ? curve_t : curve;
+            std::copy(sc.begin() + (size_t)z * na, sc.begin() + (size_t)(z + 1) * na, dst);
+            *((t_first + z) ? n_curve_t : n_curve) = na;
+        }
+        if (force_degrees) {
+            t = better(curve_t[0], curve[0]);                // the plain orientation wins ties
+        } else if (nt == 1) {
+            t = t_first;
+            deg = angles_deg[best(t ? curve_t : curve)];
+        } else {
+            const int i = best(curve), j = best(curve_t);
+            t = better(curve_t[j], curve[i]);
+            deg = angles_deg[t ? j : i];
+        }
     }
-    const int t = maximize ? curve_t[j] > curve[i] : curve_t[j] < curve[i];   // the plain orientation wins ties
     *transposed = t;
-    *degrees = angles_deg[t ? j : i];
-    const double rad = *degrees * (M_PI / 180.0);
+    *degrees = deg;
+    const double rad = deg * (M_PI / 180.0);
     hipLaunchKernelGGL(rotate_apply, dim3(blocks(n)), dim3(kThreads), 0, d.s, dnx, dny, n, std::cos(rad), std::sin(rad),
                        t, dcx, dcy);
     CUBO_HIP_TRY(hipGetLastError(), "rotating the field");

@@ -6,43 +6,101 @@ iteration and of a fixed-step chain of iterations, which the reference does not 
 from __future__ import annotations
 
 import ctypes as ct
+import threading
 from typing import Optional, Sequence
 
 import numpy as np
 
 from . import _lib as L
-from .virtual import CuboError, _check
+from .virtual import CancellationError, CuboError, _check
 
 
 ANGLES = np.arange(179, dtype=np.float64) - 89.0      # RotationSolver's default angles, -89 ... 89 degrees
-_REFERENCE = {"none": 0, "plane": 1, "constant": 2}
+_REFERENCE = {"none": 0, "plane": 1, "constant": 2, "given": 3}
 
 
 def normalize(mx, my, rows: int, cols: int, reference: str = "plane", sampling: float = 1.0, tol: float = 1e-12,
-              device: int = 0):
-    """CenterOfMassReference.normalize: (reference row, reference column, normalised row, normalised column)."""
+              given=None, device: int = 0):
+    """CenterOfMassReference.normalize: (reference row, reference column, normalised row, normalised column).
+    reference "given" is .given(field): given = (row field, column field), each rows x cols, e.g. an OriginMap's
+    row and column maps (py4DSTEM's force_com_shifts)."""
     n = rows * cols
     x = np.ascontiguousarray(mx, np.float64).reshape(n)
     y = np.ascontiguousarray(my, np.float64).reshape(n)
+    g = [None, None]
+    if reference == "given":
+        if given is None:
+            raise CuboError("the given reference needs a field", case="invalidField")
+        g = [np.ascontiguousarray(getattr(v, "values", v), np.float64) for v in given]
+        if any(v.size != n for v in g):                          # CenterOfMassReference.swift:46-48
+            raise CuboError(f"the given reference has {g[0].size} and {g[1].size} entries, the field {rows} x {cols}",
+                            case="invalidField")
+        g = [v.reshape(n) for v in g]
     fx, fy, nx, ny = (np.zeros(n) for _ in range(4))
     _check(L.lib().cubo_hip_phase_normalize(device, x.ctypes.data, y.ctypes.data, rows, cols, _REFERENCE[reference],
-                                            float(sampling), float(tol), fx.ctypes.data, fy.ctypes.data,
-                                            nx.ctypes.data, ny.ctypes.data), "normalize")
+                                            float(sampling), float(tol), None if g[0] is None else g[0].ctypes.data,
+                                            None if g[1] is None else g[1].ctypes.data, fx.ctypes.data,
+                                            fy.ctypes.data, nx.ctypes.data, ny.ctypes.data), "normalize")
     return fx, fy, nx, ny
 
 
-def rotation_solve(nx, ny, rows: int, cols: int, angles=ANGLES, maximize: bool = False, device: int = 0) -> dict:
-    """RotationSolver.solve: the chosen angle and orientation, both curves and the corrected field."""
+def _cancellable(is_cancelled, call):
+    """Runs call(flag pointer or None). The reference's isCancelled closure becomes the C ABI's flag: it is asked
+    before the call, then polled from a helper thread while the library works (ctypes releases the GIL), which sets
+    the flag the library reads."""
+    if is_cancelled is None:
+        return call(None)
+    if is_cancelled():
+        raise CancellationError()
+    flag = ct.c_int32(0)
+    stop = threading.Event()
+
+    def watch():
+        while not stop.wait(0.0002):
+            if is_cancelled():
+                flag.value = 1
+                return
+
+    th = threading.Thread(target=watch, daemon=True)
+    th.start()
+    try:
+        return call(ct.byref(flag))
+    finally:
+        stop.set()
+        th.join()
+
+
+def rotation_solve(nx, ny, rows: int, cols: int, angles=ANGLES, maximize: bool = False,
+                   force_degrees: Optional[float] = None, force_transpose: Optional[bool] = None,
+                   is_cancelled=None, device: int = 0) -> dict:
+    """RotationSolver.solve: the chosen angle and orientation, the curves that were evaluated (empty otherwise) and
+    the corrected field. force_degrees / force_transpose are forceRotationDegrees / forceTranspose; is_cancelled a
+    callable (CancellationError when it answers True)."""
     n = rows * cols
     x = np.ascontiguousarray(nx, np.float64).reshape(n)
     y = np.ascontiguousarray(ny, np.float64).reshape(n)
     a = np.ascontiguousarray(angles, np.float64)
-    deg, t = ct.c_double(), ct.c_int32()
-    curve, curve_t, cx, cy = np.zeros(a.size), np.zeros(a.size), np.zeros(n), np.zeros(n)
-    _check(L.lib().cubo_hip_phase_rotation(device, x.ctypes.data, y.ctypes.data, rows, cols, a.ctypes.data, a.size,
-                                           int(maximize), ct.byref(deg), ct.byref(t), curve.ctypes.data,
-                                           curve_t.ctypes.data, cx.ctypes.data, cy.ctypes.data), "rotation")
-    return dict(degrees=deg.value, transposed=bool(t.value), curve=curve, curve_transposed=curve_t, cx=cx, cy=cy)
+    deg, t, n0, n1 = ct.c_double(), ct.c_int32(), ct.c_int32(), ct.c_int32()
+    m = 1 if force_degrees is not None else max(a.size, 1)
+    curve, curve_t, cx, cy = np.zeros(m), np.zeros(m), np.zeros(n), np.zeros(n)
+    forced = None if force_degrees is None else ct.c_double(float(force_degrees))
+    ft = -1 if force_transpose is None else int(bool(force_transpose))
+    _cancellable(is_cancelled, lambda flag: _check(L.lib().cubo_hip_phase_rotation(
+        device, x.ctypes.data, y.ctypes.data, rows, cols, a.ctypes.data, a.size, int(maximize),
+        None if forced is None else ct.addressof(forced), ft, flag, ct.byref(deg), ct.byref(t), ct.byref(n0),
+        ct.byref(n1), curve.ctypes.data, curve_t.ctypes.data, cx.ctypes.data, cy.ctypes.data), "rotation"))
+    return dict(degrees=deg.value, radians=deg.value * (np.pi / 180), transposed=bool(t.value),
+                angles=(np.array([deg.value]) if force_degrees is not None and force_transpose is None
+                        else (a if force_degrees is None else np.zeros(0))),
+                curve=curve[: n0.value], curve_transposed=curve_t[: n1.value], cx=cx, cy=cy)
+
+
+def rotation_curve(nx, ny, rows: int, cols: int, angles=ANGLES, maximize: bool = False, transpose: bool = False,
+                   is_cancelled=None, device: int = 0) -> np.ndarray:
+    """RotationSolver.curve (RotationSolver.swift:36-44): the criterion over the angles for one orientation."""
+    s = rotation_solve(nx, ny, rows, cols, angles, maximize, force_transpose=transpose, is_cancelled=is_cancelled,
+                       device=device)
+    return s["curve_transposed"] if transpose else s["curve"]
 
 
 class DPC:
@@ -131,4 +189,4 @@ class DPC:
             pass
 
 
-__all__ = ["ANGLES", "DPC", "CuboError", "normalize", "rotation_solve"]
+__all__ = ["ANGLES", "DPC", "CuboError", "normalize", "rotation_curve", "rotation_solve"]

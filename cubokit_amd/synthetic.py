@@ -79,27 +79,30 @@ def stack_masks(qy: int, qx: int, seed: int = 20261103, n_sectors: int = 29, n_d
 
 BATCH_SCANS = [(32, 32), (64, 64), (96, 128), (128, 128), (256, 256)]
 BATCH_DETECTORS = [64, 96, 128, 256]
-BATCH_CUBE_LIMIT = 2 << 30
 
 
 def batch_items(n: int = 48, seed: int = 20261104):
     """A seeded screening batch of n uneven uint16 cubes: scan in {32x32, 64x64, 96x128, 128x128, 256x256},
-    square detectors of side {64, 96, 128, 256}, cubes of at most 2 GiB, 8..64 masks each (the first k rows of
-    stack_masks for that detector). Returns [(shape, SynthParams, k)]."""
+    square detectors of side {64, 96, 128, 256} (up to 256x256x256x256, 8 GiB), 8..64 masks each (the first k
+    rows of stack_masks for that detector). Returns [(shape, SynthParams, k)]; batch_bytes gives their sum (47.4 GiB
+    for the default seed), which the caller checks against the device (StackBatch.availability)."""
     rng = np.random.default_rng(seed)
     out = []
     while len(out) < n:
         scan = BATCH_SCANS[rng.integers(len(BATCH_SCANS))]
         det = BATCH_DETECTORS[rng.integers(len(BATCH_DETECTORS))]
         shape = (scan[0], scan[1], det, det)
-        if 2 * scan[0] * scan[1] * det * det > BATCH_CUBE_LIMIT:
-            continue
         k = int(rng.integers(8, 65))
         p = params(int(rng.integers(1 << 62)), (det / 2, det / 2), det / 10, float(rng.uniform(100, 3000)),
                    edge=0.5, background=2.0, rings=[(det / 4, 20.0, 1.5)],
                    waves=random_waves(rng, 2, 3.0, scan))
         out.append((shape, p, k))
     return out
+
+
+def batch_bytes(items) -> int:
+    """The uint16 bytes of every cube of a batch_items list."""
+    return int(sum(2 * int(np.prod(shape, dtype=np.int64)) for shape, _, _ in items))
 
 
 def differentiation_u16(seed: int = 20261105, scan=(512, 512)):

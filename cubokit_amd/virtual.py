@@ -70,6 +70,8 @@ def _check(status: int, what: str) -> None:
         raise CuboVirtualError("noHIPDevice", msg or what)
     if status == L.ERR_INVALID_ARGUMENT:
         raise CuboVirtualError("invalidInput", f"{what}: {msg}")
+    if status == L.ERR_CANCELLED:
+        raise CancellationError(f"{what}: {msg}")
     if status == L.ERR_FILE:
         raise CuboError(f"{what}: {msg}", case=msg.split(":")[0].strip())
     raise CuboVirtualError("hip", f"{what}: {msg} (status {status})")
@@ -1025,6 +1027,14 @@ class StackBatch:
         self._h = h
         self.shape = None
 
+    @staticmethod
+    def availability(shapes: Sequence, dtype=np.uint16, hbm_bytes: int = 0, device: int = 0):
+        """ResidentCube.availability for the cubes of a batch held together: None when their summed bytes fit the
+        memory cap and the free HBM, else the UnavailableReason (of the summed bytes). shapes: CubeShapes or
+        4-tuples."""
+        total = sum(int(np.prod(getattr(s, "extents", s), dtype=np.int64)) for s in shapes)
+        return HIPResidentCube.availability(CubeShape(1, 1, 1, max(1, total)), dtype, hbm_bytes, device)
+
     def tune(self) -> dict:
         w, b, o = ct.c_int32(), ct.c_int32(), ct.c_int32()
         _check(L.lib().cubo_hip_batch_tune(self._h, ct.byref(w), ct.byref(b), ct.byref(o)), "tuning the batch")
@@ -1034,8 +1044,8 @@ class StackBatch:
 
     def run(self, out: Optional[Sequence[np.ndarray]] = None, copy_out: bool = True, view: bool = False) -> list:
         """[(scanCount_i, k_i) Double] per item; copy_out=False replays the graph without the host copy.
-        view=True returns read-only arrays over the batch's pinned result buffer (no copy), valid until the next
-        run or release."""
+        view=True returns read-only arrays over the batch's pinned result buffer (no copy); they keep the batch
+        alive, and the next run overwrites them."""
         if view or not copy_out:
             _check(L.lib().cubo_hip_batch_run(self._h, None), "running the batch")
             return self.view() if view else []
@@ -1045,12 +1055,17 @@ class StackBatch:
         return ys
 
     def view(self) -> list:
-        """The last run's results in place: read-only (scanCount_i, k_i) arrays over the pinned buffer."""
+        """The last run's results in place: read-only (scanCount_i, k_i) arrays over the pinned buffer. Each array
+        holds the batch alive through its base (a ctypes array over the buffer that keeps a reference to this
+        batch), so the buffer outlives every view; the values change at the next run."""
         out = []
         for i, (c, k) in enumerate(zip(self.cubes, self.k)):
             p = ct.POINTER(ct.c_double)()
             _check(L.lib().cubo_hip_batch_view(self._h, i, ct.byref(p)), "viewing the batch results")
-            a = np.ctypeslib.as_array(p, (c.shape.scan_count, k))
+            n = c.shape.scan_count * k
+            raw = (ct.c_double * n).from_address(ct.addressof(p.contents))
+            raw._batch = self
+            a = np.frombuffer(raw, dtype=np.float64).reshape(c.shape.scan_count, k)
             a.flags.writeable = False
             out.append(a)
         return out

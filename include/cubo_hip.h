@@ -34,10 +34,12 @@ enum {
     CUBO_HIP_ERR_UNSUPPORTED = -6,
     CUBO_HIP_ERR_INCOMPLETE = -7,
     CUBO_HIP_ERR_EXCEEDS_BUFFER_LIMIT = -8,
-    CUBO_HIP_ERR_FILE = -9     /* a CuboError of the file readers (Sources/CuboCore/CuboError.swift); the message
+    CUBO_HIP_ERR_FILE = -9,    /* a CuboError of the file readers (Sources/CuboCore/CuboError.swift); the message
                                   starts with the case: cannotOpen, notNPY, fortranOrderNotSupported,
                                   bigEndianNotSupported, unsupportedElementType, wrongRank, fileTooShort,
                                   misalignedOffset, invalidShape */
+    CUBO_HIP_ERR_CANCELLED = -10  /* the caller's cancel flag was set: Swift's CancellationError
+                                     (Sources/CuboPhase/ParallelScores.swift:20-38); nothing was written */
 };
 
 /* Element types held on the device: the native detector counts (uint16) or Float32, the type every
@@ -342,21 +344,43 @@ int cubo_hip_shift_plan_destroy(cubo_hip_shift_plan *plan);
 
 /* ---- the centre-of-mass field's reference and rotation (CuboPhase), on the device in Double ----
  * Both calls are thread-safe: they share one stream and one set of device buffers per device (kept between
- * calls), so concurrent calls on one device run one after another.
- * CenterOfMassReference.normalize (Sources/CuboPhase/CenterOfMassReference.swift:27-84): reference 0 = none,
- * 1 = least-squares plane over the finite positions (refused when degenerate: det <= tol * max(suu svv, 1)),
- * 2 = the constant mean; fx/fy get the reference, nx/ny = (measured - reference) with NaN -> 0 and
- * +-inf -> +-max finite, times sampling. All arrays rows x cols. */
+ * calls), so concurrent calls on one device run one after another. */
+
+/* CenterOfMassReference's cases (Sources/CuboPhase/CenterOfMassReference.swift:13-22). */
+enum {
+    CUBO_HIP_REFERENCE_NONE = 0,      /* .none: zero */
+    CUBO_HIP_REFERENCE_PLANE = 1,     /* .planeFit: the least-squares plane over the finite positions */
+    CUBO_HIP_REFERENCE_CONSTANT = 2,  /* .constant: the NumPy mean of each component */
+    CUBO_HIP_REFERENCE_GIVEN = 3      /* .given(field): given_row / given_col, e.g. origins measured by
+                                         cubo_hip_origins (py4DSTEM's force_com_shifts) */
+};
+/* CenterOfMassReference.normalize (CenterOfMassReference.swift:27-84): fx/fy get the reference (the plane is refused
+ * when degenerate: det <= tol * max(suu svv, 1)), nx/ny = (measured - reference) with NaN -> 0 and +-inf -> +-max
+ * finite, times sampling. All arrays rows x cols; given_row/given_col are read only for CUBO_HIP_REFERENCE_GIVEN (the
+ * caller checks that the given field has the measured field's shape, :46-48). tol must be finite and >= 0 (:30-32). */
 int cubo_hip_phase_normalize(int device, const double *mx, const double *my, int32_t rows, int32_t cols,
-                             int32_t reference, double sampling, double tol, double *fx, double *fy, double *nx,
-                             double *ny);
-/* RotationSolver.solve (Sources/CuboPhase/RotationSolver.swift:48-150): curve[a] and curve_t[a] (components
- * swapped first) = mean over the interior of |d_row y' - d_col x'| (the divergence when maximize) for the
- * field rotated by angles_deg[a]; the first best angle wins, the plain orientation wins ties; cx/cy = the
- * field rotated by the chosen angle and orientation. */
+                             int32_t reference, double sampling, double tol, const double *given_row,
+                             const double *given_col, double *fx, double *fy, double *nx, double *ny);
+/* RotationSolver.solve (Sources/CuboPhase/RotationSolver.swift:48-150): the criterion for the field rotated by an
+ * angle (deg * (pi / 180); components swapped first when transposed) is the mean over the interior of
+ * |d_row y' - d_col x'|, or of the divergence |d_row x' + d_col y'| when maximize. The three branches of :66-90:
+ *  - force_degrees NULL, force_transpose -1: both curves over angles_deg (nangles entries each); the first best angle
+ *    wins and the plain orientation wins ties;
+ *  - force_degrees NULL, force_transpose 0 or 1: only that orientation's curve over angles_deg (RotationSolver.curve,
+ *    :36-44); the other curve is not evaluated;
+ *  - *force_degrees, force_transpose -1: both orientations scored at that angle only (one entry each);
+ *  - *force_degrees, force_transpose 0 or 1: nothing scored.
+ * n_curve / n_curve_t receive the number of entries written to curve / curve_t (0 for a curve not evaluated; it may
+ * then be NULL). angles_deg is not read when an angle is forced. cx/cy = the field rotated by the chosen angle and
+ * orientation (RotationSolution.corrected). Refusals (CUBO_HIP_ERR_INVALID_ARGUMENT): a scan under 3 x 3, a
+ * non-finite field, no angles, a non-finite angle (invalidAngles). cancel: NULL or a flag another thread may set,
+ * read before any work and once more after the scores (the reference asks isCancelled at the start and at the end,
+ * ParallelScores.swift:20-38): CUBO_HIP_ERR_CANCELLED, no output written. */
 int cubo_hip_phase_rotation(int device, const double *nx, const double *ny, int32_t rows, int32_t cols,
-                            const double *angles_deg, int32_t nangles, int32_t maximize, double *degrees,
-                            int32_t *transposed, double *curve, double *curve_t, double *cx, double *cy);
+                            const double *angles_deg, int32_t nangles, int32_t maximize, const double *force_degrees,
+                            int32_t force_transpose, const volatile int32_t *cancel, double *degrees,
+                            int32_t *transposed, int32_t *n_curve, int32_t *n_curve_t, double *curve, double *curve_t,
+                            double *cx, double *cy);
 
 /* ---- iterative DPC (DPC.reconstruct, Sources/CuboPhase/DPC.swift:26-102) on the device, in Double, and
  * its adjoint. A plan fixes the scan (rows x cols), the padding (the grid is round(rows * padding) x

@@ -584,8 +584,9 @@ static double clean_nan(double v) {
 }
 
 int oc_normalize(const double *mx, const double *my, int R, int C, int reference, double sampling, double tol,
-                 double *fx, double *fy, double *nx, double *ny) {
+                 const double *gx, const double *gy, double *fx, double *fy, double *nx, double *ny) {
     int64_t N = (int64_t)R * C;
+    if (!(tol >= 0) || !isfinite(tol)) return OC_INVALID_PARAMETER;        /* CenterOfMassReference.swift:30-32 */
     if (reference == 1) {
         int st = oc_plane_fit(mx, R, C, tol, fx);
         if (st) return st;
@@ -594,6 +595,8 @@ int oc_normalize(const double *mx, const double *my, int R, int C, int reference
     } else if (reference == 2) {
         double a = np_mean(mx, N), b = np_mean(my, N);
         for (int64_t k = 0; k < N; ++k) { fx[k] = a; fy[k] = b; }
+    } else if (reference == 3) {                                             /* .given(field), :45-49 */
+        for (int64_t k = 0; k < N; ++k) { fx[k] = gx[k]; fy[k] = gy[k]; }
     } else {
         for (int64_t k = 0; k < N; ++k) { fx[k] = 0; fy[k] = 0; }
     }
@@ -660,21 +663,58 @@ int oc_rotation_curve(const double *nx, const double *ny, int R, int C, const do
     return OC_OK;
 }
 
+/* RotationSolver.swift:48-90, with its three branches: a forced angle (both orientations scored there unless the
+ * orientation is forced too), a forced orientation (one curve), or the full search (both curves). */
+static int better(int maximize, double a, double b) { return maximize ? a > b : a < b; }
+
+static int best_index(int maximize, const double *curve, int n) {
+    int k = 0;
+    for (int i = 1; i < n; ++i)
+        if (better(maximize, curve[i], curve[k])) k = i;
+    return k;
+}
+
 int oc_rotation_solve(const double *nx, const double *ny, int R, int C, const double *angles_deg, int nangles,
-                      int maximize, double *degrees, int *transposed, double *curve, double *curve_t,
-                      double *cx, double *cy) {
-    int st = oc_rotation_curve(nx, ny, R, C, angles_deg, nangles, 0, maximize, curve);
+                      int maximize, const double *force_deg, int force_t, double *degrees, int *transposed,
+                      int *n_curve, int *n_curve_t, double *curve, double *curve_t, double *cx, double *cy) {
+    int st = usable(nx, ny, R, C);
     if (st) return st;
-    oc_rotation_curve(nx, ny, R, C, angles_deg, nangles, 1, maximize, curve_t);
-    int i = 0, j = 0;
-    for (int k = 1; k < nangles; ++k) {
-        if (maximize ? curve[k] > curve[i] : curve[k] < curve[i]) i = k;
-        if (maximize ? curve_t[k] > curve_t[j] : curve_t[k] < curve_t[j]) j = k;
+    *n_curve = 0;
+    *n_curve_t = 0;
+    if (force_deg) {
+        if (!isfinite(*force_deg)) return OC_INVALID_PARAMETER;              /* invalidAngles, :67 */
+        *degrees = *force_deg;
+        if (force_t >= 0) {
+            *transposed = force_t;
+        } else {
+            oc_rotation_curve(nx, ny, R, C, force_deg, 1, 0, maximize, curve);
+            oc_rotation_curve(nx, ny, R, C, force_deg, 1, 1, maximize, curve_t);
+            *n_curve = 1;
+            *n_curve_t = 1;
+            *transposed = better(maximize, curve_t[0], curve[0]);
+        }
+    } else {
+        if (nangles < 1) return OC_INVALID_PARAMETER;
+        for (int a = 0; a < nangles; ++a)
+            if (!isfinite(angles_deg[a])) return OC_INVALID_PARAMETER;     /* :74-76 */
+        if (force_t >= 0) {
+            double *c = force_t ? curve_t : curve;
+            oc_rotation_curve(nx, ny, R, C, angles_deg, nangles, force_t, maximize, c);
+            *(force_t ? n_curve_t : n_curve) = nangles;
+            *transposed = force_t;
+            *degrees = angles_deg[best_index(maximize, c, nangles)];
+        } else {
+            oc_rotation_curve(nx, ny, R, C, angles_deg, nangles, 0, maximize, curve);
+            oc_rotation_curve(nx, ny, R, C, angles_deg, nangles, 1, maximize, curve_t);
+            *n_curve = nangles;
+            *n_curve_t = nangles;
+            int i = best_index(maximize, curve, nangles), j = best_index(maximize, curve_t, nangles);
+            int t = better(maximize, curve_t[j], curve[i]);                 /* the plain orientation wins ties */
+            *transposed = t;
+            *degrees = angles_deg[t ? j : i];
+        }
     }
-    int t = maximize ? curve_t[j] > curve[i] : curve_t[j] < curve[i];   /* the plain orientation wins ties */
-    *transposed = t;
-    *degrees = angles_deg[t ? j : i];
-    oc_rotate(nx, ny, (int64_t)R * C, *degrees * (M_PI / 180.0), t, cx, cy);
+    oc_rotate(nx, ny, (int64_t)R * C, *degrees * (M_PI / 180.0), *transposed, cx, cy);
     return OC_OK;
 }
 

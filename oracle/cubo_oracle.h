@@ -92,16 +92,19 @@ void oc_fft2d(double *re, double *im, int rows, int cols, int inverse);
 
 /* CuboPhase chain. */
 int oc_plane_fit(const double *m, int R, int C, double tol, double *out);
-int oc_normalize(const double *mx, const double *my, int R, int C, int reference /*0 none,1 plane,2 constant*/,
-                 double sampling, double tol, double *fx, double *fy, double *nx, double *ny);
+/* reference: 0 none, 1 plane, 2 constant, 3 the given field (gx, gy; unused otherwise). */
+int oc_normalize(const double *mx, const double *my, int R, int C, int reference, double sampling, double tol,
+                 const double *gx, const double *gy, double *fx, double *fy, double *nx, double *ny);
 void oc_rotate(const double *x, const double *y, int64_t n, double radians, int transpose, double *ox, double *oy);
 double oc_curl(const double *x, const double *y, int R, int C);
 double oc_divergence(const double *x, const double *y, int R, int C);
 int oc_rotation_curve(const double *nx, const double *ny, int R, int C, const double *angles_deg, int nangles,
                       int transpose, int maximize, double *curve);
+/* force_deg: NULL or the forced angle; force_t: -1 (search the orientation), 0 or 1. n_curve / n_curve_t receive
+ * the length of each curve evaluated (0 when that orientation was not evaluated). */
 int oc_rotation_solve(const double *nx, const double *ny, int R, int C, const double *angles_deg, int nangles,
-                      int maximize, double *degrees, int *transposed, double *curve, double *curve_t,
-                      double *cx, double *cy);
+                      int maximize, const double *force_deg, int force_t, double *degrees, int *transposed,
+                      int *n_curve, int *n_curve_t, double *curve, double *curve_t, double *cx, double *cy);
 int oc_dpc_reconstruct(const double *fx, const double *fy, int R, int C, double sr, double sc, double padding,
                        int max_iter, double step, double stop, int backtrack, double *phase, double *errors,
                        int *accepted, int *n_iter, double *final_step);

@@ -59,13 +59,15 @@ def _load():
         "oc_pairwise_sum": (ct.c_double, [_dp, _i64]),
         "oc_fft2d": (None, [_dp, _dp, ct.c_int, ct.c_int, ct.c_int]),
         "oc_plane_fit": (ct.c_int, [_dp, ct.c_int, ct.c_int, ct.c_double, _dp]),
-        "oc_normalize": (ct.c_int, [_dp, _dp, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double, _dp, _dp, _dp, _dp]),
+        "oc_normalize": (ct.c_int, [_dp, _dp, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_void_p,
+                                   ct.c_void_p, _dp, _dp, _dp, _dp]),
         "oc_rotate": (None, [_dp, _dp, _i64, ct.c_double, ct.c_int, _dp, _dp]),
         "oc_curl": (ct.c_double, [_dp, _dp, ct.c_int, ct.c_int]),
         "oc_divergence": (ct.c_double, [_dp, _dp, ct.c_int, ct.c_int]),
         "oc_rotation_curve": (ct.c_int, [_dp, _dp, ct.c_int, ct.c_int, _dp, ct.c_int, ct.c_int, ct.c_int, _dp]),
-        "oc_rotation_solve": (ct.c_int, [_dp, _dp, ct.c_int, ct.c_int, _dp, ct.c_int, ct.c_int,
-                                        ct.POINTER(ct.c_double), ct.POINTER(ct.c_int), _dp, _dp, _dp, _dp]),
+        "oc_rotation_solve": (ct.c_int, [_dp, _dp, ct.c_int, ct.c_int, ct.c_void_p, ct.c_int, ct.c_int,
+                                        ct.c_void_p, ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int),
+                                        ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), _dp, _dp, _dp, _dp]),
         "oc_dpc_reconstruct": (ct.c_int, [_dp, _dp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_int,
                                          ct.c_double, ct.c_double, ct.c_int, _dp, _dp, _ip,
                                          ct.POINTER(ct.c_int), ct.POINTER(ct.c_double)]),
@@ -299,7 +301,7 @@ def fft2(re, im=None, inverse=False):
 
 
 # --- CuboPhase chain ----------------------------------------------------------------------------
-_REF = {"none": 0, "plane": 1, "constant": 2}
+_REF = {"none": 0, "plane": 1, "constant": 2, "given": 3}
 
 
 def plane_fit(m, R, C, tol=1e-12):
@@ -308,26 +310,34 @@ def plane_fit(m, R, C, tol=1e-12):
     return out
 
 
-def normalize(mx, my, R, C, reference="plane", sampling=1.0, tol=1e-12):
+def normalize(mx, my, R, C, reference="plane", sampling=1.0, tol=1e-12, given=None):
+    """reference "given" takes given = (row field, column field), each R*C."""
     fx, fy, nx, ny = (np.zeros(R * C) for _ in range(4))
+    g = [None, None] if given is None else [np.ascontiguousarray(v, np.float64).reshape(R * C) for v in given]
     _check(lib().oc_normalize(np.ascontiguousarray(mx, np.float64), np.ascontiguousarray(my, np.float64), R, C,
-                              _REF[reference], sampling, tol, fx, fy, nx, ny), "normalize")
+                              _REF[reference], sampling, tol, None if g[0] is None else g[0].ctypes.data,
+                              None if g[1] is None else g[1].ctypes.data, fx, fy, nx, ny), "normalize")
     return fx, fy, nx, ny
 
 
 DEFAULT_ANGLES = np.arange(179, dtype=np.float64) - 89.0
 
 
-def rotation_solve(nx, ny, R, C, angles=DEFAULT_ANGLES, maximize=False):
+def rotation_solve(nx, ny, R, C, angles=DEFAULT_ANGLES, maximize=False, force_degrees=None, force_transpose=None):
+    """RotationSolver.solve; curves not evaluated come back empty."""
     angles = np.ascontiguousarray(angles, np.float64)
     deg = ct.c_double()
-    t = ct.c_int()
-    curve, curve_t = np.zeros(angles.size), np.zeros(angles.size)
+    t, n0, n1 = ct.c_int(), ct.c_int(), ct.c_int()
+    m = max(angles.size, 1)
+    curve, curve_t = np.zeros(m), np.zeros(m)
     cx, cy = np.zeros(R * C), np.zeros(R * C)
+    forced = None if force_degrees is None else ct.byref(ct.c_double(float(force_degrees)))
+    ft = -1 if force_transpose is None else int(bool(force_transpose))
     _check(lib().oc_rotation_solve(np.ascontiguousarray(nx, np.float64), np.ascontiguousarray(ny, np.float64), R, C,
-                                   angles, angles.size, int(maximize), ct.byref(deg), ct.byref(t), curve, curve_t,
-                                   cx, cy), "rotation")
-    return dict(degrees=deg.value, transposed=bool(t.value), curve=curve, curve_transposed=curve_t, cx=cx, cy=cy)
+                                   angles.ctypes.data, angles.size, int(maximize), forced, ft, ct.byref(deg),
+                                   ct.byref(t), ct.byref(n0), ct.byref(n1), curve, curve_t, cx, cy), "rotation")
+    return dict(degrees=deg.value, transposed=bool(t.value), curve=curve[: n0.value],
+                curve_transposed=curve_t[: n1.value], cx=cx, cy=cy)
 
 
 def rotation_curve(nx, ny, R, C, angles=DEFAULT_ANGLES, transpose=False, maximize=False):
@@ -371,13 +381,15 @@ def dpc_step_vjp(phi, fx, fy, R, C, Rp, Cp, g_phi_out, g_err, sampling=(1.0, 1.0
 
 
 def cpu_first_processing_u16(cube_u16: np.ndarray, qy: int, qx: int, positions_per_row: int, bf, adf,
-                             threads: int):
-    """The reference's .cpu pass (VI x2, mean, max, CoM) over a uint16 cube, timed as the CPU baseline."""
+                             threads: int, rows_per_tile: int = 0):
+    """The reference's .cpu pass (VI x2, mean, max, CoM) over a uint16 cube, timed as the CPU baseline.
+    rows_per_tile 0 is ScanTiling.automatic (the most scan rows whose Float32 tile fits 64 MiB)."""
     x = np.ascontiguousarray(cube_u16, np.uint16).reshape(-1)
     ndet = qy * qx
     npos = x.size // ndet
     row_bytes = positions_per_row * ndet * 4
-    rows_per_tile = max(1, min(npos // positions_per_row, (64 << 20) // max(1, row_bytes)))
+    if rows_per_tile <= 0:
+        rows_per_tile = max(1, min(npos // positions_per_row, (64 << 20) // max(1, row_bytes)))
     out = [np.zeros(npos), np.zeros(npos), np.zeros(ndet), np.zeros(ndet), np.zeros(npos), np.zeros(npos),
            np.zeros(npos)]
     lib().oc_cpu_first_processing_u16(x, npos, qy, qx, rows_per_tile, positions_per_row,

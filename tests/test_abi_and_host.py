@@ -1,5 +1,6 @@
 """CPU-only checks of the boundary and the host logic: the library loads, exports every symbol the header
 declares, and the host mirror's masks, tiling, shape checks and availability follow the reference."""
+import ctypes as ct
 import os
 import re
 
@@ -93,3 +94,40 @@ def test_non_hip_paths_are_refused_not_emulated():
     with pytest.raises(C.CuboVirtualError):
         C.VirtualImaging.image(np.zeros((1, 1, 4, 4), np.float32), C.DetectorGeometry.point(0, 0),
                                path=C.ComputePath.cpu)
+
+
+def test_phase_calls_refuse_before_any_device_call():
+    """The reference's refusals come before the device is touched, so they hold here without a GPU: a degeneracy
+    tolerance that is negative or not finite (CenterOfMassReference.swift:30-32), a given reference of the wrong size
+    (:46-48), non-finite angles (invalidAngles, RotationSolver.swift:67, 74-76), a non-finite field or a scan under
+    3 x 3 (:128-135); a cancel flag already set gives CancellationError."""
+    from cubokit_amd import phase as P
+    x = np.arange(20, dtype=np.float64)
+    for tol in (-1e-12, np.nan, np.inf):
+        with pytest.raises(C.CuboVirtualError) as e:
+            P.normalize(x, x, 4, 5, "plane", tol=tol)
+        assert e.value.case == "invalidInput"
+    with pytest.raises(C.CuboError) as e:
+        P.normalize(x, x, 4, 5, "given", given=(x[:19], x[:19]))
+    assert e.value.case == "invalidField"
+    for kw in (dict(angles=np.array([0.0, np.nan])), dict(angles=np.array([np.inf])), dict(angles=np.zeros(0)),
+               dict(force_degrees=np.nan), dict(force_degrees=np.inf, force_transpose=True)):
+        with pytest.raises(C.CuboVirtualError) as e:
+            P.rotation_solve(x, x, 4, 5, **kw)
+        assert e.value.case == "invalidInput", kw
+    bad = x.copy()
+    bad[3] = np.inf
+    with pytest.raises(C.CuboVirtualError):
+        P.rotation_solve(bad, x, 4, 5, force_degrees=3.0, force_transpose=False)
+    with pytest.raises(C.CuboVirtualError):
+        P.rotation_solve(x[:6], x[:6], 2, 3)
+    with pytest.raises(C.CancellationError):
+        P.rotation_solve(x, x, 4, 5, is_cancelled=lambda: True)
+    # through the C ABI: a forced angle and orientation (nothing scored) with the flag already set
+    flag, forced = ct.c_int32(1), ct.c_double(1.0)
+    deg, t, n0, n1 = ct.c_double(), ct.c_int32(), ct.c_int32(), ct.c_int32()
+    cx, cy = np.zeros(20), np.zeros(20)
+    st = L.lib().cubo_hip_phase_rotation(0, x.ctypes.data, x.ctypes.data, 4, 5, None, 0, 0, ct.addressof(forced), 0,
+                                         ct.addressof(flag), ct.byref(deg), ct.byref(t), ct.byref(n0), ct.byref(n1),
+                                         None, None, cx.ctypes.data, cy.ctypes.data)
+    assert st == L.ERR_CANCELLED

@@ -129,3 +129,58 @@ def test_batch_rejects_what_the_matrix_core_method_cannot_run():
         C.StackBatch([(ok, bad)])
     with pytest.raises(C.CuboError):
         C.StackBatch([])
+
+
+def test_view_keeps_the_batch_alive():
+    """The in-place views hold the batch (and its pinned buffer) through their base: a batch dropped right after
+    run(view=True) still leaves readable, unchanged results."""
+    import gc
+    items = _batch_items(seed=13)
+    copied = C.StackBatch(items).run()
+    views = C.StackBatch(items).run(view=True)             # the batch object itself is not kept here
+    gc.collect()
+    for j, (x, v) in enumerate(zip(copied, views)):
+        base = v
+        while base is not None and not hasattr(base, "_batch"):
+            base = getattr(base, "base", None)
+        assert base is not None and isinstance(base._batch, C.StackBatch), f"item {j}: no batch in the base chain"
+        assert np.array_equal(x, v), f"item {j}: the view changed after the batch was dropped"
+
+
+def test_screening_batch_of_48_cubes():
+    """The 48-cube screening workload as specified: scans {32^2, 64^2, 96x128, 128^2, 256^2}, detectors
+    {64^2, 96^2, 128^2, 256^2} (two 256^4 cubes of 8 GiB; 47.4 GiB in all), 8..64 masks each, after an
+    availability check of the whole batch. Every item equals its own cube's single split-FP16 stack bit for bit, and
+    eight sampled positions of every cube are within 1e-6 of the oracle's FP64 contraction (relative to sum x w,
+    the masks being non-negative)."""
+    specs = synthetic.batch_items()
+    assert len(specs) == 48 and max(s[2] for s, _, _ in specs) == 256 and max(s[0] for s, _, _ in specs) == 256
+    assert C.StackBatch.availability([s for s, _, _ in specs]) is None, "the batch does not fit the device"
+    items = []
+    for j, (shape, p, k) in enumerate(specs):
+        c = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=np.uint16)
+        c.fill_synthetic(p)
+        items.append((c, np.ascontiguousarray(synthetic.stack_masks(shape[2], shape[3])[:k])))
+    print(f"case: 48-cube batch, {synthetic.batch_bytes(specs) / 2**30:.1f} GiB", flush=True)
+    b = C.StackBatch(items)
+    b.tune()
+    ys = b.run()
+    rng = np.random.default_rng(48)
+    for j, ((c, masks), y) in enumerate(zip(items, ys)):
+        print(f"case: batch item {j} {c.shape.extents} k={masks.shape[0]}", flush=True)
+        single = C.MaskStack(c, masks, C.MaskStack.MFMA_SPLIT).run()
+        assert np.array_equal(y, single), f"item {j}: batch differs from the single stack"
+        rows = sorted(set(int(r) for r in rng.integers(0, c.shape.scan_rows, 2)))
+        for r in rows:
+            host = c.read_scan_tile(range(r, r + 1)).reshape(c.shape.scan_columns, -1)
+            cols = rng.choice(c.shape.scan_columns, size=min(4, c.shape.scan_columns), replace=False)
+            sub = host[cols].astype(np.float32)
+            ref = O.mask_stack(sub.reshape(len(cols), 1, c.shape.detector_rows, c.shape.detector_columns),
+                               masks.astype(np.float64))
+            got = y[r * c.shape.scan_columns + cols]
+            assert np.all(masks >= 0)
+            err = np.abs(got - ref) / np.maximum(ref, np.finfo(np.float64).tiny)
+            assert err.max() <= 1e-6, f"item {j} row {r}: error {err.max():.3e}"
+    b.release()
+    for c, _ in items:
+        c.release()

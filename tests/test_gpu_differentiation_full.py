@@ -81,8 +81,10 @@ def test_origin_adjoint_against_central_differences(workload):
 
 
 def test_dpc_chain_adjoint_on_the_full_field():
+    """The differentiation workload's 20 fixed-step DPC iterations on the 512x512 field (a 1024x1024 padded grid) and
+    their adjoint against the oracle's chain of steps and step VJPs."""
     R = C_ = 512
-    n, step = 5, 0.5
+    n, step = 20, 0.5
     rng = np.random.default_rng(4)
     yy, xx = np.mgrid[0:R, 0:C_] / 512.0
     fx = (np.sin(2 * np.pi * 3 * yy) * np.cos(2 * np.pi * 2 * xx)).reshape(-1) + 0.01 * rng.normal(size=R * C_)

@@ -13,7 +13,7 @@ import cubokit_amd as C
 from cubokit_amd import origins
 from oracle import oracle as O
 from tests.support import assert_relative, doubles, load_cube, origin_fit_pins
-from tests.test_oracle_origin_pins import iterated_window_cube
+from tests.test_oracle_origin_pins import iterated_window_cube, iterated_window_hand_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -111,6 +111,19 @@ def test_iterated_window_matches_the_oracle(ring, nan):
             assert abs(m.row[k] - tr) < 0.1 and abs(m.column[k] - tc) < 0.1
     with pytest.raises(C.CuboVirtualError):
         C.OriginMeasurement.measure_with(res, 6.2, estimator=C.OriginEstimator.iterated_window(passes=1001))
+
+
+def test_iterated_window_by_hand():
+    """The by-hand cases of the oracle's pin (pass count, margin, binned and unclamped seed, clamped pass, the median
+    fill of non-finite pixels, an all-NaN pattern) on the device, each pattern one position of a 1x1 scan."""
+    for name, pat, kw, (wr, wc) in iterated_window_hand_cases():
+        print(f"case: iterated window by hand, {name}", flush=True)
+        res = C.HIPResidentCube(np.ascontiguousarray(pat.reshape(1, 1, 16, 16)))
+        est = C.OriginEstimator.iterated_window(passes=kw["passes"], minimum_margin=kw["margin"],
+                                                binned_coarse=kw["binned"])
+        m = C.OriginMeasurement.measure_with(res, 2.0, radius_scale=1.2, estimator=est)
+        assert abs(m.row[0] - wr) <= 1e-5 * max(1, abs(wr)) and abs(m.column[0] - wc) <= 1e-5 * max(1, abs(wc)), (
+            name, m.row[0], m.column[0])
 
 
 def test_bandwidth_cube_origins_against_the_oracle_on_two_scan_rows():

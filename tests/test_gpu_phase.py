@@ -11,7 +11,7 @@ import pytest
 import cubokit_amd as C
 from cubokit_amd import phase as P
 from cubokit_amd.virtual import CuboVirtualError
-from tests.support import com_pins, doubles, load_cube, near, of_max
+from tests.support import com_pins, doubles, load_cube, near, of_max, origin_fit_pins
 
 pytestmark = pytest.mark.gpu
 
@@ -78,3 +78,123 @@ def test_reference_variants_and_refusals():
     z = np.zeros(20)
     s = P.rotation_solve(z, z, 4, 5)
     assert s["degrees"] == -89 and not s["transposed"]            # all ties: the first angle, plain orientation
+
+
+def _planted(theta):
+    f = next(f for f in com_pins()["fields"] if f["theta"] == theta)
+    R, C_ = f["shape"]
+    return doubles(f["measured_x"]), doubles(f["measured_y"]), R, C_
+
+
+def _same_solution(got, want, name):
+    assert got["degrees"] == want["degrees"] and got["transposed"] == want["transposed"], name
+    assert got["curve"].size == want["curve"].size and got["curve_transposed"].size == want["curve_transposed"].size
+    if want["curve"].size:
+        near(got["curve"], want["curve"], 1e-10, f"{name} curve")
+    if want["curve_transposed"].size:
+        near(got["curve_transposed"], want["curve_transposed"], 1e-10, f"{name} curve transposed")
+    assert np.max(np.abs(got["cx"] - want["cx"])) <= 1e-12 * max(1, np.max(np.abs(want["cx"]))), name
+    assert np.max(np.abs(got["cy"] - want["cy"])) <= 1e-12 * max(1, np.max(np.abs(want["cy"]))), name
+
+
+def test_forced_angle_and_orientation():
+    """CuboPhaseTests.swift:59-75 on the device: a forced angle keeps the plain orientation on the theta = 23 field;
+    a forced angle and orientation evaluate no curve; a forced orientation evaluates only its own curve and finds
+    the planted angle. Every branch against the oracle, on both criteria."""
+    from oracle import oracle as O
+    mx, my, R, C_ = _planted(23)
+    for maximize in (False, True):
+        for kw in (dict(force_degrees=23.0), dict(force_degrees=23.0, force_transpose=True),
+                   dict(force_transpose=False), dict(force_transpose=True), dict(force_degrees=-40.5)):
+            print(f"case: rotation maximize={maximize} {kw}", flush=True)
+            _same_solution(P.rotation_solve(mx, my, R, C_, maximize=maximize, **kw),
+                           O.rotation_solve(mx, my, R, C_, maximize=maximize, **kw), f"{maximize} {kw}")
+    forced = P.rotation_solve(mx, my, R, C_, force_degrees=23)
+    assert forced["degrees"] == 23 and not forced["transposed"] and forced["curve"].size == 1
+    both = P.rotation_solve(mx, my, R, C_, force_degrees=23, force_transpose=True)
+    assert both["transposed"] and both["curve"].size == 0 and both["curve_transposed"].size == 0
+    only = P.rotation_solve(mx, my, R, C_, force_transpose=False)
+    assert only["degrees"] == 23 and only["curve_transposed"].size == 0
+    np.testing.assert_array_equal(P.rotation_curve(mx, my, R, C_, transpose=False), only["curve"])
+    full = P.rotation_solve(mx, my, R, C_)
+    k = list(P.ANGLES).index(23.0)
+    assert full["curve"][k] <= 1e-12 * full["curve"].max()
+
+
+def test_divergence_criterion():
+    """maximize (RotationCriterion.maximizeDivergence) on the planted fields against the oracle, and on a linear
+    field rotated by -31 degrees against the closed forms 2|a+b||cos(theta - 31)| (plain) and
+    2|b-a||sin(theta + 31)| (transposed): the optimum is the planted angle in the plain orientation."""
+    from oracle import oracle as O
+    from tests.test_oracle_pins import divergence_field
+    for f in com_pins()["fields"]:
+        R, C_ = f["shape"]
+        mx, my = doubles(f["measured_x"]), doubles(f["measured_y"])
+        print(f"case: divergence on the planted field {f['theta']}", flush=True)
+        _same_solution(P.rotation_solve(mx, my, R, C_, maximize=True), O.rotation_solve(mx, my, R, C_, maximize=True),
+                       f"divergence {f['theta']}")
+    R, C_, a, b, t0 = 12, 15, 1.0, 2.0, 31.0
+    mx, my = divergence_field(R, C_, a, b, t0)
+    d = P.rotation_solve(mx, my, R, C_, maximize=True)
+    np.testing.assert_allclose(d["curve"], 2 * abs(a + b) * np.abs(np.cos(np.deg2rad(P.ANGLES - t0))), rtol=0,
+                               atol=1e-12)
+    np.testing.assert_allclose(d["curve_transposed"], 2 * abs(b - a) * np.abs(np.sin(np.deg2rad(P.ANGLES + t0))),
+                               rtol=0, atol=1e-12)
+    assert d["degrees"] == t0 and not d["transposed"]
+    z = np.zeros(20)
+    s = P.rotation_solve(z, z, 4, 5, maximize=True)
+    assert s["degrees"] == -89 and not s["transposed"]            # all ties: the first angle, plain orientation
+
+
+def test_given_reference():
+    """CuboPhaseTests.swift:95-96: .given(plane) normalises the plane to exactly zero; a given field with NaN and
+    infinite measured entries gives the oracle's bits."""
+    from oracle import oracle as O
+    p = com_pins()["plane"]
+    R, C_ = p["shape"]
+    x, y = doubles(p["x"]), doubles(p["y"])
+    fx, fy, nx, ny = P.normalize(x, y, R, C_, "given", given=(x, y))
+    assert np.all(nx == 0) and np.all(ny == 0) and np.array_equal(fx, x) and np.array_equal(fy, y)
+    rng = np.random.default_rng(3)
+    mx, my, gx, gy = (rng.normal(size=R * C_) for _ in range(4))
+    mx[2], my[4], mx[6] = np.nan, np.inf, -np.inf
+    got = P.normalize(mx, my, R, C_, "given", sampling=0.5, given=(gx, gy))
+    want = O.normalize(mx, my, R, C_, "given", sampling=0.5, given=(gx, gy))
+    for a_, b_ in zip(got, want):
+        np.testing.assert_array_equal(a_, b_)
+
+
+def test_cancellation():
+    mx, my, R, C_ = _planted(0)
+    with pytest.raises(C.CancellationError):
+        P.rotation_solve(mx, my, R, C_, is_cancelled=lambda: True)
+    s = P.rotation_solve(mx, my, R, C_, is_cancelled=lambda: False)
+    assert s["degrees"] == 0
+
+
+def test_chain_from_measured_origins():
+    """Origins measured on the device (py4DSTEM's get_origin) are the given reference of the device's own centre of
+    mass (py4DSTEM's force_com_shifts), then the rotation and DPC: each stage against the oracle fed the device's
+    output of the stage before, on CuboKit's origin-fit cubes."""
+    from oracle import oracle as O
+    for e in origin_fit_pins()["cubes"]:
+        print(f"case: origins -> given -> rotation -> DPC on {e['cube']}", flush=True)
+        arr = load_cube(e["cube"])
+        R, C_ = arr.shape[:2]
+        res = C.HIPResidentCube(arr)
+        omap, radius = C.OriginMeasurement.measure(res)
+        com = C.CenterOfMass.measure(res)
+        r, c = O.origin_py4dstem(arr, radius, 1.2)
+        assert np.max(np.abs(omap.row - r)) <= 1e-12 and np.max(np.abs(omap.column - c)) <= 1e-12
+        got = P.normalize(com.row.values, com.column.values, R, C_, "given", given=(omap.row, omap.column))
+        want = O.normalize(com.row.values, com.column.values, R, C_, "given", given=(omap.row, omap.column))
+        for a_, b_ in zip(got, want):
+            np.testing.assert_array_equal(a_, b_)
+        nx, ny = got[2], got[3]
+        sol = P.rotation_solve(nx, ny, R, C_)
+        _same_solution(sol, O.rotation_solve(nx, ny, R, C_), e["cube"])
+        dpc = P.DPC(R, C_).reconstruct(sol["cx"], sol["cy"])
+        ref = O.dpc_reconstruct(sol["cx"], sol["cy"], R, C_)
+        assert dpc["accepted"] == ref["accepted"], e["cube"]
+        near(dpc["errors"], ref["errors"], 1e-9, f"{e['cube']} errors")
+        of_max(dpc["phase"], ref["phase"], 1e-8, f"{e['cube']} phase")

@@ -154,3 +154,50 @@ def test_iterated_window_recovers_the_planted_origin():
     r, c = O.origin_iterated(cube, 6.2)
     for k, (tr, tc) in enumerate(truth):
         assert abs(r[k] - tr) < 0.1 and abs(c[k] - tc) < 0.1, (k, r[k], c[k], tr, tc)
+
+
+def iterated_window_hand_cases():
+    """Tiny 16x16 patterns whose iterated-window origins follow by hand from IteratedWindowOrigin.swift:49-168 (radius
+    2, scale 1.2: the window is max(2.4, 2 + margin)). Each case: (name, pattern, keyword arguments, (row, column)).
+    - pass count: pixels 10 at (8,8), 10 at (8,11), 5 at (8,12), seed the first brightest pixel (8,8); a window of 3
+      takes (8,11) (distance 3 is inside) and not (8,12): pass 1 gives column (80+110)/20 = 9.5, pass 2 from there
+      also takes (8,12): (80+110+60)/25 = 10, pass 3 moves by 0 and stops;
+    - margin: a window of 2.5 around (8,8) holds only (8,8), so the centre stays;
+    - binned seed: 2x2 blocks, the first brightest [8,9]x[8,9], centre (8.5, 8.5);
+    - clamping: a -4 at (8,9) lowers that block's sum to 6 (the coarse sums are not clamped), so the seed is block
+      [8,9]x[10,11], centre (8.5, 10.5); the pass clamps it to 0: (8, (80+110+60)/25);
+    - non-finite fill: a NaN at (8,8) reads as the median of its 8 neighbours 1,1,1,1,2,50,60,70 = 1.5 (not their mean
+      23.25); the seed is the 70 at (9,9) and one pass takes the 3x3 block: mass 187.5, row 1677/187.5,
+      column 1521/187.5; with an infinity at (7,7) too, (8,8) reads the median of 7 values (2) and (7,7) the median of
+      its finite neighbours 0,0,0,0,0,1,1 (0): mass 187, row 1674/187, column 1518/187;
+    - no finite neighbour: every pixel NaN reads 0, every block sums to 0 and the first block wins, (0.5, 0.5), and a
+      pass with no mass keeps it."""
+    cases = []
+    base = np.zeros((16, 16), np.float32)
+    p = base.copy()
+    p[8, 8], p[8, 11], p[8, 12] = 10, 10, 5
+    for passes, col in ((0, 8.0), (1, 9.5), (2, 10.0), (3, 10.0), (4, 10.0)):
+        cases.append((f"passes {passes}", p, dict(passes=passes, margin=1.0, binned=False), (8.0, col)))
+    cases.append(("margin 0.5", p, dict(passes=4, margin=0.5, binned=False), (8.0, 8.0)))
+    cases.append(("binned seed", p, dict(passes=0, margin=1.0, binned=True), (8.5, 8.5)))
+    q = p.copy()
+    q[8, 9] = -4
+    cases.append(("unclamped seed", q, dict(passes=0, margin=1.0, binned=True), (8.5, 10.5)))
+    cases.append(("clamped pass", q, dict(passes=1, margin=1.0, binned=True), (8.0, 10.0)))
+    f = base.copy()
+    for (i, j), v in {(7, 7): 1, (7, 8): 1, (7, 9): 1, (8, 7): 1, (8, 9): 2, (9, 7): 50, (9, 8): 60, (9, 9): 70}.items():
+        f[i, j] = v
+    f[8, 8] = np.nan
+    cases.append(("NaN median of 8", f, dict(passes=1, margin=1.0, binned=False), (1677 / 187.5, 1521 / 187.5)))
+    g = f.copy()
+    g[7, 7] = np.inf
+    cases.append(("NaN median of 7", g, dict(passes=1, margin=1.0, binned=False), (1674 / 187, 1518 / 187)))
+    cases.append(("all NaN", np.full((16, 16), np.nan, np.float32), dict(passes=4, margin=1.0, binned=True),
+                  (0.5, 0.5)))
+    return cases
+
+
+def test_iterated_window_by_hand():
+    for name, pat, kw, (wr, wc) in iterated_window_hand_cases():
+        r, c = O.origin_iterated(pat.reshape(1, 1, 16, 16), 2.0, 1.2, **kw)
+        assert abs(r[0] - wr) <= 1e-5 * max(1, abs(wr)) and abs(c[0] - wc) <= 1e-5 * max(1, abs(wc)), (name, r, c)

@@ -356,3 +356,185 @@ def test_phase_refusals():
         O.dpc_reconstruct(np.zeros(9), np.zeros(9), 3, 3)
     with pytest.raises(O.OracleError):
         O.plane_fit(np.array([1.0, 2, 3, 4]), 1, 4)
+
+
+def _planted(theta):
+    f = next(f for f in com_pins()["fields"] if f["theta"] == theta)
+    R, C = f["shape"]
+    return doubles(f["measured_x"]), doubles(f["measured_y"]), R, C
+
+
+def test_forced_angle_and_orientation():
+    """CuboPhaseTests.swift:59-75 on the theta = 23 planted field: a forced angle scores both orientations there (one
+    entry each, the same values as the full search's entries at that angle); a forced angle and orientation evaluate
+    no curve; a forced orientation evaluates only its own curve and picks the planted angle."""
+    mx, my, R, C = _planted(23)
+    full = O.rotation_solve(mx, my, R, C)
+    k = list(O.DEFAULT_ANGLES).index(23.0)
+    forced = O.rotation_solve(mx, my, R, C, force_degrees=23)
+    assert forced["degrees"] == 23 and not forced["transposed"]
+    assert forced["curve"].tolist() == [full["curve"][k]] and forced["curve_transposed"].tolist() == [
+        full["curve_transposed"][k]]
+    both = O.rotation_solve(mx, my, R, C, force_degrees=23, force_transpose=True)
+    assert both["transposed"] and both["degrees"] == 23
+    assert both["curve"].size == 0 and both["curve_transposed"].size == 0
+    only = O.rotation_solve(mx, my, R, C, force_transpose=False)
+    assert only["degrees"] == 23 and not only["transposed"] and only["curve_transposed"].size == 0
+    assert np.array_equal(only["curve"], full["curve"])
+    only_t = O.rotation_solve(mx, my, R, C, force_transpose=True)
+    assert only_t["transposed"] and only_t["curve"].size == 0
+    assert np.array_equal(only_t["curve_transposed"], full["curve_transposed"])
+    assert only_t["degrees"] == O.DEFAULT_ANGLES[int(np.argmin(full["curve_transposed"]))]
+    # the corrected field is the input rotated by deg * (pi / 180) (CuboPhaseTests.swift:127-137), components swapped
+    # first when transposed
+    for deg in (-89.0, -61.0, 0.0, 23.0, 37.0):
+        s = O.rotation_solve(mx, my, R, C, force_degrees=deg, force_transpose=False)
+        rad = deg * (np.pi / 180)
+        c, si = np.cos(rad), np.sin(rad)
+        assert np.array_equal(s["cx"], c * mx - si * my) and np.array_equal(s["cy"], si * mx + c * my), deg
+    s = O.rotation_solve(mx, my, R, C, force_degrees=10.0, force_transpose=True)
+    rad = 10.0 * (np.pi / 180)
+    assert np.array_equal(s["cx"], np.cos(rad) * my - np.sin(rad) * mx)
+
+
+def test_rotation_refuses_non_finite_angles():
+    mx, my, R, C = _planted(0)
+    for bad in (dict(angles=np.array([1.0, np.nan])), dict(angles=np.array([np.inf])), dict(force_degrees=np.nan),
+                dict(angles=np.zeros(0))):
+        with pytest.raises(O.OracleError):
+            O.rotation_solve(mx, my, R, C, **bad)
+    # a forced angle does not look at the search list
+    s = O.rotation_solve(mx, my, R, C, angles=np.array([np.nan]), force_degrees=5.0)
+    assert s["degrees"] == 5.0
+
+
+def _linear_field(R, C, a, b):
+    """x = a * row, y = b * column (flattened, row-major)."""
+    r, c = np.meshgrid(np.arange(R, dtype=np.float64), np.arange(C, dtype=np.float64), indexing="ij")
+    return (a * r).ravel(), (b * c).ravel()
+
+
+def test_divergence_and_curl_of_linear_fields():
+    """Closed forms: for x = a*row, y = b*column every interior central difference is exact, so
+    div = |2a + 2b| and curl = 0; for x = a*column, y = b*row, div = 0 and curl = |2b - 2a|."""
+    R, C = 6, 7
+    for a, b in ((1.0, 2.0), (0.5, -3.0), (-1.25, 0.25)):
+        x, y = _linear_field(R, C, a, b)
+        assert O.lib().oc_divergence(x, y, R, C) == abs(2 * a + 2 * b)
+        assert O.lib().oc_curl(x, y, R, C) == 0.0
+        # the other orientation of the same field: x = a*column, y = b*row
+        r, c = np.meshgrid(np.arange(R, dtype=np.float64), np.arange(C, dtype=np.float64), indexing="ij")
+        x2, y2 = (a * c).ravel(), (b * r).ravel()
+        assert O.lib().oc_divergence(x2, y2, R, C) == 0.0
+        assert O.lib().oc_curl(x2, y2, R, C) == abs(2 * b - 2 * a)
+    # a wrong sign or a swapped axis changes the value: x = row, y = -column has zero divergence
+    x, y = _linear_field(R, C, 1.0, -1.0)
+    assert O.lib().oc_divergence(x, y, R, C) == 0.0
+
+
+def _rotate(x, y, deg):
+    rad = deg * (np.pi / 180)
+    return np.cos(rad) * x - np.sin(rad) * y, np.sin(rad) * x + np.cos(rad) * y
+
+
+def divergence_field(R=12, C=15, a=1.0, b=2.0, theta0=31.0):
+    """The linear field (a*row, b*column) rotated by -theta0: rotating it back by theta gives the linear field
+    rotated by theta - theta0, whose divergence is 2|a + b||cos(theta - theta0)| (plain) and whose transposed
+    divergence is 2|b - a||sin(theta + theta0)|; the curl is 2|a + b||sin(theta - theta0)| and 2|b - a||cos(theta +
+    theta0)|."""
+    x, y = _linear_field(R, C, a, b)
+    return _rotate(x, y, -theta0)
+
+
+def test_divergence_criterion_on_a_rotated_linear_field():
+    R, C, a, b, t0 = 12, 15, 1.0, 2.0, 31.0
+    mx, my = divergence_field(R, C, a, b, t0)
+    ang = O.DEFAULT_ANGLES
+    d = O.rotation_solve(mx, my, R, C, maximize=True)
+    phi = np.deg2rad(ang - t0)
+    psi = np.deg2rad(ang + t0)
+    np.testing.assert_allclose(d["curve"], 2 * abs(a + b) * np.abs(np.cos(phi)), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(d["curve_transposed"], 2 * abs(b - a) * np.abs(np.sin(psi)), rtol=0, atol=1e-12)
+    assert d["degrees"] == t0 and not d["transposed"]
+    c = O.rotation_solve(mx, my, R, C)
+    np.testing.assert_allclose(c["curve"], 2 * abs(a + b) * np.abs(np.sin(phi)), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(c["curve_transposed"], 2 * abs(b - a) * np.abs(np.cos(psi)), rtol=0, atol=1e-12)
+    assert c["degrees"] == t0 and not c["transposed"]
+    # the corrected field is the linear field again
+    x, y = _linear_field(R, C, a, b)
+    assert np.max(np.abs(d["cx"] - x)) <= 1e-12 and np.max(np.abs(d["cy"] - y)) <= 1e-12
+    # transposed optimum: the components swapped before the rotation make (b*column, a*row) the field to find
+    tx, ty = _rotate(*_linear_field(R, C, a, b)[::-1], -t0)
+    dt = O.rotation_solve(tx, ty, R, C, maximize=True)
+    assert dt["transposed"] and dt["degrees"] == -t0
+
+
+def test_given_reference_and_tolerance_refusals():
+    """CuboPhaseTests.swift:95-96: .given(plane) subtracts the field itself, exactly zero; the tolerance must be
+    finite and not negative (CenterOfMassReference.swift:30-32)."""
+    p = com_pins()["plane"]
+    R, C = p["shape"]
+    x, y = doubles(p["x"]), doubles(p["y"])
+    fx, fy, nx, ny = O.normalize(x, y, R, C, "given", given=(x, y))
+    assert np.all(nx == 0) and np.all(ny == 0)
+    assert np.array_equal(fx, x) and np.array_equal(fy, y)
+    g = np.arange(R * C, dtype=np.float64)
+    xn = x.copy()
+    xn[3] = np.nan
+    xn[5] = np.inf
+    _, _, nx, ny = O.normalize(xn, y, R, C, "given", sampling=2.0, given=(g, -g))
+    want = (xn - g) * 2.0
+    want[3] = 0.0
+    want[5] = np.inf                       # the greatest finite value, times the sampling 2, overflows
+    assert np.array_equal(nx, want) and np.array_equal(ny, (y + g) * 2.0)
+    for tol in (-1e-12, np.nan, np.inf):
+        with pytest.raises(O.OracleError):
+            O.normalize(x, y, R, C, "plane", tol=tol)
+
+
+# --- the CPU-baseline pass ------------------------------------------------------------------------------
+def _first_processing_against_pinned(cube_u16, bf, adf, rows_per_tile, threads):
+    R, C, qy, qx = cube_u16.shape
+    got = O.cpu_first_processing_u16(cube_u16, qy, qx, C, bf, adf, threads=threads, rows_per_tile=rows_per_tile)
+    host = cube_u16.astype(np.float32)
+    assert_identical(got["vi_bf"], O.virtual_image(host, bf), "BF")
+    assert_identical(got["vi_adf"], O.virtual_image(host, adf), "ADF")
+    mean, mx = O.mean_max(host)
+    assert_identical(got["mean"], mean, "mean")
+    assert_identical(got["max"], mx, "max")
+    s = O.com_sums(host)
+    for k, want in zip(("intensity", "moment_row", "moment_column"), s):
+        assert_identical(got[k], want, k)
+    return got
+
+
+def test_cpu_first_processing_is_the_pinned_functions_tile_by_tile():
+    """The CPU baseline's pass (and the checker of the 2 GiB bit-exact test) gives the bits of the pinned
+    oc_virtual_image / oc_mean_max / oc_com_sums on uint16 cubes, whatever the tiling and thread count: scan rows
+    in tiles of 1, 2 and 4 rows of a 9-row scan (the last tile partial), a 192-pixel pattern (a partial 128-pixel
+    block of the mean), counts up to 65535."""
+    rng = np.random.default_rng(7)
+    cube = rng.integers(0, 65536, size=(9, 5, 12, 16), dtype=np.uint16)
+    cube[2, 3] = 65535
+    cube[4, 0] = 0
+    bf = O.mask("circle", 12, 16, centerRow=5.5, centerColumn=7.2, radius=4.1)
+    adf = O.mask("annulus", 12, 16, centerRow=5.5, centerColumn=7.2, inner=2.0, outer=6.5)
+    for rows_per_tile, threads in ((1, 1), (2, 3), (4, 8), (9, 2)):
+        _first_processing_against_pinned(cube, bf, adf, rows_per_tile, threads)
+
+
+def test_cpu_first_processing_meets_the_counts16_pins():
+    """counts16 (uint16, CuboKit's golden cube): py4DSTEM's own images of two 0/1 geometries, mean and max, bit for
+    bit, with 1-row tiles."""
+    c = next(c for c in _cubes() if c["name"] == "counts16")
+    cube = load_cube(c["file"])
+    qy, qx = c["shape"][2], c["shape"][3]
+    geo = {g["name"]: g for g in c["geometries"]}
+    g1, g2 = geo["bf_centre"], geo["annulus_fractional"]
+    m1 = O.mask(*geometry_kwargs(g1["geometry"])[:1], qy, qx, **geometry_kwargs(g1["geometry"])[1])
+    m2 = O.mask(*geometry_kwargs(g2["geometry"])[:1], qy, qx, **geometry_kwargs(g2["geometry"])[1])
+    got = _first_processing_against_pinned(cube, m1, m2, 1, 4)
+    assert_identical(got["vi_bf"], doubles(g1["image_py4dstem_native"]), "BF vs py4DSTEM")
+    assert_identical(got["vi_adf"], doubles(g2["image_py4dstem_native"]), "ADF vs py4DSTEM")
+    assert_identical(got["mean"], doubles(c["mean_py4dstem_native"]), "mean vs py4DSTEM")
+    assert_identical(got["max"], doubles(c["max"]), "max vs py4DSTEM")
