@@ -128,6 +128,7 @@ SIGNATURES = {
     "cubo_hip_reduce_view": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceView)]),
     "cubo_hip_reduce_release": (ct.c_int, [ct.c_void_p]),
     "cubo_hip_reduce": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ReduceResult)]),
+    "cubo_hip_reduce_noalloc": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ReduceResult)]),
     "cubo_hip_reduce_path": (ct.c_int, [ct.c_void_p, ct.POINTER(ReduceRequest), ct.POINTER(ct.c_int)]),
     "cubo_hip_cube_kernel_time": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),
     "cubo_hip_npy_read_header": (ct.c_int, [ct.c_char_p, ct.POINTER(NpyHeader)]),

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -187,22 +188,7 @@ cubo::RowsOut rows_out(const cubo_hip_reduction *r, int64_t pos0) {
     return o;
 }
 
-}  // namespace
-
-extern "C" {
-
-int cubo_hip_reduce_path(cubo_hip_cube *c, const cubo_hip_reduce_request *q, int *path) {
-    if (!c || !q || !path) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
-    cubo_hip_reduction tmp{};
-    tmp.cube = c;
-    if (int st = choose_path(c, q, &tmp)) return st;
-    *path = tmp.path;
-    return CUBO_HIP_OK;
-}
-
-int cubo_hip_reduce_begin(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_hip_reduction **out) {
-    if (!c || !q || !out) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
-    *out = nullptr;
+int check_request(const cubo_hip_cube *c, const cubo_hip_reduce_request *q) {
     if (q->n_masks < 0 || (q->n_masks > 0 && !q->masks))
         return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "masks: %d rows but no data", q->n_masks);
     if (q->n_masks == 0 && !q->want_mean_max && !q->want_com)
@@ -214,20 +200,21 @@ int cubo_hip_reduce_begin(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cu
         for (int64_t k = 0; k < ndet; ++k)
             if (!std::isfinite(q->com_weights[k]))
                 return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the centre-of-mass weights must be finite");
+    return CUBO_HIP_OK;
+}
 
-    auto *r = new (std::nothrow) cubo_hip_reduction();
-    if (!r) return fail(CUBO_HIP_ERR_OUT_OF_MEMORY, "host allocation");
+// The kernel family, weight rows, slices, block layout and the position groups of a whole-cube tile (sets
+// max_groups and res_count). Allocates nothing.
+int configure(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_hip_reduction *r) {
     r->cube = c;
-    if (int st = choose_path(c, q, r)) { delete r; return st; }
-    const int qx = (int)c->shape[3];
+    if (int st = choose_path(c, q, r)) return st;
+    const int64_t ndet = c->ndet;
     DeviceGuard g(c->device);
     if (r->path == 2) {
         const int nw = q->n_masks + (r->want_com ? 3 : 0);
         const int pnw = padded_f64_nw(std::max(nw, 1));
-        if (pnw < 0) {
-            delete r;
+        if (pnw < 0)
             return fail(CUBO_HIP_ERR_UNSUPPORTED, "%d weight rows in one pass (at most 16); split the masks", nw);
-        }
         r->nw = pnw;
         r->nv = pnw;
         r->slice_px = cubo::kF64Slice;
@@ -243,7 +230,247 @@ int cubo_hip_reduce_begin(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cu
     }
     // Blocks per compute unit of the launch plan: the kernel's occupancy, or a tuned multiple of it.
     if (const char *env = getenv("CUBO_HIP_BLOCKS_PER_CU")) r->blocks_per_cu = std::max(1, atoi(env));
+    int ngroups = 0, ppg = 0, ppw = 0;
+    plan(r, c->npos, &ngroups, &ppg, &ppw);
+    r->max_groups = ngroups;
+    r->res_count = (size_t)result_doubles(r);
+    return CUBO_HIP_OK;
+}
 
+// Bytes of the device scratch of a configured reduction: row partials (split layouts), column partials (sums,
+// maxima), running column sums and maxima.
+struct ScratchSizes {
+    size_t n[5];
+};
+ScratchSizes scratch_sizes(const cubo_hip_reduction *r) {
+    const cubo_hip_cube *c = r->cube;
+    const size_t ndet = (size_t)c->ndet, ng = (size_t)r->max_groups;
+    return ScratchSizes{{r->full ? 0 : (size_t)r->nslices * c->npos * std::max(r->nv, 1) * 8,
+                         ng * ndet * (r->path == 2 ? 8 : 4), ng * ndet * 4, ndet * 8, ndet * 4}};
+}
+
+// One tile's launches: the fused kernel, the row totals of the split layouts, the column totals. timed: bracket
+// the fused kernel with the cube's launch timer.
+int launch_tile(cubo_hip_reduction *r, int64_t row0, int64_t nrows, const void *tile, bool timed) {
+    cubo_hip_cube *c = r->cube;
+    const int64_t pos0 = row0 * c->shape[1];
+    const int64_t npos = nrows * c->shape[1];
+    const int ndet = (int)c->ndet;
+    int ngroups = 0, ppg = 0, ppw = 0;
+    plan(r, npos, &ngroups, &ppg, &ppw);
+    if (ngroups > r->max_groups) return fail(CUBO_HIP_ERR_HIP, "internal: group plan exceeds scratch");
+    if (timed) CUBO_HIP_TRY(c->timer.start(c->stream), "hipEventRecord");
+    hipError_t e;
+    if (r->path == 1) {
+        cubo::IntLaunch a{};
+        a.cube = (const uint16_t *)tile;
+        a.npos = npos;
+        a.ndet = ndet;
+        a.qx = (int)c->shape[3];
+        a.nslices = r->nslices;
+        a.ngroups = ngroups;
+        a.ppg = ppg;
+        a.ppw = ppw;
+        a.wpk = (const uint32_t *)r->weights;
+        a.rowpart = (uint64_t *)r->rowpart;
+        a.colsum = (uint32_t *)r->colsum_part;
+        a.colmax = (uint32_t *)r->colmax_part;
+        a.out = rows_out(r, pos0);
+        e = cubo::launch_int(a, r->nw, r->chunks, r->com_ones, r->full, c->stream);
+    } else {
+        cubo::F64Launch a{};
+        a.cube = tile;
+        a.npos = npos;
+        a.ndet = ndet;
+        a.nslices = r->nslices;
+        a.ngroups = ngroups;
+        a.ppg = ppg;
+        a.ppw = ppw;
+        a.w = (const double *)r->weights;
+        a.rowpart = (double *)r->rowpart;
+        a.colsum = (double *)r->colsum_part;
+        a.colmax = (float *)r->colmax_part;
+        e = cubo::launch_f64(a, c->dtype == CUBO_HIP_U16, r->nw, c->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "launching the fused reduction");
+    if (timed) CUBO_HIP_TRY(c->timer.stop(c->stream), "hipEventRecord");
+    // The integer kernel in the FULL layout writes its row totals itself.
+    if (r->nv > 0 && !r->full) {
+        e = r->path == 1 ? cubo::launch_finalize_rows_u64((const uint64_t *)r->rowpart, r->nslices, npos,
+                                                          rows_out(r, pos0), c->stream)
+                         : cubo::launch_finalize_rows_f64((const double *)r->rowpart, r->nslices, npos,
+                                                          rows_out(r, pos0), c->stream);
+        if (e != hipSuccess) return hip_fail(e, "launching the row totals");
+    }
+    if (r->want_cols) {
+        const bool first = r->rows_done == 0, finish = row0 + nrows == c->shape[0];
+        double *mean = r->res_hd + (size_t)std::max(r->nv, 1) * c->npos, *max = mean + c->ndet;
+        e = r->path == 1 ? cubo::launch_accumulate_cols_int((const uint32_t *)r->colsum_part,
+                                                            (const uint32_t *)r->colmax_part, ngroups, ndet,
+                                                            (uint64_t *)r->col_sum, (uint32_t *)r->col_max, first,
+                                                            finish, (double)c->npos, mean, max, c->stream)
+                         : cubo::launch_accumulate_cols_f64((const double *)r->colsum_part,
+                                                            (const float *)r->colmax_part, ngroups, ndet,
+                                                            (double *)r->col_sum, (float *)r->col_max, first, finish,
+                                                            (double)c->npos, mean, max, c->stream);
+        if (e != hipSuccess) return hip_fail(e, "launching the column totals");
+    }
+    return CUBO_HIP_OK;
+}
+
+// ---- the reduction without device allocation (cubo_hip_reduce_noalloc) ----
+//
+// Its weight rows, column and row partials, running column totals and results live in `scratch`, one
+// file-scope device array per device (held until the process ends). Host code never passes `scratch` itself
+// (its name is a host-side stand-in): the first call on a device looks up the array's device address with
+// hipGetSymbolAddress, with that device current, and keeps it. Calls on one device take turns under a mutex and
+// wait for their stream before returning, so no two of them use the array at once.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct NoAllocDevice {
+    std::mutex mu;
+    void *ptr = nullptr;    // the device address of `scratch` on this device
+};
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+// The weight value of (row, px) of a request: the masks, then (com_rows) w, w*i, w*j, then zero rows.
+template <typename T>
+__device__ __forceinline__ T weight_at(const float *masks, const float *cw, int n_masks, int64_t ndet, int qx,
+                                       int com_rows, int row, int64_t px) {
+    if (row < n_masks) return (T)masks[(int64_t)row * ndet + px];
+    const int j = row - n_masks;
+    if (!com_rows || j > 2) return (T)0;
+    const T w = cw ? (T)cw[px] : (T)1;
+    return j == 0 ? w : j == 1 ? w * (T)(px / qx) : w * (T)(px % qx);
+}
+
+// Integer kernel rows: uint16 pairs, the lower pixel in the low half (as the host packing of
+// cubo_hip_reduce_begin); rows x (ndet / 2) words.
+__global__ void pack_weights_int(const float *masks, const float *cw, int n_masks, int64_t ndet, int qx, int com_rows,
+                                 int rows, uint32_t *wpk) {
+    const int64_t pairs = ndet / 2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)rows * pairs) return;
+    const int row = (int)(i / pairs);
+    const int64_t px = 2 * (i % pairs);
+    const uint32_t lo = weight_at<uint32_t>(masks, cw, n_masks, ndet, qx, com_rows, row, px);
+    const uint32_t hi = weight_at<uint32_t>(masks, cw, n_masks, ndet, qx, com_rows, row, px + 1);
+    wpk[i] = (lo & 0xffffu) | ((hi & 0xffffu) << 16);
+}
+
+// FP64 kernel rows in Double: rows x ndet.
+__global__ void pack_weights_f64(const float *masks, const float *cw, int n_masks, int64_t ndet, int qx, int com_rows,
+                                 int rows, double *w) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)rows * ndet) return;
+    w[i] = weight_at<double>(masks, cw, n_masks, ndet, qx, com_rows, (int)(i / ndet), i % ndet);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cubo_hip_reduce_noalloc(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_hip_reduce_result *res) {
+    if (!c || !q || !res) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (int st = cubo::require_resident(c, "cubo_hip_reduce_noalloc")) return st;
+    if (c->device < 0 || c->device >= kNoAllocMaxDevices)
+        return fail(CUBO_HIP_ERR_UNSUPPORTED, "device %d (the allocation-free reduction serves devices 0..%d)",
+                    c->device, kNoAllocMaxDevices - 1);
+    if (int st = check_request(c, q)) return st;
+    cubo_hip_reduction r{};
+    if (int st = configure(c, q, &r)) return st;
+    const int64_t npos = c->npos, ndet = c->ndet;
+    // The scratch layout, every piece 256-byte aligned: masks and CoM weights as given (Float32), the kernel's
+    // weight rows, the partials, the results [rows][npos], mean, max, com row, com column.
+    const ScratchSizes z = scratch_sizes(&r);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    };
+    const int wrows = r.path == 1 ? std::max(r.nw, 1) : r.nw;
+    const size_t o_masks = take((size_t)q->n_masks * ndet * 4);
+    const size_t o_cw = take(q->com_weights ? (size_t)ndet * 4 : 0);
+    const size_t o_w = take(r.path == 1 ? (size_t)wrows * (ndet / 2) * 4 : (size_t)wrows * ndet * 8);
+    size_t o_part[5];
+    for (int i = 0; i < 5; ++i) o_part[i] = take(z.n[i]);
+    const size_t o_res = take((r.res_count + 2 * (size_t)npos) * 8);
+    if (off > kNoAllocScratchBytes)
+        return fail(CUBO_HIP_ERR_EXCEEDS_BUFFER_LIMIT,
+                    "this request needs %.1f MiB of device scratch; the allocation-free reduction holds %zu MiB "
+                    "(use cubo_hip_reduce)", off / 1048576.0, kNoAllocScratchBytes >> 20);
+    DeviceGuard g(c->device);
+    NoAllocDevice &d = g_noalloc[c->device];
+    std::lock_guard<std::mutex> lock(d.mu);
+    if (!d.ptr) CUBO_HIP_TRY(hipGetSymbolAddress(&d.ptr, HIP_SYMBOL(scratch)), "locating the device scratch");
+    char *base = (char *)d.ptr;
+    hipStream_t s = c->stream;
+    const float *dmasks = q->n_masks > 0 ? (const float *)(base + o_masks) : nullptr;
+    const float *dcw = q->com_weights ? (const float *)(base + o_cw) : nullptr;
+    if (q->n_masks > 0)
+        CUBO_HIP_TRY(hipMemcpyAsync(base + o_masks, q->masks, (size_t)q->n_masks * ndet * 4, hipMemcpyHostToDevice, s),
+                     "uploading the masks");
+    if (dcw) CUBO_HIP_TRY(hipMemcpyAsync(base + o_cw, q->com_weights, (size_t)ndet * 4, hipMemcpyHostToDevice, s),
+                          "uploading the centre-of-mass weights");
+    const int qx = (int)c->shape[3];
+    if (r.path == 1) {
+        const int64_t n = (int64_t)wrows * (ndet / 2);
+        hipLaunchKernelGGL(pack_weights_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dmasks, dcw,
+                           q->n_masks, ndet, qx, (int)(r.want_com && !r.com_ones), wrows, (uint32_t *)(base + o_w));
+    } else {
+        const int64_t n = (int64_t)wrows * ndet;
+        hipLaunchKernelGGL(pack_weights_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dmasks, dcw,
+                           q->n_masks, ndet, qx, (int)r.want_com, wrows, (double *)(base + o_w));
+    }
+    CUBO_HIP_TRY(hipGetLastError(), "packing the weight rows");
+    r.weights = base + o_w;
+    void **const bufs[] = {&r.rowpart, &r.colsum_part, &r.colmax_part, &r.col_sum, &r.col_max};
+    for (int i = 0; i < 5; ++i) *bufs[i] = z.n[i] ? base + o_part[i] : nullptr;
+    r.res_hd = (double *)(base + o_res);
+    if (int st = launch_tile(&r, 0, c->shape[0], c->data, false)) return st;
+    // The results into the caller's buffers, in the layout cubo_hip_reduce_view describes.
+    const double *rows = r.res_hd;
+    const double *mean = rows + (size_t)std::max(r.nv, 1) * npos, *max = mean + ndet;
+    const double *crow = r.res_hd + r.res_count, *ccol = crow + npos;
+    const int cb = (r.path == 1 && r.com_ones) ? r.nw : r.n_masks;
+    struct { double *dst; const double *src; int64_t n; } outs[] = {
+        {res->images, rows, (int64_t)r.n_masks * npos},
+        {r.want_cols ? res->mean : nullptr, mean, ndet},
+        {r.want_cols ? res->max : nullptr, max, ndet},
+        {r.want_com ? res->com_intensity : nullptr, rows + (size_t)cb * npos, npos},
+        {r.want_com ? res->com_moment_row : nullptr, rows + (size_t)(cb + 1) * npos, npos},
+        {r.want_com ? res->com_moment_column : nullptr, rows + (size_t)(cb + 2) * npos, npos},
+        {r.want_com ? res->com_row : nullptr, crow, npos},
+        {r.want_com ? res->com_column : nullptr, ccol, npos}};
+    for (auto &o : outs)
+        if (o.dst && o.n > 0)
+            CUBO_HIP_TRY(hipMemcpyAsync(o.dst, o.src, (size_t)o.n * 8, hipMemcpyDeviceToHost, s), "reading the results");
+    CUBO_HIP_TRY(hipStreamSynchronize(s), "waiting for the reduction");
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_reduce_path(cubo_hip_cube *c, const cubo_hip_reduce_request *q, int *path) {
+    if (!c || !q || !path) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    cubo_hip_reduction tmp{};
+    tmp.cube = c;
+    if (int st = choose_path(c, q, &tmp)) return st;
+    *path = tmp.path;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_reduce_begin(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_hip_reduction **out) {
+    if (!c || !q || !out) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (int st = check_request(c, q)) return st;
+    auto *r = new (std::nothrow) cubo_hip_reduction();
+    if (!r) return fail(CUBO_HIP_ERR_OUT_OF_MEMORY, "host allocation");
+    if (int st = configure(c, q, r)) { delete r; return st; }
+    const int64_t ndet = c->ndet;
+    const int qx = (int)c->shape[3];
+    DeviceGuard g(c->device);
     hipError_t e = hipSuccess;
     if (r->path == 1) {
         const int64_t pairs = ndet / 2;
@@ -282,18 +509,11 @@ int cubo_hip_reduce_begin(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cu
     if (e != hipSuccess) { free_reduction(r); return hip_fail(e, "uploading the weight rows"); }
 
     // Scratch for the largest tile (the whole cube).
-    int ngroups = 0, ppg = 0, ppw = 0;
-    plan(r, c->npos, &ngroups, &ppg, &ppw);
-    r->max_groups = ngroups;
-    r->res_count = (size_t)result_doubles(r);
-    struct { void **p; size_t n; } allocs[] = {
-        {&r->rowpart, r->full ? 0 : (size_t)r->nslices * c->npos * std::max(r->nv, 1) * 8},
-        {&r->colsum_part, (size_t)ngroups * ndet * (r->path == 2 ? 8 : 4)},
-        {&r->colmax_part, (size_t)ngroups * ndet * 4},
-        {&r->col_sum, (size_t)ndet * 8}, {&r->col_max, (size_t)ndet * 4}};
-    for (auto &a : allocs) {
-        if (a.n == 0) continue;
-        e = hipMalloc(a.p, a.n);
+    const ScratchSizes z = scratch_sizes(r);
+    void **const bufs[] = {&r->rowpart, &r->colsum_part, &r->colmax_part, &r->col_sum, &r->col_max};
+    for (int i = 0; i < 5; ++i) {
+        if (z.n[i] == 0) continue;
+        e = hipMalloc(bufs[i], z.n[i]);
         if (e != hipSuccess) { free_reduction(r); return hip_fail(e, "allocating reduction scratch"); }
     }
     // Results go straight from the kernels to mapped, coherent pinned memory: no copy after the pass.
@@ -315,70 +535,9 @@ int cubo_hip_reduce_rows(cubo_hip_reduction *r, int64_t row0, int64_t nrows) {
                     (long long)r->next_row, (long long)row0);
     if (nrows == 0) return CUBO_HIP_OK;
     DeviceGuard g(c->device);
-    const int64_t pos0 = row0 * c->shape[1];
-    const int64_t npos = nrows * c->shape[1];
-    const int ndet = (int)c->ndet;
     const void *tile = nullptr;   // the tile's first element: in the resident buffer or a streamed cube's ring
     if (int st = cubo::rows_data(c, row0, nrows, &tile)) return st;
-    int ngroups = 0, ppg = 0, ppw = 0;
-    plan(r, npos, &ngroups, &ppg, &ppw);
-    if (ngroups > r->max_groups) return fail(CUBO_HIP_ERR_HIP, "internal: group plan exceeds scratch");
-    CUBO_HIP_TRY(c->timer.start(c->stream), "hipEventRecord");
-    hipError_t e;
-    if (r->path == 1) {
-        cubo::IntLaunch a{};
-        a.cube = (const uint16_t *)tile;
-        a.npos = npos;
-        a.ndet = ndet;
-        a.qx = (int)c->shape[3];
-        a.nslices = r->nslices;
-        a.ngroups = ngroups;
-        a.ppg = ppg;
-        a.ppw = ppw;
-        a.wpk = (const uint32_t *)r->weights;
-        a.rowpart = (uint64_t *)r->rowpart;
-        a.colsum = (uint32_t *)r->colsum_part;
-        a.colmax = (uint32_t *)r->colmax_part;
-        a.out = rows_out(r, pos0);
-        e = cubo::launch_int(a, r->nw, r->chunks, r->com_ones, r->full, c->stream);
-    } else {
-        cubo::F64Launch a{};
-        a.cube = tile;
-        a.npos = npos;
-        a.ndet = ndet;
-        a.nslices = r->nslices;
-        a.ngroups = ngroups;
-        a.ppg = ppg;
-        a.ppw = ppw;
-        a.w = (const double *)r->weights;
-        a.rowpart = (double *)r->rowpart;
-        a.colsum = (double *)r->colsum_part;
-        a.colmax = (float *)r->colmax_part;
-        e = cubo::launch_f64(a, c->dtype == CUBO_HIP_U16, r->nw, c->stream);
-    }
-    if (e != hipSuccess) return hip_fail(e, "launching the fused reduction");
-    CUBO_HIP_TRY(c->timer.stop(c->stream), "hipEventRecord");
-    // The integer kernel in the FULL layout writes its row totals itself.
-    if (r->nv > 0 && !r->full) {
-        e = r->path == 1 ? cubo::launch_finalize_rows_u64((const uint64_t *)r->rowpart, r->nslices, npos,
-                                                          rows_out(r, pos0), c->stream)
-                         : cubo::launch_finalize_rows_f64((const double *)r->rowpart, r->nslices, npos,
-                                                          rows_out(r, pos0), c->stream);
-        if (e != hipSuccess) return hip_fail(e, "launching the row totals");
-    }
-    if (r->want_cols) {
-        const bool first = r->rows_done == 0, finish = row0 + nrows == c->shape[0];
-        double *mean = r->res_hd + (size_t)std::max(r->nv, 1) * c->npos, *max = mean + c->ndet;
-        e = r->path == 1 ? cubo::launch_accumulate_cols_int((const uint32_t *)r->colsum_part,
-                                                            (const uint32_t *)r->colmax_part, ngroups, ndet,
-                                                            (uint64_t *)r->col_sum, (uint32_t *)r->col_max, first,
-                                                            finish, (double)c->npos, mean, max, c->stream)
-                         : cubo::launch_accumulate_cols_f64((const double *)r->colsum_part,
-                                                            (const float *)r->colmax_part, ngroups, ndet,
-                                                            (double *)r->col_sum, (float *)r->col_max, first, finish,
-                                                            (double)c->npos, mean, max, c->stream);
-        if (e != hipSuccess) return hip_fail(e, "launching the column totals");
-    }
+    if (int st = launch_tile(r, row0, nrows, tile, true)) return st;
     r->next_row = row0 + nrows;
     r->rows_done += nrows;
     r->viewed = false;

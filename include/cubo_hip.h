@@ -193,6 +193,18 @@ int cubo_hip_reduce_release(cubo_hip_reduction *r);
 /* One call: begin, all rows as one tile, end. */
 int cubo_hip_reduce(cubo_hip_cube *cube, const cubo_hip_reduce_request *request, cubo_hip_reduce_result *result);
 
+/* cubo_hip_reduce without device allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same kernels and results as cubo_hip_reduce on a resident cube, all rows as one tile, with every scratch
+ * buffer (weight rows, partials, results before they reach `result`) in a static device array of 64 MiB per
+ * device, held from the library's first such call on a device until the process ends. A request that needs more
+ * is refused with CUBO_HIP_ERR_EXCEEDS_BUFFER_LIMIT before any device work (the message gives the size it needs);
+ * streamed cubes are refused (CUBO_HIP_ERR_UNSUPPORTED). Calls on one device share that array: they run one at a time (a lock held until
+ * the results are in `result`), so calls that overlap on one device wait for each other; the array keeps the last
+ * call's contents, and every call overwrites what it reads. Synchronous: returns once `result` is written. */
+int cubo_hip_reduce_noalloc(cubo_hip_cube *cube, const cubo_hip_reduce_request *request,
+                            cubo_hip_reduce_result *result);
+
 /* Which kernel family a request runs on: 1 = exact integer (uint16 cube, non-negative integral
  * weights), 2 = FP64 accumulation. Lets tests assert the path they exercise. */
 int cubo_hip_reduce_path(cubo_hip_cube *cube, const cubo_hip_reduce_request *request, int *path);
