@@ -276,6 +276,51 @@ def diff_section(C):
     return res
 
 
+def f64_section(C, cube16, bf, adf):
+    """The FP64 fused kernel (fused_f64), the path of every Float32 cube and of fractional masks: the
+    256x256x128x128 workload as Float32 (4 GiB, the same synthetic counts) with BF, ADF, mean, max and CoM, and the
+    uint16 cube with fractional masks (3/4 of the BF disk, the ADF annulus) and the same outputs. One launch per
+    pass; GB/s = cube bytes over the kernel's event-timed average."""
+    from cubokit_amd import _lib as L
+    from cubokit_amd import synthetic
+    shape, _, params = synthetic.bandwidth_u16()
+    f32 = C.HIPResidentCube(shape=C.CubeShape.of(shape), dtype=np.float32)
+    f32.fill_synthetic(params)
+    lib = L.lib()
+    out = {}
+    for name, cube, masks in (("f32_cube", f32, np.stack([bf, adf])), ("u16_fractional", cube16,
+                                                                       np.stack([0.75 * bf, adf]))):
+        masks = np.ascontiguousarray(masks, np.float32)
+        assert C.reduce_path(cube, masks, want_com=True) == 2
+        req = L.ReduceRequest(masks=masks.ctypes.data, n_masks=2, want_mean_max=1, want_com=1, com_weights=None)
+        h = ct.c_void_p()
+        view = L.ReduceView()
+        if lib.cubo_hip_reduce_begin(cube.handle, ct.byref(req), ct.byref(h)):
+            raise RuntimeError(L.last_error())
+        for _ in range(2):
+            lib.cubo_hip_reduce_reset(h)
+            lib.cubo_hip_reduce_rows(h, 0, cube.shape.scan_rows)
+        lib.cubo_hip_reduce_view(h, ct.byref(view))
+        cube.kernel_time()
+        n = 10
+        t0 = time.perf_counter()
+        for _ in range(n):
+            if lib.cubo_hip_reduce_reset(h) or lib.cubo_hip_reduce_rows(h, 0, cube.shape.scan_rows):
+                raise RuntimeError(L.last_error())
+        if lib.cubo_hip_reduce_view(h, ct.byref(view)):
+            raise RuntimeError(L.last_error())
+        el = (time.perf_counter() - t0) / n
+        kms, nl = cube.kernel_time()
+        lib.cubo_hip_reduce_release(h)
+        ms = kms / max(1, nl)
+        nb = cube.byte_count
+        out[name] = {"cube_bytes": nb, "kernel_ms": round(ms, 4), "kernel_gbs": round(nb / ms / 1e6, 1),
+                     "frac_of_hbm_peak": round(nb / ms / 1e6 / HBM_PEAK_GBS, 4), "pass_ms": round(el * 1e3, 4)}
+        log(f"fused_f64 {name}: kernel {ms:.3f} ms ({nb / ms / 1e6:.0f} GB/s), pass {el * 1e3:.3f} ms")
+    f32.release()
+    return out
+
+
 def pattern_section(C, cube):
     """Selected-area patterns on the bandwidth cube: a disk of half the scan positions (exact integer kernel), every
     position weighted (FP64 kernel), the disk moved by the rounded centre of mass, and by the Fourier shift (FP64
@@ -377,6 +422,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-cube batched-stack comparison")
     ap.add_argument("--no-diff", action="store_true", help="skip the 8 GiB differentiation workload")
     ap.add_argument("--no-pattern", action="store_true", help="skip the selected-area pattern timings")
+    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 fused-kernel timings")
     ap.add_argument("--no-ingest", action="store_true", help="skip the host-to-HBM ingestion timings")
     ap.add_argument("--no-origins", action="store_true", help="skip the origin timings")
     args = ap.parse_args()
@@ -416,22 +462,36 @@ def main() -> int:
     view = L.ReduceView()
 
     def step():
-        # One pass: reset, every row as one tile, and the results in (pinned) host memory.
-        if lib.cubo_hip_reduce_reset(h) or lib.cubo_hip_reduce_rows(h, 0, s.scan_rows) or \
-                lib.cubo_hip_reduce_view(h, ct.byref(view)):
+        # One pass: reset, every row as one tile: the fused kernel and the column totals, which write every result
+        # into (mapped, pinned) host memory. Passes are queued back to back on the cube's stream; the host waits
+        # once, after the last (a serving loop that reads a pass's results waits for it with cubo_hip_reduce_view).
+        if lib.cubo_hip_reduce_reset(h) or lib.cubo_hip_reduce_rows(h, 0, s.scan_rows):
+            raise RuntimeError(L.last_error())
+
+    def wait():
+        if lib.cubo_hip_reduce_view(h, ct.byref(view)):
             raise RuntimeError(L.last_error())
 
     for _ in range(args.warmup):
         step()
+    wait()
     cube.kernel_time()                                  # drop the warm-up launches
     log(f"warmup {args.warmup} done; timing {args.steps} steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()                                          # results() waits for the device: each step is complete
+        step()
+    wait()                                              # every pass complete, its results in host memory
     el = time.perf_counter() - t0
     kms, launches = cube.kernel_time()
     if lib.cubo_hip_reduce_results(h, ct.byref(res)):
         raise RuntimeError(L.last_error())
+    # For reference: the same passes with the host waiting for each one's results before queueing the next.
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        wait()
+    sync_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    cube.kernel_time()
     lib.cubo_hip_reduce_release(h)
     ms_per_step = el / args.steps * 1e3
     value = cube_bytes / (el / args.steps) / 1e9
@@ -473,6 +533,7 @@ def main() -> int:
         stack["mfma_gbs"] = round(cube_bytes / (stack["mfma_split_fp16_ms"] / 1e3) / 1e9, 1)
         stack["mfma_utilisation"] = mfma_utilisation(stack["mfma_split_fp16_ms"])
 
+    fused64 = None if args.no_f64 else f64_section(C, cube, bf, adf)
     patterns = None if args.no_pattern else pattern_section(C, cube)
     ingest = None if args.no_ingest else ingest_section(C, cube)
     origins = None if args.no_origins else origins_section(C, cube)
@@ -492,11 +553,14 @@ def main() -> int:
         "config": {"workload": "bandwidth_u16: 256x256x128x128 uint16 Poisson cube (2 GiB), one fused pass: "
                                "BF r=14 + ADF 20-60 images, mean, max, centre of mass",
                    "cube_bytes": cube_bytes, "pct_hbm_peak_whole_step": round(100 * value / HBM_PEAK_GBS, 2),
-                   "fused_kernel_ms": round(kernel_ms, 4), "kernel_path": "exact integer (fused_u16_int)"},
+                   "fused_kernel_ms": round(kernel_ms, 4), "kernel_path": "exact integer (fused_u16_int)",
+                   "steps_queued": "back to back on the cube's stream, one host wait after the last",
+                   "ms_per_step_waiting_each": round(sync_ms, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "mask_stack_k64": stack,
+        "fused_f64": fused64,
         "selected_area": patterns,
         "ingest": ingest,
         "origins": origins,

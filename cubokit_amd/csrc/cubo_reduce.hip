@@ -30,6 +30,8 @@ struct cubo_hip_reduction {
     int n_masks;
     bool want_cols, want_com;
     bool com_ones;          // integer path: arithmetic CoM rows
+    int com_kind;           // FP64 path: 0 no CoM, 1 unit weights, 2 weighted
+    bool f64_vec;           // FP64 path: the vectorised kernel where it applies (CUBO_HIP_F64_KERNEL=scalar: never)
     int nw;                 // weight rows in the kernel (padded)
     int nv;                 // row outputs per position
     int chunks;             // integer kernel: 8-pixel chunks per lane
@@ -203,6 +205,18 @@ int check_request(const cubo_hip_cube *c, const cubo_hip_reduce_request *q) {
     return CUBO_HIP_OK;
 }
 
+// The FP64 launch fields that select its kernel (the vectorised one or the one-pixel-per-lane one).
+cubo::F64Launch f64_probe(const cubo_hip_reduction *r) {
+    cubo::F64Launch a{};
+    a.cube = r->cube->data;
+    a.ndet = (int32_t)r->cube->ndet;
+    a.nmask = r->n_masks;
+    a.com = r->com_kind;
+    a.qx = (int32_t)r->cube->shape[3];
+    a.vec_ok = r->f64_vec;
+    return a;
+}
+
 // The kernel family, weight rows, slices, block layout and the position groups of a whole-cube tile (sets
 // max_groups and res_count). Allocates nothing.
 int configure(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_hip_reduction *r) {
@@ -217,8 +231,12 @@ int configure(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_hip_reduc
             return fail(CUBO_HIP_ERR_UNSUPPORTED, "%d weight rows in one pass (at most 16); split the masks", nw);
         r->nw = pnw;
         r->nv = pnw;
-        r->slice_px = cubo::kF64Slice;
-        r->blocks_per_cu = cubo::blocks_per_cu_f64(c->dtype == CUBO_HIP_U16, pnw);
+        r->com_kind = !r->want_com ? 0 : q->com_weights ? 2 : 1;
+        // CUBO_HIP_F64_KERNEL=scalar keeps the one-pixel-per-lane kernel (a test knob)
+        const char *k = getenv("CUBO_HIP_F64_KERNEL");
+        r->f64_vec = !(k && strcmp(k, "scalar") == 0);
+        r->slice_px = cubo::f64_slice_pixels(f64_probe(r), c->dtype == CUBO_HIP_U16);
+        r->blocks_per_cu = cubo::blocks_per_cu_f64(f64_probe(r), c->dtype == CUBO_HIP_U16, pnw);
     }
     r->nslices = (int)((ndet + r->slice_px - 1) / r->slice_px);
     if (r->path == 1) {
@@ -290,6 +308,10 @@ int launch_tile(cubo_hip_reduction *r, int64_t row0, int64_t nrows, const void *
         a.rowpart = (double *)r->rowpart;
         a.colsum = (double *)r->colsum_part;
         a.colmax = (float *)r->colmax_part;
+        a.nmask = r->n_masks;
+        a.com = r->com_kind;
+        a.qx = (int)c->shape[3];
+        a.vec_ok = r->f64_vec;
         e = cubo::launch_f64(a, c->dtype == CUBO_HIP_U16, r->nw, c->stream);
     }
     if (e != hipSuccess) return hip_fail(e, "launching the fused reduction");

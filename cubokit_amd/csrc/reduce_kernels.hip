@@ -471,6 +471,246 @@ __global__ __launch_bounds__(kBlock) void fused_f64(F64Launch a) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Vectorised FP64 kernel: uint16 or Float32 cube whose pattern length is a multiple of 16 bytes, at most
+// kVecMasks mask rows, optionally the centre-of-mass rows of unit weight. Slices of kVecSlice pixels; the same
+// position runs and the same outputs as fused_f64, with:
+//  - 16-byte loads: lane l of chunk c holds pixels s0 + (c*64 + l)*VEC ... + VEC-1 (VEC = 8 uint16 or 4 Float32),
+//    CH = 1 or 2 chunks per slice (8 pixels per lane), NB - 1 positions in flight in a register ring (as
+//    fused_u16_int);
+//  - the mask weights of the lane's pixels in registers as Double, loaded once;
+//  - the centre-of-mass rows arithmetic. uint16: exact integer sums per chunk (packed dot products), as in
+//    fused_u16_int. Float32: S and sum i*x from the chunk's sum (one detector row per chunk, so i is the chunk's
+//    row; i * sum gives the per-pixel products' special values too), sum j*x per pixel as (j0 + e) * x (a pixel's
+//    own column: 0 * inf is NaN only where j = 0, as CenterOfMass.swift:58-63 computes it);
+//  - uint16 column sums and maxima in integer registers (exact), widened when the block writes its partials.
+// TAIL: the pattern length is not a multiple of the slice; chunks past it load the row start and their values are
+// replaced by 0 where they are used (their column statistics are never written).
+// ------------------------------------------------------------------------------------------------
+constexpr int f64_rows(int nw) { return nw < 1 ? 1 : nw <= 6 ? nw : nw <= 8 ? 8 : nw <= 11 ? 11 : 16; }
+
+// An empty asm that reads and writes v: the update that produced v stays where it is written (LLVM would otherwise
+// sink the column-statistic updates of every position of an unrolled ring into the loop's last block, keeping all
+// of their data live across the row flushes).
+template <typename V>
+__device__ __forceinline__ void keep(V &v) { asm volatile("" : "+v"(v)); }
+
+template <typename T, int NWM, bool COM, bool TAIL>
+__global__ __launch_bounds__(kBlock) void fused_vec_f64(F64Launch a) {
+    constexpr bool U16 = sizeof(T) == 2;
+    constexpr int VEC = 16 / (int)sizeof(T);
+    constexpr int CH = kVecSlice / (kWave * VEC);
+    constexpr int NV = f64_rows(NWM + (COM ? 3 : 0));
+    constexpr int ROWS = 32;
+    constexpr int NB = 8 / CH + 1;
+    constexpr int CW = U16 ? VEC + VEC / 2 : 3 * VEC;           // dwords of column statistics per chunk and lane
+    constexpr int LDS_ROWS = kWavesPerBlock * ROWS * kLdsStride * 2;   // dwords
+    constexpr int LDS_COLS = (kWavesPerBlock - 1) * CH * CW * kWave;
+    __shared__ double lds[(LDS_ROWS > LDS_COLS ? LDS_ROWS : LDS_COLS) / 2];
+
+    const int lane = threadIdx.x & (kWave - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int slice = blockIdx.x % a.nslices, group = blockIdx.x / a.nslices;
+    const int s0 = slice * kVecSlice;
+
+    int px[CH], pxl[CH];
+    bool valid[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        px[c] = s0 + (c * kWave + lane) * VEC;
+        valid[c] = !TAIL || px[c] < a.ndet;
+        pxl[c] = valid[c] ? px[c] : 0;
+    }
+    double w[NWM > 0 ? NWM : 1][CH][VEC];
+#pragma unroll
+    for (int v = 0; v < NWM; ++v)
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) w[v][c][e] = valid[c] ? a.w[(size_t)v * a.ndet + px[c] + e] : 0.0;
+    double icd[CH], jcd[CH];
+    uint32_t ici[CH], jci[CH], vmask[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        ici[c] = COM ? (uint32_t)(pxl[c] / a.qx) : 0u;
+        jci[c] = COM ? (uint32_t)(pxl[c] % a.qx) : 0u;
+        icd[c] = (double)ici[c];
+        jcd[c] = (double)jci[c];
+        vmask[c] = valid[c] ? ~0u : 0u;
+    }
+
+    // column statistics: uint16 sums (uint32) and packed maxima; Float32 Double sums seeded at -0 and maxima
+    uint32_t usum[U16 ? CH : 1][VEC], umax[U16 ? CH : 1][VEC / 2];
+    double dsum[U16 ? 1 : CH][VEC];
+    float fmaxv[U16 ? 1 : CH][VEC];
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            if constexpr (U16) {
+                usum[c][e] = 0u;
+                if (e < VEC / 2) umax[c][e] = 0u;
+            } else {
+                dsum[c][e] = -0.0;
+                fmaxv[c][e] = -__builtin_inff();
+            }
+        }
+
+    RowBatch<double, NV, ROWS> rb{lds + wave * ROWS * kLdsStride};
+    double *rowslice = a.rowpart + (size_t)slice * a.npos * NV;
+
+    const int64_t gbeg = (int64_t)group * a.ppg;
+    const int64_t gend = min(a.npos, gbeg + a.ppg);
+    const int64_t pbeg = min(gend, gbeg + (int64_t)wave * a.ppw);
+    const int64_t pend = min(gend, pbeg + a.ppw);
+
+    auto load = [&](u32x4 (&d)[CH], int64_t p) {
+        const T *row = reinterpret_cast<const T *>(a.cube) + p * a.ndet;
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            d[c] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(row + pxl[c]));
+    };
+
+    int b = 0;
+    int64_t first = pbeg;
+    auto process = [&](const u32x4 (&d)[CH]) {
+        double acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const uint32_t raw[4] = {d[c].x & vmask[c], d[c].y & vmask[c], d[c].z & vmask[c], d[c].w & vmask[c]};
+            double x[VEC];
+            if constexpr (U16) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t lo = raw[k] & 0xffffu, hi = raw[k] >> 16;
+                    usum[c][2 * k] += lo;
+                    usum[c][2 * k + 1] += hi;
+                    umax[c][k] = pk_max_u16(umax[c][k], raw[k]);
+                    keep(usum[c][2 * k]);
+                    keep(usum[c][2 * k + 1]);
+                    keep(umax[c][k]);
+                    x[2 * k] = (double)lo;
+                    x[2 * k + 1] = (double)hi;
+                }
+                if constexpr (COM) {
+                    uint32_t cs = udot2(raw[0], 0x00010001u, 0u);
+                    cs = udot2(raw[1], 0x00010001u, cs);
+                    cs = udot2(raw[2], 0x00010001u, cs);
+                    cs = udot2(raw[3], 0x00010001u, cs);
+                    uint32_t jj = udot2(raw[0], 0x00010000u, 0u);
+                    jj = udot2(raw[1], 0x00030002u, jj);
+                    jj = udot2(raw[2], 0x00050004u, jj);
+                    jj = udot2(raw[3], 0x00070006u, jj);
+                    const double csd = (double)cs;
+                    acc[NWM] += csd;
+                    acc[NWM + 1] = fma(icd[c], csd, acc[NWM + 1]);
+                    acc[NWM + 2] += fma(jcd[c], csd, (double)jj);     // exact: integers below 2^53
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float xf = __builtin_bit_cast(float, raw[e]);
+                    x[e] = (double)xf;
+                    dsum[c][e] += x[e];
+                    fmaxv[c][e] = max_step(fmaxv[c][e], xf);
+                    keep(dsum[c][e]);
+                    keep(fmaxv[c][e]);
+                }
+                if constexpr (COM) {
+                    const double cs = (x[0] + x[1]) + (x[2] + x[3]);
+                    acc[NWM] += cs;
+                    acc[NWM + 1] = fma(icd[c], cs, acc[NWM + 1]);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[NWM + 2] = fma(jcd[c] + (double)e, x[e], acc[NWM + 2]);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < NWM; ++v)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) acc[v] = fma(x[e], w[v][c][e], acc[v]);
+        }
+        rb.put(b, acc, lane);
+        if (++b == rb.B) {
+            rb.flush(b, first, rowslice, lane);
+            first += b;
+            b = 0;
+        }
+    };
+
+    u32x4 buf[NB][CH];
+    int64_t p = pbeg;
+    if (pbeg < pend) {
+        const int64_t last = pend - 1;
+#pragma unroll
+        for (int j = 0; j < NB - 1; ++j) load(buf[j], min(p + j, last));
+        for (; p + NB <= pend; p += NB) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                load(buf[(j + NB - 1) % NB], min(p + j + NB - 1, last));
+                process(buf[j]);
+            }
+        }
+        const int64_t rest = pend - p;
+#pragma unroll
+        for (int j = 0; j < NB - 1; ++j)
+            if (j < rest) process(buf[j]);
+    }
+    if (b > 0) rb.flush(b, first, rowslice, lane);
+
+    // The four waves' column statistics (same pixels, consecutive position runs) combined in position order.
+    __syncthreads();
+    uint32_t *cl = reinterpret_cast<uint32_t *>(lds);
+    auto slot = [&](int ww, int c, int k) -> uint32_t & { return cl[((ww * CH + c) * CW + k) * kWave + lane]; };
+    if (wave > 0) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                if constexpr (U16) {
+                    slot(wave - 1, c, e) = usum[c][e];
+                    if (e < VEC / 2) slot(wave - 1, c, VEC + e) = umax[c][e];
+                } else {
+                    const uint64_t bits = __builtin_bit_cast(uint64_t, dsum[c][e]);
+                    slot(wave - 1, c, 2 * e) = (uint32_t)bits;
+                    slot(wave - 1, c, 2 * e + 1) = (uint32_t)(bits >> 32);
+                    slot(wave - 1, c, 2 * VEC + e) = __builtin_bit_cast(uint32_t, fmaxv[c][e]);
+                }
+            }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+#pragma unroll
+        for (int ww = 0; ww < kWavesPerBlock - 1; ++ww)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                if constexpr (U16) {
+                    usum[c][e] += slot(ww, c, e);
+                    if (e < VEC / 2) umax[c][e] = pk_max_u16(umax[c][e], slot(ww, c, VEC + e));
+                } else {
+                    const uint64_t bits = (uint64_t)slot(ww, c, 2 * e) | ((uint64_t)slot(ww, c, 2 * e + 1) << 32);
+                    dsum[c][e] += __builtin_bit_cast(double, bits);
+                    fmaxv[c][e] = max_step(fmaxv[c][e], __builtin_bit_cast(float, slot(ww, c, 2 * VEC + e)));
+                }
+            }
+        if (!valid[c]) continue;
+        double *cs = a.colsum + (size_t)group * a.ndet + px[c];
+        float *cm = a.colmax + (size_t)group * a.ndet + px[c];
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            if constexpr (U16) {
+                cs[e] = (double)usum[c][e];
+                cm[e] = (float)((umax[c][e / 2] >> (16 * (e & 1))) & 0xffffu);
+            } else {
+                cs[e] = dsum[c][e];
+                cm[e] = fmaxv[c][e];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
 // Second-stage kernels.
 // ------------------------------------------------------------------------------------------------
 template <typename Acc>
@@ -642,8 +882,38 @@ bool supported_f64(int nw) {
     return false;
 }
 
+// The vectorised kernel applies to a pattern of whole 16-byte pieces, at most kVecMasks masks, and no centre of
+// mass or one of unit weight whose detector rows are whole pieces.
+bool vec_f64_applies(const F64Launch &a, bool is_u16) {
+    const int vec = is_u16 ? 8 : 4;
+    return a.vec_ok && a.ndet % vec == 0 && a.nmask >= 0 && a.nmask <= kVecMasks && a.com >= 0 && a.com <= 1 &&
+           (!a.com || a.qx % vec == 0) && ((uintptr_t)a.cube & 15) == 0;
+}
+
+#define CUBO_VEC_CASES(X) X(0) X(1) X(2) X(3) X(4)
+
 hipError_t launch_f64(const F64Launch &a, bool is_u16, int nw, hipStream_t s) {
     const dim3 grid((unsigned)(a.nslices * a.ngroups)), block(kBlock);
+    if (vec_f64_applies(a, is_u16)) {
+        if (nw != f64_rows(a.nmask + 3 * a.com)) return hipErrorInvalidValue;
+        const bool tail = a.ndet % kVecSlice != 0;
+#define V(T, M, C, TL) hipLaunchKernelGGL((fused_vec_f64<T, M, C, TL>), grid, block, 0, s, a)
+#define X(M) \
+        if (a.nmask == M) { \
+            if (is_u16) { \
+                if (a.com) { if (tail) V(uint16_t, M, true, true); else V(uint16_t, M, true, false); } \
+                else { if (tail) V(uint16_t, M, false, true); else V(uint16_t, M, false, false); } \
+            } else { \
+                if (a.com) { if (tail) V(float, M, true, true); else V(float, M, true, false); } \
+                else { if (tail) V(float, M, false, true); else V(float, M, false, false); } \
+            } \
+            return hipGetLastError(); \
+        }
+        CUBO_VEC_CASES(X)
+#undef X
+#undef V
+        return hipErrorInvalidValue;
+    }
 #define X(NW) \
     if (nw == NW) { \
         if (is_u16) hipLaunchKernelGGL((fused_f64<uint16_t, NW>), grid, block, 0, s, a); \
@@ -655,18 +925,34 @@ hipError_t launch_f64(const F64Launch &a, bool is_u16, int nw, hipStream_t s) {
     return hipErrorInvalidValue;
 }
 
-int blocks_per_cu_f64(bool is_u16, int nw) {
+int blocks_per_cu_f64(const F64Launch &a, bool is_u16, int nw) {
     int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (vec_f64_applies(a, is_u16)) {
+        // the TAIL instantiations share the register use of the others
+#define X(M) \
+        if (a.nmask == M) \
+            e = is_u16 ? (a.com ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<uint16_t, M, true, false>, kBlock, 0) \
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<uint16_t, M, false, false>, kBlock, 0)) \
+                       : (a.com ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<float, M, true, false>, kBlock, 0) \
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<float, M, false, false>, kBlock, 0));
+        CUBO_VEC_CASES(X)
+#undef X
+        return e == hipSuccess && n > 0 ? n : 1;
+    }
 #define X(NW) \
     if (nw == NW) { \
-        hipError_t e = is_u16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_f64<uint16_t, NW>, kBlock, 0) \
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_f64<float, NW>, kBlock, 0); \
-        if (e != hipSuccess) n = 1; \
+        e = is_u16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_f64<uint16_t, NW>, kBlock, 0) \
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_f64<float, NW>, kBlock, 0); \
     }
     CUBO_F64_CASES(X)
 #undef X
-    return n;
+    return e == hipSuccess && n > 0 ? n : 1;
 }
+
+int f64_kernel(const F64Launch &a, bool is_u16) { return vec_f64_applies(a, is_u16) ? 2 : 1; }
+
+int f64_slice_pixels(const F64Launch &a, bool is_u16) { return vec_f64_applies(a, is_u16) ? kVecSlice : kF64Slice; }
 
 static unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 

@@ -60,6 +60,10 @@ struct F64Launch {
     double *rowpart;        // [nslices][npos][nw]
     double *colsum;         // [ngroups][ndet]
     float *colmax;          // [ngroups][ndet]
+    int32_t nmask;          // mask rows (the first rows of w)
+    int32_t com;            // 1: the three rows after the masks are the centre of mass of unit weight
+    int32_t qx;             // detector columns
+    int32_t vec_ok;         // 0: always the one-pixel-per-lane kernel (a test knob)
 };
 
 // Pixels per lane of the integer kernel (chunks of 8) and of the FP64 kernel (one pixel per chunk).
@@ -77,8 +81,17 @@ hipError_t launch_int(const IntLaunch &a, int nw, int chunks, bool com_ones, boo
 bool full_without_spills(int nw, int chunks, bool com_ones);
 // Resident blocks per compute unit of an instantiation (0 if it is not compiled).
 int blocks_per_cu_int(int nw, int chunks, bool com_ones, bool full, int nslices);
-int blocks_per_cu_f64(bool is_u16, int nw);
+// Most masks of the vectorised FP64 kernel (weights in registers); more masks, or weighted centre-of-mass rows, run on
+// the one-pixel-per-lane kernel.
+constexpr int kVecMasks = 4;
+constexpr int kVecSlice = kWave * 8;      // its slice: 8 pixels per lane
+// Pixels per slice of the kernel a launch would run.
+int f64_slice_pixels(const F64Launch &a, bool is_u16);
+// Resident blocks per compute unit of the kernel a launch would run (its cube pointer, ndet, nmask, com and qx set).
+int blocks_per_cu_f64(const F64Launch &a, bool is_u16, int nw);
 bool supported_f64(int nw);
+// 1: fused_f64 (one pixel per lane per chunk), 2: fused_vec_f64 (16-byte loads, weights in registers).
+int f64_kernel(const F64Launch &a, bool is_u16);
 hipError_t launch_f64(const F64Launch &a, bool is_u16, int nw, hipStream_t s);
 
 // Row totals over slices, one thread per position, into `out`.
