@@ -236,6 +236,8 @@ int configure(cubo_hip_cube *c, const cubo_hip_reduce_request *q, cubo_hip_reduc
         const char *k = getenv("CUBO_HIP_F64_KERNEL");
         r->f64_vec = !(k && strcmp(k, "scalar") == 0);
         r->slice_px = cubo::f64_slice_pixels(f64_probe(r), c->dtype == CUBO_HIP_U16);
+        // the vectorised kernel with one octet per pattern writes the row totals itself (one block per group)
+        r->full = cubo::f64_kernel(f64_probe(r), c->dtype == CUBO_HIP_U16) == 2 && ndet <= r->slice_px;
         r->blocks_per_cu = cubo::blocks_per_cu_f64(f64_probe(r), c->dtype == CUBO_HIP_U16, pnw);
     }
     r->nslices = (int)((ndet + r->slice_px - 1) / r->slice_px);
@@ -312,6 +314,7 @@ int launch_tile(cubo_hip_reduction *r, int64_t row0, int64_t nrows, const void *
         a.com = r->com_kind;
         a.qx = (int)c->shape[3];
         a.vec_ok = r->f64_vec;
+        a.out = rows_out(r, pos0);
         e = cubo::launch_f64(a, c->dtype == CUBO_HIP_U16, r->nw, c->stream);
     }
     if (e != hipSuccess) return hip_fail(e, "launching the fused reduction");

@@ -18,6 +18,7 @@
 // then sum groups (columns) and slices (rows) in a fixed order, so every result is deterministic.
 // (Device-scope atomics for the column partials measured slower: their cost grew with the block count.)
 #include "reduce_kernels.hpp"
+#include "device_sync.hpp"
 
 namespace cubo {
 
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(FULL ? kWave * kFullSlices : kBlock) void fused_u16
         const uint64_t t = rb.template total<uint64_t>(lane);
         const int r = lane % ROWS, v = r / rb.B, bb = r % rb.B;
         if (lane < ROWS && v < NV && bb < nb) part[(parity * kFullSlices + wave) * ROWS + bb * NV + v] = t;
-        __syncthreads();
+        lds_barrier();                                   // LDS only: the prefetched positions stay in flight
         if ((int)threadIdx.x < nb) {
             const int pb = threadIdx.x;
             const RowsOut &o = a.out;
@@ -472,44 +473,53 @@ __global__ __launch_bounds__(kBlock) void fused_f64(F64Launch a) {
 
 // ------------------------------------------------------------------------------------------------
 // Vectorised FP64 kernel: uint16 or Float32 cube whose pattern length is a multiple of 16 bytes, at most
-// kVecMasks mask rows, optionally the centre-of-mass rows of unit weight. Slices of kVecSlice pixels; the same
-// position runs and the same outputs as fused_f64, with:
+// kVecMasks mask rows, optionally the centre-of-mass rows of unit weight. The same outputs as fused_f64, with:
+//  - octets: a block of kVecWaves waves covers kVecWaves consecutive slices of kVecSlice pixels (one per wave)
+//    and all its waves walk the same positions of its group, so a compute unit reads whole spans of each
+//    pattern in order; each batch of row partials is added across the waves in LDS (one block barrier per
+//    batch). With one octet per pattern the block writes the row totals and the centre of mass itself (to
+//    mapped host memory, while the pass still streams); with several, one partial per octet goes to rowpart;
 //  - 16-byte loads: lane l of chunk c holds pixels s0 + (c*64 + l)*VEC ... + VEC-1 (VEC = 8 uint16 or 4 Float32),
-//    CH = 1 or 2 chunks per slice (8 pixels per lane), NB - 1 positions in flight in a register ring (as
-//    fused_u16_int);
+//    CH = 1 or 2 chunks per slice (8 pixels per lane), NB - 1 positions in flight in a register ring;
 //  - the mask weights of the lane's pixels in registers as Double, loaded once;
 //  - the centre-of-mass rows arithmetic. uint16: exact integer sums per chunk (packed dot products), as in
 //    fused_u16_int. Float32: S and sum i*x from the chunk's sum (one detector row per chunk, so i is the chunk's
 //    row; i * sum gives the per-pixel products' special values too), sum j*x per pixel as (j0 + e) * x (a pixel's
 //    own column: 0 * inf is NaN only where j = 0, as CenterOfMass.swift:58-63 computes it);
-//  - uint16 column sums and maxima in integer registers (exact), widened when the block writes its partials.
-// TAIL: the pattern length is not a multiple of the slice; chunks past it load the row start and their values are
+//  - column sums and maxima of each wave's pixels in registers (integers for uint16), written once per group.
+// TAIL: the pattern length is not a multiple of the octet; chunks past it load the row start and their values are
 // replaced by 0 where they are used (their column statistics are never written).
 // ------------------------------------------------------------------------------------------------
 constexpr int f64_rows(int nw) { return nw < 1 ? 1 : nw <= 6 ? nw : nw <= 8 ? 8 : nw <= 11 ? 11 : 16; }
 
 // An empty asm that reads and writes v: the update that produced v stays where it is written (LLVM would otherwise
 // sink the column-statistic updates of every position of an unrolled ring into the loop's last block, keeping all
-// of their data live across the row flushes).
+// of their data live across the row batches).
 template <typename V>
 __device__ __forceinline__ void keep(V &v) { asm volatile("" : "+v"(v)); }
 
+// Position buffers per wave (NB - 1 positions in flight) and LDS rows per wave of the row batches.
+#ifndef CUBO_VEC_NB
+#define CUBO_VEC_NB(CH) (8 / (CH) + 1)
+#endif
+#ifndef CUBO_VEC_ROWS
+#define CUBO_VEC_ROWS 32
+#endif
+
 template <typename T, int NWM, bool COM, bool TAIL>
-__global__ __launch_bounds__(kBlock) void fused_vec_f64(F64Launch a) {
+__global__ __launch_bounds__(kWave * kVecWaves) void fused_vec_f64(F64Launch a) {
     constexpr bool U16 = sizeof(T) == 2;
     constexpr int VEC = 16 / (int)sizeof(T);
     constexpr int CH = kVecSlice / (kWave * VEC);
     constexpr int NV = f64_rows(NWM + (COM ? 3 : 0));
-    constexpr int ROWS = 32;
-    constexpr int NB = 8 / CH + 1;
-    constexpr int CW = U16 ? VEC + VEC / 2 : 3 * VEC;           // dwords of column statistics per chunk and lane
-    constexpr int LDS_ROWS = kWavesPerBlock * ROWS * kLdsStride * 2;   // dwords
-    constexpr int LDS_COLS = (kWavesPerBlock - 1) * CH * CW * kWave;
-    __shared__ double lds[(LDS_ROWS > LDS_COLS ? LDS_ROWS : LDS_COLS) / 2];
+    constexpr int ROWS = CUBO_VEC_ROWS;
+    constexpr int NB = CUBO_VEC_NB(CH);
+    __shared__ double lds[kVecWaves * ROWS * kLdsStride];
+    __shared__ double part[2 * kVecWaves * ROWS];          // [batch parity][wave][b * NV + v]
 
     const int lane = threadIdx.x & (kWave - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int slice = blockIdx.x % a.nslices, group = blockIdx.x / a.nslices;
-    const int s0 = slice * kVecSlice;
+    const int octet = blockIdx.x % a.nslices, group = blockIdx.x / a.nslices;
+    const int s0 = (octet * kVecWaves + wave) * kVecSlice;
 
     int px[CH], pxl[CH];
     bool valid[CH];
@@ -527,13 +537,11 @@ __global__ __launch_bounds__(kBlock) void fused_vec_f64(F64Launch a) {
 #pragma unroll
             for (int e = 0; e < VEC; ++e) w[v][c][e] = valid[c] ? a.w[(size_t)v * a.ndet + px[c] + e] : 0.0;
     double icd[CH], jcd[CH];
-    uint32_t ici[CH], jci[CH], vmask[CH];
+    uint32_t vmask[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-        ici[c] = COM ? (uint32_t)(pxl[c] / a.qx) : 0u;
-        jci[c] = COM ? (uint32_t)(pxl[c] % a.qx) : 0u;
-        icd[c] = (double)ici[c];
-        jcd[c] = (double)jci[c];
+        icd[c] = COM ? (double)(pxl[c] / a.qx) : 0.0;
+        jcd[c] = COM ? (double)(pxl[c] % a.qx) : 0.0;
         vmask[c] = valid[c] ? ~0u : 0u;
     }
 
@@ -555,12 +563,44 @@ __global__ __launch_bounds__(kBlock) void fused_vec_f64(F64Launch a) {
         }
 
     RowBatch<double, NV, ROWS> rb{lds + wave * ROWS * kLdsStride};
-    double *rowslice = a.rowpart + (size_t)slice * a.npos * NV;
-
     const int64_t gbeg = (int64_t)group * a.ppg;
-    const int64_t gend = min(a.npos, gbeg + a.ppg);
-    const int64_t pbeg = min(gend, gbeg + (int64_t)wave * a.ppw);
-    const int64_t pend = min(gend, pbeg + a.ppw);
+    const int64_t pend = min(a.npos, gbeg + a.ppg);
+
+    // The wave totals of a batch of nb positions go to part[parity]; after the barrier the first nb threads add
+    // the octet's waves (slice order) for one position each and write its totals, or its octet partials. The
+    // parity alternates, so the next batch never overwrites a buffer that is still being read.
+    int parity = 0;
+    auto combine = [&](int nb, int64_t first_pos) {
+        const double t = rb.template total<double>(lane);
+        const int r = lane % ROWS, v = r / rb.B, bb = r % rb.B;
+        if (lane < ROWS && v < NV && bb < nb) part[(parity * kVecWaves + wave) * ROWS + bb * NV + v] = t;
+        lds_barrier();                                   // LDS only: the prefetched positions stay in flight
+        if ((int)threadIdx.x < nb) {
+            const int pb = threadIdx.x;
+            const int64_t pp = first_pos + pb;
+            double tot[NV];
+#pragma unroll
+            for (int vv = 0; vv < NV; ++vv) {
+                double sum = 0.0;
+#pragma unroll
+                for (int ww = 0; ww < kVecWaves; ++ww) sum += part[(parity * kVecWaves + ww) * ROWS + pb * NV + vv];
+                tot[vv] = sum;
+            }
+            if (a.nslices == 1) {
+                const RowsOut &o = a.out;
+#pragma unroll
+                for (int vv = 0; vv < NV; ++vv) o.rows[(int64_t)vv * o.total_pos + o.pos0 + pp] = tot[vv];
+                if constexpr (COM) {                                     // the CoM rows follow the masks
+                    o.com[o.pos0 + pp] = tot[NWM + 1] / tot[NWM];                 // NaN where S = 0
+                    o.com[o.total_pos + o.pos0 + pp] = tot[NWM + 2] / tot[NWM];
+                }
+            } else {
+#pragma unroll
+                for (int vv = 0; vv < NV; ++vv) a.rowpart[((int64_t)octet * a.npos + pp) * NV + vv] = tot[vv];
+            }
+        }
+        parity ^= 1;
+    };
 
     auto load = [&](u32x4 (&d)[CH], int64_t p) {
         const T *row = reinterpret_cast<const T *>(a.cube) + p * a.ndet;
@@ -570,7 +610,7 @@ __global__ __launch_bounds__(kBlock) void fused_vec_f64(F64Launch a) {
     };
 
     int b = 0;
-    int64_t first = pbeg;
+    int64_t first = gbeg;
     auto process = [&](const u32x4 (&d)[CH]) {
         double acc[NV];
 #pragma unroll
@@ -631,15 +671,15 @@ __global__ __launch_bounds__(kBlock) void fused_vec_f64(F64Launch a) {
         }
         rb.put(b, acc, lane);
         if (++b == rb.B) {
-            rb.flush(b, first, rowslice, lane);
+            combine(b, first);
             first += b;
             b = 0;
         }
     };
 
     u32x4 buf[NB][CH];
-    int64_t p = pbeg;
-    if (pbeg < pend) {
+    int64_t p = gbeg;
+    if (gbeg < pend) {
         const int64_t last = pend - 1;
 #pragma unroll
         for (int j = 0; j < NB - 1; ++j) load(buf[j], min(p + j, last));
@@ -655,45 +695,11 @@ __global__ __launch_bounds__(kBlock) void fused_vec_f64(F64Launch a) {
         for (int j = 0; j < NB - 1; ++j)
             if (j < rest) process(buf[j]);
     }
-    if (b > 0) rb.flush(b, first, rowslice, lane);
+    if (b > 0) combine(b, first);
 
-    // The four waves' column statistics (same pixels, consecutive position runs) combined in position order.
-    __syncthreads();
-    uint32_t *cl = reinterpret_cast<uint32_t *>(lds);
-    auto slot = [&](int ww, int c, int k) -> uint32_t & { return cl[((ww * CH + c) * CW + k) * kWave + lane]; };
-    if (wave > 0) {
-#pragma unroll
-        for (int c = 0; c < CH; ++c)
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-                if constexpr (U16) {
-                    slot(wave - 1, c, e) = usum[c][e];
-                    if (e < VEC / 2) slot(wave - 1, c, VEC + e) = umax[c][e];
-                } else {
-                    const uint64_t bits = __builtin_bit_cast(uint64_t, dsum[c][e]);
-                    slot(wave - 1, c, 2 * e) = (uint32_t)bits;
-                    slot(wave - 1, c, 2 * e + 1) = (uint32_t)(bits >> 32);
-                    slot(wave - 1, c, 2 * VEC + e) = __builtin_bit_cast(uint32_t, fmaxv[c][e]);
-                }
-            }
-    }
-    __syncthreads();
-    if (wave != 0) return;
+    // each wave's own pixels: no combining
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-#pragma unroll
-        for (int ww = 0; ww < kWavesPerBlock - 1; ++ww)
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-                if constexpr (U16) {
-                    usum[c][e] += slot(ww, c, e);
-                    if (e < VEC / 2) umax[c][e] = pk_max_u16(umax[c][e], slot(ww, c, VEC + e));
-                } else {
-                    const uint64_t bits = (uint64_t)slot(ww, c, 2 * e) | ((uint64_t)slot(ww, c, 2 * e + 1) << 32);
-                    dsum[c][e] += __builtin_bit_cast(double, bits);
-                    fmaxv[c][e] = max_step(fmaxv[c][e], __builtin_bit_cast(float, slot(ww, c, 2 * VEC + e)));
-                }
-            }
         if (!valid[c]) continue;
         double *cs = a.colsum + (size_t)group * a.ndet + px[c];
         float *cm = a.colmax + (size_t)group * a.ndet + px[c];
@@ -719,31 +725,72 @@ __global__ void finalize_rows(const Acc *rowpart, int nslices, int64_t npos, Row
                             min(npos, (int64_t)(blockIdx.x + 1) * blockDim.x), threadIdx.x, blockDim.x);
 }
 
-// 64 pixels x 4 group phases per block: each thread sums every fourth group of one pixel, LDS adds the four.
-__global__ __launch_bounds__(256) void accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups,
-                                                           int ndet, uint64_t *sum, uint32_t *max, bool first,
-                                                           bool finish, double count, double *mean_out,
-                                                           double *max_out) {
-    __shared__ uint64_t ps[4][64];
-    __shared__ uint32_t pm[4][64];
+// The same with the row count fixed: a position's NV partials of U slices are loaded before any is added, so a
+// thread has U * NV loads in flight; the adds keep slice order.
+template <typename Acc, int NV>
+__global__ __launch_bounds__(256) void finalize_rows_nv(const Acc *rowpart, int nslices, int64_t npos, RowsOut o) {
+    constexpr int U = NV <= 4 ? 8 : NV <= 8 ? 4 : 2;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npos) return;
+    Acc acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0;
+    int sl = 0;
+    for (; sl + U <= nslices; sl += U) {
+        Acc t[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) t[u][v] = rowpart[((int64_t)(sl + u) * npos + p) * NV + v];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] += t[u][v];
+    }
+    for (; sl < nslices; ++sl)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += rowpart[((int64_t)sl * npos + p) * NV + v];
+    double com3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const double d = (double)acc[v];
+        o.rows[(int64_t)v * o.total_pos + o.pos0 + p] = d;
+        const int k = v - o.com_base;
+        if (o.com_base >= 0 && k >= 0 && k < 3) com3[k] = d;
+    }
+    if (o.com_base >= 0) {
+        o.com[o.pos0 + p] = com3[1] / com3[0];                        // NaN where S = 0 (CenterOfMass.swift:44-45)
+        o.com[o.total_pos + o.pos0 + p] = com3[2] / com3[0];
+    }
+}
+
+// 64 pixels x 16 group phases per block: each thread sums every sixteenth group of one pixel (eight loads in
+// flight), LDS adds the sixteen. Integer sums: the order does not change them.
+constexpr int kColPhases = 16;
+__global__ __launch_bounds__(64 * kColPhases) void accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax,
+                                                                       int ngroups, int ndet, uint64_t *sum,
+                                                                       uint32_t *max, bool first, bool finish,
+                                                                       double count, double *mean_out,
+                                                                       double *max_out) {
+    __shared__ uint64_t ps[kColPhases][64];
+    __shared__ uint32_t pm[kColPhases][64];
     const int lx = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lx;
     uint64_t s = 0;
     uint32_t m = 0;
     if (i < ndet) {
-        // eight groups' loads in flight per thread before any of them is added
         int g = q;
-        for (; g + 28 < ngroups; g += 32) {
+        for (; g + 7 * kColPhases < ngroups; g += 8 * kColPhases) {
             uint32_t vs[8], vm[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                vs[u] = colsum[(size_t)(g + 4 * u) * ndet + i];
-                vm[u] = colmax[(size_t)(g + 4 * u) * ndet + i];
+                vs[u] = colsum[(size_t)(g + kColPhases * u) * ndet + i];
+                vm[u] = colmax[(size_t)(g + kColPhases * u) * ndet + i];
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) { s += vs[u]; m = vm[u] > m ? vm[u] : m; }
         }
-        for (; g < ngroups; g += 4) {
+        for (; g < ngroups; g += kColPhases) {
             s += colsum[(size_t)g * ndet + i];
             const uint32_t v = colmax[(size_t)g * ndet + i];
             m = v > m ? v : m;
@@ -753,9 +800,12 @@ __global__ __launch_bounds__(256) void accumulate_cols_int(const uint32_t *colsu
     pm[q][lx] = m;
     __syncthreads();
     if (q == 0 && i < ndet) {
-        s = ps[0][lx] + ps[1][lx] + ps[2][lx] + ps[3][lx];
+        s = 0;
         m = 0;
-        for (int k = 0; k < 4; ++k) m = pm[k][lx] > m ? pm[k][lx] : m;
+        for (int k = 0; k < kColPhases; ++k) {
+            s += ps[k][lx];
+            m = pm[k][lx] > m ? pm[k][lx] : m;
+        }
         if (!first) {
             s += sum[i];
             m = max[i] > m ? max[i] : m;
@@ -896,7 +946,8 @@ hipError_t launch_f64(const F64Launch &a, bool is_u16, int nw, hipStream_t s) {
     const dim3 grid((unsigned)(a.nslices * a.ngroups)), block(kBlock);
     if (vec_f64_applies(a, is_u16)) {
         if (nw != f64_rows(a.nmask + 3 * a.com)) return hipErrorInvalidValue;
-        const bool tail = a.ndet % kVecSlice != 0;
+        const bool tail = a.ndet % kVecOctet != 0;
+        const dim3 block(kWave * kVecWaves);
 #define V(T, M, C, TL) hipLaunchKernelGGL((fused_vec_f64<T, M, C, TL>), grid, block, 0, s, a)
 #define X(M) \
         if (a.nmask == M) { \
@@ -932,10 +983,10 @@ int blocks_per_cu_f64(const F64Launch &a, bool is_u16, int nw) {
         // the TAIL instantiations share the register use of the others
 #define X(M) \
         if (a.nmask == M) \
-            e = is_u16 ? (a.com ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<uint16_t, M, true, false>, kBlock, 0) \
-                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<uint16_t, M, false, false>, kBlock, 0)) \
-                       : (a.com ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<float, M, true, false>, kBlock, 0) \
-                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<float, M, false, false>, kBlock, 0));
+            e = is_u16 ? (a.com ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<uint16_t, M, true, false>, kWave * kVecWaves, 0) \
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<uint16_t, M, false, false>, kWave * kVecWaves, 0)) \
+                       : (a.com ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<float, M, true, false>, kWave * kVecWaves, 0) \
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_vec_f64<float, M, false, false>, kWave * kVecWaves, 0));
         CUBO_VEC_CASES(X)
 #undef X
         return e == hipSuccess && n > 0 ? n : 1;
@@ -952,29 +1003,42 @@ int blocks_per_cu_f64(const F64Launch &a, bool is_u16, int nw) {
 
 int f64_kernel(const F64Launch &a, bool is_u16) { return vec_f64_applies(a, is_u16) ? 2 : 1; }
 
-int f64_slice_pixels(const F64Launch &a, bool is_u16) { return vec_f64_applies(a, is_u16) ? kVecSlice : kF64Slice; }
+int f64_slice_pixels(const F64Launch &a, bool is_u16) { return vec_f64_applies(a, is_u16) ? kVecOctet : kF64Slice; }
 
 static unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// Row counts of the integer layouts (padded weight rows, plus three arithmetic CoM rows) and of the FP64 ones.
+#define CUBO_FINALIZE_NV(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(11) X(16) X(19)
+
+template <typename Acc>
+static hipError_t launch_finalize(const Acc *rowpart, int nslices, int64_t npos, const RowsOut &out, hipStream_t s) {
+    const dim3 grid(blocks_for(npos, 256)), block(256);
+#define X(NV) \
+    if (out.nv == NV) { \
+        hipLaunchKernelGGL((finalize_rows_nv<Acc, NV>), grid, block, 0, s, rowpart, nslices, npos, out); \
+        return hipGetLastError(); \
+    }
+    CUBO_FINALIZE_NV(X)
+#undef X
+    hipLaunchKernelGGL(finalize_rows<Acc>, grid, block, 0, s, rowpart, nslices, npos, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_finalize_rows_u64(const uint64_t *rowpart, int nslices, int64_t npos, const RowsOut &out,
                                     hipStream_t s) {
-    hipLaunchKernelGGL(finalize_rows<uint64_t>, dim3(blocks_for(npos, 256)), dim3(256), 0, s, rowpart, nslices, npos,
-                       out);
-    return hipGetLastError();
+    return launch_finalize<uint64_t>(rowpart, nslices, npos, out, s);
 }
 
 hipError_t launch_finalize_rows_f64(const double *rowpart, int nslices, int64_t npos, const RowsOut &out,
                                     hipStream_t s) {
-    hipLaunchKernelGGL(finalize_rows<double>, dim3(blocks_for(npos, 256)), dim3(256), 0, s, rowpart, nslices, npos,
-                       out);
-    return hipGetLastError();
+    return launch_finalize<double>(rowpart, nslices, npos, out, s);
 }
 
 hipError_t launch_accumulate_cols_int(const uint32_t *colsum, const uint32_t *colmax, int ngroups, int ndet,
                                       uint64_t *sum, uint32_t *max, bool first, bool finish, double count,
                                       double *mean_out, double *max_out, hipStream_t s) {
-    hipLaunchKernelGGL(accumulate_cols_int, dim3(blocks_for(ndet, 64)), dim3(256), 0, s, colsum, colmax, ngroups, ndet,
-                       sum, max, first, finish, count, mean_out, max_out);
+    hipLaunchKernelGGL(accumulate_cols_int, dim3(blocks_for(ndet, 64)), dim3(64 * kColPhases), 0, s, colsum, colmax,
+                       ngroups, ndet, sum, max, first, finish, count, mean_out, max_out);
     return hipGetLastError();
 }
 

@@ -64,6 +64,7 @@ struct F64Launch {
     int32_t com;            // 1: the three rows after the masks are the centre of mass of unit weight
     int32_t qx;             // detector columns
     int32_t vec_ok;         // 0: always the one-pixel-per-lane kernel (a test knob)
+    RowsOut out;            // the vectorised kernel with one octet (nslices = 1): row totals written by the kernel
 };
 
 // Pixels per lane of the integer kernel (chunks of 8) and of the FP64 kernel (one pixel per chunk).
@@ -84,8 +85,11 @@ int blocks_per_cu_int(int nw, int chunks, bool com_ones, bool full, int nslices)
 // Most masks of the vectorised FP64 kernel (weights in registers); more masks, or weighted centre-of-mass rows, run on
 // the one-pixel-per-lane kernel.
 constexpr int kVecMasks = 4;
-constexpr int kVecSlice = kWave * 8;      // its slice: 8 pixels per lane
-// Pixels per slice of the kernel a launch would run.
+constexpr int kVecSlice = kWave * 8;      // a wave's slice: 8 pixels per lane
+constexpr int kVecWaves = 8;              // waves per block: one octet of slices
+constexpr int kVecOctet = kVecWaves * kVecSlice;
+// Pixels per slice of the kernel a launch would run (for the vectorised kernel, per octet: nslices counts octets
+// and rowpart holds one partial per octet).
 int f64_slice_pixels(const F64Launch &a, bool is_u16);
 // Resident blocks per compute unit of the kernel a launch would run (its cube pointer, ndet, nmask, com and qx set).
 int blocks_per_cu_f64(const F64Launch &a, bool is_u16, int nw);
