@@ -116,6 +116,7 @@ SIGNATURES = {
     "cubo_hip_cube_create": (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int64), ct.c_int, ct.POINTER(ct.c_void_p)]),
     "cubo_hip_cube_upload_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p]),
     "cubo_hip_cube_read_rows": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p]),
+    "cubo_hip_cube_read_patterns": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_void_p]),
     "cubo_hip_cube_destroy": (ct.c_int, [ct.c_void_p]),
     "cubo_hip_cube_clone": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_void_p)]),
     "cubo_hip_cube_byte_count": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_uint64)]),

@@ -209,6 +209,23 @@ int cubo_hip_cube_read_rows(cubo_hip_cube *c, int64_t row0, int64_t nrows, void 
     return CUBO_HIP_OK;
 }
 
+int cubo_hip_cube_read_patterns(cubo_hip_cube *c, const int64_t *indices, int64_t n, void *host) {
+    if (!c || (n > 0 && (!indices || !host))) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n < 0) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "%lld patterns", (long long)n);
+    if (int st = cubo::require_resident(c, "reading patterns")) return st;
+    for (int64_t i = 0; i < n; ++i)                      // requirePosition (CubeShape.swift), before any copy
+        if (indices[i] < 0 || indices[i] >= c->npos)
+            return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "position %lld outside the scan's %lld positions",
+                        (long long)indices[i], (long long)c->npos);
+    DeviceGuard g(c->device);
+    const uint64_t pb = (uint64_t)c->ndet * element_size(c->dtype);
+    for (int64_t i = 0; i < n; ++i)
+        HIP_TRY(hipMemcpyAsync((char *)host + i * pb, (const char *)c->data + indices[i] * pb, pb,
+                               hipMemcpyDeviceToHost, c->stream), "reading patterns");
+    HIP_TRY(hipStreamSynchronize(c->stream), "reading patterns");
+    return CUBO_HIP_OK;
+}
+
 int cubo_hip_cube_kernel_time(cubo_hip_cube *c, double *total_ms, int64_t *launches) {
     if (!c || !total_ms || !launches) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     DeviceGuard g(c->device);

@@ -8,14 +8,16 @@
 from __future__ import annotations
 
 import ctypes as ct
+import math
 from dataclasses import dataclass
 from typing import Optional
 
 import numpy as np
 
 from . import _lib as L
-from .virtual import (CancellationError, ComputePath, CuboVirtualError, DetectorImage, HIPResidentCube, IsCancelled,
-                      OriginMap, Progress, ScanImage, ScanTiling, VirtualDiffraction, _check, _require_hip, _resident)
+from .virtual import (CancellationError, ComputePath, CubeShape, CuboError, CuboVirtualError, DetectorImage,
+                      HIPResidentCube, IsCancelled, OriginMap, Progress, ScanImage, ScanTiling, VirtualDiffraction,
+                      _check, _require_hip, _resident)
 
 
 @dataclass(frozen=True)
@@ -118,6 +120,57 @@ class OriginMeasurement:
         return _measure(source, req, tiling, progress, is_cancelled)
 
 
+def sensitivity_sample_indices(total: int, limit: int, pattern_bytes: int, budget_bytes: int) -> list:
+    """OriginMeasurement.sensitivitySampleIndices (IteratedWindowOrigin.swift:243-252): every n-th position, n the
+    smallest step that keeps the count at or under `limit` and, when pattern_bytes > 0, under
+    budget_bytes / pattern_bytes (at least one position)."""
+    if total <= 0 or limit <= 0:
+        return []
+    by_bytes = max(1, budget_bytes // pattern_bytes) if pattern_bytes > 0 else limit
+    bound = min(limit, by_bytes)
+    step = (total + bound - 1) // bound
+    return list(range(0, total, step))
+
+
+def window_sensitivity(source, radius: float, shipped_scale: float = 1.2, wide_scale: float = 2.5,
+                       sample_limit: int = 200, sample_budget_bytes: int = 256 << 20,
+                       path: ComputePath = ComputePath.hip) -> Optional[float]:
+    """OriginMeasurement.windowSensitivity (IteratedWindowOrigin.swift:254-290): the median, over a strided sample of
+    scan positions, of the distance between the iterated-window origins at radiusScale = shipped_scale and at
+    wide_scale (both with the estimator's defaults), in Float32 as the estimator computes it; None when the radius or
+    a scale is not positive and finite, or no distance is finite. The sampled patterns are gathered into a one-row
+    cube on the device and both measurements run there; the distances and their median are host arithmetic, as in
+    the reference."""
+    _require_hip(path)
+    resident = isinstance(source, HIPResidentCube)
+    arr = None if resident else np.asarray(source)
+    if arr is not None and arr.ndim != 4:
+        raise CuboError(f"invalid shape {list(arr.shape)}")
+    shape = source.shape if resident else CubeShape.of(arr.shape)
+    idx = sensitivity_sample_indices(shape.scan_count, sample_limit, shape.detector_count * 4, sample_budget_bytes)
+    if not idx or not all(math.isfinite(v) and v > 0 for v in (radius, shipped_scale, wide_scale)):
+        return None
+    # readPattern of each sampled position (on the device for a resident cube), gathered into a 1 x n scan
+    pats = source.read_patterns(idx) if resident else arr.reshape(shape.scan_count, shape.detector_rows,
+                                                                   shape.detector_columns)[idx]
+    sample = HIPResidentCube(np.ascontiguousarray(pats).reshape(1, len(idx), shape.detector_rows,
+                                                                shape.detector_columns))
+    try:
+        shipped = OriginMeasurement.measure_with(sample, radius, shipped_scale)
+        wide = OriginMeasurement.measure_with(sample, radius, wide_scale)
+    finally:
+        sample.release()
+    f = np.float32
+    dr = wide.row.astype(f) - shipped.row.astype(f)
+    dc = wide.column.astype(f) - shipped.column.astype(f)
+    d = np.sqrt(dr * dr + dc * dc, dtype=f)
+    d = np.sort(d[np.isfinite(d)])
+    if d.size == 0:
+        return None
+    m = d.size // 2
+    return float(d[m] if d.size % 2 else (d[m - 1] + d[m]) / f(2))
+
+
 def gaussian_filter(cube: HIPResidentCube, row: int, column: int, sigma: float) -> np.ndarray:
     """GaussianFilter(sigma).apply of one resident pattern, on the device."""
     out = np.empty(cube.shape.detector_count)
@@ -125,4 +178,7 @@ def gaussian_filter(cube: HIPResidentCube, row: int, column: int, sigma: float) 
     return out.reshape(cube.shape.detector_rows, cube.shape.detector_columns)
 
 
-__all__ = ["OriginEstimator", "OriginMeasurement", "ProbeEstimate", "ProbeSize", "gaussian_filter"]
+__all__ = ["OriginEstimator", "OriginMeasurement", "ProbeEstimate", "ProbeSize", "gaussian_filter",
+           "sensitivity_sample_indices", "window_sensitivity"]
+
+OriginMeasurement.window_sensitivity = staticmethod(window_sensitivity)

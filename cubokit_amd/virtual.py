@@ -432,7 +432,15 @@ class HIPResidentCube:
     def read_pattern(self, row: int, column: int) -> np.ndarray:
         if not (0 <= row < self.shape.scan_rows and 0 <= column < self.shape.scan_columns):
             raise CuboError(f"position ({row}, {column}) outside the scan")
-        return self.read_scan_tile(range(row, row + 1))[0, column].copy()
+        return self.read_patterns([row * self.shape.scan_columns + column])[0]
+
+    def read_patterns(self, indices) -> np.ndarray:
+        """readPattern for several scan positions (row * scanColumns + column): (n, qy, qx) in the cube's dtype."""
+        idx = np.ascontiguousarray(indices, np.int64).reshape(-1)
+        out = np.empty((idx.size, self.shape.detector_rows, self.shape.detector_columns), self.dtype)
+        _check(L.lib().cubo_hip_cube_read_patterns(self.handle, idx.ctypes.data, idx.size, out.ctypes.data),
+               "reading patterns")
+        return out
 
     def clone(self) -> "HIPResidentCube":
         """A second resident cube with the same contents, copied on the device."""

@@ -95,6 +95,10 @@ int cubo_hip_cube_create(int device, const int64_t shape[4], int dtype, cubo_hip
 int cubo_hip_cube_upload_rows(cubo_hip_cube *cube, int64_t row0, int64_t nrows, const void *host);
 /* readScanTile (ResidentCube.swift:113-117): copies rows out to host memory. */
 int cubo_hip_cube_read_rows(cubo_hip_cube *cube, int64_t row0, int64_t nrows, void *host);
+/* readPattern(row:column:) (ResidentCube.swift:111-116) for n scan positions (row * scanColumns + column) at once:
+ * n patterns in the cube's dtype, one after another, into host memory. Positions outside the scan are refused
+ * before any copy. */
+int cubo_hip_cube_read_patterns(cubo_hip_cube *cube, const int64_t *indices, int64_t n, void *host);
 int cubo_hip_cube_destroy(cubo_hip_cube *cube);
 /* init(loading:) from another resident cube: a new cube on the same device with the same shape, dtype and
  * contents, copied on the device by one streaming kernel (every byte read once and written once). */

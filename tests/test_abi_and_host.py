@@ -131,3 +131,16 @@ def test_phase_calls_refuse_before_any_device_call():
                                          ct.addressof(flag), ct.byref(deg), ct.byref(t), ct.byref(n0), ct.byref(n1),
                                          None, None, cx.ctypes.data, cy.ctypes.data)
     assert st == L.ERR_CANCELLED
+
+
+def test_window_sensitivity_sample_indices():
+    """OriginMeasurement.sensitivitySampleIndices (IteratedWindowOrigin.swift:243-252) by hand: 20 positions with a
+    limit of 7 take every third (IteratedWindowOriginTests.swift:77); a byte budget of five patterns takes every
+    twentieth of 100; a budget below one pattern still takes one; nothing to sample gives nothing."""
+    from cubokit_amd.origins import sensitivity_sample_indices as s
+    assert s(20, 7, 0, 0) == [0, 3, 6, 9, 12, 15, 18]
+    assert s(20, 7, 1600 * 4, 256 << 20) == [0, 3, 6, 9, 12, 15, 18]
+    assert s(100, 200, 1000, 5000) == [0, 20, 40, 60, 80]
+    assert s(100, 200, 1000, 10) == [0]
+    assert s(0, 7, 4, 100) == [] and s(5, 0, 4, 100) == []
+    assert s(65536, 200, 16384 * 4, 256 << 20) == list(range(0, 65536, 328))

@@ -147,3 +147,50 @@ def test_bandwidth_cube_origins_against_the_oracle_on_two_scan_rows():
     assert np.all(np.abs(iw.column[sl] - ic) <= 1e-5 * np.maximum(np.abs(ic), 1))
     # both estimators see the same beam: their maps agree to a fraction of a pixel
     assert np.median(np.abs(iw.row - omap.row)) < 0.5
+
+
+def _oracle_window_sensitivity(cube, radius, idx, shipped=1.2, wide=2.5):
+    """The same measurement on the oracle: the sampled patterns as a 1 x n scan, the Float32 iterated window at both
+    scales, the Float32 distances and their median."""
+    qy, qx = cube.shape[2:]
+    sample = np.ascontiguousarray(cube.reshape(-1, qy, qx)[idx]).reshape(1, len(idx), qy, qx)
+    r0, c0 = O.origin_iterated(sample, radius, shipped)
+    r1, c1 = O.origin_iterated(sample, radius, wide)
+    f = np.float32
+    dr, dc = r1.astype(f) - r0.astype(f), c1.astype(f) - c0.astype(f)
+    d = np.sort(np.sqrt(dr * dr + dc * dc, dtype=f))
+    m = d.size // 2
+    return float(d[m] if d.size % 2 else (d[m - 1] + d[m]) / f(2))
+
+
+def test_window_sensitivity():
+    """IteratedWindowOriginTests.swift:66-84: a compact probe reads near 0, a ringed one walks with the window (and
+    more than 20 times the compact one), equal scales give exactly 0, a radius of -1 gives no answer; every value
+    agrees with the oracle's Float32 estimate on the same sample within 1e-4 (the reference's bar between its CPU and
+    Metal paths), also with a sample limit of 7 (every third of 20 positions)."""
+    from cubokit_amd.origins import sensitivity_sample_indices
+    compact, _ = iterated_window_cube()
+    ringed, _ = iterated_window_cube(ring=True)
+    res_c, res_r = C.HIPResidentCube(compact), C.HIPResidentCube(ringed)
+    print("case: window sensitivity", flush=True)
+    small = C.OriginMeasurement.window_sensitivity(res_c, 6.2)
+    large = C.OriginMeasurement.window_sensitivity(res_r, 6.2)
+    assert small is not None and small < 0.1, small
+    assert large is not None and large > 0.2 and large > 20 * small, large
+    assert C.OriginMeasurement.window_sensitivity(res_r, 6.2, shipped_scale=1.2, wide_scale=1.2) == 0
+    assert C.OriginMeasurement.window_sensitivity(res_r, -1.0) is None
+    all20 = list(range(20))
+    for cube, res, val in ((compact, res_c, small), (ringed, res_r, large)):
+        want = _oracle_window_sensitivity(cube, 6.2, all20)
+        assert abs(val - want) <= 1e-4 * max(want, 1), (val, want)
+    seven = C.OriginMeasurement.window_sensitivity(res_r, 6.2, sample_limit=7)
+    idx = sensitivity_sample_indices(20, 7, 40 * 40 * 4, 256 << 20)
+    assert idx == [0, 3, 6, 9, 12, 15, 18]
+    want = _oracle_window_sensitivity(ringed, 6.2, idx)
+    assert abs(seven - want) <= 1e-4 * max(want, 1), (seven, want)
+    # a host array takes the same path (its sampled patterns gathered on the host)
+    assert abs(C.OriginMeasurement.window_sensitivity(ringed, 6.2) - large) <= 1e-6 * max(large, 1)
+    pats = res_r.read_patterns([0, 7, 19])
+    assert np.array_equal(pats, ringed.reshape(20, 40, 40)[[0, 7, 19]])
+    with pytest.raises(C.CuboVirtualError):
+        res_r.read_patterns([20])
