@@ -30,6 +30,7 @@ struct cubo_hip_dpc {
     double sr, sc;
     double *gx = nullptr, *gy = nullptr;
     double2 *tw_r = nullptr, *tw_c = nullptr;
+    cubo::FftPlan *plan_r = nullptr, *plan_c = nullptr;   // grids above kFftMax a side: the general-length plans
     double2 *z = nullptr, *f = nullptr, *tmp = nullptr;
     double *partial = nullptr, *sum = nullptr;
     double *phi = nullptr, *phi2 = nullptr;      // padded grid
@@ -55,6 +56,8 @@ void free_dpc(cubo_hip_dpc *d) {
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (double *b : d->chain) (void)hipFree(b);
+    cubo::fft_plan_destroy(d->plan_r);
+    cubo::fft_plan_destroy(d->plan_c);
     if (d->chain_sums) (void)hipFree(d->chain_sums);
     if (d->chain_g) (void)hipFree(d->chain_g);
     if (d->t0) (void)hipEventDestroy(d->t0);
@@ -103,11 +106,17 @@ hipError_t forward(cubo_hip_dpc *d, const double *phi, double *sum = nullptr) {
     return e;
 }
 
+// The 2-D FFT of the padded grid: the LDS transforms when both sides are at most kFftMax, else the plans.
+hipError_t fft2(cubo_hip_dpc *d, double2 *data, bool inverse) {
+    if (d->plan_r) return cubo::launch_fft2_planned(data, d->tmp, d->Rp, d->Cp, d->plan_r, d->plan_c, inverse, d->stream);
+    return cubo::launch_fft2(data, d->tmp, d->Rp, d->Cp, d->tw_r, d->tw_c, inverse, d->stream);
+}
+
 // phi_out = phi + step Re(IFFT(i gx FFT(dx) + i gy FFT(dy))), with z = dx + i dy from forward().
 hipError_t update(cubo_hip_dpc *d, const double *phi, double step, double *phi_out) {
-    hipError_t e = cubo::launch_fft2(d->z, d->tmp, d->Rp, d->Cp, d->tw_r, d->tw_c, false, d->stream);
+    hipError_t e = fft2(d, d->z, false);
     if (e == hipSuccess) e = cubo::launch_dpc_spectrum(grid(d), d->z, d->f, d->stream);
-    if (e == hipSuccess) e = cubo::launch_fft2(d->f, d->tmp, d->Rp, d->Cp, d->tw_r, d->tw_c, true, d->stream);
+    if (e == hipSuccess) e = fft2(d, d->f, true);
     if (e == hipSuccess) e = cubo::launch_dpc_update(grid(d), phi, d->f, step, phi_out, d->stream);
     return e;
 }
@@ -117,9 +126,9 @@ hipError_t vjp(cubo_hip_dpc *d, const double *phi, double step, const double *g_
                double msq, double *g_out) {
     const int64_t n = (int64_t)d->Rp * d->Cp;
     hipError_t e = cubo::launch_real_to_complex(g_in, d->z, n, d->stream);
-    if (e == hipSuccess) e = cubo::launch_fft2(d->z, d->tmp, d->Rp, d->Cp, d->tw_r, d->tw_c, false, d->stream);
+    if (e == hipSuccess) e = fft2(d, d->z, false);
     if (e == hipSuccess) e = cubo::launch_dpc_adjoint_spectrum(grid(d), d->z, d->f, d->stream);
-    if (e == hipSuccess) e = cubo::launch_fft2(d->f, d->tmp, d->Rp, d->Cp, d->tw_r, d->tw_c, true, d->stream);
+    if (e == hipSuccess) e = fft2(d, d->f, true);
     // error = msq / denom with denom = mean(fx^2 + fy^2): the denominator's share of the field gradient
     const double dd = -g_err * msq / (denom * denom);
     if (e == hipSuccess)
@@ -148,8 +157,8 @@ int cubo_hip_dpc_create(int device, int32_t rows, int32_t cols, double padding, 
         return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the sampling must be positive and finite");
     if (!(padding >= 1) || !std::isfinite(padding)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "padding below 1");
     const int Rp = (int)std::nearbyint((double)rows * padding), Cp = (int)std::nearbyint((double)cols * padding);
-    if (Rp > cubo::kFftMax || Cp > cubo::kFftMax)
-        return fail(CUBO_HIP_ERR_UNSUPPORTED, "padded grid %d x %d: at most %d per dimension", Rp, Cp, cubo::kFftMax);
+    if ((int64_t)Rp * Cp > ((int64_t)1 << 28))          // 4 GiB per complex grid buffer
+        return fail(CUBO_HIP_ERR_UNSUPPORTED, "padded grid %d x %d: at most 2^28 points", Rp, Cp);
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
         return fail(CUBO_HIP_ERR_NO_DEVICE, "no HIP device %d", device);
@@ -172,10 +181,11 @@ int cubo_hip_dpc_create(int device, int32_t rows, int32_t cols, double padding, 
             gx[(int64_t)r * Cp + c] = -0.25 * kx[r] * kden;
             gy[(int64_t)r * Cp + c] = -0.25 * ky[c] * kden;
         }
-    const std::vector<double2> twr = cubo::fft_twiddles(Rp), twc = cubo::fft_twiddles(Cp);
+    const bool small = Rp <= cubo::kFftMax && Cp <= cubo::kFftMax;
+    const std::vector<double2> twr = cubo::fft_twiddles(small ? Rp : 1), twc = cubo::fft_twiddles(small ? Cp : 1);
     hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     struct { void **p; size_t bytes; } a[] = {
-        {(void **)&d->gx, n * 8}, {(void **)&d->gy, n * 8}, {(void **)&d->tw_r, (size_t)Rp * 16}, {(void **)&d->tw_c, (size_t)Cp * 16},
+        {(void **)&d->gx, n * 8}, {(void **)&d->gy, n * 8}, {(void **)&d->tw_r, twr.size() * 16}, {(void **)&d->tw_c, twc.size() * 16},
         {(void **)&d->z, n * 16}, {(void **)&d->f, n * 16}, {(void **)&d->tmp, n * 16},
         {(void **)&d->partial, ((n + 255) / 256) * 8}, {(void **)&d->sum, 8}, {(void **)&d->phi, n * 8},
         {(void **)&d->phi2, n * 8}, {(void **)&d->fx, N * 8}, {(void **)&d->fy, N * 8}, {(void **)&d->g_fx, N * 8},
@@ -184,8 +194,11 @@ int cubo_hip_dpc_create(int device, int32_t rows, int32_t cols, double padding, 
         if (e == hipSuccess) e = hipMalloc(x.p, x.bytes);
     if (e == hipSuccess) e = hipMemcpy(d->gx, gx.data(), n * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d->gy, gy.data(), n * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d->tw_r, twr.data(), Rp * 16, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d->tw_c, twc.data(), Cp * 16, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d->tw_r, twr.data(), twr.size() * 16, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d->tw_c, twc.data(), twc.size() * 16, hipMemcpyHostToDevice);
+    // larger grids (DPC.swift:42-44 sets no limit): four-step and Bluestein plans of each side
+    if (e == hipSuccess && !small) e = cubo::fft_plan_create(Rp, Cp, d->stream, &d->plan_r);
+    if (e == hipSuccess && !small) e = cubo::fft_plan_create(Cp, Rp, d->stream, &d->plan_c);
     if (e == hipSuccess) e = hipEventCreate(&d->t0);
     if (e == hipSuccess) e = hipEventCreate(&d->t1);
     if (e != hipSuccess) { free_dpc(d); return hip_fail(e, "preparing DPC"); }

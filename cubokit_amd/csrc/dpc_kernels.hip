@@ -7,6 +7,9 @@
 // Twiddles exp(-2 pi i m / n) come from the host in Double.
 #include "dpc_kernels.hpp"
 
+#include <algorithm>
+#include <new>
+
 namespace cubo {
 namespace {
 
@@ -246,7 +249,185 @@ __global__ void embed(const double *scan, int R, int C, int Rp, int Cp, double *
 
 unsigned blocks256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+__device__ __forceinline__ double2 conj_if(double2 w, bool c) { return c ? make_double2(w.x, -w.y) : w; }
+
+// t[b][j2][k1] *= W_n^(j2 k1) (conjugated for the inverse); j2 k1 < n1 n2 = n, so no reduction is needed.
+__global__ __launch_bounds__(256) void fourstep_twiddle(double2 *t, int64_t total, int n1, int n2, const double2 *twn,
+                                                       bool inv) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const int k1 = (int)(i % n1), j2 = (int)((i / n1) % n2);
+    t[i] = cmul(t[i], conj_if(twn[(int64_t)j2 * k1], inv));
+}
+
+// a[b][j] = x[b][j] c_j for j < n, 0 up to M (c conjugated for the inverse)
+__global__ __launch_bounds__(256) void bluestein_pre(const double2 *x, double2 *a, int64_t total, int n, int M,
+                                                    const double2 *chirp, bool inv) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const int64_t b = i / M;
+    const int j = (int)(i % M);
+    a[i] = j < n ? cmul(x[b * n + j], conj_if(chirp[j], inv)) : make_double2(0.0, 0.0);
+}
+
+__global__ __launch_bounds__(256) void bluestein_mul(double2 *a, int64_t total, int M, const double2 *bf) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < total) a[i] = cmul(a[i], bf[i % M]);
+}
+
+// x[b][k] = c[b][k] / M * c_k for k < n
+__global__ __launch_bounds__(256) void bluestein_post(const double2 *c, double2 *x, int64_t total, int n, int M,
+                                                     const double2 *chirp, bool inv) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const int64_t b = i / n;
+    const int k = (int)(i % n);
+    const double2 v = c[b * M + k];
+    x[i] = cmul(make_double2(v.x / M, v.y / M), conj_if(chirp[k], inv));
+}
+
 }  // namespace
+
+struct FftPlan {
+    int n = 0, kind = 0;          // 0: fft_rows, 1: four-step, 2: Bluestein
+    int max_batch = 0;
+    double2 *tw = nullptr;        // kind 0: exp(-2 pi i m / n); kind 1: W_n^m, m < n
+    int n1 = 0, n2 = 0, M = 0;
+    FftPlan *p1 = nullptr, *p2 = nullptr;   // four-step: lengths n1, n2; Bluestein: length M
+    double2 *scratch = nullptr;   // four-step: max_batch * n; Bluestein: max_batch * M
+    double2 *chirp = nullptr, *bf = nullptr, *bfi = nullptr;
+};
+
+void fft_plan_destroy(FftPlan *p) {
+    if (!p) return;
+    fft_plan_destroy(p->p1);
+    fft_plan_destroy(p->p2);
+    for (double2 *b : {p->tw, p->scratch, p->chirp, p->bf, p->bfi})
+        if (b) (void)hipFree(b);
+    delete p;
+}
+
+static hipError_t upload(const std::vector<double2> &v, double2 **dst) {
+    hipError_t e = hipMalloc((void **)dst, v.size() * sizeof(double2));
+    if (e == hipSuccess) e = hipMemcpy(*dst, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice);
+    return e;
+}
+
+hipError_t fft_plan_create(int n, int max_batch, hipStream_t s, FftPlan **out) {
+    *out = nullptr;
+    if (n < 1 || max_batch < 1) return hipErrorInvalidValue;
+    FftPlan *p = new (std::nothrow) FftPlan();
+    if (!p) return hipErrorOutOfMemory;
+    p->n = n;
+    p->max_batch = max_batch;
+    const bool pow2 = (n & (n - 1)) == 0;
+    hipError_t e = hipSuccess;
+    if (n <= kFftMax) {
+        p->kind = 0;
+        e = upload(fft_twiddles(n), &p->tw);
+    } else if (pow2) {
+        p->kind = 1;
+        int k = 0;
+        while ((1 << k) < n) ++k;
+        p->n1 = 1 << (k / 2);
+        p->n2 = n / p->n1;
+        if (p->n2 > kFftMax) e = hipErrorInvalidValue;                        // n above 2^22
+        if (e == hipSuccess) e = upload(fft_twiddles(n), &p->tw);
+        if (e == hipSuccess) e = fft_plan_create(p->n1, max_batch * p->n2, s, &p->p1);
+        if (e == hipSuccess) e = fft_plan_create(p->n2, max_batch * p->n1, s, &p->p2);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->scratch, (size_t)max_batch * n * sizeof(double2));
+    } else {
+        p->kind = 2;
+        int M = 1;
+        while (M < 2 * n - 1) M <<= 1;
+        p->M = M;
+        std::vector<double2> chirp(n), b(M, make_double2(0.0, 0.0)), bc(M, make_double2(0.0, 0.0));
+        for (int j = 0; j < n; ++j) {
+            const long long q = ((long long)j * j) % (2LL * n);
+            const double ang = -M_PI * (double)q / (double)n;
+            chirp[j] = make_double2(std::cos(ang), std::sin(ang));
+            const double2 cj = make_double2(chirp[j].x, -chirp[j].y);         // conj(c_j): the kernel at +-j
+            b[j] = cj;
+            bc[j] = chirp[j];
+            if (j > 0) { b[M - j] = cj; bc[M - j] = chirp[j]; }
+        }
+        e = upload(chirp, &p->chirp);
+        if (e == hipSuccess) e = upload(b, &p->bf);
+        if (e == hipSuccess) e = upload(bc, &p->bfi);
+        if (e == hipSuccess) e = fft_plan_create(M, std::max(max_batch, 1), s, &p->p1);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->scratch, (size_t)max_batch * M * sizeof(double2));
+        // the kernel's transforms, once (forward for both directions: the inverse uses the conjugate kernel)
+        if (e == hipSuccess) e = launch_fft_rows(p->p1, p->bf, 1, false, s);
+        if (e == hipSuccess) e = launch_fft_rows(p->p1, p->bfi, 1, false, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (e != hipSuccess) {
+        fft_plan_destroy(p);
+        return e;
+    }
+    *out = p;
+    return hipSuccess;
+}
+
+hipError_t launch_fft_rows(FftPlan *p, double2 *data, int batch, bool inverse, hipStream_t s) {
+    if (!p || batch < 1 || batch > p->max_batch) return hipErrorInvalidValue;
+    const int n = p->n;
+    if (p->kind == 0) {
+        const size_t lds = 2 * (size_t)n * sizeof(double2);
+        if (inverse) hipLaunchKernelGGL(fft_rows<true>, dim3((unsigned)batch), dim3(256), lds, s, data, n, p->tw);
+        else hipLaunchKernelGGL(fft_rows<false>, dim3((unsigned)batch), dim3(256), lds, s, data, n, p->tw);
+        return hipGetLastError();
+    }
+    const int64_t total = (int64_t)batch * n;
+    if (p->kind == 1) {
+        const int n1 = p->n1, n2 = p->n2;
+        double2 *t = p->scratch;
+        for (int b0 = 0; b0 < batch; b0 += 65535) {                          // grid z of the transposes
+            const int nb = std::min(65535, batch - b0);
+            hipLaunchKernelGGL(transpose, dim3((n2 + 31) / 32, (n1 + 31) / 32, nb), dim3(256), 0, s,
+                               data + (int64_t)b0 * n, t + (int64_t)b0 * n, n1, n2, 1.0);
+        }
+        if (hipError_t e = launch_fft_rows(p->p1, t, batch * n2, inverse, s)) return e;
+        hipLaunchKernelGGL(fourstep_twiddle, dim3(blocks256(total)), dim3(256), 0, s, t, total, n1, n2, p->tw, inverse);
+        for (int b0 = 0; b0 < batch; b0 += 65535) {
+            const int nb = std::min(65535, batch - b0);
+            hipLaunchKernelGGL(transpose, dim3((n1 + 31) / 32, (n2 + 31) / 32, nb), dim3(256), 0, s,
+                               t + (int64_t)b0 * n, data + (int64_t)b0 * n, n2, n1, 1.0);
+        }
+        if (hipError_t e = launch_fft_rows(p->p2, data, batch * n1, inverse, s)) return e;
+        for (int b0 = 0; b0 < batch; b0 += 65535) {
+            const int nb = std::min(65535, batch - b0);
+            hipLaunchKernelGGL(transpose, dim3((n2 + 31) / 32, (n1 + 31) / 32, nb), dim3(256), 0, s,
+                               data + (int64_t)b0 * n, t + (int64_t)b0 * n, n1, n2, 1.0);
+        }
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(data, t, (size_t)total * sizeof(double2), hipMemcpyDeviceToDevice, s);
+        return e;
+    }
+    const int M = p->M;
+    const int64_t tm = (int64_t)batch * M;
+    double2 *a = p->scratch;
+    hipLaunchKernelGGL(bluestein_pre, dim3(blocks256(tm)), dim3(256), 0, s, data, a, tm, n, M, p->chirp, inverse);
+    if (hipError_t e = launch_fft_rows(p->p1, a, batch, false, s)) return e;
+    hipLaunchKernelGGL(bluestein_mul, dim3(blocks256(tm)), dim3(256), 0, s, a, tm, M, inverse ? p->bfi : p->bf);
+    if (hipError_t e = launch_fft_rows(p->p1, a, batch, true, s)) return e;
+    hipLaunchKernelGGL(bluestein_post, dim3(blocks256(total)), dim3(256), 0, s, a, data, total, n, M, p->chirp, inverse);
+    return hipGetLastError();
+}
+
+hipError_t launch_fft2_planned(double2 *data, double2 *tmp, int rows, int cols, FftPlan *pr, FftPlan *pc,
+                               bool inverse, hipStream_t s) {
+    if (!pr || !pc || pr->n != rows || pc->n != cols) return hipErrorInvalidValue;
+    if (hipError_t e = launch_fft_rows(pc, data, rows, inverse, s)) return e;
+    hipLaunchKernelGGL(transpose, dim3((cols + 31) / 32, (rows + 31) / 32, 1), dim3(256), 0, s, data, tmp, rows, cols,
+                       1.0);
+    if (hipError_t e = launch_fft_rows(pr, tmp, cols, inverse, s)) return e;
+    const double scale = inverse ? 1.0 / ((double)rows * cols) : 1.0;
+    hipLaunchKernelGGL(transpose, dim3((rows + 31) / 32, (cols + 31) / 32, 1), dim3(256), 0, s, tmp, data, cols, rows,
+                       scale);
+    return hipGetLastError();
+}
 
 hipError_t launch_fft2_batched(double2 *data, double2 *tmp, int batch, int rows, int cols, const double2 *tw_r,
                                const double2 *tw_c, bool inverse, hipStream_t s) {

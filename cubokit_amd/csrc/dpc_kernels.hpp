@@ -32,6 +32,22 @@ inline std::vector<double2> fft_twiddles(int n) {
 hipError_t launch_fft2_batched(double2 *data, double2 *tmp, int batch, int rows, int cols, const double2 *tw_r,
                                const double2 *tw_c, bool inverse, hipStream_t s);
 
+// A 1-D FFT of any length n >= 1 for batches of up to max_batch contiguous rows, unnormalised in both directions:
+//  - n <= kFftMax: fft_rows (Stockham stages in LDS for powers of two, the direct transform from the twiddle table
+//    otherwise);
+//  - larger powers of two: four steps, n = n1 n2 with n1, n2 <= kFftMax powers of two: a transpose of each row seen
+//    as n1 x n2, FFTs of length n1, the twiddles W_n^(j2 k1), a transpose, FFTs of length n2, a transpose;
+//  - other lengths: Bluestein's chirp convolution through a power-of-two plan of length M >= 2n - 1 (the chirps
+//    exp(-+i pi j^2 / n) from j^2 mod 2n, the transform of the convolution kernel made once per plan).
+struct FftPlan;
+hipError_t fft_plan_create(int n, int max_batch, hipStream_t s, FftPlan **out);
+void fft_plan_destroy(FftPlan *p);
+hipError_t launch_fft_rows(FftPlan *p, double2 *data, int batch, bool inverse, hipStream_t s);
+// In-place 2-D FFT with NumPy's convention (rows then columns, inverse divided by rows * cols) through plans of
+// the row length (pc, batches of `rows`) and the column length (pr, batches of `cols`); tmp: rows * cols pairs.
+hipError_t launch_fft2_planned(double2 *data, double2 *tmp, int rows, int cols, FftPlan *pr, FftPlan *pc,
+                               bool inverse, hipStream_t s);
+
 struct DpcGrid {
     int32_t R, C, Rp, Cp;     // scan region and padded grid
     double sr, sc;            // sampling

@@ -400,7 +400,8 @@ int cubo_hip_phase_rotation(int device, const double *nx, const double *ny, int3
 
 /* ---- iterative DPC (DPC.reconstruct, Sources/CuboPhase/DPC.swift:26-102) on the device, in Double, and
  * its adjoint. A plan fixes the scan (rows x cols), the padding (the grid is round(rows * padding) x
- * round(cols * padding), half-even, at most 2048 per dimension) and the sampling. Fields fx, fy are
+ * round(cols * padding), half-even, at most 2^28 points; sides above 2048 use four-step transforms for powers of two
+ * and Bluestein's chirp convolution otherwise) and the sampling. Fields fx, fy are
  * rows x cols; phases on the padded grid are padded_rows x padded_cols, row-major. ---- */
 typedef struct cubo_hip_dpc cubo_hip_dpc;
 int cubo_hip_dpc_create(int device, int32_t rows, int32_t cols, double padding, double sampling_row,

@@ -488,9 +488,91 @@ static double np_mean(const double *a, int64_t n) { return n ? oc_pairwise_sum(a
 /* of two, an exact-index direct DFT otherwise (the oracle only meets small grids).                    */
 /* ---------------------------------------------------------------------------------------------- */
 
+static void fft1d(double *re, double *im, int n, int inverse, double *tr, double *ti);
+
+/* Lengths above 2048 that are not powers of two: Bluestein's algorithm (the DFT as a chirp convolution, through
+ * power-of-two transforms of length M >= 2n - 1): with jk = (j^2 + k^2 - (k - j)^2) / 2,
+ * X_k = c_k sum_j (x_j c_j) conj(c_{k-j}), c_j = exp(-+i pi j^2 / n), the angles from j^2 mod 2n in integers. A plan
+ * holds the chirp and the transform of the kernel conj(c) (placed at m and M - m) for one length and direction. */
+typedef struct {
+    int n, M;
+    double *cr, *ci, *kr, *ki;
+} bluestein_plan;
+
+static int bluestein_needed(int n) { return n > 2048 && (n & (n - 1)) != 0; }
+
+static void bluestein_make(bluestein_plan *p, int n, int inverse) {
+    int M = 1;
+    while (M < 2 * n - 1) M <<= 1;
+    p->n = n;
+    p->M = M;
+    p->cr = (double *)malloc(sizeof(double) * (size_t)n * 2);
+    p->ci = p->cr + n;
+    p->kr = (double *)calloc((size_t)M * 2, sizeof(double));
+    p->ki = p->kr + M;
+    const double sign = inverse ? 1.0 : -1.0;
+    for (int j = 0; j < n; ++j) {
+        const long long q = ((long long)j * j) % (2LL * n);
+        const double ang = sign * M_PI * (double)q / (double)n;
+        p->cr[j] = cos(ang);
+        p->ci[j] = sin(ang);
+        p->kr[j] = p->cr[j];
+        p->ki[j] = -p->ci[j];
+        if (j > 0) { p->kr[M - j] = p->cr[j]; p->ki[M - j] = -p->ci[j]; }
+    }
+    double *t = (double *)malloc(sizeof(double) * (size_t)M * 2);
+    fft1d(p->kr, p->ki, M, 0, t, t + M);
+    free(t);
+}
+
+static void bluestein_free(bluestein_plan *p) {
+    free(p->cr);
+    free(p->kr);
+}
+
+/* work: 4 M doubles */
+static void bluestein_row(const bluestein_plan *p, double *re, double *im, double *work) {
+    const int n = p->n, M = p->M;
+    double *ar = work, *ai = work + M, *t = work + 2 * M;
+    for (int j = 0; j < M; ++j) {
+        if (j < n) {
+            ar[j] = re[j] * p->cr[j] - im[j] * p->ci[j];
+            ai[j] = re[j] * p->ci[j] + im[j] * p->cr[j];
+        } else {
+            ar[j] = 0.0;
+            ai[j] = 0.0;
+        }
+    }
+    fft1d(ar, ai, M, 0, t, t + M);
+    for (int m = 0; m < M; ++m) {
+        const double xr = ar[m] * p->kr[m] - ai[m] * p->ki[m], xi = ar[m] * p->ki[m] + ai[m] * p->kr[m];
+        ar[m] = xr;
+        ai[m] = xi;
+    }
+    fft1d(ar, ai, M, 1, t, t + M);
+    for (int k = 0; k < n; ++k) {
+        const double xr = ar[k] / M, xi = ai[k] / M;
+        re[k] = xr * p->cr[k] - xi * p->ci[k];
+        im[k] = xr * p->ci[k] + xi * p->cr[k];
+    }
+}
+
+static void bluestein(double *re, double *im, int n, int inverse) {
+    bluestein_plan p;
+    bluestein_make(&p, n, inverse);
+    double *work = (double *)malloc(sizeof(double) * (size_t)p.M * 4);
+    bluestein_row(&p, re, im, work);
+    free(work);
+    bluestein_free(&p);
+}
+
 static void fft1d(double *re, double *im, int n, int inverse, double *tr, double *ti) {
     if (n <= 1) return;
     double sign = inverse ? 1.0 : -1.0;
+    if ((n & (n - 1)) != 0 && n > 2048) {
+        bluestein(re, im, n, inverse);
+        return;
+    }
     if ((n & (n - 1)) == 0) {
         for (int i = 1, j = 0; i < n; ++i) {            /* bit reversal */
             int bit = n >> 1;
@@ -528,21 +610,39 @@ static void fft1d(double *re, double *im, int n, int inverse, double *tr, double
     }
 }
 
+/* Rows, then columns, each line independent of the others (so the threads do not change the result); the Bluestein
+ * plans are made once per axis. */
 void oc_fft2d(double *re, double *im, int rows, int cols, int inverse) {
-    int m = rows > cols ? rows : cols;
-    double *tr = (double *)malloc(sizeof(double) * m * 4);
-    double *ti = tr + m, *cr = tr + 2 * m, *ci = tr + 3 * m;
-    for (int r = 0; r < rows; ++r) fft1d(re + (int64_t)r * cols, im + (int64_t)r * cols, cols, inverse, tr, ti);
-    for (int c = 0; c < cols; ++c) {
-        for (int r = 0; r < rows; ++r) { cr[r] = re[(int64_t)r * cols + c]; ci[r] = im[(int64_t)r * cols + c]; }
-        fft1d(cr, ci, rows, inverse, tr, ti);
-        for (int r = 0; r < rows; ++r) { re[(int64_t)r * cols + c] = cr[r]; im[(int64_t)r * cols + c] = ci[r]; }
+    bluestein_plan pr, pc;
+    const int br = bluestein_needed(rows), bc = bluestein_needed(cols);
+    if (bc) bluestein_make(&pc, cols, inverse);
+    if (br) bluestein_make(&pr, rows, inverse);
+    const int m = rows > cols ? rows : cols;
+    const int mw = 4 * ((bc ? pc.M : 0) > (br ? pr.M : 0) ? (bc ? pc.M : 0) : (br ? pr.M : 0)) + 4 * m;
+#pragma omp parallel
+    {
+        double *tr = (double *)malloc(sizeof(double) * (size_t)mw);
+        double *ti = tr + m, *cr = tr + 2 * m, *ci = tr + 3 * m, *work = tr + 4 * m;
+#pragma omp for schedule(static)
+        for (int r = 0; r < rows; ++r) {
+            if (bc) bluestein_row(&pc, re + (int64_t)r * cols, im + (int64_t)r * cols, work);
+            else fft1d(re + (int64_t)r * cols, im + (int64_t)r * cols, cols, inverse, tr, ti);
+        }
+#pragma omp for schedule(static)
+        for (int c = 0; c < cols; ++c) {
+            for (int r = 0; r < rows; ++r) { cr[r] = re[(int64_t)r * cols + c]; ci[r] = im[(int64_t)r * cols + c]; }
+            if (br) bluestein_row(&pr, cr, ci, work);
+            else fft1d(cr, ci, rows, inverse, tr, ti);
+            for (int r = 0; r < rows; ++r) { re[(int64_t)r * cols + c] = cr[r]; im[(int64_t)r * cols + c] = ci[r]; }
+        }
+        free(tr);
     }
     if (inverse) {
         double s = 1.0 / ((double)rows * (double)cols);
         for (int64_t k = 0; k < (int64_t)rows * cols; ++k) { re[k] *= s; im[k] *= s; }
     }
-    free(tr);
+    if (bc) bluestein_free(&pc);
+    if (br) bluestein_free(&pr);
 }
 
 /* ---------------------------------------------------------------------------------------------- */

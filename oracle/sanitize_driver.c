@@ -133,6 +133,16 @@ int main(void) {
         reductions(shapes[i][0], shapes[i][1], shapes[i][2], shapes[i][3], shapes[i][4]);
     const int fields[][2] = {{3, 3}, {7, 9}, {10, 13}, {16, 16}};
     for (size_t i = 0; i < sizeof fields / sizeof fields[0]; ++i) phase(fields[i][0], fields[i][1]);
+    /* transforms above 2048 a side: Bluestein (2050) and radix-2 (4096) lengths, both axes */
+    const int ffts[][2] = {{3, 2050}, {2050, 2}, {2, 4096}};
+    for (size_t i = 0; i < sizeof ffts / sizeof ffts[0]; ++i) {
+        const int64_t n = (int64_t)ffts[i][0] * ffts[i][1];
+        double *re = vec(n, -1, 1), *im = vec(n, -1, 1);
+        oc_fft2d(re, im, ffts[i][0], ffts[i][1], 0);
+        oc_fft2d(re, im, ffts[i][0], ffts[i][1], 1);
+        free(re);
+        free(im);
+    }
     puts("oracle sanitize driver: done");
     return 0;
 }

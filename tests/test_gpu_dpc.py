@@ -130,8 +130,38 @@ def test_refusals():
     with pytest.raises(CuboVirtualError):
         DPC(3, 3).reconstruct(np.zeros(9), np.zeros(9))
     with pytest.raises(CuboVirtualError):
-        DPC(1500, 10)                                  # padded to 3000 rows
+        DPC(20000, 20000)                              # a padded grid of 1.6e9 points (above 2^28)
     with pytest.raises(CuboVirtualError):
         DPC(3, 3).reconstruct(np.array([1.0] * 8 + [np.inf]), np.zeros(9))
     with pytest.raises(CuboVirtualError):
         DPC(3, 3, padding=0.5)
+
+
+@pytest.mark.parametrize("R,C,padding", [(1100, 1025, 2.0), (1024, 600, 4.0), (1500, 10, 2.0)])
+def test_grids_above_2048_a_side(R, C, padding):
+    """DPC.swift:42-44 sets no size limit. Padded sides above 2048 run four-step transforms (powers of two) and
+    Bluestein's chirp convolution (other lengths): grids 2200 x 2050 (both Bluestein), 4096 x 2400 (four-step rows,
+    Bluestein columns) and 3000 x 20 (Bluestein rows, the LDS transform for the columns). One step and its adjoint
+    against the oracle (whose transforms above 2048 are pinned to numpy.fft) within 1e-10 of the largest value, and
+    three iterations of reconstruct."""
+    print(f"case: DPC grid of a {R} x {C} scan, padding {padding}", flush=True)
+    d = DPC(R, C, padding=padding, sampling=(0.9, 1.1))
+    Rp, Cp = d.padded_shape
+    assert (Rp, Cp) == (int(round(R * padding)), int(round(C * padding)))
+    fx, fy = _field(R, C, 11)
+    rng = np.random.default_rng(12)
+    phi = rng.normal(size=Rp * Cp)
+    out, err = d.step(phi, fx, fy, step=0.4)
+    ref, ref_err = O.dpc_step(phi, fx, fy, R, C, Rp, Cp, sampling=(0.9, 1.1), step=0.4)
+    assert np.max(np.abs(out - ref)) <= 1e-10 * np.max(np.abs(ref)), "phase"
+    assert abs(err - ref_err) <= 1e-10 * abs(ref_err)
+    g = rng.normal(size=Rp * Cp)
+    gp, gx, gy = d.step_backward(phi, fx, fy, g, g_error=0.7, step=0.4)
+    wp, wx, wy = O.dpc_step_vjp(phi, fx, fy, R, C, Rp, Cp, g, 0.7, sampling=(0.9, 1.1), step=0.4)
+    for a, b, name in ((gp, wp, "phi"), (gx, wx, "fx"), (gy, wy, "fy")):
+        assert np.max(np.abs(a - b)) <= 1e-10 * np.max(np.abs(b)), f"adjoint {name}"
+    got = d.reconstruct(fx, fy, max_iter=3, step=0.5)
+    want = O.dpc_reconstruct(fx, fy, R, C, sampling=(0.9, 1.1), padding=padding, max_iter=3, step=0.5)
+    assert got["accepted"] == want["accepted"]
+    near(got["errors"], want["errors"], 1e-10, "errors")
+    of_max(got["phase"], want["phase"], 1e-10, "phase")

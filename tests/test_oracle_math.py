@@ -23,7 +23,9 @@ def test_pairwise_sum_is_numpys():
 
 def test_fft2_matches_numpy_both_directions():
     r = _rng(3)
-    for shape in [(8, 16), (6, 10), (48, 40), (1, 5)]:
+    # the last three go through Bluestein's chirp convolution (lengths above 2048 that are not powers of two) and the
+    # radix-2 transform above 2048
+    for shape in [(8, 16), (6, 10), (48, 40), (1, 5), (3, 2050), (2, 3001), (2101, 3), (2, 4096)]:
         a = r.standard_normal(shape) + 1j * r.standard_normal(shape)
         re, im = O.fft2(a.real, a.imag)
         ref = np.fft.fft2(a)
