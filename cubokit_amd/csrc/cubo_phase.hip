@@ -3,12 +3,12 @@
 //
 // CenterOfMassReference.normalize (Sources/CuboPhase/CenterOfMassReference.swift:27-84): a least-squares plane
 // a + b u + c v over the finite positions (normal equations on centred coordinates, degenerate when
-// det <= tol * max(suu svv, 1)), the constant mean, or nothing, subtracted; NaN -> 0, +-inf -> +-max finite;
-// times the reciprocal sampling. RotationSolver.solve (Sources/CuboPhase/RotationSolver.swift:48-150): for each
-// angle and both orientations (components swapped first when transposed) the mean over the scan's interior of
-// |d_row y' - d_col x'| (or of the divergence when maximising); the first best angle wins and the plain
-// orientation wins ties. Each (angle, orientation) is one block group on the device; the sums of the blocks
-// are added on the host in block order.
+// det <= tol * max(suu svv, 1)), the constant mean, a given field, or nothing, subtracted; NaN -> 0, +-inf ->
+// +-max finite; times the reciprocal sampling. RotationSolver.solve (Sources/CuboPhase/RotationSolver.swift:
+// 48-150): for each angle and both orientations (components swapped first when transposed) the mean over the
+// scan's interior of |d_row y' - d_col x'| (or of the divergence when maximising); the first best angle wins and
+// the plain orientation wins ties; a forced angle and/or orientation narrows what is scored. Each (angle,
+// orientation) is one block group on the device; the sums of the blocks are added on the host in block order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
