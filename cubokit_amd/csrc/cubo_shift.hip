@@ -65,6 +65,11 @@ struct cubo_hip_shift_plan {
     double *dy_inv = nullptr;
     double *rpartial = nullptr, *rdmask = nullptr;   // the matrix-core adjoint's [groups][32][ndet] and [32][ndet]
     int rgroups = 0;
+    // batches of 16 positions of one column shift for shift_grad_t (built from the last origin map)
+    bool batches_ready = false;
+    int64_t nbatch = 0;
+    int32_t *bslots = nullptr, *bic = nullptr, *border = nullptr;   // [nbatch][16][2], [nbatch], [nbatch][16]
+    size_t bcap = 0;
     double *partial = nullptr, *dmask = nullptr;
     int ngroups = 0;
     hipEvent_t ev[6] = {};
@@ -80,7 +85,7 @@ void free_plan(cubo_hip_shift_plan *p) {
     (void)hipStreamSynchronize(p->cube->stream);
     void *bufs[] = {p->base, p->inv, p->wroll, p->perm, p->shift, p->slot_shifts, p->tiles, p->wr, p->wc, p->wts,
                     p->dy, p->g, p->dy32, p->partial, p->dmask, p->vmasks, p->vshift, p->vwr, p->vwc,
-                    p->dyw, p->dy_inv, p->rpartial, p->rdmask};
+                    p->dyw, p->dy_inv, p->rpartial, p->rdmask, p->bslots, p->bic, p->border};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->yhost) (void)hipHostFree(p->yhost);
@@ -251,11 +256,81 @@ int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol, bool
     if (e != hipSuccess) return hip_fail(e, "preparing the shift classes");
     p->ntiles = (int)tiles.size();
     p->nslots = nslots;
+    p->batches_ready = false;
     p->o_row.assign(orow, orow + npos);
     p->o_col.assign(ocol, ocol + npos);
     p->prepared = true;
     p->prepared_rounded = rounded;
     return CUBO_HIP_OK;
+}
+
+// The positions in batches of 16 of one column shift for shift_grad_t: a counting sort by ic (stable in position
+// order), each shift's last batch padded with slots that repeat its first position and carry no dL/dY (order -1).
+int prepare_batches(cubo_hip_shift_plan *p) {
+    if (p->batches_ready) return CUBO_HIP_OK;
+    cubo_hip_cube *c = p->cube;
+    const int64_t npos = c->npos;
+    const int qx = (int)c->shape[3];
+    std::vector<int64_t> count((size_t)qx + 1, 0);
+    for (int64_t i = 0; i < npos; ++i) ++count[p->h_shift[2 * i + 1] + 1];
+    int64_t nbatch = 0;
+    for (int ic = 0; ic < qx; ++ic) nbatch += (count[ic + 1] + 15) / 16;
+    for (size_t i = 1; i < count.size(); ++i) count[i] += count[i - 1];
+    std::vector<int32_t> sorted(npos);
+    {
+        std::vector<int64_t> at(count.begin(), count.end() - 1);
+        for (int64_t i = 0; i < npos; ++i) sorted[at[p->h_shift[2 * i + 1]]++] = (int32_t)i;
+    }
+    std::vector<int32_t> slots((size_t)nbatch * 32), bic((size_t)nbatch), order((size_t)nbatch * 16);
+    int64_t b = 0;
+    for (int ic = 0; ic < qx; ++ic) {
+        for (int64_t t = count[ic]; t < count[ic + 1]; t += 16, ++b) {
+            bic[b] = ic;
+            for (int sl = 0; sl < 16; ++sl) {
+                const bool live = t + sl < count[ic + 1];
+                const int32_t pos = sorted[live ? t + sl : t];
+                slots[(size_t)b * 32 + 2 * sl] = pos;
+                slots[(size_t)b * 32 + 2 * sl + 1] = p->h_shift[2 * (size_t)pos];
+                order[(size_t)b * 16 + sl] = live ? pos : -1;
+            }
+        }
+    }
+    DeviceGuard g(c->device);
+    hipError_t e = hipSuccess;
+    if (p->bcap < (size_t)nbatch) {
+        for (int32_t **q : {&p->bslots, &p->bic, &p->border}) {
+            if (*q) (void)hipFree(*q);
+            *q = nullptr;
+        }
+        p->bcap = 0;
+        e = hipMalloc((void **)&p->bslots, (size_t)nbatch * 32 * 4);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->bic, (size_t)nbatch * 4);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->border, (size_t)nbatch * 16 * 4);
+        if (e != hipSuccess) return hip_fail(e, "allocating the adjoint's batches");
+        p->bcap = (size_t)nbatch;
+    }
+    hipStream_t s = c->stream;
+    e = hipMemcpyAsync(p->bslots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->bic, bic.data(), bic.size() * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->border, order.data(), order.size() * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);     // the host vectors go out of scope
+    if (e != hipSuccess) return hip_fail(e, "uploading the adjoint's batches");
+    p->nbatch = nbatch;
+    p->batches_ready = true;
+    return CUBO_HIP_OK;
+}
+
+// Of the group counts that give at least one block per compute unit, the one whose blocks (one of them per compute
+// unit at a time) fill their last round best.
+int group_count(int64_t bands, int64_t units, int cus) {
+    int ngroups = 1;
+    double best = -1;
+    for (int64_t ng = std::max<int64_t>(1, (cus + bands - 1) / bands); ng <= std::min<int64_t>(64, units); ++ng) {
+        const int64_t blocks = bands * ng, rounds = (blocks + cus - 1) / cus;
+        const double fill = (double)blocks / (double)(rounds * cus);
+        if (fill > best + 0.02) { best = fill; ngroups = (int)ng; }
+    }
+    return ngroups;
 }
 
 cubo::ShiftLaunch launch_args(const cubo_hip_shift_plan *p) {
@@ -454,20 +529,15 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
                 scale[kk] = std::ldexp(1.0, 10 - ex);
                 inv[kk] = std::ldexp(1.0, ex - 10);
             }
-        const int qy = (int)c->shape[2];
+        const int qy = (int)c->shape[2], qx = (int)c->shape[3];
         const int cus = cubo::compute_units(c->device);
-        // groups: of the counts that give at least one block per compute unit, the one whose blocks (one of them
-        // per compute unit at a time) fill their last round best
-        const int64_t bands = (qy + cubo::shift_grad_rows_band((int)c->shape[3]) - 1) /
-                              cubo::shift_grad_rows_band((int)c->shape[3]);
-        int ngroups = 1;
-        double best = -1;
-        for (int64_t ng = std::max<int64_t>(1, (cus + bands - 1) / bands); ng <= std::min<int64_t>(64, nks); ++ng) {
-            const int64_t blocks = bands * ng, rounds = (blocks + cus - 1) / cus;
-            const double fill = (double)blocks / (double)(rounds * cus);
-            if (fill > best + 0.02) { best = fill; ngroups = (int)ng; }
-        }
-        const size_t dyw_bytes = (size_t)nks * 4 * 2 * 64 * 8 * sizeof(_Float16);
+        const bool tposed = cubo::shift_grad_t_supported(qy, qx);   // batches of one column shift, staged transposed
+        if (tposed)
+            if (int st = prepare_batches(p)) return st;
+        const int64_t units = tposed ? p->nbatch : nks;
+        const int band = tposed ? cubo::shift_grad_t_band(qx) : cubo::shift_grad_rows_band(qx);
+        const int ngroups = group_count((qy + band - 1) / band, units, cus);
+        const size_t dyw_bytes = (size_t)units * 4 * 2 * 64 * 8 * sizeof(_Float16);
         if (p->dyw_bytes < dyw_bytes) {
             if (p->dyw) (void)hipFree(p->dyw);
             p->dyw = nullptr;
@@ -486,21 +556,38 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
         if (!p->rdmask) CUBO_HIP_TRY(hipMalloc((void **)&p->rdmask, (size_t)32 * ndet * 8), "allocating dL/dmasks");
         inv.insert(inv.end(), scale.begin(), scale.end());          // dy_inv: [32] 1 / scale, then [32] scale
         CUBO_HIP_TRY(hipMemcpyAsync(p->dy_inv, inv.data(), 64 * 8, hipMemcpyHostToDevice, s), "uploading dL/dY scales");
-        CUBO_HIP_TRY(cubo::launch_split_dy(p->dy, p->wr, p->wc, npos, p->k, p->dy_inv + 32, p->dyw, s),
+        CUBO_HIP_TRY(cubo::launch_split_dy(p->dy, p->wr, p->wc, npos, p->k, p->dy_inv + 32, tposed ? p->border : nullptr,
+                                           units * 16, p->dyw, s),
                      "splitting dL/dY");
-        cubo::ShiftGradRows gw{};
-        gw.cube = (const uint16_t *)c->data;
-        gw.npos = npos;
-        gw.qy = qy;
-        gw.qx = (int32_t)c->shape[3];
-        gw.shift = p->shift;
-        gw.dyw = p->dyw;
-        gw.dy_inv = p->dy_inv;
-        gw.partial = p->rpartial;
-        gw.ngroups = ngroups;
-        gw.ppg = ((nks + ngroups - 1) / ngroups) * 16;
         CUBO_HIP_TRY(hipEventRecord(p->ev[4], s), "hipEventRecord");
-        CUBO_HIP_TRY(cubo::launch_shift_grad_rows(gw, s), "launching the mask adjoint");
+        if (tposed) {
+            cubo::ShiftGradT gt{};
+            gt.cube = (const uint16_t *)c->data;
+            gt.qy = qy;
+            gt.qx = qx;
+            gt.nbatch = p->nbatch;
+            gt.slots = p->bslots;
+            gt.bic = p->bic;
+            gt.dyw = p->dyw;
+            gt.dy_inv = p->dy_inv;
+            gt.partial = p->rpartial;
+            gt.ngroups = ngroups;
+            gt.bpg = (p->nbatch + ngroups - 1) / ngroups;
+            CUBO_HIP_TRY(cubo::launch_shift_grad_t(gt, s), "launching the mask adjoint");
+        } else {
+            cubo::ShiftGradRows gw{};
+            gw.cube = (const uint16_t *)c->data;
+            gw.npos = npos;
+            gw.qy = qy;
+            gw.qx = qx;
+            gw.shift = p->shift;
+            gw.dyw = p->dyw;
+            gw.dy_inv = p->dy_inv;
+            gw.partial = p->rpartial;
+            gw.ngroups = ngroups;
+            gw.ppg = ((nks + ngroups - 1) / ngroups) * 16;
+            CUBO_HIP_TRY(cubo::launch_shift_grad_rows(gw, s), "launching the mask adjoint");
+        }
         CUBO_HIP_TRY(cubo::launch_sum_groups(p->rpartial, ngroups, (int64_t)32 * ndet, p->rdmask, s),
                      "launching the mask adjoint sum");
         CUBO_HIP_TRY(hipEventRecord(p->ev[5], s), "hipEventRecord");

@@ -509,6 +509,213 @@ __global__ __launch_bounds__(64 * W * RB) void shift_grad_rows(ShiftGradRows g) 
     }
 }
 
+// The mask adjoint with the patterns staged transposed. A batch is 16 positions of one column shift ic (padding
+// slots repeat a position of the batch and have zero dL/dY). Block (band, g) owns output rows a0 .. a0 + RB - 1
+// and the batches of group g; wave w owns two 32-pixel tiles of output row a0 + w / WPR. Per batch the SR = RB + 1
+// pattern rows the band reads, (a0 + ir_s + rho) mod qy for slot s, are staged in LDS as FP16 (the exact split
+// x = (x & 0xF800) + (x & 0x07FF)) at [rho][beta][s] with beta = (column - ic) mod qx, so the roll by ic is done
+// by the staging and every A operand (8 positions of one pixel) is one aligned 16-byte read:
+//   corner (r, cc) of output pixel (a0 + rw, b) reads [rw + r][b + cc][8 hf .. 8 hf + 7]
+// (entry qx repeats entry 0 for the wrap). Entries are 32 bytes with a 32-byte gap after every 8, and the two
+// 16-byte halves of an entry swap places in every other group of 8: the staging writes (8 slot pairs x 8
+// consecutive 16-byte pieces per wave) and the operand reads then fall on distinct banks. The high parts are staged,
+// in a second phase behind a second barrier, only for batches where some count is 2048 or more.
+template <int WPR, int RB>
+__global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
+    constexpr int QX = 64 * WPR;
+    constexpr int SR = RB + 1;
+    constexpr int T = 64 * WPR * RB;
+    constexpr int RS = 8 * (QX + QX / 8 + 1);        // dwords per staged row (entries 0 .. QX, gaps)
+    constexpr int PLANE = SR * RS;
+    constexpr int C8 = QX / 8;                       // 16-byte pieces per pattern row
+    constexpr int TASKS = 8 * SR * C8;               // (slot pair, staged row, piece)
+    constexpr int PER = (TASKS + T - 1) / T;
+    constexpr int BVB = 4 * 2 * 64;                  // B-operand entries per batch (corner, hi/lo, lane)
+    constexpr int BPER = (BVB + T - 1) / T;
+    static_assert(T % 64 == 0, "whole waves");
+    __shared__ __align__(16) uint32_t slo[2][PLANE];
+    __shared__ __align__(16) uint32_t shi[PLANE];
+    __shared__ half8 sb[2][BVB];
+    __shared__ int32_t hiflag[2];
+    const int tid = threadIdx.x, lane = tid & 63, hf = lane >> 5, n = lane & 31;
+    const int wave = tid >> 6, rw = wave / WPR, wc = wave % WPR;
+    const int qy = g.qy;
+    const int64_t ndet = (int64_t)qy * QX;
+    const int a0 = blockIdx.x * RB, arow = a0 + rw, group = blockIdx.y;
+    const int64_t b_begin = (int64_t)group * g.bpg, b_end = min(g.nbatch, b_begin + g.bpg);
+    const CUBO_GLOBAL uint16_t *cube = (const CUBO_GLOBAL uint16_t *)g.cube;
+    const CUBO_GLOBAL half8 *dyw = (const CUBO_GLOBAL half8 *)g.dyw;
+    const CUBO_GLOBAL u32x4 *slots = (const CUBO_GLOBAL u32x4 *)g.slots;   // per slot pair: pos, ir, pos, ir
+
+    // dword address of half h of entry beta of staged row rho
+    auto ent = [](int rho, int beta, int h) { return rho * RS + 8 * (beta + (beta >> 3)) + 4 * (h ^ ((beta >> 3) & 1)); };
+
+    // The staging tasks of this thread: v = tid + k T; its slot pair q = v & 7 is the same for every k.
+    const int q = tid & 7;
+    u32x4 reg[2][PER][2];
+    half8 breg[2][BPER];
+    int reg_ic[2] = {0, 0};
+    u32x4 rec = {0u, 0u, 0u, 0u};                    // slot records of the next batch to load
+    auto fetch_rec = [&](int64_t bi) { if (bi < b_end) rec = slots[bi * 8 + q]; };
+    auto load = [&](int set, int64_t bi) {
+        const int ir0 = (int)rec.y, ir1 = (int)rec.w;
+        const CUBO_GLOBAL uint16_t *x0 = cube + (int64_t)rec.x * ndet, *x1 = cube + (int64_t)rec.z * ndet;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int v = tid + k * T;
+            if (TASKS % T == 0 || v < TASKS) {
+                const int w = v >> 3, c = w % C8, rho = w / C8;
+                int r0 = a0 + ir0 + rho, r1 = a0 + ir1 + rho;   // < 3 qy: a0, ir < qy and rho <= RB < qy
+                r0 -= r0 >= qy ? qy : 0; r0 -= r0 >= qy ? qy : 0;
+                r1 -= r1 >= qy ? qy : 0; r1 -= r1 >= qy ? qy : 0;
+                reg[set][k][0] = *(const CUBO_GLOBAL u32x4 *)(x0 + r0 * QX + 8 * c);
+                reg[set][k][1] = *(const CUBO_GLOBAL u32x4 *)(x1 + r1 * QX + 8 * c);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < BPER; ++k)
+            if (BVB % T == 0 || tid + k * T < BVB) breg[set][k] = dyw[bi * BVB + tid + k * T];
+        reg_ic[set] = g.bic[bi];
+        fetch_rec(bi + 1);
+    };
+    // One plane of the staged rows from register set `set`: mask 0x07FF (lo) or 0xF800 (hi); returns whether a count
+    // of 2048 or more was seen.
+    auto stage = [&](int set, uint32_t *dst, uint32_t mask) {
+        const int ic = reg_ic[set];
+        uint32_t any = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int v = tid + k * T;
+            if (TASKS % T == 0 || v < TASKS) {
+                const int w = v >> 3, c = w % C8, rho = w / C8;
+                const u32x4 x0 = reg[set][k][0], x1 = reg[set][k][1];
+                any |= (x0.x | x0.y | x0.z | x0.w | x1.x | x1.y | x1.z | x1.w) & 0xf800f800u;
+                int base = 8 * c - ic;
+                base += base < 0 ? QX : 0;
+                const uint32_t w0[4] = {x0.x, x0.y, x0.z, x0.w}, w1[4] = {x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    // (slot 2q, pixel e) in the low half, (slot 2q + 1, pixel e) in the high half
+                    const uint32_t d = __builtin_amdgcn_perm(w1[e >> 1], w0[e >> 1], (e & 1) ? 0x07060302u : 0x05040100u);
+                    const uint32_t hv = cvt_pair(d & mask);
+                    int beta = base + e;
+                    beta -= beta >= QX ? QX : 0;
+                    dst[ent(rho, beta, q >> 2) + (q & 3)] = hv;
+                    if (beta == 0) dst[ent(rho, QX, q >> 2) + (q & 3)] = hv;
+                }
+            }
+        }
+        return any != 0;
+    };
+    auto store = [&](int set, int b, int64_t bi) {
+        if (stage(set, slo[b], 0x07ff07ffu)) hiflag[b] = (int32_t)bi;   // every writer writes the same value
+#pragma unroll
+        for (int k = 0; k < BPER; ++k)
+            if (BVB % T == 0 || tid + k * T < BVB) sb[b][tid + k * T] = breg[set][k];
+    };
+
+    float16v acc[2];
+    double accd[2][16];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) { acc[t][r] = 0.f; accd[t][r] = 0.0; }
+    // A-operand addresses of the wave's tiles t, rows r and column corners cc (the same for every batch)
+    int aaddr[2][2][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) aaddr[t][r][cc] = ent(rw + r, 64 * wc + 32 * t + n + cc, hf);
+    int nk = 0;
+    auto compute = [&](int b, int64_t bi) {
+        half8 bh[4], bl[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            bh[c] = sb[b][(c * 2 + 0) * 64 + lane];
+            bl[c] = sb[b][(c * 2 + 1) * 64 + lane];
+        }
+        const bool use_hi = hiflag[b] == (int32_t)bi;   // uniform
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    const int c = r + 2 * cc;
+                    const half8 a = *(const half8 *)&slo[b][aaddr[t][r][cc]];
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bh[c], acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl[c], acc[t], 0, 0, 0);
+                }
+            if (use_hi) {
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < 2; ++cc) {
+                        const int c = r + 2 * cc;
+                        const half8 a = *(const half8 *)&shi[aaddr[t][r][cc]];
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bh[c], acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl[c], acc[t], 0, 0, 0);
+                    }
+            }
+        }
+        if (++nk == 2) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) { accd[t][r] += (double)acc[t][r]; acc[t][r] = 0.f; }
+            nk = 0;
+        }
+    };
+
+    if (tid < 2) hiflag[tid] = -1;
+    __syncthreads();
+    if (b_begin < b_end) {
+        fetch_rec(b_begin);
+        load(0, b_begin);
+        store(0, 0, b_begin);
+        if (b_begin + 1 < b_end) load(1, b_begin + 1);
+        __syncthreads();
+        if (hiflag[0] == (int32_t)b_begin) {                 // uniform
+            stage(0, shi, 0xf800f800u);
+            __syncthreads();
+        }
+        // step(bi, b, cur): batch bi is in LDS buffer b, batch bi + 1 in register set cur; the loads of batch bi + 2
+        // go to the other set while batch bi is multiplied
+        auto step = [&](int64_t bi, int b, int cur) {
+            if (bi + 2 < b_end) load(cur ^ 1, bi + 2);       // uniform
+            compute(b, bi);
+            if (bi + 1 < b_end) store(cur, b ^ 1, bi + 1);
+            lds_barrier();
+            if (bi + 1 < b_end && hiflag[b ^ 1] == (int32_t)(bi + 1)) {   // uniform: the high parts of batch bi + 1
+                stage(cur, shi, 0xf800f800u);
+                lds_barrier();
+            }
+        };
+        int64_t bi = b_begin;
+        for (; bi + 1 < b_end; bi += 2) {
+            step(bi, 0, 1);
+            step(bi + 1, 1, 0);
+        }
+        if (bi < b_end) step(bi, 0, 1);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) accd[t][r] += (double)acc[t][r];
+    // register r of lane l: output pixel (arow, 64 wc + 32 t + 8 (r / 4) + 4 (l / 32) + r % 4), mask l % 32
+    if (arow >= qy) return;                          // the band's last rows past the detector
+    const double scale = g.dy_inv[n];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int64_t pix = (int64_t)arow * QX + 64 * wc + 32 * t + 8 * (r >> 2) + 4 * hf + (r & 3);
+            g.partial[((int64_t)group * 32 + n) * ndet + pix] = accd[t][r] * scale;
+        }
+}
+
 template <bool U16, bool ROUNDED>
 __global__ __launch_bounds__(256) void shift_valu(ShiftValu a) {
 #pragma clang fp contract(off)   // the blend, the products and TwoSum round as written
@@ -593,14 +800,15 @@ int shift_grad_rows_band(int qx) { return qx <= 128 ? 3 : 1; }
 
 namespace {
 __global__ void split_dy(const double *dy, const double *wr, const double *wc, int64_t npos, int k,
-                         const double *scale, _Float16 *dyw, int64_t n) {
+                         const double *scale, const int32_t *order, _Float16 *dyw, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;     // (k-step, corner, lane, e)
     if (i >= n) return;
     const int64_t ks = i / 2048;
     const int c = (int)((i / 512) % 4), l = (int)((i / 8) % 64), e = (int)(i % 8), kk = l & 31;
-    const int64_t pos = ks * 16 + 8 * (l >> 5) + e;
+    const int64_t slot = ks * 16 + 8 * (l >> 5) + e;
+    const int64_t pos = order ? (int64_t)order[slot] : slot;       // order: -1 for a padding slot
     double v = 0.0;
-    if (kk < k && pos < npos) {
+    if (kk < k && pos >= 0 && pos < npos) {
         const double r = wr[pos], q = wc[pos];
         const double w = c == 0 ? (1.0 - r) * (1.0 - q) : c == 1 ? r * (1.0 - q) : c == 2 ? (1.0 - r) * q : r * q;
         v = dy[pos * k + kk] * w * scale[kk];
@@ -613,10 +821,35 @@ __global__ void split_dy(const double *dy, const double *wr, const double *wc, i
 }  // namespace
 
 hipError_t launch_split_dy(const double *dy, const double *wr, const double *wc, int64_t npos, int k,
-                           const double *scale, void *dyw, hipStream_t s) {
-    const int64_t n = (npos + 15) / 16 * 2048;
-    hipLaunchKernelGGL(split_dy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dy, wr, wc, npos, k, scale,
+                           const double *scale, const int32_t *order, int64_t nslots, void *dyw, hipStream_t s) {
+    const int64_t n = (nslots + 15) / 16 * 2048;
+    hipLaunchKernelGGL(split_dy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dy, wr, wc, npos, k, scale, order,
                        (_Float16 *)dyw, n);
+    return hipGetLastError();
+}
+
+int shift_grad_t_band(int qx) {
+    switch (qx) {
+        case 64: return 8;
+        case 128: return 4;
+        case 192: return 2;
+        case 256: return 2;
+        default: return 0;
+    }
+}
+
+bool shift_grad_t_supported(int qy, int qx) { return shift_grad_t_band(qx) > 0 && qy >= shift_grad_t_band(qx); }
+
+hipError_t launch_shift_grad_t(const ShiftGradT &a, hipStream_t s) {
+    if (!shift_grad_t_supported(a.qy, a.qx) || a.ngroups <= 0) return hipErrorInvalidValue;
+    const int rb = shift_grad_t_band(a.qx);
+    const dim3 grid((unsigned)((a.qy + rb - 1) / rb), (unsigned)a.ngroups);
+    switch (a.qx) {
+        case 64: hipLaunchKernelGGL((shift_grad_t<1, 8>), grid, dim3(512), 0, s, a); break;
+        case 128: hipLaunchKernelGGL((shift_grad_t<2, 4>), grid, dim3(512), 0, s, a); break;
+        case 192: hipLaunchKernelGGL((shift_grad_t<3, 2>), grid, dim3(384), 0, s, a); break;
+        default: hipLaunchKernelGGL((shift_grad_t<4, 2>), grid, dim3(512), 0, s, a); break;
+    }
     return hipGetLastError();
 }
 

@@ -110,9 +110,34 @@ int shift_grad_rows_band(int qx);
 // ((1-wr)(1-wc), wr(1-wc), (1-wr)wc, wr wc for the corners (0,0), (1,0), (0,1), (1,1)) times scale[kk] (a power of
 // two), rounded to Float then split into FP16 hi and lo, at [k-step][corner][hi, lo][lane][8] (lane l: mask l % 32,
 // positions 16 s + 8 (l / 32) + e), zero past the positions and the masks.
+// order (optional): the position of each of the nslots slots, -1 for a padding slot (zero B operands); without it
+// slot i is position i and nslots = npos.
 hipError_t launch_split_dy(const double *dy, const double *wr, const double *wc, int64_t npos, int k,
-                           const double *scale, void *dyw, hipStream_t s);
+                           const double *scale, const int32_t *order, int64_t nslots, void *dyw, hipStream_t s);
 hipError_t launch_shift_grad_rows(const ShiftGradRows &a, hipStream_t s);
+
+// The same adjoint for rows of 64, 128, 192 or 256 pixels, with the positions in batches of 16 of one column shift ic
+// (the host's order, padded per shift): block (band, g) owns shift_grad_t_band(qx) output rows and the batches of
+// group g, each wave two 32-pixel tiles of one row. Per batch the band + 1 pattern rows each slot reads are staged
+// in LDS as FP16 lo (and, for batches with counts of 2048 or more, hi) parts, transposed to [row][pixel][slot] and
+// rolled by -ic, so each corner's A operand (8 positions of one pixel) is one aligned 16-byte read and the B
+// operands are split_dy's, in batch order.
+struct ShiftGradT {
+    const uint16_t *cube;
+    int32_t qy, qx;
+    int64_t nbatch;           // batches of 16 slots
+    const int32_t *slots;     // [nbatch][16][2] (position, ir); a padding slot repeats a position of its batch
+    const int32_t *bic;       // [nbatch] the batch's column shift
+    const void *dyw;          // [nbatch][corner][hi, lo][lane][8] FP16 (split_dy with the batch order)
+    const double *dy_inv;     // [32] 1 / column scale (0 past the masks)
+    double *partial;          // [ngroups][32][qy * qx]
+    int32_t ngroups;
+    int64_t bpg;              // batches per group
+};
+// Output rows per block (0: not supported); the kernel needs qy >= that many rows.
+int shift_grad_t_band(int qx);
+bool shift_grad_t_supported(int qy, int qx);
+hipError_t launch_shift_grad_t(const ShiftGradT &a, hipStream_t s);
 // out[i] = sum_g partial[g][i] in group order, i < n.
 hipError_t launch_sum_groups(const double *partial, int ngroups, int64_t n, double *out, hipStream_t s);
 

@@ -77,12 +77,17 @@ def test_equal_origins_give_the_plain_stack_bit_for_bit():
     assert np.array_equal(y, C.MaskStack(res, masks, C.MaskStack.MFMA_SPLIT).run())
 
 
-@pytest.mark.parametrize("det", [32, 24, 96, 160, 64, 128, 256])
-def test_backward_matches_the_oracle(det):
-    """Rows of 32, 96 and 160 px: the matrix-core mask adjoint (bands of three output rows for 32 and 96, of one for
-    160; 96 px rows stage with 9 waves, not a divisor of the batch); 24 px rows: the vector-ALU one. Rows of 64,
-    128 and 256 px: the origin adjoint's lower corners from the LDS weight ring."""
-    res = _cube((9, 11), det, 400.0, 6)         # 99 positions in many shift classes: partial tiles
+@pytest.mark.parametrize("det,counts", [(32, 400.0), (24, 400.0), (96, 400.0), (160, 400.0), (64, 400.0),
+                                        (128, 400.0), (256, 400.0), (192, 400.0), (128, 30000.0), (96, 30000.0)])
+def test_backward_matches_the_oracle(det, counts):
+    """Rows of 32, 96 and 160 px: the matrix-core mask adjoint over per-position gathers (bands of three output rows
+    for 32 and 96, of one for 160; 96 px rows stage with 9 waves, not a divisor of the batch); 24 px rows: the
+    vector-ALU one. Rows of 64, 128, 192 and 256 px: the mask adjoint over batches of one column shift staged
+    transposed (bands of 8, 4, 2 and 2 rows), and the origin adjoint's lower corners from the LDS weight ring.
+    Counts up to 30000: the data's high parts (counts of 2048 and more) on both mask-adjoint kernels."""
+    res = _cube((9, 11), det, counts, 6)        # 99 positions in many shift classes: partial tiles
+    if counts > 2048:
+        assert res.read_scan_tile(range(0, 9)).max() >= 2048
     masks = _masks(det, 6, 7, signed=True)
     orow, ocol = _origins(99, 8)
     rng = np.random.default_rng(9)
