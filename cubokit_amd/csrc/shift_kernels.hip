@@ -512,20 +512,20 @@ __global__ __launch_bounds__(64 * W * RB) void shift_grad_rows(ShiftGradRows g) 
 // The mask adjoint with the patterns staged transposed. A batch is 16 positions of one column shift ic (padding
 // slots repeat a position of the batch and have zero dL/dY). Block (band, g) owns output rows a0 .. a0 + RB - 1
 // and the batches of group g; wave w owns two 32-pixel tiles of output row a0 + w / WPR. Per batch the SR = RB + 1
-// pattern rows the band reads, (a0 + ir_s + rho) mod qy for slot s, are staged in LDS as FP16 (the exact split
-// x = (x & 0xF800) + (x & 0x07FF)) at [rho][beta][s] with beta = (column - ic) mod qx, so the roll by ic is done
-// by the staging and every A operand (8 positions of one pixel) is one aligned 16-byte read:
-//   corner (r, cc) of output pixel (a0 + rw, b) reads [rw + r][b + cc][8 hf .. 8 hf + 7]
-// (entry qx repeats entry 0 for the wrap). Entries are 32 bytes with a 32-byte gap after every 8, and the two
-// 16-byte halves of an entry swap places in every other group of 8: the staging writes (8 slot pairs x 8
-// consecutive 16-byte pieces per wave) and the operand reads then fall on distinct banks. The high parts are staged,
-// in a second phase behind a second barrier, only for batches where some count is 2048 or more.
+// pattern rows the band reads, (a0 + ir_s + rho) mod qy for slot s, are staged in LDS as FP16, transposed to
+// [rho][column][slot], in the exact split x = (x & 0xFC00) + (x & 0x03FF). Every A operand (8 positions of one
+// pixel) is then one aligned 16-byte read; the column shift is applied by the reader:
+//   corner (r, cc) of output pixel (a0 + rw, b) reads [rw + r][(b + cc + ic) mod qx][8 hf .. 8 hf + 7].
+// Entries are 32 bytes with a 32-byte gap after every 8, and the two 16-byte halves of an entry swap places in
+// every other group of 8: the staging writes (8 slot pairs x 8 consecutive 16-byte pieces per wave, at offsets fixed
+// at compile time) and the operand reads then fall on distinct banks. The high parts are staged, in a second phase
+// behind a second barrier, only for batches where some count is 1024 or more.
 template <int WPR, int RB>
 __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     constexpr int QX = 64 * WPR;
     constexpr int SR = RB + 1;
     constexpr int T = 64 * WPR * RB;
-    constexpr int RS = 8 * (QX + QX / 8 + 1);        // dwords per staged row (entries 0 .. QX, gaps)
+    constexpr int RS = 8 * (QX + QX / 8);            // dwords per staged row (entries 0 .. QX - 1 and the gaps)
     constexpr int PLANE = SR * RS;
     constexpr int C8 = QX / 8;                       // 16-byte pieces per pattern row
     constexpr int TASKS = 8 * SR * C8;               // (slot pair, staged row, piece)
@@ -547,8 +547,8 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     const CUBO_GLOBAL half8 *dyw = (const CUBO_GLOBAL half8 *)g.dyw;
     const CUBO_GLOBAL u32x4 *slots = (const CUBO_GLOBAL u32x4 *)g.slots;   // per slot pair: pos, ir, pos, ir
 
-    // dword address of half h of entry beta of staged row rho
-    auto ent = [](int rho, int beta, int h) { return rho * RS + 8 * (beta + (beta >> 3)) + 4 * (h ^ ((beta >> 3) & 1)); };
+    // dword address of half h of column j of staged row rho
+    auto ent = [](int rho, int j, int h) { return rho * RS + 8 * (j + (j >> 3)) + 4 * (h ^ ((j >> 3) & 1)); };
 
     // The staging tasks of this thread: v = tid + k T; its slot pair q = v & 7 is the same for every k.
     const int q = tid & 7;
@@ -565,7 +565,7 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
             const int v = tid + k * T;
             if (TASKS % T == 0 || v < TASKS) {
                 const int w = v >> 3, c = w % C8, rho = w / C8;
-                int r0 = a0 + ir0 + rho, r1 = a0 + ir1 + rho;   // < 3 qy: a0, ir < qy and rho <= RB < qy
+                int r0 = a0 + ir0 + rho, r1 = a0 + ir1 + rho;   // < 3 qy: a0, ir < qy and rho <= RB <= qy
                 r0 -= r0 >= qy ? qy : 0; r0 -= r0 >= qy ? qy : 0;
                 r1 -= r1 >= qy ? qy : 0; r1 -= r1 >= qy ? qy : 0;
                 reg[set][k][0] = *(const CUBO_GLOBAL u32x4 *)(x0 + r0 * QX + 8 * c);
@@ -578,10 +578,16 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
         reg_ic[set] = g.bic[bi];
         fetch_rec(bi + 1);
     };
-    // One plane of the staged rows from register set `set`: mask 0x07FF (lo) or 0xF800 (hi); returns whether a count
-    // of 2048 or more was seen.
-    auto stage = [&](int set, uint32_t *dst, uint32_t mask) {
-        const int ic = reg_ic[set];
+    // The FP16 parts of a pair of counts: low 10 bits exactly through the magic number 1024 (0x6400 | v is
+    // 1024 + v), the high 6 bits (multiples of 1024 up to 64512) by conversion.
+    auto part_lo = [](uint32_t w) {
+        const h2 m = __builtin_bit_cast(h2, (w & 0x03ff03ffu) | 0x64006400u);
+        const h2 r = m - (h2){(_Float16)1024.f, (_Float16)1024.f};
+        return __builtin_bit_cast(uint32_t, r);
+    };
+    auto part_hi = [](uint32_t w) { return cvt_pair(w & 0xfc00fc00u); };
+    // One plane of the staged rows from register set `set`: true if a count of 1024 or more was seen.
+    auto stage = [&](int set, uint32_t *dst, bool hi) {
         uint32_t any = 0;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
@@ -589,29 +595,30 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
             if (TASKS % T == 0 || v < TASKS) {
                 const int w = v >> 3, c = w % C8, rho = w / C8;
                 const u32x4 x0 = reg[set][k][0], x1 = reg[set][k][1];
-                any |= (x0.x | x0.y | x0.z | x0.w | x1.x | x1.y | x1.z | x1.w) & 0xf800f800u;
-                int base = 8 * c - ic;
-                base += base < 0 ? QX : 0;
-                const uint32_t w0[4] = {x0.x, x0.y, x0.z, x0.w}, w1[4] = {x1.x, x1.y, x1.z, x1.w};
+                const uint32_t r0[4] = {x0.x, x0.y, x0.z, x0.w}, r1[4] = {x1.x, x1.y, x1.z, x1.w};
+                uint32_t h0[4], h1[4];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    // (slot 2q, pixel e) in the low half, (slot 2q + 1, pixel e) in the high half
-                    const uint32_t d = __builtin_amdgcn_perm(w1[e >> 1], w0[e >> 1], (e & 1) ? 0x07060302u : 0x05040100u);
-                    const uint32_t hv = cvt_pair(d & mask);
-                    int beta = base + e;
-                    beta -= beta >= QX ? QX : 0;
-                    dst[ent(rho, beta, q >> 2) + (q & 3)] = hv;
-                    if (beta == 0) dst[ent(rho, QX, q >> 2) + (q & 3)] = hv;
+                for (int i = 0; i < 4; ++i) {
+                    if (!hi) any |= (r0[i] | r1[i]) & 0xfc00fc00u;
+                    h0[i] = hi ? part_hi(r0[i]) : part_lo(r0[i]);
+                    h1[i] = hi ? part_hi(r1[i]) : part_lo(r1[i]);
                 }
+                // column 8 c + e, slots 2q (low half) and 2q + 1 (high half): offsets 8 e from the piece's first entry
+                uint32_t *d = dst + ent(rho, 8 * c, q >> 2) + (q & 3);
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    d[8 * e] = __builtin_amdgcn_perm(h1[e >> 1], h0[e >> 1], (e & 1) ? 0x07060302u : 0x05040100u);
             }
         }
         return any != 0;
     };
+    int ic_buf[2] = {0, 0};                          // the column shift of the batch in each LDS buffer
     auto store = [&](int set, int b, int64_t bi) {
-        if (stage(set, slo[b], 0x07ff07ffu)) hiflag[b] = (int32_t)bi;   // every writer writes the same value
+        if (stage(set, slo[b], false)) hiflag[b] = (int32_t)bi;   // every writer writes the same value
 #pragma unroll
         for (int k = 0; k < BPER; ++k)
             if (BVB % T == 0 || tid + k * T < BVB) sb[b][tid + k * T] = breg[set][k];
+        ic_buf[b] = reg_ic[set];
     };
 
     float16v acc[2];
@@ -620,14 +627,6 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int r = 0; r < 16; ++r) { acc[t][r] = 0.f; accd[t][r] = 0.0; }
-    // A-operand addresses of the wave's tiles t, rows r and column corners cc (the same for every batch)
-    int aaddr[2][2][2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int cc = 0; cc < 2; ++cc) aaddr[t][r][cc] = ent(rw + r, 64 * wc + 32 * t + n + cc, hf);
     int nk = 0;
     auto compute = [&](int b, int64_t bi) {
         half8 bh[4], bl[4];
@@ -636,6 +635,16 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
             bh[c] = sb[b][(c * 2 + 0) * 64 + lane];
             bl[c] = sb[b][(c * 2 + 1) * 64 + lane];
         }
+        // A-operand addresses of tile t and column corner cc in staged row rw (row rw + 1 is RS further)
+        int aaddr[2][2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                int j = 64 * wc + 32 * t + n + cc + ic_buf[b];
+                j -= j >= QX ? QX : 0;
+                aaddr[t][cc] = ent(rw, j, hf);
+            }
         const bool use_hi = hiflag[b] == (int32_t)bi;   // uniform
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
@@ -644,7 +653,7 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
 #pragma unroll
                 for (int cc = 0; cc < 2; ++cc) {
                     const int c = r + 2 * cc;
-                    const half8 a = *(const half8 *)&slo[b][aaddr[t][r][cc]];
+                    const half8 a = *(const half8 *)&slo[b][aaddr[t][cc] + r * RS];
                     acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bh[c], acc[t], 0, 0, 0);
                     acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl[c], acc[t], 0, 0, 0);
                 }
@@ -654,7 +663,7 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
 #pragma unroll
                     for (int cc = 0; cc < 2; ++cc) {
                         const int c = r + 2 * cc;
-                        const half8 a = *(const half8 *)&shi[aaddr[t][r][cc]];
+                        const half8 a = *(const half8 *)&shi[aaddr[t][cc] + r * RS];
                         acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bh[c], acc[t], 0, 0, 0);
                         acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl[c], acc[t], 0, 0, 0);
                     }
@@ -678,7 +687,7 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
         if (b_begin + 1 < b_end) load(1, b_begin + 1);
         __syncthreads();
         if (hiflag[0] == (int32_t)b_begin) {                 // uniform
-            stage(0, shi, 0xf800f800u);
+            stage(0, shi, true);
             __syncthreads();
         }
         // step(bi, b, cur): batch bi is in LDS buffer b, batch bi + 1 in register set cur; the loads of batch bi + 2
@@ -689,7 +698,7 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
             if (bi + 1 < b_end) store(cur, b ^ 1, bi + 1);
             lds_barrier();
             if (bi + 1 < b_end && hiflag[b ^ 1] == (int32_t)(bi + 1)) {   // uniform: the high parts of batch bi + 1
-                stage(cur, shi, 0xf800f800u);
+                stage(cur, shi, true);
                 lds_barrier();
             }
         };

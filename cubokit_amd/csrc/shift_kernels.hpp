@@ -119,8 +119,8 @@ hipError_t launch_shift_grad_rows(const ShiftGradRows &a, hipStream_t s);
 // The same adjoint for rows of 64, 128, 192 or 256 pixels, with the positions in batches of 16 of one column shift ic
 // (the host's order, padded per shift): block (band, g) owns shift_grad_t_band(qx) output rows and the batches of
 // group g, each wave two 32-pixel tiles of one row. Per batch the band + 1 pattern rows each slot reads are staged
-// in LDS as FP16 lo (and, for batches with counts of 2048 or more, hi) parts, transposed to [row][pixel][slot] and
-// rolled by -ic, so each corner's A operand (8 positions of one pixel) is one aligned 16-byte read and the B
+// in LDS as FP16 lo (and, for batches with counts of 1024 or more, hi) parts, transposed to [row][pixel][slot], so
+// each corner's A operand (8 positions of one pixel, the column moved by ic) is one aligned 16-byte read and the B
 // operands are split_dy's, in batch order.
 struct ShiftGradT {
     const uint16_t *cube;
