@@ -320,9 +320,10 @@ int prepare_batches(cubo_hip_shift_plan *p) {
     return CUBO_HIP_OK;
 }
 
-// Of the group counts that give at least one block per compute unit, the one whose blocks (one of them per compute
-// unit at a time) fill their last round best.
-int group_count(int64_t bands, int64_t units, int cus) {
+// Of the group counts that give at least one block per block slot (per_cu of them on each compute unit), the one whose
+// blocks fill their last round best.
+int group_count(int64_t bands, int64_t units, int cus, int per_cu) {
+    cus *= per_cu;
     int ngroups = 1;
     double best = -1;
     for (int64_t ng = std::max<int64_t>(1, (cus + bands - 1) / bands); ng <= std::min<int64_t>(64, units); ++ng) {
@@ -536,7 +537,7 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
             if (int st = prepare_batches(p)) return st;
         const int64_t units = tposed ? p->nbatch : nks;
         const int band = tposed ? cubo::shift_grad_t_band(qx) : cubo::shift_grad_rows_band(qx);
-        const int ngroups = group_count((qy + band - 1) / band, units, cus);
+        const int ngroups = group_count((qy + band - 1) / band, units, cus, tposed ? cubo::shift_grad_t_blocks_per_cu(qx) : 1);
         const size_t dyw_bytes = (size_t)units * 4 * 2 * 64 * 8 * sizeof(_Float16);
         if (p->dyw_bytes < dyw_bytes) {
             if (p->dyw) (void)hipFree(p->dyw);

@@ -536,7 +536,8 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     __shared__ __align__(16) uint32_t slo[2][PLANE];
     __shared__ __align__(16) uint32_t shi[PLANE];
     __shared__ half8 sb[2][BVB];
-    __shared__ int32_t hiflag[2];
+    constexpr int WAVES = T / 64;
+    __shared__ int32_t hiflag[2][WAVES];             // per buffer and wave: the batch, if the wave saw high parts
     const int tid = threadIdx.x, lane = tid & 63, hf = lane >> 5, n = lane & 31;
     const int wave = tid >> 6, rw = wave / WPR, wc = wave % WPR;
     const int qy = g.qy;
@@ -556,7 +557,9 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     half8 breg[2][BPER];
     int reg_ic[2] = {0, 0};
     u32x4 rec = {0u, 0u, 0u, 0u};                    // slot records of the next batch to load
-    auto fetch_rec = [&](int64_t bi) { if (bi < b_end) rec = slots[bi * 8 + q]; };
+    // Loads are unconditional (a batch past the group re-reads the group's last one), so every wait counts the same
+    // loads on every path and never waits for the prefetch issued after the loads it needs.
+    auto fetch_rec = [&](int64_t bi) { rec = slots[min(bi, b_end - 1) * 8 + q]; };
     auto load = [&](int set, int64_t bi) {
         const int ir0 = (int)rec.y, ir1 = (int)rec.w;
         const CUBO_GLOBAL uint16_t *x0 = cube + (int64_t)rec.x * ndet, *x1 = cube + (int64_t)rec.z * ndet;
@@ -586,39 +589,51 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
         return __builtin_bit_cast(uint32_t, r);
     };
     auto part_hi = [](uint32_t w) { return cvt_pair(w & 0xfc00fc00u); };
-    // One plane of the staged rows from register set `set`: true if a count of 1024 or more was seen.
-    auto stage = [&](int set, uint32_t *dst, bool hi) {
-        uint32_t any = 0;
+    // Task k of this thread into one plane of the staged rows from register set `set`; returns the bits of counts
+    // of 1024 or more.
+    auto stage_task = [&](int set, int k, uint32_t *dst, bool hi) {
+        const int v = tid + k * T;
+        const int w = v >> 3, c = w % C8, rho = w / C8;
+        const u32x4 x0 = reg[set][k][0], x1 = reg[set][k][1];
+        const uint32_t r0[4] = {x0.x, x0.y, x0.z, x0.w}, r1[4] = {x1.x, x1.y, x1.z, x1.w};
+        uint32_t h0[4], h1[4], any = 0;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int v = tid + k * T;
-            if (TASKS % T == 0 || v < TASKS) {
-                const int w = v >> 3, c = w % C8, rho = w / C8;
-                const u32x4 x0 = reg[set][k][0], x1 = reg[set][k][1];
-                const uint32_t r0[4] = {x0.x, x0.y, x0.z, x0.w}, r1[4] = {x1.x, x1.y, x1.z, x1.w};
-                uint32_t h0[4], h1[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    if (!hi) any |= (r0[i] | r1[i]) & 0xfc00fc00u;
-                    h0[i] = hi ? part_hi(r0[i]) : part_lo(r0[i]);
-                    h1[i] = hi ? part_hi(r1[i]) : part_lo(r1[i]);
-                }
-                // column 8 c + e, slots 2q (low half) and 2q + 1 (high half): offsets 8 e from the piece's first entry
-                uint32_t *d = dst + ent(rho, 8 * c, q >> 2) + (q & 3);
-#pragma unroll
-                for (int e = 0; e < 8; ++e)
-                    d[8 * e] = __builtin_amdgcn_perm(h1[e >> 1], h0[e >> 1], (e & 1) ? 0x07060302u : 0x05040100u);
-            }
+        for (int i = 0; i < 4; ++i) {
+            any |= (r0[i] | r1[i]) & 0xfc00fc00u;
+            h0[i] = hi ? part_hi(r0[i]) : part_lo(r0[i]);
+            h1[i] = hi ? part_hi(r1[i]) : part_lo(r1[i]);
         }
-        return any != 0;
+        // column 8 c + e, slots 2q (low half) and 2q + 1 (high half): offsets 8 e from the piece's first entry
+        uint32_t *d = dst + ent(rho, 8 * c, q >> 2) + (q & 3);
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            d[8 * e] = __builtin_amdgcn_perm(h1[e >> 1], h0[e >> 1], (e & 1) ? 0x07060302u : 0x05040100u);
+        return any;
+    };
+    constexpr int KFULL = TASKS / T;                 // tasks every thread has; the last one only the first threads
+    // The whole plane: the full tasks, then the partial one (a branch uniform over waves, as T is whole waves and
+    // the partial task count TASKS % T is too).
+    static_assert((TASKS % T) % 64 == 0, "the partial task covers whole waves");
+    auto stage_rest = [&](int set, uint32_t *dst, bool hi) {
+        uint32_t any = 0;
+        if (TASKS % T != 0 && tid + KFULL * T < TASKS) any |= stage_task(set, KFULL, dst, hi);
+        return any;
     };
     int ic_buf[2] = {0, 0};                          // the column shift of the batch in each LDS buffer
-    auto store = [&](int set, int b, int64_t bi) {
-        if (stage(set, slo[b], false)) hiflag[b] = (int32_t)bi;   // every writer writes the same value
+    // The batch's B operands and column shift, and the wave's high-part flag (every lane writes the same value).
+    auto store_misc = [&](int set, int b, int64_t bi, uint32_t any) {
 #pragma unroll
         for (int k = 0; k < BPER; ++k)
             if (BVB % T == 0 || tid + k * T < BVB) sb[b][tid + k * T] = breg[set][k];
         ic_buf[b] = reg_ic[set];
+        hiflag[b][wave] = __builtin_amdgcn_ballot_w64(any != 0) != 0 ? (int32_t)bi : -1;
+    };
+    // Whether the batch bi in buffer b has high parts (uniform).
+    auto has_hi = [&](int b, int64_t bi) {
+        bool h = false;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) h |= hiflag[b][w] == (int32_t)bi;
+        return h;
     };
 
     float16v acc[2];
@@ -628,15 +643,16 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) { acc[t][r] = 0.f; accd[t][r] = 0.0; }
     int nk = 0;
-    auto compute = [&](int b, int64_t bi) {
-        half8 bh[4], bl[4];
+    half8 bh[4], bl[4];
+    int aaddr[2][2];
+    // The B operands and A addresses of the batch in buffer b.
+    auto operands = [&](int b) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             bh[c] = sb[b][(c * 2 + 0) * 64 + lane];
             bl[c] = sb[b][(c * 2 + 1) * 64 + lane];
         }
         // A-operand addresses of tile t and column corner cc in staged row rw (row rw + 1 is RS further)
-        int aaddr[2][2];
 #pragma unroll
         for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -645,30 +661,19 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
                 j -= j >= QX ? QX : 0;
                 aaddr[t][cc] = ent(rw, j, hf);
             }
-        const bool use_hi = hiflag[b] == (int32_t)bi;   // uniform
+    };
+    auto mfma_tile = [&](const uint32_t *plane, int t) {
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int r = 0; r < 2; ++r)
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
-#pragma unroll
-                for (int cc = 0; cc < 2; ++cc) {
-                    const int c = r + 2 * cc;
-                    const half8 a = *(const half8 *)&slo[b][aaddr[t][cc] + r * RS];
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bh[c], acc[t], 0, 0, 0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl[c], acc[t], 0, 0, 0);
-                }
-            if (use_hi) {
-#pragma unroll
-                for (int r = 0; r < 2; ++r)
-#pragma unroll
-                    for (int cc = 0; cc < 2; ++cc) {
-                        const int c = r + 2 * cc;
-                        const half8 a = *(const half8 *)&shi[aaddr[t][cc] + r * RS];
-                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bh[c], acc[t], 0, 0, 0);
-                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl[c], acc[t], 0, 0, 0);
-                    }
+            for (int cc = 0; cc < 2; ++cc) {
+                const int c = r + 2 * cc;
+                const half8 a = *(const half8 *)&plane[aaddr[t][cc] + r * RS];
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bh[c], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl[c], acc[t], 0, 0, 0);
             }
-        }
+    };
+    auto flush = [&]() {
         if (++nk == 2) {
 #pragma unroll
             for (int t = 0; t < 2; ++t)
@@ -678,36 +683,66 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
         }
     };
 
-    if (tid < 2) hiflag[tid] = -1;
-    __syncthreads();
     if (b_begin < b_end) {
         fetch_rec(b_begin);
         load(0, b_begin);
-        store(0, 0, b_begin);
-        if (b_begin + 1 < b_end) load(1, b_begin + 1);
+        load(1, min(b_begin + 1, b_end - 1));
+        {
+            uint32_t any = 0;
+#pragma unroll
+            for (int k = 0; k < KFULL; ++k) any |= stage_task(0, k, slo[0], false);
+            any |= stage_rest(0, slo[0], false);
+            store_misc(0, 0, b_begin, any);
+        }
         __syncthreads();
-        if (hiflag[0] == (int32_t)b_begin) {                 // uniform
-            stage(0, shi, true);
+        if (has_hi(0, b_begin)) {
+#pragma unroll
+            for (int k = 0; k < KFULL; ++k) stage_task(0, k, shi, true);
+            stage_rest(0, shi, true);
             __syncthreads();
         }
-        // step(bi, b, cur): batch bi is in LDS buffer b, batch bi + 1 in register set cur; the loads of batch bi + 2
-        // go to the other set while batch bi is multiplied
-        auto step = [&](int64_t bi, int b, int cur) {
-            if (bi + 2 < b_end) load(cur ^ 1, bi + 2);       // uniform
-            compute(b, bi);
-            if (bi + 1 < b_end) store(cur, b ^ 1, bi + 1);
+        // step(bi, b, cur): batch bi is in LDS buffer b, batch bi + 1 (if STORE) in register set cur; the loads of
+        // batch bi + 2 go to the other set while batch bi is multiplied. The staging of batch bi + 1 sits between the
+        // matrix-core instructions of batch bi (one basic block), so the two overlap.
+        auto step = [&](int64_t bi, int b, int cur, bool store) {
+            load(cur ^ 1, min(bi + 2, b_end - 1));
+            operands(b);
+            const bool use_hi = has_hi(b, bi);
+            uint32_t any = 0;
+            mfma_tile(slo[b], 0);
+            if (store) {
+#pragma unroll
+                for (int k = 0; k < KFULL; ++k) any |= stage_task(cur, k, slo[b ^ 1], false);
+            }
+            mfma_tile(slo[b], 1);
+            if (store) {
+                any |= stage_rest(cur, slo[b ^ 1], false);
+                store_misc(cur, b ^ 1, bi + 1, any);
+            }
+            if (use_hi) {
+                mfma_tile(shi, 0);
+                mfma_tile(shi, 1);
+            }
+            flush();
             lds_barrier();
-            if (bi + 1 < b_end && hiflag[b ^ 1] == (int32_t)(bi + 1)) {   // uniform: the high parts of batch bi + 1
-                stage(cur, shi, true);
+            if (store && has_hi(b ^ 1, bi + 1)) {           // the high parts of batch bi + 1
+#pragma unroll
+                for (int k = 0; k < KFULL; ++k) stage_task(cur, k, shi, true);
+                stage_rest(cur, shi, true);
                 lds_barrier();
             }
         };
         int64_t bi = b_begin;
-        for (; bi + 1 < b_end; bi += 2) {
-            step(bi, 0, 1);
-            step(bi + 1, 1, 0);
+        for (; bi + 2 < b_end; bi += 2) {
+            step(bi, 0, 1, true);
+            step(bi + 1, 1, 0, true);
         }
-        if (bi < b_end) step(bi, 0, 1);
+        if (bi + 1 < b_end) {
+            step(bi, 0, 1, true);
+            step(bi + 1, 1, 0, false);
+        } else if (bi < b_end) {
+            step(bi, 0, 1, false);
+        }
     }
 #pragma unroll
     for (int t = 0; t < 2; ++t)
@@ -837,6 +872,9 @@ hipError_t launch_split_dy(const double *dy, const double *wr, const double *wc,
     return hipGetLastError();
 }
 
+// Output rows per block: blocks of 8 waves (6 for rows of 192 pixels), one per compute unit. (Blocks of 4 waves,
+// two per compute unit, bands of half the rows: 3.85 instead of 3.55 ms on 128-pixel rows, the pattern rows re-read
+// 1.5 instead of 1.25 times.)
 int shift_grad_t_band(int qx) {
     switch (qx) {
         case 64: return 8;
@@ -846,6 +884,8 @@ int shift_grad_t_band(int qx) {
         default: return 0;
     }
 }
+
+int shift_grad_t_blocks_per_cu(int) { return 1; }
 
 bool shift_grad_t_supported(int qy, int qx) { return shift_grad_t_band(qx) > 0 && qy >= shift_grad_t_band(qx); }
 

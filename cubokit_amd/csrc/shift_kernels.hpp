@@ -136,6 +136,8 @@ struct ShiftGradT {
 };
 // Output rows per block (0: not supported); the kernel needs qy >= that many rows.
 int shift_grad_t_band(int qx);
+// Blocks of shift_grad_t that run at once on a compute unit.
+int shift_grad_t_blocks_per_cu(int qx);
 bool shift_grad_t_supported(int qy, int qx);
 hipError_t launch_shift_grad_t(const ShiftGradT &a, hipStream_t s);
 // out[i] = sum_g partial[g][i] in group order, i < n.
