@@ -537,7 +537,8 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     __shared__ __align__(16) uint32_t shi[PLANE];
     __shared__ half8 sb[2][BVB];
     constexpr int WAVES = T / 64;
-    __shared__ int32_t hiflag[2][WAVES];             // per buffer and wave: the batch, if the wave saw high parts
+    static_assert(WAVES <= 8, "the flags are read as two 16-byte words");
+    __shared__ __align__(16) int32_t hiflag[2][8];   // per buffer and wave: the batch, if the wave saw high parts
     const int tid = threadIdx.x, lane = tid & 63, hf = lane >> 5, n = lane & 31;
     const int wave = tid >> 6, rw = wave / WPR, wc = wave % WPR;
     const int qy = g.qy;
@@ -560,17 +561,25 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     // Loads are unconditional (a batch past the group re-reads the group's last one), so every wait counts the same
     // loads on every path and never waits for the prefetch issued after the loads it needs.
     auto fetch_rec = [&](int64_t bi) { rec = slots[min(bi, b_end - 1) * 8 + q]; };
+    // (a0 + rho) mod qy of each task (rho <= RB <= qy); a row a0 + rho + ir is then one subtraction away
+    uint32_t arho[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int rho = ((tid + k * T) >> 3) / C8;
+        arho[k] = (uint32_t)(a0 + rho >= qy ? a0 + rho - qy : a0 + rho);
+    }
     auto load = [&](int set, int64_t bi) {
-        const int ir0 = (int)rec.y, ir1 = (int)rec.w;
+        const uint32_t ir0 = rec.y, ir1 = rec.w;
         const CUBO_GLOBAL uint16_t *x0 = cube + (int64_t)rec.x * ndet, *x1 = cube + (int64_t)rec.z * ndet;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int v = tid + k * T;
             if (TASKS % T == 0 || v < TASKS) {
-                const int w = v >> 3, c = w % C8, rho = w / C8;
-                int r0 = a0 + ir0 + rho, r1 = a0 + ir1 + rho;   // < 3 qy: a0, ir < qy and rho <= RB <= qy
-                r0 -= r0 >= qy ? qy : 0; r0 -= r0 >= qy ? qy : 0;
-                r1 -= r1 >= qy ? qy : 0; r1 -= r1 >= qy ? qy : 0;
+                const int c = (v >> 3) % C8;
+                // r < 2 qy; min(r, r - qy) as unsigned is r mod qy
+                uint32_t r0 = arho[k] + ir0, r1 = arho[k] + ir1;
+                r0 = min(r0, r0 - (uint32_t)qy);
+                r1 = min(r1, r1 - (uint32_t)qy);
                 reg[set][k][0] = *(const CUBO_GLOBAL u32x4 *)(x0 + r0 * QX + 8 * c);
                 reg[set][k][1] = *(const CUBO_GLOBAL u32x4 *)(x1 + r1 * QX + 8 * c);
             }
@@ -630,9 +639,11 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     };
     // Whether the batch bi in buffer b has high parts (uniform).
     auto has_hi = [&](int b, int64_t bi) {
+        const u32x4 f0 = *(const u32x4 *)&hiflag[b][0], f1 = *(const u32x4 *)&hiflag[b][4];
+        const uint32_t f[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
         bool h = false;
 #pragma unroll
-        for (int w = 0; w < WAVES; ++w) h |= hiflag[b][w] == (int32_t)bi;
+        for (int w = 0; w < WAVES; ++w) h |= f[w] == (uint32_t)bi;
         return h;
     };
 
