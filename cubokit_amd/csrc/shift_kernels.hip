@@ -9,7 +9,9 @@
 //   forward   Y   = sum_c w_c C_c
 //   backward  dY/dwr = (1 - wc)(C_1 - C_0) + wc (C_3 - C_2),  dY/dwc = (1 - wr)(C_2 - C_0) + wr (C_3 - C_1),
 //             g_r(p) = sum_k dL/dY[p, k] dY/dwr[p, k]   (d wr / d origin_r = 1 inside a bilinear cell).
-// Mask adjoint: a second pass, pixel-major on the vector ALUs (see shift_grad_w).
+// Mask adjoint: a second pass with the positions as the reduction dimension: shift_grad_t (rows of 64-256 pixels,
+// batches of one column shift staged transposed in LDS), shift_grad_rows (other multiples of 32) or shift_grad_w
+// (any width, vector ALUs).
 #include "device_sync.hpp"
 #include "shift_kernels.hpp"
 
