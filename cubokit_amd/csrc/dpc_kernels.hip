@@ -1,9 +1,11 @@
 // dpc_kernels.hip — FP64 FFT and the iterative-DPC phase step (Sources/CuboPhase/DPC.swift:26-102,
 // Sources/CuboCore/DoubleFFT.swift:119-195 for the FFT conventions) and its vector-Jacobian product.
 //
-// FFT: one block per row, the row in LDS. Lengths that are powers of two run Stockham radix-2 stages
-// (no bit reversal: each stage reads one LDS buffer and writes the other in natural order); other lengths a
-// direct DFT from the twiddle table. 2-D transforms are rows, a tiled transpose, rows, a transpose back.
+// FFT: one block per row, the row in LDS. Lengths that are powers of two run Stockham radix-4 stages (after one
+// radix-2 stage for odd powers; no bit reversal: each stage reads one LDS buffer and writes the other in natural
+// order); other lengths a direct DFT from the twiddle table. 2-D transforms of powers of two are two passes of
+// fft_rows_t (several rows per block, the result stored transposed); others rows, a tiled transpose, rows, a
+// transpose back.
 // Twiddles exp(-2 pi i m / n) come from the host in Double.
 #include "dpc_kernels.hpp"
 
@@ -81,6 +83,84 @@ __global__ __launch_bounds__(256) void fft_rows(double2 *data, int n, const doub
         double2 *t = a; a = b; b = t;
     }
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = a[i];
+}
+
+// R rows of a power-of-two length n per block, each transformed in LDS by the same Stockham stages as fft_rows,
+// the results stored transposed: out[j][r0 + r] = scale X_r[j] (out is n x rows), for batch blockIdx.y (in: rows x n,
+// both arrays batch-strided by rows * n). Two such passes are a 2-D transform with no transpose kernels. LDS rows are
+// padded by 16 / R entries, so the transposed reads (R rows at each of 16 / R consecutive j per 16 lanes) fall on
+// distinct banks.
+template <bool INV, int R>
+__global__ __launch_bounds__(512) void fft_rows_t(const double2 *in, double2 *out, int n, int rows, const double2 *tw,
+                                                  double scale) {
+    extern __shared__ double2 sh[];
+    const int S = n + 16 / R;
+    double2 *a = sh, *b = sh + R * S;
+    const int r0 = blockIdx.x * R, lg = __builtin_ctz((unsigned)n);
+    const int64_t boff = (int64_t)blockIdx.y * rows * n;
+    const double2 *x = in + boff + (int64_t)r0 * n;
+    for (int v = threadIdx.x; v < R * n; v += blockDim.x) a[(v >> lg) * S + (v & (n - 1))] = x[v];
+    __syncthreads();
+    int ns = 1;
+    if (lg & 1) {
+        const int half = n >> 1;
+        for (int v = threadIdx.x; v < R * half; v += blockDim.x) {
+            const int r = v >> (lg - 1), j = v & (half - 1);
+            const double2 u = a[r * S + j], w = a[r * S + j + half];
+            b[r * S + 2 * j] = make_double2(u.x + w.x, u.y + w.y);
+            b[r * S + 2 * j + 1] = make_double2(u.x - w.x, u.y - w.y);
+        }
+        __syncthreads();
+        double2 *t = a; a = b; b = t;
+        ns = 2;
+    }
+    const int quarter = n >> 2;
+    for (; ns < n; ns <<= 2) {
+        const int step = quarter / ns;
+        for (int v = threadIdx.x; v < R * quarter; v += blockDim.x) {
+            const int r = v >> (lg - 2), j = v & (quarter - 1), k = j & (ns - 1);
+            const double2 *ar = a + r * S;
+            double2 *br = b + r * S;
+            double2 w1 = tw[k * step], w2 = tw[2 * k * step], w3 = tw[3 * k * step];
+            if (INV) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+            const double2 x0 = ar[j], x1 = cmul(ar[j + quarter], w1), x2 = cmul(ar[j + 2 * quarter], w2),
+                          x3 = cmul(ar[j + 3 * quarter], w3);
+            const double2 s02 = make_double2(x0.x + x2.x, x0.y + x2.y), d02 = make_double2(x0.x - x2.x, x0.y - x2.y);
+            const double2 s13 = make_double2(x1.x + x3.x, x1.y + x3.y), d13 = make_double2(x1.x - x3.x, x1.y - x3.y);
+            const double2 r13 = INV ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
+            const int o = ((j - k) << 2) + k;
+            br[o] = make_double2(s02.x + s13.x, s02.y + s13.y);
+            br[o + ns] = make_double2(d02.x + r13.x, d02.y + r13.y);
+            br[o + 2 * ns] = make_double2(s02.x - s13.x, s02.y - s13.y);
+            br[o + 3 * ns] = make_double2(d02.x - r13.x, d02.y - r13.y);
+        }
+        __syncthreads();
+        double2 *t = a; a = b; b = t;
+    }
+    double2 *y = out + boff + r0;
+    for (int v = threadIdx.x; v < R * n; v += blockDim.x) {
+        const int r = v % R, j = v / R;
+        const double2 e = a[r * S + j];
+        y[(int64_t)j * rows + r] = make_double2(e.x * scale, e.y * scale);
+    }
+}
+
+// Rows per block of fft_rows_t for a row length: two LDS buffers of R padded rows within 160 KiB.
+int fft_rows_t_rows(int n) { return n <= 1024 ? 4 : 2; }
+
+// One pass of fft_rows_t over batch arrays of rows x n, or false if the shapes do not suit it.
+bool launch_fft_rows_t(const double2 *in, double2 *out, int batch, int rows, int n, const double2 *tw, bool inverse,
+                       double scale, hipStream_t s) {
+    if (n < 4 || (n & (n - 1)) || n > kFftMax) return false;
+    const int R = fft_rows_t_rows(n);
+    if (rows % R) return false;
+    const size_t lds = 2 * (size_t)R * (n + 16 / R) * sizeof(double2);
+    const dim3 grid((unsigned)(rows / R), (unsigned)batch);
+#define CUBO_RT(I, RR) hipLaunchKernelGGL((fft_rows_t<I, RR>), grid, dim3(512), lds, s, in, out, n, rows, tw, scale)
+    if (R == 4) { if (inverse) CUBO_RT(true, 4); else CUBO_RT(false, 4); }
+    else { if (inverse) CUBO_RT(true, 2); else CUBO_RT(false, 2); }
+#undef CUBO_RT
+    return true;
 }
 
 // out[c][r] = scale * in[r][c], for each of gridDim.z arrays of rows x cols
@@ -433,6 +513,14 @@ hipError_t launch_fft2_batched(double2 *data, double2 *tmp, int batch, int rows,
                                const double2 *tw_c, bool inverse, hipStream_t s) {
     if (rows > kFftMax || cols > kFftMax || rows < 1 || cols < 1 || batch < 1 || batch > 65535)
         return hipErrorInvalidValue;
+    const double scale = inverse ? 1.0 / ((double)rows * cols) : 1.0;
+    // powers of two whose row counts divide into the blocks: two passes of rows stored transposed
+    const auto fits = [](int n, int m) { return n >= 4 && !(n & (n - 1)) && m % fft_rows_t_rows(n) == 0; };
+    if (fits(cols, rows) && fits(rows, cols)) {
+        launch_fft_rows_t(data, tmp, batch, rows, cols, tw_c, inverse, 1.0, s);
+        launch_fft_rows_t(tmp, data, batch, cols, rows, tw_r, inverse, scale, s);
+        return hipGetLastError();
+    }
     const size_t lds_c = 2 * (size_t)cols * sizeof(double2), lds_r = 2 * (size_t)rows * sizeof(double2);
     const unsigned nr = (unsigned)((int64_t)batch * rows), nc = (unsigned)((int64_t)batch * cols);
     if (inverse) hipLaunchKernelGGL(fft_rows<true>, dim3(nr), dim3(256), lds_c, s, data, cols, tw_c);
@@ -441,7 +529,6 @@ hipError_t launch_fft2_batched(double2 *data, double2 *tmp, int batch, int rows,
                        1.0);
     if (inverse) hipLaunchKernelGGL(fft_rows<true>, dim3(nc), dim3(256), lds_r, s, tmp, rows, tw_r);
     else hipLaunchKernelGGL(fft_rows<false>, dim3(nc), dim3(256), lds_r, s, tmp, rows, tw_r);
-    const double scale = inverse ? 1.0 / ((double)rows * cols) : 1.0;
     hipLaunchKernelGGL(transpose, dim3((rows + 31) / 32, (cols + 31) / 32, batch), dim3(256), 0, s, tmp, data, cols, rows,
                        scale);
     return hipGetLastError();
