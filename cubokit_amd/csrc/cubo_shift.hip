@@ -53,6 +53,9 @@ struct cubo_hip_shift_plan {
     void *wroll = nullptr; size_t wroll_slots = 0;
     int32_t *perm = nullptr, *shift = nullptr, *slot_shifts = nullptr;
     cubo::ShiftTile *tiles = nullptr; size_t tiles_cap = 0;
+    // split tiles (the classes' partial tiles, each over nsplit chunk ranges) after the whole ones
+    int first_split = 0, split_groups = 0, nsplit = 1;
+    double *part = nullptr; size_t part_cap = 0;
     double *wr = nullptr, *wc = nullptr;
     float *wts = nullptr;
     // per-call buffers
@@ -83,7 +86,7 @@ void free_plan(cubo_hip_shift_plan *p) {
     if (!p) return;
     DeviceGuard g(p->cube->device);
     (void)hipStreamSynchronize(p->cube->stream);
-    void *bufs[] = {p->base, p->inv, p->wroll, p->perm, p->shift, p->slot_shifts, p->tiles, p->wr, p->wc, p->wts,
+    void *bufs[] = {p->base, p->inv, p->wroll, p->perm, p->shift, p->slot_shifts, p->tiles, p->part, p->wr, p->wc, p->wts,
                     p->dy, p->g, p->dy32, p->partial, p->dmask, p->vmasks, p->vshift, p->vwr, p->vwc,
                     p->dyw, p->dy_inv, p->rpartial, p->rdmask, p->bslots, p->bic, p->border};
     for (void *b : bufs)
@@ -220,11 +223,39 @@ int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol, bool
             tl.start = (int32_t)t;
             tl.count = (int32_t)std::min<int64_t>(TP, e - t);
             tl.slot[0] = s0; tl.slot[1] = s1; tl.slot[2] = s2; tl.slot[3] = s3;
+            tl.c0 = 0;
+            tl.c1 = p->nchunks;
+            tl.part = -1;
             tiles.push_back(tl);
         }
     }
-    const int nslots = (int)slot_shifts.size() / 2;
     DeviceGuard g(c->device);
+    // One block per compute unit at a time: when the whole tiles fill at least one round, the classes' partial
+    // tiles would add a round of their own, so each is split into nsplit chunk ranges (their partial outputs added
+    // afterwards in split order) and they run after the whole tiles.
+    const int cus = cubo::compute_units(c->device);
+    {
+        std::vector<cubo::ShiftTile> whole, partial;
+        for (const auto &t : tiles) (t.count == TP ? whole : partial).push_back(t);
+        p->first_split = p->split_groups = 0;
+        p->nsplit = 1;
+        if ((int64_t)whole.size() >= cus && !partial.empty() && p->nchunks >= 2) {
+            const int ns = std::min(16, p->nchunks);
+            tiles = whole;
+            p->first_split = (int)whole.size();
+            p->split_groups = (int)partial.size();
+            p->nsplit = ns;
+            for (size_t gi = 0; gi < partial.size(); ++gi)
+                for (int sp = 0; sp < ns; ++sp) {
+                    cubo::ShiftTile tl = partial[gi];
+                    tl.c0 = (int32_t)((int64_t)p->nchunks * sp / ns);
+                    tl.c1 = (int32_t)((int64_t)p->nchunks * (sp + 1) / ns);
+                    tl.part = (int32_t)(gi * ns + sp);
+                    tiles.push_back(tl);
+                }
+        }
+    }
+    const int nslots = (int)slot_shifts.size() / 2;
     hipError_t e = hipSuccess;
     size_t cap_tiles = p->tiles_cap;
     if (!p->perm) {
@@ -238,6 +269,8 @@ int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol, bool
     }
     e = grow(&p->tiles, &cap_tiles, tiles.size());
     p->tiles_cap = cap_tiles;
+    if (e == hipSuccess && p->split_groups > 0)
+        e = grow(&p->part, &p->part_cap, (size_t)p->split_groups * p->nsplit * TP * std::max(p->k, 2));
     if (e == hipSuccess) e = grow((char **)&p->wroll, &p->wroll_slots, (size_t)nslots * p->slot_stride * 16);
     if (e != hipSuccess) return hip_fail(e, "allocating the rolled weights");
     hipStream_t s = c->stream;
@@ -351,7 +384,16 @@ cubo::ShiftLaunch launch_args(const cubo_hip_shift_plan *p) {
     a.perm = p->perm;
     a.wr = p->wr;
     a.wc = p->wc;
+    a.part = p->part;
     return a;
+}
+
+// The stack kernel over every tile, then the split tiles' partial outputs added.
+hipError_t run_stack(const cubo_hip_shift_plan *p, const cubo::ShiftLaunch &a, bool backward, bool rounded,
+                     hipStream_t s) {
+    hipError_t e = cubo::launch_shift_stack(a, p->ntiles, backward, rounded, s);
+    if (e == hipSuccess) e = cubo::launch_shift_split_sum(a, p->first_split, p->split_groups, p->nsplit, backward, s);
+    return e;
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -471,7 +513,7 @@ int forward(cubo_hip_shift_plan *p, const double *origin_row, const double *orig
     cubo::ShiftLaunch a = launch_args(p);
     a.y = p->y;
     CUBO_HIP_TRY(hipEventRecord(p->ev[0], c->stream), "hipEventRecord");
-    CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, false, rounded, c->stream), "launching the shifted stack");
+    CUBO_HIP_TRY(run_stack(p, a, false, rounded, c->stream), "launching the shifted stack");
     CUBO_HIP_TRY(hipEventRecord(p->ev[1], c->stream), "hipEventRecord");
     CUBO_HIP_TRY(hipStreamSynchronize(c->stream), "waiting for the shifted stack");
     copy_y(p, y, ny);
@@ -516,7 +558,7 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
         a.gr = p->g;
         a.gc = p->g + npos;
         CUBO_HIP_TRY(hipEventRecord(p->ev[2], s), "hipEventRecord");
-        CUBO_HIP_TRY(cubo::launch_shift_stack(a, p->ntiles, true, false, s), "launching the origin adjoint");
+        CUBO_HIP_TRY(run_stack(p, a, true, false, s), "launching the origin adjoint");
         CUBO_HIP_TRY(hipEventRecord(p->ev[3], s), "hipEventRecord");
     }
     if (dmasks && cubo::shift_grad_rows_supported((int)c->shape[3])) {

@@ -57,6 +57,7 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
     __shared__ double wts[2][TP];
 
     const ShiftTile t = a.tiles[blockIdx.x];
+    const int c0 = t.c0, c1 = t.c1;                            // the tile's chunk range
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int q = wave * 32 + (lane & 31);                     // this lane's position slot (A-operand row)
     const bool live = q < t.count;
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
                 }
             }
         }
-        if ((chunk + 1) % kFlush == 0 || chunk + 1 == nchunks) flush();
+        if ((chunk + 1) % kFlush == 0 || chunk + 1 == c1) flush();
     };
 
     // As the mask stack: weights through LDS (double-buffered, one barrier per chunk), data two chunks ahead in
@@ -185,32 +186,33 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
     // step c.
     u32x4 X0[4], X1[4], X2[4];
     if constexpr (RING) {
-        // the last D chunks into the slots of chunks -D .. -1, before the first data loads
+        // the D chunks before c0 (the last D chunks of the pattern for c0 = 0: the roll wraps) into the slots of
+        // chunks c0 - D .. c0 - 1, before the first data loads
 #pragma unroll
         for (int v = -D; v < 0; ++v) {
-            fetch_w(0, ((v % nchunks) + nchunks) % nchunks);
-            put_w(0, buf_of(v));
+            fetch_w(0, (((c0 + v) % nchunks) + nchunks) % nchunks);
+            put_w(0, buf_of(c0 + v));
         }
     }
-    fetch_w(0, 0);
+    fetch_w(0, c0);
     __builtin_amdgcn_sched_barrier(0);
-    load_x(X0, 0);
-    load_x(X1, min(1, nchunks - 1));
-    put_w(0, buf_of(0));
-    if constexpr (DEEP) fetch_w(1, min(1, nchunks - 1));
+    load_x(X0, c0);
+    load_x(X1, min(c0 + 1, c1 - 1));
+    put_w(0, buf_of(c0));
+    if constexpr (DEEP) fetch_w(1, min(c0 + 1, c1 - 1));
     __syncthreads();                                   // also publishes wts
-    // cur: the register set holding chunk c + 1 (the forward alternates: set (c + 1) & 1)
+    // cur: the register set holding chunk c + 1 (the forward alternates: set (c + 1 - c0) & 1)
     auto step = [&](const u32x4 (&xc)[4], u32x4 (&xn)[4], int c, int cur) {
-        if constexpr (DEEP) fetch_w(cur ^ 1, min(c + 2, nchunks - 1));
-        else fetch_w(0, min(c + 1, nchunks - 1));
+        if constexpr (DEEP) fetch_w(cur ^ 1, min(c + 2, c1 - 1));
+        else fetch_w(0, min(c + 1, c1 - 1));
         __builtin_amdgcn_sched_barrier(0);
-        load_x(xn, min(c + 2, nchunks - 1));
+        load_x(xn, min(c + 2, c1 - 1));
         compute(xc, c);
         put_w(DEEP ? cur : 0, buf_of(c + 1));
         lds_barrier();
     };
-    int c = 0;
-    for (; c + 6 <= nchunks; c += 6) {
+    int c = c0;
+    for (; c + 6 <= c1; c += 6) {
         step(X0, X2, c, 1);
         step(X1, X0, c + 1, 0);
         step(X2, X1, c + 2, 1);
@@ -218,11 +220,11 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
         step(X1, X0, c + 4, 1);
         step(X2, X1, c + 5, 0);
     }
-    if (c < nchunks) step(X0, X2, c, 1);
-    if (c + 1 < nchunks) step(X1, X0, c + 1, 0);
-    if (c + 2 < nchunks) step(X2, X1, c + 2, 1);
-    if (c + 3 < nchunks) step(X0, X2, c + 3, 0);
-    if (c + 4 < nchunks) step(X1, X0, c + 4, 1);
+    if (c < c1) step(X0, X2, c, 1);
+    if (c + 1 < c1) step(X1, X0, c + 1, 0);
+    if (c + 2 < c1) step(X2, X1, c + 2, 1);
+    if (c + 3 < c1) step(X0, X2, c + 3, 0);
+    if (c + 4 < c1) step(X1, X0, c + 4, 1);
 
     const int col = lane & 31;
     const double inv = col < a.k ? ((const CUBO_GLOBAL double *)a.inv_scale)[col] : 0.0;
@@ -245,11 +247,45 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
                 g1 += __shfl_xor(g1, off, 64);
             }
             if (rl && col == 0) {
-                ((CUBO_GLOBAL double *)a.gr)[pr] = g0;
-                ((CUBO_GLOBAL double *)a.gc)[pr] = g1;
+                if (t.part < 0) {
+                    ((CUBO_GLOBAL double *)a.gr)[pr] = g0;
+                    ((CUBO_GLOBAL double *)a.gc)[pr] = g1;
+                } else {                               // one chunk range of a split tile
+                    ((CUBO_GLOBAL double *)a.part)[((int64_t)t.part * 256 + row) * 2] = g0;
+                    ((CUBO_GLOBAL double *)a.part)[((int64_t)t.part * 256 + row) * 2 + 1] = g1;
+                }
             }
         } else {
-            if (rl && col < a.k) ((CUBO_GLOBAL double *)a.y)[pr * a.k + col] = accd[0][r] * inv;
+            if (rl && col < a.k) {
+                if (t.part < 0) ((CUBO_GLOBAL double *)a.y)[pr * a.k + col] = accd[0][r] * inv;
+                else ((CUBO_GLOBAL double *)a.part)[((int64_t)t.part * 256 + row) * a.k + col] = accd[0][r] * inv;
+            }
+        }
+    }
+}
+
+// The split tiles' partial outputs added in split order: for split tile group g (tiles first_split + g nsplit ..,
+// one position range), Y[p][k] (forward) or gr, gc[p] (adjoint) of each of its positions.
+__global__ void shift_split_sum(ShiftLaunch a, int first_split, int nsplit, bool backward) {
+    const ShiftTile t = a.tiles[first_split + blockIdx.x * nsplit];
+    const int q = threadIdx.x;
+    if (q >= t.count) return;
+    const int64_t pr = a.perm[t.start + q];
+    const double *part = a.part;
+    const int64_t base = (int64_t)blockIdx.x * nsplit * 256;
+    if (backward) {
+        double g0 = 0.0, g1 = 0.0;
+        for (int s = 0; s < nsplit; ++s) {
+            g0 += part[(base + (int64_t)s * 256 + q) * 2];
+            g1 += part[(base + (int64_t)s * 256 + q) * 2 + 1];
+        }
+        a.gr[pr] = g0;
+        a.gc[pr] = g1;
+    } else {
+        for (int kk = 0; kk < a.k; ++kk) {
+            double y = 0.0;
+            for (int s = 0; s < nsplit; ++s) y += part[(base + (int64_t)s * 256 + q) * a.k + kk];
+            a.y[pr * a.k + kk] = y;
         }
     }
 }
@@ -827,6 +863,13 @@ __global__ void sum_groups(const double *partial, int ngroups, int64_t n, double
 }
 
 }  // namespace
+
+hipError_t launch_shift_split_sum(const ShiftLaunch &a, int first_split, int ngroups, int nsplit, bool backward,
+                                  hipStream_t s) {
+    if (ngroups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(shift_split_sum, dim3((unsigned)ngroups), dim3(256), 0, s, a, first_split, nsplit, backward);
+    return hipGetLastError();
+}
 
 hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, bool rounded, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;

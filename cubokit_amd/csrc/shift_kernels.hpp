@@ -16,10 +16,13 @@
 
 namespace cubo {
 
-// A tile: up to 32 * WAVES positions of one shift class, perm[start .. start + count).
+// A tile: up to 32 * WAVES positions of one shift class, perm[start .. start + count), over the chunks c0 .. c1 - 1.
+// A whole tile (c0 = 0, c1 = nchunks, part = -1) writes its outputs; a split tile (one chunk range of a tile split
+// across several blocks) writes partial outputs at part (256 positions each), added by launch_shift_split_sum.
 struct ShiftTile {
     int32_t start, count;
     int32_t slot[4];          // rolled weight tables of the four corners
+    int32_t c0, c1, part;
 };
 
 struct ShiftLaunch {
@@ -38,12 +41,17 @@ struct ShiftLaunch {
     double *y;                // forward: [npos][k]
     const double *dy;         // backward: [npos][k] dL/dY
     double *gr, *gc;          // backward: [npos] dL/d(shift row), dL/d(shift column), before the mean removal
+    double *part;             // split tiles' partial outputs: [part][256][k] (forward) or [part][256][2] (adjoint)
 };
 
 // Tiles of 32 * kShiftWaves positions.
 constexpr int kShiftWaves = 8;
 // rounded: MaskShift.rounded, one roll per class (tiles use slot[0] only); the forward pass only.
 hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, bool rounded, hipStream_t s);
+// The split tiles' partial outputs added in split order into Y or gr, gc: ngroups groups of nsplit consecutive split
+// tiles from first_split (each group one tile's position range, its parts consecutive).
+hipError_t launch_shift_split_sum(const ShiftLaunch &a, int first_split, int ngroups, int nsplit, bool backward,
+                                  hipStream_t s);
 
 // Any cube (uint16 or Float32, any detector shape): each position's masks moved by the bilinear or the rounded
 // shift and dotted with the pattern in Double, as MaskShift's CPU loop does (ShiftedImaging.swift:55-71): the blend

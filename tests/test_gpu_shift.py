@@ -189,6 +189,41 @@ def test_large_classes_span_several_tiles():
     assert np.max(np.abs(dm[4] - ref) / np.maximum(cond, 1e-300)) <= 1e-6
 
 
+def test_split_tiles_match_the_oracle():
+    """A scan whose whole tiles (256 positions of one shift class) fill more than one round of the device, so each
+    class's partial tile is split into chunk ranges run by separate blocks and added afterwards: 320 x 256
+    positions, origins in a 2 x 2 pixel square (four classes of about 20,480 positions, each with a partial tile).
+    Y on every position and the origin adjoint against the oracle."""
+    scan, det, n = (320, 256), 64, 320 * 256
+    res = _cube(scan, det, 300.0, 22)
+    masks = _masks(det, 5, 23, signed=True)
+    rng = np.random.default_rng(24)
+    orow, ocol = 30.0 + rng.uniform(-1, 1, n), 30.0 + rng.uniform(-1, 1, n)
+    # the shift classes as the host forms them: whole tiles and partial ones
+    fr, fc = np.floor(orow - orow.mean()), np.floor(ocol - ocol.mean())
+    _, counts = np.unique(fr * 1000 + fc, return_counts=True)
+    assert np.sum(counts // 256) >= C.device_info(0).compute_units and np.any(counts % 256)
+    st = C.ShiftedStack(res, masks)
+    y = st.forward(orow, ocol)
+    host = _host(res)
+    for k in range(masks.shape[0]):
+        ref = O.vi_bilinear(host, masks[k].astype(np.float64), orow, ocol)
+        cond = O.vi_bilinear(host, np.abs(masks[k]).astype(np.float64), orow, ocol)
+        err = np.max(np.abs(y[:, k] - ref) / cond)
+        assert err <= 1e-6, f"mask {k}: {err:.3e} of sum |x w|"
+    dy = rng.normal(size=(n, masks.shape[0]))
+    _, dr, dc = st.backward(orow, ocol, dy, masks=False)
+    gr_ref, gc_ref, scale = np.zeros(n), np.zeros(n), np.zeros(n)
+    for k in range(masks.shape[0]):
+        g = O.vi_bilinear_grad_origin(host, masks[k].astype(np.float64), orow, ocol, dy[:, k])
+        gr_ref += g[0]
+        gc_ref += g[1]
+        scale += np.abs(dy[:, k]) * O.vi_bilinear(host, np.abs(masks[k]).astype(np.float64), orow, ocol)
+    tol = 2e-6 * (scale + scale.mean())
+    assert np.all(np.abs(dr - gr_ref) <= tol), f"dL/dorigin_row: {np.max(np.abs(dr - gr_ref) / tol):.3f} of tol"
+    assert np.all(np.abs(dc - gc_ref) <= tol), f"dL/dorigin_col: {np.max(np.abs(dc - gc_ref) / tol):.3f} of tol"
+
+
 def test_refusals():
     res = _cube((2, 2), 32, 100.0, 21)
     with pytest.raises(C.CuboVirtualError):
