@@ -12,6 +12,7 @@
 // lane whole window rows, and lane 0 adds the row sums in row order.
 #include "origin_kernels.hpp"
 
+#include <algorithm>
 #include <climits>
 #include <type_traits>
 
@@ -240,7 +241,6 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin_lds(Py4Launch a, int lS, 
     extern __shared__ double py_lds[];
     const int tid = threadIdx.x;
     const int qy = a.qy, qx = a.qx, R = a.R, ndet = qy * qx, S = 1 << lS, SP = S + 1;
-    auto row_at = [&](int k) { return POW2 ? k << lqx : k * qx; };     // k * qx
     auto col_at = [&](int k) { return (k << lS) + k; };                  // k * (S + 1)
     constexpr int lT = sizeof(T) == 2 ? 1 : 2;
     const unsigned qxb = (unsigned)qx * sizeof(T), sp8 = (unsigned)SP * 8;   // row strides in bytes (< 2^24)
@@ -423,18 +423,12 @@ __device__ float filled(const void *pat, int row, int col, int qy, int qx) {
     }
 }
 
+// One position of the iterated window (IteratedWindowOrigin.swift:136-241) by one wave: the pattern at pat (global
+// memory or the wave's copy in LDS), at most rows_per_wave window rows; returns the origin.
 template <bool U16>
-__global__ __launch_bounds__(64 * kIWWaves) void iterated_window(IWLaunch a, int rows_per_wave, int waves) {
-    extern __shared__ float part[];            // [waves][3][rows_per_wave]
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t pos = (int64_t)blockIdx.x * waves + wave;
-    if (pos >= a.npos) return;                 // whole waves; no block barrier below
+__device__ __forceinline__ void iw_position(const IWLaunch &a, const void *pat, int rows_per_wave, int lane,
+                                            float *out_r, float *out_c) {
     const int qy = a.qy, qx = a.qx;
-    const int64_t ndet = (int64_t)qy * qx;
-    const void *pat = (const char *)a.cube + pos * ndet * (U16 ? 2 : 4);
-    float *pI = part + (size_t)wave * 3 * rows_per_wave, *pX = pI + rows_per_wave, *pY = pX + rows_per_wave;
-
     // coarse: the brightest bin x bin block (the first in raster order), its centre
     const float r = a.radius;
     const int bin = a.binned ? max(1, (int)__builtin_roundf(r)) : 1;
@@ -445,8 +439,19 @@ __global__ __launch_bounds__(64 * kIWWaves) void iterated_window(IWLaunch a, int
         const int by = (b / nbx) * bin, bx = (b % nbx) * bin;
         const int yEnd = min(by + bin, qy), xEnd = min(bx + bin, qx);
         float s = 0.0f;
-        for (int y = by; y < yEnd; ++y)
-            for (int x = bx; x < xEnd; ++x) s += filled<U16>(pat, y, x, qy, qx);
+        for (int y = by; y < yEnd; ++y) {
+            // four pixels' reads issued together, added in raster order
+            int x = bx;
+            for (; x + 4 <= xEnd; x += 4) {
+                const float f0 = filled<U16>(pat, y, x, qy, qx), f1 = filled<U16>(pat, y, x + 1, qy, qx),
+                            f2 = filled<U16>(pat, y, x + 2, qy, qx), f3 = filled<U16>(pat, y, x + 3, qy, qx);
+                s += f0;
+                s += f1;
+                s += f2;
+                s += f3;
+            }
+            for (; x < xEnd; ++x) s += filled<U16>(pat, y, x, qy, qx);
+        }
         if (s > best) { best = s; bestb = b; }
     }
     for (int off = 32; off >= 1; off >>= 1) {
@@ -471,34 +476,44 @@ __global__ __launch_bounds__(64 * kIWWaves) void iterated_window(IWLaunch a, int
         const int x0 = max(0, (int)__builtin_floorf(cc - win));
         const int x1 = min(qx - 1, (int)__builtin_ceilf(cc + win));
         const int nrows = min(y1 >= y0 && x1 >= x0 ? y1 - y0 + 1 : 0, rows_per_wave);
+        float tI = 0.0f, tX = 0.0f, tY = 0.0f;        // this lane's rows k, k + 64, ..., added in that order
         for (int k = lane; k < nrows; k += 64) {
             const int y = y0 + k;
             float sI = 0.0f, sX = 0.0f, sY = 0.0f;
             const float dy = (float)y - cr;
             const float dy2 = dy * dy;
             if (dy2 <= win2) {
-                for (int x = x0; x <= x1; ++x) {
+                // pixels inside the window, in column order; four reads issued together
+                auto add = [&](int x, float f) {
                     const float dx = (float)x - cc;
-                    if (__builtin_fmaf(dx, dx, dy2) > win2) continue;
-                    const float f = filled<U16>(pat, y, x, qy, qx);
+                    if (__builtin_fmaf(dx, dx, dy2) > win2) return;
                     const float v = 0.0f >= f ? 0.0f : f;
                     sI += v;
                     sX = __builtin_fmaf(v, (float)x, sX);
                     sY = __builtin_fmaf(v, (float)y, sY);
+                };
+                int x = x0;
+                for (; x + 4 <= x1 + 1; x += 4) {
+                    const float f0 = filled<U16>(pat, y, x, qy, qx), f1 = filled<U16>(pat, y, x + 1, qy, qx),
+                                f2 = filled<U16>(pat, y, x + 2, qy, qx), f3 = filled<U16>(pat, y, x + 3, qy, qx);
+                    add(x, f0);
+                    add(x + 1, f1);
+                    add(x + 2, f2);
+                    add(x + 3, f3);
                 }
+                for (; x <= x1; ++x) add(x, filled<U16>(pat, y, x, qy, qx));
             }
-            pI[k] = sI; pX[k] = sX; pY[k] = sY;
+            tI += sI; tX += sX; tY += sY;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-        float sumI = 0.0f, sumIX = 0.0f, sumIY = 0.0f;
-        if (lane == 0)
-            for (int k = 0; k < nrows; ++k) { sumI += pI[k]; sumIX += pX[k]; sumIY += pY[k]; }
-        sumI = __shfl(sumI, 0, 64);
-        sumIX = __shfl(sumIX, 0, 64);
-        sumIY = __shfl(sumIY, 0, 64);
-        __builtin_amdgcn_wave_barrier();        // the partials are read before the next pass writes them
+        // the row sums added across the wave in a fixed tree (lane 0's result, broadcast): the reference adds all
+        // window pixels in one running sum, so the totals differ from it by Float32 rounding only
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            tI += __shfl_down(tI, off, 64);
+            tX += __shfl_down(tX, off, 64);
+            tY += __shfl_down(tY, off, 64);
+        }
+        const float sumI = __shfl(tI, 0, 64), sumIX = __shfl(tX, 0, 64), sumIY = __shfl(tY, 0, 64);
         if (!(sumI > 0.0f)) break;
         const float nc = sumIX / sumI, nr = sumIY / sumI;
         const bool settled = __builtin_fabsf(nc - cc) < 1e-4f && __builtin_fabsf(nr - cr) < 1e-4f;
@@ -506,6 +521,20 @@ __global__ __launch_bounds__(64 * kIWWaves) void iterated_window(IWLaunch a, int
         cr = nr;
         if (settled) break;
     }
+    *out_r = cr;
+    *out_c = cc;
+}
+
+template <bool U16>
+__global__ __launch_bounds__(64 * kIWWaves) void iterated_window(IWLaunch a, int rows_per_wave, int waves) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t pos = (int64_t)blockIdx.x * waves + wave;
+    if (pos >= a.npos) return;                 // whole waves; no block barrier below
+    const int64_t ndet = (int64_t)a.qy * a.qx;
+    const void *pat = (const char *)a.cube + pos * ndet * (U16 ? 2 : 4);
+    float cr, cc;
+    iw_position<U16>(a, pat, rows_per_wave, lane, &cr, &cc);
     if (lane == 0) {
         a.out_row[pos] = (double)cr;
         a.out_col[pos] = (double)cc;
@@ -548,13 +577,12 @@ hipError_t launch_iterated_window(const IWLaunch &a, hipStream_t s) {
     const double win = rm >= rs ? rm : rs;
     const int64_t span = (int64_t)(2.0 * win) + 3;
     const int rows = (int)(span < a.qy ? span : a.qy);
-    const size_t per_wave = (size_t)3 * rows * sizeof(float);
-    const int waves = per_wave * kIWWaves <= 65536 ? kIWWaves : 1;
+    const int waves = kIWWaves;
     const dim3 grid((unsigned)((a.npos + waves - 1) / waves));
     if (a.u16)
-        hipLaunchKernelGGL(iterated_window<true>, grid, dim3(64 * waves), per_wave * waves, s, a, rows, waves);
+        hipLaunchKernelGGL(iterated_window<true>, grid, dim3(64 * waves), 0, s, a, rows, waves);
     else
-        hipLaunchKernelGGL(iterated_window<false>, grid, dim3(64 * waves), per_wave * waves, s, a, rows, waves);
+        hipLaunchKernelGGL(iterated_window<false>, grid, dim3(64 * waves), 0, s, a, rows, waves);
     return hipGetLastError();
 }
 
