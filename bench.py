@@ -194,7 +194,8 @@ def batch_section(C):
 def diff_section(C):
     """The 512x512x128x128 uint16 differentiation workload (8 GiB): origins from one fused CoM pass, the K = 16
     + CoM shifted stack forward, 20 fixed-step DPC iterations on the 512x512 centre-of-mass field and their
-    adjoint, then the stack's adjoint with respect to the masks and the origins (both re-reading the cube), and
+    adjoint, then the stack's adjoint with respect to the masks (re-reading the cube) and the origins (from the
+    forward's kept Jacobian rows; also timed as a pass of its own, for a plan with no forward), and
     CuboPhase's plane reference and rotation solve on the same field. Kernel times from HIP events; GB/s are cube
     bytes per kernel second; the phase calls are timed whole (host arrays in and out)."""
     from cubokit_amd import synthetic
@@ -210,6 +211,13 @@ def diff_section(C):
     masks = synthetic.differentiation_masks()
     st = C.ShiftedStack(cube, masks)
     y = st.forward(orow, ocol)                              # prepares the shift classes
+    st.kernel_time()
+    for _ in range(3):
+        y = st.forward(orow, ocol)
+    ms, _ = st.kernel_time()
+    plain_fwd_ms = ms[0] / 3
+    st.keep_jacobian(True)                                 # the forward of a differentiation step
+    y = st.forward(orow, ocol)
     st.kernel_time()
     t0 = time.perf_counter()
     for _ in range(3):
@@ -235,14 +243,27 @@ def diff_section(C):
     dy[:, :16] = rng.normal(size=(npos, 16))
     dy[:, 17], dy[:, 18] = gcr / S, gcc / S
     dy[:, 16] = -(gcr * Mr + gcc * Mc) / (S * S)
+    # the backward after the forward on the same origin map: dL/dorigin from the forward's kept Jacobian rows, the
+    # mask adjoint re-reading the cube
     st.backward(orow, ocol, dy)                            # warm-up
     st.kernel_time()
+    st.jacobian_uses()
     t0 = time.perf_counter()
     for _ in range(2):
         dm, dr, dc = st.backward(orow, ocol, dy)
     bwd_wall = (time.perf_counter() - t0) / 2
     ms, calls = st.kernel_time()
+    kept = st.jacobian_uses() == calls
     org_ms, msk_ms = ms[1] / calls, ms[2] / calls
+    # the origin adjoint of a plan with no forward: its own pass over the cube
+    st2 = C.ShiftedStack(cube, masks)
+    st2.backward(orow, ocol, dy, masks=False)              # warm-up (prepares the shift classes)
+    st2.kernel_time()
+    for _ in range(2):
+        st2.backward(orow, ocol, dy, masks=False)
+    ms2, calls2 = st2.kernel_time()
+    org_pass_ms = ms2[1] / calls2
+    st2.release()
     # CuboPhase's reference plane and rotation on the same 512x512 centre-of-mass field (host arrays in and out,
     # so the call times include the copies over the host link)
     from cubokit_amd.phase import normalize, rotation_solve
@@ -260,14 +281,20 @@ def diff_section(C):
     res = {"workload": "512x512x128x128 uint16 (8 GiB), disk r=12, 8-sinusoid +-2 px deflection; K=16 masks + CoM "
                        "rows, bilinear per-position origin; 20 fixed-step DPC iterations on the 512x512 field",
            "cube_bytes": nb, "forward_kernel_ms": round(fwd_ms, 3), "forward_gbs": round(nb / fwd_ms / 1e6, 1),
-           "origin_adjoint_ms": round(org_ms, 3), "origin_adjoint_gbs": round(nb / org_ms / 1e6, 1),
+           "forward_keeps_jacobian": True,
+           "plain_forward_kernel_ms": round(plain_fwd_ms, 3), "plain_forward_gbs": round(nb / plain_fwd_ms / 1e6, 1),
+           "origin_adjoint_ms": round(org_ms, 3),
+           "origin_adjoint_from": "the forward's kept Jacobian rows (no pass over the cube)" if kept else "a pass",
+           "origin_adjoint_pass_ms": round(org_pass_ms, 3), "origin_adjoint_pass_gbs": round(nb / org_pass_ms / 1e6, 1),
            "mask_adjoint_ms": round(msk_ms, 3), "mask_adjoint_gbs": round(nb / msk_ms / 1e6, 1),
+           "forward_and_backward_device_ms": round(fwd_ms + org_ms + msk_ms, 3),
            "dpc_20_steps_and_adjoint_device_ms": round(dpc_ms, 3), "dpc_20_steps_and_adjoint_wall_ms": round(dpc_wall * 1e3, 3),
            "forward_call_wall_ms": round(fwd_wall * 1e3, 3), "backward_call_wall_ms": round(bwd_wall * 1e3, 3),
            "dpc_final_error": float(errors[-1]), "plane_normalize_call_ms": round(norm_ms, 3),
            "rotation_solve_call_ms": round(rot_ms, 3), "rotation_degrees": rot["degrees"],
            "rotation_transposed": rot["transposed"], "finite": ok}
-    log(f"differentiation: forward {fwd_ms:.3f} ms, origin adjoint {org_ms:.3f} ms, mask adjoint {msk_ms:.3f} ms, "
+    log(f"differentiation: forward {fwd_ms:.3f} ms, origin adjoint {org_ms:.3f} ms (kept rows: {kept}; as a pass "
+        f"{org_pass_ms:.3f} ms), mask adjoint {msk_ms:.3f} ms, "
         f"DPC 20 steps + adjoint {dpc_ms:.3f} ms device ({dpc_wall * 1e3:.1f} ms wall); plane reference "
         f"{norm_ms:.3f} ms, rotation (358 scores) {rot_ms:.3f} ms per call")
     st.release()

@@ -172,6 +172,8 @@ SIGNATURES = {
     "cubo_hip_shift_backward": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                            ct.c_void_p, ct.c_void_p]),
     "cubo_hip_shift_kernel_time": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_int64)]),
+    "cubo_hip_shift_jacobian_uses": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64)]),
+    "cubo_hip_shift_keep_jacobian": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "cubo_hip_shift_plan_destroy": (ct.c_int, [ct.c_void_p]),
     "cubo_hip_phase_normalize": (ct.c_int, [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_int32,
                                             ct.c_double, ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p,

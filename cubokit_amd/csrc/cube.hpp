@@ -22,6 +22,9 @@ struct cubo_hip_cube {
     void *data;             // one HBM buffer, storage order
     hipStream_t stream;     // every launch on this cube goes to this stream
     cubo::LaunchTimer timer;   // device time of the fused-kernel launches, until queried
+    // Counts the writes to the resident buffer (uploads, feeds, synthesis): results kept from a pass over the cube
+    // (the shifted stack's Jacobian rows) hold only while it is unchanged.
+    uint64_t generation = 0;
     // A streamed cube holds no whole-cube buffer: `data` is NULL and the scan-row tiles fed by
     // cubo_hip_cube_feed_rows live in the feed's two-slot device ring. Every cube that has been fed owns a feed
     // (pinned double buffers, a copy stream).

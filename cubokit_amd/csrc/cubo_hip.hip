@@ -190,6 +190,7 @@ int cubo_hip_cube_upload_rows(cubo_hip_cube *c, int64_t row0, int64_t nrows, con
     if (int st = cubo::require_resident(c, "uploading rows (a streamed cube is fed)")) return st;
     if (int st = rows_range(c, row0, nrows)) return st;
     DeviceGuard g(c->device);
+    ++c->generation;
     const uint64_t row_bytes = (uint64_t)c->shape[1] * c->ndet * element_size(c->dtype);
     HIP_TRY(hipMemcpyAsync((char *)c->data + row0 * row_bytes, host, nrows * row_bytes, hipMemcpyHostToDevice,
                            c->stream), "uploading scan rows");

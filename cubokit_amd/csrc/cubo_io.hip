@@ -505,6 +505,7 @@ int cubo_hip_cube_feed_rows(cubo_hip_cube *c, int64_t row0, int64_t nrows, const
         dst = f->ring[slot];
         f->ring_row0[slot] = -1;
     } else {
+        ++c->generation;
         dst = (char *)c->data + (size_t)row0 * c->shape[1] * c->ndet * element_size(c->dtype);
         // earlier launches on the cube may still read these rows
         CUBO_HIP_TRY(hipEventRecord(f->consumed[0], c->stream), "hipEventRecord");

@@ -56,6 +56,13 @@ struct cubo_hip_shift_plan {
     // split tiles (the classes' partial tiles, each over nsplit chunk ranges) after the whole ones
     int first_split = 0, split_groups = 0, nsplit = 1;
     double *part = nullptr; size_t part_cap = 0;
+    // dY/dwr and dY/dwc of the last bilinear forward ([2][npos][k]), valid while the origin map (prepared) and the
+    // cube (its generation) are those of that forward: the origin adjoint then needs no pass over the cube
+    double *jac = nullptr; size_t jac_cap = 0;
+    bool keep_jac = false;           // cubo_hip_shift_keep_jacobian
+    bool jac_valid = false;
+    uint64_t jac_generation = 0;
+    int64_t jac_uses = 0;            // origin adjoints taken from kept rows since the last kernel-time query
     double *wr = nullptr, *wc = nullptr;
     float *wts = nullptr;
     // per-call buffers
@@ -86,7 +93,7 @@ void free_plan(cubo_hip_shift_plan *p) {
     if (!p) return;
     DeviceGuard g(p->cube->device);
     (void)hipStreamSynchronize(p->cube->stream);
-    void *bufs[] = {p->base, p->inv, p->wroll, p->perm, p->shift, p->slot_shifts, p->tiles, p->part, p->wr, p->wc, p->wts,
+    void *bufs[] = {p->base, p->inv, p->wroll, p->perm, p->shift, p->slot_shifts, p->tiles, p->part, p->jac, p->wr, p->wc, p->wts,
                     p->dy, p->g, p->dy32, p->partial, p->dmask, p->vmasks, p->vshift, p->vwr, p->vwc,
                     p->dyw, p->dy_inv, p->rpartial, p->rdmask, p->bslots, p->bic, p->border};
     for (void *b : bufs)
@@ -167,6 +174,8 @@ int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol, bool
     for (int64_t i = 0; i < npos; ++i)
         if (!std::isfinite(orow[i]) || !std::isfinite(ocol[i]))
             return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the origin map must be finite (position %lld)", (long long)i);
+    p->prepared = false;
+    p->jac_valid = false;
     const double mr = pairwise_sum(orow, npos) / (double)npos, mc = pairwise_sum(ocol, npos) / (double)npos;
     p->h_shift.resize(2 * npos);
     p->h_wts.resize(4 * npos);
@@ -270,7 +279,7 @@ int prepare(cubo_hip_shift_plan *p, const double *orow, const double *ocol, bool
     e = grow(&p->tiles, &cap_tiles, tiles.size());
     p->tiles_cap = cap_tiles;
     if (e == hipSuccess && p->split_groups > 0)
-        e = grow(&p->part, &p->part_cap, (size_t)p->split_groups * p->nsplit * TP * std::max(p->k, 2));
+        e = grow(&p->part, &p->part_cap, (size_t)p->split_groups * p->nsplit * TP * 3 * std::max(p->k, 1));
     if (e == hipSuccess) e = grow((char **)&p->wroll, &p->wroll_slots, (size_t)nslots * p->slot_stride * 16);
     if (e != hipSuccess) return hip_fail(e, "allocating the rolled weights");
     hipStream_t s = c->stream;
@@ -389,10 +398,9 @@ cubo::ShiftLaunch launch_args(const cubo_hip_shift_plan *p) {
 }
 
 // The stack kernel over every tile, then the split tiles' partial outputs added.
-hipError_t run_stack(const cubo_hip_shift_plan *p, const cubo::ShiftLaunch &a, bool backward, bool rounded,
-                     hipStream_t s) {
-    hipError_t e = cubo::launch_shift_stack(a, p->ntiles, backward, rounded, s);
-    if (e == hipSuccess) e = cubo::launch_shift_split_sum(a, p->first_split, p->split_groups, p->nsplit, backward, s);
+hipError_t run_stack(const cubo_hip_shift_plan *p, const cubo::ShiftLaunch &a, int pass, bool rounded, hipStream_t s) {
+    hipError_t e = cubo::launch_shift_stack(a, p->ntiles, pass, rounded, s);
+    if (e == hipSuccess) e = cubo::launch_shift_split_sum(a, p->first_split, p->split_groups, p->nsplit, pass, s);
     return e;
 }
 
@@ -510,14 +518,24 @@ int forward(cubo_hip_shift_plan *p, const double *origin_row, const double *orig
     DeviceGuard g(c->device);
     const size_t ny = (size_t)c->npos * p->k;
     CUBO_HIP_TRY(ensure_y(p, ny), "allocating the stack output");
+    // When asked to, the bilinear forward also keeps the Jacobian rows for the origin adjoint of the same map.
+    const int pass = p->keep_jac && !rounded && cubo::shift_jac_supported(launch_args(p).row_chunks) ? cubo::kShiftYJac
+                                                                                                   : cubo::kShiftY;
+    p->jac_valid = false;
+    if (pass == cubo::kShiftYJac) CUBO_HIP_TRY(grow(&p->jac, &p->jac_cap, 2 * ny), "allocating the Jacobian rows");
     cubo::ShiftLaunch a = launch_args(p);
     a.y = p->y;
+    a.jac = p->jac;
     CUBO_HIP_TRY(hipEventRecord(p->ev[0], c->stream), "hipEventRecord");
-    CUBO_HIP_TRY(run_stack(p, a, false, rounded, c->stream), "launching the shifted stack");
+    CUBO_HIP_TRY(run_stack(p, a, pass, rounded, c->stream), "launching the shifted stack");
     CUBO_HIP_TRY(hipEventRecord(p->ev[1], c->stream), "hipEventRecord");
     CUBO_HIP_TRY(hipStreamSynchronize(c->stream), "waiting for the shifted stack");
     copy_y(p, y, ny);
     p->ms[0] += elapsed(p->ev[0], p->ev[1]);
+    if (pass == cubo::kShiftYJac) {
+        p->jac_valid = true;
+        p->jac_generation = c->generation;
+    }
     return CUBO_HIP_OK;
 }
 
@@ -553,12 +571,19 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
     CUBO_HIP_TRY(hipMemcpyAsync(p->dy, p->dyhost, ny * 8, hipMemcpyHostToDevice, s), "uploading dL/dY");
     if (d_origin_row) {
         if (!p->g) CUBO_HIP_TRY(hipMalloc((void **)&p->g, (size_t)npos * 16), "allocating the origin gradient");
-        cubo::ShiftLaunch a = launch_args(p);
-        a.dy = p->dy;
-        a.gr = p->g;
-        a.gc = p->g + npos;
         CUBO_HIP_TRY(hipEventRecord(p->ev[2], s), "hipEventRecord");
-        CUBO_HIP_TRY(run_stack(p, a, true, false, s), "launching the origin adjoint");
+        if (p->jac_valid && p->jac_generation == c->generation) {
+            // the forward on this origin map kept dY/dwr and dY/dwc: one dot product per position
+            CUBO_HIP_TRY(cubo::launch_shift_origin_from_jac(p->jac, p->dy, npos, p->k, p->g, p->g + npos, s),
+                         "launching the origin adjoint");
+            ++p->jac_uses;
+        } else {
+            cubo::ShiftLaunch a = launch_args(p);
+            a.dy = p->dy;
+            a.gr = p->g;
+            a.gc = p->g + npos;
+            CUBO_HIP_TRY(run_stack(p, a, cubo::kShiftOrigin, false, s), "launching the origin adjoint");
+        }
         CUBO_HIP_TRY(hipEventRecord(p->ev[3], s), "hipEventRecord");
     }
     if (dmasks && cubo::shift_grad_rows_supported((int)c->shape[3])) {
@@ -704,6 +729,30 @@ int cubo_hip_shift_kernel_time(cubo_hip_shift_plan *p, double *ms3, int64_t *cal
     for (int i = 0; i < 3; ++i) { ms3[i] = p->ms[i]; p->ms[i] = 0; }
     *calls = p->calls;
     p->calls = 0;
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_shift_keep_jacobian(cubo_hip_shift_plan *p, int32_t keep) {
+    if (!p) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL plan");
+    if (keep != 0 && keep != 1) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "keep: 0 or 1, not %d", keep);
+    p->keep_jac = keep == 1;
+    if (!p->keep_jac) {
+        p->jac_valid = false;
+        if (p->jac) {
+            DeviceGuard g(p->cube->device);
+            (void)hipStreamSynchronize(p->cube->stream);
+            (void)hipFree(p->jac);
+            p->jac = nullptr;
+            p->jac_cap = 0;
+        }
+    }
+    return CUBO_HIP_OK;
+}
+
+int cubo_hip_shift_jacobian_uses(cubo_hip_shift_plan *p, int64_t *uses) {
+    if (!p || !uses) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *uses = p->jac_uses;
+    p->jac_uses = 0;
     return CUBO_HIP_OK;
 }
 

@@ -9,6 +9,8 @@
 //   forward   Y   = sum_c w_c C_c
 //   backward  dY/dwr = (1 - wc)(C_1 - C_0) + wc (C_3 - C_2),  dY/dwc = (1 - wr)(C_2 - C_0) + wr (C_3 - C_1),
 //             g_r(p) = sum_k dL/dY[p, k] dY/dwr[p, k]   (d wr / d origin_r = 1 inside a bilinear cell).
+// The forward can also keep the Jacobian rows dY/dwr, dY/dwc (2 x positions x masks Doubles, not cube-sized):
+// the origin adjoint of the same origin map is then a dot product per position, with no pass over the cube.
 // Mask adjoint: a second pass with the positions as the reduction dimension: shift_grad_t (rows of 64-256 pixels,
 // batches of one column shift staged transposed in LDS), shift_grad_rows (other multiples of 32) or shift_grad_w
 // (any width, vector ALUs).
@@ -40,9 +42,12 @@ __device__ __forceinline__ uint32_t cvt_pair(uint32_t w) {
 // hold the weights of the corners above them D chunks earlier (a roll by one row moves every pixel qx = 64 D
 // places), so only corners 0 and 2 are loaded and LDS keeps a ring of D + 2 chunks of them; the lower corners
 // read the ring D chunks back. The ring's slots of chunks -D .. -1 hold the last D chunks (the roll wraps).
-template <int WAVES, bool BWD, int NC, int D>
+// PASS: kShiftY (Y), kShiftOrigin (gr, gc from dL/dY) or kShiftYJac (Y and the Jacobian rows; Y is accumulated in
+// LDS, the two rows in registers, as the origin adjoint's).
+template <int WAVES, int PASS, int NC, int D>
 __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
-    static_assert(NC == 4 || (NC == 1 && !BWD && D == 0), "the adjoint needs the four corners");
+    constexpr bool BWD = PASS == kShiftOrigin, JAC = PASS == kShiftYJac;
+    static_assert(NC == 4 || (NC == 1 && PASS == kShiftY && D == 0), "the adjoint needs the four corners");
     constexpr int kCorners = NC;
     constexpr bool RING = D > 0;
     constexpr int RS = RING ? D + 2 : 2;                     // chunk buffers in LDS
@@ -54,7 +59,9 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
     __shared__ half8 wlds[RS][WCH];
     // the buffer of chunk c (c >= -D)
     auto buf_of = [](int c) { return RING ? (c + RS) % RS : c & 1; };
-    __shared__ double wts[2][TP];
+    __shared__ double wts[2][TP];                              // the positions' bilinear fractions wr, wc
+    // JAC: Y of (position slot, mask column), each entry owned by one lane
+    __shared__ double yacc[JAC ? TP * 32 : 1];
 
     const ShiftTile t = a.tiles[blockIdx.x];
     const int c0 = t.c0, c1 = t.c1;                            // the tile's chunk range
@@ -84,7 +91,7 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
     // Entry e of a chunk's block copy belongs to corner e / kTileEntries (a compile-time constant per j).
     // The forward keeps two chunks of weights in registers (fetched two chunks ahead); the adjoint, short of
     // registers, one.
-    constexpr bool DEEP = !BWD;
+    constexpr bool DEEP = PASS == kShiftY;
     half8 wreg[DEEP ? 2 : 1][WPT];
     auto fetch_w = [&](int set, int chunk) {
 #pragma unroll
@@ -100,34 +107,47 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
         for (int j = 0; j < WPT; ++j) wlds[buf][tid + j * 64 * WAVES] = wreg[set][j];
     };
 
+    constexpr int NACC = PASS == kShiftY ? 1 : 2;
     float16v acc[kCorners];
-    double accd[BWD ? 2 : 1][16];
+    double accd[NACC][16];
 #pragma unroll
     for (int c = 0; c < kCorners; ++c)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[c][r] = 0.f;
 #pragma unroll
-    for (int m = 0; m < (BWD ? 2 : 1); ++m)
+    for (int m = 0; m < NACC; ++m)
 #pragma unroll
         for (int r = 0; r < 16; ++r) accd[m][r] = 0.0;
+    // Register r of lane l holds row 8 (r / 4) + 4 (l / 32) + r % 4 of the wave's 32 positions, mask column l % 32.
+    auto slot_of = [&](int r) { return wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); };
+    if constexpr (JAC) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) yacc[slot_of(r) * 32 + (lane & 31)] = 0.0;
+    }
 
     const bool any_lo = a.any_lo != 0;
-    // Register r of lane l holds row 8 (r / 4) + 4 (l / 32) + r % 4 of the wave's 32 positions.
+    // The corner sums of a flush combined in FP64 through d1 = c1 - c0, d2 = c2 - c0 and the second difference
+    // e = c0 - c1 - c2 + c3:  dY/dwr = d1 + wc e,  dY/dwc = d2 + wr e,  Y = c0 + wr dY/dwr + wc d2
+    // (the blend (1-wr)(1-wc) c0 + wr(1-wc) c1 + (1-wr)wc c2 + wr wc c3 in four multiply-adds).
     auto flush = [&]() {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int row = wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const double wr = wts[0][row], wc = wts[1][row];
+            const int row = slot_of(r);
             if constexpr (NC == 1) {
                 accd[0][r] += (double)acc[0][r];
             } else {
+                const double wr = wts[0][row], wc = wts[1][row];
                 const double c0 = acc[0][r], c1 = acc[1][r], c2 = acc[2][r], c3 = acc[3][r];
-                if constexpr (BWD) {
-                    accd[0][r] += (1.0 - wc) * (c1 - c0) + wc * (c3 - c2);
-                    accd[1][r] += (1.0 - wr) * (c2 - c0) + wr * (c3 - c1);
-                } else {
-                    accd[0][r] += (1.0 - wr) * (1.0 - wc) * c0 + wr * (1.0 - wc) * c1 + (1.0 - wr) * wc * c2 +
-                                  wr * wc * c3;
+                const double d1 = c1 - c0, d2 = c2 - c0, e = (c3 - c2) - d1;
+                const double jr = __builtin_fma(wc, e, d1);
+                if constexpr (BWD || JAC) {
+                    accd[0][r] += jr;
+                    accd[1][r] += __builtin_fma(wr, e, d2);
+                }
+                if constexpr (!BWD) {
+                    const double y = __builtin_fma(wc, d2, __builtin_fma(wr, jr, c0));
+                    if constexpr (JAC) yacc[row * 32 + (lane & 31)] += y;
+                    else accd[0][r] += y;
                 }
             }
         }
@@ -230,10 +250,24 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
     const double inv = col < a.k ? ((const CUBO_GLOBAL double *)a.inv_scale)[col] : 0.0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const int row = wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int row = slot_of(r);
         const bool rl = row < t.count;
         const int64_t pr = a.perm[t.start + (rl ? row : 0)];
-        if constexpr (BWD) {
+        if constexpr (JAC) {
+            if (rl && col < a.k) {
+                const double y = yacc[row * 32 + col] * inv, jr = accd[0][r] * inv, jc = accd[1][r] * inv;
+                if (t.part < 0) {
+                    ((CUBO_GLOBAL double *)a.y)[pr * a.k + col] = y;
+                    ((CUBO_GLOBAL double *)a.jac)[pr * a.k + col] = jr;
+                    ((CUBO_GLOBAL double *)a.jac)[(a.npos + pr) * a.k + col] = jc;
+                } else {                               // one chunk range of a split tile: [part][256][3][k]
+                    CUBO_GLOBAL double *q = (CUBO_GLOBAL double *)a.part + ((int64_t)t.part * 256 + row) * 3 * a.k + col;
+                    q[0] = y;
+                    q[a.k] = jr;
+                    q[2 * a.k] = jc;
+                }
+            }
+        } else if constexpr (BWD) {
             double g0 = 0.0, g1 = 0.0;
             if (rl && col < a.k) {
                 const double d = ((const CUBO_GLOBAL double *)a.dy)[pr * a.k + col] * inv;
@@ -265,15 +299,16 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
 }
 
 // The split tiles' partial outputs added in split order: for split tile group g (tiles first_split + g nsplit ..,
-// one position range), Y[p][k] (forward) or gr, gc[p] (adjoint) of each of its positions.
-__global__ void shift_split_sum(ShiftLaunch a, int first_split, int nsplit, bool backward) {
+// one position range), Y[p][k] (kShiftY), gr, gc[p] (kShiftOrigin) or Y and the Jacobian rows (kShiftYJac) of each
+// of its positions.
+__global__ void shift_split_sum(ShiftLaunch a, int first_split, int nsplit, int pass) {
     const ShiftTile t = a.tiles[first_split + blockIdx.x * nsplit];
     const int q = threadIdx.x;
     if (q >= t.count) return;
     const int64_t pr = a.perm[t.start + q];
     const double *part = a.part;
     const int64_t base = (int64_t)blockIdx.x * nsplit * 256;
-    if (backward) {
+    if (pass == kShiftOrigin) {
         double g0 = 0.0, g1 = 0.0;
         for (int s = 0; s < nsplit; ++s) {
             g0 += part[(base + (int64_t)s * 256 + q) * 2];
@@ -281,12 +316,49 @@ __global__ void shift_split_sum(ShiftLaunch a, int first_split, int nsplit, bool
         }
         a.gr[pr] = g0;
         a.gc[pr] = g1;
+    } else if (pass == kShiftYJac) {
+        for (int m = 0; m < 3; ++m)
+            for (int kk = 0; kk < a.k; ++kk) {
+                double v = 0.0;
+                for (int s = 0; s < nsplit; ++s) v += part[((base + (int64_t)s * 256 + q) * 3 + m) * a.k + kk];
+                if (m == 0) a.y[pr * a.k + kk] = v;
+                else a.jac[((m - 1) * a.npos + pr) * a.k + kk] = v;
+            }
     } else {
         for (int kk = 0; kk < a.k; ++kk) {
             double y = 0.0;
             for (int s = 0; s < nsplit; ++s) y += part[(base + (int64_t)s * 256 + q) * a.k + kk];
             a.y[pr * a.k + kk] = y;
         }
+    }
+}
+
+// The origin adjoint from kept Jacobian rows: gr[p] = sum_k dy[p][k] jac[0][p][k], gc[p] likewise with jac[1].
+// A block takes 64 positions: the products are formed with coalesced reads of the block's 64 k consecutive
+// elements of each array, then each position's k products are added in mask order.
+constexpr int kJacPositions = 64;
+__global__ __launch_bounds__(256) void shift_origin_from_jac(const double *jac, const double *dy, int64_t npos, int k,
+                                                             double *gr, double *gc) {
+    __shared__ double pr[kJacPositions * 32], pc[kJacPositions * 32];
+    const int64_t p0 = (int64_t)blockIdx.x * kJacPositions;
+    const int np = (int)min<int64_t>(kJacPositions, npos - p0);
+    const int n = np * k;
+    const double *d = dy + p0 * k, *jr = jac + p0 * k, *jc = jac + (npos + p0) * k;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const double v = d[e];
+        pr[e] = v * jr[e];
+        pc[e] = v * jc[e];
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (q < np) {
+        double g0 = 0.0, g1 = 0.0;
+        for (int kk = 0; kk < k; ++kk) {
+            g0 += pr[q * k + kk];
+            g1 += pc[q * k + kk];
+        }
+        gr[p0 + q] = g0;
+        gc[p0 + q] = g1;
     }
 }
 
@@ -864,33 +936,51 @@ __global__ void sum_groups(const double *partial, int ngroups, int64_t n, double
 
 }  // namespace
 
-hipError_t launch_shift_split_sum(const ShiftLaunch &a, int first_split, int ngroups, int nsplit, bool backward,
+hipError_t launch_shift_split_sum(const ShiftLaunch &a, int first_split, int ngroups, int nsplit, int pass,
                                   hipStream_t s) {
     if (ngroups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(shift_split_sum, dim3((unsigned)ngroups), dim3(256), 0, s, a, first_split, nsplit, backward);
+    hipLaunchKernelGGL(shift_split_sum, dim3((unsigned)ngroups), dim3(256), 0, s, a, first_split, nsplit, pass);
     return hipGetLastError();
 }
 
-hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, bool rounded, hipStream_t s) {
+hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, int pass, bool rounded, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    if (backward && rounded) return hipErrorInvalidValue;
+    if (rounded && pass != kShiftY) return hipErrorInvalidValue;
     const dim3 grid(ntiles), block(64 * kShiftWaves);
     // the weight ring for detectors whose rows are 1, 2 or 4 whole chunks (64, 128 or 256 columns)
-#define CUBO_SHIFT_FOUR(BWD)                                                                            \
-    switch (a.row_chunks) {                                                                             \
-        case 1: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, BWD, 4, 1>), grid, block, 0, s, a); break; \
-        case 2: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, BWD, 4, 2>), grid, block, 0, s, a); break; \
-        case 4: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, BWD, 4, 4>), grid, block, 0, s, a); break; \
-        default: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, BWD, 4, 0>), grid, block, 0, s, a); break; \
+#define CUBO_SHIFT_FOUR(PASS)                                                                            \
+    switch (a.row_chunks) {                                                                              \
+        case 1: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, PASS, 4, 1>), grid, block, 0, s, a); break; \
+        case 2: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, PASS, 4, 2>), grid, block, 0, s, a); break; \
+        case 4: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, PASS, 4, 4>), grid, block, 0, s, a); break; \
+        default: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, PASS, 4, 0>), grid, block, 0, s, a); break; \
     }
-    if (backward) {
-        CUBO_SHIFT_FOUR(true)
-    } else if (rounded) {
-        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, false, 1, 0>), grid, block, 0, s, a);
+    if (rounded) {
+        hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, kShiftY, 1, 0>), grid, block, 0, s, a);
+    } else if (pass == kShiftOrigin) {
+        CUBO_SHIFT_FOUR(kShiftOrigin)
+    } else if (pass == kShiftYJac) {
+        // the Y accumulators take 64 KiB of LDS beside the weight ring: rows of 64 or 128 pixels
+        switch (a.row_chunks) {
+            case 1: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, kShiftYJac, 4, 1>), grid, block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, kShiftYJac, 4, 2>), grid, block, 0, s, a); break;
+            default: return hipErrorInvalidValue;
+        }
     } else {
-        CUBO_SHIFT_FOUR(false)
+        CUBO_SHIFT_FOUR(kShiftY)
     }
 #undef CUBO_SHIFT_FOUR
+    return hipGetLastError();
+}
+
+bool shift_jac_supported(int row_chunks) { return row_chunks == 1 || row_chunks == 2; }
+
+hipError_t launch_shift_origin_from_jac(const double *jac, const double *dy, int64_t npos, int k, double *gr, double *gc,
+                                        hipStream_t s) {
+    if (npos <= 0) return hipSuccess;
+    if (k < 1 || k > 32) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(shift_origin_from_jac, dim3((unsigned)((npos + kJacPositions - 1) / kJacPositions)), dim3(256), 0,
+                       s, jac, dy, npos, k, gr, gc);
     return hipGetLastError();
 }
 

@@ -41,17 +41,31 @@ struct ShiftLaunch {
     double *y;                // forward: [npos][k]
     const double *dy;         // backward: [npos][k] dL/dY
     double *gr, *gc;          // backward: [npos] dL/d(shift row), dL/d(shift column), before the mean removal
-    double *part;             // split tiles' partial outputs: [part][256][k] (forward) or [part][256][2] (adjoint)
+    double *part;             // split tiles' partial outputs: [part][256][k] (Y), [part][256][2] (origin adjoint) or
+                              // [part][256][3][k] (Y and the Jacobian rows)
+    double *jac;              // kShiftYJac: [2][npos][k] dY/dwr and dY/dwc
+};
+
+// What one pass of the shifted stack computes.
+enum ShiftPass : int {
+    kShiftY = 0,              // Y
+    kShiftOrigin = 1,         // gr, gc from dL/dY (the origin adjoint)
+    kShiftYJac = 2,           // Y, and the Jacobian rows dY/dwr, dY/dwc kept for an origin adjoint without a pass
 };
 
 // Tiles of 32 * kShiftWaves positions.
 constexpr int kShiftWaves = 8;
-// rounded: MaskShift.rounded, one roll per class (tiles use slot[0] only); the forward pass only.
-hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, bool backward, bool rounded, hipStream_t s);
-// The split tiles' partial outputs added in split order into Y or gr, gc: ngroups groups of nsplit consecutive split
-// tiles from first_split (each group one tile's position range, its parts consecutive).
-hipError_t launch_shift_split_sum(const ShiftLaunch &a, int first_split, int ngroups, int nsplit, bool backward,
+// rounded: MaskShift.rounded, one roll per class (tiles use slot[0] only); kShiftY only.
+hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, int pass, bool rounded, hipStream_t s);
+// The split tiles' partial outputs added in split order into the pass's outputs: ngroups groups of nsplit consecutive
+// split tiles from first_split (each group one tile's position range, its parts consecutive).
+hipError_t launch_shift_split_sum(const ShiftLaunch &a, int first_split, int ngroups, int nsplit, int pass,
                                   hipStream_t s);
+// Whether the forward can keep the Jacobian rows (kShiftYJac): detectors whose rows are 1 or 2 whole chunks.
+bool shift_jac_supported(int row_chunks);
+// The origin adjoint from kept Jacobian rows jac [2][npos][k]: gr[p] = sum_k dy[p][k] jac[0][p][k], gc likewise.
+hipError_t launch_shift_origin_from_jac(const double *jac, const double *dy, int64_t npos, int k, double *gr, double *gc,
+                                        hipStream_t s);
 
 // Any cube (uint16 or Float32, any detector shape): each position's masks moved by the bilinear or the rounded
 // shift and dotted with the pattern in Double, as MaskShift's CPU loop does (ShiftedImaging.swift:55-71): the blend

@@ -122,6 +122,7 @@ extern "C" int cubo_hip_cube_fill_synthetic(cubo_hip_cube *c, const cubo_hip_syn
     int prev = 0;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(c->device);
+    ++c->generation;
     const int64_t total = c->npos * c->ndet;
     const int64_t step = (int64_t)1 << 30;     // elements per launch keeps the grid within 2^31 threads
     hipError_t e = hipSuccess;

@@ -1101,9 +1101,11 @@ class ShiftedStack:
     (include/cubo_hip.h, "shifted stack"; Sources/CuboVirtual/ShiftedImaging.swift:43-73).
 
     forward(origin_row, origin_col) -> Y (scanCount, K); backward(...) -> (dL/dmasks (K, detectorCount),
-    dL/dorigin_row, dL/dorigin_col) for a given dL/dY, re-reading the resident cube."""
+    dL/dorigin_row, dL/dorigin_col) for a given dL/dY, re-reading the resident cube. With keep_jacobian=True the
+    bilinear forward keeps the Jacobian rows dY/d(shift) on the device (detector rows of 64 or 128 pixels), so a
+    backward on the same origin map (cube unchanged) reads the cube only for dL/dmasks."""
 
-    def __init__(self, cube: HIPResidentCube, masks: np.ndarray):
+    def __init__(self, cube: HIPResidentCube, masks: np.ndarray, keep_jacobian: bool = False):
         self.cube = cube
         self.masks = np.ascontiguousarray(masks, np.float32).reshape(-1, cube.shape.detector_count)
         self.k = self.masks.shape[0]
@@ -1111,6 +1113,12 @@ class ShiftedStack:
         _check(L.lib().cubo_hip_shift_plan_create(cube.handle, self.masks.ctypes.data, self.k, ct.byref(h)),
                "preparing the shifted stack")
         self._h = h
+        if keep_jacobian:
+            self.keep_jacobian(True)
+
+    def keep_jacobian(self, keep: bool) -> None:
+        """Whether the bilinear forward keeps dY/d(shift) for the next backward on its origin map."""
+        _check(L.lib().cubo_hip_shift_keep_jacobian(self._h, 1 if keep else 0), "keeping the Jacobian rows")
 
     def _origins(self, origin_row, origin_col):
         n = self.cube.shape.scan_count
@@ -1144,6 +1152,12 @@ class ShiftedStack:
         n = ct.c_int64()
         _check(L.lib().cubo_hip_shift_kernel_time(self._h, ms, ct.byref(n)), "shift kernel time")
         return list(ms), n.value
+
+    def jacobian_uses(self) -> int:
+        """Backward calls since the previous call whose dL/dorigin came from the forward's kept Jacobian rows."""
+        n = ct.c_int64()
+        _check(L.lib().cubo_hip_shift_jacobian_uses(self._h, ct.byref(n)), "shift Jacobian uses")
+        return n.value
 
     def release(self):
         if getattr(self, "_h", None) is not None:

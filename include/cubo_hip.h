@@ -342,7 +342,12 @@ int cubo_hip_batch_destroy(cubo_hip_batch *batch);
  * Backward (the vector-Jacobian product of Y, re-reading the cube): given dL/dY, dmasks = dL/dmasks
  * (k x detectorCount, Double) and dL/dorigin (row and column, scanCount each; inside a bilinear cell, through
  * the mean). Any output pointer of backward may be NULL (origin rows: both or neither). uint16 cubes whose
- * pattern length is a multiple of 8, 1..32 masks (FP32, k x detectorCount), finite origins. ---- */
+ * pattern length is a multiple of 8, 1..32 masks (FP32, k x detectorCount), finite origins.
+ * With cubo_hip_shift_keep_jacobian(plan, 1), the bilinear forward also keeps, on the device, the Jacobian
+ * rows dY/d(shift row) and dY/d(shift column) (2 x scanCount x k Doubles; detectors whose rows are 64 or 128
+ * pixels). A backward on the same origin map, with the cube unchanged since that forward, then takes dL/dorigin
+ * from them (a dot product per position), and only dL/dmasks re-reads the cube. Otherwise the origin adjoint
+ * is a pass over the cube of its own. ---- */
 typedef struct cubo_hip_shift_plan cubo_hip_shift_plan;
 int cubo_hip_shift_plan_create(cubo_hip_cube *cube, const float *masks, int32_t k, cubo_hip_shift_plan **out);
 /* y: scanCount x k Double, row-major. */
@@ -356,6 +361,14 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *plan, const double *origin_row,
 /* Device milliseconds since the previous query: [forward kernel, origin-adjoint kernel, mask-adjoint kernels],
  * and the number of backward calls. */
 int cubo_hip_shift_kernel_time(cubo_hip_shift_plan *plan, double *ms3, int64_t *calls);
+/* keep = 1: the bilinear forward keeps the Jacobian rows for the next backward (its own pass costs more than a
+ * plain forward: the Y sums move to LDS); 0, the default: it does not, and their memory is freed. No counterpart
+ * in the reference, which has no adjoint. */
+int cubo_hip_shift_keep_jacobian(cubo_hip_shift_plan *plan, int32_t keep);
+/* The number of backward calls since the previous query whose dL/dorigin came from the Jacobian rows kept by the
+ * forward (no pass over the cube). No counterpart in the reference, which has no adjoint; for tests and callers
+ * that account for the passes. */
+int cubo_hip_shift_jacobian_uses(cubo_hip_shift_plan *plan, int64_t *uses);
 int cubo_hip_shift_plan_destroy(cubo_hip_shift_plan *plan);
 
 /* ---- the centre-of-mass field's reference and rotation (CuboPhase), on the device in Double ----

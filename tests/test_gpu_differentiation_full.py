@@ -5,8 +5,9 @@ properties and sampled checks against the oracle.
   sample's origin mean equal to the whole map's), within 1e-6 of sum |x w|;
 - mask adjoint: the adjoint identity <dY, Y(V)> = <dL/dmasks(dY), V> for random masks V (Y is linear in the
   masks), within 1e-6 of sum |dY| |Y(|V|)|;
-- origin adjoint: central differences of the GPU forward (h = 1e-3) within 1e-3 of the scale;
-- DPC: 5 fixed-step iterations on the 512x512 field (a 1024x1024 grid) and their adjoint against the oracle's
+- origin adjoint: central differences of the GPU forward (h = 1e-3) within 1e-3 of the scale, and the adjoint
+  taken from the forward's kept Jacobian rows against the one re-reading the cube;
+- DPC: 20 fixed-step iterations on the 512x512 field (a 1024x1024 grid) and their adjoint against the oracle's
   composition, within 1e-10 of the largest value.
 """
 import numpy as np
@@ -76,8 +77,18 @@ def test_origin_adjoint_against_central_differences(workload):
     lm = float(np.sum(dy * st.forward(orow - h * vr, ocol - h * vc)))
     fd = (lp - lm) / (2 * h)
     an = float(np.dot(dr, vr) + np.dot(dc, vc))
-    scale = float(np.sum(np.abs(dy) * np.abs(st.forward(orow, ocol))))
+    y0 = st.forward(orow, ocol)
+    scale = float(np.sum(np.abs(dy) * np.abs(y0)))
     assert abs(fd - an) <= 1e-3 * scale, f"{fd} vs {an} (scale {scale:.3e})"
+    # a forward that keeps its Jacobian rows: the backward takes dL/dorigin from them (no pass over the cube); it
+    # adds the same FP64 corner combinations in another order
+    st.keep_jacobian(True)
+    assert np.array_equal(st.forward(orow, ocol), y0)
+    st.jacobian_uses()
+    _, drk, dck = st.backward(orow, ocol, dy, masks=False)
+    assert st.jacobian_uses() == 1
+    assert np.max(np.abs(drk - dr)) <= 1e-9 * np.max(np.abs(dr))
+    assert np.max(np.abs(dck - dc)) <= 1e-9 * np.max(np.abs(dc))
 
 
 def test_dpc_chain_adjoint_on_the_full_field():

@@ -117,6 +117,72 @@ def test_backward_matches_the_oracle(det, counts):
     assert dm3 is None and np.array_equal(dr3, dr) and np.array_equal(dc3, dc)
 
 
+@pytest.mark.parametrize("det,counts", [(64, 400.0), (128, 400.0), (128, 30000.0), (32, 400.0), (256, 400.0)])
+def test_origin_adjoint_from_the_forwards_jacobian_rows(det, counts):
+    """A bilinear forward keeps dY/d(shift) for its origin map (rows of 64 or 128 pixels): the next backward on that
+    map takes dL/dorigin from them, with no pass over the cube. Both routes against the oracle, and the cases that
+    must take the pass: another origin map, a rounded forward in between, the cube written since the forward, rows
+    of 32 and 256 pixels."""
+    import ctypes as ct
+    from cubokit_amd import _lib as L
+    res = _cube((9, 11), det, counts, 31)
+    masks = _masks(det, 6, 32, signed=True)
+    orow, ocol = _origins(99, 33)
+    dy = np.random.default_rng(34).normal(size=(99, 6))
+    host = _host(res)
+    gr_ref, gc_ref, scale = np.zeros(99), np.zeros(99), np.zeros(99)
+    for k in range(6):
+        g = O.vi_bilinear_grad_origin(host, masks[k].astype(np.float64), orow, ocol, dy[:, k])
+        gr_ref += g[0]
+        gc_ref += g[1]
+        scale += np.abs(dy[:, k]) * O.vi_bilinear(host, np.abs(masks[k]).astype(np.float64), orow, ocol)
+    tol = 2e-6 * (scale + scale.mean())
+
+    def check(dr, dc, what):
+        assert np.all(np.abs(dr - gr_ref) <= tol), f"{what} row: {np.max(np.abs(dr - gr_ref) / tol):.3f} of tol"
+        assert np.all(np.abs(dc - gc_ref) <= tol), f"{what} column: {np.max(np.abs(dc - gc_ref) / tol):.3f} of tol"
+
+    kept = det in (64, 128)
+    st = C.ShiftedStack(res, masks, keep_jacobian=True)
+    y = st.forward(orow, ocol)
+    for k in (0, 5):
+        ref = O.vi_bilinear(host, masks[k].astype(np.float64), orow, ocol)
+        cond = O.vi_bilinear(host, np.abs(masks[k]).astype(np.float64), orow, ocol)
+        assert np.max(np.abs(y[:, k] - ref) / cond) <= 1e-6
+    st.jacobian_uses()
+    dm, dr, dc = st.backward(orow, ocol, dy)
+    assert st.jacobian_uses() == (1 if kept else 0)
+    check(dr, dc, "from the forward" if kept else "pass")
+    # a plan with no forward takes the pass; both routes agree within the tolerance
+    fresh = C.ShiftedStack(res, masks)
+    dm2, dr2, dc2 = fresh.backward(orow, ocol, dy)
+    assert fresh.jacobian_uses() == 0 and np.array_equal(dm2, dm)
+    check(dr2, dc2, "pass")
+    # the kept rows serve every backward on the map until it changes
+    _, dr3, _ = st.backward(orow, ocol, dy, masks=False)
+    assert st.jacobian_uses() == (1 if kept else 0) and np.array_equal(dr3, dr)
+    # another map: the pass
+    orow2 = orow + 0.37
+    st.backward(orow2, ocol, dy, masks=False)
+    assert st.jacobian_uses() == 0
+    # a rounded forward on the map, then the bilinear backward: the pass
+    st.forward(orow, ocol)
+    st.forward(orow, ocol, rounded=True)
+    st.backward(orow, ocol, dy, masks=False)
+    assert st.jacobian_uses() == 0
+    # the cube written after the forward (the same rows back): the pass, with the same gradients
+    st.forward(orow, ocol)
+    row = np.ascontiguousarray(host[2:3])
+    assert L.lib().cubo_hip_cube_upload_rows(res.handle, 2, 1, row.ctypes.data_as(ct.c_void_p)) == 0
+    _, dr4, dc4 = st.backward(orow, ocol, dy, masks=False)
+    assert st.jacobian_uses() == 0 and np.array_equal(dr4, dr2) and np.array_equal(dc4, dc2)
+    # switched off: the forward keeps nothing (and gives the same Y)
+    st.keep_jacobian(False)
+    assert np.array_equal(st.forward(orow, ocol), y)
+    st.backward(orow, ocol, dy, masks=False)
+    assert st.jacobian_uses() == 0
+
+
 def test_gradients_against_central_differences_of_the_fp64_forward():
     res = _cube((4, 5), 32, 400.0, 10)
     masks = _masks(32, 3, 11)
@@ -203,7 +269,7 @@ def test_split_tiles_match_the_oracle():
     fr, fc = np.floor(orow - orow.mean()), np.floor(ocol - ocol.mean())
     _, counts = np.unique(fr * 1000 + fc, return_counts=True)
     assert np.sum(counts // 256) >= C.device_info(0).compute_units and np.any(counts % 256)
-    st = C.ShiftedStack(res, masks)
+    st = C.ShiftedStack(res, masks, keep_jacobian=True)
     y = st.forward(orow, ocol)
     host = _host(res)
     for k in range(masks.shape[0]):
@@ -212,7 +278,9 @@ def test_split_tiles_match_the_oracle():
         err = np.max(np.abs(y[:, k] - ref) / cond)
         assert err <= 1e-6, f"mask {k}: {err:.3e} of sum |x w|"
     dy = rng.normal(size=(n, masks.shape[0]))
+    st.jacobian_uses()
     _, dr, dc = st.backward(orow, ocol, dy, masks=False)
+    assert st.jacobian_uses() == 1                  # the split tiles' Jacobian rows added as their Y
     gr_ref, gc_ref, scale = np.zeros(n), np.zeros(n), np.zeros(n)
     for k in range(masks.shape[0]):
         g = O.vi_bilinear_grad_origin(host, masks[k].astype(np.float64), orow, ocol, dy[:, k])
