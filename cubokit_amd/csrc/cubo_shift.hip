@@ -519,8 +519,7 @@ int forward(cubo_hip_shift_plan *p, const double *origin_row, const double *orig
     const size_t ny = (size_t)c->npos * p->k;
     CUBO_HIP_TRY(ensure_y(p, ny), "allocating the stack output");
     // When asked to, the bilinear forward also keeps the Jacobian rows for the origin adjoint of the same map.
-    const int pass = p->keep_jac && !rounded && cubo::shift_jac_supported(launch_args(p).row_chunks) ? cubo::kShiftYJac
-                                                                                                   : cubo::kShiftY;
+    const int pass = p->keep_jac && !rounded ? cubo::kShiftYJac : cubo::kShiftY;
     p->jac_valid = false;
     if (pass == cubo::kShiftYJac) CUBO_HIP_TRY(grow(&p->jac, &p->jac_cap, 2 * ny), "allocating the Jacobian rows");
     cubo::ShiftLaunch a = launch_args(p);

@@ -42,8 +42,8 @@ __device__ __forceinline__ uint32_t cvt_pair(uint32_t w) {
 // hold the weights of the corners above them D chunks earlier (a roll by one row moves every pixel qx = 64 D
 // places), so only corners 0 and 2 are loaded and LDS keeps a ring of D + 2 chunks of them; the lower corners
 // read the ring D chunks back. The ring's slots of chunks -D .. -1 hold the last D chunks (the roll wraps).
-// PASS: kShiftY (Y), kShiftOrigin (gr, gc from dL/dY) or kShiftYJac (Y and the Jacobian rows; Y is accumulated in
-// LDS, the two rows in registers, as the origin adjoint's).
+// PASS: kShiftY (Y), kShiftOrigin (gr, gc from dL/dY) or kShiftYJac (Y and the Jacobian rows: Y in FP64 as the
+// forward's, the two rows across flushes in FP32, whose bar is the corner sums' scale, not the rows' own size).
 template <int WAVES, int PASS, int NC, int D>
 __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
     constexpr bool BWD = PASS == kShiftOrigin, JAC = PASS == kShiftYJac;
@@ -60,8 +60,6 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
     // the buffer of chunk c (c >= -D)
     auto buf_of = [](int c) { return RING ? (c + RS) % RS : c & 1; };
     __shared__ double wts[2][TP];                              // the positions' bilinear fractions wr, wc
-    // JAC: Y of (position slot, mask column), each entry owned by one lane
-    __shared__ double yacc[JAC ? TP * 32 : 1];
 
     const ShiftTile t = a.tiles[blockIdx.x];
     const int c0 = t.c0, c1 = t.c1;                            // the tile's chunk range
@@ -91,7 +89,7 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
     // Entry e of a chunk's block copy belongs to corner e / kTileEntries (a compile-time constant per j).
     // The forward keeps two chunks of weights in registers (fetched two chunks ahead); the adjoint, short of
     // registers, one.
-    constexpr bool DEEP = PASS == kShiftY;
+    constexpr bool DEEP = PASS == kShiftY || (JAC && D > 0);   // registers allow it
     half8 wreg[DEEP ? 2 : 1][WPT];
     auto fetch_w = [&](int set, int chunk) {
 #pragma unroll
@@ -107,9 +105,10 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
         for (int j = 0; j < WPT; ++j) wlds[buf][tid + j * 64 * WAVES] = wreg[set][j];
     };
 
-    constexpr int NACC = PASS == kShiftY ? 1 : 2;
+    constexpr int NACC = BWD ? 2 : 1;
     float16v acc[kCorners];
     double accd[NACC][16];
+    float jacc[JAC ? 2 : 1][16];                               // JAC: dY/dwr, dY/dwc
 #pragma unroll
     for (int c = 0; c < kCorners; ++c)
 #pragma unroll
@@ -120,10 +119,10 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
         for (int r = 0; r < 16; ++r) accd[m][r] = 0.0;
     // Register r of lane l holds row 8 (r / 4) + 4 (l / 32) + r % 4 of the wave's 32 positions, mask column l % 32.
     auto slot_of = [&](int r) { return wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); };
-    if constexpr (JAC) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) yacc[slot_of(r) * 32 + (lane & 31)] = 0.0;
-    }
+    for (int m = 0; m < (JAC ? 2 : 1); ++m)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) jacc[m][r] = 0.f;
 
     const bool any_lo = a.any_lo != 0;
     // The corner sums of a flush combined in FP64 through d1 = c1 - c0, d2 = c2 - c0 and the second difference
@@ -140,14 +139,15 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
                 const double c0 = acc[0][r], c1 = acc[1][r], c2 = acc[2][r], c3 = acc[3][r];
                 const double d1 = c1 - c0, d2 = c2 - c0, e = (c3 - c2) - d1;
                 const double jr = __builtin_fma(wc, e, d1);
-                if constexpr (BWD || JAC) {
+                if constexpr (BWD) {
                     accd[0][r] += jr;
                     accd[1][r] += __builtin_fma(wr, e, d2);
-                }
-                if constexpr (!BWD) {
-                    const double y = __builtin_fma(wc, d2, __builtin_fma(wr, jr, c0));
-                    if constexpr (JAC) yacc[row * 32 + (lane & 31)] += y;
-                    else accd[0][r] += y;
+                } else {
+                    accd[0][r] += __builtin_fma(wc, d2, __builtin_fma(wr, jr, c0));
+                    if constexpr (JAC) {
+                        jacc[0][r] += (float)jr;
+                        jacc[1][r] += (float)__builtin_fma(wr, e, d2);
+                    }
                 }
             }
         }
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(64 * WAVES) void shift_stack_mfma(ShiftLaunch a) {
         const int64_t pr = a.perm[t.start + (rl ? row : 0)];
         if constexpr (JAC) {
             if (rl && col < a.k) {
-                const double y = yacc[row * 32 + col] * inv, jr = accd[0][r] * inv, jc = accd[1][r] * inv;
+                const double y = accd[0][r] * inv, jr = (double)jacc[0][r] * inv, jc = (double)jacc[1][r] * inv;
                 if (t.part < 0) {
                     ((CUBO_GLOBAL double *)a.y)[pr * a.k + col] = y;
                     ((CUBO_GLOBAL double *)a.jac)[pr * a.k + col] = jr;
@@ -960,20 +960,13 @@ hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, int pass, bool r
     } else if (pass == kShiftOrigin) {
         CUBO_SHIFT_FOUR(kShiftOrigin)
     } else if (pass == kShiftYJac) {
-        // the Y accumulators take 64 KiB of LDS beside the weight ring: rows of 64 or 128 pixels
-        switch (a.row_chunks) {
-            case 1: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, kShiftYJac, 4, 1>), grid, block, 0, s, a); break;
-            case 2: hipLaunchKernelGGL((shift_stack_mfma<kShiftWaves, kShiftYJac, 4, 2>), grid, block, 0, s, a); break;
-            default: return hipErrorInvalidValue;
-        }
+        CUBO_SHIFT_FOUR(kShiftYJac)
     } else {
         CUBO_SHIFT_FOUR(kShiftY)
     }
 #undef CUBO_SHIFT_FOUR
     return hipGetLastError();
 }
-
-bool shift_jac_supported(int row_chunks) { return row_chunks == 1 || row_chunks == 2; }
 
 hipError_t launch_shift_origin_from_jac(const double *jac, const double *dy, int64_t npos, int k, double *gr, double *gc,
                                         hipStream_t s) {

@@ -61,8 +61,6 @@ hipError_t launch_shift_stack(const ShiftLaunch &a, int ntiles, int pass, bool r
 // split tiles from first_split (each group one tile's position range, its parts consecutive).
 hipError_t launch_shift_split_sum(const ShiftLaunch &a, int first_split, int ngroups, int nsplit, int pass,
                                   hipStream_t s);
-// Whether the forward can keep the Jacobian rows (kShiftYJac): detectors whose rows are 1 or 2 whole chunks.
-bool shift_jac_supported(int row_chunks);
 // The origin adjoint from kept Jacobian rows jac [2][npos][k]: gr[p] = sum_k dy[p][k] jac[0][p][k], gc likewise.
 hipError_t launch_shift_origin_from_jac(const double *jac, const double *dy, int64_t npos, int k, double *gr, double *gc,
                                         hipStream_t s);

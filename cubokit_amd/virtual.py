@@ -1102,8 +1102,8 @@ class ShiftedStack:
 
     forward(origin_row, origin_col) -> Y (scanCount, K); backward(...) -> (dL/dmasks (K, detectorCount),
     dL/dorigin_row, dL/dorigin_col) for a given dL/dY, re-reading the resident cube. With keep_jacobian=True the
-    bilinear forward keeps the Jacobian rows dY/d(shift) on the device (detector rows of 64 or 128 pixels), so a
-    backward on the same origin map (cube unchanged) reads the cube only for dL/dmasks."""
+    bilinear forward keeps the Jacobian rows dY/d(shift) on the device, so a backward on the same origin map (cube
+    unchanged) reads the cube only for dL/dmasks."""
 
     def __init__(self, cube: HIPResidentCube, masks: np.ndarray, keep_jacobian: bool = False):
         self.cube = cube

@@ -344,8 +344,7 @@ int cubo_hip_batch_destroy(cubo_hip_batch *batch);
  * the mean). Any output pointer of backward may be NULL (origin rows: both or neither). uint16 cubes whose
  * pattern length is a multiple of 8, 1..32 masks (FP32, k x detectorCount), finite origins.
  * With cubo_hip_shift_keep_jacobian(plan, 1), the bilinear forward also keeps, on the device, the Jacobian
- * rows dY/d(shift row) and dY/d(shift column) (2 x scanCount x k Doubles; detectors whose rows are 64 or 128
- * pixels). A backward on the same origin map, with the cube unchanged since that forward, then takes dL/dorigin
+ * rows dY/d(shift row) and dY/d(shift column) (2 x scanCount x k Doubles). A backward on the same origin map, with the cube unchanged since that forward, then takes dL/dorigin
  * from them (a dot product per position), and only dL/dmasks re-reads the cube. Otherwise the origin adjoint
  * is a pass over the cube of its own. ---- */
 typedef struct cubo_hip_shift_plan cubo_hip_shift_plan;
@@ -361,9 +360,9 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *plan, const double *origin_row,
 /* Device milliseconds since the previous query: [forward kernel, origin-adjoint kernel, mask-adjoint kernels],
  * and the number of backward calls. */
 int cubo_hip_shift_kernel_time(cubo_hip_shift_plan *plan, double *ms3, int64_t *calls);
-/* keep = 1: the bilinear forward keeps the Jacobian rows for the next backward (its own pass costs more than a
- * plain forward: the Y sums move to LDS); 0, the default: it does not, and their memory is freed. No counterpart
- * in the reference, which has no adjoint. */
+/* keep = 1: the bilinear forward keeps the Jacobian rows for the next backward (that pass costs a little more than
+ * a plain forward: two more sums per output, and the rows written); 0, the default: it does not, and their memory
+ * is freed. No counterpart in the reference, which has no adjoint. */
 int cubo_hip_shift_keep_jacobian(cubo_hip_shift_plan *plan, int32_t keep);
 /* The number of backward calls since the previous query whose dL/dorigin came from the Jacobian rows kept by the
  * forward (no pass over the cube). No counterpart in the reference, which has no adjoint; for tests and callers

@@ -5,8 +5,8 @@ properties and sampled checks against the oracle.
   sample's origin mean equal to the whole map's), within 1e-6 of sum |x w|;
 - mask adjoint: the adjoint identity <dY, Y(V)> = <dL/dmasks(dY), V> for random masks V (Y is linear in the
   masks), within 1e-6 of sum |dY| |Y(|V|)|;
-- origin adjoint: central differences of the GPU forward (h = 1e-3) within 1e-3 of the scale, and the adjoint
-  taken from the forward's kept Jacobian rows against the one re-reading the cube;
+- origin adjoint: central differences of the GPU forward (h = 1e-3) within 1e-3 of the scale, both for the pass
+  re-reading the cube and for the adjoint taken from the forward's kept Jacobian rows (which agree within 1e-6);
 - DPC: 20 fixed-step iterations on the 512x512 field (a 1024x1024 grid) and their adjoint against the oracle's
   composition, within 1e-10 of the largest value.
 """
@@ -80,15 +80,16 @@ def test_origin_adjoint_against_central_differences(workload):
     y0 = st.forward(orow, ocol)
     scale = float(np.sum(np.abs(dy) * np.abs(y0)))
     assert abs(fd - an) <= 1e-3 * scale, f"{fd} vs {an} (scale {scale:.3e})"
-    # a forward that keeps its Jacobian rows: the backward takes dL/dorigin from them (no pass over the cube); it
-    # adds the same FP64 corner combinations in another order
+    # a forward that keeps its Jacobian rows (Y unchanged): the backward takes dL/dorigin from them (no pass over
+    # the cube); the rows are added across flushes in FP32 there, in FP64 in the pass
     st.keep_jacobian(True)
     assert np.array_equal(st.forward(orow, ocol), y0)
     st.jacobian_uses()
     _, drk, dck = st.backward(orow, ocol, dy, masks=False)
     assert st.jacobian_uses() == 1
-    assert np.max(np.abs(drk - dr)) <= 1e-9 * np.max(np.abs(dr))
-    assert np.max(np.abs(dck - dc)) <= 1e-9 * np.max(np.abs(dc))
+    ank = float(np.dot(drk, vr) + np.dot(dck, vc))
+    assert abs(fd - ank) <= 1e-3 * scale, f"{fd} vs {ank} (scale {scale:.3e})"
+    assert abs(ank - an) <= 1e-6 * scale, f"kept {ank} vs pass {an} (scale {scale:.3e})"
 
 
 def test_dpc_chain_adjoint_on_the_full_field():

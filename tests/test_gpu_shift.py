@@ -119,10 +119,10 @@ def test_backward_matches_the_oracle(det, counts):
 
 @pytest.mark.parametrize("det,counts", [(64, 400.0), (128, 400.0), (128, 30000.0), (32, 400.0), (256, 400.0)])
 def test_origin_adjoint_from_the_forwards_jacobian_rows(det, counts):
-    """A bilinear forward keeps dY/d(shift) for its origin map (rows of 64 or 128 pixels): the next backward on that
-    map takes dL/dorigin from them, with no pass over the cube. Both routes against the oracle, and the cases that
-    must take the pass: another origin map, a rounded forward in between, the cube written since the forward, rows
-    of 32 and 256 pixels."""
+    """A bilinear forward keeps dY/d(shift) for its origin map: the next backward on that map takes dL/dorigin from
+    them, with no pass over the cube. Both routes against the oracle (rows of 32 pixels: every corner's weights
+    loaded; 64, 128 and 256: the weight ring), and the cases that must take the pass: another origin map, a rounded
+    forward in between, the cube written since the forward, the option switched off."""
     import ctypes as ct
     from cubokit_amd import _lib as L
     res = _cube((9, 11), det, counts, 31)
@@ -142,7 +142,7 @@ def test_origin_adjoint_from_the_forwards_jacobian_rows(det, counts):
         assert np.all(np.abs(dr - gr_ref) <= tol), f"{what} row: {np.max(np.abs(dr - gr_ref) / tol):.3f} of tol"
         assert np.all(np.abs(dc - gc_ref) <= tol), f"{what} column: {np.max(np.abs(dc - gc_ref) / tol):.3f} of tol"
 
-    kept = det in (64, 128)
+    kept = True
     st = C.ShiftedStack(res, masks, keep_jacobian=True)
     y = st.forward(orow, ocol)
     for k in (0, 5):
