@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -33,6 +34,67 @@ std::vector<double> gaussian_weights(double sigma) {
     const double s = cubo::pairwise_sum(w.data(), (int64_t)w.size());
     for (double &v : w) v = v / s;
     return w;
+}
+
+// The matrix-core get_origin's filter matrix (py4_origin_mfma): G[a][b] = the sum of the weights of the taps of
+// output a that mode nearest maps to input b, scaled by a power of two s (largest entry below 30000) and split into
+// FP16 hi + lo (lo parts below FP16's normal range dropped), as [4][n][n]: hi and lo, then both again with each
+// row's columns in the order of the second product's k-steps: column 16 k + 8 h + e of the copy holds column
+// 32 (k / 2) + 16 (k % 2) + 8 (e / 4) + 4 h + e % 4 of G. Returns false when the representation's relative error
+// would not leave the candidates' margin at least twice the filter's relative error bound.
+struct MfmaFilter {
+    std::vector<_Float16> g;
+    float pass2_scale = 0, margin = 0, slack = 0;
+};
+bool mfma_filter(const std::vector<double> &w, int n, MfmaFilter *f) {
+    const int R = (int)(w.size() / 2);
+    std::vector<double> G((size_t)n * n, 0.0);
+    for (int a = 0; a < n; ++a)
+        for (int t = -R; t <= R; ++t) {
+            const int b = a + t < 0 ? 0 : (a + t >= n ? n - 1 : a + t);
+            G[(size_t)a * n + b] += w[R + t];
+        }
+    double gmax = 0, rowmax = 0;
+    for (int a = 0; a < n; ++a) {
+        double rs = 0;
+        for (int b = 0; b < n; ++b) { gmax = std::max(gmax, G[(size_t)a * n + b]); rs += G[(size_t)a * n + b]; }
+        rowmax = std::max(rowmax, rs);
+    }
+    if (!(gmax > 0)) return false;
+    const double s = std::ldexp(1.0, (int)std::floor(std::log2(30000.0 / gmax)));
+    const double tiny = std::ldexp(1.0, -14);             // FP16's smallest normal
+    f->g.assign((size_t)4 * n * n, (_Float16)0.f);
+    double rep = 0;                                        // largest relative error of hi + lo
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            const double v = s * G[(size_t)a * n + b];
+            _Float16 hi = (_Float16)v;
+            if (std::fabs((double)hi) < tiny) hi = (_Float16)0.f;
+            _Float16 lo = (_Float16)(v - (double)hi);
+            if (std::fabs((double)lo) < tiny) lo = (_Float16)0.f;
+            if (v > 0) rep = std::max(rep, std::fabs((double)hi + (double)lo - v) / v);
+            f->g[(size_t)a * n + b] = hi;
+            f->g[(size_t)n * n + (size_t)a * n + b] = lo;
+        }
+    for (int a = 0; a < n; ++a)
+        for (int c = 0; c < n; ++c) {
+            const int k = c / 16, hh = (c / 8) % 2, e = c % 8;
+            const int b = 32 * (k / 2) + 16 * (k % 2) + 8 * (e / 4) + 4 * hh + e % 4;
+            f->g[(size_t)2 * n * n + (size_t)a * n + c] = f->g[(size_t)a * n + b];
+            f->g[(size_t)3 * n * n + (size_t)a * n + c] = f->g[(size_t)n * n + (size_t)a * n + b];
+        }
+    // the first product's sums reach s * 65535 * (largest row sum): scaled into FP16's range for the second
+    const double b1max = s * 65535.0 * rowmax * (1 + 1e-3);
+    const double p2 = std::ldexp(1.0, -(int)std::ceil(std::log2(b1max / 30000.0)));
+    // e: the two FP32 sums (at most 512 terms each), G's representation, B1's split and the dropped lo x lo term
+    const double e = 2 * 512 * std::ldexp(1.0, -24) + rep + 2 * std::ldexp(1.0, -22);
+    const double margin = 1e-3;
+    if (!(2 * e < margin)) return false;
+    f->pass2_scale = (float)p2;
+    f->margin = (float)margin;
+    // second-pass inputs below FP16's normal range: at most 2^-13 each, times a row of the scaled G, both sides
+    f->slack = (float)(2 * s * rowmax * (1 + 1e-3) * std::ldexp(1.0, -13));
+    return true;
 }
 
 }  // namespace
@@ -122,6 +184,8 @@ int cubo_hip_origins(cubo_hip_cube *c, const cubo_hip_origin_request *q, int64_t
     hipError_t e = hipSuccess;
     double *d_w = nullptr, *scratch = nullptr;
     uint8_t *d_take = nullptr;
+    _Float16 *d_g = nullptr;
+    int32_t *d_undecided = nullptr;      // [npos] positions, then their count
     // host data of the copies below stays alive until the stream is synchronised at the end
     const std::vector<double> w = q->estimator == CUBO_HIP_ORIGIN_PY4DSTEM ? gaussian_weights(r) : std::vector<double>();
     if (q->estimator == CUBO_HIP_ORIGIN_PY4DSTEM) {
@@ -132,23 +196,47 @@ int cubo_hip_origins(cubo_hip_cube *c, const cubo_hip_origin_request *q, int64_t
         if (e == hipSuccess) e = hipMemcpyAsync(d_w, w.data(), w.size() * 8, hipMemcpyHostToDevice, s);
         if (e == hipSuccess && q->scan_mask)
             e = hipMemcpyAsync(d_take, q->scan_mask + pos0, (size_t)npos, hipMemcpyHostToDevice, s);
+        cubo::Py4Launch a{};
+        a.cube = tile;
+        a.u16 = c->dtype == CUBO_HIP_U16;
+        a.npos = npos;
+        a.qy = (int)c->shape[2];
+        a.qx = (int)c->shape[3];
+        a.weights = d_w;
+        a.R = (int)(w.size() / 2);
+        a.limit = r * scale;
+        a.take = d_take;
+        a.fast_center = q->fast_center;
+        a.scratch = scratch;
+        a.out_row = d_out;
+        a.out_col = d_out + npos;
+        a.filtered = nullptr;
+        // uint16 patterns of 128 x 128: the matrix-core kernel, the exact one for the patterns it leaves undecided
+        // (CUBO_HIP_ORIGIN_KERNEL=exact keeps the exact kernel for every pattern: a test knob)
+        MfmaFilter mf;
+        const char *knob = getenv("CUBO_HIP_ORIGIN_KERNEL");
+        const bool mfma = cubo::py4_mfma_supported(a) && !(knob && strcmp(knob, "exact") == 0) && mfma_filter(w, a.qx, &mf);
+        if (e == hipSuccess && mfma) {
+            e = hipMalloc((void **)&d_g, mf.g.size() * sizeof(_Float16));
+            if (e == hipSuccess) e = hipMalloc((void **)&d_undecided, (size_t)(npos + 1) * 4);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(d_g, mf.g.data(), mf.g.size() * sizeof(_Float16), hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipMemsetAsync(d_undecided + npos, 0, 4, s);
+        }
         if (e == hipSuccess) {
-            cubo::Py4Launch a{};
-            a.cube = tile;
-            a.u16 = c->dtype == CUBO_HIP_U16;
-            a.npos = npos;
-            a.qy = (int)c->shape[2];
-            a.qx = (int)c->shape[3];
-            a.weights = d_w;
-            a.R = (int)(w.size() / 2);
-            a.limit = r * scale;
-            a.take = d_take;
-            a.fast_center = q->fast_center;
-            a.scratch = scratch;
-            a.out_row = d_out;
-            a.out_col = d_out + npos;
-            a.filtered = nullptr;
-            e = c->timer.start(s);     // the launch's device time, reported by cubo_hip_cube_kernel_time
+            e = c->timer.start(s);     // the launches' device time, reported by cubo_hip_cube_kernel_time
+            if (e == hipSuccess && mfma) {
+                cubo::Py4Launch m = a;
+                m.gmat = d_g;
+                m.pass2_scale = mf.pass2_scale;
+                m.candidate_margin = mf.margin;
+                m.slack = mf.slack;
+                m.undecided = d_undecided;
+                m.n_undecided = d_undecided + npos;
+                e = cubo::launch_py4_origin_mfma(m, (int)std::min<int64_t>(cubo::compute_units(c->device), npos), s);
+                a.list = d_undecided;
+                a.nlist = d_undecided + npos;
+            }
             if (e == hipSuccess) e = cubo::launch_py4_origin(a, blocks, s);
             if (e == hipSuccess) e = c->timer.stop(s);
         }
@@ -174,7 +262,7 @@ int cubo_hip_origins(cubo_hip_cube *c, const cubo_hip_origin_request *q, int64_t
     if (e == hipSuccess) e = hipMemcpyAsync(out_col + pos0, d_out + npos, (size_t)npos * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     else (void)hipStreamSynchronize(s);
-    for (void *b : {(void *)d_out, (void *)d_w, (void *)scratch, (void *)d_take})
+    for (void *b : {(void *)d_out, (void *)d_w, (void *)scratch, (void *)d_take, (void *)d_g, (void *)d_undecided})
         if (b) (void)hipFree(b);
     if (e != hipSuccess) return hip_fail(e, "measuring the origins");
     return CUBO_HIP_OK;

@@ -53,7 +53,9 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin(Py4Launch a) {
     __shared__ int64_t red_i[kPyBlock], red_n[kPyBlock];
     __shared__ double red_s[6][kPyBlock];
     __shared__ int64_t best_at;
-    for (int64_t pos = blockIdx.x; pos < a.npos; pos += gridDim.x) {
+    const int64_t count = a.list ? (int64_t)*a.nlist : a.npos;
+    for (int64_t k = blockIdx.x; k < count; k += gridDim.x) {
+        const int64_t pos = a.list ? (int64_t)a.list[k] : k;
         if (a.take && !a.take[pos]) {
             if (tid == 0) { a.out_row[pos] = __builtin_nan(""); a.out_col[pos] = __builtin_nan(""); }
             continue;   // uniform across the block
@@ -256,7 +258,9 @@ __global__ __launch_bounds__(kPyBlock) void py4_origin_lds(Py4Launch a, int lS, 
     for (int i = tid; i < 2 * R + 1; i += kPyBlock) wl[i] = a.weights[i];
     const double centre = a.weights[R];
     const bool vec = (ndet * (int)sizeof(T)) % 16 == 0;
-    for (int64_t pos = blockIdx.x; pos < a.npos; pos += gridDim.x) {
+    const int64_t count = a.list ? (int64_t)*a.nlist : a.npos;
+    for (int64_t k = blockIdx.x; k < count; k += gridDim.x) {
+        const int64_t pos = a.list ? (int64_t)a.list[k] : k;
         if (a.take && !a.take[pos]) {
             if (tid == 0) { a.out_row[pos] = __builtin_nan(""); a.out_col[pos] = __builtin_nan(""); }
             continue;   // uniform across the block
@@ -382,6 +386,325 @@ int py4_lds_strip(int qy, int qx, int R, bool u16, int *bytes) {
             }
         }
     return 0;
+}
+
+// ---- get_origin on the matrix cores: the filter approximated, the argmax's candidates recomputed exactly --------
+//
+// uint16 patterns of 128 x 128 pixels. The separable filter is two matrix products, g = G P G^T, with G the
+// 128 x 128 matrix of the weights (mode nearest folds the taps past an edge onto the edge pixel). On the matrix
+// cores: the pattern in its exact FP16 split (x & 0xF800) + (x & 0x07FF), G scaled by a power of two and split into
+// FP16 hi + lo (lo parts below the FP16 normal range zeroed on the host), FP32 sums. The first product
+// B1 = P G^T leaves the accumulators in the layout the second product's B operand needs (the second product's K
+// order is permuted to match, in G's second stored copy); they are scaled by a power of two and split into FP16
+// hi + lo again. Every term is non-negative, so each approximate value g~ is within a relative e of its exact value
+// g, plus an absolute slack for second-pass inputs below the FP16 normal range: the two FP32 sums of at most 512
+// terms each (2 x 512 x 2^-24), G's representation (measured on the host), B1's split and the dropped lo x lo
+// product. The pixel the reference picks has g = max g >= M~ / (1 + e), so its g~ >= M~ (1 - e) / (1 + e) - slack
+// > M~ (1 - 2e) - slack: the pixels at or above M~ (1 - margin) - slack, with margin >= 2e set on the host, contain
+// it. They (a few pixels around the peak of the smooth filtered pattern) are recomputed in Double in the reference's
+// operation order, the axis-0 values of their rows and then the axis-1 sum, and the first largest wins: the argmax
+// is the reference's. More than kPyCand candidates, or no positive value (an empty pattern), leave the pattern to
+// the exact kernel (the undecided list).
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float float16v __attribute__((ext_vector_type(16)));
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+constexpr int kPyN = kPyMfmaSide;             // detector side
+constexpr int kPyNB = kPyN / 32;              // 32-pixel blocks per side (and waves per block)
+constexpr int kPyKS = kPyN / 16;              // k-steps of one product
+constexpr int kPyCand = 32;                   // candidates recomputed exactly
+constexpr int kPyRows = 4;                    // distinct candidate rows whose axis-0 values are held at once
+constexpr int kPyTaps = 511;                  // weights held in LDS (2R + 1 at most: R <= 255)
+
+__device__ __forceinline__ uint32_t py_cvt_pair(uint32_t w) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(__builtin_bit_cast(us2, w), h2));
+}
+
+__global__ __launch_bounds__(64 * kPyNB) __attribute__((amdgpu_waves_per_eu(1, 1))) void py4_origin_mfma(Py4Launch a) {
+    constexpr int N = kPyN, NB = kPyNB, KS = kPyKS;
+    // pattern rows padded by 16 bytes: the first product's A operands (16 bytes of 32 consecutive rows at one column)
+    // then fall on distinct banks
+    constexpr int NP = N + 8;
+    __shared__ half8 g2s[2][NB][KS][64];                       // the second product's A operands (hi, lo)
+    __shared__ __align__(16) uint16_t pat[2][N * NP];          // the pattern, double-buffered
+    __shared__ double arow[kPyRows][N];                        // exact axis-0 values of candidate rows
+    __shared__ float wmax[NB];
+    __shared__ int cand[kPyCand], cand_n, rows_of[kPyRows];
+    __shared__ double cand_g[kPyCand];
+    __shared__ double red_s[3][NB];
+    __shared__ double w[kPyTaps];                              // the weights, for the exact recomputation
+    const int tid = threadIdx.x, lane = tid & 63, n = lane & 31, h = lane >> 5;
+    const int J = __builtin_amdgcn_readfirstlane(tid >> 6);    // this wave's 32-column block
+    const int R = a.R;
+    for (int t = tid; t < 2 * R + 1; t += 64 * NB) w[t] = a.weights[t];
+    const double centre = a.weights[R];
+
+    // the first product's B operands (rows 32 J + n of G, columns 16 k + 8 h ..), kept in registers
+    half8 gb[KS][2];
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+            gb[k][p] = *(const half8 *)(a.gmat + (size_t)p * N * N + (size_t)(32 * J + n) * N + 16 * k + 8 * h);
+    for (int e = tid; e < 2 * NB * KS * 64; e += 64 * NB) {
+        const int l = e & 63, k = (e >> 6) % KS, I = (e / (64 * KS)) % NB, p = e / (64 * KS * NB);
+        g2s[p][I][k][l] = *(const half8 *)(a.gmat + (size_t)(2 + p) * N * N + (size_t)(32 * I + (l & 31)) * N +
+                                           16 * k + 8 * (l >> 5));
+    }
+
+    // positions blockIdx.x, + gridDim.x, ... that are taken
+    auto next_taken = [&](int64_t k) {
+        while (k < a.npos && a.take && !a.take[k]) {
+            a.out_row[k] = __builtin_nan("");                   // every thread writes the same value
+            a.out_col[k] = __builtin_nan("");
+            k += gridDim.x;
+        }
+        return k;
+    };
+    constexpr int PV = N * N / 8 / (64 * NB);                  // 16-byte pieces per thread
+    u32x4 pf[PV];
+    auto fetch = [&](int64_t pos) {
+        const u32x4 *src = (const u32x4 *)((const uint16_t *)a.cube + pos * (int64_t)(N * N));
+#pragma unroll
+        for (int v = 0; v < PV; ++v) pf[v] = __builtin_nontemporal_load((const CUBO_GLOBAL u32x4 *)src + tid + v * 64 * NB);
+    };
+    // the pattern from registers into LDS buffer b; whether any count needs the FP16 high part (the block's OR)
+    auto stage = [&](int b) {
+        uint32_t any = 0;
+#pragma unroll
+        for (int v = 0; v < PV; ++v) {
+            const int e = tid + v * 64 * NB;                   // 16-byte piece e % 16 of row e / 16
+            *(u32x4 *)&pat[b][(e >> 4) * NP + 8 * (e & 15)] = pf[v];
+            any |= (pf[v].x | pf[v].y | pf[v].z | pf[v].w) & 0xf800f800u;
+        }
+        return __syncthreads_or(any != 0) != 0;
+    };
+    // B1 = P G^T for this wave's 32 columns: acc1[I] rows 32 I + 8 (r / 4) + 4 h + r % 4, column 32 J + n. The row
+    // blocks' accumulations interleave (k outer), so consecutive matrix instructions do not depend on each other.
+    auto pass1 = [&](const uint16_t *P, float16v (&acc1)[NB], auto hi_tag) {
+        constexpr bool HI = decltype(hi_tag)::value;
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc1[I][r] = 0.f;
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            u32x4 x[NB];
+#pragma unroll
+            for (int I = 0; I < NB; ++I) x[I] = *(const u32x4 *)&P[(32 * I + n) * NP + 16 * k + 8 * h];
+#pragma unroll
+            for (int I = 0; I < NB; ++I) {
+                u32x4 lo;
+                lo.x = py_cvt_pair(x[I].x & 0x07ff07ffu); lo.y = py_cvt_pair(x[I].y & 0x07ff07ffu);
+                lo.z = py_cvt_pair(x[I].z & 0x07ff07ffu); lo.w = py_cvt_pair(x[I].w & 0x07ff07ffu);
+                const half8 alo = __builtin_bit_cast(half8, lo);
+                acc1[I] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, gb[k][0], acc1[I], 0, 0, 0);
+                acc1[I] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, gb[k][1], acc1[I], 0, 0, 0);
+                if constexpr (HI) {
+                    u32x4 hi;
+                    hi.x = py_cvt_pair(x[I].x & 0xf800f800u); hi.y = py_cvt_pair(x[I].y & 0xf800f800u);
+                    hi.z = py_cvt_pair(x[I].z & 0xf800f800u); hi.w = py_cvt_pair(x[I].w & 0xf800f800u);
+                    const half8 ahi = __builtin_bit_cast(half8, hi);
+                    acc1[I] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, gb[k][0], acc1[I], 0, 0, 0);
+                    acc1[I] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, gb[k][1], acc1[I], 0, 0, 0);
+                }
+            }
+        }
+    };
+    auto undecided = [&](int64_t pos) {
+        if (tid == 0) a.undecided[atomicAdd(a.n_undecided, 1)] = (int32_t)pos;
+    };
+
+    int64_t pos = next_taken(blockIdx.x);
+    int cur = 0;
+    bool phi = false;                                          // the current pattern has counts of 2048 or more
+    if (pos < a.npos) {
+        fetch(pos);
+        phi = stage(0);
+    }
+    while (pos < a.npos) {
+        const int64_t nxt = next_taken(pos + gridDim.x);
+        if (nxt < a.npos) fetch(nxt);
+        const uint16_t *P = pat[cur];
+
+        float16v acc1[NB];
+        if (phi) pass1(P, acc1, std::true_type{});
+        else pass1(P, acc1, std::false_type{});
+        // g~ = G B1: k-step k takes rows 32 (k / 2) + 16 (k % 2) + 8 (e / 4) + 4 h + e % 4 of B1 (registers
+        // 8 (k % 2) + e of acc1[k / 2]); G's second copy has its columns in that order
+        float16v acc2[NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc2[I][r] = 0.f;
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            half8 bh, bl;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float v = acc1[k / 2][8 * (k % 2) + e] * a.pass2_scale;
+                const _Float16 hv = (_Float16)v;
+                bh[e] = hv;
+                bl[e] = (_Float16)(v - (float)hv);
+            }
+#pragma unroll
+            for (int I = 0; I < NB; ++I) {
+                const half8 ah = g2s[0][I][k][lane], al = g2s[1][I][k][lane];
+                acc2[I] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc2[I], 0, 0, 0);
+                acc2[I] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc2[I], 0, 0, 0);
+                acc2[I] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc2[I], 0, 0, 0);
+            }
+        }
+        // the largest approximate value, then the candidates at or above the threshold (only waves holding some
+        // look at their values one by one)
+        float m = 0.f;
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) m = __builtin_fmaxf(m, acc2[I][r]);
+        float wm = m;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) wm = __builtin_fmaxf(wm, __shfl_xor(wm, off, 64));
+        if (lane == 0) wmax[J] = wm;
+        if (tid == 0) cand_n = 0;
+        __syncthreads();
+        float mx = wmax[0];
+#pragma unroll
+        for (int k = 1; k < NB; ++k) mx = __builtin_fmaxf(mx, wmax[k]);
+        const float th = mx * (1.0f - a.candidate_margin) - a.slack;
+        if (mx > 0.f && th > 0.f && __builtin_amdgcn_ballot_w64(m >= th) != 0) {
+            uint64_t bits = 0;                                 // bit 16 I + r: acc2[I][r] >= th
+#pragma unroll
+            for (int I = 0; I < NB; ++I)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) bits |= acc2[I][r] >= th ? (uint64_t)1 << (16 * I + r) : 0;
+            while (bits) {
+                const int b = __builtin_ctzll(bits), I = b >> 4, r = b & 15;
+                bits &= bits - 1;
+                const int i = 32 * I + 8 * (r >> 2) + 4 * h + (r & 3), j = 32 * J + n;
+                const int slot = atomicAdd(&cand_n, 1);
+                if (slot < kPyCand) cand[slot] = i * N + j;
+            }
+        }
+        __syncthreads();
+        const int nc = cand_n;
+        // the candidates' distinct rows (in candidate order; past kPyRows a repeat may count again: the pattern goes
+        // to the exact kernel either way), by every thread from LDS
+        int nr = 0;
+        if (nc >= 1 && nc <= kPyCand) {
+            for (int c = 0; c < nc && nr <= kPyRows; ++c) {
+                const int i = cand[c] / N;
+                bool seen = false;
+                for (int q = 0; q < min(nr, kPyRows); ++q) seen = seen || rows_of[q] == i;
+                if (!seen) {
+                    if (nr < kPyRows && tid == 0) rows_of[nr] = i;
+                    ++nr;
+                }
+                if (!seen && nr <= kPyRows) __syncthreads();   // every thread sees the row it just added
+            }
+        }
+        if (nc == 0 || nc > kPyCand || nr > kPyRows) {
+            // no positive value, or too many near the top: the exact kernel measures this pattern
+            undecided(pos);
+        } else {
+            // axis 0 in the reference's order: a(i, j') = p(i, j') c + sum_{t = R..1} (p(i - t, j') + p(i + t, j')) w[R - t]
+            // (eight taps' terms formed together, then added one by one in that order)
+            // (a wave's 64 lanes share one row q: the taps' rows and weights are uniform)
+            const uint16_t *Pc = pat[cur];
+            for (int e = tid; e < nr * N; e += 64 * NB) {
+                const int q = __builtin_amdgcn_readfirstlane(e / N), jj = e - q * N;
+                const int i = __builtin_amdgcn_readfirstlane(rows_of[q]);
+                double acc = (double)Pc[i * NP + jj] * centre;
+                int t = R;
+                for (; t >= 8; t -= 8) {
+                    double pr[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int tt = t - u, ia = i - tt < 0 ? 0 : i - tt, ib = i + tt >= N ? N - 1 : i + tt;
+                        pr[u] = ((double)Pc[ia * NP + jj] + (double)Pc[ib * NP + jj]) * a.weights[R - tt];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc += pr[u];
+                }
+                for (; t >= 1; --t) {
+                    const int ia = i - t < 0 ? 0 : i - t, ib = i + t >= N ? N - 1 : i + t;
+                    acc += ((double)Pc[ia * NP + jj] + (double)Pc[ib * NP + jj]) * a.weights[R - t];
+                }
+                arow[q][jj] = acc;
+            }
+            __syncthreads();
+            // axis 1: one thread per candidate
+            if (tid < nc) {
+                const int i = cand[tid] / N, j = cand[tid] - i * N;
+                int q = 0;
+                while (rows_of[q] != i) ++q;
+                const double *ar = arow[q];
+                double acc = ar[j] * centre;
+                int t = R;
+                for (; t >= 8; t -= 8) {
+                    double pr[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int tt = t - u, ja = j - tt < 0 ? 0 : j - tt, jb = j + tt >= N ? N - 1 : j + tt;
+                        pr[u] = (ar[ja] + ar[jb]) * w[R - tt];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc += pr[u];
+                }
+                for (; t >= 1; --t) {
+                    const int ja = j - t < 0 ? 0 : j - t, jb = j + t >= N ? N - 1 : j + t;
+                    acc += (ar[ja] + ar[jb]) * w[R - t];
+                }
+                cand_g[tid] = acc;
+            }
+            __syncthreads();
+            // the largest value, the smallest index among equal ones (uint16 patterns: no NaN), by every thread
+            int at = cand[0];
+            double v = cand_g[0];
+            for (int c = 1; c < nc; ++c)
+                if (cand_g[c] > v || (cand_g[c] == v && cand[c] < at)) { v = cand_g[c]; at = cand[c]; }
+            const int ai = at / N, aj = at - ai * N;
+            const double i0 = (double)ai, j0 = (double)aj;
+            if (a.fast_center) {
+                if (tid == 0) { a.out_row[pos] = i0; a.out_col[pos] = j0; }
+            } else {
+                // get_CoM(dp * mask). The pixels outside the disk add exact zeros, so only its bounding box is
+                // read; for integer counts every partial sum is an integer below 2^53, so plain Double adds in any
+                // order give the exact sums, as the reference's compensated sums do.
+                const int ext = (int)__builtin_ceil(a.limit) + 1;
+                const int r0 = max(0, ai - ext), r1 = min(N - 1, ai + ext), c0 = max(0, aj - ext), c1 = min(N - 1, aj + ext);
+                const int bw = c1 - c0 + 1, cells = (r1 - r0 + 1) * bw;
+                double s0 = 0, s1 = 0, s2 = 0;
+                for (int e = tid; e < cells; e += 64 * NB) {
+                    const int ii = r0 + e / bw, jj = c0 + e % bw;
+                    const double fi = (double)ii, fj = (double)jj;
+                    const double di = fi - i0, dj = fj - j0;
+                    const double v = (double)P[ii * NP + jj] * (__builtin_sqrt(di * di + dj * dj) < a.limit ? 1.0 : 0.0);
+                    s0 += v;
+                    s1 += fi * v;
+                    s2 += fj * v;
+                }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    s0 += __shfl_xor(s0, off, 64);
+                    s1 += __shfl_xor(s1, off, 64);
+                    s2 += __shfl_xor(s2, off, 64);
+                }
+                if (lane == 0) { red_s[0][J] = s0; red_s[1][J] = s1; red_s[2][J] = s2; }
+                __syncthreads();
+                if (tid == 0) {
+                    double S0 = 0, MR = 0, MC = 0;
+                    for (int k = 0; k < NB; ++k) { S0 += red_s[0][k]; MR += red_s[1][k]; MC += red_s[2][k]; }
+                    a.out_row[pos] = MR / S0;
+                    a.out_col[pos] = MC / S0;
+                }
+            }
+        }
+        // the next pattern into the other buffer (its last readers finished before the barriers above)
+        if (nxt < a.npos) phi = stage(cur ^ 1);
+        cur ^= 1;
+        pos = nxt;
+    }
 }
 
 // ---- iterated window (Float32) ----
@@ -567,6 +890,17 @@ hipError_t launch_py4_origin(const Py4Launch &a, int blocks, hipStream_t s) {
     }
     if (a.u16) hipLaunchKernelGGL(py4_origin<true>, dim3(blocks), dim3(kPyBlock), 0, s, a);
     else hipLaunchKernelGGL(py4_origin<false>, dim3(blocks), dim3(kPyBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+bool py4_mfma_supported(const Py4Launch &a) {
+    return a.u16 && a.qy == kPyN && a.qx == kPyN && !a.filtered && 2 * a.R + 1 <= kPyTaps;
+}
+
+hipError_t launch_py4_origin_mfma(const Py4Launch &a, int blocks, hipStream_t s) {
+    if (!py4_mfma_supported(a) || !a.gmat || !a.undecided || !a.n_undecided || a.list) return hipErrorInvalidValue;
+    if (a.npos <= 0) return hipSuccess;
+    hipLaunchKernelGGL(py4_origin_mfma, dim3(blocks), dim3(64 * kPyNB), 0, s, a);
     return hipGetLastError();
 }
 

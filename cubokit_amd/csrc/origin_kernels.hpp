@@ -26,6 +26,18 @@ struct Py4Launch {
     double *scratch;            // [gridDim.x][qy * qx]: the axis-0 pass of each block
     double *out_row, *out_col;  // [npos]
     double *filtered;           // optional: the whole filtered pattern of the tile's first position, or NULL
+    // optional: measure only positions list[0 .. *nlist) (the matrix-core kernel's undecided patterns)
+    const int32_t *list;
+    const int32_t *nlist;
+    // the matrix-core kernel (py4_origin_mfma): the filter matrix split into FP16 hi/lo, [4][n][n] (G hi, G lo, and
+    // G with permuted columns hi, lo), the scale of its second-pass input, the relative margin of the candidates,
+    // and where it lists the patterns it leaves to the exact kernel
+    const _Float16 *gmat;
+    float pass2_scale;
+    float candidate_margin;
+    float slack;                // the absolute margin of the candidates (second-pass inputs below FP16's normals)
+    int32_t *undecided;
+    int32_t *n_undecided;
 };
 
 struct IWLaunch {
@@ -42,6 +54,12 @@ struct IWLaunch {
 // Blocks of the get_origin kernel (each loops over positions with its own scratch pattern).
 int py4_blocks(int cus, int64_t npos);
 hipError_t launch_py4_origin(const Py4Launch &a, int blocks, hipStream_t s);
+// The matrix-core get_origin (py4_origin_mfma): uint16 patterns of kPyMfmaSide x kPyMfmaSide pixels, no filtered
+// pattern asked for. It measures every taken position but those it lists in undecided (n_undecided, cleared by the
+// caller), which launch_py4_origin with list = undecided then measures exactly.
+constexpr int kPyMfmaSide = 128;
+bool py4_mfma_supported(const Py4Launch &a);
+hipError_t launch_py4_origin_mfma(const Py4Launch &a, int blocks, hipStream_t s);
 hipError_t launch_iterated_window(const IWLaunch &a, hipStream_t s);
 
 }  // namespace cubo

@@ -73,6 +73,64 @@ def test_filter_shapes_against_the_oracle(dtype, det, radius):
     res.release()
 
 
+def _mfma_origin_cube():
+    """128 x 128 uint16 patterns for the matrix-core get_origin: disks of several radii and doses (some with counts
+    of 2048 and more, the data's FP16 high parts), a single bright pixel, two equal disks placed symmetrically, and
+    the patterns it must leave to the exact kernel: all zero, flat, a dose of a few counts."""
+    rng = np.random.default_rng(61)
+    n = 128
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    pats = []
+    for k in range(40):
+        cy, cx = rng.uniform(40, 88, 2)
+        rad = rng.uniform(8, 22)
+        dose = [500.0, 30000.0, 60.0][k % 3]
+        disk = dose / (1 + np.exp(((ii - cy) ** 2 + (jj - cx) ** 2) ** 0.5 - rad))
+        pats.append(rng.poisson(disk + 2.0))
+    one = np.zeros((n, n)); one[70, 45] = 1000; pats.append(one)
+    two = np.zeros((n, n))
+    for cy in (40.0, 87.0):
+        two += 800.0 / (1 + np.exp(((ii - cy) ** 2 + (jj - 63.5) ** 2) ** 0.5 - 10))
+    pats.append(np.round(two))
+    pats.append(np.zeros((n, n)))
+    pats.append(np.full((n, n), 7.0))
+    pats.append(rng.poisson(0.05, (n, n)).astype(float))
+    pats.append(rng.poisson(np.full((n, n), 3.0)))
+    pats += [rng.poisson(np.full((n, n), 2.0)) * 0 + pats[k] for k in range(2)]
+    arr = np.clip(np.array(pats), 0, 65535).astype(np.uint16)
+    return arr.reshape(6, 8, n, n)
+
+
+def test_matrix_core_get_origin_against_the_exact_kernel_and_the_oracle(monkeypatch):
+    """uint16 128 x 128 patterns take the matrix-core get_origin (the filter on the matrix cores, the argmax's
+    candidates recomputed in the reference's Double order): the origins within 1e-12 px of the exact kernel and of
+    the oracle (NaN where the pattern is empty), the fast centre (the filtered argmax) exactly the oracle's, with a
+    scan mask, and through scan-row tiles."""
+    cube = _mfma_origin_cube()
+    print("case: matrix-core get_origin, 48 patterns", flush=True)
+    res = C.HIPResidentCube(cube)
+    for radius in (14.1, 5.3, 31.0):
+        r, c = O.origin_py4dstem(cube, radius, 1.2)
+        omap, _ = C.OriginMeasurement.measure(res, radius, 1.2, tiling=C.ScanTiling.rows(4))
+        np.testing.assert_allclose(omap.row, r, rtol=0, atol=1e-12, equal_nan=True)
+        np.testing.assert_allclose(omap.column, c, rtol=0, atol=1e-12, equal_nan=True)
+        monkeypatch.setenv("CUBO_HIP_ORIGIN_KERNEL", "exact")
+        ex, _ = C.OriginMeasurement.measure(res, radius, 1.2)
+        monkeypatch.delenv("CUBO_HIP_ORIGIN_KERNEL")
+        np.testing.assert_allclose(omap.row, ex.row, rtol=0, atol=1e-12, equal_nan=True)
+        np.testing.assert_allclose(omap.column, ex.column, rtol=0, atol=1e-12, equal_nan=True)
+        fast, _ = C.OriginMeasurement.measure(res, radius, fast_center=True)
+        fr, fc = O.origin_py4dstem(cube, radius, 1.2, fast_center=True)
+        np.testing.assert_array_equal(fast.row, fr)
+        np.testing.assert_array_equal(fast.column, fc)
+    mask = np.arange(48) % 4 != 2
+    part, _ = C.OriginMeasurement.measure(res, 14.1, scan_mask=mask)
+    r, c = O.origin_py4dstem(cube, 14.1, 1.2)
+    assert np.all(np.isnan(part.row[~mask]))
+    np.testing.assert_allclose(part.row[mask], r[mask], rtol=0, atol=1e-12, equal_nan=True)
+    res.release()
+
+
 def test_argmax_ties_boundaries_and_nan():
     """OriginFitTests.swift:50-73 and 88-105 on the device."""
     q = 9
