@@ -43,7 +43,7 @@ struct cubo_hip_reduction {
     void *weights;          // packed uint16 pairs (integer) or Double rows (FP64)
     void *rowpart;          // [nslices][npos][nv]
     void *colsum_part;      // [groups][ndet] uint32 (integer) or double (FP64)
-    void *colmax_part;      // [groups][ndet] uint32 (integer) or float (FP64)
+    void *colmax_part;      // [groups][ndet] packed uint16 pairs (integer) or float (FP64)
     void *col_sum;          // uint64 or double [ndet], running over tiles
     void *col_max;          // uint32 or float [ndet]
     bool full;              // integer kernel in the FULL layout (one block per group, rows written in-kernel)
@@ -265,8 +265,9 @@ struct ScratchSizes {
 ScratchSizes scratch_sizes(const cubo_hip_reduction *r) {
     const cubo_hip_cube *c = r->cube;
     const size_t ndet = (size_t)c->ndet, ng = (size_t)r->max_groups;
+    // (integer path: the group maxima as packed uint16 pairs)
     return ScratchSizes{{r->full ? 0 : (size_t)r->nslices * c->npos * std::max(r->nv, 1) * 8,
-                         ng * ndet * (r->path == 2 ? 8 : 4), ng * ndet * 4, ndet * 8, ndet * 4}};
+                         ng * ndet * (r->path == 2 ? 8 : 4), ng * ndet * (r->path == 2 ? 4 : 2), ndet * 8, ndet * 4}};
 }
 
 // One tile's launches: the fused kernel, the row totals of the split layouts, the column totals. timed: bracket

@@ -324,13 +324,10 @@ __global__ __launch_bounds__(FULL ? kWave * kFullSlices : kBlock) void fused_u16
         for (int c = 0; c < CH; ++c)
             if (valid[c]) {
                 uint32_t *cs = a.colsum + (size_t)group * a.ndet + px[c];
-                uint32_t *cm = a.colmax + (size_t)group * a.ndet + px[c];
+                uint32_t *cm = a.colmax + ((size_t)group * a.ndet + px[c]) / 2;    // the maxima stay packed
                 reinterpret_cast<uint4 *>(cs)[0] = make_uint4(csum[c][0], csum[c][1], csum[c][2], csum[c][3]);
                 reinterpret_cast<uint4 *>(cs)[1] = make_uint4(csum[c][4], csum[c][5], csum[c][6], csum[c][7]);
-                reinterpret_cast<uint4 *>(cm)[0] = make_uint4(cmax[c][0] & 0xffffu, cmax[c][0] >> 16,
-                                                              cmax[c][1] & 0xffffu, cmax[c][1] >> 16);
-                reinterpret_cast<uint4 *>(cm)[1] = make_uint4(cmax[c][2] & 0xffffu, cmax[c][2] >> 16,
-                                                              cmax[c][3] & 0xffffu, cmax[c][3] >> 16);
+                reinterpret_cast<uint4 *>(cm)[0] = make_uint4(cmax[c][0], cmax[c][1], cmax[c][2], cmax[c][3]);
             }
         return;
     }
@@ -362,13 +359,10 @@ __global__ __launch_bounds__(FULL ? kWave * kFullSlices : kBlock) void fused_u16
             }
             if (valid[c]) {
                 uint32_t *cs = a.colsum + (size_t)group * a.ndet + px[c];
-                uint32_t *cm = a.colmax + (size_t)group * a.ndet + px[c];
+                uint32_t *cm = a.colmax + ((size_t)group * a.ndet + px[c]) / 2;    // the maxima stay packed
                 reinterpret_cast<uint4 *>(cs)[0] = make_uint4(csum[c][0], csum[c][1], csum[c][2], csum[c][3]);
                 reinterpret_cast<uint4 *>(cs)[1] = make_uint4(csum[c][4], csum[c][5], csum[c][6], csum[c][7]);
-                reinterpret_cast<uint4 *>(cm)[0] = make_uint4(cmax[c][0] & 0xffffu, cmax[c][0] >> 16,
-                                                              cmax[c][1] & 0xffffu, cmax[c][1] >> 16);
-                reinterpret_cast<uint4 *>(cm)[1] = make_uint4(cmax[c][2] & 0xffffu, cmax[c][2] >> 16,
-                                                              cmax[c][3] & 0xffffu, cmax[c][3] >> 16);
+                reinterpret_cast<uint4 *>(cm)[0] = make_uint4(cmax[c][0], cmax[c][1], cmax[c][2], cmax[c][3]);
             }
         }
     }
@@ -779,20 +773,23 @@ __global__ __launch_bounds__(64 * kColPhases) void accumulate_cols_int(const uin
     uint64_t s = 0;
     uint32_t m = 0;
     if (i < ndet) {
+        // the maxima are packed uint16 pairs: pixel i is the half i % 2 of word i / 2
+        const int sh = 16 * (i & 1);
+        const size_t half = (size_t)ndet / 2;
         int g = q;
         for (; g + 7 * kColPhases < ngroups; g += 8 * kColPhases) {
             uint32_t vs[8], vm[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 vs[u] = colsum[(size_t)(g + kColPhases * u) * ndet + i];
-                vm[u] = colmax[(size_t)(g + kColPhases * u) * ndet + i];
+                vm[u] = (colmax[(size_t)(g + kColPhases * u) * half + i / 2] >> sh) & 0xffffu;
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) { s += vs[u]; m = vm[u] > m ? vm[u] : m; }
         }
         for (; g < ngroups; g += kColPhases) {
             s += colsum[(size_t)g * ndet + i];
-            const uint32_t v = colmax[(size_t)g * ndet + i];
+            const uint32_t v = (colmax[(size_t)g * half + i / 2] >> sh) & 0xffffu;
             m = v > m ? v : m;
         }
     }

@@ -44,7 +44,7 @@ struct IntLaunch {
     const uint32_t *wpk;    // [nw][ndet/2] weights as packed uint16 pairs
     uint64_t *rowpart;      // [nslices][npos][nv]
     uint32_t *colsum;       // [ngroups][ndet]
-    uint32_t *colmax;       // [ngroups][ndet]
+    uint32_t *colmax;       // [ngroups][ndet / 2]: packed uint16 pairs, the lower pixel in the low half
     RowsOut out;            // row totals (written by the kernel itself in the FULL layout)
 };
 
