@@ -37,7 +37,7 @@ std::vector<double> gaussian_weights(double sigma) {
 }
 
 // The matrix-core get_origin's filter matrix (py4_origin_mfma): G[a][b] = the sum of the weights of the taps of
-// output a that mode nearest maps to input b, scaled by a power of two s (largest entry below 30000) and split into
+// output a that mode nearest maps to input b, scaled by a power of two s (largest entry below 60000) and split into
 // FP16 hi + lo (lo parts below FP16's normal range dropped), as [4][n][n]: hi and lo, then both again with each
 // row's columns in the order of the second product's k-steps: column 16 k + 8 h + e of the copy holds column
 // 32 (k / 2) + 16 (k % 2) + 8 (e / 4) + 4 h + e % 4 of G. Returns false when the representation's relative error
@@ -61,7 +61,7 @@ bool mfma_filter(const std::vector<double> &w, int n, MfmaFilter *f) {
         rowmax = std::max(rowmax, rs);
     }
     if (!(gmax > 0)) return false;
-    const double s = std::ldexp(1.0, (int)std::floor(std::log2(30000.0 / gmax)));
+    const double s = std::ldexp(1.0, (int)std::floor(std::log2(60000.0 / gmax)));
     const double tiny = std::ldexp(1.0, -14);             // FP16's smallest normal
     f->g.assign((size_t)4 * n * n, (_Float16)0.f);
     double rep = 0;                                        // largest relative error of hi + lo
@@ -88,7 +88,8 @@ bool mfma_filter(const std::vector<double> &w, int n, MfmaFilter *f) {
     const double p2 = std::ldexp(1.0, -(int)std::ceil(std::log2(b1max / 30000.0)));
     // e: the two FP32 sums (at most 512 terms each), G's representation, B1's split and the dropped lo x lo term
     const double e = 2 * 512 * std::ldexp(1.0, -24) + rep + 2 * std::ldexp(1.0, -22);
-    const double margin = 1e-3;
+    // the candidates' margin: twice the bound with room to spare (rounded up to 5e-4, or 1e-3)
+    const double margin = 2 * e < 2.5e-4 ? 5e-4 : 1e-3;
     if (!(2 * e < margin)) return false;
     f->pass2_scale = (float)p2;
     f->margin = (float)margin;

@@ -608,12 +608,13 @@ __global__ __launch_bounds__(64 * kPyNB) __attribute__((amdgpu_waves_per_eu(1, 1
             undecided(pos);
         } else {
             // axis 0 in the reference's order: a(i, j') = p(i, j') c + sum_{t = R..1} (p(i - t, j') + p(i + t, j')) w[R - t]
-            // (eight taps' terms formed together, then added one by one in that order)
             // (a wave's 64 lanes share one row q: the taps' rows and weights are uniform)
             const uint16_t *Pc = pat[cur];
             for (int e = tid; e < nr * N; e += 64 * NB) {
                 const int q = __builtin_amdgcn_readfirstlane(e / N), jj = e - q * N;
                 const int i = __builtin_amdgcn_readfirstlane(rows_of[q]);
+                // (the two counts added as integers, exact: Double(p_a) + Double(p_b) to the bit; eight taps' terms
+                // formed together, then added one by one in order)
                 double acc = (double)Pc[i * NP + jj] * centre;
                 int t = R;
                 for (; t >= 8; t -= 8) {
@@ -621,14 +622,14 @@ __global__ __launch_bounds__(64 * kPyNB) __attribute__((amdgpu_waves_per_eu(1, 1
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         const int tt = t - u, ia = i - tt < 0 ? 0 : i - tt, ib = i + tt >= N ? N - 1 : i + tt;
-                        pr[u] = ((double)Pc[ia * NP + jj] + (double)Pc[ib * NP + jj]) * a.weights[R - tt];
+                        pr[u] = (double)((uint32_t)Pc[ia * NP + jj] + (uint32_t)Pc[ib * NP + jj]) * w[R - tt];
                     }
 #pragma unroll
                     for (int u = 0; u < 8; ++u) acc += pr[u];
                 }
                 for (; t >= 1; --t) {
                     const int ia = i - t < 0 ? 0 : i - t, ib = i + t >= N ? N - 1 : i + t;
-                    acc += ((double)Pc[ia * NP + jj] + (double)Pc[ib * NP + jj]) * a.weights[R - t];
+                    acc += (double)((uint32_t)Pc[ia * NP + jj] + (uint32_t)Pc[ib * NP + jj]) * w[R - t];
                 }
                 arow[q][jj] = acc;
             }
