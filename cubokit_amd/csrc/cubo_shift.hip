@@ -312,7 +312,7 @@ int prepare_batches(cubo_hip_shift_plan *p) {
     if (p->batches_ready) return CUBO_HIP_OK;
     cubo_hip_cube *c = p->cube;
     const int64_t npos = c->npos;
-    const int qx = (int)c->shape[3];
+    const int qy = (int)c->shape[2], qx = (int)c->shape[3];
     std::vector<int64_t> count((size_t)qx + 1, 0);
     for (int64_t i = 0; i < npos; ++i) ++count[p->h_shift[2 * i + 1] + 1];
     int64_t nbatch = 0;
@@ -331,7 +331,7 @@ int prepare_batches(cubo_hip_shift_plan *p) {
             for (int sl = 0; sl < 16; ++sl) {
                 const bool live = t + sl < count[ic + 1];
                 const int32_t pos = sorted[live ? t + sl : t];
-                slots[(size_t)b * 32 + 2 * sl] = pos;
+                slots[(size_t)b * 32 + 2 * sl] = (int32_t)((uint32_t)pos * (uint32_t)qy);   // the first row
                 slots[(size_t)b * 32 + 2 * sl + 1] = p->h_shift[2 * (size_t)pos];
                 order[(size_t)b * 16 + sl] = live ? pos : -1;
             }
@@ -598,7 +598,8 @@ int cubo_hip_shift_backward(cubo_hip_shift_plan *p, const double *origin_row, co
             }
         const int qy = (int)c->shape[2], qx = (int)c->shape[3];
         const int cus = cubo::compute_units(c->device);
-        const bool tposed = cubo::shift_grad_t_supported(qy, qx);   // batches of one column shift, staged transposed
+        // batches of one column shift, staged transposed; the kernel indexes the cube's rows in 32 bits
+        const bool tposed = cubo::shift_grad_t_supported(qy, qx) && (uint64_t)npos * (uint64_t)qy <= 0xffffffffull;
         if (tposed)
             if (int st = prepare_batches(p)) return st;
         const int64_t units = tposed ? p->nbatch : nks;

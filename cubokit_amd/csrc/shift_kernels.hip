@@ -648,7 +648,10 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     __shared__ half8 sb[2][BVB];
     constexpr int WAVES = T / 64;
     static_assert(WAVES <= 8, "the flags are read as two 16-byte words");
-    __shared__ __align__(16) int32_t hiflag[2][8];   // per buffer and wave: the batch, if the wave saw high parts
+    __shared__ __align__(16) int32_t hiflag[2][8];   // per buffer and wave: whether the wave saw high parts
+    // byte offset in a plane of half h of column v mod QX of staged row rho, v < 2 QX: a lane's A-operand addresses
+    // are then one table read at its column plus the batch's column shift
+    __shared__ int32_t atab[RB][2][2 * QX];
     const int tid = threadIdx.x, lane = tid & 63, hf = lane >> 5, n = lane & 31;
     const int wave = tid >> 6, rw = wave / WPR, wc = wave % WPR;
     const int qy = g.qy;
@@ -657,10 +660,15 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     const int64_t b_begin = (int64_t)group * g.bpg, b_end = min(g.nbatch, b_begin + g.bpg);
     const CUBO_GLOBAL uint16_t *cube = (const CUBO_GLOBAL uint16_t *)g.cube;
     const CUBO_GLOBAL half8 *dyw = (const CUBO_GLOBAL half8 *)g.dyw;
-    const CUBO_GLOBAL u32x4 *slots = (const CUBO_GLOBAL u32x4 *)g.slots;   // per slot pair: pos, ir, pos, ir
+    // per slot pair: the first row of each position's pattern (position x qy) and its row shift
+    const CUBO_GLOBAL u32x4 *slots = (const CUBO_GLOBAL u32x4 *)g.slots;
 
     // dword address of half h of column j of staged row rho
     auto ent = [](int rho, int j, int h) { return rho * RS + 8 * (j + (j >> 3)) + 4 * (h ^ ((j >> 3) & 1)); };
+    for (int i = tid; i < RB * 4 * QX; i += T) {
+        const int v = i % (2 * QX), h = (i / (2 * QX)) & 1, rho = i / (4 * QX);
+        (&atab[0][0][0])[i] = 4 * ent(rho, v >= QX ? v - QX : v, h);
+    }
 
     // The staging tasks of this thread: v = tid + k T; its slot pair q = v & 7 is the same for every k.
     const int q = tid & 7;
@@ -679,19 +687,21 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
         arho[k] = (uint32_t)(a0 + rho >= qy ? a0 + rho - qy : a0 + rho);
     }
     auto load = [&](int set, int64_t bi) {
-        const uint32_t ir0 = rec.y, ir1 = rec.w;
-        const CUBO_GLOBAL uint16_t *x0 = cube + (int64_t)rec.x * ndet, *x1 = cube + (int64_t)rec.z * ndet;
+        // the slot record and everything issued before it (the previous batch's loads) have arrived: one wait before
+        // the branches of the partial task, so no path waits for the loads issued below
+        __builtin_amdgcn_s_waitcnt(0x0f70);          // vmcnt(0)
+        const uint32_t f0 = rec.x, ir0 = rec.y, f1 = rec.z, ir1 = rec.w;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int v = tid + k * T;
             if (TASKS % T == 0 || v < TASKS) {
                 const int c = (v >> 3) % C8;
-                // r < 2 qy; min(r, r - qy) as unsigned is r mod qy
+                // r < 2 qy; min(r, r - qy) as unsigned is r mod qy; the cube's row index fits 32 bits
                 uint32_t r0 = arho[k] + ir0, r1 = arho[k] + ir1;
-                r0 = min(r0, r0 - (uint32_t)qy);
-                r1 = min(r1, r1 - (uint32_t)qy);
-                reg[set][k][0] = *(const CUBO_GLOBAL u32x4 *)(x0 + r0 * QX + 8 * c);
-                reg[set][k][1] = *(const CUBO_GLOBAL u32x4 *)(x1 + r1 * QX + 8 * c);
+                r0 = min(r0, r0 - (uint32_t)qy) + f0;
+                r1 = min(r1, r1 - (uint32_t)qy) + f1;
+                reg[set][k][0] = *(const CUBO_GLOBAL u32x4 *)(cube + 8 * c + (uint64_t)r0 * QX);
+                reg[set][k][1] = *(const CUBO_GLOBAL u32x4 *)(cube + 8 * c + (uint64_t)r1 * QX);
             }
         }
 #pragma unroll
@@ -702,8 +712,10 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
     };
     // The FP16 parts of a pair of counts: low 10 bits exactly through the magic number 1024 (0x6400 | v is
     // 1024 + v), the high 6 bits (multiples of 1024 up to 64512) by conversion.
-    auto part_lo = [](uint32_t w) {
-        const h2 m = __builtin_bit_cast(h2, (w & 0x03ff03ffu) | 0x64006400u);
+    uint32_t magic = 0x64006400u;
+    asm volatile("" : "+v"(magic));                  // a register operand: the mask and the OR in one instruction
+    auto part_lo = [&](uint32_t w) {
+        const h2 m = __builtin_bit_cast(h2, (w & 0x03ff03ffu) | magic);
         const h2 r = m - (h2){(_Float16)1024.f, (_Float16)1024.f};
         return __builtin_bit_cast(uint32_t, r);
     };
@@ -738,23 +750,21 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
         if (TASKS % T != 0 && tid + KFULL * T < TASKS) any |= stage_task(set, KFULL, dst, hi);
         return any;
     };
-    int ic_buf[2] = {0, 0};                          // the column shift of the batch in each LDS buffer
-    // The batch's B operands and column shift, and the wave's high-part flag (every lane writes the same value).
-    auto store_misc = [&](int set, int b, int64_t bi, uint32_t any) {
+    // The batch's B operands and the wave's high-part flag (every lane writes the same value).
+    auto store_misc = [&](int set, int b, uint32_t any) {
 #pragma unroll
         for (int k = 0; k < BPER; ++k)
             if (BVB % T == 0 || tid + k * T < BVB) sb[b][tid + k * T] = breg[set][k];
-        ic_buf[b] = reg_ic[set];
-        hiflag[b][wave] = __builtin_amdgcn_ballot_w64(any != 0) != 0 ? (int32_t)bi : -1;
+        hiflag[b][wave] = __builtin_amdgcn_ballot_w64(any != 0) != 0;
     };
-    // Whether the batch bi in buffer b has high parts (uniform).
-    auto has_hi = [&](int b, int64_t bi) {
+    // Whether the batch in buffer b has high parts (uniform; every staging of a buffer rewrites its flags).
+    auto has_hi = [&](int b) {
         const u32x4 f0 = *(const u32x4 *)&hiflag[b][0], f1 = *(const u32x4 *)&hiflag[b][4];
         const uint32_t f[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
-        bool h = false;
+        uint32_t h = 0;
 #pragma unroll
-        for (int w = 0; w < WAVES; ++w) h |= f[w] == (uint32_t)bi;
-        return h;
+        for (int w = 0; w < WAVES; ++w) h |= f[w];
+        return __builtin_amdgcn_readfirstlane(h) != 0;
     };
 
     float16v acc[2];
@@ -765,23 +775,24 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
         for (int r = 0; r < 16; ++r) { acc[t][r] = 0.f; accd[t][r] = 0.0; }
     int nk = 0;
     half8 bh[4], bl[4];
-    int aaddr[2][2];
-    // The B operands and A addresses of the batch in buffer b.
+    int aaddr[2][2], anext[2][2];
+    const int32_t *tab = &atab[rw][hf][64 * wc + n];
+    // A-operand byte offsets of tile t and column corner cc in staged row rw (row rw + 1 is RS dwords further) for
+    // column shift ic
+    auto addresses = [&](int ic, int (&dst)[2][2]) {
+        const int32_t *tb = tab + ic;
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) dst[t][cc] = tb[32 * t + cc];
+    };
+    // The B operands of the batch in buffer b.
     auto operands = [&](int b) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             bh[c] = sb[b][(c * 2 + 0) * 64 + lane];
             bl[c] = sb[b][(c * 2 + 1) * 64 + lane];
         }
-        // A-operand addresses of tile t and column corner cc in staged row rw (row rw + 1 is RS further)
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int cc = 0; cc < 2; ++cc) {
-                int j = 64 * wc + 32 * t + n + cc + ic_buf[b];
-                j -= j >= QX ? QX : 0;
-                aaddr[t][cc] = ent(rw, j, hf);
-            }
     };
     auto mfma_tile = [&](const uint32_t *plane, int t) {
 #pragma unroll
@@ -789,7 +800,7 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
                 const int c = r + 2 * cc;
-                const half8 a = *(const half8 *)&plane[aaddr[t][cc] + r * RS];
+                const half8 a = *(const half8 *)((const char *)plane + aaddr[t][cc] + 4 * r * RS);
                 acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bh[c], acc[t], 0, 0, 0);
                 acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl[c], acc[t], 0, 0, 0);
             }
@@ -813,22 +824,24 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
 #pragma unroll
             for (int k = 0; k < KFULL; ++k) any |= stage_task(0, k, slo[0], false);
             any |= stage_rest(0, slo[0], false);
-            store_misc(0, 0, b_begin, any);
+            store_misc(0, 0, any);
         }
         __syncthreads();
-        if (has_hi(0, b_begin)) {
+        bool hi_cur = has_hi(0);                     // the high-part flag of the batch being multiplied
+        if (hi_cur) {
 #pragma unroll
             for (int k = 0; k < KFULL; ++k) stage_task(0, k, shi, true);
             stage_rest(0, shi, true);
             __syncthreads();
         }
+        addresses(reg_ic[0], aaddr);
         // step(bi, b, cur): batch bi is in LDS buffer b, batch bi + 1 (if STORE) in register set cur; the loads of
         // batch bi + 2 go to the other set while batch bi is multiplied. The staging of batch bi + 1 sits between the
-        // matrix-core instructions of batch bi (one basic block), so the two overlap.
+        // matrix-core instructions of batch bi (one basic block), so the two overlap, and so does the table read of
+        // batch bi + 1's A addresses.
         auto step = [&](int64_t bi, int b, int cur, bool store) {
             load(cur ^ 1, min(bi + 2, b_end - 1));
             operands(b);
-            const bool use_hi = has_hi(b, bi);
             uint32_t any = 0;
             mfma_tile(slo[b], 0);
             if (store) {
@@ -838,19 +851,27 @@ __global__ __launch_bounds__(64 * WPR * RB) void shift_grad_t(ShiftGradT g) {
             mfma_tile(slo[b], 1);
             if (store) {
                 any |= stage_rest(cur, slo[b ^ 1], false);
-                store_misc(cur, b ^ 1, bi + 1, any);
+                store_misc(cur, b ^ 1, any);
+                addresses(reg_ic[cur], anext);
             }
-            if (use_hi) {
+            if (hi_cur) {
                 mfma_tile(shi, 0);
                 mfma_tile(shi, 1);
             }
             flush();
             lds_barrier();
-            if (store && has_hi(b ^ 1, bi + 1)) {           // the high parts of batch bi + 1
+            if (store) {
+                hi_cur = has_hi(b ^ 1);
+                if (hi_cur) {                        // the high parts of batch bi + 1
 #pragma unroll
-                for (int k = 0; k < KFULL; ++k) stage_task(cur, k, shi, true);
-                stage_rest(cur, shi, true);
-                lds_barrier();
+                    for (int k = 0; k < KFULL; ++k) stage_task(cur, k, shi, true);
+                    stage_rest(cur, shi, true);
+                    lds_barrier();
+                }
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int cc = 0; cc < 2; ++cc) aaddr[t][cc] = anext[t][cc];
             }
         };
         int64_t bi = b_begin;

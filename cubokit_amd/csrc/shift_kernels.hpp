@@ -146,7 +146,7 @@ struct ShiftGradT {
     const uint16_t *cube;
     int32_t qy, qx;
     int64_t nbatch;           // batches of 16 slots
-    const int32_t *slots;     // [nbatch][16][2] (position, ir); a padding slot repeats a position of its batch
+    const int32_t *slots;     // [nbatch][16][2] (position x qy < 2^32, ir); a padding slot repeats a position of its batch
     const int32_t *bic;       // [nbatch] the batch's column shift
     const void *dyw;          // [nbatch][corner][hi, lo][lane][8] FP16 (split_dy with the batch order)
     const double *dy_inv;     // [32] 1 / column scale (0 past the masks)
