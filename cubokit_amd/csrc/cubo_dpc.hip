@@ -97,11 +97,16 @@ int field(cubo_hip_dpc *d, const double *fx, const double *fy, double *denom) {
     return CUBO_HIP_OK;
 }
 
-// dx, dy of phi into z and their mean square over the scan region into d->sum (device).
 // The differences of phi into z, and their squared-error sum into *sum (default: d->sum).
+// On grids the fused passes take, z is not stored: its row transforms go to tmp (the transform's first pass),
+// which update() completes.
+bool fused(const cubo_hip_dpc *d) { return !d->plan_r && cubo::dpc_fused_supported(d->Rp, d->Cp); }
+
 hipError_t forward(cubo_hip_dpc *d, const double *phi, double *sum = nullptr) {
     int nb = 0;
-    hipError_t e = cubo::launch_dpc_forward(grid(d), phi, d->fx, d->fy, d->z, d->partial, &nb, d->stream);
+    hipError_t e = fused(d) ? cubo::launch_dpc_forward_pass(grid(d), phi, d->fx, d->fy, d->tw_c, d->tmp, d->partial, &nb,
+                                                            d->stream)
+                            : cubo::launch_dpc_forward(grid(d), phi, d->fx, d->fy, d->z, d->partial, &nb, d->stream);
     if (e == hipSuccess) e = cubo::launch_sum_partials(d->partial, nb, sum ? sum : d->sum, d->stream);
     return e;
 }
@@ -114,6 +119,13 @@ hipError_t fft2(cubo_hip_dpc *d, double2 *data, bool inverse) {
 
 // phi_out = phi + step Re(IFFT(i gx FFT(dx) + i gy FFT(dy))), with z = dx + i dy from forward().
 hipError_t update(cubo_hip_dpc *d, const double *phi, double step, double *phi_out) {
+    if (fused(d)) {
+        hipError_t e = cubo::launch_fft2_second_pass(d->tmp, d->z, d->Rp, d->Cp, d->tw_r, d->stream);
+        if (e == hipSuccess) e = cubo::launch_dpc_spectrum(grid(d), d->z, d->f, d->stream);
+        if (e == hipSuccess)
+            e = cubo::launch_ifft2_update(grid(d), d->f, d->tmp, d->tw_r, d->tw_c, phi, step, phi_out, d->stream);
+        return e;
+    }
     hipError_t e = fft2(d, d->z, false);
     if (e == hipSuccess) e = cubo::launch_dpc_spectrum(grid(d), d->z, d->f, d->stream);
     if (e == hipSuccess) e = fft2(d, d->f, true);
@@ -125,8 +137,9 @@ hipError_t update(cubo_hip_dpc *d, const double *phi, double step, double *phi_o
 hipError_t vjp(cubo_hip_dpc *d, const double *phi, double step, const double *g_in, double g_err, double denom,
                double msq, double *g_out) {
     const int64_t n = (int64_t)d->Rp * d->Cp;
-    hipError_t e = cubo::launch_real_to_complex(g_in, d->z, n, d->stream);
-    if (e == hipSuccess) e = fft2(d, d->z, false);
+    hipError_t e = fused(d) ? cubo::launch_fft2_real(g_in, d->tmp, d->z, d->Rp, d->Cp, d->tw_r, d->tw_c, d->stream)
+                            : cubo::launch_real_to_complex(g_in, d->z, n, d->stream);
+    if (e == hipSuccess && !fused(d)) e = fft2(d, d->z, false);
     if (e == hipSuccess) e = cubo::launch_dpc_adjoint_spectrum(grid(d), d->z, d->f, d->stream);
     if (e == hipSuccess) e = fft2(d, d->f, true);
     // error = msq / denom with denom = mean(fx^2 + fy^2): the denominator's share of the field gradient

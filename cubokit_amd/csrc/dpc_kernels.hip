@@ -15,6 +15,17 @@
 namespace cubo {
 namespace {
 
+enum { kFftPlain = 0, kFftStencil = 1, kFftReal = 2, kFftUpdate = 3 };
+// What a fused pass of fft_rows_t reads and writes besides its rows.
+struct FftFuse {
+    DpcGrid g;
+    const double *phi, *fx, *fy;    // kFftStencil: the phase and the field; kFftUpdate: the phase
+    double *partial;                // kFftStencil
+    const double *real;             // kFftReal
+    double *phi_out;                // kFftUpdate
+    double step;
+};
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -85,22 +96,73 @@ __global__ __launch_bounds__(256) void fft_rows(double2 *data, int n, const doub
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = a[i];
 }
 
+// dx, dy of the scan position (r, c): the central differences of phi plus the field (DPC.swift:59-69).
+__device__ __forceinline__ void dpc_d(const DpcGrid &g, const double *phi, const double *fx, const double *fy, int r,
+                                      int c, double *dx, double *dy) {
+    const int Rp = g.Rp, Cp = g.Cp;
+    const double up = phi[(int64_t)((r - 1 + Rp) % Rp) * Cp + c], down = phi[(int64_t)((r + 1) % Rp) * Cp + c];
+    const double left = phi[(int64_t)r * Cp + (c - 1 + Cp) % Cp], right = phi[(int64_t)r * Cp + (c + 1) % Cp];
+    *dx = (up - down) / (2 * g.sr) + fx[(int64_t)r * g.C + c];
+    *dy = (left - right) / (2 * g.sc) + fy[(int64_t)r * g.C + c];
+}
+
+// dx^2 + dy^2 and phi + step u with one rounding order in the fused and the separate kernels
+__device__ __forceinline__ double dpc_sq(double dx, double dy) { return __builtin_fma(dx, dx, dy * dy); }
+__device__ __forceinline__ double dpc_step(double phi, double step, double u) { return __builtin_fma(step, u, phi); }
+
 // R rows of a power-of-two length n per block, each transformed in LDS by the same Stockham stages as fft_rows,
 // the results stored transposed: out[j][r0 + r] = scale X_r[j] (out is n x rows), for batch blockIdx.y (in: rows x n,
 // both arrays batch-strided by rows * n). Two such passes are a 2-D transform with no transpose kernels. LDS rows are
 // padded by 16 / R entries, so the transposed reads (R rows at each of 16 / R consecutive j per 16 lanes) fall on
 // distinct banks.
-template <bool INV, int R>
+//
+// MODE fuses the DPC step's element-wise work into a pass (the values and their rounding are those of the separate
+// kernels dpc_forward, real_to_complex and dpc_update):
+//   kFftStencil: the rows are z = dx + i dy of the padded grid, computed from phi and the field as they are loaded
+//                (zero outside the scan region), and f.partial[k] = dpc_forward's tree sum of dx^2 + dy^2 over the
+//                256 grid entries 256 k .. 256 k + 255 (R n a multiple of 256; the tree runs in buffer b);
+//   kFftReal:    the rows are (f.real, 0);
+//   kFftUpdate:  (second pass of an inverse transform) the result u is not stored: f.phi_out = phi + step Re(u).
+template <bool INV, int R, int MODE>
 __global__ __launch_bounds__(512) void fft_rows_t(const double2 *in, double2 *out, int n, int rows, const double2 *tw,
-                                                  double scale) {
+                                                  double scale, FftFuse f) {
     extern __shared__ double2 sh[];
     const int S = n + 16 / R;
     double2 *a = sh, *b = sh + R * S;
     const int r0 = blockIdx.x * R, lg = __builtin_ctz((unsigned)n);
     const int64_t boff = (int64_t)blockIdx.y * rows * n;
-    const double2 *x = in + boff + (int64_t)r0 * n;
-    for (int v = threadIdx.x; v < R * n; v += blockDim.x) a[(v >> lg) * S + (v & (n - 1))] = x[v];
-    __syncthreads();
+    if constexpr (MODE == kFftStencil) {
+        double *q = (double *)b;
+        for (int v = threadIdx.x; v < R * n; v += blockDim.x) {
+            const int r = r0 + (v >> lg), c = v & (n - 1);
+            double dx = 0.0, dy = 0.0, sq = 0.0;
+            if (r < f.g.R && c < f.g.C) {
+                dpc_d(f.g, f.phi, f.fx, f.fy, r, c, &dx, &dy);
+                sq = dpc_sq(dx, dy);
+            }
+            a[(v >> lg) * S + c] = make_double2(dx, dy);
+            q[v] = sq;
+        }
+        __syncthreads();
+        const int chunks = R * n / 256;
+        for (int h = 128; h > 0; h >>= 1) {
+            for (int v = threadIdx.x; v < chunks * h; v += blockDim.x) {
+                const int k = v / h, t = v - k * h;
+                q[k * 256 + t] += q[k * 256 + t + h];
+            }
+            __syncthreads();
+        }
+        for (int k = threadIdx.x; k < chunks; k += blockDim.x) f.partial[(int64_t)r0 * n / 256 + k] = q[k * 256];
+        __syncthreads();                                // b is the first stage's output
+    } else if constexpr (MODE == kFftReal) {
+        const double *x = f.real + boff + (int64_t)r0 * n;
+        for (int v = threadIdx.x; v < R * n; v += blockDim.x) a[(v >> lg) * S + (v & (n - 1))] = make_double2(x[v], 0.0);
+        __syncthreads();
+    } else {
+        const double2 *x = in + boff + (int64_t)r0 * n;
+        for (int v = threadIdx.x; v < R * n; v += blockDim.x) a[(v >> lg) * S + (v & (n - 1))] = x[v];
+        __syncthreads();
+    }
     int ns = 1;
     if (lg & 1) {
         const int half = n >> 1;
@@ -137,11 +199,21 @@ __global__ __launch_bounds__(512) void fft_rows_t(const double2 *in, double2 *ou
         __syncthreads();
         double2 *t = a; a = b; b = t;
     }
-    double2 *y = out + boff + r0;
-    for (int v = threadIdx.x; v < R * n; v += blockDim.x) {
-        const int r = v % R, j = v / R;
-        const double2 e = a[r * S + j];
-        y[(int64_t)j * rows + r] = make_double2(e.x * scale, e.y * scale);
+    if constexpr (MODE == kFftUpdate) {
+        const double *ph = f.phi + boff + r0;
+        double *po = f.phi_out + boff + r0;
+        for (int v = threadIdx.x; v < R * n; v += blockDim.x) {
+            const int r = v % R, j = v / R;
+            const int64_t o = (int64_t)j * rows + r;
+            po[o] = dpc_step(ph[o], f.step, a[r * S + j].x * scale);
+        }
+    } else {
+        double2 *y = out + boff + r0;
+        for (int v = threadIdx.x; v < R * n; v += blockDim.x) {
+            const int r = v % R, j = v / R;
+            const double2 e = a[r * S + j];
+            y[(int64_t)j * rows + r] = make_double2(e.x * scale, e.y * scale);
+        }
     }
 }
 
@@ -149,14 +221,15 @@ __global__ __launch_bounds__(512) void fft_rows_t(const double2 *in, double2 *ou
 int fft_rows_t_rows(int n) { return n <= 1024 ? 4 : 2; }
 
 // One pass of fft_rows_t over batch arrays of rows x n, or false if the shapes do not suit it.
+template <int MODE = kFftPlain>
 bool launch_fft_rows_t(const double2 *in, double2 *out, int batch, int rows, int n, const double2 *tw, bool inverse,
-                       double scale, hipStream_t s) {
+                       double scale, hipStream_t s, const FftFuse &f = FftFuse{}) {
     if (n < 4 || (n & (n - 1)) || n > kFftMax) return false;
     const int R = fft_rows_t_rows(n);
     if (rows % R) return false;
     const size_t lds = 2 * (size_t)R * (n + 16 / R) * sizeof(double2);
     const dim3 grid((unsigned)(rows / R), (unsigned)batch);
-#define CUBO_RT(I, RR) hipLaunchKernelGGL((fft_rows_t<I, RR>), grid, dim3(512), lds, s, in, out, n, rows, tw, scale)
+#define CUBO_RT(I, RR) hipLaunchKernelGGL((fft_rows_t<I, RR, MODE>), grid, dim3(512), lds, s, in, out, n, rows, tw, scale, f)
     if (R == 4) { if (inverse) CUBO_RT(true, 4); else CUBO_RT(false, 4); }
     else { if (inverse) CUBO_RT(true, 2); else CUBO_RT(false, 2); }
 #undef CUBO_RT
@@ -183,16 +256,6 @@ __global__ __launch_bounds__(256) void transpose(const double2 *in, double2 *out
     }
 }
 
-// dx, dy of the scan position (r, c): the central differences of phi plus the field (DPC.swift:59-69).
-__device__ __forceinline__ void dpc_d(const DpcGrid &g, const double *phi, const double *fx, const double *fy, int r,
-                                      int c, double *dx, double *dy) {
-    const int Rp = g.Rp, Cp = g.Cp;
-    const double up = phi[(int64_t)((r - 1 + Rp) % Rp) * Cp + c], down = phi[(int64_t)((r + 1) % Rp) * Cp + c];
-    const double left = phi[(int64_t)r * Cp + (c - 1 + Cp) % Cp], right = phi[(int64_t)r * Cp + (c + 1) % Cp];
-    *dx = (up - down) / (2 * g.sr) + fx[(int64_t)r * g.C + c];
-    *dy = (left - right) / (2 * g.sc) + fy[(int64_t)r * g.C + c];
-}
-
 __global__ __launch_bounds__(256) void dpc_forward(DpcGrid g, const double *phi, const double *fx, const double *fy,
                                                    double2 *z, double *partial) {
     __shared__ double red[256];
@@ -204,7 +267,7 @@ __global__ __launch_bounds__(256) void dpc_forward(DpcGrid g, const double *phi,
         double dx = 0.0, dy = 0.0;
         if (r < g.R && c < g.C) {
             dpc_d(g, phi, fx, fy, r, c, &dx, &dy);
-            sq = dx * dx + dy * dy;
+            sq = dpc_sq(dx, dy);
         }
         z[i] = make_double2(dx, dy);
     }
@@ -235,7 +298,7 @@ __global__ __launch_bounds__(256) void dpc_spectrum(DpcGrid g, const double2 *f,
 __global__ __launch_bounds__(256) void dpc_update(DpcGrid g, const double *phi, const double2 *u, double step,
                                                   double *phi_out) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < (int64_t)g.Rp * g.Cp) phi_out[i] = phi[i] + step * u[i].x;
+    if (i < (int64_t)g.Rp * g.Cp) phi_out[i] = dpc_step(phi[i], step, u[i].x);
 }
 
 __global__ __launch_bounds__(256) void real_to_complex(const double *in, double2 *z, int64_t n) {
@@ -537,6 +600,55 @@ hipError_t launch_fft2_batched(double2 *data, double2 *tmp, int batch, int rows,
 hipError_t launch_fft2(double2 *data, double2 *tmp, int rows, int cols, const double2 *tw_r, const double2 *tw_c,
                        bool inverse, hipStream_t s) {
     return launch_fft2_batched(data, tmp, 1, rows, cols, tw_r, tw_c, inverse, s);
+}
+
+bool dpc_fused_supported(int Rp, int Cp) {
+    const auto fits = [](int n, int m) { return n >= 4 && !(n & (n - 1)) && n <= kFftMax && m % fft_rows_t_rows(n) == 0; };
+    return fits(Cp, Rp) && fits(Rp, Cp) && (fft_rows_t_rows(Cp) * Cp) % 256 == 0;
+}
+
+hipError_t launch_dpc_forward_pass(const DpcGrid &g, const double *phi, const double *fx, const double *fy,
+                                   const double2 *tw_c, double2 *tmp, double *partial, int *nblocks, hipStream_t s) {
+    if (!dpc_fused_supported(g.Rp, g.Cp)) return hipErrorInvalidValue;
+    FftFuse f{};
+    f.g = g;
+    f.phi = phi;
+    f.fx = fx;
+    f.fy = fy;
+    f.partial = partial;
+    *nblocks = (int)((int64_t)g.Rp * g.Cp / 256);
+    launch_fft_rows_t<kFftStencil>(nullptr, tmp, 1, g.Rp, g.Cp, tw_c, false, 1.0, s, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_fft2_second_pass(const double2 *tmp, double2 *data, int rows, int cols, const double2 *tw_r,
+                                   hipStream_t s) {
+    if (!dpc_fused_supported(rows, cols)) return hipErrorInvalidValue;
+    launch_fft_rows_t(tmp, data, 1, cols, rows, tw_r, false, 1.0, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_fft2_real(const double *in, double2 *tmp, double2 *data, int rows, int cols, const double2 *tw_r,
+                            const double2 *tw_c, hipStream_t s) {
+    if (!dpc_fused_supported(rows, cols)) return hipErrorInvalidValue;
+    FftFuse f{};
+    f.real = in;
+    launch_fft_rows_t<kFftReal>(nullptr, tmp, 1, rows, cols, tw_c, false, 1.0, s, f);
+    launch_fft_rows_t(tmp, data, 1, cols, rows, tw_r, false, 1.0, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_ifft2_update(const DpcGrid &g, const double2 *spec, double2 *tmp, const double2 *tw_r,
+                               const double2 *tw_c, const double *phi, double step, double *phi_out, hipStream_t s) {
+    if (!dpc_fused_supported(g.Rp, g.Cp)) return hipErrorInvalidValue;
+    FftFuse f{};
+    f.g = g;
+    f.phi = phi;
+    f.phi_out = phi_out;
+    f.step = step;
+    launch_fft_rows_t(spec, tmp, 1, g.Rp, g.Cp, tw_c, true, 1.0, s);
+    launch_fft_rows_t<kFftUpdate>(tmp, nullptr, 1, g.Cp, g.Rp, tw_r, true, 1.0 / ((double)g.Rp * g.Cp), s, f);
+    return hipGetLastError();
 }
 
 hipError_t launch_dpc_forward(const DpcGrid &g, const double *phi, const double *fx, const double *fy, double2 *z,
