@@ -28,7 +28,7 @@ struct cubo_hip_dpc {
     hipStream_t stream = nullptr;
     int R, C, Rp, Cp;
     double sr, sc;
-    double *gx = nullptr, *gy = nullptr;
+    double *kx = nullptr, *ky = nullptr;     // numpy.fft.fftfreq of each side
     double2 *tw_r = nullptr, *tw_c = nullptr;
     cubo::FftPlan *plan_r = nullptr, *plan_c = nullptr;   // grids above kFftMax a side: the general-length plans
     double2 *z = nullptr, *f = nullptr, *tmp = nullptr;
@@ -51,7 +51,7 @@ void free_dpc(cubo_hip_dpc *d) {
     if (!d) return;
     DeviceGuard g(d->device);
     if (d->stream) (void)hipStreamSynchronize(d->stream);
-    void *bufs[] = {d->gx, d->gy, d->tw_r, d->tw_c, d->z, d->f, d->tmp, d->partial, d->sum, d->phi, d->phi2,
+    void *bufs[] = {d->kx, d->ky, d->tw_r, d->tw_c, d->z, d->f, d->tmp, d->partial, d->sum, d->phi, d->phi2,
                     d->fx, d->fy, d->g_fx, d->g_fy};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -77,7 +77,7 @@ cubo::DpcGrid grid(const cubo_hip_dpc *d) {
     cubo::DpcGrid g{};
     g.R = d->R; g.C = d->C; g.Rp = d->Rp; g.Cp = d->Cp;
     g.sr = d->sr; g.sc = d->sc;
-    g.gx = d->gx; g.gy = d->gy;
+    g.kx = d->kx; g.ky = d->ky;
     return g;
 }
 
@@ -182,31 +182,24 @@ int cubo_hip_dpc_create(int device, int32_t rows, int32_t cols, double padding, 
     d->sr = sampling_row; d->sc = sampling_col;
     DeviceGuard g(device);
     const size_t n = (size_t)Rp * Cp, N = (size_t)rows * cols;
-    // operators (DPC.swift:40-47): k from numpy.fft.fftfreq, kden = 1 / (kx^2 + ky^2) with infinity at 0
-    std::vector<double> kx(Rp), ky(Cp), gx(n), gy(n);
+    // operators (DPC.swift:40-47): k from numpy.fft.fftfreq, kden = 1 / (kx^2 + ky^2) with infinity at 0,
+    // gx = -0.25 kx kden, gy = -0.25 ky kden: formed in the spectrum kernels from kx and ky
+    std::vector<double> kx(Rp), ky(Cp);
     fftfreq(Rp, sampling_row, kx.data());
     fftfreq(Cp, sampling_col, ky.data());
-    for (int r = 0; r < Rp; ++r)
-        for (int c = 0; c < Cp; ++c) {
-            double den = kx[r] * kx[r] + ky[c] * ky[c];
-            if (r == 0 && c == 0) den = INFINITY;
-            const double kden = 1.0 / den;
-            gx[(int64_t)r * Cp + c] = -0.25 * kx[r] * kden;
-            gy[(int64_t)r * Cp + c] = -0.25 * ky[c] * kden;
-        }
     const bool small = Rp <= cubo::kFftMax && Cp <= cubo::kFftMax;
     const std::vector<double2> twr = cubo::fft_twiddles(small ? Rp : 1), twc = cubo::fft_twiddles(small ? Cp : 1);
     hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     struct { void **p; size_t bytes; } a[] = {
-        {(void **)&d->gx, n * 8}, {(void **)&d->gy, n * 8}, {(void **)&d->tw_r, twr.size() * 16}, {(void **)&d->tw_c, twc.size() * 16},
+        {(void **)&d->kx, (size_t)Rp * 8}, {(void **)&d->ky, (size_t)Cp * 8}, {(void **)&d->tw_r, twr.size() * 16}, {(void **)&d->tw_c, twc.size() * 16},
         {(void **)&d->z, n * 16}, {(void **)&d->f, n * 16}, {(void **)&d->tmp, n * 16},
         {(void **)&d->partial, ((n + 255) / 256) * 8}, {(void **)&d->sum, 8}, {(void **)&d->phi, n * 8},
         {(void **)&d->phi2, n * 8}, {(void **)&d->fx, N * 8}, {(void **)&d->fy, N * 8}, {(void **)&d->g_fx, N * 8},
         {(void **)&d->g_fy, N * 8}};
     for (auto &x : a)
         if (e == hipSuccess) e = hipMalloc(x.p, x.bytes);
-    if (e == hipSuccess) e = hipMemcpy(d->gx, gx.data(), n * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d->gy, gy.data(), n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d->kx, kx.data(), (size_t)Rp * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d->ky, ky.data(), (size_t)Cp * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d->tw_r, twr.data(), twr.size() * 16, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d->tw_c, twc.data(), twc.size() * 16, hipMemcpyHostToDevice);
     // larger grids (DPC.swift:42-44 sets no limit): four-step and Bluestein plans of each side

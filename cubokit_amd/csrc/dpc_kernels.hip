@@ -280,19 +280,48 @@ __global__ __launch_bounds__(256) void dpc_forward(DpcGrid g, const double *phi,
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// The operators at frequency (r, c), rounded as the host's table was (no fused products).
+__device__ __forceinline__ void dpc_ops(const DpcGrid &g, int r, int c, double *gx, double *gy) {
+#pragma clang fp contract(off)
+    const double kx = g.kx[r], ky = g.ky[c];
+    double den = kx * kx + ky * ky;
+    if (r == 0 && c == 0) den = __builtin_inf();
+    const double kden = 1.0 / den;
+    *gx = -0.25 * kx * kden;
+    *gy = -0.25 * ky * kden;
+}
+
+// Index m of the frequency -k for k = i, and (r, c) of each; false when m < i (the pair is another thread's).
+__device__ __forceinline__ bool dpc_pair(const DpcGrid &g, int64_t i, int *r, int *c, int *rm, int *cm, int64_t *m) {
+    *r = (int)(i / g.Cp);
+    *c = (int)(i % g.Cp);
+    *rm = (g.Rp - *r) % g.Rp;
+    *cm = (g.Cp - *c) % g.Cp;
+    *m = (int64_t)*rm * g.Cp + *cm;
+    return *m >= i;
+}
+
 // F = FFT(dx + i dy): FFT(dx)[k] = (F[k] + conj F[-k]) / 2, FFT(dy)[k] = (F[k] - conj F[-k]) / (2i);
-// out = i (gx FFT(dx) + gy FFT(dy)).
+// the update's spectrum i (gx FFT(dx) + gy FFT(dy)) at k from fk = F[k], fm = F[-k].
+__device__ __forceinline__ double2 dpc_spec(const DpcGrid &g, int r, int c, double2 fk, double2 fm) {
+    const double2 X = make_double2(0.5 * (fk.x + fm.x), 0.5 * (fk.y - fm.y));
+    const double2 Y = make_double2(0.5 * (fk.y + fm.y), -0.5 * (fk.x - fm.x));
+    double gx, gy;
+    dpc_ops(g, r, c, &gx, &gy);
+    const double sx = gx * X.x + gy * Y.x, sy = gx * X.y + gy * Y.y;
+    return make_double2(-sy, sx);
+}
+
+// One thread per pair {k, -k}: both spectra from one read of each.
 __global__ __launch_bounds__(256) void dpc_spectrum(DpcGrid g, const double2 *f, double2 *out) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (int64_t)g.Rp * g.Cp) return;
-    const int r = (int)(i / g.Cp), c = (int)(i % g.Cp);
-    const int rm = (g.Rp - r) % g.Rp, cm = (g.Cp - c) % g.Cp;
-    const double2 fk = f[i], fm = f[(int64_t)rm * g.Cp + cm];
-    const double2 X = make_double2(0.5 * (fk.x + fm.x), 0.5 * (fk.y - fm.y));
-    const double2 Y = make_double2(0.5 * (fk.y + fm.y), -0.5 * (fk.x - fm.x));
-    const double gx = g.gx[i], gy = g.gy[i];
-    const double sx = gx * X.x + gy * Y.x, sy = gx * X.y + gy * Y.y;
-    out[i] = make_double2(-sy, sx);
+    int r, c, rm, cm;
+    int64_t m;
+    if (!dpc_pair(g, i, &r, &c, &rm, &cm, &m)) return;
+    const double2 fk = f[i], fm = f[m];
+    out[i] = dpc_spec(g, r, c, fk, fm);
+    if (m != i) out[m] = dpc_spec(g, rm, cm, fm, fk);
 }
 
 __global__ __launch_bounds__(256) void dpc_update(DpcGrid g, const double *phi, const double2 *u, double step,
@@ -309,59 +338,81 @@ __global__ __launch_bounds__(256) void real_to_complex(const double *in, double2
 // G = FFT(g), g real. The adjoint needs Re(IFFT(P)) and Re(IFFT(Q)) for P = G conj(i gx), Q = G conj(i gy).
 // Re(IFFT(P)) = IFFT of P's Hermitian part (P[k] + conj P[-k]) / 2 (P itself is not Hermitian on the Nyquist
 // row and column of an even length, where gx[-k] = gx[k]), so out = Herm(P) + i Herm(Q) gives both real parts
-// from one inverse transform.
+// from one inverse transform. At k from vk = G[k], vm = G[-k] and the operators at k and -k:
+__device__ __forceinline__ double2 dpc_adj(double2 vk, double2 vm, double gxk, double gxm, double gyk, double gym) {
+    // P[k] = -i gx[k] G[k] = (gx G.y, -gx G.x); conj P[-k] = (gx[-k] G[-k].y, gx[-k] G[-k].x); each product at k
+    // fused with the one at -k, written out so the rounding does not depend on the compiler's contraction
+    const double2 P = make_double2(0.5 * __builtin_fma(gxk, vk.y, gxm * vm.y), 0.5 * __builtin_fma(-gxk, vk.x, gxm * vm.x));
+    const double2 Q = make_double2(0.5 * __builtin_fma(gyk, vk.y, gym * vm.y), 0.5 * __builtin_fma(-gyk, vk.x, gym * vm.x));
+    return make_double2(P.x - Q.y, P.y + Q.x);
+}
+
 __global__ __launch_bounds__(256) void dpc_adjoint_spectrum(DpcGrid g, const double2 *G, double2 *out) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (int64_t)g.Rp * g.Cp) return;
-    const int r = (int)(i / g.Cp), c = (int)(i % g.Cp);
-    const int64_t m = (int64_t)((g.Rp - r) % g.Rp) * g.Cp + (g.Cp - c) % g.Cp;
+    int r, c, rm, cm;
+    int64_t m;
+    if (!dpc_pair(g, i, &r, &c, &rm, &cm, &m)) return;
     const double2 vk = G[i], vm = G[m];
-    // P[k] = -i gx[k] G[k] = (gx G.y, -gx G.x); conj P[-k] = (gx[-k] G[-k].y, gx[-k] G[-k].x)
-    const double gxk = g.gx[i], gxm = g.gx[m], gyk = g.gy[i], gym = g.gy[m];
-    const double2 P = make_double2(0.5 * (gxk * vk.y + gxm * vm.y), 0.5 * (-gxk * vk.x + gxm * vm.x));
-    const double2 Q = make_double2(0.5 * (gyk * vk.y + gym * vm.y), 0.5 * (-gyk * vk.x + gym * vm.x));
-    out[i] = make_double2(P.x - Q.y, P.y + Q.x);
+    double gxk, gyk, gxm, gym;
+    dpc_ops(g, r, c, &gxk, &gyk);
+    dpc_ops(g, rm, cm, &gxm, &gym);
+    out[i] = dpc_adj(vk, vm, gxk, gxm, gyk, gym);
+    if (m != i) out[m] = dpc_adj(vm, vk, gxm, gxk, gym, gyk);
 }
 
-__device__ __forceinline__ void gd(const DpcGrid &g, const double *phi, const double *fx, const double *fy,
-                                   const double2 *w, double step, double g_err, double denom, int r, int c,
-                                   double *a, double *b) {
-    // gdx, gdy of scan entry (r, c) scaled by the stencil's 1 / (2 sampling); zero off the scan region
-    if (r >= g.R || c >= g.C) { *a = 0.0; *b = 0.0; return; }
+// gdx, gdy of scan entry (r, c): dL/d(dx), dL/d(dy) of the step (the update's share w step and the error's).
+__device__ __forceinline__ void gdxy(const DpcGrid &g, const double *phi, const double *fx, const double *fy,
+                                     const double2 *w, double step, double g_err, double denom, int r, int c,
+                                     double *gdx, double *gdy) {
     double dx, dy;
     dpc_d(g, phi, fx, fy, r, c, &dx, &dy);
     const double2 wv = w[(int64_t)r * g.Cp + c];
     const double N = (double)g.R * g.C;
-    const double gdx = wv.x * step + g_err * 2.0 * dx / (N * denom);
-    const double gdy = wv.y * step + g_err * 2.0 * dy / (N * denom);
-    *a = gdx / (2 * g.sr);
-    *b = gdy / (2 * g.sc);
+    *gdx = wv.x * step + g_err * 2.0 * dx / (N * denom);
+    *gdy = wv.y * step + g_err * 2.0 * dy / (N * denom);
 }
 
+// The stencil's adjoint over tiles of kVjRows x kVjCols grid entries: the pushes a = gdx / (2 sr), b = gdy / (2 sc)
+// of the tile and a one-entry halo (wrapping round the padded grid, zero off the scan region) are formed once each in
+// LDS, then g_phi = g_phi_out + a(r + 1, c) - a(r - 1, c) + b(r, c + 1) - b(r, c - 1) (dx at (r + 1, c) pushes +a
+// into phi (r, c), dx at (r - 1, c) pushes -a; dy likewise along the row) and, on the scan region, the field's
+// gradient g_fx += gdx + dd 2 fx / N, g_fy likewise.
+constexpr int kVjRows = 8, kVjCols = 32;
 __global__ __launch_bounds__(256) void dpc_vjp(DpcGrid g, const double *phi, const double *fx, const double *fy,
                                                const double2 *w, double step, double g_err, double denom, double dd,
                                                const double *g_phi_out, double *g_phi, double *g_fx, double *g_fy) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (int64_t)g.Rp * g.Cp) return;
+    constexpr int HR = kVjRows + 2, HC = kVjCols + 2;
+    __shared__ double sa[HR][HC], sb[HR][HC], sx[kVjRows][kVjCols], sy[kVjRows][kVjCols];
     const int Rp = g.Rp, Cp = g.Cp;
-    const int r = (int)(i / Cp), c = (int)(i % Cp);
-    double a_dn, b_dn, a_up, b_up, a_rt, b_rt, a_lf, b_lf;
-    // dx at (r + 1, c) pushes +a into phi (r, c); dx at (r - 1, c) pushes -a; dy at (r, c + 1) pushes +b, at (r, c - 1) -b
-    gd(g, phi, fx, fy, w, step, g_err, denom, (r + 1) % Rp, c, &a_dn, &b_dn);
-    gd(g, phi, fx, fy, w, step, g_err, denom, (r - 1 + Rp) % Rp, c, &a_up, &b_up);
-    gd(g, phi, fx, fy, w, step, g_err, denom, r, (c + 1) % Cp, &a_rt, &b_rt);
-    gd(g, phi, fx, fy, w, step, g_err, denom, r, (c - 1 + Cp) % Cp, &a_lf, &b_lf);
-    g_phi[i] = g_phi_out[i] + a_dn - a_up + b_rt - b_lf;
+    const int r0 = blockIdx.y * kVjRows, c0 = blockIdx.x * kVjCols;
+    for (int p = threadIdx.x; p < HR * HC; p += blockDim.x) {
+        const int pr = p / HC, pc = p % HC;
+        const int r = (r0 - 1 + pr + Rp) % Rp, c = (c0 - 1 + pc + Cp) % Cp;
+        double a = 0.0, b = 0.0, gdx = 0.0, gdy = 0.0;
+        if (r < g.R && c < g.C) {
+            gdxy(g, phi, fx, fy, w, step, g_err, denom, r, c, &gdx, &gdy);
+            a = gdx / (2 * g.sr);
+            b = gdy / (2 * g.sc);
+        }
+        sa[pr][pc] = a;
+        sb[pr][pc] = b;
+        if (pr >= 1 && pr <= kVjRows && pc >= 1 && pc <= kVjCols) {
+            sx[pr - 1][pc - 1] = gdx;
+            sy[pr - 1][pc - 1] = gdy;
+        }
+    }
+    __syncthreads();
+    const int ty = threadIdx.x / kVjCols, tx = threadIdx.x % kVjCols;
+    const int r = r0 + ty, c = c0 + tx;
+    if (r >= Rp || c >= Cp) return;
+    const int64_t i = (int64_t)r * Cp + c;
+    g_phi[i] = g_phi_out[i] + sa[ty + 2][tx + 1] - sa[ty][tx + 1] + sb[ty + 1][tx + 2] - sb[ty + 1][tx];
     if (r < g.R && c < g.C) {
-        double dx, dy;
-        dpc_d(g, phi, fx, fy, r, c, &dx, &dy);
-        const double2 wv = w[i];
         const double N = (double)g.R * g.C;
         const int64_t k = (int64_t)r * g.C + c;
-        const double gdx = wv.x * step + g_err * 2.0 * dx / (N * denom);
-        const double gdy = wv.y * step + g_err * 2.0 * dy / (N * denom);
-        g_fx[k] += gdx + dd * 2.0 * fx[k] / N;
-        g_fy[k] += gdy + dd * 2.0 * fy[k] / N;
+        g_fx[k] += sx[ty][tx] + dd * 2.0 * fx[k] / N;
+        g_fy[k] += sy[ty][tx] + dd * 2.0 * fy[k] / N;
     }
 }
 
@@ -683,8 +734,9 @@ hipError_t launch_dpc_adjoint_spectrum(const DpcGrid &g, const double2 *G, doubl
 hipError_t launch_dpc_vjp(const DpcGrid &g, const double *phi, const double *fx, const double *fy, const double2 *w,
                           double step, double g_err, double denom, double dd, const double *g_phi_out, double *g_phi,
                           double *g_fx, double *g_fy, hipStream_t s) {
-    hipLaunchKernelGGL(dpc_vjp, dim3(blocks256((int64_t)g.Rp * g.Cp)), dim3(256), 0, s, g, phi, fx, fy, w, step, g_err,
-                       denom, dd, g_phi_out, g_phi, g_fx, g_fy);
+    const dim3 grid((unsigned)((g.Cp + kVjCols - 1) / kVjCols), (unsigned)((g.Rp + kVjRows - 1) / kVjRows));
+    hipLaunchKernelGGL(dpc_vjp, grid, dim3(kVjRows * kVjCols), 0, s, g, phi, fx, fy, w, step, g_err, denom, dd,
+                       g_phi_out, g_phi, g_fx, g_fy);
     return hipGetLastError();
 }
 

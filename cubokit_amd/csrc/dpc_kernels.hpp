@@ -51,7 +51,9 @@ hipError_t launch_fft2_planned(double2 *data, double2 *tmp, int rows, int cols, 
 struct DpcGrid {
     int32_t R, C, Rp, Cp;     // scan region and padded grid
     double sr, sc;            // sampling
-    const double *gx, *gy;    // [Rp][Cp] -0.25 k kden (the update operator is i g)
+    // [Rp], [Cp]: numpy.fft.fftfreq of each side. The kernels form the operators gx = -0.25 kx / (kx^2 + ky^2),
+    // gy = -0.25 ky / (kx^2 + ky^2) (0 at the origin; the update operator is i g) from them, in the host's order.
+    const double *kx, *ky;
 };
 
 // z = dx + i dy on the padded grid (zero outside the scan region; DPC.swift:59-69) and sq = dx^2 + dy^2
