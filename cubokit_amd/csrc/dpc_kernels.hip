@@ -110,7 +110,8 @@ __device__ __forceinline__ void dpc_d(const DpcGrid &g, const double *phi, const
 __device__ __forceinline__ double dpc_sq(double dx, double dy) { return __builtin_fma(dx, dx, dy * dy); }
 __device__ __forceinline__ double dpc_step(double phi, double step, double u) { return __builtin_fma(step, u, phi); }
 
-// R rows of a power-of-two length n per block, each transformed in LDS by the same Stockham stages as fft_rows,
+// R rows of a power-of-two length n per block (1024 threads: one radix-4 butterfly per thread and stage for 4 rows
+// of 1024), each transformed in LDS by the same Stockham stages as fft_rows,
 // the results stored transposed: out[j][r0 + r] = scale X_r[j] (out is n x rows), for batch blockIdx.y (in: rows x n,
 // both arrays batch-strided by rows * n). Two such passes are a 2-D transform with no transpose kernels. LDS rows are
 // padded by 16 / R entries, so the transposed reads (R rows at each of 16 / R consecutive j per 16 lanes) fall on
@@ -124,7 +125,7 @@ __device__ __forceinline__ double dpc_step(double phi, double step, double u) { 
 //   kFftReal:    the rows are (f.real, 0);
 //   kFftUpdate:  (second pass of an inverse transform) the result u is not stored: f.phi_out = phi + step Re(u).
 template <bool INV, int R, int MODE>
-__global__ __launch_bounds__(512) void fft_rows_t(const double2 *in, double2 *out, int n, int rows, const double2 *tw,
+__global__ __launch_bounds__(1024) void fft_rows_t(const double2 *in, double2 *out, int n, int rows, const double2 *tw,
                                                   double scale, FftFuse f) {
     extern __shared__ double2 sh[];
     const int S = n + 16 / R;
@@ -229,7 +230,7 @@ bool launch_fft_rows_t(const double2 *in, double2 *out, int batch, int rows, int
     if (rows % R) return false;
     const size_t lds = 2 * (size_t)R * (n + 16 / R) * sizeof(double2);
     const dim3 grid((unsigned)(rows / R), (unsigned)batch);
-#define CUBO_RT(I, RR) hipLaunchKernelGGL((fft_rows_t<I, RR, MODE>), grid, dim3(512), lds, s, in, out, n, rows, tw, scale, f)
+#define CUBO_RT(I, RR) hipLaunchKernelGGL((fft_rows_t<I, RR, MODE>), grid, dim3(1024), lds, s, in, out, n, rows, tw, scale, f)
     if (R == 4) { if (inverse) CUBO_RT(true, 4); else CUBO_RT(false, 4); }
     else { if (inverse) CUBO_RT(true, 2); else CUBO_RT(false, 2); }
 #undef CUBO_RT
