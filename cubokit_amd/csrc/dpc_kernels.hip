@@ -124,7 +124,11 @@ __device__ __forceinline__ double dpc_step(double phi, double step, double u) { 
 //                256 grid entries 256 k .. 256 k + 255 (R n a multiple of 256; the tree runs in buffer b);
 //   kFftReal:    the rows are (f.real, 0);
 //   kFftUpdate:  (second pass of an inverse transform) the result u is not stored: f.phi_out = phi + step Re(u).
-template <bool INV, int R, int MODE>
+// TWL: the radix-4 stages read their twiddles from per-stage tables in LDS behind the two row buffers, copied from
+// tw at the start: for the stage of span ns, w^(m k) (m = 1, 2, 3, k < ns) at offset (m - 1) ns + k of the stage's
+// table, so the lanes of a stage (consecutive k) read consecutive entries (tw itself is read at a stride of
+// n / (4 ns) entries, all in one bank). The values, and so the results, are tw's.
+template <bool INV, int R, int MODE, bool TWL>
 __global__ __launch_bounds__(1024) void fft_rows_t(const double2 *in, double2 *out, int n, int rows, const double2 *tw,
                                                   double scale, FftFuse f) {
     extern __shared__ double2 sh[];
@@ -132,6 +136,18 @@ __global__ __launch_bounds__(1024) void fft_rows_t(const double2 *in, double2 *o
     double2 *a = sh, *b = sh + R * S;
     const int r0 = blockIdx.x * R, lg = __builtin_ctz((unsigned)n);
     const int64_t boff = (int64_t)blockIdx.y * rows * n;
+    const int quarter = n >> 2, ns0 = (lg & 1) ? 2 : 1;
+    double2 *tws = sh + 2 * R * S;
+    if constexpr (TWL) {
+        int total = 0;
+        for (int m = ns0; m < n; m <<= 2) total += 3 * m;
+        for (int e = threadIdx.x; e < total; e += blockDim.x) {
+            int m = ns0, base = 0;
+            while (e >= base + 3 * m) { base += 3 * m; m <<= 2; }
+            const int p = (e - base) / m + 1, k = (e - base) - (p - 1) * m;
+            tws[e] = tw[p * k * (quarter / m)];
+        }
+    }
     if constexpr (MODE == kFftStencil) {
         double *q = (double *)b;
         for (int v = threadIdx.x; v < R * n; v += blockDim.x) {
@@ -177,14 +193,23 @@ __global__ __launch_bounds__(1024) void fft_rows_t(const double2 *in, double2 *o
         double2 *t = a; a = b; b = t;
         ns = 2;
     }
-    const int quarter = n >> 2;
-    for (; ns < n; ns <<= 2) {
+    int tb = 0;                                         // the stage's table (TWL)
+    for (; ns < n; tb += 3 * ns, ns <<= 2) {
         const int step = quarter / ns;
         for (int v = threadIdx.x; v < R * quarter; v += blockDim.x) {
             const int r = v >> (lg - 2), j = v & (quarter - 1), k = j & (ns - 1);
             const double2 *ar = a + r * S;
             double2 *br = b + r * S;
-            double2 w1 = tw[k * step], w2 = tw[2 * k * step], w3 = tw[3 * k * step];
+            double2 w1, w2, w3;
+            if constexpr (TWL) {
+                w1 = tws[tb + k];
+                w2 = tws[tb + ns + k];
+                w3 = tws[tb + 2 * ns + k];
+            } else {
+                w1 = tw[k * step];
+                w2 = tw[2 * k * step];
+                w3 = tw[3 * k * step];
+            }
             if (INV) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
             const double2 x0 = ar[j], x1 = cmul(ar[j + quarter], w1), x2 = cmul(ar[j + 2 * quarter], w2),
                           x3 = cmul(ar[j + 3 * quarter], w3);
@@ -228,11 +253,18 @@ bool launch_fft_rows_t(const double2 *in, double2 *out, int batch, int rows, int
     if (n < 4 || (n & (n - 1)) || n > kFftMax) return false;
     const int R = fft_rows_t_rows(n);
     if (rows % R) return false;
-    const size_t lds = 2 * (size_t)R * (n + 16 / R) * sizeof(double2);
+    // the per-stage twiddle tables when they fit beside the rows (n <= 1024)
+    size_t tw_entries = 0;
+    for (int m = (__builtin_ctz((unsigned)n) & 1) ? 2 : 1; m < n; m <<= 2) tw_entries += 3 * (size_t)m;
+    const size_t rows_lds = 2 * (size_t)R * (n + 16 / R) * sizeof(double2), tw_lds = tw_entries * sizeof(double2);
+    const bool twl = rows_lds + tw_lds <= 160 * 1024;
+    const size_t lds = rows_lds + (twl ? tw_lds : 0);
     const dim3 grid((unsigned)(rows / R), (unsigned)batch);
-#define CUBO_RT(I, RR) hipLaunchKernelGGL((fft_rows_t<I, RR, MODE>), grid, dim3(1024), lds, s, in, out, n, rows, tw, scale, f)
-    if (R == 4) { if (inverse) CUBO_RT(true, 4); else CUBO_RT(false, 4); }
-    else { if (inverse) CUBO_RT(true, 2); else CUBO_RT(false, 2); }
+#define CUBO_RT(I, RR, T) hipLaunchKernelGGL((fft_rows_t<I, RR, MODE, T>), grid, dim3(1024), lds, s, in, out, n, rows, tw, scale, f)
+#define CUBO_RT2(I, RR) { if (twl) CUBO_RT(I, RR, true); else CUBO_RT(I, RR, false); }
+    if (R == 4) { if (inverse) CUBO_RT2(true, 4) else CUBO_RT2(false, 4) }
+    else { if (inverse) CUBO_RT2(true, 2) else CUBO_RT2(false, 2) }
+#undef CUBO_RT2
 #undef CUBO_RT
     return true;
 }
