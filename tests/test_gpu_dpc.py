@@ -48,8 +48,11 @@ def _field(R, C, seed):
     return rng.normal(size=R * C), rng.normal(size=R * C)
 
 
-@pytest.mark.parametrize("R,C", [(32, 16), (7, 9), (40, 24)])
+@pytest.mark.parametrize("R,C", [(32, 16), (7, 9), (40, 24), (128, 64), (1024, 64)])
 def test_step_and_its_adjoint_match_the_oracle(R, C):
+    """One step and its adjoint against the oracle. Grids 64x32, 14x18 and 80x48 take the separate element-wise
+    kernels; 256x128 and 2048x128 the passes with them fused in (128-point rows: a radix-2 stage first, so the
+    per-stage twiddle tables start at span 2; 2048-point rows: the global twiddle table)."""
     d = DPC(R, C, sampling=(0.7, 1.3))
     Rp, Cp = d.padded_shape
     fx, fy = _field(R, C, R * C)
