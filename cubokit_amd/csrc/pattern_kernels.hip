@@ -482,9 +482,13 @@ __device__ __forceinline__ int swz_el(int i) {
     return i ^ (g | (g << 2)) ^ (((i >> 6) & 1) << 4);
 }
 
+// Entries of the per-stage twiddle tables for transforms of length up to l: 3 h for each radix-4 stage of quarter
+// h = 1, 2, .., l / 4 and h for each radix-2 stage of span h = 1, .., l / 2.
+__host__ __device__ constexpr int ff_twiddle_entries(int l) { return 3 * (l / 2 - 1) + (l - 1); }
+
 int fourier_fused_lds(int qy, int qx) {
     const int h = qx / 2 + 1, l = qy > qx ? qy : qx;
-    return 8 * (qy * (qx + 2) + 2 * qy + 2 * h + 2 * l + 2);
+    return 8 * (qy * (qx + 2) + 2 * qy + 2 * h + 2 * ff_twiddle_entries(l) + 2);
 }
 
 template <bool U16, int STAT, bool FIRST, int LM>
@@ -503,8 +507,13 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
     double2 *C = (double2 *)ff_lds;
     double2 *A = (double2 *)(ff_lds + N * R2);
     double2 *B = A + N;
-    double2 *TW = B + H;
-    double2 *NYQ = TW + L;
+    // Per-stage twiddle tables (the entries of a.tw, exp(-2 pi i m / L), that each stage uses, in the order its lanes
+    // read them, so consecutive lanes read consecutive entries instead of a.tw at a stride of L / (4 h)):
+    //   radix-4 stage of quarter h = 2^lq: TWS[3 (h - 1) + (m - 1) h + j] = a.tw[(m j) << (LL - lq - 2)], m = 1, 2, 3;
+    //   radix-2 stage of span h = 2^lh:    TWS[R4T + h - 1 + j] = a.tw[j << (LL - 1 - lh)], R4T = 3 (L / 2 - 1).
+    double2 *TWS = B + H;
+    const int R4T = 3 * (L / 2 - 1);
+    double2 *NYQ = TWS + ff_twiddle_entries(L);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int NP = N / 16;                               // pairs per wave
@@ -517,7 +526,23 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
     auto flip = [&](int q) { return swz_row(2 * q) & 1; };
     const int64_t e_begin = a.first + (int64_t)blockIdx.x * a.ppg;
     const int64_t e_end = min(a.first + a.count, e_begin + a.ppg);
-    for (int i = tid; i < L; i += kFfBlock) TW[i] = a.tw[i];
+    for (int e = tid; e < ff_twiddle_entries(L); e += kFfBlock) {
+        int idx;
+        if (e < R4T) {
+            int lq = 0;
+            while (e >= 3 * ((2 << lq) - 1)) ++lq;
+            const int h = 1 << lq, off = e - 3 * (h - 1), m = off / h + 1, j = off - (m - 1) * h;
+            idx = (m * j) << (LL - lq - 2);
+        } else {
+            const int e2 = e - R4T;
+            int lh = 0;
+            while (e2 >= (2 << lh) - 1) ++lh;
+            idx = (e2 - ((1 << lh) - 1)) << (LL - 1 - lh);
+        }
+        TWS[e] = a.tw[idx];
+    }
+    auto tw2 = [&](int lh, int j) { return TWS[R4T + (1 << lh) - 1 + j]; };
+    auto tw4 = [&](int lq, int m, int j) { return TWS[3 * ((1 << lq) - 1) + (m - 1) * (1 << lq) + j]; };
 
     // per slot: the running sum (or maximum) in Double and the TwoSum compensation in Float (it gathers rounding
     // errors far below the sum's last bit, so 24 bits of it are plenty; registers are short here)
@@ -553,7 +578,7 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             double *re = S + region(wave + kFfWaves * t), *im = re + R2;
             const double2 x = make_double2(re[i0], im[i0]), y = make_double2(re[i1], im[i1]);
             double2 d = csub(x, y);
-            if (h > 1) d = cmul(d, TW[j << (LL - 1 - lh)]);
+            if (h > 1) d = cmul(d, tw2(lh, j));
             re[i0] = x.x + y.x;
             im[i0] = x.y + y.y;
             re[i1] = d.x;
@@ -569,7 +594,7 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             double *re = S + region(wave + kFfWaves * t), *im = re + R2;
             const double2 x = make_double2(re[i0], im[i0]);
             double2 y = make_double2(re[i1], im[i1]);
-            if (h > 1) y = cmul_conj(y, TW[j << (LL - 1 - lh)]);
+            if (h > 1) y = cmul_conj(y, tw2(lh, j));
             re[i0] = x.x + y.x;
             im[i0] = x.y + y.y;
             re[i1] = x.x - y.x;
@@ -586,9 +611,8 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             double *re = S + region(wave + kFfWaves * t), *im = re + R2;
             double2 x0 = make_double2(re[e0], im[e0]), x1 = make_double2(re[e1], im[e1]);
             double2 x2 = make_double2(re[e2], im[e2]), x3 = make_double2(re[e3], im[e3]);
-            const int ts = LL - lq - 2;
-            if (inverse) dit4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], h > 1);
-            else dif4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], TW[(3 * j) << ts], h > 1);
+            if (inverse) dit4(x0, x1, x2, x3, tw4(lq, 1, j), tw4(lq, 2, j), h > 1);
+            else dif4(x0, x1, x2, x3, tw4(lq, 1, j), tw4(lq, 2, j), tw4(lq, 3, j), h > 1);
             re[e0] = x0.x; im[e0] = x0.y;
             re[e1] = x1.x; im[e1] = x1.y;
             re[e2] = x2.x; im[e2] = x2.y;
@@ -607,7 +631,7 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             const int r0 = swz_row(((b >> lh) << (lh + 1)) + j), r1 = swz_row(((b >> lh) << (lh + 1)) + j + h);
             const double2 x = C[r0 * H + k], y = C[r1 * H + k];
             double2 d = csub(x, y);
-            if (h > 1) d = cmul(d, TW[j << (LL - 1 - lh)]);
+            if (h > 1) d = cmul(d, tw2(lh, j));
             C[r0 * H + k] = cadd(x, y);
             C[r1 * H + k] = d;
         }
@@ -621,7 +645,7 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             const int r0 = swz_row(((b >> lh) << (lh + 1)) + j), r1 = swz_row(((b >> lh) << (lh + 1)) + j + h);
             const double2 x = C[r0 * H + k];
             double2 y = C[r1 * H + k];
-            if (h > 1) y = cmul_conj(y, TW[j << (LL - 1 - lh)]);
+            if (h > 1) y = cmul_conj(y, tw2(lh, j));
             C[r0 * H + k] = cadd(x, y);
             C[r1 * H + k] = csub(x, y);
         }
@@ -637,11 +661,10 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             double2 *c0 = C + swz_row(i0) * H + k, *c1 = C + swz_row(i0 + h) * H + k;
             double2 *c2 = C + swz_row(i0 + 2 * h) * H + k, *c3 = C + swz_row(i0 + 3 * h) * H + k;
             double2 x0 = *c0, x1 = *c1, x2 = *c2, x3 = *c3;
-            const int ts = LL - lq - 2;
             if (inverse) {
-                dit4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], h > 1);
+                dit4(x0, x1, x2, x3, tw4(lq, 1, j), tw4(lq, 2, j), h > 1);
             } else {
-                dif4(x0, x1, x2, x3, TW[j << ts], TW[(2 * j) << ts], TW[(3 * j) << ts], h > 1);
+                dif4(x0, x1, x2, x3, tw4(lq, 1, j), tw4(lq, 2, j), tw4(lq, 3, j), h > 1);
                 if (lq == 0) {                         // the last forward stage: frequencies bit-reversed
                     x0 = cmul(x0, ramp(bit_reverse(i0, LN), k));
                     x1 = cmul(x1, ramp(bit_reverse(i0 + 1, LN), k));
