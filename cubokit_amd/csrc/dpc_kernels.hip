@@ -418,7 +418,8 @@ __global__ __launch_bounds__(256) void dpc_vjp(DpcGrid g, const double *phi, con
     constexpr int HR = kVjRows + 2, HC = kVjCols + 2;
     __shared__ double sa[HR][HC], sb[HR][HC], sx[kVjRows][kVjCols], sy[kVjRows][kVjCols];
     const int Rp = g.Rp, Cp = g.Cp;
-    const int r0 = blockIdx.y * kVjRows, c0 = blockIdx.x * kVjCols;
+    const int ntx = (Cp + kVjCols - 1) / kVjCols;         // tiles along a row (one-dimensional grid of tiles)
+    const int r0 = (int)(blockIdx.x / ntx) * kVjRows, c0 = (int)(blockIdx.x % ntx) * kVjCols;
     for (int p = threadIdx.x; p < HR * HC; p += blockDim.x) {
         const int pr = p / HC, pc = p % HC;
         const int r = (r0 - 1 + pr + Rp) % Rp, c = (c0 - 1 + pc + Cp) % Cp;
@@ -767,9 +768,9 @@ hipError_t launch_dpc_adjoint_spectrum(const DpcGrid &g, const double2 *G, doubl
 hipError_t launch_dpc_vjp(const DpcGrid &g, const double *phi, const double *fx, const double *fy, const double2 *w,
                           double step, double g_err, double denom, double dd, const double *g_phi_out, double *g_phi,
                           double *g_fx, double *g_fy, hipStream_t s) {
-    const dim3 grid((unsigned)((g.Cp + kVjCols - 1) / kVjCols), (unsigned)((g.Rp + kVjRows - 1) / kVjRows));
-    hipLaunchKernelGGL(dpc_vjp, grid, dim3(kVjRows * kVjCols), 0, s, g, phi, fx, fy, w, step, g_err, denom, dd,
-                       g_phi_out, g_phi, g_fx, g_fy);
+    const int64_t tiles = (int64_t)((g.Cp + kVjCols - 1) / kVjCols) * ((g.Rp + kVjRows - 1) / kVjRows);
+    hipLaunchKernelGGL(dpc_vjp, dim3((unsigned)tiles), dim3(kVjRows * kVjCols), 0, s, g, phi, fx, fy, w, step, g_err,
+                       denom, dd, g_phi_out, g_phi, g_fx, g_fy);
     return hipGetLastError();
 }
 
