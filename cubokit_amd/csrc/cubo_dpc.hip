@@ -121,9 +121,8 @@ hipError_t fft2(cubo_hip_dpc *d, double2 *data, bool inverse) {
 hipError_t update(cubo_hip_dpc *d, const double *phi, double step, double *phi_out) {
     if (fused(d)) {
         hipError_t e = cubo::launch_fft2_second_pass(d->tmp, d->z, d->Rp, d->Cp, d->tw_r, d->stream);
-        if (e == hipSuccess) e = cubo::launch_dpc_spectrum(grid(d), d->z, d->f, d->stream);
         if (e == hipSuccess)
-            e = cubo::launch_ifft2_update(grid(d), d->f, d->tmp, d->tw_r, d->tw_c, phi, step, phi_out, d->stream);
+            e = cubo::launch_ifft2_spectrum_update(grid(d), d->z, d->tmp, d->tw_r, d->tw_c, phi, step, phi_out, d->stream);
         return e;
     }
     hipError_t e = fft2(d, d->z, false);
@@ -139,9 +138,14 @@ hipError_t vjp(cubo_hip_dpc *d, const double *phi, double step, const double *g_
     const int64_t n = (int64_t)d->Rp * d->Cp;
     hipError_t e = fused(d) ? cubo::launch_fft2_real(g_in, d->tmp, d->z, d->Rp, d->Cp, d->tw_r, d->tw_c, d->stream)
                             : cubo::launch_real_to_complex(g_in, d->z, n, d->stream);
-    if (e == hipSuccess && !fused(d)) e = fft2(d, d->z, false);
-    if (e == hipSuccess) e = cubo::launch_dpc_adjoint_spectrum(grid(d), d->z, d->f, d->stream);
-    if (e == hipSuccess) e = fft2(d, d->f, true);
+    if (fused(d)) {
+        if (e == hipSuccess)
+            e = cubo::launch_ifft2_adjoint_spectrum(grid(d), d->z, d->tmp, d->f, d->tw_r, d->tw_c, d->stream);
+    } else {
+        if (e == hipSuccess) e = fft2(d, d->z, false);
+        if (e == hipSuccess) e = cubo::launch_dpc_adjoint_spectrum(grid(d), d->z, d->f, d->stream);
+        if (e == hipSuccess) e = fft2(d, d->f, true);
+    }
     // error = msq / denom with denom = mean(fx^2 + fy^2): the denominator's share of the field gradient
     const double dd = -g_err * msq / (denom * denom);
     if (e == hipSuccess)

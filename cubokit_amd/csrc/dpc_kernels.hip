@@ -15,7 +15,7 @@
 namespace cubo {
 namespace {
 
-enum { kFftPlain = 0, kFftStencil = 1, kFftReal = 2, kFftUpdate = 3 };
+enum { kFftPlain = 0, kFftStencil = 1, kFftReal = 2, kFftUpdate = 3, kFftSpectrum = 4, kFftAdjointSpectrum = 5 };
 // What a fused pass of fft_rows_t reads and writes besides its rows.
 struct FftFuse {
     DpcGrid g;
@@ -110,6 +110,40 @@ __device__ __forceinline__ void dpc_d(const DpcGrid &g, const double *phi, const
 __device__ __forceinline__ double dpc_sq(double dx, double dy) { return __builtin_fma(dx, dx, dy * dy); }
 __device__ __forceinline__ double dpc_step(double phi, double step, double u) { return __builtin_fma(step, u, phi); }
 
+// The operators at frequency (r, c), rounded as the host's table was (no fused products).
+__device__ __forceinline__ void dpc_ops(const DpcGrid &g, int r, int c, double *gx, double *gy) {
+#pragma clang fp contract(off)
+    const double kx = g.kx[r], ky = g.ky[c];
+    double den = kx * kx + ky * ky;
+    if (r == 0 && c == 0) den = __builtin_inf();
+    const double kden = 1.0 / den;
+    *gx = -0.25 * kx * kden;
+    *gy = -0.25 * ky * kden;
+}
+
+// F = FFT(dx + i dy): FFT(dx)[k] = (F[k] + conj F[-k]) / 2, FFT(dy)[k] = (F[k] - conj F[-k]) / (2i);
+// the update's spectrum i (gx FFT(dx) + gy FFT(dy)) at k from fk = F[k], fm = F[-k].
+__device__ __forceinline__ double2 dpc_spec(const DpcGrid &g, int r, int c, double2 fk, double2 fm) {
+    const double2 X = make_double2(0.5 * (fk.x + fm.x), 0.5 * (fk.y - fm.y));
+    const double2 Y = make_double2(0.5 * (fk.y + fm.y), -0.5 * (fk.x - fm.x));
+    double gx, gy;
+    dpc_ops(g, r, c, &gx, &gy);
+    const double sx = gx * X.x + gy * Y.x, sy = gx * X.y + gy * Y.y;
+    return make_double2(-sy, sx);
+}
+
+// G = FFT(g), g real. The adjoint needs Re(IFFT(P)) and Re(IFFT(Q)) for P = G conj(i gx), Q = G conj(i gy).
+// Re(IFFT(P)) = IFFT of P's Hermitian part (P[k] + conj P[-k]) / 2 (P itself is not Hermitian on the Nyquist
+// row and column of an even length, where gx[-k] = gx[k]), so out = Herm(P) + i Herm(Q) gives both real parts
+// from one inverse transform. At k from vk = G[k], vm = G[-k] and the operators at k and -k:
+__device__ __forceinline__ double2 dpc_adj(double2 vk, double2 vm, double gxk, double gxm, double gyk, double gym) {
+    // P[k] = -i gx[k] G[k] = (gx G.y, -gx G.x); conj P[-k] = (gx[-k] G[-k].y, gx[-k] G[-k].x); each product at k
+    // fused with the one at -k, written out so the rounding does not depend on the compiler's contraction
+    const double2 P = make_double2(0.5 * __builtin_fma(gxk, vk.y, gxm * vm.y), 0.5 * __builtin_fma(-gxk, vk.x, gxm * vm.x));
+    const double2 Q = make_double2(0.5 * __builtin_fma(gyk, vk.y, gym * vm.y), 0.5 * __builtin_fma(-gyk, vk.x, gym * vm.x));
+    return make_double2(P.x - Q.y, P.y + Q.x);
+}
+
 // R rows of a power-of-two length n per block (1024 threads: one radix-4 butterfly per thread and stage for 4 rows
 // of 1024), each transformed in LDS by the same Stockham stages as fft_rows,
 // the results stored transposed: out[j][r0 + r] = scale X_r[j] (out is n x rows), for batch blockIdx.y (in: rows x n,
@@ -123,7 +157,10 @@ __device__ __forceinline__ double dpc_step(double phi, double step, double u) { 
 //                (zero outside the scan region), and f.partial[k] = dpc_forward's tree sum of dx^2 + dy^2 over the
 //                256 grid entries 256 k .. 256 k + 255 (R n a multiple of 256; the tree runs in buffer b);
 //   kFftReal:    the rows are (f.real, 0);
-//   kFftUpdate:  (second pass of an inverse transform) the result u is not stored: f.phi_out = phi + step Re(u).
+//   kFftUpdate:  (second pass of an inverse transform) the result u is not stored: f.phi_out = phi + step Re(u);
+//   kFftSpectrum, kFftAdjointSpectrum: (first pass of an inverse transform) the rows are the update's spectrum
+//                (dpc_spec) or the adjoint's (dpc_adj) of the forward transform in, formed from its entries at k and
+//                -k as they are loaded (the values dpc_spectrum and dpc_adjoint_spectrum write).
 // TWL: the radix-4 stages read their twiddles from per-stage tables in LDS behind the two row buffers, copied from
 // tw at the start: for the stage of span ns, w^(m k) (m = 1, 2, 3, k < ns) at offset (m - 1) ns + k of the stage's
 // table, so the lanes of a stage (consecutive k) read consecutive entries (tw itself is read at a stride of
@@ -171,6 +208,23 @@ __global__ __launch_bounds__(1024) void fft_rows_t(const double2 *in, double2 *o
         }
         for (int k = threadIdx.x; k < chunks; k += blockDim.x) f.partial[(int64_t)r0 * n / 256 + k] = q[k * 256];
         __syncthreads();                                // b is the first stage's output
+    } else if constexpr (MODE == kFftSpectrum || MODE == kFftAdjointSpectrum) {
+        for (int v = threadIdx.x; v < R * n; v += blockDim.x) {
+            const int r = r0 + (v >> lg), c = v & (n - 1);
+            const int rm = (f.g.Rp - r) % f.g.Rp, cm = (f.g.Cp - c) % f.g.Cp;
+            const double2 fk = in[(int64_t)r * n + c], fm = in[(int64_t)rm * n + cm];
+            double2 sv;
+            if constexpr (MODE == kFftSpectrum) {
+                sv = dpc_spec(f.g, r, c, fk, fm);
+            } else {
+                double gxk, gyk, gxm, gym;
+                dpc_ops(f.g, r, c, &gxk, &gyk);
+                dpc_ops(f.g, rm, cm, &gxm, &gym);
+                sv = dpc_adj(fk, fm, gxk, gxm, gyk, gym);
+            }
+            a[(v >> lg) * S + c] = sv;
+        }
+        __syncthreads();
     } else if constexpr (MODE == kFftReal) {
         const double *x = f.real + boff + (int64_t)r0 * n;
         for (int v = threadIdx.x; v < R * n; v += blockDim.x) a[(v >> lg) * S + (v & (n - 1))] = make_double2(x[v], 0.0);
@@ -313,17 +367,6 @@ __global__ __launch_bounds__(256) void dpc_forward(DpcGrid g, const double *phi,
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// The operators at frequency (r, c), rounded as the host's table was (no fused products).
-__device__ __forceinline__ void dpc_ops(const DpcGrid &g, int r, int c, double *gx, double *gy) {
-#pragma clang fp contract(off)
-    const double kx = g.kx[r], ky = g.ky[c];
-    double den = kx * kx + ky * ky;
-    if (r == 0 && c == 0) den = __builtin_inf();
-    const double kden = 1.0 / den;
-    *gx = -0.25 * kx * kden;
-    *gy = -0.25 * ky * kden;
-}
-
 // Index m of the frequency -k for k = i, and (r, c) of each; false when m < i (the pair is another thread's).
 __device__ __forceinline__ bool dpc_pair(const DpcGrid &g, int64_t i, int *r, int *c, int *rm, int *cm, int64_t *m) {
     *r = (int)(i / g.Cp);
@@ -332,17 +375,6 @@ __device__ __forceinline__ bool dpc_pair(const DpcGrid &g, int64_t i, int *r, in
     *cm = (g.Cp - *c) % g.Cp;
     *m = (int64_t)*rm * g.Cp + *cm;
     return *m >= i;
-}
-
-// F = FFT(dx + i dy): FFT(dx)[k] = (F[k] + conj F[-k]) / 2, FFT(dy)[k] = (F[k] - conj F[-k]) / (2i);
-// the update's spectrum i (gx FFT(dx) + gy FFT(dy)) at k from fk = F[k], fm = F[-k].
-__device__ __forceinline__ double2 dpc_spec(const DpcGrid &g, int r, int c, double2 fk, double2 fm) {
-    const double2 X = make_double2(0.5 * (fk.x + fm.x), 0.5 * (fk.y - fm.y));
-    const double2 Y = make_double2(0.5 * (fk.y + fm.y), -0.5 * (fk.x - fm.x));
-    double gx, gy;
-    dpc_ops(g, r, c, &gx, &gy);
-    const double sx = gx * X.x + gy * Y.x, sy = gx * X.y + gy * Y.y;
-    return make_double2(-sy, sx);
 }
 
 // One thread per pair {k, -k}: both spectra from one read of each.
@@ -366,18 +398,6 @@ __global__ __launch_bounds__(256) void dpc_update(DpcGrid g, const double *phi, 
 __global__ __launch_bounds__(256) void real_to_complex(const double *in, double2 *z, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) z[i] = make_double2(in[i], 0.0);
-}
-
-// G = FFT(g), g real. The adjoint needs Re(IFFT(P)) and Re(IFFT(Q)) for P = G conj(i gx), Q = G conj(i gy).
-// Re(IFFT(P)) = IFFT of P's Hermitian part (P[k] + conj P[-k]) / 2 (P itself is not Hermitian on the Nyquist
-// row and column of an even length, where gx[-k] = gx[k]), so out = Herm(P) + i Herm(Q) gives both real parts
-// from one inverse transform. At k from vk = G[k], vm = G[-k] and the operators at k and -k:
-__device__ __forceinline__ double2 dpc_adj(double2 vk, double2 vm, double gxk, double gxm, double gyk, double gym) {
-    // P[k] = -i gx[k] G[k] = (gx G.y, -gx G.x); conj P[-k] = (gx[-k] G[-k].y, gx[-k] G[-k].x); each product at k
-    // fused with the one at -k, written out so the rounding does not depend on the compiler's contraction
-    const double2 P = make_double2(0.5 * __builtin_fma(gxk, vk.y, gxm * vm.y), 0.5 * __builtin_fma(-gxk, vk.x, gxm * vm.x));
-    const double2 Q = make_double2(0.5 * __builtin_fma(gyk, vk.y, gym * vm.y), 0.5 * __builtin_fma(-gyk, vk.x, gym * vm.x));
-    return make_double2(P.x - Q.y, P.y + Q.x);
 }
 
 __global__ __launch_bounds__(256) void dpc_adjoint_spectrum(DpcGrid g, const double2 *G, double2 *out) {
@@ -723,16 +743,27 @@ hipError_t launch_fft2_real(const double *in, double2 *tmp, double2 *data, int r
     return hipGetLastError();
 }
 
-hipError_t launch_ifft2_update(const DpcGrid &g, const double2 *spec, double2 *tmp, const double2 *tw_r,
-                               const double2 *tw_c, const double *phi, double step, double *phi_out, hipStream_t s) {
+hipError_t launch_ifft2_spectrum_update(const DpcGrid &g, const double2 *F, double2 *tmp, const double2 *tw_r,
+                                        const double2 *tw_c, const double *phi, double step, double *phi_out,
+                                        hipStream_t s) {
     if (!dpc_fused_supported(g.Rp, g.Cp)) return hipErrorInvalidValue;
     FftFuse f{};
     f.g = g;
     f.phi = phi;
     f.phi_out = phi_out;
     f.step = step;
-    launch_fft_rows_t(spec, tmp, 1, g.Rp, g.Cp, tw_c, true, 1.0, s);
+    launch_fft_rows_t<kFftSpectrum>(F, tmp, 1, g.Rp, g.Cp, tw_c, true, 1.0, s, f);
     launch_fft_rows_t<kFftUpdate>(tmp, nullptr, 1, g.Cp, g.Rp, tw_r, true, 1.0 / ((double)g.Rp * g.Cp), s, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_ifft2_adjoint_spectrum(const DpcGrid &g, const double2 *G, double2 *tmp, double2 *out,
+                                         const double2 *tw_r, const double2 *tw_c, hipStream_t s) {
+    if (!dpc_fused_supported(g.Rp, g.Cp)) return hipErrorInvalidValue;
+    FftFuse f{};
+    f.g = g;
+    launch_fft_rows_t<kFftAdjointSpectrum>(G, tmp, 1, g.Rp, g.Cp, tw_c, true, 1.0, s, f);
+    launch_fft_rows_t(tmp, out, 1, g.Cp, g.Rp, tw_r, true, 1.0 / ((double)g.Rp * g.Cp), s);
     return hipGetLastError();
 }
 

@@ -71,7 +71,9 @@ hipError_t launch_dpc_update(const DpcGrid &g, const double *phi, const double2 
 //     row transforms of z, stored transposed in tmp, and the partial sums;
 //   - launch_fft2_second_pass: the second pass, tmp -> data (the transform of z);
 //   - launch_fft2_real: the forward transform of a real array in (real_to_complex and launch_fft2);
-//   - launch_ifft2_update: the inverse transform of spec followed by launch_dpc_update (u itself is not stored).
+//   - launch_ifft2_spectrum_update: launch_dpc_spectrum of F, its inverse transform and launch_dpc_update (the
+//     spectrum formed as the inverse's first pass loads its rows, u itself not stored);
+//   - launch_ifft2_adjoint_spectrum: launch_dpc_adjoint_spectrum of G and its inverse transform, into out.
 bool dpc_fused_supported(int Rp, int Cp);
 hipError_t launch_dpc_forward_pass(const DpcGrid &g, const double *phi, const double *fx, const double *fy,
                                    const double2 *tw_c, double2 *tmp, double *partial, int *nblocks, hipStream_t s);
@@ -79,8 +81,11 @@ hipError_t launch_fft2_second_pass(const double2 *tmp, double2 *data, int rows, 
                                    hipStream_t s);
 hipError_t launch_fft2_real(const double *in, double2 *tmp, double2 *data, int rows, int cols, const double2 *tw_r,
                             const double2 *tw_c, hipStream_t s);
-hipError_t launch_ifft2_update(const DpcGrid &g, const double2 *spec, double2 *tmp, const double2 *tw_r,
-                               const double2 *tw_c, const double *phi, double step, double *phi_out, hipStream_t s);
+hipError_t launch_ifft2_spectrum_update(const DpcGrid &g, const double2 *F, double2 *tmp, const double2 *tw_r,
+                                        const double2 *tw_c, const double *phi, double step, double *phi_out,
+                                        hipStream_t s);
+hipError_t launch_ifft2_adjoint_spectrum(const DpcGrid &g, const double2 *G, double2 *tmp, double2 *out,
+                                         const double2 *tw_r, const double2 *tw_c, hipStream_t s);
 // z = g_phi_out + 0 i (the adjoint's input), padded grid.
 hipError_t launch_real_to_complex(const double *in, double2 *z, int64_t n, hipStream_t s);
 // spectrum of the adjoint: G = FFT(g) -> G (gy - i gx), whose inverse is A_x^T g + i A_y^T g.
