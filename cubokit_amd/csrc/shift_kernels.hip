@@ -1003,32 +1003,56 @@ bool shift_grad_rows_supported(int qx) { return qx % 32 == 0 && qx >= 32 && qx <
 int shift_grad_rows_band(int qx) { return qx <= 128 ? 3 : 1; }
 
 namespace {
-__global__ void split_dy(const double *dy, const double *wr, const double *wc, int64_t npos, int k,
-                         const double *scale, const int32_t *order, _Float16 *dyw, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;     // (k-step, corner, lane, e)
-    if (i >= n) return;
-    const int64_t ks = i / 2048;
-    const int c = (int)((i / 512) % 4), l = (int)((i / 8) % 64), e = (int)(i % 8), kk = l & 31;
-    const int64_t slot = ks * 16 + 8 * (l >> 5) + e;
-    const int64_t pos = order ? (int64_t)order[slot] : slot;       // order: -1 for a padding slot
-    double v = 0.0;
-    if (kk < k && pos >= 0 && pos < npos) {
-        const double r = wr[pos], q = wc[pos];
-        const double w = c == 0 ? (1.0 - r) * (1.0 - q) : c == 1 ? r * (1.0 - q) : c == 2 ? (1.0 - r) * q : r * q;
-        v = dy[pos * k + kk] * w * scale[kk];
+// One block per batch of 16 slots: their positions, fractions and dL/dY rows staged in LDS, then thread (corner c,
+// lane l) forms the 8 entries e of its half8 (slot 8 (l / 32) + e, mask l % 32) and writes the hi and lo parts as one
+// 16-byte store each.
+constexpr int kSplitMaxK = 32;
+__global__ __launch_bounds__(256) void split_dy(const double *dy, const double *wr, const double *wc, int64_t npos,
+                                                int k, const double *scale, const int32_t *order, half8 *dyw) {
+    __shared__ double sdy[16][kSplitMaxK];
+    __shared__ double sr[16], sq[16];
+    __shared__ int64_t sp[16];
+    const int64_t ks = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (tid < 16) {
+        const int64_t slot = ks * 16 + tid;
+        const int64_t pos = order ? (int64_t)order[slot] : slot;       // order: -1 for a padding slot
+        const bool live = pos >= 0 && pos < npos;
+        sp[tid] = live ? pos : -1;
+        sr[tid] = live ? wr[pos] : 0.0;
+        sq[tid] = live ? wc[pos] : 0.0;
     }
-    const _Float16 hi = (_Float16)(float)v;
-    const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
-    dyw[(((ks * 4 + c) * 2 + 0) * 64 + l) * 8 + e] = hi;
-    dyw[(((ks * 4 + c) * 2 + 1) * 64 + l) * 8 + e] = lo;
+    __syncthreads();
+    for (int v = tid; v < 16 * kSplitMaxK; v += 256) {
+        const int e = v / kSplitMaxK, kk = v % kSplitMaxK;
+        sdy[e][kk] = (sp[e] >= 0 && kk < k) ? dy[sp[e] * k + kk] : 0.0;
+    }
+    __syncthreads();
+    const int c = tid >> 6, l = tid & 63, kk = l & 31;
+    half8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int sl = 8 * (l >> 5) + e;
+        double v = 0.0;
+        if (kk < k && sp[sl] >= 0) {
+            const double r = sr[sl], q = sq[sl];
+            const double w = c == 0 ? (1.0 - r) * (1.0 - q) : c == 1 ? r * (1.0 - q) : c == 2 ? (1.0 - r) * q : r * q;
+            v = sdy[sl][kk] * w * scale[kk];
+        }
+        hi[e] = (_Float16)(float)v;
+        lo[e] = (_Float16)(float)(v - (double)(float)hi[e]);
+    }
+    dyw[((ks * 4 + c) * 2 + 0) * 64 + l] = hi;
+    dyw[((ks * 4 + c) * 2 + 1) * 64 + l] = lo;
 }
 }  // namespace
 
 hipError_t launch_split_dy(const double *dy, const double *wr, const double *wc, int64_t npos, int k,
                            const double *scale, const int32_t *order, int64_t nslots, void *dyw, hipStream_t s) {
-    const int64_t n = (nslots + 15) / 16 * 2048;
-    hipLaunchKernelGGL(split_dy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dy, wr, wc, npos, k, scale, order,
-                       (_Float16 *)dyw, n);
+    if (k > kSplitMaxK) return hipErrorInvalidValue;
+    const int64_t nbatch = (nslots + 15) / 16;
+    hipLaunchKernelGGL(split_dy, dim3((unsigned)nbatch), dim3(256), 0, s, dy, wr, wc, npos, k, scale, order,
+                       (half8 *)dyw);
     return hipGetLastError();
 }
 
