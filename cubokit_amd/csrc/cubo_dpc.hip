@@ -117,7 +117,8 @@ hipError_t fft2(cubo_hip_dpc *d, double2 *data, bool inverse) {
     return cubo::launch_fft2(data, d->tmp, d->Rp, d->Cp, d->tw_r, d->tw_c, inverse, d->stream);
 }
 
-// phi_out = phi + step Re(IFFT(i gx FFT(dx) + i gy FFT(dy))), with z = dx + i dy from forward().
+// phi_out = phi + step Re(IFFT(i gx FFT(dx) + i gy FFT(dy))), with z = dx + i dy from forward() (on the fused grids,
+// the first pass of its transform, in tmp).
 hipError_t update(cubo_hip_dpc *d, const double *phi, double step, double *phi_out) {
     if (fused(d)) {
         hipError_t e = cubo::launch_fft2_second_pass(d->tmp, d->z, d->Rp, d->Cp, d->tw_r, d->stream);
@@ -187,7 +188,7 @@ int cubo_hip_dpc_create(int device, int32_t rows, int32_t cols, double padding, 
     DeviceGuard g(device);
     const size_t n = (size_t)Rp * Cp, N = (size_t)rows * cols;
     // operators (DPC.swift:40-47): k from numpy.fft.fftfreq, kden = 1 / (kx^2 + ky^2) with infinity at 0,
-    // gx = -0.25 kx kden, gy = -0.25 ky kden: formed in the spectrum kernels from kx and ky
+    // gx = -0.25 kx kden, gy = -0.25 ky kden: formed in the kernels from kx and ky
     std::vector<double> kx(Rp), ky(Cp);
     fftfreq(Rp, sampling_row, kx.data());
     fftfreq(Cp, sampling_col, ky.data());

@@ -164,7 +164,7 @@ __device__ __forceinline__ double2 dpc_adj(double2 vk, double2 vm, double gxk, d
 // TWL: the radix-4 stages read their twiddles from per-stage tables in LDS behind the two row buffers, copied from
 // tw at the start: for the stage of span ns, w^(m k) (m = 1, 2, 3, k < ns) at offset (m - 1) ns + k of the stage's
 // table, so the lanes of a stage (consecutive k) read consecutive entries (tw itself is read at a stride of
-// n / (4 ns) entries, all in one bank). The values, and so the results, are tw's.
+// n / (4 ns) entries). The values, and so the results, are tw's. The fused modes assume one array (batch 1).
 template <bool INV, int R, int MODE, bool TWL>
 __global__ __launch_bounds__(1024) void fft_rows_t(const double2 *in, double2 *out, int n, int rows, const double2 *tw,
                                                   double scale, FftFuse f) {
