@@ -9,6 +9,7 @@
 // one FFT of the incoming phase gradient and one inverse.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -43,6 +44,8 @@ struct cubo_hip_dpc {
     hipEvent_t t0 = nullptr, t1 = nullptr;
     double ms = 0;
     int64_t calls = 0;
+    double *pin = nullptr;                        // pinned staging of the host arrays (doubles)
+    size_t pin_cap = 0;
 };
 
 namespace {
@@ -63,7 +66,53 @@ void free_dpc(cubo_hip_dpc *d) {
     if (d->t0) (void)hipEventDestroy(d->t0);
     if (d->t1) (void)hipEventDestroy(d->t1);
     if (d->stream) (void)hipStreamDestroy(d->stream);
+    if (d->pin) (void)hipHostFree(d->pin);
     delete d;
+}
+
+// Host arrays go to and from the device through a pinned staging buffer of the plan, copied on host threads (a DMA
+// from or to pageable memory runs at a fraction of the link's rate). A call whose arrays would need more than
+// kStageMax doubles, or whose staging cannot be allocated, copies them directly.
+constexpr size_t kStageMax = (size_t)32 << 20;
+double *stage(cubo_hip_dpc *d, size_t doubles) {
+    if (doubles > kStageMax) return nullptr;
+    if (d->pin_cap < doubles) {
+        if (d->pin) (void)hipHostFree(d->pin);
+        d->pin = nullptr;
+        d->pin_cap = 0;
+        if (hipHostMalloc((void **)&d->pin, doubles * 8, hipHostMallocDefault) != hipSuccess) {
+            d->pin = nullptr;
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        d->pin_cap = doubles;
+    }
+    return d->pin;
+}
+
+// count doubles on host threads (about 1 MiB each)
+void host_copy(double *dst, const double *src, size_t count) {
+    const int threads = (int)std::min<size_t>((size_t)cubo::reader_threads(), std::max<size_t>(1, (count * 8) >> 20));
+    cubo::parallel_pieces(count * 8, threads, [&](size_t off, size_t len) {
+        memcpy((char *)dst + off, (const char *)src + off, len);
+        return true;
+    });
+}
+
+// host -> dev, through the staging region at (the caller keeps the region until the copy has run) or directly
+hipError_t upload(cubo_hip_dpc *d, double *dev, const double *host, size_t count, double *at) {
+    if (!at) return hipMemcpyAsync(dev, host, count * 8, hipMemcpyHostToDevice, d->stream);
+    host_copy(at, host, count);
+    return hipMemcpyAsync(dev, at, count * 8, hipMemcpyHostToDevice, d->stream);
+}
+
+// dev -> host: the copy into the staging region at (or directly into host), completed by finish_download after
+// the stream has been synchronized
+hipError_t start_download(cubo_hip_dpc *d, double *host, const double *dev, size_t count, double *at) {
+    return hipMemcpyAsync(at ? at : host, dev, count * 8, hipMemcpyDeviceToHost, d->stream);
+}
+void finish_download(double *host, const double *at, size_t count) {
+    if (at) host_copy(host, at, count);
 }
 
 // numpy.fft.fftfreq(n, d)
@@ -82,18 +131,36 @@ cubo::DpcGrid grid(const cubo_hip_dpc *d) {
 }
 
 // Field checks and the denominator mean(fx^2 + fy^2) (NumPy's pairwise order, as DPC.swift:49-54).
-int field(cubo_hip_dpc *d, const double *fx, const double *fy, double *denom) {
+// The field checked (finite, first offending entry reported), its denominator mean(fx^2 + fy^2) summed pairwise in
+// NumPy's order, and fx, fy uploaded (staged in pin[0 .. 2N) when pin is given); on host threads.
+int field(cubo_hip_dpc *d, const double *fx, const double *fy, double *denom, double *pin = nullptr) {
     const int64_t N = (int64_t)d->R * d->C;
     std::vector<double> t(N);
-    for (int64_t k = 0; k < N; ++k) {
-        if (!std::isfinite(fx[k]) || !std::isfinite(fy[k]))
-            return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the field must be finite (entry %lld)", (long long)k);
-        t[k] = fx[k] * fx[k] + fy[k] * fy[k];
-    }
+    std::atomic<int64_t> bad{N};
+    const int threads = (int)std::min<int64_t>(cubo::reader_threads(), std::max<int64_t>(1, (N * 8) >> 20));
+    cubo::parallel_pieces((size_t)N * 8, threads, [&](size_t off, size_t len) {
+        const int64_t k0 = (int64_t)(off / 8), k1 = k0 + (int64_t)(len / 8);
+        for (int64_t k = k0; k < k1; ++k) {
+            if (!std::isfinite(fx[k]) || !std::isfinite(fy[k])) {
+                int64_t b = bad.load();
+                while (k < b && !bad.compare_exchange_weak(b, k)) {}
+                return true;
+            }
+            t[k] = fx[k] * fx[k] + fy[k] * fy[k];
+        }
+        if (pin) {
+            memcpy(pin + k0, fx + k0, len);
+            memcpy(pin + N + k0, fy + k0, len);
+        }
+        return true;
+    });
+    if (bad.load() < N)
+        return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the field must be finite (entry %lld)", (long long)bad.load());
     *denom = cubo::pairwise_sum(t.data(), N) / (double)N;
     if (!(*denom > 0)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "the field is zero: DPC has nothing to reconstruct");
-    CUBO_HIP_TRY(hipMemcpyAsync(d->fx, fx, N * 8, hipMemcpyHostToDevice, d->stream), "uploading the field");
-    CUBO_HIP_TRY(hipMemcpyAsync(d->fy, fy, N * 8, hipMemcpyHostToDevice, d->stream), "uploading the field");
+    CUBO_HIP_TRY(hipMemcpyAsync(d->fx, pin ? pin : fx, N * 8, hipMemcpyHostToDevice, d->stream), "uploading the field");
+    CUBO_HIP_TRY(hipMemcpyAsync(d->fy, pin ? pin + N : fy, N * 8, hipMemcpyHostToDevice, d->stream),
+                 "uploading the field");
     return CUBO_HIP_OK;
 }
 
@@ -285,8 +352,9 @@ int cubo_hip_dpc_reconstruct(cubo_hip_dpc *d, const double *fx, const double *fy
     if (max_iter > 0 && (!errors || !accepted)) return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     DeviceGuard g(d->device);
     double denom = 0;
-    if (int st = field(d, fx, fy, &denom)) return st;
     const int64_t n = (int64_t)d->Rp * d->Cp, N = (int64_t)d->R * d->C;
+    double *pin = stage(d, 3 * (size_t)N);            // staging: fx, fy, the phase
+    if (int st = field(d, fx, fy, &denom, pin)) return st;
     CUBO_HIP_TRY(hipMemsetAsync(d->phi, 0, n * 8, d->stream), "clearing the phase");
     double error = INFINITY;
     int ne = 0, ni = 0;
@@ -311,8 +379,10 @@ int cubo_hip_dpc_reconstruct(cubo_hip_dpc *d, const double *fx, const double *fy
     }
     CUBO_HIP_TRY(hipEventRecord(d->t1, d->stream), "hipEventRecord");
     CUBO_HIP_TRY(cubo::launch_crop(cur, d->R, d->C, d->Cp, d->g_fx, d->stream), "launching the crop");
-    CUBO_HIP_TRY(hipMemcpyAsync(phase, d->g_fx, N * 8, hipMemcpyDeviceToHost, d->stream), "reading the phase");
+    double *at = pin ? pin + 2 * N : nullptr;
+    CUBO_HIP_TRY(start_download(d, phase, d->g_fx, N, at), "reading the phase");
     CUBO_HIP_TRY(hipStreamSynchronize(d->stream), "waiting for DPC");
+    finish_download(phase, at, N);
     float ms = 0;
     if (hipEventElapsedTime(&ms, d->t0, d->t1) == hipSuccess) d->ms += ms;
     ++d->calls;
@@ -329,8 +399,11 @@ int cubo_hip_dpc_chain_backward(cubo_hip_dpc *d, const double *fx, const double 
         return fail(CUBO_HIP_ERR_INVALID_ARGUMENT, "at least one step, with a positive step size");
     DeviceGuard g(d->device);
     double denom = 0;
-    if (int st = field(d, fx, fy, &denom)) return st;
     const int64_t n = (int64_t)d->Rp * d->Cp, N = (int64_t)d->R * d->C;
+    // staging: fx, fy, dL/dphase, the phase, dL/dfx, dL/dfy
+    double *pin = stage(d, 6 * (size_t)N);
+    auto at = [&](int i) { return pin ? pin + (size_t)i * N : nullptr; };
+    if (int st = field(d, fx, fy, &denom, pin)) return st;
     while ((int)d->chain.size() < nsteps + 1) {
         double *b = nullptr;
         CUBO_HIP_TRY(hipMalloc((void **)&b, n * 8), "allocating the stored iterates");
@@ -346,7 +419,7 @@ int cubo_hip_dpc_chain_backward(cubo_hip_dpc *d, const double *fx, const double 
     if (!d->chain_g) CUBO_HIP_TRY(hipMalloc((void **)&d->chain_g, N * 8), "allocating dL/dphase");
     std::vector<double> msq(nsteps);
     // dL/dphase goes up before the steps (its copy from pageable memory stays out of the chain's timing)
-    CUBO_HIP_TRY(hipMemcpyAsync(d->chain_g, g_phase, N * 8, hipMemcpyHostToDevice, d->stream), "uploading dL/dphase");
+    CUBO_HIP_TRY(upload(d, d->chain_g, g_phase, N, at(2)), "uploading dL/dphase");
     CUBO_HIP_TRY(hipMemsetAsync(d->chain[0], 0, n * 8, d->stream), "clearing");
     CUBO_HIP_TRY(hipEventRecord(d->t0, d->stream), "hipEventRecord");
     // forward: phi_{i+1} = phi_i + step * update(phi_i), keeping every iterate and every step's squared-error
@@ -364,8 +437,9 @@ int cubo_hip_dpc_chain_backward(cubo_hip_dpc *d, const double *fx, const double 
     }
     if (phase) {
         CUBO_HIP_TRY(cubo::launch_crop(d->chain[nsteps], d->R, d->C, d->Cp, d->g_fx, d->stream), "launching the crop");
-        CUBO_HIP_TRY(hipMemcpyAsync(phase, d->g_fx, N * 8, hipMemcpyDeviceToHost, d->stream), "reading the phase");
+        CUBO_HIP_TRY(start_download(d, phase, d->g_fx, N, at(3)), "reading the phase");
         CUBO_HIP_TRY(hipStreamSynchronize(d->stream), "waiting for DPC");
+        finish_download(phase, at(3), N);
     }
     // backward: dL/dphase of the cropped phase into the padded grid, then each step's adjoint in reverse
     CUBO_HIP_TRY(hipMemsetAsync(d->g_fx, 0, N * 8, d->stream), "clearing");
@@ -378,9 +452,11 @@ int cubo_hip_dpc_chain_backward(cubo_hip_dpc *d, const double *fx, const double 
         std::swap(gin, gout);
     }
     CUBO_HIP_TRY(hipEventRecord(d->t1, d->stream), "hipEventRecord");
-    CUBO_HIP_TRY(hipMemcpyAsync(g_fx, d->g_fx, N * 8, hipMemcpyDeviceToHost, d->stream), "reading dL/dfx");
-    CUBO_HIP_TRY(hipMemcpyAsync(g_fy, d->g_fy, N * 8, hipMemcpyDeviceToHost, d->stream), "reading dL/dfy");
+    CUBO_HIP_TRY(start_download(d, g_fx, d->g_fx, N, at(4)), "reading dL/dfx");
+    CUBO_HIP_TRY(start_download(d, g_fy, d->g_fy, N, at(5)), "reading dL/dfy");
     CUBO_HIP_TRY(hipStreamSynchronize(d->stream), "waiting for the DPC adjoint");
+    finish_download(g_fx, at(4), N);
+    finish_download(g_fy, at(5), N);
     float ms = 0;
     if (hipEventElapsedTime(&ms, d->t0, d->t1) == hipSuccess) d->ms += ms;
     ++d->calls;
