@@ -488,7 +488,7 @@ __host__ __device__ constexpr int ff_twiddle_entries(int l) { return 3 * (l / 2 
 
 int fourier_fused_lds(int qy, int qx) {
     const int h = qx / 2 + 1, l = qy > qx ? qy : qx;
-    return 8 * (qy * (qx + 2) + 2 * qy + 2 * h + 2 * ff_twiddle_entries(l) + 2);
+    return 8 * (qy * (qx + 2) + 2 * qy + 2 * h + 2 * ff_twiddle_entries(l) + 2 + 2 * qy);
 }
 
 template <bool U16, int STAT, bool FIRST, int LM>
@@ -514,6 +514,7 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
     double2 *TWS = B + H;
     const int R4T = 3 * (L / 2 - 1);
     double2 *NYQ = TWS + ff_twiddle_entries(L);
+    double2 *NQ = NYQ + 1;                               // the mean: sums of the Nyquist column's ramped bins
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int NP = N / 16;                               // pairs per wave
@@ -544,15 +545,10 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
     auto tw2 = [&](int lh, int j) { return TWS[R4T + (1 << lh) - 1 + j]; };
     auto tw4 = [&](int lq, int m, int j) { return TWS[3 * ((1 << lq) - 1) + (m - 1) * (1 << lq) + j]; };
 
-    // per slot: the running sum (or maximum) in Double and the TwoSum compensation in Float (it gathers rounding
-    // errors far below the sum's last bit, so 24 bits of it are plenty; registers are short here)
+    // per slot: the running maximum, or, for the mean, the block's sum once its summed spectrum is inverted
     double acc_s[kFfPix];
-    float acc_c[kFfPix];
 #pragma unroll
-    for (int j = 0; j < kFfPix; ++j) {
-        acc_s[j] = MEAN ? 0.0 : -__builtin_inf();
-        acc_c[j] = 0.f;
-    }
+    for (int j = 0; j < kFfPix; ++j) acc_s[j] = MEAN ? 0.0 : -__builtin_inf();
     u32x4 pre[CH];
 #pragma unroll
     for (int j = 0; j < CH; ++j) pre[j] = u32x4{0u, 0u, 0u, 0u};
@@ -681,8 +677,127 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
     };
     auto col_r4_dif = [&](int lq) { col_r4(lq, false); };
     auto col_r4_dit = [&](int lq) { col_r4(lq, true); };
+    // The mean: the sum of the shifted patterns is the inverse transform of the sum of their ramped spectra, so each
+    // pattern only goes forward. The last forward column stage (radix-4 of quarter 1) adds its ramped outputs,
+    // times the position's weight, into sums instead of storing them, and the block inverts its summed spectrum
+    // once. Columns below M / 2 (M / 16 of them per wave) are summed in registers, spec[it][m] holding bin
+    // (bit-reversed row 4 b + m, column k) of task u = lane + 64 it, the same bins for every pattern (at most 4
+    // tasks: 64 registers); the Nyquist column, wave 0's extra one, in LDS (NQ, each entry always the same lane's).
+    constexpr int KB = MEAN ? (M / 16) * (64 * kFfWaves * kFfPix / M / 4) / 64 : 1;   // tasks per lane
+    static_assert(!MEAN || (M >= 16 && KB >= 1), "whole columns per wave below the Nyquist column");
+    const int NCL = M / 16;                              // this wave's columns below M / 2
+    double2 spec[KB][4];
+#pragma unroll
+    for (int it = 0; it < KB; ++it)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) spec[it][m] = make_double2(0.0, 0.0);
+    auto col_task = [&](int u, int &i0, int &k) {
+        const int i = u >> (LN - 2), b = u & (N / 4 - 1);
+        k = wave + kFfWaves * i;
+        i0 = 4 * b;
+    };
+    // the ramped outputs of the last stage's butterfly of rows i0 .. i0 + 3 of column k
+    auto last_bins = [&](int i0, int k, double wgt, double2 (&x)[4]) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) x[m] = C[swz_row(i0 + m) * H + k];
+        dif4(x[0], x[1], x[2], x[3], tw4(0, 1, 0), tw4(0, 2, 0), tw4(0, 3, 0), false);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int f = bit_reverse(i0 + m, LN);
+            const double2 r = (k == M / 2 && f == N / 2) ? NYQ[0] : cmul(A[f], B[k]);
+            x[m] = cmul(x[m], r);
+            if (a.w) x[m] = make_double2(x[m].x * wgt, x[m].y * wgt);
+        }
+    };
+    auto col_sum_last = [&](double wgt) {
+#pragma unroll
+        for (int it = 0; it < KB; ++it) {
+            const int u = lane + 64 * it;
+            if (u < NCL * (N / 4)) {
+                int i0, k;
+                col_task(u, i0, k);
+                double2 x[4];
+                last_bins(i0, k, wgt, x);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) spec[it][m] = cadd(spec[it][m], x[m]);
+            }
+            __builtin_amdgcn_sched_barrier(0);           // one task's loads in flight at a time (registers)
+        }
+        if (wave == 0) {
+            for (int b = lane; b < N / 4; b += 64) {
+                double2 x[4];
+                last_bins(4 * b, M / 2, wgt, x);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) NQ[4 * b + m] = cadd(NQ[4 * b + m], x[m]);
+            }
+        }
+    };
+    // The block's summed spectrum into the bins the last forward stage stores.
+    auto col_store_sums = [&]() {
+#pragma unroll
+        for (int it = 0; it < KB; ++it) {
+            const int u = lane + 64 * it;
+            if (u < NCL * (N / 4)) {
+                int i0, k;
+                col_task(u, i0, k);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) C[swz_row(i0 + m) * H + k] = spec[it][m];
+            }
+        }
+        if (wave == 0)
+            for (int i = lane; i < N; i += 64) C[swz_row(i) * H + M / 2] = NQ[i];
+        wave_sync();
+    };
+    // The inverse: the columns (decimation in time: bit-reversed in, natural out); then pack W = X0 + i X1 over the
+    // full row (W[-k] = conj X0[k] + i conj X1[k]), bit-reversed, and the rows (decimation in time), after which
+    // rows 2q and 2q + 1 are real.
+    auto to_pixels = [&]() {
+        {
+            int lq = 0;
+            for (; lq + 2 <= LN; lq += 2) col_r4_dit(lq);
+            if (lq < LN) col_r2_dit(lq);
+        }
+        __syncthreads();
+        for (int t = 0; t < NP; ++t) {
+            const int q = wave + kFfWaves * t;
+            double *re = S + region(q), *im = re + R2;
+            double2 z0[kFfItems], z1[kFfItems];
+#pragma unroll
+            for (int i = 0; i < kFfItems; ++i) {
+                const int k = lane + 64 * i;
+                if (k <= M / 2) {
+                    z0[i] = ((const double2 *)(re + flip(q) * R2))[k];
+                    z1[i] = ((const double2 *)(re + (flip(q) ^ 1) * R2))[k];
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < kFfItems; ++i) {
+                const int k = lane + 64 * i;
+                if (k <= M / 2) {
+                    const int pk = swz_el(bit_reverse(k, LM));
+                    re[pk] = z0[i].x - z1[i].y;
+                    im[pk] = z0[i].y + z1[i].x;
+                    if (k > 0 && k < M / 2) {
+                        const int pm = swz_el(bit_reverse(M - k, LM));
+                        re[pm] = z0[i].x + z1[i].y;
+                        im[pm] = z1[i].x - z0[i].y;
+                    }
+                }
+            }
+            wave_sync();
+        }
+        // inverse row transforms (decimation in time: bit-reversed in, natural out): rows 2q and 2q+1 real
+        {
+            int lq = 0;
+            for (; lq + 2 <= LM; lq += 2) row_r4_dit(lq);
+            if (lq < LM) row_r2_dit(lq);
+        }
+    };
 
-    __syncthreads();                                     // the twiddles
+    if constexpr (MEAN)
+        for (int i = tid; i < N; i += kFfBlock) NQ[i] = make_double2(0.0, 0.0);
+    __syncthreads();                                     // the twiddles (and the zeroed Nyquist sums)
     if (e_begin < e_end) prefetch(e_begin);
     for (int64_t e = e_begin; e < e_end; ++e) {
         // the raw pattern into the split layout of this wave's pairs
@@ -783,59 +898,26 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             wave_sync();
         }
         __syncthreads();
-        // the columns: forward (decimation in frequency; the last stage, radix-4, applies the ramp), then inverse
-        // (decimation in time: bit-reversed in, natural out)
+        // the columns: forward (decimation in frequency; the last stage, radix-4, applies the ramp), then, for the
+        // maximum, inverse
+        const double wgt = a.w ? a.w[pe] : 1.0;
         {
             int top = LN - 1;
             if (LN & 1) {
                 col_r2_dif(top);
                 top -= 1;
             }
-            for (; top >= 1; top -= 2) col_r4_dif(top - 1);
-            int lq = 0;
-            for (; lq + 2 <= LN; lq += 2) col_r4_dit(lq);
-            if (lq < LN) col_r2_dit(lq);
+            for (; top >= 3; top -= 2) col_r4_dif(top - 1);
+            if constexpr (MEAN) col_sum_last(wgt);
+            else col_r4_dif(0);
         }
-        __syncthreads();
-        // pack: W = X0 + i X1 over the full row (W[-k] = conj X0[k] + i conj X1[k]), bit-reversed for the inverse
-        for (int t = 0; t < NP; ++t) {
-            const int q = wave + kFfWaves * t;
-            double *re = S + region(q), *im = re + R2;
-            double2 z0[kFfItems], z1[kFfItems];
-#pragma unroll
-            for (int i = 0; i < kFfItems; ++i) {
-                const int k = lane + 64 * i;
-                if (k <= M / 2) {
-                    z0[i] = ((const double2 *)(re + flip(q) * R2))[k];
-                    z1[i] = ((const double2 *)(re + (flip(q) ^ 1) * R2))[k];
-                }
-            }
-            wave_sync();
-#pragma unroll
-            for (int i = 0; i < kFfItems; ++i) {
-                const int k = lane + 64 * i;
-                if (k <= M / 2) {
-                    const int pk = swz_el(bit_reverse(k, LM));
-                    re[pk] = z0[i].x - z1[i].y;
-                    im[pk] = z0[i].y + z1[i].x;
-                    if (k > 0 && k < M / 2) {
-                        const int pm = swz_el(bit_reverse(M - k, LM));
-                        re[pm] = z0[i].x + z1[i].y;
-                        im[pm] = z1[i].x - z0[i].y;
-                    }
-                }
-            }
-            wave_sync();
+        if constexpr (MEAN) {
+            __syncthreads();                             // every wave's columns read before the next pattern's load
+            continue;
         }
-        // inverse row transforms (decimation in time: bit-reversed in, natural out): rows 2q and 2q+1 real
-        {
-            int lq = 0;
-            for (; lq + 2 <= LM; lq += 2) row_r4_dit(lq);
-            if (lq < LM) row_r2_dit(lq);
-        }
+        to_pixels();
         // this lane's slots: stored element rem & (M - 1) of plane rem >= M (the same logical pixel for every
-        // entry: swz_el is its own inverse); the shifted value (times the weight), TwoSum or the maximum
-        const double wgt = a.w ? a.w[pe] : 1.0;
+        // entry: swz_el is its own inverse); the shifted value (times the weight) and the maximum
 #pragma unroll
         for (int j = 0; j < kFfPix; ++j) {
             const int u = lane + 64 * j;
@@ -843,17 +925,24 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
                 const int t = u >> (LM + 1), rem = u & (2 * M - 1);
                 double v = S[region(wave + kFfWaves * t) + (rem >= M ? R2 : 0) + (rem & (M - 1))];
                 if (a.w) v = v * wgt;
-                if constexpr (MEAN) {
-                    const double t = acc_s[j] + v, bb = t - acc_s[j];
-                    acc_c[j] += (float)((acc_s[j] - (t - bb)) + (v - bb));
-                    acc_s[j] = t;
-                } else {
-                    acc_s[j] = max_step<FIRST, true>(acc_s[j], v);
-                }
+                acc_s[j] = max_step<FIRST, true>(acc_s[j], v);
             }
             if (j % 8 == 7) __builtin_amdgcn_sched_barrier(0);   // at most 8 reads in flight
         }
         wave_sync();
+    }
+    if constexpr (MEAN) {
+        // the block's sum of shifted patterns from its summed spectrum; the slots as for the maximum
+        col_store_sums();
+        to_pixels();
+#pragma unroll
+        for (int j = 0; j < kFfPix; ++j) {
+            const int u = lane + 64 * j;
+            if (j < J) {
+                const int t = u >> (LM + 1), rem = u & (2 * M - 1);
+                acc_s[j] = S[region(wave + kFfWaves * t) + (rem >= M ? R2 : 0) + (rem & (M - 1))];
+            }
+        }
     }
 #pragma unroll
     for (int j = 0; j < kFfPix; ++j) {
@@ -862,7 +951,7 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             const int t = u >> (LM + 1), rem = u & (2 * M - 1);
             const int64_t pix = (int64_t)(wave + kFfWaves * t) * 2 * M + (rem & M) + swz_el(rem & (M - 1));
             const int64_t at = (int64_t)blockIdx.x * a.ndet + pix;
-            if constexpr (MEAN) a.part_sum[at] = make_double2(acc_s[j], (double)acc_c[j]);
+            if constexpr (MEAN) a.part_sum[at] = make_double2(acc_s[j], 0.0);   // (sum, compensation)
             else a.part_max[at] = acc_s[j];
         }
     }
