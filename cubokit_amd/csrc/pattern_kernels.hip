@@ -553,16 +553,17 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
 #pragma unroll
     for (int j = 0; j < CH; ++j) pre[j] = u32x4{0u, 0u, 0u, 0u};
     auto position = [&](int64_t e) { return a.list ? (int64_t)a.list[e] : e; };
+    // The next pattern's loads, all issued back to back: a uniform base and 32-bit offsets (a pattern is at most
+    // 64 KiB), a lane without a chunk re-reading the first one, so no branch and no 64-bit addresses to keep.
     auto prefetch = [&](int64_t e) {
         const char *base = (const char *)a.cube + position(e) * a.ndet * ES;
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
             const int u = lane + 64 * j;
-            if (u < nchunks) {
-                const int per = 2 * M / VE, t = u / per, cc = u - t * per;
-                const int q = wave + kFfWaves * t;
-                pre[j] = load16(base + ((int64_t)2 * q * M + cc * VE) * ES);
-            }
+            const int per = 2 * M / VE, t = u / per, cc = u - t * per;
+            const int q = wave + kFfWaves * t;
+            const uint32_t off = u < nchunks ? (uint32_t)((2 * q * M + cc * VE) * ES) : 0u;
+            pre[j] = load16(base + off);
         }
     };
     // Stages over this wave's packed pairs (split layout). Radix-2 span h = 2^lh, radix-4 quarter h = 2^lq.
