@@ -351,7 +351,8 @@ def f64_section(C, cube16, bf, adf):
 def pattern_section(C, cube):
     """Selected-area patterns on the bandwidth cube: a disk of half the scan positions (exact integer kernel), every
     position weighted (FP64 kernel), the disk moved by the rounded centre of mass, and by the Fourier shift (FP64
-    FFTs of every pattern). GB/s over the bytes of the patterns each pass reads."""
+    FFTs of every pattern; the mean inverts one summed spectrum per block, the maximum every pattern). GB/s over
+    the bytes of the patterns each pass reads."""
     s = cube.shape
     npos = s.scan_count
     r, c = np.divmod(np.arange(npos), s.scan_columns)
@@ -365,6 +366,8 @@ def pattern_section(C, cube):
              ("weights_mean", C.DiffractionStatistic.mean, C.ScanSelection.weights(weights), None, npos),
              ("shifted_mask_mean", C.DiffractionStatistic.mean, C.ScanSelection.mask(disk), shift, int(disk.sum())),
              ("fourier_mask_mean", C.DiffractionStatistic.mean, C.ScanSelection.mask(disk),
+              C.PatternShift.fourier_shift(C.OriginMap(com.row.values, com.column.values)), int(disk.sum())),
+             ("fourier_mask_max", C.DiffractionStatistic.max, C.ScanSelection.mask(disk),
               C.PatternShift.fourier_shift(C.OriginMap(com.row.values, com.column.values)), int(disk.sum())))
     out = {"selected_positions": int(disk.sum())}
     for name, stat, scan, sh, n in cases:

@@ -347,7 +347,8 @@ def test_bandwidth_cube_selected_and_shifted_patterns():
 
 
 @pytest.mark.parametrize("dtype,det", [(np.float32, (16, 24)), (np.uint16, (15, 12)),          # batched FFTs
-                                       (np.uint16, (32, 64)), (np.float32, (64, 32)), (np.uint16, (128, 128))])
+                                       (np.uint16, (32, 64)), (np.float32, (64, 32)), (np.uint16, (128, 128)),
+                                       (np.uint16, (16, 256)), (np.float32, (256, 32))])
 def test_fourier_shift_matches_the_oracle(dtype, det):
     """PatternShift.fourier over a mask, weights and every position (first-kept maxima) and meanAndMax(of:shift:)
     (later-kept): FP64 FFTs on both sides, normwise within 1e-12 of max|oracle|, through several tiles. Power-of-two
