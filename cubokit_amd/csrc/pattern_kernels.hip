@@ -405,7 +405,6 @@ __global__ void fourier_reduce(FourierBatch b) {
 constexpr int kFfWaves = 8;
 constexpr int kFfBlock = 64 * kFfWaves;
 constexpr int kFfPix = 32;        // pixels per lane: qy * qx <= 64 * kFfWaves * kFfPix
-constexpr int kFfItems = 3;       // half-spectrum bins per lane and pair: qx / 2 + 1 <= 64 * kFfItems
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -759,27 +758,30 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             if (lq < LN) col_r2_dit(lq);
         }
         __syncthreads();
+        // (the bins below qx / 2 pair by pair, then the Nyquist bin of every pair at once, a pair per lane: its
+        // interleaved slot lies in the split rows' padding and its split element is written by no other bin)
+        constexpr int PI = (M / 2 + 63) / 64;
         for (int t = 0; t < NP; ++t) {
             const int q = wave + kFfWaves * t;
             double *re = S + region(q), *im = re + R2;
-            double2 z0[kFfItems], z1[kFfItems];
+            double2 z0[PI], z1[PI];
 #pragma unroll
-            for (int i = 0; i < kFfItems; ++i) {
+            for (int i = 0; i < PI; ++i) {
                 const int k = lane + 64 * i;
-                if (k <= M / 2) {
+                if (k < M / 2) {
                     z0[i] = ((const double2 *)(re + flip(q) * R2))[k];
                     z1[i] = ((const double2 *)(re + (flip(q) ^ 1) * R2))[k];
                 }
             }
             wave_sync();
 #pragma unroll
-            for (int i = 0; i < kFfItems; ++i) {
+            for (int i = 0; i < PI; ++i) {
                 const int k = lane + 64 * i;
-                if (k <= M / 2) {
+                if (k < M / 2) {
                     const int pk = swz_el(bit_reverse(k, LM));
                     re[pk] = z0[i].x - z1[i].y;
                     im[pk] = z0[i].y + z1[i].x;
-                    if (k > 0 && k < M / 2) {
+                    if (k > 0) {
                         const int pm = swz_el(bit_reverse(M - k, LM));
                         re[pm] = z0[i].x + z1[i].y;
                         im[pm] = z1[i].x - z0[i].y;
@@ -788,6 +790,16 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             }
             wave_sync();
         }
+        if (lane < NP) {
+            const int qn = wave + kFfWaves * lane;
+            double *re = S + region(qn), *im = re + R2;
+            const double2 z0 = ((const double2 *)(re + flip(qn) * R2))[M / 2];
+            const double2 z1 = ((const double2 *)(re + (flip(qn) ^ 1) * R2))[M / 2];
+            const int pn = swz_el(bit_reverse(M / 2, LM));
+            re[pn] = z0.x - z1.y;
+            im[pn] = z0.y + z1.x;
+        }
+        wave_sync();
         // inverse row transforms (decimation in time: bit-reversed in, natural out): rows 2q and 2q+1 real
         {
             int lq = 0;
@@ -872,14 +884,25 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
         }
         // unpack: Z = FFT(row 2q + i row 2q+1) gives both half spectra, X0[k] = (Z[k] + conj Z[-k]) / 2 and
         // X1[k] = -i (Z[k] - conj Z[-k]) / 2, written interleaved
+        // (the Nyquist bin k = qx / 2, its own mirror, of every pair at once, a pair per lane: read before the pairs'
+        // writes and written after them, into the interleaved slot that lies in the split rows' padding, which no
+        // other write touches; the pairs' loop then takes the qx / 2 bins below it, whole waves at 128 columns)
+        constexpr int UI = (M / 2 + 63) / 64;
+        const int qn = wave + kFfWaves * lane;
+        double2 zn = make_double2(0.0, 0.0);
+        if (lane < NP) {
+            const int pn = swz_el(bit_reverse(M / 2, LM));
+            zn = make_double2(S[region(qn) + pn], S[region(qn) + R2 + pn]);
+        }
+        wave_sync();
         for (int t = 0; t < NP; ++t) {
             const int q = wave + kFfWaves * t;
             double *re = S + region(q), *im = re + R2;
-            double2 z0[kFfItems], z1[kFfItems];
+            double2 z0[UI], z1[UI];
 #pragma unroll
-            for (int i = 0; i < kFfItems; ++i) {
+            for (int i = 0; i < UI; ++i) {
                 const int k = lane + 64 * i;
-                if (k <= M / 2) {
+                if (k < M / 2) {
                     const int pk = swz_el(bit_reverse(k, LM)), pm = swz_el(bit_reverse((M - k) & (M - 1), LM));
                     z0[i] = make_double2(re[pk], im[pk]);
                     z1[i] = make_double2(re[pm], im[pm]);
@@ -887,9 +910,9 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             }
             wave_sync();
 #pragma unroll
-            for (int i = 0; i < kFfItems; ++i) {
+            for (int i = 0; i < UI; ++i) {
                 const int k = lane + 64 * i;
-                if (k <= M / 2) {
+                if (k < M / 2) {
                     // logical rows 2q, 2q + 1 are the region's stored rows swz_row(2q), swz_row(2q) ^ 1
                     ((double2 *)(re + flip(q) * R2))[k] = make_double2(0.5 * (z0[i].x + z1[i].x), 0.5 * (z0[i].y - z1[i].y));
                     ((double2 *)(re + (flip(q) ^ 1) * R2))[k] =
@@ -898,6 +921,12 @@ __global__ __launch_bounds__(kFfBlock) void fourier_fused(FourierFused a) {
             }
             wave_sync();
         }
+        if (lane < NP) {
+            double *re = S + region(qn);
+            ((double2 *)(re + flip(qn) * R2))[M / 2] = make_double2(0.5 * (zn.x + zn.x), 0.5 * (zn.y - zn.y));
+            ((double2 *)(re + (flip(qn) ^ 1) * R2))[M / 2] = make_double2(0.5 * (zn.y + zn.y), -0.5 * (zn.x - zn.x));
+        }
+        wave_sync();
         __syncthreads();
         // the columns: forward (decimation in frequency; the last stage, radix-4, applies the ramp), then, for the
         // maximum, inverse
@@ -1045,7 +1074,7 @@ hipError_t launch_fourier_phase(const FourierBatch &b, hipStream_t s) {
 
 bool fourier_fused_supported(int qy, int qx) {
     auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
-    return pow2(qy) && pow2(qx) && qy >= 16 && qx >= 32 && qx <= 256 && qx <= 2 * 64 * kFfItems - 2 &&
+    return pow2(qy) && pow2(qx) && qy >= 16 && qx >= 32 && qx <= 256 &&
            (int64_t)qy * qx <= (int64_t)64 * kFfWaves * kFfPix && fourier_fused_lds(qy, qx) <= 160 * 1024;
 }
 
