@@ -393,7 +393,10 @@ __global__ void fourier_reduce(FourierBatch b) {
 // ---- PatternShift.fourier in one kernel -------------------------------------------------------------------------
 //
 // LDS (doubles): the pattern region [qy / 2 pairs][2 (qx / 2 + 1) complex], then the ramp factors A [qy] and
-// B [qx / 2 + 1], the twiddles [max / 2] and the doubly-Nyquist factor. Pair q (rows 2q, 2q + 1) owns doubles
+// B [qx / 2 + 1], the per-stage twiddle tables, the doubly-Nyquist factor and, for the mean, the sums of the
+// Nyquist column's ramped bins [qy]. The mean transforms each pattern forward only and adds its ramped spectrum
+// into per-lane sums; the block inverts the summed spectrum once. The maximum transforms each pattern forward and
+// back. Pair q (rows 2q, 2q + 1) owns doubles
 // q * 2 (qx + 2) .. (q + 1) * 2 (qx + 2) - 1 in two layouts:
 //   - split, for the row transforms: the real parts of the packed row z = row 2q + i row 2q+1 at offset c, the
 //     imaginary parts at offset qx + 2 + c;
